@@ -1,0 +1,195 @@
+/*
+ * tdi_api.h — C ABI of the MI355X (gfx950) 3DMM per-frame fitting step.
+ *
+ * The drop-in boundary.  The reference has no FFI for this path: its fitting
+ * step is the C++ library `open3DI` (src/CMakeLists.txt:41) whose entry points
+ * take cuBLAS/cuSOLVER handles and raw device pointers (src/lib/model_fitter.h:29-79).
+ * This header is the thin, type-clean layer beneath the C++ host mirror
+ * (3di_amd/csrc/host/model_fitter.h): plain pointers, explicit sizes, int status
+ * codes, a hipStream_t passed as void*, no exceptions across the ABI, no torch
+ * types.  Each entry point names the reference interface it replaces.
+ *
+ * Conventions (shared by the HIP library and by the CPU oracle in oracle/):
+ *   - vertices are AoS float [V][3]; bases are row-major [3V][K] with row 3*v+axis;
+ *   - triangles are int32 [NT][3] in the reference's (tl0,tl1,tl2) = (b,c,a) order,
+ *     0-based (renderer.cu:194-205);
+ *   - images are column-major crops, index (x-x0)*H + (y-y0)
+ *     (the reference transposes its crop, video_fitter.cu:542-549);
+ *   - the per-frame unknown vector follows OptimizationVariables
+ *     (Optimizer.h:71-91, T=1): [beta | alpha | epsilon | tau_x tau_y tau_z u1 u2 u3 | light],
+ *     a block present only when its fit_* flag is set; light = (lambda1..3, L).
+ */
+#ifndef TDI_API_H_
+#define TDI_API_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDI_NLANDMARKS 51
+#define TDI_ABI_VERSION 1
+
+/* Status codes: per call, and per frame in batched fits. */
+enum tdi_status {
+    TDI_OK = 0,
+    TDI_ERR_ARG = 1,          /* bad size / null pointer / unsupported option             */
+    TDI_ERR_HIP = 2,          /* a HIP runtime call failed                                */
+    TDI_ERR_NOMEM = 3,        /* device allocation failed                                 */
+    TDI_NOT_PD = 4,           /* Cholesky met a non-positive pivot (solver.h:86-92)        */
+    TDI_NAN = 5,              /* NaN search direction (model_fitter.cu:206-213)            */
+    TDI_OUT_OF_FRAME = 6,     /* landmarks outside the image (video_fitter.cu:187-208)     */
+    TDI_TOO_FEW_PIXELS = 7    /* nothing rendered inside the crop                          */
+};
+
+/* Per-frame termination reasons reported by the fits (bit flags). */
+enum tdi_stop {
+    TDI_STOP_MAXITER = 1,     /* iteration budget used up                                  */
+    TDI_STOP_DECREMENT = 2,   /* -delta.g < decrement_tol (model_fitter.cu:341)            */
+    TDI_STOP_STEP = 4,        /* step t fell below t_min (model_fitter.cu:241)             */
+    TDI_STOP_NAN = 8,         /* NaN direction                                             */
+    TDI_STOP_NOT_PD = 16      /* Cholesky failure                                          */
+};
+
+/* Shared model: mesh, bases and coefficient bounds (Renderer ctor, renderer.cu:22-442;
+ * Logbarrier_Initializer bounds, logbarrier_initializer.cu:270-286). */
+typedef struct tdi_model_desc {
+    int32_t n_vertices;
+    int32_t n_triangles;
+    int32_t k_id, k_tex, k_exp;
+    const float* mean_shape;     /* [V][3]                                   */
+    const float* id_basis;       /* [3V][k_id]  (NULL if k_id == 0)          */
+    const float* exp_basis;      /* [3V][k_exp]                              */
+    const float* tex_mean;       /* [V]                                      */
+    const float* tex_basis;      /* [V][k_tex] (NULL if k_tex == 0)          */
+    const int32_t* triangles;    /* [NT][3]                                  */
+    const int32_t* landmark_ids; /* [51] vertex ids of the 51 landmarks      */
+    const float* id_lb;  const float* id_ub;    /* [k_id]  */
+    const float* tex_lb; const float* tex_ub;   /* [k_tex] */
+    const float* exp_lb; const float* exp_ub;   /* [k_exp] */
+} tdi_model_desc;
+
+/* One frame: camera, crop, observation, landmarks, landmark box, fixed light. */
+typedef struct tdi_frame_desc {
+    float phix, phiy, cx, cy;          /* pinhole intrinsics in image pixels (camera.h:15-60)   */
+    int32_t x0, y0, width, height;     /* crop origin and size in image pixels                   */
+    float face_size;                   /* compute_face_size of the landmarks (funcs.cu:44-53)    */
+    const float* obs;                  /* [width][height] raw grey crop, column-major            */
+    const float* lmk_x;                /* [51] detected landmarks, image pixels                  */
+    const float* lmk_y;
+    const float* box_lx;               /* [51] landmark box, face-size units                     */
+    const float* box_ux;               /*      (logbarrier_initializer.cu:501-507)               */
+    const float* box_ly;
+    const float* box_uy;
+    float lambda[3];                   /* light position when light is held fixed                 */
+    float Lintensity;                  /* (video_fitter.cu:115-116 initial values)                */
+} tdi_frame_desc;
+
+/* Fit options.  Defaults (tdi_default_options) are the reference's constants. */
+typedef struct tdi_fit_options {
+    int32_t fit_id, fit_tex, fit_exp, fit_light;  /* unknown blocks                                 */
+    int32_t max_iters;          /* GN iterations (model_fitter.cu:163: 100)                        */
+    int32_t max_ls;             /* line-search evaluations per iteration (model_fitter.cu:224: 100) */
+    int32_t ntmp_cap;           /* pixels kept per triangle, 0 = all (constants.h:19 uses 4)       */
+    int32_t use_landmarks;      /* landmark-box log barrier on/off                                  */
+    int32_t niters_success;     /* success if iterations > this (config.cu:99: 6)                  */
+    float data_weight;          /* w = tau_logbarrier = 5*5 (model_fitter.cu:137-159)              */
+    float armijo_alpha;         /* 0.4 (model_fitter.cu:219)                                       */
+    float armijo_beta;          /* 0.8 (model_fitter.cu:220)                                       */
+    float t_min;                /* 0.075 for T=1 (model_fitter.cu:149-151)                          */
+    float decrement_tol;        /* 3e-4 (model_fitter.cu:341)                                      */
+    float blur_sigma;           /* KERNEL_SIGMA config key (config.cu:53; cfg files use 0.75)      */
+    float ref_face_size;        /* REF_FACE_SIZE; "face too large" if > 2x (renderer.cu:845-857), 0 = off */
+} tdi_fit_options;
+
+/* Per-frame fit report. */
+typedef struct tdi_fit_report {
+    int32_t status;        /* tdi_status                        */
+    int32_t stop;          /* tdi_stop flags                    */
+    int32_t iters;         /* outer GN iterations started       */
+    int32_t ls_evals;      /* line-search objective evaluations */
+    int32_t n_pixels;      /* rendered pixels at the last render */
+    int32_t success;       /* iters > niters_success (model_fitter.cu:374) */
+    float objective;       /* F at the returned theta           */
+    float pad;
+} tdi_fit_report;
+
+typedef struct tdi_ctx tdi_ctx;       /* model resident in HBM                        */
+typedef struct tdi_batch tdi_batch;   /* frames resident in HBM + per-frame scratch    */
+
+/* Number of unknowns per frame for a model/options pair. */
+int32_t tdi_num_unknowns(const tdi_model_desc* model, const tdi_fit_options* opts);
+
+/* Fill *opts with the reference defaults (config.cu:15-140, model_fitter.cu:137-345). */
+void tdi_default_options(tdi_fit_options* opts);
+
+/* Blur taps, 15 entries, for a KERNEL_SIGMA value (video_fitter.cu:51-63).  Returns
+ * TDI_ERR_ARG for a sigma the reference has no table for. */
+int tdi_blur_taps(float sigma, float taps[15]);
+
+/* ---- Context: replaces Renderer/Optimizer/Solver construction (renderer.cu:22-442,
+ *      Optimizer.h:249-405, solver.h:20-68, video_fitter.cu:16-81). ---- */
+int tdi_ctx_create(const tdi_model_desc* model, const tdi_fit_options* opts, int device, tdi_ctx** out);
+int tdi_ctx_destroy(tdi_ctx* ctx);
+const char* tdi_last_error(void);
+
+/* ---- Batch: frames uploaded once; the observed crop is blurred once and its normalised
+ *      gradients precomputed (video_fitter.cu:537-550, funcs.cu:876-896). ---- */
+int tdi_batch_create(tdi_ctx* ctx, int32_t n_frames, const tdi_frame_desc* frames, tdi_batch** out);
+int tdi_batch_destroy(tdi_batch* batch);
+
+/* ---- Building blocks (all on device pointers, stream-ordered) ---- */
+
+/* R(u) and dR/du_i, column-major 3x3 — RotationComputer::process (rotation_header.cu:46-149).
+ * u: [n][3], R: [n][9], dR: [n][27] (may be NULL). */
+int tdi_rotation(int32_t n, const float* u, float* R, float* dR, void* stream);
+
+/* Vertex synthesis + view transform + projection for every frame —
+ * Renderer::compute_nonrigid_shape2 (renderer.cu:806-861), view_transform_3d_pts_and_render_2d
+ * (funcs.cu:395-453).  theta: [n][K] device.  xyz: [n][V][3] camera space, xyp: [n][V][2] pixels. */
+int tdi_synthesize(tdi_batch* b, const float* theta, float* xyz, float* xyp, void* stream);
+
+/* Rasterise projected vertices into packed depth|triangle keys —
+ * get_pixels_to_render + atomicMinFloat + keep_only_minZ + remove_if (funcs.cu:900-1029,
+ * renderer.cu:657-667).  Stand-alone form used by the visibility parity tests: xyz/xyp as
+ * produced by tdi_synthesize (or any float input), keys: [sum W*H] u64 in crop order,
+ * bary: [sum W*H][2] (alpha,beta) of the winning triangle.  Pixel offsets per frame are
+ * the exclusive prefix sum of W*H over the batch. */
+int tdi_rasterize(tdi_batch* b, const float* xyz, const float* xyp, uint64_t* keys, float* bary, void* stream);
+
+/* Render + objective at theta: Renderer::render (renderer.cu:635-730) plus the barrier
+ * objective (logbarrier_initializer.cu:1921-2081).  img: [sum W*H] blurred rendered crop
+ * (may be NULL); obj: [n]; npix: [n] rendered pixel counts. */
+int tdi_render(tdi_batch* b, const float* theta, float* img, float* obj, int32_t* npix, void* stream);
+
+/* Normal equations at theta, FP32 accumulation: H [n][K][K], g [n][K], obj [n] —
+ * get_obj_hessian_and_gradient_multiframe + DerivativeComputer::compute_hessian_and_gradient
+ * (model_fitter.cu:21-116, derivative_computer.cu:22-213) with the barrier terms merged. */
+int tdi_normal_equations(tdi_batch* b, const float* theta, float* H, float* g, float* obj, void* stream);
+
+/* Same, with J split into BF16 hi/lo parts and J^T J, J^T r on MFMA (FP32 accumulate). */
+int tdi_normal_equations_mixed(tdi_batch* b, const float* theta, float* H, float* g, float* obj, void* stream);
+
+/* Batched Cholesky solve of H delta = -g.  mode 0: FP64 factorisation (Solver::solve,
+ * solver.h:70-105, cusolverDnDpotrf/Dpotrs); mode 1: FP32 factorisation + one FP64
+ * residual-refinement step.  status: [n] tdi_status. */
+int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, float* delta,
+                       int32_t* status, int32_t mode, void* stream);
+
+/* ---- Fits ---- */
+
+/* Plain path: fit_3DMM_shape_rigid (model_fitter.cu:121-375) for one frame at a time,
+ * host-driven loop, FP32 J^T J, FP64 Cholesky, no graph.  theta: [n][K] HOST, in/out. */
+int tdi_fit_plain(tdi_batch* b, float* theta, tdi_fit_report* reports);
+
+/* Batched mixed-precision path: every frame of the batch in one device-resident loop,
+ * BF16 hi/lo J^T J on MFMA, FP32 Cholesky + FP64 refinement, per-frame line-search masks,
+ * iterations captured in a hipGraph.  theta: [n][K] DEVICE, in/out; reports: HOST [n]. */
+int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDI_API_H_ */

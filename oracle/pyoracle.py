@@ -1,0 +1,167 @@
+"""ctypes loader of the FP64 OpenMP oracle (libtdi_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  Never imported by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import importlib
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libtdi_oracle.so")
+abi = importlib.import_module("3di_amd.abi")
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint64)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        L.orc_num_unknowns.restype = C.c_int
+        L.orc_blur_taps.argtypes = [C.c_double, _dp]
+        L.orc_rodrigues.argtypes = [_dp, _dp, _dp]
+        L.orc_blur.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp]
+        L.orc_rasterize.argtypes = [C.c_int, _ip, _fp, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    _up, _fp]
+        L.orc_geometry.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.orc_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _up, _fp, _dp, _dp, _dp]
+        L.orc_render.restype = C.c_int
+        L.orc_normal_equations.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, _dp, _ip,
+                                           C.c_int]
+        L.orc_normal_equations.restype = C.c_int
+        L.orc_barrier.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _dp]
+        L.orc_barrier.restype = C.c_double
+        L.orc_cholesky_solve.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_fit.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, C.c_void_p, C.c_int]
+        L.orc_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def blur_taps(sigma):
+    t = np.zeros(15)
+    rc = lib().orc_blur_taps(sigma, p(t))
+    if rc:
+        raise ValueError(sigma)
+    return t
+
+
+def rodrigues(u):
+    R = np.zeros(9)
+    dR = np.zeros(27)
+    lib().orc_rodrigues(p(_d(u)), p(R), p(dR))
+    return R.reshape(3, 3).T.copy(), dR.reshape(3, 3, 3).transpose(0, 2, 1).copy()
+
+
+def blur(img, W, H, sigma):
+    out = np.zeros(W * H)
+    lib().orc_blur(p(_d(img)), p(out), W, H, p(blur_taps(sigma)))
+    return out
+
+
+def rasterize(tris, xp, yp, z, x0, y0, W, H, ntmp=0):
+    keys = np.zeros(W * H, np.uint64)
+    bary = np.zeros(2 * W * H, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    xp, yp, z = (np.ascontiguousarray(a, np.float32) for a in (xp, yp, z))
+    lib().orc_rasterize(tris.shape[0], p(tris, _ip), p(xp, _fp), p(yp, _fp), p(z, _fp), x0, y0, W, H, ntmp,
+                        p(keys, _up), p(bary, _fp))
+    return keys, bary
+
+
+class Scene:
+    """A model + frames + options bound to the oracle."""
+
+    def __init__(self, model, frames, opts):
+        self.model = model
+        self.md = abi.ModelDesc(model)
+        self.fd = abi.FrameDescs(frames)
+        self.frames = frames
+        self.opts = opts
+        self.K = lib().orc_num_unknowns(self.md.ref, C.byref(opts))
+
+    def _f(self, i):
+        return self.fd.ptr(i)
+
+    def geometry(self, i, theta):
+        V = self.model.n_vertices
+        xyz = np.zeros(3 * V)
+        xyp = np.zeros(2 * V)
+        R = np.zeros(9)
+        dR = np.zeros(27)
+        lib().orc_geometry(self.md.ref, self._f(i), C.byref(self.opts), p(_d(theta)), p(xyz), p(xyp), p(R), p(dR))
+        return xyz.reshape(V, 3), xyp.reshape(V, 2), R, dR
+
+    def render(self, i, theta):
+        f = self.frames[i]
+        A = f.width * f.height
+        keys = np.zeros(A, np.uint64)
+        bary = np.zeros(2 * A, np.float32)
+        raw = np.zeros(A)
+        img = np.zeros(A)
+        obj = np.zeros(3)
+        n = lib().orc_render(self.md.ref, self._f(i), C.byref(self.opts), p(_d(theta)), p(keys, _up), p(bary, _fp),
+                             p(raw), p(img), p(obj))
+        return dict(keys=keys, bary=bary, raw=raw, img=img, obj=obj, npix=n)
+
+    def normal_equations(self, i, theta, want_dI=False, cap=200000):
+        K = self.K
+        H = np.zeros(K * K)
+        g = np.zeros(K)
+        obj = np.zeros(3)
+        dI = np.zeros(cap * K) if want_dI else None
+        nb = np.zeros(cap * 4, np.int32) if want_dI else None
+        n = lib().orc_normal_equations(self.md.ref, self._f(i), C.byref(self.opts), p(_d(theta)), p(H), p(g),
+                                       p(obj), p(dI), p(nb, _ip), cap)
+        out = dict(H=H.reshape(K, K), g=g, obj=obj, npix=n)
+        if want_dI:
+            out["dI"] = dI[:n * K].reshape(n, K)
+            out["nbr"] = nb[:4 * n].reshape(n, 4)
+        return out
+
+    def barrier(self, i, theta):
+        K = self.K
+        g = np.zeros(K)
+        H = np.zeros(K * K)
+        obj = lib().orc_barrier(self.md.ref, self._f(i), C.byref(self.opts), p(_d(theta)), p(g), p(H))
+        return obj, g, H.reshape(K, K)
+
+    def fit(self, theta0, n_threads=0, frames=None):
+        n = len(self.frames)
+        th = np.ascontiguousarray(np.array(theta0, np.float64).reshape(n, self.K))
+        reps = (abi.tdi_fit_report * n)()
+        lib().orc_fit(self.md.ref, n, self.fd.ptr(0), C.byref(self.opts), p(th), reps, n_threads)
+        return th, list(reps)
+
+
+def cholesky_solve(H, g):
+    K = len(g)
+    d = np.zeros(K)
+    rc = lib().orc_cholesky_solve(K, p(_d(H)), p(_d(g)), p(d))
+    return rc, d
