@@ -1,0 +1,766 @@
+// capi.hip — the C ABI of include/tdi_api.h: context, batch, building blocks and the two fits.
+//
+// Ownership follows the reference's split (Renderer/Optimizer own device memory,
+// renderer.cu:28-72, Optimizer.h:249-405) but everything lives in two objects:
+//   tdi_ctx   — the model in HBM (mean, bases, triangles, landmark ids, bounds, blur taps);
+//   tdi_batch — the frames in HBM (normalised observation gradients, landmarks) plus one
+//               workspace sized for a chunk of frames.  No allocation happens inside a fit.
+#include "common.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+namespace tdi {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+// Reference taps (video_fitter.cu:51-63): KERNEL_SIGMA 0.5/0.75/1 use that sigma, 2.0 is the
+// sigma = 1.5 table; normalised Gaussian of width 15 (video_fitter.cu:2836-2849).
+static int taps_for(float key, float* taps)
+{
+    double s;
+    if (key == 0.5f) s = 0.5;
+    else if (key == 0.75f) s = 0.75;
+    else if (key == 1.0f) s = 1.0;
+    else if (key == 2.0f) s = 1.5;
+    else return TDI_ERR_ARG;
+    double t[15], sum = 0;
+    for (int i = 0; i < 15; ++i) {
+        const double x = i - 7;
+        t[i] = std::exp(-x * x / (2 * s * s));
+        sum += t[i];
+    }
+    for (int i = 0; i < 15; ++i) taps[i] = (float)(t[i] / sum);
+    return TDI_OK;
+}
+
+int launch_misc_pack(int n, int V, const float* xyz, const float* xyp, float4* vproj, hipStream_t s);
+int launch_misc_unpack(int n, int V, const float* P, const float4* vproj, float* xyz, float* xyp, hipStream_t s);
+int launch_misc_fill_f32(float* p, long long count, float v, hipStream_t s);
+int launch_misc_f64_to_f32(const double* in, float* out, int n, hipStream_t s);
+int launch_misc_rotation(int n, const float* u, float* R, float* dR, hipStream_t s);
+
+} // namespace tdi
+
+using namespace tdi;
+
+struct Alloc {
+    std::vector<void*> ptrs;
+    template <typename T>
+    T* get(size_t count)
+    {
+        void* p = nullptr;
+        if (count == 0) count = 1;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+    void release()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+struct tdi_ctx {
+    int device = 0;
+    tdi_fit_options opts{};
+    Layout L{};
+    ModelGPU m{};
+    float taps[15];
+    Alloc mem;
+};
+
+struct Workspace {
+    int C = 0;                 // frame slots
+    long long cap = 0;         // crop pixels
+    FrameGPU* frames = nullptr;
+    float *P0 = nullptr, *P = nullptr;
+    float4* vproj = nullptr;
+    uint64_t* keys = nullptr;
+    int *cum = nullptr, *vis = nullptr;
+    float* bary = nullptr;
+    float *img = nullptr, *tmp = nullptr, *blurred = nullptr;
+    float4* pixg = nullptr;
+    float* dI = nullptr;
+    float *theta = nullptr, *theta_try = nullptr, *delta = nullptr, *H = nullptr, *g = nullptr;
+    double *obj = nullptr, *objb = nullptr;
+    int *npix = nullptr, *status = nullptr, *face_ok = nullptr, *mask = nullptr, *searching = nullptr,
+        *evalmask = nullptr;
+    FitState* st = nullptr;
+    float* partials = nullptr;
+    float* lmk = nullptr;      // [C][6][51]
+    int S = 1;                 // J^T J splits per frame (fixed per workspace)
+    // chunk bookkeeping (host)
+    std::vector<FrameGPU> hframes;
+    int n_cur = 0;
+    int max_area = 0;
+};
+
+struct tdi_batch {
+    tdi_ctx* ctx = nullptr;
+    int n = 0;
+    long long total = 0;
+    int max_area = 0;
+    std::vector<FrameGPU> hframes;    // obs_off = batch offsets
+    float2* obs_g = nullptr;
+    float* lmk = nullptr;             // [n][6][51]
+    Workspace ws;
+    Alloc mem;
+    hipGraphExec_t graph = nullptr;
+    hipStream_t stream = nullptr;     // capture-capable stream owned by the batch (default priority)
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+};
+
+extern "C" {
+
+const char* tdi_last_error(void) { return g_err.c_str(); }
+
+void tdi_default_options(tdi_fit_options* o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->fit_exp = 1;
+    o->max_iters = 100;
+    o->max_ls = 100;
+    o->ntmp_cap = 0;
+    o->use_landmarks = 1;
+    o->niters_success = 6;
+    o->data_weight = 25.f;
+    o->armijo_alpha = 0.4f;
+    o->armijo_beta = 0.8f;
+    o->t_min = 0.075f;
+    o->decrement_tol = 3e-4f;
+    o->blur_sigma = 0.75f;
+    o->ref_face_size = 0.f;
+}
+
+int32_t tdi_num_unknowns(const tdi_model_desc* m, const tdi_fit_options* o) { return make_layout(m, o).K; }
+
+int tdi_blur_taps(float sigma, float taps[15]) { return taps_for(sigma, taps); }
+
+int tdi_ctx_create(const tdi_model_desc* md, const tdi_fit_options* opts, int device, tdi_ctx** out)
+{
+    if (!md || !opts || !out) { set_error("null argument"); return TDI_ERR_ARG; }
+    if (md->n_vertices <= 0 || md->n_triangles <= 0 || !md->mean_shape || !md->triangles || !md->landmark_ids ||
+        !md->tex_mean) {
+        set_error("model: empty mesh or missing arrays");
+        return TDI_ERR_ARG;
+    }
+    if ((opts->fit_exp && (md->k_exp <= 0 || !md->exp_basis || !md->exp_lb || !md->exp_ub)) ||
+        (opts->fit_id && (md->k_id <= 0 || !md->id_basis || !md->id_lb || !md->id_ub)) ||
+        (opts->fit_tex && (md->k_tex <= 0 || !md->tex_basis || !md->tex_lb || !md->tex_ub))) {
+        set_error("model: fitted block without basis or bounds");
+        return TDI_ERR_ARG;
+    }
+    for (int i = 0; i < NLMK; ++i)
+        if (md->landmark_ids[i] < 0 || md->landmark_ids[i] >= md->n_vertices) {
+            set_error("landmark id %d out of range", md->landmark_ids[i]);
+            return TDI_ERR_ARG;
+        }
+    for (long long i = 0; i < 3LL * md->n_triangles; ++i)
+        if (md->triangles[i] < 0 || md->triangles[i] >= md->n_vertices) {
+            set_error("triangle index out of range at %lld", i);
+            return TDI_ERR_ARG;
+        }
+    tdi_ctx* c = new tdi_ctx;
+    c->device = device;
+    c->opts = *opts;
+    c->L = make_layout(md, opts);
+    if (taps_for(opts->blur_sigma, c->taps) != TDI_OK) {
+        delete c;
+        set_error("blur_sigma %g has no reference table (0.5, 0.75, 1, 2)", opts->blur_sigma);
+        return TDI_ERR_ARG;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        set_error("hipSetDevice(%d) failed", device);
+        return TDI_ERR_HIP;
+    }
+    const int V = md->n_vertices, NT = md->n_triangles;
+    auto up = [&](const void* src, size_t bytes) -> void* {
+        void* d = c->mem.get<char>(bytes);
+        if (d && src) (void)hipMemcpy(d, src, bytes, hipMemcpyHostToDevice);
+        return d;
+    };
+    ModelGPU& m = c->m;
+    m.V = V; m.NT = NT; m.k_id = md->k_id; m.k_tex = md->k_tex; m.k_exp = md->k_exp;
+    m.mean = (const float*)up(md->mean_shape, sizeof(float) * 3 * V);
+    m.exp_basis = md->k_exp ? (const float*)up(md->exp_basis, sizeof(float) * 3 * V * (size_t)md->k_exp) : nullptr;
+    m.id_basis = md->k_id ? (const float*)up(md->id_basis, sizeof(float) * 3 * V * (size_t)md->k_id) : nullptr;
+    m.tex_mean = (const float*)up(md->tex_mean, sizeof(float) * V);
+    m.tex_basis = md->k_tex ? (const float*)up(md->tex_basis, sizeof(float) * V * (size_t)md->k_tex) : nullptr;
+    m.tris = (const int*)up(md->triangles, sizeof(int) * 3 * (size_t)NT);
+    m.lmk_ids = (const int*)up(md->landmark_ids, sizeof(int) * NLMK);
+    const Layout& L = c->L;
+    std::vector<float> lb(L.K, -INFINITY), ub(L.K, INFINITY);
+    for (int k = 0; k < L.ntex; ++k) { lb[L.off_tex + k] = md->tex_lb[k]; ub[L.off_tex + k] = md->tex_ub[k]; }
+    for (int k = 0; k < L.nid; ++k) { lb[L.off_id + k] = md->id_lb[k]; ub[L.off_id + k] = md->id_ub[k]; }
+    for (int k = 0; k < L.nexp; ++k) { lb[L.off_exp + k] = md->exp_lb[k]; ub[L.off_exp + k] = md->exp_ub[k]; }
+    m.lb = (const float*)up(lb.data(), sizeof(float) * L.K);
+    m.ub = (const float*)up(ub.data(), sizeof(float) * L.K);
+    for (void* p : c->mem.ptrs)
+        if (!p) { c->mem.release(); delete c; set_error("hipMalloc failed for the model"); return TDI_ERR_NOMEM; }
+    if (hipDeviceSynchronize() != hipSuccess) { c->mem.release(); delete c; set_error("upload failed"); return TDI_ERR_HIP; }
+    *out = c;
+    return TDI_OK;
+}
+
+int tdi_ctx_destroy(tdi_ctx* c)
+{
+    if (!c) return TDI_OK;
+    (void)hipSetDevice(c->device);
+    c->mem.release();
+    delete c;
+    return TDI_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------- workspace
+static int ws_alloc(tdi_batch* b, int C, long long cap)
+{
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    Alloc& A = b->mem;
+    const int V = c->m.V, K = c->L.K;
+    w.C = C;
+    w.cap = cap;
+    w.max_area = b->max_area;
+    w.frames = A.get<FrameGPU>(C);
+    w.P0 = A.get<float>((size_t)C * V * 3);
+    w.P = A.get<float>((size_t)C * V * 3);
+    w.vproj = A.get<float4>((size_t)C * V);
+    w.keys = A.get<uint64_t>(cap);
+    w.cum = A.get<int>(cap);
+    w.vis = A.get<int>(cap);
+    w.bary = A.get<float>(2 * cap);
+    w.img = A.get<float>(cap);
+    w.tmp = A.get<float>(cap);
+    w.blurred = A.get<float>(cap);
+    w.pixg = A.get<float4>(cap);
+    w.dI = A.get<float>((size_t)cap * K);
+    w.theta = A.get<float>((size_t)C * K);
+    w.theta_try = A.get<float>((size_t)C * K);
+    w.delta = A.get<float>((size_t)C * K);
+    w.H = A.get<float>((size_t)C * K * K);
+    w.g = A.get<float>((size_t)C * K);
+    w.obj = A.get<double>(C);
+    w.objb = A.get<double>(C);
+    w.npix = A.get<int>(C);
+    w.status = A.get<int>(C);
+    w.face_ok = A.get<int>(C);
+    w.mask = A.get<int>(C);
+    w.searching = A.get<int>(C);
+    w.evalmask = A.get<int>(C);
+    w.st = A.get<FitState>(C);
+    w.S = jtj_splits(C, b->max_area);
+    const size_t pf = std::max(jtj_partials_floats(C, w.S, K, false), jtj_partials_floats(C, w.S, K, true));
+    w.partials = A.get<float>(pf);
+    w.lmk = A.get<float>((size_t)C * 6 * NLMK);
+    for (void* p : A.ptrs)
+        if (!p) { set_error("workspace allocation failed (C=%d, cap=%lld)", C, cap); return TDI_ERR_NOMEM; }
+    return TDI_OK;
+}
+
+// Load frames [i0, i0+n) into the workspace: rebased frame descriptors and landmarks.
+static int ws_load(tdi_batch* b, int i0, int n, hipStream_t s)
+{
+    Workspace& w = b->ws;
+    if (n > w.C) { set_error("chunk of %d frames exceeds workspace (%d)", n, w.C); return TDI_ERR_ARG; }
+    w.hframes.assign(w.C, FrameGPU{});
+    long long off = 0;
+    for (int i = 0; i < n; ++i) {
+        FrameGPU f = b->hframes[i0 + i];
+        f.pix_off = off;
+        off += (long long)f.W * f.H;
+        w.hframes[i] = f;
+    }
+    if (off > w.cap) { set_error("chunk needs %lld crop pixels, workspace has %lld", off, w.cap); return TDI_ERR_ARG; }
+    // padding slots: a 1x1 crop at offset 0, never active
+    for (int i = n; i < w.C; ++i) {
+        FrameGPU f = b->hframes[i0];
+        f.W = 1; f.H = 1; f.pix_off = 0;
+        w.hframes[i] = f;
+    }
+    w.n_cur = n;
+    TDI_HIP(hipMemcpyAsync(w.frames, w.hframes.data(), sizeof(FrameGPU) * w.C, hipMemcpyHostToDevice, s));
+    TDI_HIP(hipMemcpyAsync(w.lmk, b->lmk + (size_t)i0 * 6 * NLMK, sizeof(float) * 6 * NLMK * n, hipMemcpyDeviceToDevice, s));
+    return TDI_OK;
+}
+
+static LmkArrays lmk_arrays(const Workspace& w)
+{
+    LmkArrays a;
+    a.base = w.lmk;
+    return a;
+}
+
+extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd, tdi_batch** out)
+{
+    if (!c || !fd || !out || n <= 0) { set_error("null argument or empty batch"); return TDI_ERR_ARG; }
+    (void)hipSetDevice(c->device);
+    tdi_batch* b = new tdi_batch;
+    b->ctx = c;
+    b->n = n;
+    long long off = 0;
+    int maxA = 0;
+    for (int i = 0; i < n; ++i) {
+        const tdi_frame_desc& d = fd[i];
+        if (d.width <= 0 || d.height <= 0 || !d.obs || !d.lmk_x || !d.lmk_y || !d.box_lx || !d.box_ux || !d.box_ly ||
+            !d.box_uy) {
+            delete b;
+            set_error("frame %d: empty crop or missing arrays", i);
+            return TDI_ERR_ARG;
+        }
+        if ((long long)d.width * d.height > (1LL << 30)) { delete b; set_error("frame %d: crop too large", i); return TDI_ERR_ARG; }
+        FrameGPU f{};
+        f.phix = d.phix; f.phiy = d.phiy; f.cx = d.cx; f.cy = d.cy;
+        f.x0 = d.x0; f.y0 = d.y0; f.W = d.width; f.H = d.height;
+        f.face_size = d.face_size;
+        for (int a = 0; a < 3; ++a) f.lam[a] = d.lambda[a];
+        f.Lint = d.Lintensity;
+        f.obs_off = off;
+        f.pix_off = off;
+        off += (long long)d.width * d.height;
+        maxA = std::max(maxA, d.width * d.height);
+        b->hframes.push_back(f);
+    }
+    b->total = off;
+    b->max_area = maxA;
+    hipStream_t s = 0;
+    // observations: upload, blur, normalised gradients (once per batch)
+    b->obs_g = b->mem.get<float2>(off);
+    b->lmk = b->mem.get<float>((size_t)n * 6 * NLMK);
+    FrameGPU* dfr = b->mem.get<FrameGPU>(n);
+    if (!b->obs_g || !b->lmk || !dfr) { b->mem.release(); delete b; set_error("batch allocation failed"); return TDI_ERR_NOMEM; }
+    {
+        float *raw = nullptr, *tmp = nullptr, *blr = nullptr;
+        if (hipMalloc(&raw, sizeof(float) * off) != hipSuccess || hipMalloc(&tmp, sizeof(float) * off) != hipSuccess ||
+            hipMalloc(&blr, sizeof(float) * off) != hipSuccess) {
+            (void)hipFree(raw); (void)hipFree(tmp); (void)hipFree(blr);
+            b->mem.release();
+            delete b;
+            set_error("observation staging allocation failed");
+            return TDI_ERR_NOMEM;
+        }
+        for (int i = 0; i < n; ++i)
+            (void)hipMemcpy(raw + b->hframes[i].obs_off, fd[i].obs, sizeof(float) * fd[i].width * fd[i].height,
+                            hipMemcpyHostToDevice);
+        (void)hipMemcpy(dfr, b->hframes.data(), sizeof(FrameGPU) * n, hipMemcpyHostToDevice);
+        int rc = launch_obs_prepare(dfr, n, c->taps, raw, tmp, blr, b->obs_g, maxA, s);
+        (void)hipDeviceSynchronize();
+        (void)hipFree(raw); (void)hipFree(tmp); (void)hipFree(blr);
+        if (rc) { b->mem.release(); delete b; return rc; }
+    }
+    {
+        std::vector<float> h((size_t)n * 6 * NLMK);
+        const size_t S = (size_t)n * NLMK;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < NLMK; ++j) {
+                h[0 * S + i * NLMK + j] = fd[i].lmk_x[j];
+                h[1 * S + i * NLMK + j] = fd[i].lmk_y[j];
+                h[2 * S + i * NLMK + j] = fd[i].box_lx[j];
+                h[3 * S + i * NLMK + j] = fd[i].box_ux[j];
+                h[4 * S + i * NLMK + j] = fd[i].box_ly[j];
+                h[5 * S + i * NLMK + j] = fd[i].box_uy[j];
+            }
+        // per-frame layout [n][6][51] so that a chunk is one contiguous copy
+        std::vector<float> hp((size_t)n * 6 * NLMK);
+        for (int i = 0; i < n; ++i)
+            for (int q = 0; q < 6; ++q)
+                for (int j = 0; j < NLMK; ++j) hp[((size_t)i * 6 + q) * NLMK + j] = h[q * S + i * NLMK + j];
+        (void)hipMemcpy(b->lmk, hp.data(), sizeof(float) * hp.size(), hipMemcpyHostToDevice);
+    }
+    // workspace: whole batch if it fits a budget, else chunks
+    const char* env = getenv("TDI_WORKSPACE_GB");
+    double budget_gb = env ? atof(env) : 48.0;
+    const int V = c->m.V, K = c->L.K;
+    const double per_frame = (double)V * 40 + 64.0 * K * K;
+    const double per_pixel = 60.0 + 4.0 * K;
+    int C = n;
+    long long cap = off;
+    while (C > 1 && (per_frame * C + per_pixel * (double)maxA * C) > budget_gb * 1e9) C = (C + 1) / 2;
+    if (C < n) cap = (long long)maxA * C;
+    int rc = ws_alloc(b, C, cap);
+    if (rc) { b->mem.release(); delete b; return rc; }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess) {
+        b->mem.release();
+        delete b;
+        set_error("stream/event creation failed");
+        return TDI_ERR_HIP;
+    }
+    *out = b;
+    return TDI_OK;
+}
+
+extern "C" int tdi_batch_destroy(tdi_batch* b)
+{
+    if (!b) return TDI_OK;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+    if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+    b->mem.release();
+    delete b;
+    return TDI_OK;
+}
+
+// ---------------------------------------------------------------------------- evaluation
+// Objective at theta for the frames in `mask`: synthesis, face check, raster, resolve,
+// barrier objective, shading, blur, gradients (Renderer::render + evaluate_objective_function).
+static int eval_render(tdi_batch* b, int n, const float* theta, const int* mask, bool mixed, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    int rc;
+    if (mixed)
+        rc = launch_synthesize_gemm(c->m, c->L, w.frames, n, theta, mask, w.P0, w.P, w.vproj, nullptr, s);
+    else
+        rc = launch_synthesize(c->m, c->L, w.frames, n, theta, mask, w.P0, w.P, w.vproj, s);
+    if (rc) return rc;
+    if ((rc = launch_face_check(n, c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, mask, w.face_ok, s))) return rc;
+    if ((rc = launch_clear_keys_masked(w.frames, n, mask, w.keys, w.max_area, s))) return rc;
+    if ((rc = launch_raster(c->m, w.frames, n, mask, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+    if ((rc = launch_resolve(c->m, w.frames, n, mask, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, s)))
+        return rc;
+    if ((rc = launch_barrier(c->m, c->L, w.frames, n, mask, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
+                             w.objb, w.H, w.g, s)))
+        return rc;
+    if ((rc = launch_shade(c->m, c->L, w.frames, n, mask, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
+                           w.max_area, s)))
+        return rc;
+    if ((rc = launch_blur(w.frames, n, mask, c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
+    return launch_grads(w.frames, n, mask, w.vis, w.npix, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.obj,
+                        w.max_area, s);
+}
+
+// Objective, gradient and Gauss-Newton matrix at theta (get_obj_hessian_and_gradient_multiframe).
+static int eval_full(tdi_batch* b, int n, const float* theta, const int* mask, bool mixed, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    int rc = eval_render(b, n, theta, mask, mixed, s);
+    if (rc) return rc;
+    if ((rc = launch_dI(c->m, c->L, w.frames, n, mask, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
+                        w.max_area, s)))
+        return rc;
+    if (mixed)
+        rc = launch_jtj_mfma(c->L, w.frames, n, mask, w.vis, w.npix, w.cum, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
+                             w.partials, w.S, s);
+    else
+        rc = launch_jtj_f32(c->L, w.frames, n, mask, w.vis, w.npix, w.cum, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
+                            w.partials, w.S, s);
+    if (rc) return rc;
+    return launch_barrier(c->m, c->L, w.frames, n, mask, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj, w.objb,
+                          w.H, w.g, s);
+}
+
+static int need_single_chunk(tdi_batch* b)
+{
+    if (b->ws.C < b->n) {
+        set_error("building-block calls need the whole batch in one workspace (%d frames > %d)", b->n, b->ws.C);
+        return TDI_ERR_ARG;
+    }
+    return TDI_OK;
+}
+
+// active mask for real frames of the current chunk
+static int set_mask_all(tdi_batch* b, hipStream_t s)
+{
+    std::vector<int> h(b->ws.C, 0);
+    for (int i = 0; i < b->ws.n_cur; ++i) h[i] = 1;
+    TDI_HIP(hipMemcpyAsync(b->ws.mask, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s));
+    TDI_HIP(hipStreamSynchronize(s));
+    return TDI_OK;
+}
+
+extern "C" {
+
+int tdi_rotation(int32_t n, const float* u, float* R, float* dR, void* stream)
+{
+    return launch_misc_rotation(n, u, R, dR, (hipStream_t)stream);
+}
+
+int tdi_synthesize(tdi_batch* b, const float* theta, float* xyz, float* xyp, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    int rc = need_single_chunk(b);
+    if (rc) return rc;
+    tdi_ctx* c = b->ctx;
+    if ((rc = ws_load(b, 0, b->n, s))) return rc;
+    if ((rc = launch_synthesize(c->m, c->L, b->ws.frames, b->n, theta, nullptr, b->ws.P0, b->ws.P, b->ws.vproj, s)))
+        return rc;
+    return launch_misc_unpack(b->n, c->m.V, b->ws.P, b->ws.vproj, xyz, xyp, s);
+}
+
+int tdi_rasterize(tdi_batch* b, const float* xyz, const float* xyp, uint64_t* keys, float* bary, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    int rc = need_single_chunk(b);
+    if (rc) return rc;
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    if ((rc = ws_load(b, 0, b->n, s))) return rc;
+    if ((rc = launch_misc_pack(b->n, c->m.V, xyz, xyp, w.vproj, s))) return rc;
+    if ((rc = launch_clear_keys(w.keys, b->total, s))) return rc;
+    if ((rc = launch_misc_fill_f32(w.bary, 2 * b->total, 0.f, s))) return rc;
+    if ((rc = launch_raster(c->m, w.frames, b->n, nullptr, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+    if ((rc = launch_resolve(c->m, w.frames, b->n, nullptr, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area,
+                             s)))
+        return rc;
+    if (keys) TDI_HIP(hipMemcpyAsync(keys, w.keys, sizeof(uint64_t) * b->total, hipMemcpyDeviceToDevice, s));
+    if (bary) TDI_HIP(hipMemcpyAsync(bary, w.bary, sizeof(float) * 2 * b->total, hipMemcpyDeviceToDevice, s));
+    return TDI_OK;
+}
+
+int tdi_render(tdi_batch* b, const float* theta, float* img, float* obj, int32_t* npix, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    int rc = need_single_chunk(b);
+    if (rc) return rc;
+    Workspace& w = b->ws;
+    if ((rc = ws_load(b, 0, b->n, s))) return rc;
+    if ((rc = eval_render(b, b->n, theta, nullptr, false, s))) return rc;
+    if (img) TDI_HIP(hipMemcpyAsync(img, w.img, sizeof(float) * b->total, hipMemcpyDeviceToDevice, s));
+    if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj, b->n, s))) return rc;
+    if (npix) TDI_HIP(hipMemcpyAsync(npix, w.npix, sizeof(int) * b->n, hipMemcpyDeviceToDevice, s));
+    return TDI_OK;
+}
+
+static int normal_eq(tdi_batch* b, const float* theta, float* H, float* g, float* obj, void* stream, bool mixed)
+{
+    hipStream_t s = (hipStream_t)stream;
+    int rc = need_single_chunk(b);
+    if (rc) return rc;
+    Workspace& w = b->ws;
+    const int K = b->ctx->L.K;
+    if ((rc = ws_load(b, 0, b->n, s))) return rc;
+    if ((rc = eval_full(b, b->n, theta, nullptr, mixed, s))) return rc;
+    if (H) TDI_HIP(hipMemcpyAsync(H, w.H, sizeof(float) * b->n * K * K, hipMemcpyDeviceToDevice, s));
+    if (g) TDI_HIP(hipMemcpyAsync(g, w.g, sizeof(float) * b->n * K, hipMemcpyDeviceToDevice, s));
+    if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj, b->n, s))) return rc;
+    return TDI_OK;
+}
+
+int tdi_normal_equations(tdi_batch* b, const float* theta, float* H, float* g, float* obj, void* stream)
+{
+    return normal_eq(b, theta, H, g, obj, stream, false);
+}
+
+int tdi_normal_equations_mixed(tdi_batch* b, const float* theta, float* H, float* g, float* obj, void* stream)
+{
+    return normal_eq(b, theta, H, g, obj, stream, true);
+}
+
+int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, float* delta, int32_t* status,
+                       int32_t mode, void* stream)
+{
+    if (n <= 0 || K <= 0 || (mode != 0 && mode != 1)) { set_error("bad cholesky arguments"); return TDI_ERR_ARG; }
+    return launch_cholesky(n, K, H, g, delta, status, mode, nullptr, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------- plain fit
+// fit_3DMM_shape_rigid (model_fitter.cu:121-375) one frame at a time: the host owns the
+// loop and reads the objective, the direction and the status back after every step, like
+// the reference's blocking cudaMemcpy pattern; FP32 J^T J, FP64 Cholesky, no graph.
+int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
+{
+    tdi_ctx* c = b->ctx;
+    const tdi_fit_options& o = c->opts;
+    const int K = c->L.K, off_tz = c->L.off_pose + 2;
+    hipStream_t s = 0;
+    Workspace& w = b->ws;
+    std::vector<float> th(K), tt(K), delta(K), g(K);
+    for (int i = 0; i < b->n; ++i) {
+        int rc = ws_load(b, i, 1, s);
+        if (rc) return rc;
+        if ((rc = set_mask_all(b, s))) return rc;
+        tdi_fit_report& rep = reports[i];
+        std::memset(&rep, 0, sizeof(rep));
+        std::memcpy(th.data(), theta_host + (size_t)i * K, sizeof(float) * K);
+        TDI_HIP(hipMemcpyAsync(w.theta, th.data(), sizeof(float) * K, hipMemcpyHostToDevice, s));
+        int outer = 0, performed = 0, ls_evals = 0;
+        bool terminate = false;
+        for (int it = 0; it < o.max_iters; ++it) {
+            outer++;
+            if (terminate) break;
+            if ((rc = eval_full(b, 1, w.theta, w.mask, false, s))) return rc;
+            if ((rc = launch_cholesky(1, K, w.H, w.g, w.delta, w.status, 0, w.mask, s))) return rc;
+            int st = 0, np = 0;
+            double obj0 = 0;
+            TDI_HIP(hipMemcpyAsync(&st, w.status, sizeof(int), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(delta.data(), w.delta, sizeof(float) * K, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(g.data(), w.g, sizeof(float) * K, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(&obj0, w.obj, sizeof(double), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(&np, w.npix, sizeof(int), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipStreamSynchronize(s));
+            performed++;
+            rep.n_pixels = np;
+            if (st != TDI_OK) { rep.stop |= TDI_STOP_NOT_PD; break; }
+            if (std::isnan(delta[0])) { rep.stop |= TDI_STOP_NAN; break; }
+            double dg = 0;
+            for (int k = 0; k < K; ++k) dg += (double)delta[k] * g[k];
+            float t = 1.f;
+            int inner = 0;
+            while (inner < o.max_ls) {
+                if (t < o.t_min) { terminate = true; rep.stop |= TDI_STOP_STEP; break; }
+                for (int k = 0; k < K; ++k) tt[k] = th[k] + t * delta[k];
+                if (tt[off_tz] < 0.f) break;
+                TDI_HIP(hipMemcpyAsync(w.theta_try, tt.data(), sizeof(float) * K, hipMemcpyHostToDevice, s));
+                if ((rc = eval_render(b, 1, w.theta_try, w.mask, false, s))) return rc;
+                double objt = 0, objb = 0;
+                int fok = 1;
+                TDI_HIP(hipMemcpyAsync(&objt, w.obj, sizeof(double), hipMemcpyDeviceToHost, s));
+                TDI_HIP(hipMemcpyAsync(&objb, w.objb, sizeof(double), hipMemcpyDeviceToHost, s));
+                TDI_HIP(hipMemcpyAsync(&fok, w.face_ok, sizeof(int), hipMemcpyDeviceToHost, s));
+                TDI_HIP(hipStreamSynchronize(s));
+                if (!fok || std::isnan(objb)) { t *= o.armijo_beta; continue; }
+                inner++;
+                ls_evals++;
+                if (-dg < o.decrement_tol) { terminate = true; rep.stop |= TDI_STOP_DECREMENT; break; }
+                if (objt < obj0 + (double)(o.armijo_alpha * t) * dg) {
+                    th = tt;
+                    TDI_HIP(hipMemcpyAsync(w.theta, th.data(), sizeof(float) * K, hipMemcpyHostToDevice, s));
+                    break;
+                }
+                t *= o.armijo_beta;
+            }
+        }
+        if (!terminate && !(rep.stop & (TDI_STOP_NAN | TDI_STOP_NOT_PD))) rep.stop |= TDI_STOP_MAXITER;
+        if ((rc = eval_render(b, 1, w.theta, w.mask, false, s))) return rc;
+        double fin = 0;
+        TDI_HIP(hipMemcpyAsync(&fin, w.obj, sizeof(double), hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipStreamSynchronize(s));
+        rep.objective = (float)fin;
+        rep.iters = performed;
+        rep.ls_evals = ls_evals;
+        rep.success = outer > o.niters_success;
+        rep.status = (rep.stop & TDI_STOP_NOT_PD) ? TDI_NOT_PD : (rep.stop & TDI_STOP_NAN) ? TDI_NAN : TDI_OK;
+        std::memcpy(theta_host + (size_t)i * K, th.data(), sizeof(float) * K);
+    }
+    return TDI_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------- batched fit
+// One GN iteration for every frame of the workspace, all decisions on the device.
+static int record_fit(tdi_batch* b, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    const tdi_fit_options& o = c->opts;
+    Workspace& w = b->ws;
+    const int n = w.C, K = c->L.K, off_tz = c->L.off_pose + 2;
+    // evaluation rounds of the line search: steps t = beta^k >= t_min, capped by max_ls
+    int rounds = 0;
+    for (double t = 1.0; t >= o.t_min && rounds < o.max_ls; t *= o.armijo_beta) ++rounds;
+    int rc;
+    for (int it = 0; it < o.max_iters; ++it) {
+        if ((rc = launch_copy_mask(n, w.st, w.mask, s))) return rc;
+        if ((rc = eval_full(b, w.C, w.theta, w.mask, true, s))) return rc;
+        if ((rc = launch_cholesky(n, K, w.H, w.g, w.delta, w.status, 1, w.mask, s))) return rc;
+        if ((rc = launch_ls_init(n, K, w.st, w.status, w.delta, w.g, w.obj, w.searching, s))) return rc;
+        for (int r = 0; r < rounds; ++r) {
+            if ((rc = launch_ls_propose(n, K, off_tz, o.t_min, w.st, w.theta, w.delta, w.theta_try, w.searching,
+                                        w.evalmask, s)))
+                return rc;
+            if ((rc = eval_render(b, w.C, w.theta_try, w.evalmask, true, s))) return rc;
+            if ((rc = launch_ls_decide(n, K, w.st, w.evalmask, w.searching, w.obj, w.objb, w.face_ok, w.theta,
+                                       w.theta_try, o, s)))
+                return rc;
+        }
+        if ((rc = launch_ls_propose(n, K, off_tz, o.t_min, w.st, w.theta, w.delta, w.theta_try, w.searching, w.evalmask,
+                                    s)))
+            return rc;
+    }
+    return TDI_OK;
+}
+
+extern "C" int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream)
+{
+    tdi_ctx* c = b->ctx;
+    const tdi_fit_options& o = c->opts;
+    Workspace& w = b->ws;
+    const int K = c->L.K;
+    // All work runs on the batch's own stream (graph capture is not allowed on the legacy
+    // default stream); it is ordered after the caller's stream and the caller's stream is
+    // ordered after it.
+    hipStream_t caller = (hipStream_t)stream;
+    hipStream_t s = b->stream;
+    TDI_HIP(hipEventRecord(b->ev_in, caller));
+    TDI_HIP(hipStreamWaitEvent(s, b->ev_in, 0));
+    const bool use_graph = getenv("TDI_NO_GRAPH") == nullptr;
+    std::vector<FitState> hst(w.C);
+    for (int i0 = 0; i0 < b->n; i0 += w.C) {
+        const int n = std::min(w.C, b->n - i0);
+        int rc = ws_load(b, i0, n, s);
+        if (rc) return rc;
+        for (int i = 0; i < w.C; ++i) {
+            FitState f{};
+            f.active = i < n;
+            hst[i] = f;
+        }
+        TDI_HIP(hipMemcpyAsync(w.st, hst.data(), sizeof(FitState) * w.C, hipMemcpyHostToDevice, s));
+        TDI_HIP(hipMemcpyAsync(w.theta, theta + (size_t)i0 * K, sizeof(float) * n * K, hipMemcpyDeviceToDevice, s));
+        if (use_graph) {
+            if (!b->graph) {
+                hipGraph_t g;
+                TDI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                rc = record_fit(b, s);
+                hipError_t ec = hipStreamEndCapture(s, &g);
+                if (rc) return rc;
+                if (ec != hipSuccess) { set_error("graph capture failed: %s", hipGetErrorString(ec)); return TDI_ERR_HIP; }
+                TDI_HIP(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(g);
+            }
+            TDI_HIP(hipGraphLaunch(b->graph, s));
+        } else {
+            if ((rc = record_fit(b, s))) return rc;
+        }
+        TDI_HIP(hipMemcpyAsync(theta + (size_t)i0 * K, w.theta, sizeof(float) * n * K, hipMemcpyDeviceToDevice, s));
+        TDI_HIP(hipMemcpyAsync(hst.data(), w.st, sizeof(FitState) * w.C, hipMemcpyDeviceToHost, s));
+        if (reports) {
+            // objective at the returned parameters
+            if ((rc = set_mask_all(b, s))) return rc;
+            if ((rc = eval_render(b, w.C, w.theta, w.mask, true, s))) return rc;
+        }
+        std::vector<double> hobj(w.C);
+        std::vector<int> hnp(w.C);
+        TDI_HIP(hipMemcpyAsync(hobj.data(), w.obj, sizeof(double) * w.C, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipMemcpyAsync(hnp.data(), w.npix, sizeof(int) * w.C, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipStreamSynchronize(s));
+        if (reports) {
+            for (int i = 0; i < n; ++i) {
+                const FitState& f = hst[i];
+                tdi_fit_report& r = reports[i0 + i];
+                std::memset(&r, 0, sizeof(r));
+                r.stop = f.stop | (f.active ? TDI_STOP_MAXITER : 0);
+                r.iters = f.iters;
+                r.ls_evals = f.ls_evals;
+                r.n_pixels = hnp[i];
+                const bool term = (f.stop & (TDI_STOP_STEP | TDI_STOP_DECREMENT)) != 0;
+                r.success = (f.iters + (term && f.iters < o.max_iters ? 1 : 0)) > o.niters_success;
+                r.status = f.status;
+                r.objective = (float)hobj[i];
+            }
+        }
+    }
+    TDI_HIP(hipEventRecord(b->ev_out, s));
+    TDI_HIP(hipStreamWaitEvent(caller, b->ev_out, 0));
+    return TDI_OK;
+}

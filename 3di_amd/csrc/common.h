@@ -1,0 +1,239 @@
+// common.h — internal types shared by the gfx950 kernels and the C-ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/tdi_api.h"
+
+namespace tdi {
+
+constexpr int NLMK = TDI_NLANDMARKS;
+constexpr uint64_t KEY_EMPTY = ~0ull;
+
+// Offsets of the unknown blocks, OptimizationVariables order (Optimizer.h:71-91).
+struct Layout {
+    int K, off_tex, off_id, off_exp, off_pose, off_light;
+    int ntex, nid, nexp, nlight;
+};
+
+inline Layout make_layout(const tdi_model_desc* m, const tdi_fit_options* o)
+{
+    Layout L{};
+    L.ntex = o->fit_tex ? m->k_tex : 0;
+    L.nid = o->fit_id ? m->k_id : 0;
+    L.nexp = o->fit_exp ? m->k_exp : 0;
+    L.nlight = o->fit_light ? 4 : 0;
+    L.off_tex = 0;
+    L.off_id = L.ntex;
+    L.off_exp = L.off_id + L.nid;
+    L.off_pose = L.off_exp + L.nexp;
+    L.off_light = L.off_pose + 6;
+    L.K = L.off_light + L.nlight;
+    return L;
+}
+
+// Model resident in HBM (Renderer's device arrays, renderer.cu:28-72).
+struct ModelGPU {
+    int V, NT, k_id, k_tex, k_exp;
+    const float* mean;       // [V][3]
+    const float* id_basis;   // [3V][k_id]
+    const float* exp_basis;  // [3V][k_exp]
+    const float* tex_mean;   // [V]
+    const float* tex_basis;  // [V][k_tex]
+    const int* tris;         // [NT][3]
+    const int* lmk_ids;      // [51]
+    const float* lb;         // [K] lower bounds of the unknowns (-inf = unbounded)
+    const float* ub;         // [K]
+};
+
+// One frame of a batch, device-resident.
+struct FrameGPU {
+    float phix, phiy, cx, cy;
+    int x0, y0, W, H;
+    float face_size;
+    float lam[3];
+    float Lint;
+    int pad0;
+    long long pix_off;       // offset of this frame in the workspace's crop-sized buffers
+    long long obs_off;       // offset of this frame in the batch's observation buffers
+};
+
+// Per-frame fit state (device) for the batched path.
+struct FitState {
+    int active;              // 1 while the GN loop runs for this frame
+    int searching;           // 1 while the line search of this iteration runs
+    int stop;                // tdi_stop flags
+    int status;              // tdi_status
+    int iters, outer, ls_evals, inner;
+    float t;                 // current step
+    float dg;                // delta . g
+    double obj0;             // F(theta) of the current iteration
+};
+
+struct Options {
+    tdi_fit_options o;
+    Layout L;
+    float taps[15];
+};
+
+// ------------------------------------------------------------------ device helpers
+__device__ __forceinline__ uint32_t orderable_bits(float z)
+{
+    uint32_t u = __float_as_uint(z);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// FP32 coverage test, reference operation order (funcs.cu:950-979), no contraction:
+// every product and sum is rounded separately so it matches a scalar CPU build.  The
+// operators are written out here, under the pragma: HIP's __fmul_rn/__fadd_rn carry the
+// header's contraction setting and clang would fuse them into v_fma_f32.
+__device__ __forceinline__ bool cover_rn(float bx, float by, float cx, float cy, float ax, float ay,
+                                         int x, int y, float* al, float* be)
+{
+#pragma clang fp contract(off)
+    const float v0x = bx - ax, v1x = cx - ax, v2x = (float)x - ax;
+    const float v0y = by - ay, v1y = cy - ay, v2y = (float)y - ay;
+    const float d00 = v0x * v0x + v0y * v0y;
+    const float d01 = v0x * v1x + v0y * v1y;
+    const float d11 = v1x * v1x + v1y * v1y;
+    const float d20 = v0x * v2x + v0y * v2y;
+    const float d21 = v1x * v2x + v1y * v2y;
+    const float den = d00 * d11 - d01 * d01;
+    const float a = (d11 * d20 - d01 * d21) / den;
+    const float b = (d00 * d21 - d01 * d20) / den;
+    *al = a;
+    *be = b;
+    return a > 0.f && b > 0.f && (a + b <= 1.f);
+}
+
+// 1 - alpha - beta rounded as the reference does ((1 - alpha) - beta in FP32).
+__device__ __forceinline__ float gamma_rn(float al, float be)
+{
+#pragma clang fp contract(off)
+    return (1.f - al) - be;
+}
+
+// Rodrigues rotation and its derivatives in FP32 (fill_theta_etc, rotation_header.cu:46-149).
+// Column-major R[col*3+row].
+__device__ inline void rodrigues_f32(const float* u, float* R, float* dR)
+{
+    float th = sqrtf(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    float n0 = u[0] / th, n1 = u[1] / th, n2 = u[2] / th;
+    float N[9] = {0, n2, -n1, -n2, 0, n0, n1, -n0, 0};
+    float s = sinf(th), c1 = 1.f - cosf(th);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            float acc = 0.f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc += N[k * 3 + r] * N[c * 3 + k];
+            R[c * 3 + r] = c1 * acc + s * N[c * 3 + r] + (r == c ? 1.f : 0.f);
+        }
+    if (!dR) return;
+    float U[9] = {0, u[2], -u[1], -u[2], 0, u[0], u[1], -u[0], 0};
+    float inv = 1.f / (th * th);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float e0 = -R[i * 3 + 0] + (i == 0), e1 = -R[i * 3 + 1] + (i == 1), e2 = -R[i * 3 + 2] + (i == 2);
+        float w0 = u[1] * e2 - u[2] * e1, w1 = u[2] * e0 - u[0] * e2, w2 = u[0] * e1 - u[1] * e0;
+        float A[9] = {u[i] * U[0], u[i] * U[1] + w2, u[i] * U[2] - w1,
+                      u[i] * U[3] - w2, u[i] * U[4], u[i] * U[5] + w0,
+                      u[i] * U[6] + w1, u[i] * U[7] - w0, u[i] * U[8]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                float acc = 0.f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc += A[k * 3 + r] * R[c * 3 + k];
+                dR[9 * i + c * 3 + r] = acc * inv;
+            }
+    }
+}
+
+// ------------------------------------------------------------------ host helpers
+void set_error(const char* fmt, ...);
+
+#define TDI_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::tdi::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            return TDI_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+// ------------------------------------------------------------------ kernel launchers
+// All launchers take device pointers and a stream; n = frames in the call; mask selects
+// which frames take part (NULL = all).
+
+// Landmarks per frame, [n][6][51]: x, y (image pixels), box lx, ux, ly, uy (face-size units).
+struct LmkArrays {
+    const float* base;
+};
+
+// geometry.hip
+int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const float* theta,
+                      const int* mask, float* P0, float* P, float4* vproj, hipStream_t s);
+int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const float* theta,
+                           const int* mask, float* P0, float* P, float4* vproj, float* scratch, hipStream_t s);
+
+// raster.hip
+int launch_clear_keys(uint64_t* keys, long long total, hipStream_t s);
+int launch_clear_keys_masked(const FrameGPU* frames, int n, const int* mask, uint64_t* keys, int max_area,
+                             hipStream_t s);
+int launch_raster(const ModelGPU& m, const FrameGPU* frames, int n, const int* mask, const float4* vproj, int ntmp,
+                  uint64_t* keys, hipStream_t s);
+int launch_resolve(const ModelGPU& m, const FrameGPU* frames, int n, const int* mask, const float4* vproj,
+                   const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int max_area,
+                   hipStream_t s);
+
+// shade.hip
+int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask, const float* theta,
+                 const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
+                 const float* bary, float* img, int max_area, hipStream_t s);
+int launch_blur(const FrameGPU* frames, int n, const int* mask, const float* taps, const float* in, float* tmp,
+                float* out, int max_area, hipStream_t s);
+int launch_grads(const FrameGPU* frames, int n, const int* mask, const int* vis_list, const int* npix,
+                 const float* blurred, const float2* obs_g, float w, float4* pixg, double* obj, int max_area,
+                 hipStream_t s);
+int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const float* obs, float* tmp, float* blurred,
+                       float2* obs_g, int max_area, hipStream_t s);
+
+// jacobian.hip
+int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask, const float* theta,
+              const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
+              const float* bary, const float4* pixg, float* dI, int max_area, hipStream_t s);
+int jtj_splits(int n, int max_area);
+size_t jtj_partials_floats(int n, int S, int K, bool mfma);
+int launch_jtj_f32(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
+                   const int* npix, const int* cum, const float4* pixg, const float* dI, float w, float* H, float* g,
+                   float* partials, int S, hipStream_t s);
+int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
+                    const int* npix, const int* cum, const float4* pixg, const float* dI, float w, float* H, float* g,
+                    float* partials, int S, hipStream_t s);
+
+// solve.hip
+int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask,
+                   const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
+                   float* H, float* g, hipStream_t s);
+int launch_cholesky(int n, int K, const float* H, const float* g, float* delta, int* status, int mode, const int* mask,
+                    hipStream_t s);
+int launch_ls_init(int n, int K, FitState* st, const int* chol_status, const float* delta, const float* g,
+                   const double* obj, int* searching, hipStream_t s);
+int launch_ls_propose(int n, int K, int off_tz, float t_min, FitState* st, const float* theta, const float* delta,
+                      float* theta_try, int* searching, int* evalmask, hipStream_t s);
+int launch_ls_decide(int n, int K, FitState* st, const int* evalmask, int* searching, const double* obj,
+                     const double* objb, const int* face_ok, float* theta, const float* theta_try,
+                     const tdi_fit_options& o, hipStream_t s);
+int launch_face_check(int n, int V, const int* lmk_ids, const float4* vproj, float ref_face_size, const int* mask,
+                      int* face_ok, hipStream_t s);
+int launch_copy_mask(int n, const FitState* st, int* mask, hipStream_t s);
+
+} // namespace tdi

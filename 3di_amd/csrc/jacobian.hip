@@ -1,0 +1,534 @@
+// jacobian.hip — per-pixel Jacobian and the normal equations on gfx950.
+//
+// Reference:
+//   fill_optimization_auxiliary_variables_phase1 (newfuncs.cu:173-255),
+//   fill_optimization_dI_depsilon_userotated (265-299), ..._dI_dalpha (314-373),
+//   ..._dI_dbeta (383-403): dI/dtheta_j = -(gx dWx/dtheta_j + gy dWy/dtheta_j) for the
+//   geometric unknowns (W = pinhole projection of v = a P_b + b P_c + g P_a), and
+//   T_j (1 + L Id) for albedo; the light columns reuse dI_dlambda = L rho R n and
+//   dI_dLintensity = rho Id (newfuncs.cu:151-155);
+//   fill_optimization_auxiliary_variables_phase2_new (newfuncs.cu:417-463):
+//   J_x = c dy s, J_y = -c dx s, s = dy (dI[r]-dI[l]) - dx (dI[b]-dI[a]), c = 1/(0.001+h^3),
+//   with the compacted-neighbour positions of fill_krels/fill_krels2 (funcs.cu:288-340);
+//   cublasSgemm (derivative_computer.cu:159-164): H += w (Jx^T Jx + Jy^T Jy),
+//   g -= w (Jx^T g^s_x + Jy^T g^s_y).
+//
+// k_dI          : one wavefront per pixel, lanes over the K columns; the pixel's triangle,
+//                 barycentrics and camera-space point are wave-uniform, the basis rows of the
+//                 three vertices are read with unit stride across lanes.
+// k_jtj_f32     : FP32 J^T J / J^T r from LDS-staged J rows, 4x4 register tiles (plain path).
+// k_jtj_mfma    : J staged in LDS as BF16 hi/lo, [J | r]^T [J | r] on
+//                 v_mfma_f32_16x16x16bf16_1k with FP32 accumulation: hi*hi + hi*lo + lo*hi.
+//                 The residual r = g^s rides along as an extra column, so J^T r comes out of
+//                 the same MFMA tiles.
+// k_jtj_reduce  : sums the per-split partial tiles, scales by w, writes a dense symmetric H.
+#include "common.h"
+
+namespace tdi {
+
+namespace {
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+struct FrameRot {
+    float R[9], dR[27], tau[3], lam[3], L;
+};
+
+__device__ inline void load_frame_rot(const Layout& L, const FrameGPU& fr, const float* tf, FrameRot& s)
+{
+    rodrigues_f32(tf + L.off_pose + 3, s.R, s.dR);
+    for (int a = 0; a < 3; ++a) s.tau[a] = tf[L.off_pose + a];
+    if (L.nlight) {
+        for (int a = 0; a < 3; ++a) s.lam[a] = tf[L.off_light + a];
+        s.L = tf[L.off_light + 3];
+    } else {
+        for (int a = 0; a < 3; ++a) s.lam[a] = fr.lam[a];
+        s.L = fr.Lint;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+                                             const int* __restrict__ mask, const float* __restrict__ theta,
+                                             const float* __restrict__ P0, const float* __restrict__ P,
+                                             const int* __restrict__ vis, const int* __restrict__ npix,
+                                             const unsigned long long* __restrict__ keys,
+                                             const float2* __restrict__ bary, const float4* __restrict__ pixg,
+                                             float* __restrict__ dI)
+{
+    const int f = blockIdx.y;
+    if (mask && !mask[f]) return;
+    __shared__ FrameRot sr;
+    __shared__ float stex[512];
+    const FrameGPU fr = frames[f];
+    const float* tf = theta + (size_t)f * L.K;
+    if (threadIdx.x == 0) load_frame_rot(L, fr, tf, sr);
+    for (int q = threadIdx.x; q < L.ntex; q += blockDim.x) stex[q] = tf[L.off_tex + q];
+    __syncthreads();
+    const int N = npix[f];
+    const long long base = fr.pix_off;
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const float* P0f = P0 + (size_t)f * m.V * 3;
+    const float* Pf = P + (size_t)f * m.V * 3;
+    const int K = L.K;
+    for (int pos = blockIdx.x * 4 + wave; pos < N; pos += gridDim.x * 4) {
+        const int k = vis[base + pos];
+        const int t = (int)(unsigned)(keys[base + k] & 0xffffffffull);
+        const float2 ab = bary[base + k];
+        const float al = ab.x, be = ab.y, ga = gamma_rn(al, be);
+        const int vb = m.tris[3 * t], vc = m.tris[3 * t + 1], va = m.tris[3 * t + 2];
+        float v[3], p0[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            v[a] = al * Pf[3 * vb + a] + be * Pf[3 * vc + a] + ga * Pf[3 * va + a];
+            p0[a] = al * P0f[3 * vb + a] + be * P0f[3 * vc + a] + ga * P0f[3 * va + a];
+        }
+        const float4 gg = pixg[base + pos];
+        const float iz = 1.f / v[2];
+        // dI = c . dv with c = (-gx phix / vz, -gy phiy / vz, (gx phix vx + gy phiy vy) / vz^2)
+        const float c0 = -gg.x * fr.phix * iz;
+        const float c1 = -gg.y * fr.phiy * iz;
+        const float c2 = (gg.x * fr.phix * v[0] + gg.y * fr.phiy * v[1]) * iz * iz;
+        // q = R^T c, so that dI for a model-space direction e is q . e
+        const float q0 = sr.R[0] * c0 + sr.R[1] * c1 + sr.R[2] * c2;
+        const float q1 = sr.R[3] * c0 + sr.R[4] * c1 + sr.R[5] * c2;
+        const float q2 = sr.R[6] * c0 + sr.R[7] * c1 + sr.R[8] * c2;
+        // shading quantities for albedo and light columns
+        float rho = 0.f, Id = 0.f, rn[3] = {0.f, 0.f, 0.f};
+        if (L.ntex || L.nlight) {
+            const float* B0 = P0f + 3 * vb;
+            const float* C0 = P0f + 3 * vc;
+            const float* A0 = P0f + 3 * va;
+            const float e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
+            const float e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
+            float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+            const float inv = 1.f / sqrtf(nx * nx + ny * ny + nz * nz);
+            nx *= inv; ny *= inv; nz *= inv;
+            const float d0 = sr.lam[0] - sr.tau[0], d1 = sr.lam[1] - sr.tau[1], d2 = sr.lam[2] - sr.tau[2];
+            const float lx = sr.R[0] * d0 + sr.R[1] * d1 + sr.R[2] * d2;
+            const float ly = sr.R[3] * d0 + sr.R[4] * d1 + sr.R[5] * d2;
+            const float lz = sr.R[6] * d0 + sr.R[7] * d1 + sr.R[8] * d2;
+            Id = nx * lx + ny * ly + nz * lz - (nx * p0[0] + ny * p0[1] + nz * p0[2]);
+            float rb = m.tex_mean[vb], rc = m.tex_mean[vc], ra = m.tex_mean[va];
+            for (int q = 0; q < L.ntex; ++q) {
+                rb = fmaf(m.tex_basis[(size_t)vb * m.k_tex + q], stex[q], rb);
+                rc = fmaf(m.tex_basis[(size_t)vc * m.k_tex + q], stex[q], rc);
+                ra = fmaf(m.tex_basis[(size_t)va * m.k_tex + q], stex[q], ra);
+            }
+            rho = al * rb + be * rc + ga * ra;
+            rn[0] = sr.R[0] * nx + sr.R[3] * ny + sr.R[6] * nz;
+            rn[1] = sr.R[1] * nx + sr.R[4] * ny + sr.R[7] * nz;
+            rn[2] = sr.R[2] * nx + sr.R[5] * ny + sr.R[8] * nz;
+        }
+        float* out = dI + (size_t)(base + pos) * K;
+        for (int j = lane; j < K; j += 64) {
+            float val;
+            if (j >= L.off_exp && j < L.off_pose) {
+                const int c = j - L.off_exp;
+                const float* E = m.exp_basis;
+                const int ke = m.k_exp;
+                const float ex = al * E[(size_t)(3 * vb) * ke + c] + be * E[(size_t)(3 * vc) * ke + c] + ga * E[(size_t)(3 * va) * ke + c];
+                const float ey = al * E[(size_t)(3 * vb + 1) * ke + c] + be * E[(size_t)(3 * vc + 1) * ke + c] + ga * E[(size_t)(3 * va + 1) * ke + c];
+                const float ez = al * E[(size_t)(3 * vb + 2) * ke + c] + be * E[(size_t)(3 * vc + 2) * ke + c] + ga * E[(size_t)(3 * va + 2) * ke + c];
+                val = q0 * ex + q1 * ey + q2 * ez;
+            } else if (j >= L.off_pose && j < L.off_pose + 3) {
+                val = (j == L.off_pose) ? c0 : (j == L.off_pose + 1) ? c1 : c2;
+            } else if (j >= L.off_pose + 3 && j < L.off_light) {
+                const float* d = sr.dR + 9 * (j - L.off_pose - 3);
+                const float dvx = d[0] * p0[0] + d[3] * p0[1] + d[6] * p0[2];
+                const float dvy = d[1] * p0[0] + d[4] * p0[1] + d[7] * p0[2];
+                const float dvz = d[2] * p0[0] + d[5] * p0[1] + d[8] * p0[2];
+                val = c0 * dvx + c1 * dvy + c2 * dvz;
+            } else if (j >= L.off_light) {
+                const int c = j - L.off_light;
+                val = (c < 3) ? sr.L * rho * rn[c] : rho * Id;
+            } else if (j >= L.off_id) {
+                const int c = j - L.off_id;
+                const float* E = m.id_basis;
+                const int ki = m.k_id;
+                const float ex = al * E[(size_t)(3 * vb) * ki + c] + be * E[(size_t)(3 * vc) * ki + c] + ga * E[(size_t)(3 * va) * ki + c];
+                const float ey = al * E[(size_t)(3 * vb + 1) * ki + c] + be * E[(size_t)(3 * vc + 1) * ki + c] + ga * E[(size_t)(3 * va + 1) * ki + c];
+                const float ez = al * E[(size_t)(3 * vb + 2) * ki + c] + be * E[(size_t)(3 * vc + 2) * ki + c] + ga * E[(size_t)(3 * va + 2) * ki + c];
+                val = q0 * ex + q1 * ey + q2 * ez;
+            } else {
+                const int c = j - L.off_tex;
+                const float tt = al * m.tex_basis[(size_t)vb * m.k_tex + c] + be * m.tex_basis[(size_t)vc * m.k_tex + c]
+                               + ga * m.tex_basis[(size_t)va * m.k_tex + c];
+                val = tt + sr.L * Id * tt;
+            }
+            out[j] = val;
+        }
+    }
+}
+
+// Compacted neighbour positions of sorted position `pos` (fill_krels, funcs.cu:288-320,
+// with cumM0 = inclusive occupancy count): left/right = last rendered pixel at or before
+// the horizontal neighbour, above/below = previous/next rendered pixel; the first and last
+// positions keep 0.  Returns (kl, kr, ka, kb).
+__device__ inline int4 pixel_nbrs(const int* __restrict__ cum, int k, long long base, int pos, int N, int W, int H)
+{
+    if (pos == 0 || pos == N - 1) return make_int4(0, 0, 0, 0);
+    const int x = k / H;
+    const int kleft = (x > 0) ? k - H : 0;
+    const int kright = (x < W - 1) ? k + H : 0;
+    const int cl = cum[base + kleft];
+    int kl = cl > 1 ? cl - 1 : 0;
+    if (kl >= N - 1) kl = N - 1;
+    int kr = cum[base + kright] - 1;
+    if (kr < 0) kr = 0;
+    return make_int4(kl, kr, pos - 1, pos + 1);
+}
+
+// Per-pixel coefficients for the normalised-gradient Jacobian.
+struct PixCoef {
+    int4 nb;
+    float c, dx, dy, gsx, gsy;
+    int valid;
+};
+
+__device__ inline PixCoef pix_coef(const FrameGPU& fr, const int* cum, const int* vis, const float4* pixg, int pos,
+                                   int N)
+{
+    PixCoef pc;
+    pc.valid = pos < N;
+    if (!pc.valid) {
+        pc.nb = make_int4(0, 0, 0, 0);
+        pc.c = pc.dx = pc.dy = pc.gsx = pc.gsy = 0.f;
+        return pc;
+    }
+    const long long base = fr.pix_off;
+    const int k = vis[base + pos];
+    pc.nb = pixel_nbrs(cum, k, base, pos, N, fr.W, fr.H);
+    const float4 gg = pixg[base + pos];
+    const float h = 1e-16f + sqrtf(4.f * gg.x * gg.x + 4.f * gg.y * gg.y);
+    pc.c = 1.f / (0.001f + h * h * h);
+    pc.dx = 2.f * gg.x;
+    pc.dy = 2.f * gg.y;
+    pc.gsx = gg.z;
+    pc.gsy = gg.w;
+    return pc;
+}
+
+__device__ __forceinline__ void jrow(const PixCoef& pc, const float* __restrict__ dIf, int K, int j, float& jx,
+                                     float& jy)
+{
+    const float dh = dIf[(size_t)pc.nb.y * K + j] - dIf[(size_t)pc.nb.x * K + j];
+    const float dv = dIf[(size_t)pc.nb.w * K + j] - dIf[(size_t)pc.nb.z * K + j];
+    const float s = pc.dy * dh - pc.dx * dv;
+    jx = pc.c * pc.dy * s;
+    jy = -pc.c * pc.dx * s;
+}
+
+// ------------------------------------------------------------------ FP32 J^T J
+constexpr int F32_CH = 32;     // pixels per LDS chunk -> 64 rows
+constexpr int MAXKP = 128;
+
+__device__ __forceinline__ int tri_index(int I, int J, int T) { return I * T - (I * (I - 1)) / 2 + (J - I); }
+
+__global__ void __launch_bounds__(256) k_jtj_f32(Layout L, const FrameGPU* __restrict__ frames,
+                                                  const int* __restrict__ mask, const int* __restrict__ vis,
+                                                  const int* __restrict__ npix, const int* __restrict__ cum,
+                                                  const float4* __restrict__ pixg, const float* __restrict__ dI,
+                                                  int S, int KP, float* __restrict__ Hp, float* __restrict__ gp)
+{
+    const int f = blockIdx.y, s = blockIdx.x;
+    if (mask && !mask[f]) return;
+    const FrameGPU fr = frames[f];
+    const int N = npix[f];
+    const int K = L.K;
+    const int T = KP / 4;
+    const int ntiles = T * (T + 1) / 2;
+    const int p0 = (int)((long long)N * s / S), p1 = (int)((long long)N * (s + 1) / S);
+    __shared__ float sJ[2 * F32_CH][MAXKP + 1];
+    __shared__ float sgs[2 * F32_CH];
+    __shared__ PixCoef spc[F32_CH];
+    const float* dIf = dI + (size_t)fr.pix_off * K;
+    const int tid = threadIdx.x;
+    float acc[2][16];
+    int tI[2], tJ[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int tile = tid + u * 256;
+        int I = 0, rem = tile;
+        while (I < T && rem >= T - I) { rem -= T - I; ++I; }
+        tI[u] = I;
+        tJ[u] = I + rem;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[u][e] = 0.f;
+    }
+    float gacc = 0.f;
+    for (int c0 = p0; c0 < p1; c0 += F32_CH) {
+        if (tid < F32_CH) {
+            const int pos = c0 + tid;
+            spc[tid] = pix_coef(fr, cum, vis, pixg, pos < p1 ? pos : N, N);
+            sgs[2 * tid] = spc[tid].gsx;
+            sgs[2 * tid + 1] = spc[tid].gsy;
+        }
+        __syncthreads();
+        for (int e = tid; e < F32_CH * KP; e += 256) {
+            const int p = e / KP, j = e - p * KP;
+            float jx = 0.f, jy = 0.f;
+            if (spc[p].valid && j < K) jrow(spc[p], dIf, K, j, jx, jy);
+            sJ[2 * p][j] = jx;
+            sJ[2 * p + 1][j] = jy;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (tid + u * 256 >= ntiles) continue;
+            const int a0 = 4 * tI[u], b0 = 4 * tJ[u];
+            for (int r = 0; r < 2 * F32_CH; ++r) {
+                float xa[4], xb[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    xa[q] = sJ[r][a0 + q];
+                    xb[q] = sJ[r][b0 + q];
+                }
+#pragma unroll
+                for (int qa = 0; qa < 4; ++qa)
+#pragma unroll
+                    for (int qb = 0; qb < 4; ++qb) acc[u][qa * 4 + qb] = fmaf(xa[qa], xb[qb], acc[u][qa * 4 + qb]);
+            }
+        }
+        if (tid < KP)
+            for (int r = 0; r < 2 * F32_CH; ++r) gacc = fmaf(sJ[r][tid], sgs[r], gacc);
+        __syncthreads();
+    }
+    float* Hout = Hp + ((size_t)f * S + s) * ntiles * 16;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int tile = tid + u * 256;
+        if (tile >= ntiles) continue;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) Hout[(size_t)tile * 16 + e] = acc[u][e];
+    }
+    if (tid < KP) gp[((size_t)f * S + s) * KP + tid] = gacc;
+}
+
+// Sum the S partials; tiles are 4x4 (f32 kernel) or 16x16 (MFMA kernel), upper triangle.
+__global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ mask, int K, int KP, int TS, int S,
+                                                     const float* __restrict__ Hp, const float* __restrict__ gp,
+                                                     float w, float* __restrict__ H, float* __restrict__ g,
+                                                     int g_from_tiles)
+{
+    const int f = blockIdx.y;
+    if (mask && !mask[f]) return;
+    const int T = KP / TS;
+    const int ntiles = T * (T + 1) / 2;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K * K + K; e += gridDim.x * blockDim.x) {
+        if (e < K * K) {
+            int a = e / K, b = e - (e / K) * K;
+            if (a / TS > b / TS) { const int t = a; a = b; b = t; }
+            const int tile = tri_index(a / TS, b / TS, T);
+            const int off = (a % TS) * TS + (b % TS);
+            float sum = 0.f;
+            for (int s = 0; s < S; ++s) sum += Hp[(((size_t)f * S + s) * ntiles + tile) * TS * TS + off];
+            H[(size_t)f * K * K + e] = w * sum;
+        } else {
+            const int j = e - K * K;
+            float sum = 0.f;
+            if (g_from_tiles) {
+                // J^T r lives in column K of the augmented product: entry (j, K)
+                const int tile = tri_index(j / TS, K / TS, T);
+                const int off = (j % TS) * TS + (K % TS);
+                for (int s = 0; s < S; ++s) sum += Hp[(((size_t)f * S + s) * ntiles + tile) * TS * TS + off];
+            } else {
+                for (int s = 0; s < S; ++s) sum += gp[((size_t)f * S + s) * KP + j];
+            }
+            g[(size_t)f * K + j] = -w * sum;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ BF16 hi/lo MFMA J^T J
+constexpr int MF_CH = 64;            // pixels per chunk -> 128 rows
+constexpr int MF_ROWS = 2 * MF_CH;
+constexpr int MF_STRIDE = MF_ROWS + 8; // bf16 elements per column in LDS (padding vs bank conflicts)
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ void split_bf16(float x, unsigned short& hi, unsigned short& lo)
+{
+    hi = bf16_rne(x);
+    const float r = x - __uint_as_float((unsigned)hi << 16);
+    lo = bf16_rne(r);
+}
+
+__global__ void __launch_bounds__(256) k_jtj_mfma(Layout L, const FrameGPU* __restrict__ frames,
+                                                   const int* __restrict__ mask, const int* __restrict__ vis,
+                                                   const int* __restrict__ npix, const int* __restrict__ cum,
+                                                   const float4* __restrict__ pixg, const float* __restrict__ dI,
+                                                   int S, int KP, float* __restrict__ Hp)
+{
+    const int f = blockIdx.y, s = blockIdx.x;
+    if (mask && !mask[f]) return;
+    const FrameGPU fr = frames[f];
+    const int N = npix[f];
+    const int K = L.K;
+    const int T = KP / 16;
+    const int ntiles = T * (T + 1) / 2;
+    const int p0 = (int)((long long)N * s / S), p1 = (int)((long long)N * (s + 1) / S);
+    extern __shared__ unsigned short smem[];
+    unsigned short* sJh = smem;                         // [KP][MF_STRIDE]
+    unsigned short* sJl = smem + KP * MF_STRIDE;        // [KP][MF_STRIDE]
+    __shared__ PixCoef spc[MF_CH];
+    const float* dIf = dI + (size_t)fr.pix_off * K;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // wave w owns tiles w, w+4, ... (at most 9 for KP <= 128)
+    constexpr int MAXT = 7;            // KP <= 112 -> at most 28 tiles, 7 per wave
+    v4f acc[MAXT];
+    int tI[MAXT], tJ[MAXT];
+#pragma unroll
+    for (int u = 0; u < MAXT; ++u) {
+        const int tile = wave + 4 * u;
+        int I = 0, rem = tile;
+        while (I < T && rem >= T - I) { rem -= T - I; ++I; }
+        tI[u] = I;
+        tJ[u] = I + rem;
+        acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
+    }
+    for (int c0 = p0; c0 < p1; c0 += MF_CH) {
+        if (tid < MF_CH) {
+            const int pos = c0 + tid;
+            spc[tid] = pix_coef(fr, cum, vis, pixg, pos < p1 ? pos : N, N);
+        }
+        __syncthreads();
+        for (int e = tid; e < MF_CH * KP; e += 256) {
+            const int j = e / MF_CH, p = e - j * MF_CH;   // p fastest: contiguous LDS stores per column
+            float jx = 0.f, jy = 0.f;
+            if (spc[p].valid) {
+                if (j < K) jrow(spc[p], dIf, K, j, jx, jy);
+                else if (j == K) { jx = spc[p].gsx; jy = spc[p].gsy; }
+            }
+            unsigned short hx, lx, hy, ly;
+            split_bf16(jx, hx, lx);
+            split_bf16(jy, hy, ly);
+            sJh[j * MF_STRIDE + 2 * p] = hx;
+            sJh[j * MF_STRIDE + 2 * p + 1] = hy;
+            sJl[j * MF_STRIDE + 2 * p] = lx;
+            sJl[j * MF_STRIDE + 2 * p + 1] = ly;
+        }
+        __syncthreads();
+        const int col = lane & 15, kq = 4 * (lane >> 4);
+#pragma unroll
+        for (int ks = 0; ks < MF_ROWS; ks += 16) {
+#pragma unroll
+            for (int u = 0; u < MAXT; ++u) {
+                if (wave + 4 * u >= ntiles) continue;
+                const int ca = tI[u] * 16 + col, cb = tJ[u] * 16 + col;
+                const v4s ah = *(const v4s*)(sJh + ca * MF_STRIDE + ks + kq);
+                const v4s al = *(const v4s*)(sJl + ca * MF_STRIDE + ks + kq);
+                const v4s bh = *(const v4s*)(sJh + cb * MF_STRIDE + ks + kq);
+                const v4s bl = *(const v4s*)(sJl + cb * MF_STRIDE + ks + kq);
+                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[u], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // accumulator layout of 16x16x16: lane l, element r -> C[4*(l/16) + r][l%16]
+    float* Hout = Hp + ((size_t)f * S + s) * ntiles * 256;
+#pragma unroll
+    for (int u = 0; u < MAXT; ++u) {
+        const int tile = wave + 4 * u;
+        if (tile >= ntiles) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 4 * (lane >> 4) + r, cc = lane & 15;
+            Hout[(size_t)tile * 256 + row * 16 + cc] = acc[u][r];
+        }
+    }
+}
+
+} // namespace
+
+int jtj_splits(int n, int per_frame_pixels_hint)
+{
+    // aim for >= 2048 workgroups overall, at least ~2 chunks per workgroup
+    int S = (2048 + n - 1) / n;
+    const int maxS = per_frame_pixels_hint / 128 + 1;
+    if (S > maxS) S = maxS;
+    if (S < 1) S = 1;
+    if (S > 256) S = 256;
+    return S;
+}
+
+int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask, const float* theta,
+              const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
+              const float* bary, const float4* pixg, float* dI, int max_area, hipStream_t s)
+{
+    if (L.ntex > 512) {
+        set_error("dI: k_tex %d > 512", L.ntex);
+        return TDI_ERR_ARG;
+    }
+    int bx = (max_area / 2 + 3) / 4;
+    if (bx > 2048) bx = 2048;
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(k_dI, dim3(bx, n), dim3(256), 0, s, m, L, frames, mask, theta, P0, P, vis_list, npix,
+                       (const unsigned long long*)keys, (const float2*)bary, pixg, dI);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+size_t jtj_partials_floats(int n, int S, int K, bool mfma)
+{
+    if (mfma) {
+        const int KP = ((K + 1 + 15) / 16) * 16;
+        const int T = KP / 16;
+        return (size_t)n * S * (T * (T + 1) / 2) * 256;
+    }
+    const int KP = ((K + 3) / 4) * 4;
+    const int T = KP / 4;
+    return (size_t)n * S * (T * (T + 1) / 2) * 16 + (size_t)n * S * KP;
+}
+
+int launch_jtj_f32(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
+                   const int* npix, const int* cum, const float4* pixg, const float* dI, float w, float* H, float* g,
+                   float* partials, int S, hipStream_t s)
+{
+    const int K = L.K;
+    const int KP = ((K + 3) / 4) * 4;
+    if (KP > MAXKP) {
+        set_error("jtj_f32: K=%d > %d", K, MAXKP);
+        return TDI_ERR_ARG;
+    }
+    const int T = KP / 4;
+    float* Hp = partials;
+    float* gp = partials + (size_t)n * S * (T * (T + 1) / 2) * 16;
+    hipLaunchKernelGGL(k_jtj_f32, dim3(S, n), dim3(256), 0, s, L, frames, mask, vis_list, npix, cum, pixg, dI, S, KP, Hp,
+                       gp);
+    int bx = (K * K + K + 255) / 256;
+    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, n), dim3(256), 0, s, mask, K, KP, 4, S, Hp, gp, w, H, g, 0);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
+                    const int* npix, const int* cum, const float4* pixg, const float* dI, float w, float* H, float* g,
+                    float* partials, int S, hipStream_t s)
+{
+    const int K = L.K;
+    const int KP = ((K + 1 + 15) / 16) * 16;
+    if (KP > 112) {
+        set_error("jtj_mfma: K=%d too large (max 111)", K);
+        return TDI_ERR_ARG;
+    }
+    const size_t lds = (size_t)2 * KP * MF_STRIDE * sizeof(unsigned short);
+    hipLaunchKernelGGL(k_jtj_mfma, dim3(S, n), dim3(256), lds, s, L, frames, mask, vis_list, npix, cum, pixg, dI, S, KP,
+                       partials);
+    int bx = (K * K + K + 255) / 256;
+    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, n), dim3(256), 0, s, mask, K, KP, 16, S, partials, nullptr, w, H, g, 1);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+} // namespace tdi

@@ -1,0 +1,244 @@
+// shade.hip — shading, blur, normalised image gradients and the data objective on gfx950.
+//
+// Reference:
+//   fill_diffuse_component_and_populate_texture_and_shape (newfuncs.cu:18-167):
+//       n = normalise((P0_b - P0_a) x (P0_c - P0_a)), p0 = a P0_b + b P0_c + g P0_a,
+//       Id = n.(R^T (lambda - tau)) - n.p0, I = rho (1 + L Id), rho barycentric albedo;
+//   fill_tex_im1 (funcs.cu:800-817): scatter I into the crop (zero elsewhere);
+//   convolutionRowsGPU/ColumnsGPU (newfuncs.cu:539-715): separable 15-tap Gaussian,
+//       zero boundary;
+//   fill_grads (funcs.cu:842-897): g = (I[x+1]-I[x-1], I[y+1]-I[y-1]) / 2 with crop-border
+//       neighbours reading crop index 0, h = 1e-16 + sqrt(4 gx^2 + 4 gy^2), g^ = 2 g / h,
+//       the same for the blurred observation;
+//   objective SGEMM (renderer.cu:725-729): C = w sum g^_r . g^_s (the fit minimises -C).
+#include "common.h"
+
+namespace tdi {
+
+struct Taps {
+    float t[15];
+};
+
+namespace {
+
+__global__ void __launch_bounds__(256) k_clear_crops(const FrameGPU* __restrict__ frames, const int* __restrict__ mask,
+                                                      float* __restrict__ img)
+{
+    const int f = blockIdx.y;
+    if (mask && !mask[f]) return;
+    const FrameGPU fr = frames[f];
+    const int A = fr.W * fr.H;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) img[fr.pix_off + k] = 0.f;
+}
+
+__global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+                                                const int* __restrict__ mask, const float* __restrict__ theta,
+                                                const float* __restrict__ P0, const float* __restrict__ P,
+                                                const int* __restrict__ vis, const int* __restrict__ npix,
+                                                const unsigned long long* __restrict__ keys,
+                                                const float2* __restrict__ bary, float* __restrict__ img)
+{
+    const int f = blockIdx.y;
+    if (mask && !mask[f]) return;
+    __shared__ float sR[9], sdl[3], sL, stex[512];
+    const float* tf = theta + (size_t)f * L.K;
+    const FrameGPU fr = frames[f];
+    if (threadIdx.x == 0) {
+        rodrigues_f32(tf + L.off_pose + 3, sR, nullptr);
+        float lam[3] = {fr.lam[0], fr.lam[1], fr.lam[2]};
+        float Li = fr.Lint;
+        if (L.nlight) {
+            lam[0] = tf[L.off_light]; lam[1] = tf[L.off_light + 1]; lam[2] = tf[L.off_light + 2];
+            Li = tf[L.off_light + 3];
+        }
+        for (int a = 0; a < 3; ++a) sdl[a] = lam[a] - tf[L.off_pose + a];
+        sL = Li;
+    }
+    for (int k = threadIdx.x; k < L.ntex; k += blockDim.x) stex[k] = tf[L.off_tex + k];
+    __syncthreads();
+    const int N = npix[f];
+    const long long base = fr.pix_off;
+    const float* P0f = P0 + (size_t)f * m.V * 3;
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += gridDim.x * blockDim.x) {
+        const int k = vis[base + pos];
+        const int t = (int)(unsigned)(keys[base + k] & 0xffffffffull);
+        const float2 ab = bary[base + k];
+        const float al = ab.x, be = ab.y, ga = gamma_rn(al, be);
+        const int vb = m.tris[3 * t], vc = m.tris[3 * t + 1], va = m.tris[3 * t + 2];
+        const float* B0 = P0f + 3 * vb;
+        const float* C0 = P0f + 3 * vc;
+        const float* A0 = P0f + 3 * va;
+        const float e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
+        const float e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
+        float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+        const float inv = 1.f / sqrtf(nx * nx + ny * ny + nz * nz);
+        nx *= inv; ny *= inv; nz *= inv;
+        const float px = al * B0[0] + be * C0[0] + ga * A0[0];
+        const float py = al * B0[1] + be * C0[1] + ga * A0[1];
+        const float pz = al * B0[2] + be * C0[2] + ga * A0[2];
+        // R^T (lambda - tau)
+        const float lx = sR[0] * sdl[0] + sR[1] * sdl[1] + sR[2] * sdl[2];
+        const float ly = sR[3] * sdl[0] + sR[4] * sdl[1] + sR[5] * sdl[2];
+        const float lz = sR[6] * sdl[0] + sR[7] * sdl[1] + sR[8] * sdl[2];
+        const float Id = nx * lx + ny * ly + nz * lz - (nx * px + ny * py + nz * pz);
+        float rb = m.tex_mean[vb], rc = m.tex_mean[vc], ra = m.tex_mean[va];
+        for (int q = 0; q < L.ntex; ++q) {
+            rb = fmaf(m.tex_basis[(size_t)vb * m.k_tex + q], stex[q], rb);
+            rc = fmaf(m.tex_basis[(size_t)vc * m.k_tex + q], stex[q], rc);
+            ra = fmaf(m.tex_basis[(size_t)va * m.k_tex + q], stex[q], ra);
+        }
+        const float rho = al * rb + be * rc + ga * ra;
+        img[base + k] = rho + sL * rho * Id;
+    }
+}
+
+// 15-tap separable blur, zero outside the crop.  dir 0 = along y (contiguous), 1 = along x.
+__global__ void __launch_bounds__(256) k_blur(const FrameGPU* __restrict__ frames, const int* __restrict__ mask,
+                                               Taps taps, const float* __restrict__ in, float* __restrict__ out, int dir)
+{
+    const int f = blockIdx.y;
+    if (mask && !mask[f]) return;
+    const FrameGPU fr = frames[f];
+    const int W = fr.W, H = fr.H, A = W * H;
+    const float* src = in + fr.pix_off;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        const int x = k / H, y = k - x * H;
+        float s = 0.f;
+        if (dir == 0) {
+#pragma unroll
+            for (int j = -7; j <= 7; ++j) {
+                const int yy = y + j;
+                if (yy >= 0 && yy < H) s = fmaf(taps.t[7 - j], src[k + j], s);
+            }
+        } else {
+#pragma unroll
+            for (int j = -7; j <= 7; ++j) {
+                const int xx = x + j;
+                if (xx >= 0 && xx < W) s = fmaf(taps.t[7 - j], src[k + j * H], s);
+            }
+        }
+        out[fr.pix_off + k] = s;
+    }
+}
+
+__device__ __forceinline__ void nbr_index(int k, int W, int H, int& kr, int& kl, int& ka, int& kb)
+{
+    const int x = k / H, y = k - x * H;
+    kr = (x < W - 1) ? k + H : 0;
+    kl = (x > 0) ? k - H : 0;
+    ka = (y > 0) ? k - 1 : 0;
+    kb = (y < H - 1) ? k + 1 : 0;
+}
+
+// Gradients of the blurred rendered crop at every visible pixel, the objective
+// -w sum g^_r.g^_s accumulated in FP64, and the per-pixel record (gx, gy, g^s_x, g^s_y).
+__global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ frames, const int* __restrict__ mask,
+                                                const int* __restrict__ vis, const int* __restrict__ npix,
+                                                const float* __restrict__ blurred, const float2* __restrict__ obs_g,
+                                                float w, float4* __restrict__ pixg, double* __restrict__ obj)
+{
+    const int f = blockIdx.y;
+    if (mask && !mask[f]) return;
+    const FrameGPU fr = frames[f];
+    const int N = npix[f];
+    const long long base = fr.pix_off;
+    const float* I = blurred + base;
+    double acc = 0.0;
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += gridDim.x * blockDim.x) {
+        const int k = vis[base + pos];
+        int kr, kl, ka, kb;
+        nbr_index(k, fr.W, fr.H, kr, kl, ka, kb);
+        const float gx = 0.5f * (I[kr] - I[kl]);
+        const float gy = 0.5f * (I[kb] - I[ka]);
+        const float h = 1e-16f + sqrtf(4.f * gx * gx + 4.f * gy * gy);
+        const float2 gs = obs_g[fr.obs_off + k];
+        acc += (double)(2.f * gx / h) * gs.x + (double)(2.f * gy / h) * gs.y;
+        pixg[base + pos] = make_float4(gx, gy, gs.x, gs.y);
+    }
+    // block reduction (wave shuffles, then LDS)
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    __shared__ double sw[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sw[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double s = sw[0] + sw[1] + sw[2] + sw[3];
+        if (s != 0.0) atomicAdd(obj + f, -(double)w * s);
+    }
+}
+
+// Observation: blur once, then normalised gradients at every crop pixel.
+__global__ void __launch_bounds__(256) k_obs_grad(const FrameGPU* __restrict__ frames, const float* __restrict__ blurred,
+                                                   float2* __restrict__ obs_g)
+{
+    const int f = blockIdx.y;
+    const FrameGPU fr = frames[f];
+    const int A = fr.W * fr.H;
+    const float* S = blurred + fr.pix_off;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        int kr, kl, ka, kb;
+        nbr_index(k, fr.W, fr.H, kr, kl, ka, kb);
+        const float sx = 0.5f * (S[kr] - S[kl]);
+        const float sy = 0.5f * (S[kb] - S[ka]);
+        const float hs = 1e-16f + sqrtf(4.f * sx * sx + 4.f * sy * sy);
+        obs_g[fr.obs_off + k] = make_float2(2.f * sx / hs, 2.f * sy / hs);
+    }
+}
+
+inline dim3 crop_grid(int max_area, int n)
+{
+    int bx = (max_area + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    if (bx < 1) bx = 1;
+    return dim3(bx, n);
+}
+
+} // namespace
+
+int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask, const float* theta,
+                 const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
+                 const float* bary, float* img, int max_area, hipStream_t s)
+{
+    if (L.ntex > 512) {
+        set_error("shade: k_tex %d > 512", L.ntex);
+        return TDI_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_clear_crops, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, img);
+    hipLaunchKernelGGL(k_shade, crop_grid(max_area, n), dim3(256), 0, s, m, L, frames, mask, theta, P0, P, vis_list,
+                       npix, (const unsigned long long*)keys, (const float2*)bary, img);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_blur(const FrameGPU* frames, int n, const int* mask, const float* taps, const float* in, float* tmp,
+                float* out, int max_area, hipStream_t s)
+{
+    Taps t;
+    for (int i = 0; i < 15; ++i) t.t[i] = taps[i];
+    hipLaunchKernelGGL(k_blur, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, t, in, tmp, 0);
+    hipLaunchKernelGGL(k_blur, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, t, tmp, out, 1);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_grads(const FrameGPU* frames, int n, const int* mask, const int* vis_list, const int* npix,
+                 const float* blurred, const float2* obs_g, float w, float4* pixg, double* obj, int max_area,
+                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_grads, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, vis_list, npix, blurred, obs_g,
+                       w, pixg, obj);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const float* obs, float* tmp, float* blurred,
+                       float2* obs_g, int max_area, hipStream_t s)
+{
+    int rc = launch_blur(frames, n, nullptr, taps, obs, tmp, blurred, max_area, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_obs_grad, crop_grid(max_area, n), dim3(256), 0, s, frames, blurred, obs_g);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+} // namespace tdi

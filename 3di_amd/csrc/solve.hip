@@ -1,0 +1,497 @@
+// solve.hip — log barrier, batched Cholesky and the Gauss-Newton/line-search state machine.
+//
+// Reference:
+//   Logbarrier_Initializer::compute_gradient_and_hessian (logbarrier_initializer.cu:1272-1842)
+//   with compute_Gs (458-518), fill_optimization_dc_landmark/_dI_depsilon_landmark/
+//   _dI_dalpha_landmark (587-910), update_gradient/update_diagonal_of_hessian_wbounds
+//   (915-948) and evaluate_objective_function (1921-2081): landmark-box constraints
+//   G = box edge distance of the projected landmark, objective -sum log G, gradient
+//   sum -dG/G, Gauss-Newton Hessian sum (dG/G)(dG/G)^T; coefficient bounds
+//   -log(ub-x) - log(x-lb) with exact gradient and diagonal Hessian.
+//   Solver::solve (solver.h:70-105): cusolverDnDpotrf + Dpotrs in FP64, delta = -H^-1 g.
+//   fit_3DMM_shape_rigid (model_fitter.cu:121-375): Armijo backtracking with ALPHA 0.4,
+//   BETA 0.8, stop when -delta.g < 3e-4 or t < t_min, reject steps with tau_z < 0, shrink
+//   on "face too large" or a NaN barrier.
+//
+// The batched state machine keeps every per-frame decision of the reference but runs it
+// for all frames of a batch with masks, so the whole fit is a fixed launch sequence
+// (graph-capturable): frames that have accepted a step or stopped sit out later rounds.
+#include "common.h"
+
+namespace tdi {
+
+namespace {
+
+constexpr int BMAXK = 112;
+
+// mode 0: objective only -> obj[f] = objb[f] = barrier value (FP64)
+// mode 1: derivatives only -> H[f] += barrier Hessian, g[f] += barrier gradient
+__global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+                                                  const int* __restrict__ mask, const float* __restrict__ theta,
+                                                  LmkArrays lm, int use_lmk, int mode, double* __restrict__ obj,
+                                                  double* __restrict__ objb, float* __restrict__ H,
+                                                  float* __restrict__ g)
+{
+    const int f = blockIdx.x;
+    if (mask && !mask[f]) return;
+    const int K = L.K;
+    const FrameGPU fr = frames[f];
+    const float* tf = theta + (size_t)f * K;
+    __shared__ float sR[9], sdR[27];
+    __shared__ float sv[NLMK][3], sp[NLMK][3];
+    __shared__ float swx[NLMK], swy[NLMK], sgx[NLMK], sgy[NLMK];
+    __shared__ float sdx[NLMK][BMAXK + 1], sdy[NLMK][BMAXK + 1];
+    __shared__ double sred[4];
+    const int tid = threadIdx.x;
+    if (tid == 0) rodrigues_f32(tf + L.off_pose + 3, sR, sdR);
+    __syncthreads();
+    double acc = 0.0;
+    // coefficient bounds (only finite bounds take part)
+    for (int j = tid; j < K; j += blockDim.x) {
+        const float lb = m.lb[j], ub = m.ub[j];
+        if (!(ub < INFINITY) || !(lb > -INFINITY)) continue;
+        const float x = tf[j];
+        const float fub = x - ub, flb = lb - x;
+        if (mode == 0) {
+            acc += -log(-(double)fub) - log(-(double)flb);
+        } else {
+            g[(size_t)f * K + j] += -1.f / fub + 1.f / flb;
+            const float a = -1.f / (fub + 1e-12f), b = 1.f / (flb + 1e-12f);
+            H[(size_t)f * K * K + (size_t)j * K + j] += a * a + b * b;
+        }
+    }
+    if (use_lmk) {
+        if (tid < NLMK) {
+            const int i = tid, v = m.lmk_ids[i];
+            float p[3];
+            for (int a = 0; a < 3; ++a) {
+                const int r = 3 * v + a;
+                float s = m.mean[r];
+                for (int k = 0; k < L.nexp; ++k) s = fmaf(m.exp_basis[(size_t)r * m.k_exp + k], tf[L.off_exp + k], s);
+                for (int k = 0; k < L.nid; ++k) s = fmaf(m.id_basis[(size_t)r * m.k_id + k], tf[L.off_id + k], s);
+                p[a] = s;
+                sp[i][a] = s;
+            }
+            const float vx = sR[0] * p[0] + sR[3] * p[1] + sR[6] * p[2] + tf[L.off_pose + 0];
+            const float vy = sR[1] * p[0] + sR[4] * p[1] + sR[7] * p[2] + tf[L.off_pose + 1];
+            const float vz = sR[2] * p[0] + sR[5] * p[1] + sR[8] * p[2] + tf[L.off_pose + 2];
+            sv[i][0] = vx; sv[i][1] = vy; sv[i][2] = vz;
+            const float xproj = fr.phix * vx / vz + fr.cx;
+            const float yproj = fr.phiy * vy / vz + fr.cy;
+            const float* L6 = lm.base + (size_t)f * 6 * NLMK;
+            const float lx = L6[i], ly = L6[NLMK + i];
+            const float blx = L6[2 * NLMK + i], bux = L6[3 * NLMK + i];
+            const float bly = L6[4 * NLMK + i], buy = L6[5 * NLMK + i];
+            const float fs = fr.face_size;
+            const float Gxm = xproj - (lx + blx * fs);
+            const float Gym = yproj - (ly + bly * fs);
+            const float Gxp = (lx + bux * fs) - xproj;
+            const float Gyp = (ly + buy * fs) - yproj;
+            if (mode == 0) {
+                acc += -log((double)Gxm) - log((double)Gym) - log((double)Gxp) - log((double)Gyp);
+            } else {
+                swx[i] = 1.f / (Gxm * Gxm) + 1.f / (Gxp * Gxp);
+                swy[i] = 1.f / (Gym * Gym) + 1.f / (Gyp * Gyp);
+                sgx[i] = -1.f / Gxm + 1.f / Gxp;
+                sgy[i] = -1.f / Gym + 1.f / Gyp;
+            }
+        }
+        __syncthreads();
+        if (mode == 1) {
+            // projected-landmark gradients dx[i][j], dy[i][j]
+            for (int e = tid; e < NLMK * K; e += blockDim.x) {
+                const int i = e / K, j = e - i * K;
+                float dv[3] = {0.f, 0.f, 0.f};
+                bool geo = true;
+                if (j >= L.off_pose && j < L.off_pose + 3) {
+                    dv[j - L.off_pose] = 1.f;
+                } else if (j >= L.off_pose + 3 && j < L.off_light) {
+                    const float* d = sdR + 9 * (j - L.off_pose - 3);
+                    for (int a = 0; a < 3; ++a) dv[a] = d[a] * sp[i][0] + d[3 + a] * sp[i][1] + d[6 + a] * sp[i][2];
+                } else if (j >= L.off_exp && j < L.off_pose) {
+                    const int v = m.lmk_ids[i], c = j - L.off_exp;
+                    float e3[3];
+                    for (int a = 0; a < 3; ++a) e3[a] = m.exp_basis[(size_t)(3 * v + a) * m.k_exp + c];
+                    for (int a = 0; a < 3; ++a) dv[a] = sR[a] * e3[0] + sR[3 + a] * e3[1] + sR[6 + a] * e3[2];
+                } else if (j >= L.off_id && j < L.off_exp) {
+                    const int v = m.lmk_ids[i], c = j - L.off_id;
+                    float e3[3];
+                    for (int a = 0; a < 3; ++a) e3[a] = m.id_basis[(size_t)(3 * v + a) * m.k_id + c];
+                    for (int a = 0; a < 3; ++a) dv[a] = sR[a] * e3[0] + sR[3 + a] * e3[1] + sR[6 + a] * e3[2];
+                } else {
+                    geo = false;
+                }
+                float dxj = 0.f, dyj = 0.f;
+                if (geo) {
+                    const float vx = sv[i][0], vy = sv[i][1], vz = sv[i][2];
+                    dxj = fr.phix * (dv[0] * vz - dv[2] * vx) / (vz * vz);
+                    dyj = fr.phiy * (dv[1] * vz - dv[2] * vy) / (vz * vz);
+                }
+                sdx[i][j] = dxj;
+                sdy[i][j] = dyj;
+            }
+            __syncthreads();
+            for (int j = tid; j < K; j += blockDim.x) {
+                float s = 0.f;
+                for (int i = 0; i < NLMK; ++i) s += sgx[i] * sdx[i][j] + sgy[i] * sdy[i][j];
+                g[(size_t)f * K + j] += s;
+            }
+            for (int e = tid; e < K * K; e += blockDim.x) {
+                const int a = e / K, b = e - a * K;
+                float s = 0.f;
+                for (int i = 0; i < NLMK; ++i) s += swx[i] * sdx[i][a] * sdx[i][b] + swy[i] * sdy[i][a] * sdy[i][b];
+                H[(size_t)f * K * K + e] += s;
+            }
+        }
+    }
+    if (mode == 0) {
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if ((tid & 63) == 0) sred[tid >> 6] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            const double s = sred[0] + sred[1] + sred[2] + sred[3];
+            obj[f] = s;
+            objb[f] = s;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ batched Cholesky
+__device__ __forceinline__ int pk(int i, int j) { return i * (i + 1) / 2 + j; }   // i >= j
+
+template <typename T>
+__device__ bool chol_packed(T* A, int K, T* sflag)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int j = 0; j < K; ++j) {
+        if (tid == 0) {
+            const T d = A[pk(j, j)];
+            if (!(d > T(0))) sflag[0] = T(1);
+            A[pk(j, j)] = sqrt(d);
+        }
+        __syncthreads();
+        if (sflag[0] != T(0)) return false;
+        const T djj = A[pk(j, j)];
+        for (int i = j + 1 + tid; i < K; i += nt) A[pk(i, j)] /= djj;
+        __syncthreads();
+        const int m = K - j - 1;
+        const int npairs = m * (m + 1) / 2;
+        for (int e = tid; e < npairs; e += nt) {
+            // e -> (ii, kk) with 0 <= kk <= ii < m
+            int ii = (int)((sqrtf(8.f * e + 1.f) - 1.f) * 0.5f);
+            while ((ii + 1) * (ii + 2) / 2 <= e) ++ii;
+            while (ii * (ii + 1) / 2 > e) --ii;
+            const int kk = e - ii * (ii + 1) / 2;
+            const int i = j + 1 + ii, k = j + 1 + kk;
+            A[pk(i, k)] -= A[pk(i, j)] * A[pk(k, j)];
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// Solve L L^T x = b in place (x overwrites b), b in shared memory.
+template <typename T>
+__device__ void chol_solve_packed(const T* A, int K, T* b)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = 0; i < K; ++i) {
+        if (tid == 0) b[i] /= A[pk(i, i)];
+        __syncthreads();
+        const T yi = b[i];
+        for (int k = i + 1 + tid; k < K; k += nt) b[k] -= A[pk(k, i)] * yi;
+        __syncthreads();
+    }
+    for (int i = K - 1; i >= 0; --i) {
+        if (tid == 0) b[i] /= A[pk(i, i)];
+        __syncthreads();
+        const T xi = b[i];
+        for (int k = tid; k < i; k += nt) b[k] -= A[pk(i, k)] * xi;
+        __syncthreads();
+    }
+}
+
+constexpr int CMAXK = 112;
+
+// mode 0: FP64 factorisation and solve.  mode 1: FP32 factorisation and solve, then one
+// FP64 residual-refinement step (r = -g - H x computed in FP64, correction solved with the
+// FP32 factor).
+__global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict__ H, const float* __restrict__ g,
+                                                   float* __restrict__ delta, int* __restrict__ status,
+                                                   const int* __restrict__ mask, int mode)
+{
+    const int f = blockIdx.x;
+    if (mask && !mask[f]) return;
+    extern __shared__ double dsm[];
+    __shared__ double sflag[1];
+    __shared__ double sb[CMAXK];
+    __shared__ float sbf[CMAXK];
+    __shared__ float sflagf[1];
+    const float* Hf = H + (size_t)f * K * K;
+    const float* gf = g + (size_t)f * K;
+    const int tid = threadIdx.x;
+    const int np = K * (K + 1) / 2;
+    if (tid == 0) { sflag[0] = 0.0; sflagf[0] = 0.f; }
+    if (mode == 0) {
+        double* A = dsm;
+        for (int e = tid; e < np; e += blockDim.x) {
+            int i = (int)((sqrtf(8.f * e + 1.f) - 1.f) * 0.5f);
+            while ((i + 1) * (i + 2) / 2 <= e) ++i;
+            while (i * (i + 1) / 2 > e) --i;
+            const int j = e - i * (i + 1) / 2;
+            A[e] = (double)Hf[(size_t)i * K + j];
+        }
+        for (int i = tid; i < K; i += blockDim.x) sb[i] = -(double)gf[i];
+        __syncthreads();
+        if (!chol_packed<double>(A, K, sflag)) {
+            if (tid == 0) status[f] = TDI_NOT_PD;
+            return;
+        }
+        chol_solve_packed<double>(A, K, sb);
+        for (int i = tid; i < K; i += blockDim.x) delta[(size_t)f * K + i] = (float)sb[i];
+        if (tid == 0) status[f] = TDI_OK;
+    } else {
+        float* A = (float*)dsm;
+        for (int e = tid; e < np; e += blockDim.x) {
+            int i = (int)((sqrtf(8.f * e + 1.f) - 1.f) * 0.5f);
+            while ((i + 1) * (i + 2) / 2 <= e) ++i;
+            while (i * (i + 1) / 2 > e) --i;
+            const int j = e - i * (i + 1) / 2;
+            A[e] = Hf[(size_t)i * K + j];
+        }
+        for (int i = tid; i < K; i += blockDim.x) sbf[i] = -gf[i];
+        __syncthreads();
+        if (!chol_packed<float>(A, K, sflagf)) {
+            if (tid == 0) status[f] = TDI_NOT_PD;
+            return;
+        }
+        chol_solve_packed<float>(A, K, sbf);
+        // refinement: r = -g - H x0 in FP64
+        for (int i = tid; i < K; i += blockDim.x) {
+            double r = -(double)gf[i];
+            for (int k = 0; k < K; ++k) r -= (double)Hf[(size_t)i * K + k] * (double)sbf[k];
+            sb[i] = r;
+        }
+        __syncthreads();
+        // correction with the FP32 factor, carried in FP64 between the triangular solves
+        for (int i = 0; i < K; ++i) {
+            if (tid == 0) sb[i] /= (double)A[pk(i, i)];
+            __syncthreads();
+            const double yi = sb[i];
+            for (int k = i + 1 + tid; k < K; k += blockDim.x) sb[k] -= (double)A[pk(k, i)] * yi;
+            __syncthreads();
+        }
+        for (int i = K - 1; i >= 0; --i) {
+            if (tid == 0) sb[i] /= (double)A[pk(i, i)];
+            __syncthreads();
+            const double xi = sb[i];
+            for (int k = tid; k < i; k += blockDim.x) sb[k] -= (double)A[pk(i, k)] * xi;
+            __syncthreads();
+        }
+        for (int i = tid; i < K; i += blockDim.x) delta[(size_t)f * K + i] = (float)((double)sbf[i] + sb[i]);
+        if (tid == 0) status[f] = TDI_OK;
+    }
+}
+
+// ------------------------------------------------------------------ line-search state machine
+__global__ void k_ls_init(int n, int K, FitState* __restrict__ st, const int* __restrict__ chol_status,
+                          const float* __restrict__ delta, const float* __restrict__ g, const double* __restrict__ obj,
+                          int* __restrict__ searching)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    FitState s = st[f];
+    searching[f] = 0;
+    if (!s.active) return;
+    s.iters++;
+    if (chol_status[f] != TDI_OK) {
+        s.active = 0;
+        s.stop |= TDI_STOP_NOT_PD;
+        s.status = TDI_NOT_PD;
+    } else if (isnan(delta[(size_t)f * K])) {
+        s.active = 0;
+        s.stop |= TDI_STOP_NAN;
+        s.status = TDI_NAN;
+    } else {
+        double dg = 0.0;
+        for (int k = 0; k < K; ++k) dg += (double)delta[(size_t)f * K + k] * (double)g[(size_t)f * K + k];
+        s.dg = (float)dg;
+        s.obj0 = obj[f];
+        s.t = 1.f;
+        s.inner = 0;
+        s.searching = 1;
+        searching[f] = 1;
+    }
+    st[f] = s;
+}
+
+__global__ void k_ls_propose(int n, int K, int off_tz, float t_min, FitState* __restrict__ st,
+                             const float* __restrict__ theta, const float* __restrict__ delta,
+                             float* __restrict__ theta_try, int* __restrict__ searching, int* __restrict__ evalmask)
+{
+    const int f = blockIdx.x;
+    if (f >= n) return;
+    __shared__ int go;
+    if (threadIdx.x == 0) {
+        FitState s = st[f];
+        int ev = 0;
+        if (searching[f]) {
+            if (s.t < t_min) {
+                s.active = 0;
+                s.searching = 0;
+                s.stop |= TDI_STOP_STEP;
+                searching[f] = 0;
+            } else {
+                const float tz = theta[(size_t)f * K + off_tz] + s.t * delta[(size_t)f * K + off_tz];
+                if (tz < 0.f) {
+                    s.searching = 0;
+                    searching[f] = 0;
+                } else {
+                    ev = 1;
+                }
+            }
+            st[f] = s;
+        }
+        evalmask[f] = ev;
+        go = ev;
+    }
+    __syncthreads();
+    if (!go) return;
+    const float t = st[f].t;
+    for (int k = threadIdx.x; k < K; k += blockDim.x)
+        theta_try[(size_t)f * K + k] = theta[(size_t)f * K + k] + t * delta[(size_t)f * K + k];
+}
+
+__global__ void k_ls_decide(int n, int K, FitState* __restrict__ st, const int* __restrict__ evalmask,
+                            int* __restrict__ searching, const double* __restrict__ obj, const double* __restrict__ objb,
+                            const int* __restrict__ face_ok, float* __restrict__ theta,
+                            const float* __restrict__ theta_try, tdi_fit_options o)
+{
+    const int f = blockIdx.x;
+    if (f >= n || !evalmask[f]) return;
+    __shared__ int accept;
+    if (threadIdx.x == 0) {
+        FitState s = st[f];
+        int acc = 0;
+        if (!face_ok[f] || isnan(objb[f])) {
+            s.t *= o.armijo_beta;
+        } else {
+            s.inner++;
+            s.ls_evals++;
+            if (-s.dg < o.decrement_tol) {
+                s.active = 0;
+                s.searching = 0;
+                s.stop |= TDI_STOP_DECREMENT;
+            } else if (obj[f] < s.obj0 + (double)(o.armijo_alpha * s.t * s.dg)) {
+                acc = 1;
+                s.searching = 0;
+            } else {
+                s.t *= o.armijo_beta;
+            }
+            if (s.inner >= o.max_ls) s.searching = 0;
+        }
+        searching[f] = s.searching;
+        st[f] = s;
+        accept = acc;
+    }
+    __syncthreads();
+    if (!accept) return;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) theta[(size_t)f * K + k] = theta_try[(size_t)f * K + k];
+}
+
+// "Face too large" test of compute_nonrigid_shape2 (renderer.cu:836-857) on projected
+// landmark vertices 0, 9 and 40.
+__global__ void k_face_check(int n, int V, const int* __restrict__ lmk_ids, const float4* __restrict__ vproj,
+                             float ref_face_size, const int* __restrict__ mask, int* __restrict__ face_ok)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    if (mask && !mask[f]) return;
+    if (ref_face_size <= 0.f) { face_ok[f] = 1; return; }
+    const float4* vp = vproj + (size_t)f * V;
+    const float4 a = vp[lmk_ids[0]], b = vp[lmk_ids[9]], c = vp[lmk_ids[40]];
+    const float iod = fabsf(b.x - a.x), lem = fabsf(a.y - c.y), rem = fabsf(b.y - c.y);
+    const float lim = 2.f * ref_face_size;
+    face_ok[f] = !(iod > lim || lem > lim || rem > lim);
+}
+
+__global__ void k_copy_mask(int n, const FitState* __restrict__ st, int* __restrict__ mask)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < n) mask[f] = st[f].active;
+}
+
+} // namespace
+
+int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask,
+                   const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
+                   float* H, float* g, hipStream_t s)
+{
+    if (L.K > BMAXK) {
+        set_error("barrier: K=%d > %d", L.K, BMAXK);
+        return TDI_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_barrier, dim3(n), dim3(256), 0, s, m, L, frames, mask, theta, lm, use_landmarks, mode, obj,
+                       objb, H, g);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_cholesky(int n, int K, const float* H, const float* g, float* delta, int* status, int mode, const int* mask,
+                    hipStream_t s)
+{
+    if (K > CMAXK) {
+        set_error("cholesky: K=%d > %d", K, CMAXK);
+        return TDI_ERR_ARG;
+    }
+    const size_t lds = (size_t)K * (K + 1) / 2 * (mode == 0 ? sizeof(double) : sizeof(float));
+    hipLaunchKernelGGL(k_cholesky, dim3(n), dim3(256), lds, s, K, H, g, delta, status, mask, mode);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_ls_init(int n, int K, FitState* st, const int* chol_status, const float* delta, const float* g,
+                   const double* obj, int* searching, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ls_init, dim3((n + 255) / 256), dim3(256), 0, s, n, K, st, chol_status, delta, g, obj,
+                       searching);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_ls_propose(int n, int K, int off_tz, float t_min, FitState* st, const float* theta, const float* delta,
+                      float* theta_try, int* searching, int* evalmask, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ls_propose, dim3(n), dim3(64), 0, s, n, K, off_tz, t_min, st, theta, delta, theta_try,
+                       searching, evalmask);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_ls_decide(int n, int K, FitState* st, const int* evalmask, int* searching, const double* obj,
+                     const double* objb, const int* face_ok, float* theta, const float* theta_try,
+                     const tdi_fit_options& o, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ls_decide, dim3(n), dim3(64), 0, s, n, K, st, evalmask, searching, obj, objb, face_ok, theta,
+                       theta_try, o);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_face_check(int n, int V, const int* lmk_ids, const float4* vproj, float ref_face_size, const int* mask,
+                      int* face_ok, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_face_check, dim3((n + 255) / 256), dim3(256), 0, s, n, V, lmk_ids, vproj, ref_face_size, mask,
+                       face_ok);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_copy_mask(int n, const FitState* st, int* mask, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_mask, dim3((n + 255) / 256), dim3(256), 0, s, n, st, mask);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+} // namespace tdi

@@ -1,0 +1,221 @@
+"""Python binding of libtdi.so (the gfx950 fitting step) over its C ABI.
+
+Device memory and streams come from PyTorch; every computation runs in the HIP library.
+The library must be built (``python 3di_amd/build.py``); loading fails loudly otherwise —
+there is no CPU fallback.
+
+Mirrors the reference's fitting entry points for one frame and for many:
+``Batch.fit_plain`` is fit_3DMM_shape_rigid (model_fitter.cu:121-375) run frame by frame,
+``Batch.fit_batched`` runs the same algorithm for every frame of the batch on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import importlib
+import os
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+abi = importlib.import_module("3di_amd.abi")
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtdi.so")
+_lib = None
+
+_vp = C.c_void_p
+
+
+class TdiError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TdiError(f"{LIB_PATH} is missing: build it with `python 3di_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        L.tdi_last_error.restype = C.c_char_p
+        L.tdi_num_unknowns.restype = C.c_int32
+        L.tdi_ctx_create.argtypes = [_vp, _vp, C.c_int, C.POINTER(_vp)]
+        L.tdi_ctx_destroy.argtypes = [_vp]
+        L.tdi_batch_create.argtypes = [_vp, C.c_int32, _vp, C.POINTER(_vp)]
+        L.tdi_batch_destroy.argtypes = [_vp]
+        L.tdi_rotation.argtypes = [C.c_int32, _vp, _vp, _vp, _vp]
+        L.tdi_synthesize.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.tdi_rasterize.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_render.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_normal_equations.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_normal_equations_mixed.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_cholesky_solve.argtypes = [C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, C.c_int32, _vp]
+        L.tdi_fit_plain.argtypes = [_vp, _vp, _vp]
+        L.tdi_fit_batched.argtypes = [_vp, _vp, _vp, _vp]
+        L.tdi_blur_taps.argtypes = [C.c_float, _vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise TdiError(f"tdi error {rc}: {lib().tdi_last_error().decode()}")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dptr(t) -> int:
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise TdiError("expected a contiguous device tensor")
+    return t.data_ptr()
+
+
+def _stream():
+    torch = _torch()
+    return torch.cuda.current_stream().cuda_stream
+
+
+def blur_taps(sigma: float) -> np.ndarray:
+    t = np.zeros(15, np.float32)
+    _check(lib().tdi_blur_taps(C.c_float(sigma), t.ctypes.data))
+    return t
+
+
+class Context:
+    """The model resident in HBM (tdi_ctx)."""
+
+    def __init__(self, model, opts=None, device: int = 0):
+        self.model = model
+        self.opts = opts if opts is not None else abi.default_options()
+        self.md = abi.ModelDesc(model)
+        self.K = lib().tdi_num_unknowns(self.md.ref, C.byref(self.opts))
+        self.device = device
+        h = _vp()
+        _check(lib().tdi_ctx_create(C.addressof(self.md.desc), C.addressof(self.opts), device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().tdi_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Frames resident in HBM plus a workspace (tdi_batch)."""
+
+    def __init__(self, ctx: Context, frames: Sequence):
+        self.ctx = ctx
+        self.frames = list(frames)
+        self.fd = abi.FrameDescs(self.frames)
+        h = _vp()
+        _check(lib().tdi_batch_create(ctx.h, len(self.frames), C.addressof(self.fd.arr), C.byref(h)))
+        self.h = h
+        self.areas = np.array([f.width * f.height for f in self.frames], np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.areas)])
+
+    @property
+    def n(self):
+        return len(self.frames)
+
+    @property
+    def K(self):
+        return self.ctx.K
+
+    def close(self):
+        if self.h:
+            lib().tdi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _theta(self, theta):
+        torch = _torch()
+        if isinstance(theta, np.ndarray):
+            theta = torch.from_numpy(np.ascontiguousarray(theta, np.float32)).cuda()
+        return theta.float().contiguous().reshape(self.n, self.K)
+
+    # -------------------------------------------------------------- building blocks
+    def synthesize(self, theta):
+        torch = _torch()
+        th = self._theta(theta)
+        V = self.ctx.model.n_vertices
+        xyz = torch.empty((self.n, V, 3), device="cuda", dtype=torch.float32)
+        xyp = torch.empty((self.n, V, 2), device="cuda", dtype=torch.float32)
+        _check(lib().tdi_synthesize(self.h, _dptr(th), _dptr(xyz), _dptr(xyp), _stream()))
+        return xyz, xyp
+
+    def rasterize(self, xyz, xyp):
+        torch = _torch()
+        keys = torch.empty(int(self.offsets[-1]), device="cuda", dtype=torch.int64)
+        bary = torch.empty(int(self.offsets[-1]) * 2, device="cuda", dtype=torch.float32)
+        _check(lib().tdi_rasterize(self.h, _dptr(xyz.contiguous()), _dptr(xyp.contiguous()), _dptr(keys), _dptr(bary),
+                                   _stream()))
+        return keys, bary
+
+    def render(self, theta):
+        torch = _torch()
+        th = self._theta(theta)
+        img = torch.empty(int(self.offsets[-1]), device="cuda", dtype=torch.float32)
+        obj = torch.empty(self.n, device="cuda", dtype=torch.float32)
+        npix = torch.empty(self.n, device="cuda", dtype=torch.int32)
+        _check(lib().tdi_render(self.h, _dptr(th), _dptr(img), _dptr(obj), _dptr(npix), _stream()))
+        return img, obj, npix
+
+    def normal_equations(self, theta, mixed: bool = False):
+        torch = _torch()
+        th = self._theta(theta)
+        K = self.K
+        H = torch.empty((self.n, K, K), device="cuda", dtype=torch.float32)
+        g = torch.empty((self.n, K), device="cuda", dtype=torch.float32)
+        obj = torch.empty(self.n, device="cuda", dtype=torch.float32)
+        fn = lib().tdi_normal_equations_mixed if mixed else lib().tdi_normal_equations
+        _check(fn(self.h, _dptr(th), _dptr(H), _dptr(g), _dptr(obj), _stream()))
+        return H, g, obj
+
+    # -------------------------------------------------------------- fits
+    def fit_plain(self, theta0) -> Tuple[np.ndarray, List]:
+        th = np.ascontiguousarray(np.array(theta0, np.float32).reshape(self.n, self.K))
+        reps = (abi.tdi_fit_report * self.n)()
+        _check(lib().tdi_fit_plain(self.h, th.ctypes.data, C.addressof(reps)))
+        return th, list(reps)
+
+    def fit_batched(self, theta):
+        """theta: device tensor [n, K], updated in place.  Returns the per-frame reports."""
+        th = self._theta(theta)
+        reps = (abi.tdi_fit_report * self.n)()
+        _check(lib().tdi_fit_batched(self.h, _dptr(th), C.addressof(reps), _stream()))
+        return th, list(reps)
+
+
+def cholesky_solve(H, g, mode: int = 0):
+    torch = _torch()
+    n, K = g.shape
+    delta = torch.empty_like(g)
+    status = torch.empty(n, device="cuda", dtype=torch.int32)
+    _check(lib().tdi_cholesky_solve(n, K, _dptr(H.contiguous()), _dptr(g.contiguous()), _dptr(delta), _dptr(status),
+                                    mode, _stream()))
+    return delta, status
+
+
+def rotation(u):
+    torch = _torch()
+    u = u.float().contiguous()
+    n = u.shape[0]
+    R = torch.empty((n, 9), device="cuda", dtype=torch.float32)
+    dR = torch.empty((n, 27), device="cuda", dtype=torch.float32)
+    _check(lib().tdi_rotation(n, _dptr(u), _dptr(R), _dptr(dR), _stream()))
+    return R, dR

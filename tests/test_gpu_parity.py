@@ -1,0 +1,273 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 OpenMP oracle.
+
+Tolerances (FP32 device arithmetic vs FP64 oracle):
+  rotation R, dR            : 2e-6 absolute
+  projected vertices        : 2e-3 px, camera-space points 1e-5 relative
+  visibility / barycentrics : bit-exact given identical FP32 projected vertices
+  rendered intensity        : 2e-4 absolute where both sides see the same triangle
+  objective                 : 1e-4 relative
+  H, g (FP32 and BF16 hi/lo): 2e-3 relative (Frobenius / 2-norm) against FP64
+  Cholesky solve            : 1e-9 (FP64 mode) / 1e-6 (FP32 + FP64 refinement) relative
+"""
+import importlib
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from scenes import make_scene
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+tdi = importlib.import_module("3di_amd.tdi")
+abi = importlib.import_module("3di_amd.abi")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return make_scene(grid=48, k_exp=12, crops=(96, 128, 80), seed=11)
+
+
+@pytest.fixture(scope="module")
+def gpu(scene):
+    ctx = tdi.Context(scene.model, scene.opts)
+    b = tdi.Batch(ctx, scene.frames)
+    yield b
+    b.close()
+    ctx.close()
+
+
+def test_rotation():
+    _say("case: rotation")
+    rng = np.random.default_rng(3)
+    u = rng.normal(0, 0.5, (16, 3))
+    R, dR = tdi.rotation(torch.from_numpy(u.astype(np.float32)).cuda())
+    R = R.cpu().numpy()
+    dR = dR.cpu().numpy()
+    for i in range(16):
+        Ro, dRo = O.rodrigues(u[i])
+        assert np.abs(R[i].reshape(3, 3).T - Ro).max() < 2e-6
+        assert np.abs(dR[i].reshape(3, 3, 3).transpose(0, 2, 1) - dRo).max() < 2e-6
+
+
+def test_synthesize(scene, gpu):
+    _say("case: synthesize")
+    th = scene.init()
+    xyz, xyp = gpu.synthesize(th)
+    xyz = xyz.cpu().numpy()
+    xyp = xyp.cpu().numpy()
+    orc = scene.oracle()
+    for i in range(len(scene.frames)):
+        oxyz, oxyp, _, _ = orc.geometry(i, th[i])
+        assert np.abs(xyp[i] - oxyp).max() < 2e-3
+        assert np.abs(xyz[i] - oxyz).max() / np.abs(oxyz).max() < 1e-5
+
+
+@pytest.mark.parametrize("ntmp", [0, 4])
+def test_raster_bitexact(scene, ntmp):
+    _say(f"case: raster bit-exact ntmp={ntmp}")
+    opts = abi.default_options(**{f[0]: getattr(scene.opts, f[0]) for f in abi.tdi_fit_options._fields_})
+    opts.ntmp_cap = ntmp
+    ctx = tdi.Context(scene.model, opts)
+    b = tdi.Batch(ctx, scene.frames)
+    th = scene.init()
+    xyz, xyp = b.synthesize(th)
+    keys, bary = b.rasterize(xyz, xyp)
+    keys = keys.cpu().numpy().view(np.uint64)
+    bary = bary.cpu().numpy()
+    xyz_h = xyz.cpu().numpy()
+    xyp_h = xyp.cpu().numpy()
+    for i, f in enumerate(scene.frames):
+        o0, o1 = b.offsets[i], b.offsets[i + 1]
+        ok, ob = O.rasterize(scene.model.triangles, xyp_h[i, :, 0], xyp_h[i, :, 1], xyz_h[i, :, 2],
+                             f.x0, f.y0, f.width, f.height, ntmp)
+        gk = keys[o0:o1]
+        assert np.array_equal(gk, ok), f"frame {i}: {np.sum(gk != ok)} visibility mismatches"
+        vis = ok != np.uint64(~np.uint64(0))
+        gb = bary[2 * o0:2 * o1].reshape(-1, 2)[vis]
+        obv = ob.reshape(-1, 2)[vis]
+        assert np.array_equal(gb.view(np.uint32), obv.view(np.uint32)), "barycentrics differ"
+        assert vis.sum() > 100
+    b.close()
+    ctx.close()
+
+
+def test_render(scene, gpu):
+    _say("case: render")
+    th = scene.init()
+    img, obj, npix = gpu.render(th)
+    img = img.cpu().numpy()
+    obj = obj.cpu().numpy()
+    npix = npix.cpu().numpy()
+    orc = scene.oracle()
+    for i in range(len(scene.frames)):
+        r = orc.render(i, th[i])
+        o0, o1 = gpu.offsets[i], gpu.offsets[i + 1]
+        assert abs(int(npix[i]) - r["npix"]) <= max(3, 0.01 * r["npix"])
+        gi = img[o0:o1]
+        both = (r["raw"] != 0) & (gi != 0)
+        assert both.sum() > 0.95 * r["npix"]
+        assert np.abs(gi[both] - r["raw"][both]).max() < 2e-4
+        assert abs(obj[i] - r["obj"][2]) / abs(r["obj"][2]) < 1e-4, (obj[i], r["obj"])
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_normal_equations(scene, gpu, mixed):
+    _say(f"case: normal equations mixed={mixed}")
+    th = scene.init()
+    H, g, obj = gpu.normal_equations(th, mixed=mixed)
+    H = H.cpu().numpy().astype(np.float64)
+    g = g.cpu().numpy().astype(np.float64)
+    orc = scene.oracle()
+    for i in range(len(scene.frames)):
+        r = orc.normal_equations(i, th[i])
+        eH = np.linalg.norm(H[i] - r["H"]) / np.linalg.norm(r["H"])
+        eg = np.linalg.norm(g[i] - r["g"]) / np.linalg.norm(r["g"])
+        _say(f"  frame {i}: |dH|/|H| = {eH:.2e}  |dg|/|g| = {eg:.2e}")
+        assert eH < 2e-3 and eg < 2e-3
+
+
+def test_mixed_matches_fp32(scene, gpu):
+    _say("case: BF16 hi/lo vs FP32 normal equations")
+    th = scene.init()
+    H1, g1, _ = gpu.normal_equations(th, mixed=False)
+    H2, g2, _ = gpu.normal_equations(th, mixed=True)
+    H1, H2 = H1.cpu().numpy().astype(np.float64), H2.cpu().numpy().astype(np.float64)
+    g1, g2 = g1.cpu().numpy().astype(np.float64), g2.cpu().numpy().astype(np.float64)
+    for i in range(len(scene.frames)):
+        assert np.linalg.norm(H1[i] - H2[i]) / np.linalg.norm(H1[i]) < 1e-4
+        assert np.linalg.norm(g1[i] - g2[i]) / np.linalg.norm(g1[i]) < 1e-4
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_cholesky(mode):
+    _say(f"case: cholesky mode={mode}")
+    rng = np.random.default_rng(5)
+    n, K = 6, 89
+    A = rng.normal(size=(n, K, K))
+    H = np.einsum("nij,nkj->nik", A, A) + 0.1 * np.eye(K)
+    g = rng.normal(size=(n, K))
+    d, st = tdi.cholesky_solve(torch.from_numpy(H.astype(np.float32)).cuda(), torch.from_numpy(g.astype(np.float32)).cuda(),
+                               mode)
+    d = d.cpu().numpy().astype(np.float64)
+    assert (st.cpu().numpy() == 0).all()
+    for i in range(n):
+        ref = np.linalg.solve(H[i].astype(np.float32).astype(np.float64), -g[i].astype(np.float32).astype(np.float64))
+        err = np.linalg.norm(d[i] - ref) / np.linalg.norm(ref)
+        assert err < (1e-6 if mode == 0 else 1e-5), err
+    # a non-PD matrix reports TDI_NOT_PD
+    Hb = H.copy()
+    Hb[0] = -np.eye(K)
+    _, st = tdi.cholesky_solve(torch.from_numpy(Hb.astype(np.float32)).cuda(), torch.from_numpy(g.astype(np.float32)).cuda(),
+                               mode)
+    st = st.cpu().numpy()
+    assert st[0] == abi.TDI_NOT_PD and (st[1:] == 0).all()
+
+
+def _pose_exp_rms(a, b, lay):
+    sl = slice(lay["exp"], lay["pose"] + 6)
+    return float(np.sqrt(np.mean((a[:, sl] - b[:, sl]) ** 2)))
+
+
+def _f32(x):
+    return np.asarray(x, np.float32).astype(np.float64)
+
+
+def _one_iteration_opts(scene):
+    opts = abi.default_options(**{f[0]: getattr(scene.opts, f[0]) for f in abi.tdi_fit_options._fields_})
+    opts.max_iters = 1
+    return opts
+
+
+def test_one_iteration_plain_vs_oracle(scene):
+    """One GN iteration (render, J^T J, FP64 Cholesky, Armijo line search) from the same
+    FP32 start: the step must match the FP64 oracle tightly."""
+    _say("case: one GN iteration, plain path vs oracle")
+    opts = _one_iteration_opts(scene)
+    th0 = _f32(scene.init())
+    orc = scene.oracle(opts)
+    tho, repo = orc.fit(th0)
+    ctx = tdi.Context(scene.model, opts)
+    b = tdi.Batch(ctx, scene.frames)
+    thg, repg = b.fit_plain(th0)
+    step = _pose_exp_rms(tho, th0, scene.lay)
+    rms = _pose_exp_rms(thg.astype(np.float64), tho, scene.lay)
+    _say(f"  step RMS {step:.3e}, GPU vs oracle RMS {rms:.3e}, ls evals gpu "
+         f"{[r.ls_evals for r in repg]} oracle {[r.ls_evals for r in repo]}")
+    assert [r.ls_evals for r in repg] == [r.ls_evals for r in repo]
+    assert rms < 1e-4 * max(1.0, step)
+    b.close()
+    ctx.close()
+
+
+def test_one_iteration_batched_vs_oracle(scene):
+    _say("case: one GN iteration, batched mixed path vs oracle")
+    opts = _one_iteration_opts(scene)
+    th0 = _f32(scene.init())
+    orc = scene.oracle(opts)
+    tho, repo = orc.fit(th0)
+    ctx = tdi.Context(scene.model, opts)
+    b = tdi.Batch(ctx, scene.frames)
+    thg, repg = b.fit_batched(torch.from_numpy(th0.astype(np.float32)).cuda())
+    thg = thg.cpu().numpy().astype(np.float64)
+    step = _pose_exp_rms(tho, th0, scene.lay)
+    rms = _pose_exp_rms(thg, tho, scene.lay)
+    _say(f"  step RMS {step:.3e}, GPU vs oracle RMS {rms:.3e}, ls evals gpu "
+         f"{[r.ls_evals for r in repg]} oracle {[r.ls_evals for r in repo]}")
+    assert [r.ls_evals for r in repg] == [r.ls_evals for r in repo]
+    assert rms < 1e-4 * max(1.0, step)
+    b.close()
+    ctx.close()
+
+
+def _oracle_floor(scene, th0):
+    """The oracle's own spread under a 1e-6 perturbation of the start: the rasterised
+    objective is piecewise constant in visibility, so whole-fit trajectories amplify
+    round-off; parity of whole fits is judged against this floor."""
+    orc = scene.oracle()
+    a, ra = orc.fit(th0)
+    bb, _ = orc.fit(th0 + 1e-6)
+    return a, ra, _pose_exp_rms(a, bb, scene.lay)
+
+
+def test_fit_plain_vs_oracle(scene, gpu):
+    _say("case: plain fit vs oracle")
+    th0 = _f32(scene.init())
+    tho, repo, floor = _oracle_floor(scene, th0)
+    thg, repg = gpu.fit_plain(th0)
+    rms = _pose_exp_rms(thg.astype(np.float64), tho, scene.lay)
+    err_o = _pose_exp_rms(tho, scene.truth(), scene.lay)
+    err_g = _pose_exp_rms(thg.astype(np.float64), scene.truth(), scene.lay)
+    _say(f"  plain GPU vs FP64 oracle RMS {rms:.2e} (oracle self-spread {floor:.2e}); error vs truth "
+         f"gpu {err_g:.3f} oracle {err_o:.3f}; iters gpu {[r.iters for r in repg]} oracle {[r.iters for r in repo]}")
+    assert rms <= max(3 * floor, 1e-3)
+    assert err_g < 1.5 * err_o + 0.05
+    obj_g = np.array([r.objective for r in repg])
+    obj_o = np.array([r.objective for r in repo])
+    assert np.all(obj_g < obj_o + 0.01 * np.abs(obj_o))
+
+
+def test_fit_batched_vs_oracle(scene, gpu):
+    _say("case: batched fit vs oracle")
+    th0 = _f32(scene.init())
+    tho, repo, floor = _oracle_floor(scene, th0)
+    thg = torch.from_numpy(th0.astype(np.float32)).cuda()
+    thg, repg = gpu.fit_batched(thg)
+    thg = thg.cpu().numpy().astype(np.float64)
+    rms = _pose_exp_rms(thg, tho, scene.lay)
+    err_o = _pose_exp_rms(tho, scene.truth(), scene.lay)
+    err_g = _pose_exp_rms(thg, scene.truth(), scene.lay)
+    _say(f"  batched GPU vs FP64 oracle RMS {rms:.2e} (oracle self-spread {floor:.2e}); error vs truth "
+         f"gpu {err_g:.3f} oracle {err_o:.3f}; iters gpu {[r.iters for r in repg]} oracle {[r.iters for r in repo]}")
+    assert rms <= max(3 * floor, 1e-3)
+    assert err_g < 1.5 * err_o + 0.05
+    obj_g = np.array([r.objective for r in repg])
+    obj_o = np.array([r.objective for r in repo])
+    assert np.all(obj_g < obj_o + 0.01 * np.abs(obj_o))
