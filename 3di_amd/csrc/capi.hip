@@ -94,6 +94,8 @@ struct Workspace {
     float* bary = nullptr;
     float *img = nullptr, *tmp = nullptr, *blurred = nullptr;
     float4* pixg = nullptr;
+    int4* pnb = nullptr;
+    float* pcoef = nullptr;
     float* dI = nullptr;
     float *theta = nullptr, *theta_try = nullptr, *delta = nullptr, *H = nullptr, *g = nullptr;
     double *obj = nullptr, *objb = nullptr;
@@ -144,6 +146,7 @@ void tdi_default_options(tdi_fit_options* o)
     o->decrement_tol = 3e-4f;
     o->blur_sigma = 0.75f;
     o->ref_face_size = 0.f;
+    o->mixed_flags = TDI_MIXED_SYNTH_MFMA | TDI_MIXED_JTJ_BF16 | TDI_MIXED_CHOL_F32;
 }
 
 int32_t tdi_num_unknowns(const tdi_model_desc* m, const tdi_fit_options* o) { return make_layout(m, o).K; }
@@ -250,6 +253,8 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     w.tmp = A.get<float>(cap);
     w.blurred = A.get<float>(cap);
     w.pixg = A.get<float4>(cap);
+    w.pnb = A.get<int4>(cap);
+    w.pcoef = A.get<float>(cap);
     w.dI = A.get<float>((size_t)cap * K);
     w.theta = A.get<float>((size_t)C * K);
     w.theta_try = A.get<float>((size_t)C * K);
@@ -388,7 +393,7 @@ extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd,
     double budget_gb = env ? atof(env) : 48.0;
     const int V = c->m.V, K = c->L.K;
     const double per_frame = (double)V * 40 + 64.0 * K * K;
-    const double per_pixel = 60.0 + 4.0 * K;
+    const double per_pixel = 80.0 + 4.0 * K;
     int C = n;
     long long cap = off;
     while (C > 1 && (per_frame * C + per_pixel * (double)maxA * C) > budget_gb * 1e9) C = (C + 1) / 2;
@@ -423,12 +428,12 @@ extern "C" int tdi_batch_destroy(tdi_batch* b)
 // ---------------------------------------------------------------------------- evaluation
 // Objective at theta for the frames in `mask`: synthesis, face check, raster, resolve,
 // barrier objective, shading, blur, gradients (Renderer::render + evaluate_objective_function).
-static int eval_render(tdi_batch* b, int n, const float* theta, const int* mask, bool mixed, hipStream_t s)
+static int eval_render(tdi_batch* b, int n, const float* theta, const int* mask, int mixed, hipStream_t s)
 {
     tdi_ctx* c = b->ctx;
     Workspace& w = b->ws;
     int rc;
-    if (mixed)
+    if (mixed & TDI_MIXED_SYNTH_MFMA)
         rc = launch_synthesize_gemm(c->m, c->L, w.frames, n, theta, mask, w.P0, w.P, w.vproj, nullptr, s);
     else
         rc = launch_synthesize(c->m, c->L, w.frames, n, theta, mask, w.P0, w.P, w.vproj, s);
@@ -445,12 +450,13 @@ static int eval_render(tdi_batch* b, int n, const float* theta, const int* mask,
                            w.max_area, s)))
         return rc;
     if ((rc = launch_blur(w.frames, n, mask, c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
-    return launch_grads(w.frames, n, mask, w.vis, w.npix, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.obj,
+    return launch_grads(w.frames, n, mask, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
+                        w.pcoef, w.obj,
                         w.max_area, s);
 }
 
 // Objective, gradient and Gauss-Newton matrix at theta (get_obj_hessian_and_gradient_multiframe).
-static int eval_full(tdi_batch* b, int n, const float* theta, const int* mask, bool mixed, hipStream_t s)
+static int eval_full(tdi_batch* b, int n, const float* theta, const int* mask, int mixed, hipStream_t s)
 {
     tdi_ctx* c = b->ctx;
     Workspace& w = b->ws;
@@ -459,24 +465,16 @@ static int eval_full(tdi_batch* b, int n, const float* theta, const int* mask, b
     if ((rc = launch_dI(c->m, c->L, w.frames, n, mask, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
                         w.max_area, s)))
         return rc;
-    if (mixed)
-        rc = launch_jtj_mfma(c->L, w.frames, n, mask, w.vis, w.npix, w.cum, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
-                             w.partials, w.S, s);
+    if (mixed & TDI_MIXED_JTJ_BF16)
+        rc = launch_jtj_mfma(c->L, w.frames, n, mask, w.vis, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
+                             w.partials, w.S, (mixed & TDI_MIXED_JTR_BF16) ? 1 : 0,
+                             (mixed & TDI_MIXED_TWO_TERM) ? 1 : 0, s);
     else
-        rc = launch_jtj_f32(c->L, w.frames, n, mask, w.vis, w.npix, w.cum, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
+        rc = launch_jtj_f32(c->L, w.frames, n, mask, w.vis, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
                             w.partials, w.S, s);
     if (rc) return rc;
     return launch_barrier(c->m, c->L, w.frames, n, mask, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj, w.objb,
                           w.H, w.g, s);
-}
-
-static int need_single_chunk(tdi_batch* b)
-{
-    if (b->ws.C < b->n) {
-        set_error("building-block calls need the whole batch in one workspace (%d frames > %d)", b->n, b->ws.C);
-        return TDI_ERR_ARG;
-    }
-    return TDI_OK;
 }
 
 // active mask for real frames of the current chunk
@@ -491,69 +489,89 @@ static int set_mask_all(tdi_batch* b, hipStream_t s)
 
 extern "C" {
 
+int tdi_workspace_frames(tdi_batch* b) { return b->ws.C; }
+
 int tdi_rotation(int32_t n, const float* u, float* R, float* dR, void* stream)
 {
     return launch_misc_rotation(n, u, R, dR, (hipStream_t)stream);
 }
 
+// The building blocks run chunk by chunk over the workspace; outputs are in batch order.
 int tdi_synthesize(tdi_batch* b, const float* theta, float* xyz, float* xyp, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    int rc = need_single_chunk(b);
-    if (rc) return rc;
     tdi_ctx* c = b->ctx;
-    if ((rc = ws_load(b, 0, b->n, s))) return rc;
-    if ((rc = launch_synthesize(c->m, c->L, b->ws.frames, b->n, theta, nullptr, b->ws.P0, b->ws.P, b->ws.vproj, s)))
-        return rc;
-    return launch_misc_unpack(b->n, c->m.V, b->ws.P, b->ws.vproj, xyz, xyp, s);
+    const int K = c->L.K, V = c->m.V;
+    for (int i0 = 0; i0 < b->n; i0 += b->ws.C) {
+        const int n = std::min(b->ws.C, b->n - i0);
+        int rc = ws_load(b, i0, n, s);
+        if (rc) return rc;
+        if ((rc = launch_synthesize(c->m, c->L, b->ws.frames, n, theta + (size_t)i0 * K, nullptr, b->ws.P0, b->ws.P,
+                                    b->ws.vproj, s)))
+            return rc;
+        if ((rc = launch_misc_unpack(n, V, b->ws.P, b->ws.vproj, xyz + (size_t)i0 * V * 3, xyp + (size_t)i0 * V * 2, s)))
+            return rc;
+    }
+    return TDI_OK;
 }
 
 int tdi_rasterize(tdi_batch* b, const float* xyz, const float* xyp, uint64_t* keys, float* bary, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    int rc = need_single_chunk(b);
-    if (rc) return rc;
     tdi_ctx* c = b->ctx;
     Workspace& w = b->ws;
-    if ((rc = ws_load(b, 0, b->n, s))) return rc;
-    if ((rc = launch_misc_pack(b->n, c->m.V, xyz, xyp, w.vproj, s))) return rc;
-    if ((rc = launch_clear_keys(w.keys, b->total, s))) return rc;
-    if ((rc = launch_misc_fill_f32(w.bary, 2 * b->total, 0.f, s))) return rc;
-    if ((rc = launch_raster(c->m, w.frames, b->n, nullptr, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-    if ((rc = launch_resolve(c->m, w.frames, b->n, nullptr, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area,
-                             s)))
-        return rc;
-    if (keys) TDI_HIP(hipMemcpyAsync(keys, w.keys, sizeof(uint64_t) * b->total, hipMemcpyDeviceToDevice, s));
-    if (bary) TDI_HIP(hipMemcpyAsync(bary, w.bary, sizeof(float) * 2 * b->total, hipMemcpyDeviceToDevice, s));
+    const int V = c->m.V;
+    for (int i0 = 0; i0 < b->n; i0 += w.C) {
+        const int n = std::min(w.C, b->n - i0);
+        int rc = ws_load(b, i0, n, s);
+        if (rc) return rc;
+        const long long off = b->hframes[i0].obs_off;
+        const long long len = (i0 + n < b->n ? b->hframes[i0 + n].obs_off : b->total) - off;
+        if ((rc = launch_misc_pack(n, V, xyz + (size_t)i0 * V * 3, xyp + (size_t)i0 * V * 2, w.vproj, s))) return rc;
+        if ((rc = launch_clear_keys(w.keys, len, s))) return rc;
+        if ((rc = launch_misc_fill_f32(w.bary, 2 * len, 0.f, s))) return rc;
+        if ((rc = launch_raster(c->m, w.frames, n, nullptr, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+        if ((rc = launch_resolve(c->m, w.frames, n, nullptr, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, s)))
+            return rc;
+        if (keys) TDI_HIP(hipMemcpyAsync(keys + off, w.keys, sizeof(uint64_t) * len, hipMemcpyDeviceToDevice, s));
+        if (bary) TDI_HIP(hipMemcpyAsync(bary + 2 * off, w.bary, sizeof(float) * 2 * len, hipMemcpyDeviceToDevice, s));
+    }
     return TDI_OK;
 }
 
 int tdi_render(tdi_batch* b, const float* theta, float* img, float* obj, int32_t* npix, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    int rc = need_single_chunk(b);
-    if (rc) return rc;
     Workspace& w = b->ws;
-    if ((rc = ws_load(b, 0, b->n, s))) return rc;
-    if ((rc = eval_render(b, b->n, theta, nullptr, false, s))) return rc;
-    if (img) TDI_HIP(hipMemcpyAsync(img, w.img, sizeof(float) * b->total, hipMemcpyDeviceToDevice, s));
-    if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj, b->n, s))) return rc;
-    if (npix) TDI_HIP(hipMemcpyAsync(npix, w.npix, sizeof(int) * b->n, hipMemcpyDeviceToDevice, s));
+    const int K = b->ctx->L.K;
+    for (int i0 = 0; i0 < b->n; i0 += w.C) {
+        const int n = std::min(w.C, b->n - i0);
+        int rc = ws_load(b, i0, n, s);
+        if (rc) return rc;
+        const long long off = b->hframes[i0].obs_off;
+        const long long len = (i0 + n < b->n ? b->hframes[i0 + n].obs_off : b->total) - off;
+        if ((rc = eval_render(b, n, theta + (size_t)i0 * K, nullptr, 0, s))) return rc;
+        if (img) TDI_HIP(hipMemcpyAsync(img + off, w.img, sizeof(float) * len, hipMemcpyDeviceToDevice, s));
+        if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj + i0, n, s))) return rc;
+        if (npix) TDI_HIP(hipMemcpyAsync(npix + i0, w.npix, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    }
     return TDI_OK;
 }
 
 static int normal_eq(tdi_batch* b, const float* theta, float* H, float* g, float* obj, void* stream, bool mixed)
 {
     hipStream_t s = (hipStream_t)stream;
-    int rc = need_single_chunk(b);
-    if (rc) return rc;
     Workspace& w = b->ws;
     const int K = b->ctx->L.K;
-    if ((rc = ws_load(b, 0, b->n, s))) return rc;
-    if ((rc = eval_full(b, b->n, theta, nullptr, mixed, s))) return rc;
-    if (H) TDI_HIP(hipMemcpyAsync(H, w.H, sizeof(float) * b->n * K * K, hipMemcpyDeviceToDevice, s));
-    if (g) TDI_HIP(hipMemcpyAsync(g, w.g, sizeof(float) * b->n * K, hipMemcpyDeviceToDevice, s));
-    if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj, b->n, s))) return rc;
+    for (int i0 = 0; i0 < b->n; i0 += w.C) {
+        const int n = std::min(w.C, b->n - i0);
+        int rc = ws_load(b, i0, n, s);
+        if (rc) return rc;
+        if ((rc = eval_full(b, n, theta + (size_t)i0 * K, nullptr, mixed ? TDI_MIXED_JTJ_BF16 : 0, s))) return rc;
+        if (H) TDI_HIP(hipMemcpyAsync(H + (size_t)i0 * K * K, w.H, sizeof(float) * n * K * K, hipMemcpyDeviceToDevice, s));
+        if (g) TDI_HIP(hipMemcpyAsync(g + (size_t)i0 * K, w.g, sizeof(float) * n * K, hipMemcpyDeviceToDevice, s));
+        if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj + i0, n, s))) return rc;
+    }
     return TDI_OK;
 }
 
@@ -572,6 +590,87 @@ int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, flo
 {
     if (n <= 0 || K <= 0 || (mode != 0 && mode != 1)) { set_error("bad cholesky arguments"); return TDI_ERR_ARG; }
     return launch_cholesky(n, K, H, g, delta, status, mode, nullptr, (hipStream_t)stream);
+}
+
+// Stage timers: one full evaluation (the work of one GN iteration before the solve) with
+// HIP events around each stage, on the batch's stream.  Mirrors the reference's compile-time
+// MEASURE_TIME timers (model_fitter.cu:43-58, derivative_computer.cu:32-37,141-172).
+// Stages: 0 synthesis, 1 raster (clear+raster+resolve), 2 shade+blur+grads, 3 barrier,
+// 4 dI, 5 J^T J (+reduce), 6 Cholesky.  ms[7] receives the mean per repetition.
+int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t reps, float* ms)
+{
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    hipStream_t s = b->stream;
+    const int n = std::min(b->n, w.C);   // the first workspace chunk
+    int rc;
+    if ((rc = ws_load(b, 0, n, s))) return rc;
+    hipEvent_t ev[8];
+    for (int i = 0; i < 8; ++i) TDI_HIP(hipEventCreate(&ev[i]));
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int r = 0; r < reps; ++r) {
+        TDI_HIP(hipEventRecord(ev[0], s));
+        if (mixed & TDI_MIXED_SYNTH_MFMA)
+            rc = launch_synthesize_gemm(c->m, c->L, w.frames, n, theta, nullptr, w.P0, w.P, w.vproj, nullptr, s);
+        else
+            rc = launch_synthesize(c->m, c->L, w.frames, n, theta, nullptr, w.P0, w.P, w.vproj, s);
+        if (rc) return rc;
+        TDI_HIP(hipEventRecord(ev[1], s));
+        if ((rc = launch_clear_keys_masked(w.frames, n, nullptr, w.keys, w.max_area, s))) return rc;
+        if ((rc = launch_raster(c->m, w.frames, n, nullptr, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+        if ((rc = launch_resolve(c->m, w.frames, n, nullptr, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, s)))
+            return rc;
+        TDI_HIP(hipEventRecord(ev[2], s));
+        if ((rc = launch_shade(c->m, c->L, w.frames, n, nullptr, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
+                               w.max_area, s)))
+            return rc;
+        if ((rc = launch_blur(w.frames, n, nullptr, c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
+        if ((rc = launch_grads(w.frames, n, nullptr, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg,
+                               w.pnb, w.pcoef, w.obj,
+                               w.max_area, s)))
+            return rc;
+        TDI_HIP(hipEventRecord(ev[3], s));
+        if ((rc = launch_barrier(c->m, c->L, w.frames, n, nullptr, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
+                                 w.objb, w.H, w.g, s)))
+            return rc;
+        TDI_HIP(hipEventRecord(ev[4], s));
+        if ((rc = launch_dI(c->m, c->L, w.frames, n, nullptr, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
+                            w.max_area, s)))
+            return rc;
+        TDI_HIP(hipEventRecord(ev[5], s));
+        if (mixed & TDI_MIXED_JTJ_BF16)
+            rc = launch_jtj_mfma(c->L, w.frames, n, nullptr, w.vis, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H,
+                                 w.g, w.partials, w.S, (mixed & TDI_MIXED_JTR_BF16) ? 1 : 0,
+                                 (mixed & TDI_MIXED_TWO_TERM) ? 1 : 0, s);
+        else
+            rc = launch_jtj_f32(c->L, w.frames, n, nullptr, w.vis, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H,
+                                w.g, w.partials, w.S, s);
+        if (rc) return rc;
+        if ((rc = launch_barrier(c->m, c->L, w.frames, n, nullptr, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj,
+                                 w.objb, w.H, w.g, s)))
+            return rc;
+        TDI_HIP(hipEventRecord(ev[6], s));
+        if ((rc = launch_cholesky(n, c->L.K, w.H, w.g, w.delta, w.status, (mixed & TDI_MIXED_CHOL_F32) ? 1 : 0, nullptr, s)))
+            return rc;
+        TDI_HIP(hipEventRecord(ev[7], s));
+        TDI_HIP(hipEventSynchronize(ev[7]));
+        for (int k = 0; k < 7; ++k) {
+            float t = 0;
+            TDI_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+            acc[k] += t;
+        }
+    }
+    for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
+    for (int k = 0; k < 7; ++k) ms[k] = (float)(acc[k] / reps);
+    return TDI_OK;
+}
+
+// Visible-pixel counts of the last evaluation (host copy), for algorithmic-work accounting.
+int tdi_last_pixel_counts(tdi_batch* b, int32_t* npix)
+{
+    TDI_HIP(hipStreamSynchronize(b->stream));
+    TDI_HIP(hipMemcpy(npix, b->ws.npix, sizeof(int) * b->ws.n_cur, hipMemcpyDeviceToHost));
+    return TDI_OK;
 }
 
 // ---------------------------------------------------------------------------- plain fit
@@ -599,7 +698,7 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
         for (int it = 0; it < o.max_iters; ++it) {
             outer++;
             if (terminate) break;
-            if ((rc = eval_full(b, 1, w.theta, w.mask, false, s))) return rc;
+            if ((rc = eval_full(b, 1, w.theta, w.mask, 0, s))) return rc;
             if ((rc = launch_cholesky(1, K, w.H, w.g, w.delta, w.status, 0, w.mask, s))) return rc;
             int st = 0, np = 0;
             double obj0 = 0;
@@ -622,7 +721,7 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
                 for (int k = 0; k < K; ++k) tt[k] = th[k] + t * delta[k];
                 if (tt[off_tz] < 0.f) break;
                 TDI_HIP(hipMemcpyAsync(w.theta_try, tt.data(), sizeof(float) * K, hipMemcpyHostToDevice, s));
-                if ((rc = eval_render(b, 1, w.theta_try, w.mask, false, s))) return rc;
+                if ((rc = eval_render(b, 1, w.theta_try, w.mask, 0, s))) return rc;
                 double objt = 0, objb = 0;
                 int fok = 1;
                 TDI_HIP(hipMemcpyAsync(&objt, w.obj, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -642,7 +741,7 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
             }
         }
         if (!terminate && !(rep.stop & (TDI_STOP_NAN | TDI_STOP_NOT_PD))) rep.stop |= TDI_STOP_MAXITER;
-        if ((rc = eval_render(b, 1, w.theta, w.mask, false, s))) return rc;
+        if ((rc = eval_render(b, 1, w.theta, w.mask, 0, s))) return rc;
         double fin = 0;
         TDI_HIP(hipMemcpyAsync(&fin, w.obj, sizeof(double), hipMemcpyDeviceToHost, s));
         TDI_HIP(hipStreamSynchronize(s));
@@ -672,14 +771,16 @@ static int record_fit(tdi_batch* b, hipStream_t s)
     int rc;
     for (int it = 0; it < o.max_iters; ++it) {
         if ((rc = launch_copy_mask(n, w.st, w.mask, s))) return rc;
-        if ((rc = eval_full(b, w.C, w.theta, w.mask, true, s))) return rc;
-        if ((rc = launch_cholesky(n, K, w.H, w.g, w.delta, w.status, 1, w.mask, s))) return rc;
+        if ((rc = eval_full(b, w.C, w.theta, w.mask, o.mixed_flags, s))) return rc;
+        if ((rc = launch_cholesky(n, K, w.H, w.g, w.delta, w.status, (o.mixed_flags & TDI_MIXED_CHOL_F32) ? 1 : 0,
+                                  w.mask, s)))
+            return rc;
         if ((rc = launch_ls_init(n, K, w.st, w.status, w.delta, w.g, w.obj, w.searching, s))) return rc;
         for (int r = 0; r < rounds; ++r) {
             if ((rc = launch_ls_propose(n, K, off_tz, o.t_min, w.st, w.theta, w.delta, w.theta_try, w.searching,
                                         w.evalmask, s)))
                 return rc;
-            if ((rc = eval_render(b, w.C, w.theta_try, w.evalmask, true, s))) return rc;
+            if ((rc = eval_render(b, w.C, w.theta_try, w.evalmask, o.mixed_flags, s))) return rc;
             if ((rc = launch_ls_decide(n, K, w.st, w.evalmask, w.searching, w.obj, w.objb, w.face_ok, w.theta,
                                        w.theta_try, o, s)))
                 return rc;
@@ -737,7 +838,7 @@ extern "C" int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* repor
         if (reports) {
             // objective at the returned parameters
             if ((rc = set_mask_all(b, s))) return rc;
-            if ((rc = eval_render(b, w.C, w.theta, w.mask, true, s))) return rc;
+            if ((rc = eval_render(b, w.C, w.theta, w.mask, c->opts.mixed_flags, s))) return rc;
         }
         std::vector<double> hobj(w.C);
         std::vector<int> hnp(w.C);

@@ -104,8 +104,21 @@ __device__ __forceinline__ bool cover_rn(float bx, float by, float cx, float cy,
     const float d20 = v0x * v2x + v0y * v2y;
     const float d21 = v1x * v2x + v1y * v2y;
     const float den = d00 * d11 - d01 * d01;
-    const float a = (d11 * d20 - d01 * d21) / den;
-    const float b = (d00 * d21 - d01 * d20) / den;
+    const float na = d11 * d20 - d01 * d21;
+    const float nb = d00 * d21 - d01 * d20;
+    // Sign pre-test: alpha > 0 and beta > 0 need num/den > 0.  For every finite non-zero
+    // den the quotients here are normal numbers (|num| >= ~1e-21, |den| <= ~1e11 at pixel
+    // coordinates), so the quotient's sign is sign(num) * sign(den) and the test only skips
+    // the divisions of samples the full test rejects; den == 0 never covers (inf + inf > 1).
+    const bool pa = (na > 0.f && den > 0.f) || (na < 0.f && den < 0.f);
+    const bool pb = (nb > 0.f && den > 0.f) || (nb < 0.f && den < 0.f);
+    if (!(pa && pb)) {
+        *al = 0.f;
+        *be = 0.f;
+        return false;
+    }
+    const float a = na / den;
+    const float b = nb / den;
     *al = a;
     *be = b;
     return a > 0.f && b > 0.f && (a + b <= 1.f);
@@ -116,6 +129,24 @@ __device__ __forceinline__ float gamma_rn(float al, float be)
 {
 #pragma clang fp contract(off)
     return (1.f - al) - be;
+}
+
+// Compacted neighbour positions of sorted position `pos` (fill_krels, funcs.cu:288-320,
+// with cumM0 = inclusive occupancy count): left/right = last rendered pixel at or before
+// the horizontal neighbour, above/below = previous/next rendered pixel; the first and last
+// positions keep 0.  Returns (kl, kr, ka, kb).
+__device__ inline int4 pixel_nbrs(const int* __restrict__ cum, int k, long long base, int pos, int N, int W, int H)
+{
+    if (pos == 0 || pos == N - 1) return make_int4(0, 0, 0, 0);
+    const int x = k / H;
+    const int kleft = (x > 0) ? k - H : 0;
+    const int kright = (x < W - 1) ? k + H : 0;
+    const int cl = cum[base + kleft];
+    int kl = cl > 1 ? cl - 1 : 0;
+    if (kl >= N - 1) kl = N - 1;
+    int kr = cum[base + kright] - 1;
+    if (kr < 0) kr = 0;
+    return make_int4(kl, kr, pos - 1, pos + 1);
 }
 
 // Rodrigues rotation and its derivatives in FP32 (fill_theta_etc, rotation_header.cu:46-149).
@@ -155,6 +186,23 @@ __device__ inline void rodrigues_f32(const float* u, float* R, float* dR)
                 dR[9 * i + c * 3 + r] = acc * inv;
             }
     }
+}
+
+// Rodrigues rotation in FP64 (used for the vertex transform so that projected vertices come
+// out correctly rounded to FP32).
+__device__ inline void rodrigues_f64(const float* uf, double* R)
+{
+    const double u[3] = {uf[0], uf[1], uf[2]};
+    const double th = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double n0 = u[0] / th, n1 = u[1] / th, n2 = u[2] / th;
+    const double N[9] = {0, n2, -n1, -n2, 0, n0, n1, -n0, 0};
+    const double s = sin(th), c1 = 1.0 - cos(th);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; ++k) acc += N[k * 3 + r] * N[c * 3 + k];
+            R[c * 3 + r] = c1 * acc + s * N[c * 3 + r] + (r == c ? 1.0 : 0.0);
+        }
 }
 
 // ------------------------------------------------------------------ host helpers
@@ -201,8 +249,8 @@ int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int
 int launch_blur(const FrameGPU* frames, int n, const int* mask, const float* taps, const float* in, float* tmp,
                 float* out, int max_area, hipStream_t s);
 int launch_grads(const FrameGPU* frames, int n, const int* mask, const int* vis_list, const int* npix,
-                 const float* blurred, const float2* obs_g, float w, float4* pixg, double* obj, int max_area,
-                 hipStream_t s);
+                 const int* cum, const float* blurred, const float2* obs_g, float w, float4* pixg, int4* pnb,
+                 float* pcoef, double* obj, int max_area, hipStream_t s);
 int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const float* obs, float* tmp, float* blurred,
                        float2* obs_g, int max_area, hipStream_t s);
 
@@ -213,11 +261,11 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n,
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);
 int launch_jtj_f32(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
-                   const int* npix, const int* cum, const float4* pixg, const float* dI, float w, float* H, float* g,
-                   float* partials, int S, hipStream_t s);
+                   const int* npix, const int4* pnb, const float* pcoef, const float4* pixg, const float* dI, float w,
+                   float* H, float* g, float* partials, int S, hipStream_t s);
 int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
-                    const int* npix, const int* cum, const float4* pixg, const float* dI, float w, float* H, float* g,
-                    float* partials, int S, hipStream_t s);
+                    const int* npix, const int4* pnb, const float* pcoef, const float4* pixg, const float* dI, float w,
+                    float* H, float* g, float* partials, int S, int g_on_mfma, int two_term, hipStream_t s);
 
 // solve.hip
 int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask,

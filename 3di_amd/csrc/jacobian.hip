@@ -13,14 +13,17 @@
 //   cublasSgemm (derivative_computer.cu:159-164): H += w (Jx^T Jx + Jy^T Jy),
 //   g -= w (Jx^T g^s_x + Jy^T g^s_y).
 //
-// k_dI          : one wavefront per pixel, lanes over the K columns; the pixel's triangle,
-//                 barycentrics and camera-space point are wave-uniform, the basis rows of the
-//                 three vertices are read with unit stride across lanes.
+// k_dI          : four pixels per wavefront, 16 lanes over the K columns of each, so four
+//                 independent gather chains (triangle -> vertices -> basis rows) are in flight
+//                 per wave; basis rows are read with unit stride across the 16 lanes.
 // k_jtj_f32     : FP32 J^T J / J^T r from LDS-staged J rows, 4x4 register tiles (plain path).
-// k_jtj_mfma    : J staged in LDS as BF16 hi/lo, [J | r]^T [J | r] on
-//                 v_mfma_f32_16x16x16bf16_1k with FP32 accumulation: hi*hi + hi*lo + lo*hi.
+// k_jtj_mfma    : J staged in LDS as BF16 parts (hi/mid/lo), [J | r]^T [J | r] on
+//                 v_mfma_f32_16x16x16bf16_1k with FP32 accumulation (six products, or three
+//                 for the two-term split).
 //                 The residual r = g^s rides along as an extra column, so J^T r comes out of
-//                 the same MFMA tiles.
+//                 the same MFMA tiles; J^T r is also accumulated in FP32 while J is staged, and
+//                 the caller picks which one feeds the gradient (the GN fixed point depends on
+//                 g alone, so FP32 J^T r keeps batched trajectories on the FP64 ones).
 // k_jtj_reduce  : sums the per-split partial tiles, scales by w, writes a dense symmetric H.
 #include "common.h"
 
@@ -68,10 +71,13 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
     const long long base = fr.pix_off;
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
+    const int sub = lane >> 4, l16 = lane & 15;      // four pixels per wavefront, 16 lanes each
     const float* P0f = P0 + (size_t)f * m.V * 3;
     const float* Pf = P + (size_t)f * m.V * 3;
     const int K = L.K;
-    for (int pos = blockIdx.x * 4 + wave; pos < N; pos += gridDim.x * 4) {
+    for (int pos0 = (blockIdx.x * 4 + wave) * 4; pos0 < N; pos0 += gridDim.x * 16) {
+        const int pos = pos0 + sub;
+        if (pos >= N) continue;
         const int k = vis[base + pos];
         const int t = (int)(unsigned)(keys[base + k] & 0xffffffffull);
         const float2 ab = bary[base + k];
@@ -121,7 +127,7 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
             rn[2] = sr.R[2] * nx + sr.R[5] * ny + sr.R[8] * nz;
         }
         float* out = dI + (size_t)(base + pos) * K;
-        for (int j = lane; j < K; j += 64) {
+        for (int j = l16; j < K; j += 16) {
             float val;
             if (j >= L.off_exp && j < L.off_pose) {
                 const int c = j - L.off_exp;
@@ -161,24 +167,6 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
     }
 }
 
-// Compacted neighbour positions of sorted position `pos` (fill_krels, funcs.cu:288-320,
-// with cumM0 = inclusive occupancy count): left/right = last rendered pixel at or before
-// the horizontal neighbour, above/below = previous/next rendered pixel; the first and last
-// positions keep 0.  Returns (kl, kr, ka, kb).
-__device__ inline int4 pixel_nbrs(const int* __restrict__ cum, int k, long long base, int pos, int N, int W, int H)
-{
-    if (pos == 0 || pos == N - 1) return make_int4(0, 0, 0, 0);
-    const int x = k / H;
-    const int kleft = (x > 0) ? k - H : 0;
-    const int kright = (x < W - 1) ? k + H : 0;
-    const int cl = cum[base + kleft];
-    int kl = cl > 1 ? cl - 1 : 0;
-    if (kl >= N - 1) kl = N - 1;
-    int kr = cum[base + kright] - 1;
-    if (kr < 0) kr = 0;
-    return make_int4(kl, kr, pos - 1, pos + 1);
-}
-
 // Per-pixel coefficients for the normalised-gradient Jacobian.
 struct PixCoef {
     int4 nb;
@@ -186,7 +174,8 @@ struct PixCoef {
     int valid;
 };
 
-__device__ inline PixCoef pix_coef(const FrameGPU& fr, const int* cum, const int* vis, const float4* pixg, int pos,
+// The neighbour positions and c were computed once per pixel by k_grads (pnb, pcoef).
+__device__ inline PixCoef pix_coef(const FrameGPU& fr, const int4* pnb, const float* pcoef, const float4* pixg, int pos,
                                    int N)
 {
     PixCoef pc;
@@ -197,11 +186,9 @@ __device__ inline PixCoef pix_coef(const FrameGPU& fr, const int* cum, const int
         return pc;
     }
     const long long base = fr.pix_off;
-    const int k = vis[base + pos];
-    pc.nb = pixel_nbrs(cum, k, base, pos, N, fr.W, fr.H);
+    pc.nb = pnb[base + pos];
     const float4 gg = pixg[base + pos];
-    const float h = 1e-16f + sqrtf(4.f * gg.x * gg.x + 4.f * gg.y * gg.y);
-    pc.c = 1.f / (0.001f + h * h * h);
+    pc.c = pcoef[base + pos];
     pc.dx = 2.f * gg.x;
     pc.dy = 2.f * gg.y;
     pc.gsx = gg.z;
@@ -227,7 +214,8 @@ __device__ __forceinline__ int tri_index(int I, int J, int T) { return I * T - (
 
 __global__ void __launch_bounds__(256) k_jtj_f32(Layout L, const FrameGPU* __restrict__ frames,
                                                   const int* __restrict__ mask, const int* __restrict__ vis,
-                                                  const int* __restrict__ npix, const int* __restrict__ cum,
+                                                  const int* __restrict__ npix, const int4* __restrict__ pnb,
+                                                  const float* __restrict__ pcoef,
                                                   const float4* __restrict__ pixg, const float* __restrict__ dI,
                                                   int S, int KP, float* __restrict__ Hp, float* __restrict__ gp)
 {
@@ -260,7 +248,7 @@ __global__ void __launch_bounds__(256) k_jtj_f32(Layout L, const FrameGPU* __res
     for (int c0 = p0; c0 < p1; c0 += F32_CH) {
         if (tid < F32_CH) {
             const int pos = c0 + tid;
-            spc[tid] = pix_coef(fr, cum, vis, pixg, pos < p1 ? pos : N, N);
+            spc[tid] = pix_coef(fr, pnb, pcoef, pixg, pos < p1 ? pos : N, N);
             sgs[2 * tid] = spc[tid].gsx;
             sgs[2 * tid + 1] = spc[tid].gsy;
         }
@@ -340,8 +328,11 @@ __global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ mask
     }
 }
 
-// ------------------------------------------------------------------ BF16 hi/lo MFMA J^T J
-constexpr int MF_CH = 64;            // pixels per chunk -> 128 rows
+// ------------------------------------------------------------------ BF16 split MFMA J^T J
+// J is split into BF16 parts: x = hi + mid + lo (three terms, each rounded to nearest even),
+// and J^T J is accumulated in FP32 from the six products that matter down to 2^-24:
+// hh + hm + mh + hl + lh + mm.  With nsplit == 2 only hi + lo and hh + hl + lh are used.
+constexpr int MF_CH = 32;            // pixels per chunk -> 64 rows
 constexpr int MF_ROWS = 2 * MF_CH;
 constexpr int MF_STRIDE = MF_ROWS + 8; // bf16 elements per column in LDS (padding vs bank conflicts)
 
@@ -355,18 +346,15 @@ __device__ __forceinline__ unsigned short bf16_rne(float x)
     return (unsigned short)(u >> 16);
 }
 
-__device__ __forceinline__ void split_bf16(float x, unsigned short& hi, unsigned short& lo)
-{
-    hi = bf16_rne(x);
-    const float r = x - __uint_as_float((unsigned)hi << 16);
-    lo = bf16_rne(r);
-}
+__device__ __forceinline__ float bf16_f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
 __global__ void __launch_bounds__(256) k_jtj_mfma(Layout L, const FrameGPU* __restrict__ frames,
                                                    const int* __restrict__ mask, const int* __restrict__ vis,
-                                                   const int* __restrict__ npix, const int* __restrict__ cum,
+                                                   const int* __restrict__ npix, const int4* __restrict__ pnb,
+                                                   const float* __restrict__ pcoef,
                                                    const float4* __restrict__ pixg, const float* __restrict__ dI,
-                                                   int S, int KP, float* __restrict__ Hp)
+                                                   int S, int KP, int nsplit, float* __restrict__ Hp,
+                                                   float* __restrict__ gp)
 {
     const int f = blockIdx.y, s = blockIdx.x;
     if (mask && !mask[f]) return;
@@ -378,14 +366,16 @@ __global__ void __launch_bounds__(256) k_jtj_mfma(Layout L, const FrameGPU* __re
     const int p0 = (int)((long long)N * s / S), p1 = (int)((long long)N * (s + 1) / S);
     extern __shared__ unsigned short smem[];
     unsigned short* sJh = smem;                         // [KP][MF_STRIDE]
-    unsigned short* sJl = smem + KP * MF_STRIDE;        // [KP][MF_STRIDE]
+    unsigned short* sJm = smem + KP * MF_STRIDE;        // [KP][MF_STRIDE]
+    unsigned short* sJl = smem + 2 * KP * MF_STRIDE;    // [KP][MF_STRIDE] (three-term split only)
     __shared__ PixCoef spc[MF_CH];
     const float* dIf = dI + (size_t)fr.pix_off * K;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // wave w owns tiles w, w+4, ... (at most 9 for KP <= 128)
     constexpr int MAXT = 7;            // KP <= 112 -> at most 28 tiles, 7 per wave
     v4f acc[MAXT];
     int tI[MAXT], tJ[MAXT];
+    float gacc[2] = {0.f, 0.f};        // J^T r (FP32) for columns lane and lane + 64
+    __shared__ float sg[4][128];
 #pragma unroll
     for (int u = 0; u < MAXT; ++u) {
         const int tile = wave + 4 * u;
@@ -398,23 +388,43 @@ __global__ void __launch_bounds__(256) k_jtj_mfma(Layout L, const FrameGPU* __re
     for (int c0 = p0; c0 < p1; c0 += MF_CH) {
         if (tid < MF_CH) {
             const int pos = c0 + tid;
-            spc[tid] = pix_coef(fr, cum, vis, pixg, pos < p1 ? pos : N, N);
+            spc[tid] = pix_coef(fr, pnb, pcoef, pixg, pos < p1 ? pos : N, N);
         }
         __syncthreads();
-        for (int e = tid; e < MF_CH * KP; e += 256) {
-            const int j = e / MF_CH, p = e - j * MF_CH;   // p fastest: contiguous LDS stores per column
-            float jx = 0.f, jy = 0.f;
-            if (spc[p].valid) {
-                if (j < K) jrow(spc[p], dIf, K, j, jx, jy);
-                else if (j == K) { jx = spc[p].gsx; jy = spc[p].gsy; }
+        // staging: wave w takes pixels w, w+4, ...; lanes run over the columns, so the four
+        // neighbour rows of dI are read with unit stride
+#pragma unroll 2
+        for (int q = 0; q < MF_CH / 4; ++q) {
+            const int p = wave + 4 * q;
+            const PixCoef pc = spc[p];
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const int j = lane + 64 * hh;
+                if (j < KP) {
+                    float jx = 0.f, jy = 0.f;
+                    if (pc.valid) {
+                        if (j < K) {
+                            jrow(pc, dIf, K, j, jx, jy);
+                            gacc[hh] = fmaf(jx, pc.gsx, fmaf(jy, pc.gsy, gacc[hh]));   // J^T r in FP32
+                        } else if (j == K) {
+                            jx = pc.gsx;
+                            jy = pc.gsy;
+                        }
+                    }
+                    const unsigned short hx = bf16_rne(jx), hy = bf16_rne(jy);
+                    const float rx = jx - bf16_f(hx), ry = jy - bf16_f(hy);
+                    const unsigned short mx = bf16_rne(rx), my = bf16_rne(ry);
+                    const int o = j * MF_STRIDE + 2 * p;
+                    sJh[o] = hx;
+                    sJh[o + 1] = hy;
+                    sJm[o] = mx;
+                    sJm[o + 1] = my;
+                    if (nsplit == 3) {
+                        sJl[o] = bf16_rne(rx - bf16_f(mx));
+                        sJl[o + 1] = bf16_rne(ry - bf16_f(my));
+                    }
+                }
             }
-            unsigned short hx, lx, hy, ly;
-            split_bf16(jx, hx, lx);
-            split_bf16(jy, hy, ly);
-            sJh[j * MF_STRIDE + 2 * p] = hx;
-            sJh[j * MF_STRIDE + 2 * p + 1] = hy;
-            sJl[j * MF_STRIDE + 2 * p] = lx;
-            sJl[j * MF_STRIDE + 2 * p + 1] = ly;
         }
         __syncthreads();
         const int col = lane & 15, kq = 4 * (lane >> 4);
@@ -423,18 +433,28 @@ __global__ void __launch_bounds__(256) k_jtj_mfma(Layout L, const FrameGPU* __re
 #pragma unroll
             for (int u = 0; u < MAXT; ++u) {
                 if (wave + 4 * u >= ntiles) continue;
-                const int ca = tI[u] * 16 + col, cb = tJ[u] * 16 + col;
-                const v4s ah = *(const v4s*)(sJh + ca * MF_STRIDE + ks + kq);
-                const v4s al = *(const v4s*)(sJl + ca * MF_STRIDE + ks + kq);
-                const v4s bh = *(const v4s*)(sJh + cb * MF_STRIDE + ks + kq);
-                const v4s bl = *(const v4s*)(sJl + cb * MF_STRIDE + ks + kq);
+                const int ca = (tI[u] * 16 + col) * MF_STRIDE + ks + kq;
+                const int cb = (tJ[u] * 16 + col) * MF_STRIDE + ks + kq;
+                const v4s ah = *(const v4s*)(sJh + ca), am = *(const v4s*)(sJm + ca);
+                const v4s bh = *(const v4s*)(sJh + cb), bm = *(const v4s*)(sJm + cb);
                 acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[u], 0, 0, 0);
-                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[u], 0, 0, 0);
-                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bm, acc[u], 0, 0, 0);
+                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, acc[u], 0, 0, 0);
+                if (nsplit == 3) {
+                    const v4s al = *(const v4s*)(sJl + ca), bl = *(const v4s*)(sJl + cb);
+                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[u], 0, 0, 0);
+                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[u], 0, 0, 0);
+                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm, acc[u], 0, 0, 0);
+                }
             }
         }
         __syncthreads();
     }
+    // J^T r partials: sum the four waves' column sums
+    sg[wave][lane] = gacc[0];
+    sg[wave][lane + 64] = gacc[1];
+    __syncthreads();
+    if (tid < KP) gp[((size_t)f * S + s) * KP + tid] = sg[0][tid] + sg[1][tid] + sg[2][tid] + sg[3][tid];
     // accumulator layout of 16x16x16: lane l, element r -> C[4*(l/16) + r][l%16]
     float* Hout = Hp + ((size_t)f * S + s) * ntiles * 256;
 #pragma unroll
@@ -470,7 +490,7 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n,
         set_error("dI: k_tex %d > 512", L.ntex);
         return TDI_ERR_ARG;
     }
-    int bx = (max_area / 2 + 3) / 4;
+    int bx = (max_area / 2 + 15) / 16;
     if (bx > 2048) bx = 2048;
     if (bx < 1) bx = 1;
     hipLaunchKernelGGL(k_dI, dim3(bx, n), dim3(256), 0, s, m, L, frames, mask, theta, P0, P, vis_list, npix,
@@ -484,7 +504,7 @@ size_t jtj_partials_floats(int n, int S, int K, bool mfma)
     if (mfma) {
         const int KP = ((K + 1 + 15) / 16) * 16;
         const int T = KP / 16;
-        return (size_t)n * S * (T * (T + 1) / 2) * 256;
+        return (size_t)n * S * (T * (T + 1) / 2) * 256 + (size_t)n * S * KP;
     }
     const int KP = ((K + 3) / 4) * 4;
     const int T = KP / 4;
@@ -492,8 +512,8 @@ size_t jtj_partials_floats(int n, int S, int K, bool mfma)
 }
 
 int launch_jtj_f32(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
-                   const int* npix, const int* cum, const float4* pixg, const float* dI, float w, float* H, float* g,
-                   float* partials, int S, hipStream_t s)
+                   const int* npix, const int4* pnb, const float* pcoef, const float4* pixg, const float* dI, float w,
+                   float* H, float* g, float* partials, int S, hipStream_t s)
 {
     const int K = L.K;
     const int KP = ((K + 3) / 4) * 4;
@@ -504,8 +524,8 @@ int launch_jtj_f32(const Layout& L, const FrameGPU* frames, int n, const int* ma
     const int T = KP / 4;
     float* Hp = partials;
     float* gp = partials + (size_t)n * S * (T * (T + 1) / 2) * 16;
-    hipLaunchKernelGGL(k_jtj_f32, dim3(S, n), dim3(256), 0, s, L, frames, mask, vis_list, npix, cum, pixg, dI, S, KP, Hp,
-                       gp);
+    hipLaunchKernelGGL(k_jtj_f32, dim3(S, n), dim3(256), 0, s, L, frames, mask, vis_list, npix, pnb, pcoef, pixg, dI, S,
+                       KP, Hp, gp);
     int bx = (K * K + K + 255) / 256;
     hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, n), dim3(256), 0, s, mask, K, KP, 4, S, Hp, gp, w, H, g, 0);
     TDI_HIP(hipGetLastError());
@@ -513,8 +533,8 @@ int launch_jtj_f32(const Layout& L, const FrameGPU* frames, int n, const int* ma
 }
 
 int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
-                    const int* npix, const int* cum, const float4* pixg, const float* dI, float w, float* H, float* g,
-                    float* partials, int S, hipStream_t s)
+                    const int* npix, const int4* pnb, const float* pcoef, const float4* pixg, const float* dI, float w,
+                    float* H, float* g, float* partials, int S, int g_on_mfma, int two_term, hipStream_t s)
 {
     const int K = L.K;
     const int KP = ((K + 1 + 15) / 16) * 16;
@@ -522,11 +542,15 @@ int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, int n, const int* m
         set_error("jtj_mfma: K=%d too large (max 111)", K);
         return TDI_ERR_ARG;
     }
-    const size_t lds = (size_t)2 * KP * MF_STRIDE * sizeof(unsigned short);
-    hipLaunchKernelGGL(k_jtj_mfma, dim3(S, n), dim3(256), lds, s, L, frames, mask, vis_list, npix, cum, pixg, dI, S, KP,
-                       partials);
+    const int nsplit = two_term ? 2 : 3;
+    const size_t lds = (size_t)nsplit * KP * MF_STRIDE * sizeof(unsigned short);
+    const int T = KP / 16;
+    float* gp = partials + (size_t)n * S * (T * (T + 1) / 2) * 256;
+    hipLaunchKernelGGL(k_jtj_mfma, dim3(S, n), dim3(256), lds, s, L, frames, mask, vis_list, npix, pnb, pcoef, pixg, dI,
+                       S, KP, nsplit, partials, gp);
     int bx = (K * K + K + 255) / 256;
-    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, n), dim3(256), 0, s, mask, K, KP, 16, S, partials, nullptr, w, H, g, 1);
+    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, n), dim3(256), 0, s, mask, K, KP, 16, S, partials, gp, w, H, g,
+                       g_on_mfma);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

@@ -14,8 +14,9 @@
 //               The winner per pixel is therefore unique and deterministic: minimum depth,
 //               then lowest triangle id (the reference keeps every exact-depth tie and lets
 //               them race in the scatter, funcs.cu:816,1027).
-//   k_resolve — one workgroup per frame walks the crop in column-major order; a wavefront
-//               ballot + popcount gives each occupied pixel its rank, which is at once the
+//   k_resolve — one workgroup per frame walks the crop in column-major order, 4 pixels per
+//               lane; three wavefront ballots (one per bit of the lane's 0..4 count) and
+//               popcounts give each occupied pixel its rank, which is at once the
 //               inclusive occupancy prefix sum (the reference's cumM0), the position in the
 //               sorted visible-pixel list (the reference's sort_by_key output) and the slot
 //               of its barycentric weights.  No sort is needed because the crop is walked in
@@ -68,12 +69,17 @@ __global__ void __launch_bounds__(256) k_raster(const int* __restrict__ tris, in
     }
 }
 
-__global__ void __launch_bounds__(256) k_resolve(const int* __restrict__ tris, int V,
-                                                  const FrameGPU* __restrict__ frames, const int* __restrict__ mask,
-                                                  const float4* __restrict__ vproj,
-                                                  const unsigned long long* __restrict__ keys,
-                                                  int* __restrict__ cum, int* __restrict__ vis,
-                                                  int* __restrict__ npix, float2* __restrict__ bary)
+constexpr int RES_THREADS = 1024;
+constexpr int RES_PER_THREAD = 4;
+constexpr int RES_TILE = RES_THREADS * RES_PER_THREAD;
+
+__global__ void __launch_bounds__(RES_THREADS) k_resolve(const int* __restrict__ tris, int V,
+                                                          const FrameGPU* __restrict__ frames,
+                                                          const int* __restrict__ mask,
+                                                          const float4* __restrict__ vproj,
+                                                          const unsigned long long* __restrict__ keys,
+                                                          int* __restrict__ cum, int* __restrict__ vis,
+                                                          int* __restrict__ npix, float2* __restrict__ bary)
 {
     const int f = blockIdx.x;
     if (mask && !mask[f]) return;
@@ -82,34 +88,45 @@ __global__ void __launch_bounds__(256) k_resolve(const int* __restrict__ tris, i
     const long long base = fr.pix_off;
     const int A = fr.W * fr.H;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    __shared__ int swave[4];
+    constexpr int NW = RES_THREADS / 64;
+    __shared__ int swave[NW];
     int running = 0;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int tile = 0; tile < A; tile += 256) {
-        const int k = tile + tid;
-        unsigned long long key = KEY_EMPTY;
-        if (k < A) key = keys[base + k];
-        const bool occ = key != KEY_EMPTY;
-        const unsigned long long bal = __ballot(occ);
-        const int before = __popcll(bal & lt);
-        if (lane == 0) swave[wave] = __popcll(bal);
+    for (int tile = 0; tile < A; tile += RES_TILE) {
+        const int k0 = tile + RES_PER_THREAD * tid;     // this thread's 4 consecutive pixels
+        unsigned long long key[RES_PER_THREAD];
+        int cnt = 0;
+#pragma unroll
+        for (int q = 0; q < RES_PER_THREAD; ++q) {
+            key[q] = (k0 + q < A) ? keys[base + k0 + q] : KEY_EMPTY;
+            cnt += key[q] != KEY_EMPTY;
+        }
+        // wave prefix of the per-thread counts (0..4) from three ballots, one per count bit
+        const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
+        const int before = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
+        if (lane == 0) swave[wave] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
         __syncthreads();
         int woff = 0, total = 0;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
+        for (int w = 0; w < NW; ++w) {
             woff += (w < wave) ? swave[w] : 0;
             total += swave[w];
         }
-        const int pos = running + woff + before;
-        if (k < A) {
+        int pos = running + woff + before;
+#pragma unroll
+        for (int q = 0; q < RES_PER_THREAD; ++q) {
+            const int k = k0 + q;
+            if (k >= A) break;
+            const bool occ = key[q] != KEY_EMPTY;
             cum[base + k] = pos + (occ ? 1 : 0);
             if (occ) {
                 vis[base + pos] = k;
-                const int t = (int)(unsigned)(key & 0xffffffffull);
+                const int t = (int)(unsigned)(key[q] & 0xffffffffull);
                 const float4 B = vp[tris[3 * t]], Cc = vp[tris[3 * t + 1]], Av = vp[tris[3 * t + 2]];
                 float al, be;
                 cover_rn(B.x, B.y, Cc.x, Cc.y, Av.x, Av.y, fr.x0 + k / fr.H, fr.y0 + k % fr.H, &al, &be);
                 bary[base + k] = make_float2(al, be);
+                ++pos;
             }
         }
         running += total;
@@ -165,7 +182,7 @@ int launch_resolve(const ModelGPU& m, const FrameGPU* frames, int n, const int* 
                    const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int /*max_area*/,
                    hipStream_t s)
 {
-    hipLaunchKernelGGL(k_resolve, dim3(n), dim3(256), 0, s, m.tris, m.V, frames, mask, vproj,
+    hipLaunchKernelGGL(k_resolve, dim3(n), dim3(RES_THREADS), 0, s, m.tris, m.V, frames, mask, vproj,
                        (const unsigned long long*)keys, cum, vis_list, npix, (float2*)bary);
     TDI_HIP(hipGetLastError());
     return TDI_OK;

@@ -131,11 +131,14 @@ __device__ __forceinline__ void nbr_index(int k, int W, int H, int& kr, int& kl,
 }
 
 // Gradients of the blurred rendered crop at every visible pixel, the objective
-// -w sum g^_r.g^_s accumulated in FP64, and the per-pixel record (gx, gy, g^s_x, g^s_y).
+// -w sum g^_r.g^_s accumulated in FP64, and the per-pixel records (gx, gy, g^s_x, g^s_y),
+// neighbour positions and Jacobian coefficient used by the normal equations.
 __global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ frames, const int* __restrict__ mask,
                                                 const int* __restrict__ vis, const int* __restrict__ npix,
+                                                const int* __restrict__ cum,
                                                 const float* __restrict__ blurred, const float2* __restrict__ obs_g,
-                                                float w, float4* __restrict__ pixg, double* __restrict__ obj)
+                                                float w, float4* __restrict__ pixg, int4* __restrict__ pnb,
+                                                float* __restrict__ pcoef, double* __restrict__ obj)
 {
     const int f = blockIdx.y;
     if (mask && !mask[f]) return;
@@ -154,6 +157,10 @@ __global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ fram
         const float2 gs = obs_g[fr.obs_off + k];
         acc += (double)(2.f * gx / h) * gs.x + (double)(2.f * gy / h) * gs.y;
         pixg[base + pos] = make_float4(gx, gy, gs.x, gs.y);
+        // per-pixel record for the normalised-gradient Jacobian: compacted neighbour
+        // positions and c = 1 / (0.001 + h^3) (newfuncs.cu:446)
+        pnb[base + pos] = pixel_nbrs(cum, k, base, pos, N, fr.W, fr.H);
+        pcoef[base + pos] = 1.f / (0.001f + h * h * h);
     }
     // block reduction (wave shuffles, then LDS)
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
@@ -222,11 +229,11 @@ int launch_blur(const FrameGPU* frames, int n, const int* mask, const float* tap
 }
 
 int launch_grads(const FrameGPU* frames, int n, const int* mask, const int* vis_list, const int* npix,
-                 const float* blurred, const float2* obs_g, float w, float4* pixg, double* obj, int max_area,
-                 hipStream_t s)
+                 const int* cum, const float* blurred, const float2* obs_g, float w, float4* pixg, int4* pnb,
+                 float* pcoef, double* obj, int max_area, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_grads, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, vis_list, npix, blurred, obs_g,
-                       w, pixg, obj);
+    hipLaunchKernelGGL(k_grads, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, vis_list, npix, cum, blurred,
+                       obs_g, w, pixg, pnb, pcoef, obj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

@@ -213,9 +213,9 @@ __device__ void chol_solve_packed(const T* A, int K, T* b)
 
 constexpr int CMAXK = 112;
 
-// mode 0: FP64 factorisation and solve.  mode 1: FP32 factorisation and solve, then one
-// FP64 residual-refinement step (r = -g - H x computed in FP64, correction solved with the
-// FP32 factor).
+// mode 0: FP64 factorisation and solve.  mode 1: FP32 factorisation of the equilibrated
+// matrix and solve, then one FP64 residual-refinement step (r = -g - H x computed in FP64,
+// correction solved with the FP32 factor).
 __global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict__ H, const float* __restrict__ g,
                                                    float* __restrict__ delta, int* __restrict__ status,
                                                    const int* __restrict__ mask, int mode)
@@ -251,29 +251,40 @@ __global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict
         for (int i = tid; i < K; i += blockDim.x) delta[(size_t)f * K + i] = (float)sb[i];
         if (tid == 0) status[f] = TDI_OK;
     } else {
+        // FP32 factorisation of the equilibrated matrix D^-1/2 H D^-1/2 (D = diag H), which
+        // removes the scale spread between pose and coefficient columns, then one FP64
+        // residual-refinement step on the original system.
         float* A = (float*)dsm;
+        __shared__ float sdinv[CMAXK];
+        for (int i = tid; i < K; i += blockDim.x) {
+            const float d = Hf[(size_t)i * K + i];
+            sdinv[i] = d > 0.f ? rsqrtf(d) : 1.f;
+        }
+        __syncthreads();
         for (int e = tid; e < np; e += blockDim.x) {
             int i = (int)((sqrtf(8.f * e + 1.f) - 1.f) * 0.5f);
             while ((i + 1) * (i + 2) / 2 <= e) ++i;
             while (i * (i + 1) / 2 > e) --i;
             const int j = e - i * (i + 1) / 2;
-            A[e] = Hf[(size_t)i * K + j];
+            A[e] = Hf[(size_t)i * K + j] * sdinv[i] * sdinv[j];
         }
-        for (int i = tid; i < K; i += blockDim.x) sbf[i] = -gf[i];
+        for (int i = tid; i < K; i += blockDim.x) sbf[i] = -gf[i] * sdinv[i];
         __syncthreads();
         if (!chol_packed<float>(A, K, sflagf)) {
             if (tid == 0) status[f] = TDI_NOT_PD;
             return;
         }
         chol_solve_packed<float>(A, K, sbf);
-        // refinement: r = -g - H x0 in FP64
+        // x0 = D^-1/2 y
+        for (int i = tid; i < K; i += blockDim.x) sbf[i] *= sdinv[i];
+        __syncthreads();
+        // refinement: r = -g - H x0 in FP64, correction solved with the same scaled factor
         for (int i = tid; i < K; i += blockDim.x) {
             double r = -(double)gf[i];
             for (int k = 0; k < K; ++k) r -= (double)Hf[(size_t)i * K + k] * (double)sbf[k];
-            sb[i] = r;
+            sb[i] = r * (double)sdinv[i];
         }
         __syncthreads();
-        // correction with the FP32 factor, carried in FP64 between the triangular solves
         for (int i = 0; i < K; ++i) {
             if (tid == 0) sb[i] /= (double)A[pk(i, i)];
             __syncthreads();
@@ -288,7 +299,8 @@ __global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict
             for (int k = tid; k < i; k += blockDim.x) sb[k] -= (double)A[pk(i, k)] * xi;
             __syncthreads();
         }
-        for (int i = tid; i < K; i += blockDim.x) delta[(size_t)f * K + i] = (float)((double)sbf[i] + sb[i]);
+        for (int i = tid; i < K; i += blockDim.x)
+            delta[(size_t)f * K + i] = (float)((double)sbf[i] + sb[i] * (double)sdinv[i]);
         if (tid == 0) status[f] = TDI_OK;
     }
 }

@@ -6,9 +6,14 @@ array layout and statistics family:
 
 * mesh: an ``G x G`` vertex grid lifted onto an ellipsoid cap with semi-axes
   (80, 100, 60) mm, two triangles per grid cell, nose towards the camera (-z);
-* bases: each column is seeded Gaussian noise over the grid, Gaussian-blurred with
-  sigma = 5 grid cells, normalised to unit RMS and scaled by ``s0 * k**-0.7``
-  (s0 = 5 mm for shape, 0.1 for albedo), so coefficients are unit-variance;
+* bases: each column is seeded Gaussian noise over the grid, Gaussian-blurred, normalised
+  to unit RMS and scaled by ``s0 * k**-0.7`` (s0 = 5 mm for shape, 0.1 for albedo), so
+  coefficients are unit-variance.  Albedo columns are blurred with sigma = G/20 cells
+  (5 cells on the 100x100 grid).  Shape (identity/expression) columns are blurred with
+  sigma = G/4 cells: at 5 mm
+  amplitude a 5-cell blur folds the surface (36% of the triangles of a 100x100 grid flip
+  orientation in the image, and more at 231x231), whereas a face-scale blur leaves ~2%
+  flipped, at the silhouette only;
 * truths: coefficients ~ N(0, 1) clipped to +-3, bounds at +-3.5 so truths stay
   strictly feasible for the log barrier;
 * 51 landmark vertices on a fixed face-like pattern; indices 0, 9 and 40 play the
@@ -164,12 +169,16 @@ def _smooth_fields(rng, G: int, n: int, sigma: float) -> np.ndarray:
     return out
 
 
+SHAPE_BLUR_FRACTION = 0.25   # blur sigma of shape basis columns, in units of the grid size
+TEX_BLUR_FRACTION = 0.05     # blur sigma of albedo basis columns, in units of the grid size
+
+
 def _shape_basis(rng, G: int, K: int, s0: float) -> np.ndarray:
     V = G * G
     scale = s0 * np.arange(1, K + 1, dtype=np.float64) ** -0.7
     B = np.empty((V, 3, K), np.float64)
     for axis in range(3):
-        B[:, axis, :] = _smooth_fields(rng, G, K, 5.0) * scale
+        B[:, axis, :] = _smooth_fields(rng, G, K, SHAPE_BLUR_FRACTION * G) * scale
     return B.reshape(3 * V, K).astype(np.float32)
 
 
@@ -182,11 +191,12 @@ def make_model(grid: int, k_id: int, k_tex: int, k_exp: int, seed: int) -> Model
     id_basis = _shape_basis(rng, grid, k_id, 5.0) if k_id else None
     tex_basis = None
     if k_tex:
-        tex_basis = (_smooth_fields(rng, grid, k_tex, 5.0)
+        tex_basis = (_smooth_fields(rng, grid, k_tex, TEX_BLUR_FRACTION * grid)
                      * (0.1 * np.arange(1, k_tex + 1, dtype=np.float64) ** -0.7)).astype(np.float32)
-    # albedo mean: skin-like level with structure at a few-cell scale so that the
-    # normalised-gradient objective has edges to follow
-    tex = 0.55 + 0.18 * _smooth_fields(rng, grid, 1, 2.0)[:, 0] + 0.08 * _smooth_fields(rng, grid, 1, 6.0)[:, 0]
+    # albedo mean: skin-like level with structure at 1/40 and 1/12 of the face, so that the
+    # normalised-gradient objective has edges to follow at every crop size
+    tex = (0.55 + 0.18 * _smooth_fields(rng, grid, 1, grid / 40.0)[:, 0]
+           + 0.08 * _smooth_fields(rng, grid, 1, grid / 12.0)[:, 0])
     tex = np.clip(tex, 0.15, 0.95).astype(np.float32)
     bnd = np.float32(3.5)
     m = Model(grid=grid, mean_shape=P.astype(np.float32), exp_basis=exp_basis, tex_mean=tex,
@@ -255,7 +265,8 @@ class Truth:
 
 def draw_truth(m: Model, rng, lay: dict, pose_deg: float, tz_range=(400.0, 600.0),
                fit_light: bool = False, exp_scale: float = 1.0,
-               exp_coeffs: Optional[np.ndarray] = None, pose: Optional[np.ndarray] = None) -> Truth:
+               exp_coeffs: Optional[np.ndarray] = None, pose: Optional[np.ndarray] = None,
+               coef_pert: float = 0.05) -> Truth:
     K = lay["K"]
     th = np.zeros(K)
     if lay["exp"] < lay["pose"]:
@@ -278,12 +289,14 @@ def draw_truth(m: Model, rng, lay: dict, pose_deg: float, tz_range=(400.0, 600.0
         th[lay["light"] + 3] = Lint
     init = th.copy()
     # start from a perturbed expression/identity, a perturbed pose and the reference's
-    # initial light; the caller makes it landmark-feasible with make_feasible()
+    # initial light, at roughly the one-pixel accuracy of the reference's landmark-driven
+    # initialisation (video_fitter.cu:479-485); the caller makes it landmark-feasible with
+    # make_feasible()
     nb = lay["pose"]
-    init[:nb] = np.clip(th[:nb] + rng.normal(0, 0.5, nb), -3.2, 3.2)
+    init[:nb] = np.clip(th[:nb] + rng.normal(0, coef_pert, nb), -3.2, 3.2)
     p = lay["pose"]
-    init[p:p + 3] += rng.normal(0, [1.5, 1.5, 4.0])
-    init[p + 3:p + 6] += rng.normal(0, np.deg2rad(1.0), 3)
+    init[p:p + 3] += rng.normal(0, [0.3, 0.3, 1.0])
+    init[p + 3:p + 6] += rng.normal(0, np.deg2rad(0.2), 3)
     if fit_light:
         init[lay["light"]:lay["light"] + 3] = LAMBDA0
         init[lay["light"] + 3] = L0

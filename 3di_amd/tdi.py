@@ -52,6 +52,9 @@ def lib():
         L.tdi_fit_plain.argtypes = [_vp, _vp, _vp]
         L.tdi_fit_batched.argtypes = [_vp, _vp, _vp, _vp]
         L.tdi_blur_taps.argtypes = [C.c_float, _vp]
+        L.tdi_stage_times.argtypes = [_vp, _vp, C.c_int32, C.c_int32, _vp]
+        L.tdi_last_pixel_counts.argtypes = [_vp, _vp]
+        L.tdi_workspace_frames.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -185,6 +188,21 @@ class Batch:
         fn = lib().tdi_normal_equations_mixed if mixed else lib().tdi_normal_equations
         _check(fn(self.h, _dptr(th), _dptr(H), _dptr(g), _dptr(obj), _stream()))
         return H, g, obj
+
+    def stage_times(self, theta, mixed_flags: int = 7, reps: int = 3) -> np.ndarray:
+        """Mean ms per stage of one evaluation + solve (see tdi_stage_times)."""
+        th = self._theta(theta)
+        ms = np.zeros(7, np.float32)
+        _check(lib().tdi_stage_times(self.h, _dptr(th), mixed_flags, reps, ms.ctypes.data))
+        return ms
+
+    def workspace_frames(self) -> int:
+        return lib().tdi_workspace_frames(self.h)
+
+    def pixel_counts(self) -> np.ndarray:
+        out = np.zeros(min(self.n, self.workspace_frames()), np.int32)
+        _check(lib().tdi_last_pixel_counts(self.h, out.ctypes.data))
+        return out
 
     # -------------------------------------------------------------- fits
     def fit_plain(self, theta0) -> Tuple[np.ndarray, List]:

@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched mixed-precision per-frame 3DMM fit on one MI355X.
+
+One step = a complete fit (12 Gauss-Newton iterations with Armijo line search) of every
+frame of the cfg2 workload (4096 synthetic frames, crops U{96..384}, 89 unknowns per frame),
+inputs resident in HBM, through the batched path (device-resident loop in a hipGraph).
+
+Also measured in the same run:
+  * the plain path (one frame at a time, host-driven loop, FP32 J^T J, FP64 Cholesky) on a
+    prefix of the same frames -> frames/s and the speed-up of the batched path;
+  * the FP64 OpenMP oracle on a prefix of the same frames on the host cores (cpu_baseline),
+    which also gives the batched path's parameter error against FP64;
+  * per-stage times with HIP events on the first workspace chunk -> the dominant kernel's
+    roofline entry.
+
+Usage: python bench.py [--steps K] [--warmup W] [--frames N] [--gpus 1]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "frames/s per-frame fit, batched+mixed vs plain, at <=1e-4 param RMS vs FP64"
+PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"bf16": 2500.0, "fp64": 78.6, "fp32": 157.3}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def stage_work(w, K, npix, n):
+    """Algorithmic work of each timed stage over the first workspace chunk.
+
+    Returns a list of (name, bound, amount, unit) with amount in bytes or FLOPs."""
+    m = w.model
+    V, NT, Ke = m.n_vertices, m.n_triangles, m.k_exp
+    A = np.array([f.width * f.height for f in w.frames[:n]], np.float64)
+    P = npix[:n].astype(np.float64)
+    synth_flops = 2.0 * 3 * V * Ke * n
+    raster_bytes = float(np.sum(8 * A + 8 * A + 4 * A + NT * (12 + 3 * 16) + 8 * P + 12 * P))
+    shade_bytes = float(np.sum(4 * A + P * (4 + 8 + 8 + 3 * 12 + 3 * 4 + 4) + 16 * A + P * (4 + 4 * 4 + 8 + 16)))
+    barrier_flops = 2.0 * 51 * 3 * Ke * n
+    dI_bytes = float(np.sum(P * (4 + 8 + 8 + 3 * 24 + 16 + 9 * Ke * 4 + 4 * K)))
+    jtj_flops = float(np.sum(4.0 * K * K * P))
+    chol_flops = n * K ** 3 / 3.0
+    return [("synthesis", "mfma", synth_flops, "fp64"), ("raster", "hbm", raster_bytes, None),
+            ("shade+blur+grads", "hbm", shade_bytes, None), ("barrier", "mfma", barrier_flops, "fp32"),
+            ("jacobian dI", "hbm", dI_bytes, None), ("JtJ (BF16 split MFMA)", "mfma", jtj_flops, "bf16"),
+            ("cholesky", "mfma", chol_flops, "fp32")]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=4096)
+    ap.add_argument("--plain-frames", type=int, default=64)
+    ap.add_argument("--cpu-frames", type=int, default=32)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the fitting step shards by frames with no exchange, "
+              "so --gpus N > 1 is not run here (see DESIGN.md, 'More than one GPU').", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    tdi = importlib.import_module("3di_amd.tdi")
+    workloads = importlib.import_module("3di_amd.workloads")
+
+    w = workloads.cfg2(n_frames=args.frames, log=log)
+    K = w.lay["K"]
+    init = w.init()
+
+    ctx = tdi.Context(w.model, w.opts)
+    batch = tdi.Batch(ctx, w.frames)
+    th_init = torch.from_numpy(init).cuda()
+    th = th_init.clone()
+    log(f"[bench] workspace chunk = {batch.workspace_frames()} frames")
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        th.copy_(th_init)
+        _, reps = batch.fit_batched(th)
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i}: {time.perf_counter() - t0:.2f}s")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        th.copy_(th_init)
+        _, reps = batch.fit_batched(th)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = 1e3 * dt / args.steps
+    fps = args.frames * args.steps / dt
+    log(f"[bench] batched: {ms_per_step:.1f} ms/step, {fps:.1f} frames/s")
+    th_batched = th.cpu().numpy().astype(np.float64)
+    iters = np.array([r.iters for r in reps])
+
+    # plain path on a prefix of the same frames
+    plain = None
+    if not args.no_plain and args.plain_frames > 0:
+        npf = min(args.plain_frames, args.frames)
+        ctxp = tdi.Context(w.model, w.opts)
+        bp = tdi.Batch(ctxp, w.frames[:npf])
+        t0 = time.perf_counter()
+        th_plain, rp = bp.fit_plain(init[:npf])
+        dtp = time.perf_counter() - t0
+        plain = dict(frames=npf, seconds=dtp, fps=npf / dtp)
+        log(f"[bench] plain: {npf} frames in {dtp:.2f}s = {npf / dtp:.2f} frames/s")
+        bp.close()
+        ctxp.close()
+
+    # stage times on the first chunk (live HIP events on the batch's stream)
+    n_chunk = min(args.frames, batch.workspace_frames())
+    ms = batch.stage_times(th_init, mixed_flags=w.opts.mixed_flags, reps=3)
+    npix = batch.pixel_counts()
+    work = stage_work(w, K, npix, n_chunk)
+    stages = []
+    for (name, bound, amount, prec), t in zip(work, ms):
+        if bound == "hbm":
+            ach = amount / (t * 1e-3) / 1e9
+            peak = PEAK_HBM_GBS
+            unit = "GB/s"
+        else:
+            ach = amount / (t * 1e-3) / 1e12
+            peak = PEAK_TFLOPS[prec]
+            unit = "TFLOP/s"
+        stages.append(dict(stage=name, ms=round(float(t), 4), bound=bound, achieved=round(float(ach), 3), peak=peak,
+                           unit=unit, frac=round(float(ach / peak), 5), frames=int(n_chunk)))
+        log(f"[bench] stage {name:24s} {t:8.3f} ms  {ach:9.2f} {unit} ({100 * ach / peak:.1f}% of peak)")
+    dom = max(stages, key=lambda s: s["ms"])
+    roofline = dict(kernel=dom["stage"], bound=dom["bound"], achieved=dom["achieved"], peak=dom["peak"],
+                    unit=dom["unit"], frac=dom["frac"], traffic=None,
+                    launch_ms=dom["ms"], frames_per_launch=int(n_chunk))
+
+    # FP64 oracle on a prefix: host baseline and parameter error of the batched fit
+    cpu = None
+    accuracy = None
+    if not args.no_cpu and args.cpu_frames > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        nc = min(args.cpu_frames, args.frames)
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        sc = O.Scene(w.model, w.frames[:nc], w.opts)
+        t0 = time.perf_counter()
+        th_o, ro = sc.fit(init[:nc].astype(np.float64), n_threads=threads)
+        dtc = time.perf_counter() - t0
+        cpu = dict(value=round(nc / dtc, 3), unit="frames/s", cores=threads, kind="port",
+                   sample=f"first {nc} frames of the workload, FP64 OpenMP oracle, one frame per thread, "
+                          f"{w.opts.max_iters} GN iterations, {dtc:.1f} s")
+        sl = slice(w.lay["exp"], w.lay["pose"] + 6)
+        d = th_batched[:nc, sl] - th_o[:, sl]
+        per = np.sqrt(np.mean(d * d, axis=1))
+        accuracy = dict(frames=nc, rms_pose_expr=float(np.sqrt(np.mean(d * d))),
+                        median_frame_rms=float(np.median(per)),
+                        frames_within_1e4=int(np.sum(per <= 1e-4)))
+        if plain is not None:
+            npc = min(nc, plain["frames"])
+            dp = th_plain[:npc, sl].astype(np.float64) - th_o[:npc, sl]
+            perp = np.sqrt(np.mean(dp * dp, axis=1))
+            accuracy["plain_rms_pose_expr"] = float(np.sqrt(np.mean(dp * dp)))
+            accuracy["plain_frames_within_1e4"] = int(np.sum(perp <= 1e-4))
+            accuracy["plain_frames"] = int(npc)
+        stops = {}
+        for r in ro:
+            stops[int(r.stop)] = stops.get(int(r.stop), 0) + 1
+        accuracy["oracle_stop_flags"] = stops
+        log(f"[bench] oracle: {nc} frames in {dtc:.1f}s on {threads} threads; batched vs FP64 RMS "
+            f"{accuracy['rms_pose_expr']:.3e}, median per-frame {accuracy['median_frame_rms']:.2e}, "
+            f"{accuracy['frames_within_1e4']}/{nc} frames within 1e-4")
+
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "frames": args.frames, "unknowns_per_frame": K,
+                   "gn_iterations": int(w.opts.max_iters), "mean_iterations_used": float(iters.mean()),
+                   "precision": "FP64 vertex synthesis (f64 MFMA), BF16 hi/mid/lo J^T J on MFMA with FP32 "
+                                "accumulation, FP32 J^T r, equilibrated FP32 Cholesky + 1 FP64 refinement"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+        "speedup_vs_plain": (round(fps / plain["fps"], 2) if plain else None),
+        "batched_stop_flags": {str(k): int(v) for k, v in zip(*np.unique([r.stop for r in reps], return_counts=True))},
+        "accuracy_vs_fp64": accuracy,
+        "stages": stages,
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -101,7 +101,18 @@ typedef struct tdi_fit_options {
     float decrement_tol;        /* 3e-4 (model_fitter.cu:341)                                      */
     float blur_sigma;           /* KERNEL_SIGMA config key (config.cu:53; cfg files use 0.75)      */
     float ref_face_size;        /* REF_FACE_SIZE; "face too large" if > 2x (renderer.cu:845-857), 0 = off */
+    int32_t mixed_flags;        /* batched path arithmetic (no reference counterpart), bit set:
+                                   1 = FP64-MFMA basis GEMM over frames, 2 = BF16 hi/lo MFMA J^T J,
+                                   4 = FP32 Cholesky + FP64 refinement, 8 = J^T r from the BF16
+                                   MFMA tiles (else FP32), 16 = two-term hi/lo split of J
+                                   (else three-term hi/mid/lo).  Default 7.               */
 } tdi_fit_options;
+
+#define TDI_MIXED_SYNTH_MFMA 1
+#define TDI_MIXED_JTJ_BF16 2
+#define TDI_MIXED_CHOL_F32 4
+#define TDI_MIXED_JTR_BF16 8
+#define TDI_MIXED_TWO_TERM 16
 
 /* Per-frame fit report. */
 typedef struct tdi_fit_report {
@@ -176,6 +187,18 @@ int tdi_normal_equations_mixed(tdi_batch* b, const float* theta, float* H, float
  * residual-refinement step.  status: [n] tdi_status. */
 int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, float* delta,
                        int32_t* status, int32_t mode, void* stream);
+
+/* Frames per workspace chunk (the batch is processed in chunks of this many frames). */
+int tdi_workspace_frames(tdi_batch* b);
+
+/* Stage timers (HIP events on the batch's stream) for one full evaluation + solve of the
+ * first workspace chunk, mean over `reps`: ms[0] synthesis, [1] raster, [2] shade+blur+gradients, [3] barrier objective,
+ * [4] per-pixel Jacobian, [5] J^T J / J^T r (+ barrier derivatives), [6] Cholesky.
+ * Mirrors the reference's MEASURE_TIME stage timers (model_fitter.cu:43-58). */
+int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed_flags, int32_t reps, float ms[7]);
+
+/* Rendered pixel counts of the batch's last evaluation (host array of n). */
+int tdi_last_pixel_counts(tdi_batch* b, int32_t* npix);
 
 /* ---- Fits ---- */
 
