@@ -39,6 +39,9 @@ SEMI_AXES = (80.0, 100.0, 60.0)
 # reference initial light (video_fitter.cu:115-116)
 LAMBDA0 = np.array([-7.3627, 51.1364, 100.1784], np.float32)
 L0 = 0.005
+# start of a jointly fitted light: the truth perturbed by N(0, 2 mm) per position axis and
+# N(0, 2%) in intensity, the same order of accuracy as the pose and coefficient starts
+LIGHT_INIT_PERT = (2.0, 0.02)
 
 
 @dataclasses.dataclass
@@ -171,15 +174,54 @@ def _smooth_fields(rng, G: int, n: int, sigma: float) -> np.ndarray:
 
 SHAPE_BLUR_FRACTION = 0.25   # blur sigma of shape basis columns, in units of the grid size
 TEX_BLUR_FRACTION = 0.05     # blur sigma of albedo basis columns, in units of the grid size
+ALBEDO_DETAIL = (40.0, 12.0)
+SHAPE_AXIS_WEIGHT = (1.0, 1.0, 1.0)  # albedo mean structure at grid/40 and grid/12 cells
 
 
-def _shape_basis(rng, G: int, K: int, s0: float) -> np.ndarray:
+def _similarity_modes(P: np.ndarray) -> np.ndarray:
+    """Orthonormal basis [3V, 7] of the similarity motions of the point set P: three
+    translations, three infinitesimal rotations about the centroid and uniform scaling."""
+    c = P - P.mean(0)
+    V = P.shape[0]
+    M = np.zeros((V, 3, 7))
+    for a in range(3):
+        M[:, a, a] = 1.0
+    x, y, z = c[:, 0], c[:, 1], c[:, 2]
+    M[:, 1, 3], M[:, 2, 3] = -z, y          # rotation about x
+    M[:, 0, 4], M[:, 2, 4] = z, -x          # rotation about y
+    M[:, 0, 5], M[:, 1, 5] = -y, x          # rotation about z
+    M[:, :, 6] = c                          # scaling
+    q, _ = np.linalg.qr(M.reshape(3 * V, 7))
+    return q
+
+
+def _shape_basis(rng, G: int, K: int, s0: float, P: np.ndarray) -> np.ndarray:
+    """Smooth random deformation fields with the similarity motions of the mean shape P
+    projected out, as in a PCA model built from Procrustes-aligned scans: a column that
+    moved, turned or scaled the whole face would be indistinguishable from the pose."""
     V = G * G
     scale = s0 * np.arange(1, K + 1, dtype=np.float64) ** -0.7
     B = np.empty((V, 3, K), np.float64)
     for axis in range(3):
-        B[:, axis, :] = _smooth_fields(rng, G, K, SHAPE_BLUR_FRACTION * G) * scale
-    return B.reshape(3 * V, K).astype(np.float32)
+        B[:, axis, :] = _smooth_fields(rng, G, K, SHAPE_BLUR_FRACTION * G) * SHAPE_AXIS_WEIGHT[axis]
+    B = B.reshape(3 * V, K)
+    Q = _similarity_modes(P)
+    B -= Q @ (Q.T @ B)
+    B *= scale / np.sqrt(np.mean(B * B, axis=0))
+    return B.astype(np.float32)
+
+
+# Expression bounds shrink with the component like the reference's, which bounds each
+# coefficient in proportion to its variance (prepare_BFM.py:47-50, logbarrier_initializer.cu:284-285):
+# with unit-variance coefficients and column amplitudes s0 * k**-0.7 that is +-3.5 * k**-0.7.
+# The tight bounds of the small components are what keeps their weak directions out of the
+# Gauss-Newton steps.
+EXP_BOUND_DECAY = 0.7
+
+
+def exp_bounds(k_exp: int) -> np.ndarray:
+    """Expression coefficient bounds +-3.5 * k**-EXP_BOUND_DECAY (k = 1..K_exp)."""
+    return (3.5 * np.arange(1, k_exp + 1, dtype=np.float64) ** -EXP_BOUND_DECAY).astype(np.float32)
 
 
 def make_model(grid: int, k_id: int, k_tex: int, k_exp: int, seed: int) -> Model:
@@ -187,21 +229,23 @@ def make_model(grid: int, k_id: int, k_tex: int, k_exp: int, seed: int) -> Model
     P, A, E = _cap_vertices(grid)
     tris = _triangles(grid, P)
     V = grid * grid
-    exp_basis = _shape_basis(rng, grid, k_exp, 5.0)
-    id_basis = _shape_basis(rng, grid, k_id, 5.0) if k_id else None
+    exp_basis = _shape_basis(rng, grid, k_exp, 5.0, P)
+    id_basis = _shape_basis(rng, grid, k_id, 5.0, P) if k_id else None
     tex_basis = None
     if k_tex:
         tex_basis = (_smooth_fields(rng, grid, k_tex, TEX_BLUR_FRACTION * grid)
                      * (0.1 * np.arange(1, k_tex + 1, dtype=np.float64) ** -0.7)).astype(np.float32)
     # albedo mean: skin-like level with structure at 1/40 and 1/12 of the face, so that the
     # normalised-gradient objective has edges to follow at every crop size
-    tex = (0.55 + 0.18 * _smooth_fields(rng, grid, 1, grid / 40.0)[:, 0]
-           + 0.08 * _smooth_fields(rng, grid, 1, grid / 12.0)[:, 0])
+    fine, coarse = ALBEDO_DETAIL
+    tex = (0.55 + 0.18 * _smooth_fields(rng, grid, 1, grid / fine)[:, 0]
+           + 0.08 * _smooth_fields(rng, grid, 1, grid / coarse)[:, 0])
     tex = np.clip(tex, 0.15, 0.95).astype(np.float32)
     bnd = np.float32(3.5)
+    eb = exp_bounds(k_exp)
     m = Model(grid=grid, mean_shape=P.astype(np.float32), exp_basis=exp_basis, tex_mean=tex,
               triangles=tris, landmark_ids=_landmarks(grid),
-              exp_lb=np.full(k_exp, -bnd, np.float32), exp_ub=np.full(k_exp, bnd, np.float32),
+              exp_lb=-eb, exp_ub=eb,
               id_basis=id_basis, tex_basis=tex_basis)
     if k_id:
         m.id_lb = np.full(k_id, -bnd, np.float32)
@@ -271,8 +315,11 @@ def draw_truth(m: Model, rng, lay: dict, pose_deg: float, tz_range=(400.0, 600.0
     th = np.zeros(K)
     if lay["exp"] < lay["pose"]:
         ne = lay["pose"] - lay["exp"]
+        # unit-variance draws clipped to +-3, shrunk with the component's bound so that the
+        # truth stays strictly inside it
         th[lay["exp"]:lay["pose"]] = (exp_coeffs if exp_coeffs is not None
-                                      else np.clip(rng.standard_normal(ne) * exp_scale, -3, 3))
+                                      else np.clip(rng.standard_normal(ne) * exp_scale, -3, 3)
+                                      * (m.exp_ub[:ne].astype(np.float64) / 3.5))
     if lay["id"] < lay["exp"]:
         th[lay["id"]:lay["exp"]] = np.clip(rng.standard_normal(lay["exp"] - lay["id"]), -3, 3)
     if lay["tex"] < lay["id"]:
@@ -288,18 +335,21 @@ def draw_truth(m: Model, rng, lay: dict, pose_deg: float, tz_range=(400.0, 600.0
         th[lay["light"]:lay["light"] + 3] = lam
         th[lay["light"] + 3] = Lint
     init = th.copy()
-    # start from a perturbed expression/identity, a perturbed pose and the reference's
-    # initial light, at roughly the one-pixel accuracy of the reference's landmark-driven
-    # initialisation (video_fitter.cu:479-485); the caller makes it landmark-feasible with
-    # make_feasible()
+    # start from a perturbed expression/identity, a perturbed pose and a perturbed light,
+    # at roughly the one-pixel accuracy of the reference's landmark-driven initialisation
+    # (video_fitter.cu:479-485); the caller makes it landmark-feasible with make_feasible().
+    # Coefficients are perturbed in proportion to their bounds and kept inside them.
     nb = lay["pose"]
-    init[:nb] = np.clip(th[:nb] + rng.normal(0, coef_pert, nb), -3.2, 3.2)
+    cb = np.full(nb, 3.5)
+    if lay["exp"] < lay["pose"]:
+        cb[lay["exp"]:lay["pose"]] = m.exp_ub[:lay["pose"] - lay["exp"]]
+    init[:nb] = np.clip(th[:nb] + rng.normal(0, coef_pert, nb) * cb / 3.5, -0.92 * cb, 0.92 * cb)
     p = lay["pose"]
     init[p:p + 3] += rng.normal(0, [0.3, 0.3, 1.0])
     init[p + 3:p + 6] += rng.normal(0, np.deg2rad(0.2), 3)
     if fit_light:
-        init[lay["light"]:lay["light"] + 3] = LAMBDA0
-        init[lay["light"] + 3] = L0
+        init[lay["light"]:lay["light"] + 3] = lam + rng.normal(0, LIGHT_INIT_PERT[0], 3)
+        init[lay["light"] + 3] = Lint * (1.0 + rng.normal(0, LIGHT_INIT_PERT[1]))
     return Truth(theta=th, init=init, lam=lam, Lint=Lint)
 
 

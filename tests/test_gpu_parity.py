@@ -30,9 +30,19 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.fixture(scope="module")
-def scene():
-    return make_scene(grid=48, k_exp=12, crops=(96, 128, 80), seed=11)
+SCENES = {
+    # expression + pose, fixed light, small crops
+    "small": dict(grid=48, k_exp=12, crops=(96, 128, 80), seed=11),
+    # the benchmark's block structure: light fitted jointly (pose + expression + 4 light
+    # unknowns), more bases, larger and uneven crops, wider poses
+    "light": dict(grid=64, k_exp=40, fit_light=True, crops=(160, 224, 120), pose_deg=25.0, seed=13, cfg=10),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(SCENES))
+def scene(request):
+    _say(f"scene: {request.param}")
+    return make_scene(**SCENES[request.param])
 
 
 @pytest.fixture(scope="module")
@@ -130,8 +140,15 @@ def test_normal_equations(scene, gpu, mixed):
         r = orc.normal_equations(i, th[i])
         eH = np.linalg.norm(H[i] - r["H"]) / np.linalg.norm(r["H"])
         eg = np.linalg.norm(g[i] - r["g"]) / np.linalg.norm(r["g"])
-        _say(f"  frame {i}: |dH|/|H| = {eH:.2e}  |dg|/|g| = {eg:.2e}")
+        lay = scene.lay
+        blocks = {"exp": slice(lay["exp"], lay["pose"]), "pose": slice(lay["pose"], lay["pose"] + 6),
+                  "light": slice(lay["light"], lay["K"])}
+        per = {k: np.linalg.norm(H[i][v, v] - r["H"][v, v]) / max(np.linalg.norm(r["H"][v, v]), 1e-30)
+               for k, v in blocks.items() if v.stop > v.start}
+        _say(f"  frame {i}: {r['npix']} px, |dH|/|H| = {eH:.2e}  |dg|/|g| = {eg:.2e}  diagonal blocks "
+             + " ".join(f"{k} {e:.1e}" for k, e in per.items()))
         assert eH < 2e-3 and eg < 2e-3
+        assert all(e < 2e-3 for e in per.values())
 
 
 def test_mixed_matches_fp32(scene, gpu):
@@ -186,9 +203,35 @@ def _one_iteration_opts(scene):
     return opts
 
 
+def _check_one_iteration(scene, th0, tho, repo, thg, repg, label):
+    """Same GN step: equal line-search decisions, and the step difference small in the energy
+    norm of the oracle's H at the start, sqrt(d^T H d) / sqrt(s^T H s) < 1e-3 (d = GPU - oracle
+    result, s = oracle step).  The energy norm weighs each direction by what the data
+    determine; along directions the images barely see (scaled condition numbers reach 1e6
+    when the light is fitted) FP32 J^T J moves the solution further, so the plain parameter
+    RMS bound 1e-4 * max(1, |step|) is asserted for the well-conditioned scene only."""
+    orc = scene.oracle()
+    step = _pose_exp_rms(tho, th0, scene.lay)
+    rms = _pose_exp_rms(thg, tho, scene.lay)
+    en = []
+    for i in range(len(scene.frames)):
+        Hm = orc.normal_equations(i, th0[i])["H"]
+        d = thg[i] - tho[i]
+        st = tho[i] - th0[i]
+        en.append(float(np.sqrt(max(d @ Hm @ d, 0.0)) / max(np.sqrt(max(st @ Hm @ st, 0.0)), 1e-30)))
+    _say(f"  {label}: step RMS {step:.3e}, GPU vs oracle RMS {rms:.3e}, energy-norm rel "
+         f"{' '.join(f'{e:.1e}' for e in en)}, ls evals gpu {[r.ls_evals for r in repg]} oracle "
+         f"{[r.ls_evals for r in repo]}")
+    assert [r.ls_evals for r in repg] == [r.ls_evals for r in repo]
+    assert max(en) < 1e-3
+    if not scene.opts.fit_light:
+        assert rms < 1e-4 * max(1.0, step)
+    assert rms < 1e-3 * max(1.0, step)
+
+
 def test_one_iteration_plain_vs_oracle(scene):
     """One GN iteration (render, J^T J, FP64 Cholesky, Armijo line search) from the same
-    FP32 start: the step must match the FP64 oracle tightly."""
+    FP32 start: the step must match the FP64 oracle's."""
     _say("case: one GN iteration, plain path vs oracle")
     opts = _one_iteration_opts(scene)
     th0 = _f32(scene.init())
@@ -197,12 +240,7 @@ def test_one_iteration_plain_vs_oracle(scene):
     ctx = tdi.Context(scene.model, opts)
     b = tdi.Batch(ctx, scene.frames)
     thg, repg = b.fit_plain(th0)
-    step = _pose_exp_rms(tho, th0, scene.lay)
-    rms = _pose_exp_rms(thg.astype(np.float64), tho, scene.lay)
-    _say(f"  step RMS {step:.3e}, GPU vs oracle RMS {rms:.3e}, ls evals gpu "
-         f"{[r.ls_evals for r in repg]} oracle {[r.ls_evals for r in repo]}")
-    assert [r.ls_evals for r in repg] == [r.ls_evals for r in repo]
-    assert rms < 1e-4 * max(1.0, step)
+    _check_one_iteration(scene, th0, tho, repo, thg.astype(np.float64), repg, "plain")
     b.close()
     ctx.close()
 
@@ -216,25 +254,24 @@ def test_one_iteration_batched_vs_oracle(scene):
     ctx = tdi.Context(scene.model, opts)
     b = tdi.Batch(ctx, scene.frames)
     thg, repg = b.fit_batched(torch.from_numpy(th0.astype(np.float32)).cuda())
-    thg = thg.cpu().numpy().astype(np.float64)
-    step = _pose_exp_rms(tho, th0, scene.lay)
-    rms = _pose_exp_rms(thg, tho, scene.lay)
-    _say(f"  step RMS {step:.3e}, GPU vs oracle RMS {rms:.3e}, ls evals gpu "
-         f"{[r.ls_evals for r in repg]} oracle {[r.ls_evals for r in repo]}")
-    assert [r.ls_evals for r in repg] == [r.ls_evals for r in repo]
-    assert rms < 1e-4 * max(1.0, step)
+    _check_one_iteration(scene, th0, tho, repo, thg.cpu().numpy().astype(np.float64), repg, "batched")
     b.close()
     ctx.close()
 
 
 def _oracle_floor(scene, th0):
-    """The oracle's own spread under a 1e-6 perturbation of the start: the rasterised
-    objective is piecewise constant in visibility, so whole-fit trajectories amplify
-    round-off; parity of whole fits is judged against this floor."""
+    """The oracle's own spread: the largest pose/expression RMS between its fit from th0 and
+    its fits from three starts 1e-6 away (random signs).  The rasterised objective is
+    piecewise constant in visibility and the line search takes discrete decisions, so
+    whole-fit trajectories amplify round-off; parity of whole fits is judged against this."""
     orc = scene.oracle()
     a, ra = orc.fit(th0)
-    bb, _ = orc.fit(th0 + 1e-6)
-    return a, ra, _pose_exp_rms(a, bb, scene.lay)
+    rng = np.random.default_rng(17)
+    spread = 0.0
+    for _ in range(3):
+        bb, _ = orc.fit(th0 + 1e-6 * rng.choice([-1.0, 1.0], th0.shape))
+        spread = max(spread, _pose_exp_rms(a, bb, scene.lay))
+    return a, ra, spread
 
 
 def test_fit_plain_vs_oracle(scene, gpu):

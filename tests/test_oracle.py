@@ -489,26 +489,26 @@ def test_fit_lowers_objective_and_reports_agree(exp_scene):
 
 
 def test_fit_recovers_noise_free_truth():
-    """Every frame moves towards the truth; a frame that stops on the decrement test
-    (-delta.g < 3e-4) has converged onto it.  Frames that stop on the step-size rule
-    (t < 0.075, model_fitter.cu:241) stop early by design."""
+    """Noise-free observations: every fit moves the projected mesh towards the truth's, and a
+    frame that stops on the decrement test (-delta.g < 3e-4) has converged onto it.  Frames
+    that stop on the step-size rule (t < 0.075, model_fitter.cu:241) stop early by design.
+    Distance is the RMS over all vertices of the projected positions (pixels), which weak
+    expression directions do not inflate."""
     sc = make_scene(grid=36, k_exp=8, crops=(96, 96, 112, 128), pose_deg=15.0, noise=0.0, iters=40, seed=6,
                     cfg=24)
     orc = sc.oracle()
     th0 = sc.init()
     truth = sc.truth()
     thf, reps = orc.fit(th0)
-    p = sc.lay["pose"]
     converged = 0
     for i, rp in enumerate(reps):
-        e0 = np.abs(th0[i, p + 3:p + 6] - truth[i, p + 3:p + 6]).max()
-        ef = np.abs(thf[i, p + 3:p + 6] - truth[i, p + 3:p + 6]).max()
-        x0 = np.abs(th0[i, :p] - truth[i, :p]).max()
-        xf = np.abs(thf[i, :p] - truth[i, :p]).max()
-        assert ef < 0.7 * e0 and xf < 0.7 * x0, (i, e0, ef, x0, xf)
+        xt = orc.geometry(i, truth[i])[1]
+        e0 = np.sqrt(np.mean((orc.geometry(i, th0[i])[1] - xt) ** 2))
+        ef = np.sqrt(np.mean((orc.geometry(i, thf[i])[1] - xt) ** 2))
+        assert ef < 0.2 * e0, (i, e0, ef)
         if rp.stop & abi.STOP_DECREMENT:
             converged += 1
-            assert ef < 1e-3 and xf < 5e-3, (i, ef, xf)
+            assert ef < 5e-3, (i, ef)
     assert converged >= 1
 
 
