@@ -101,6 +101,10 @@ struct Workspace {
     double *obj = nullptr, *objb = nullptr;
     int *npix = nullptr, *status = nullptr, *face_ok = nullptr, *mask = nullptr, *searching = nullptr,
         *evalmask = nullptr;
+    int *act_m = nullptr, *act_e = nullptr;   // frame lists of mask / evalmask ([0] = count)
+    int* tcount = nullptr;                     // [C][tmax] resolve tile counts
+    int tmax = 0;
+    double* rt = nullptr;                      // [C][12] FP64 rotation (column-major) + translation
     FitState* st = nullptr;
     float* partials = nullptr;
     float* lmk = nullptr;      // [C][6][51]
@@ -269,6 +273,11 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     w.mask = A.get<int>(C);
     w.searching = A.get<int>(C);
     w.evalmask = A.get<int>(C);
+    w.act_m = A.get<int>(C + 1);
+    w.act_e = A.get<int>(C + 1);
+    w.tmax = resolve_tiles(b->max_area);
+    w.tcount = A.get<int>((size_t)C * (w.tmax > 0 ? w.tmax : 1));
+    w.rt = A.get<double>((size_t)C * 12);
     w.st = A.get<FitState>(C);
     w.S = jtj_splits(C, b->max_area);
     const size_t pf = std::max(jtj_partials_floats(C, w.S, K, false), jtj_partials_floats(C, w.S, K, true));
@@ -428,52 +437,56 @@ extern "C" int tdi_batch_destroy(tdi_batch* b)
 // ---------------------------------------------------------------------------- evaluation
 // Objective at theta for the frames in `mask`: synthesis, face check, raster, resolve,
 // barrier objective, shading, blur, gradients (Renderer::render + evaluate_objective_function).
-static int eval_render(tdi_batch* b, int n, const float* theta, const int* mask, int mixed, hipStream_t s)
+static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s)
 {
     tdi_ctx* c = b->ctx;
     Workspace& w = b->ws;
     int rc;
     if (mixed & TDI_MIXED_SYNTH_MFMA)
-        rc = launch_synthesize_gemm(c->m, c->L, w.frames, n, theta, mask, w.P0, w.P, w.vproj, nullptr, s);
+        rc = launch_synthesize_gemm(c->m, c->L, w.frames, fl, theta, w.rt, w.P0, w.P, w.vproj, s);
     else
-        rc = launch_synthesize(c->m, c->L, w.frames, n, theta, mask, w.P0, w.P, w.vproj, s);
+        rc = launch_synthesize(c->m, c->L, w.frames, fl, theta, w.rt, w.P0, w.P, w.vproj, s);
     if (rc) return rc;
-    if ((rc = launch_face_check(n, c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, mask, w.face_ok, s))) return rc;
-    if ((rc = launch_clear_keys_masked(w.frames, n, mask, w.keys, w.max_area, s))) return rc;
-    if ((rc = launch_raster(c->m, w.frames, n, mask, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-    if ((rc = launch_resolve(c->m, w.frames, n, mask, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, s)))
+    if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
+    if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.max_area, s))) return rc;
+    if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, s)))
         return rc;
-    if ((rc = launch_barrier(c->m, c->L, w.frames, n, mask, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
+    if ((rc = launch_barrier(c->m, c->L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
         return rc;
-    if ((rc = launch_shade(c->m, c->L, w.frames, n, mask, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
+    if ((rc = launch_shade(c->m, c->L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
                            w.max_area, s)))
         return rc;
-    if ((rc = launch_blur(w.frames, n, mask, c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
-    return launch_grads(w.frames, n, mask, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
-                        w.pcoef, w.obj,
-                        w.max_area, s);
+    if ((rc = launch_blur(w.frames, fl, c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
+    return launch_grads(w.frames, fl, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
+                        w.pcoef, w.obj, w.max_area, s);
 }
 
-// Objective, gradient and Gauss-Newton matrix at theta (get_obj_hessian_and_gradient_multiframe).
-static int eval_full(tdi_batch* b, int n, const float* theta, const int* mask, int mixed, hipStream_t s)
+static int launch_normal_eq(tdi_batch* b, FrameList fl, int mixed, hipStream_t s)
 {
     tdi_ctx* c = b->ctx;
     Workspace& w = b->ws;
-    int rc = eval_render(b, n, theta, mask, mixed, s);
+    if (mixed & TDI_MIXED_JTJ_BF16)
+        return launch_jtj_mfma(c->L, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H,
+                               w.g, w.partials, w.S, (mixed & TDI_MIXED_JTR_BF16) ? 1 : 0,
+                               (mixed & TDI_MIXED_TWO_TERM) ? 1 : 0, s);
+    return launch_jtj_f32(c->L, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
+                          w.partials, w.S, s);
+}
+
+// Objective, gradient and Gauss-Newton matrix at theta (get_obj_hessian_and_gradient_multiframe).
+static int eval_full(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    int rc = eval_render(b, fl, theta, mixed, s);
     if (rc) return rc;
-    if ((rc = launch_dI(c->m, c->L, w.frames, n, mask, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
+    if ((rc = launch_dI(c->m, c->L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
                         w.max_area, s)))
         return rc;
-    if (mixed & TDI_MIXED_JTJ_BF16)
-        rc = launch_jtj_mfma(c->L, w.frames, n, mask, w.vis, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
-                             w.partials, w.S, (mixed & TDI_MIXED_JTR_BF16) ? 1 : 0,
-                             (mixed & TDI_MIXED_TWO_TERM) ? 1 : 0, s);
-    else
-        rc = launch_jtj_f32(c->L, w.frames, n, mask, w.vis, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
-                            w.partials, w.S, s);
-    if (rc) return rc;
-    return launch_barrier(c->m, c->L, w.frames, n, mask, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj, w.objb,
+    if ((rc = launch_normal_eq(b, fl, mixed, s))) return rc;
+    return launch_barrier(c->m, c->L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj, w.objb,
                           w.H, w.g, s);
 }
 
@@ -506,7 +519,7 @@ int tdi_synthesize(tdi_batch* b, const float* theta, float* xyz, float* xyp, voi
         const int n = std::min(b->ws.C, b->n - i0);
         int rc = ws_load(b, i0, n, s);
         if (rc) return rc;
-        if ((rc = launch_synthesize(c->m, c->L, b->ws.frames, n, theta + (size_t)i0 * K, nullptr, b->ws.P0, b->ws.P,
+        if ((rc = launch_synthesize(c->m, c->L, b->ws.frames, all_frames(n), theta + (size_t)i0 * K, b->ws.rt, b->ws.P0, b->ws.P,
                                     b->ws.vproj, s)))
             return rc;
         if ((rc = launch_misc_unpack(n, V, b->ws.P, b->ws.vproj, xyz + (size_t)i0 * V * 3, xyp + (size_t)i0 * V * 2, s)))
@@ -530,8 +543,8 @@ int tdi_rasterize(tdi_batch* b, const float* xyz, const float* xyp, uint64_t* ke
         if ((rc = launch_misc_pack(n, V, xyz + (size_t)i0 * V * 3, xyp + (size_t)i0 * V * 2, w.vproj, s))) return rc;
         if ((rc = launch_clear_keys(w.keys, len, s))) return rc;
         if ((rc = launch_misc_fill_f32(w.bary, 2 * len, 0.f, s))) return rc;
-        if ((rc = launch_raster(c->m, w.frames, n, nullptr, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-        if ((rc = launch_resolve(c->m, w.frames, n, nullptr, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, s)))
+        if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, s)))
             return rc;
         if (keys) TDI_HIP(hipMemcpyAsync(keys + off, w.keys, sizeof(uint64_t) * len, hipMemcpyDeviceToDevice, s));
         if (bary) TDI_HIP(hipMemcpyAsync(bary + 2 * off, w.bary, sizeof(float) * 2 * len, hipMemcpyDeviceToDevice, s));
@@ -550,7 +563,7 @@ int tdi_render(tdi_batch* b, const float* theta, float* img, float* obj, int32_t
         if (rc) return rc;
         const long long off = b->hframes[i0].obs_off;
         const long long len = (i0 + n < b->n ? b->hframes[i0 + n].obs_off : b->total) - off;
-        if ((rc = eval_render(b, n, theta + (size_t)i0 * K, nullptr, 0, s))) return rc;
+        if ((rc = eval_render(b, all_frames(n), theta + (size_t)i0 * K, 0, s))) return rc;
         if (img) TDI_HIP(hipMemcpyAsync(img + off, w.img, sizeof(float) * len, hipMemcpyDeviceToDevice, s));
         if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj + i0, n, s))) return rc;
         if (npix) TDI_HIP(hipMemcpyAsync(npix + i0, w.npix, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
@@ -567,7 +580,7 @@ static int normal_eq(tdi_batch* b, const float* theta, float* H, float* g, float
         const int n = std::min(w.C, b->n - i0);
         int rc = ws_load(b, i0, n, s);
         if (rc) return rc;
-        if ((rc = eval_full(b, n, theta + (size_t)i0 * K, nullptr, mixed ? TDI_MIXED_JTJ_BF16 : 0, s))) return rc;
+        if ((rc = eval_full(b, all_frames(n), theta + (size_t)i0 * K, mixed ? TDI_MIXED_JTJ_BF16 : 0, s))) return rc;
         if (H) TDI_HIP(hipMemcpyAsync(H + (size_t)i0 * K * K, w.H, sizeof(float) * n * K * K, hipMemcpyDeviceToDevice, s));
         if (g) TDI_HIP(hipMemcpyAsync(g + (size_t)i0 * K, w.g, sizeof(float) * n * K, hipMemcpyDeviceToDevice, s));
         if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj + i0, n, s))) return rc;
@@ -611,42 +624,35 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
     for (int r = 0; r < reps; ++r) {
         TDI_HIP(hipEventRecord(ev[0], s));
         if (mixed & TDI_MIXED_SYNTH_MFMA)
-            rc = launch_synthesize_gemm(c->m, c->L, w.frames, n, theta, nullptr, w.P0, w.P, w.vproj, nullptr, s);
+            rc = launch_synthesize_gemm(c->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, w.P, w.vproj, s);
         else
-            rc = launch_synthesize(c->m, c->L, w.frames, n, theta, nullptr, w.P0, w.P, w.vproj, s);
+            rc = launch_synthesize(c->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, w.P, w.vproj, s);
         if (rc) return rc;
         TDI_HIP(hipEventRecord(ev[1], s));
-        if ((rc = launch_clear_keys_masked(w.frames, n, nullptr, w.keys, w.max_area, s))) return rc;
-        if ((rc = launch_raster(c->m, w.frames, n, nullptr, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-        if ((rc = launch_resolve(c->m, w.frames, n, nullptr, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, s)))
+        if ((rc = launch_clear_keys_frames(w.frames, all_frames(n), w.keys, w.max_area, s))) return rc;
+        if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[2], s));
-        if ((rc = launch_shade(c->m, c->L, w.frames, n, nullptr, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
+        if ((rc = launch_shade(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
                                w.max_area, s)))
             return rc;
-        if ((rc = launch_blur(w.frames, n, nullptr, c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
-        if ((rc = launch_grads(w.frames, n, nullptr, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg,
+        if ((rc = launch_blur(w.frames, all_frames(n), c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
+        if ((rc = launch_grads(w.frames, all_frames(n), w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg,
                                w.pnb, w.pcoef, w.obj,
                                w.max_area, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[3], s));
-        if ((rc = launch_barrier(c->m, c->L, w.frames, n, nullptr, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
+        if ((rc = launch_barrier(c->m, c->L, w.frames, all_frames(n), theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                                  w.objb, w.H, w.g, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[4], s));
-        if ((rc = launch_dI(c->m, c->L, w.frames, n, nullptr, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
+        if ((rc = launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
                             w.max_area, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[5], s));
-        if (mixed & TDI_MIXED_JTJ_BF16)
-            rc = launch_jtj_mfma(c->L, w.frames, n, nullptr, w.vis, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H,
-                                 w.g, w.partials, w.S, (mixed & TDI_MIXED_JTR_BF16) ? 1 : 0,
-                                 (mixed & TDI_MIXED_TWO_TERM) ? 1 : 0, s);
-        else
-            rc = launch_jtj_f32(c->L, w.frames, n, nullptr, w.vis, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H,
-                                w.g, w.partials, w.S, s);
-        if (rc) return rc;
-        if ((rc = launch_barrier(c->m, c->L, w.frames, n, nullptr, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj,
+        if ((rc = launch_normal_eq(b, all_frames(n), mixed, s))) return rc;
+        if ((rc = launch_barrier(c->m, c->L, w.frames, all_frames(n), theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj,
                                  w.objb, w.H, w.g, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[6], s));
@@ -698,7 +704,7 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
         for (int it = 0; it < o.max_iters; ++it) {
             outer++;
             if (terminate) break;
-            if ((rc = eval_full(b, 1, w.theta, w.mask, 0, s))) return rc;
+            if ((rc = eval_full(b, all_frames(1), w.theta, 0, s))) return rc;
             if ((rc = launch_cholesky(1, K, w.H, w.g, w.delta, w.status, 0, w.mask, s))) return rc;
             int st = 0, np = 0;
             double obj0 = 0;
@@ -721,7 +727,7 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
                 for (int k = 0; k < K; ++k) tt[k] = th[k] + t * delta[k];
                 if (tt[off_tz] < 0.f) break;
                 TDI_HIP(hipMemcpyAsync(w.theta_try, tt.data(), sizeof(float) * K, hipMemcpyHostToDevice, s));
-                if ((rc = eval_render(b, 1, w.theta_try, w.mask, 0, s))) return rc;
+                if ((rc = eval_render(b, all_frames(1), w.theta_try, 0, s))) return rc;
                 double objt = 0, objb = 0;
                 int fok = 1;
                 TDI_HIP(hipMemcpyAsync(&objt, w.obj, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -741,7 +747,7 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
             }
         }
         if (!terminate && !(rep.stop & (TDI_STOP_NAN | TDI_STOP_NOT_PD))) rep.stop |= TDI_STOP_MAXITER;
-        if ((rc = eval_render(b, 1, w.theta, w.mask, 0, s))) return rc;
+        if ((rc = eval_render(b, all_frames(1), w.theta, 0, s))) return rc;
         double fin = 0;
         TDI_HIP(hipMemcpyAsync(&fin, w.obj, sizeof(double), hipMemcpyDeviceToHost, s));
         TDI_HIP(hipStreamSynchronize(s));
@@ -771,7 +777,8 @@ static int record_fit(tdi_batch* b, hipStream_t s)
     int rc;
     for (int it = 0; it < o.max_iters; ++it) {
         if ((rc = launch_copy_mask(n, w.st, w.mask, s))) return rc;
-        if ((rc = eval_full(b, w.C, w.theta, w.mask, o.mixed_flags, s))) return rc;
+        if ((rc = launch_compact_frames(n, w.mask, w.act_m, s))) return rc;
+        if ((rc = eval_full(b, FrameList{w.act_m, w.C}, w.theta, o.mixed_flags, s))) return rc;
         if ((rc = launch_cholesky(n, K, w.H, w.g, w.delta, w.status, (o.mixed_flags & TDI_MIXED_CHOL_F32) ? 1 : 0,
                                   w.mask, s)))
             return rc;
@@ -780,7 +787,8 @@ static int record_fit(tdi_batch* b, hipStream_t s)
             if ((rc = launch_ls_propose(n, K, off_tz, o.t_min, w.st, w.theta, w.delta, w.theta_try, w.searching,
                                         w.evalmask, s)))
                 return rc;
-            if ((rc = eval_render(b, w.C, w.theta_try, w.evalmask, o.mixed_flags, s))) return rc;
+            if ((rc = launch_compact_frames(n, w.evalmask, w.act_e, s))) return rc;
+            if ((rc = eval_render(b, FrameList{w.act_e, w.C}, w.theta_try, o.mixed_flags, s))) return rc;
             if ((rc = launch_ls_decide(n, K, w.st, w.evalmask, w.searching, w.obj, w.objb, w.face_ok, w.theta,
                                        w.theta_try, o, s)))
                 return rc;
@@ -838,7 +846,7 @@ extern "C" int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* repor
         if (reports) {
             // objective at the returned parameters
             if ((rc = set_mask_all(b, s))) return rc;
-            if ((rc = eval_render(b, w.C, w.theta, w.mask, c->opts.mixed_flags, s))) return rc;
+            if ((rc = eval_render(b, all_frames(w.n_cur), w.theta, c->opts.mixed_flags, s))) return rc;
         }
         std::vector<double> hobj(w.C);
         std::vector<int> hnp(w.C);

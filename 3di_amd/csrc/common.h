@@ -222,53 +222,70 @@ void set_error(const char* fmt, ...);
 // which frames take part (NULL = all).
 
 // Landmarks per frame, [n][6][51]: x, y (image pixels), box lx, ux, ly, uy (face-size units).
+// The frames a per-frame launch works on: every workspace slot 0..n-1 (act == nullptr), or
+// the compacted list act[1 .. act[0]] of slot indices built on the device from a mask
+// (launch_compact_frames).  Per-frame kernels walk the list with a grid-stride loop over
+// blockIdx.y (or blockIdx.x), so a launch over few frames costs little even though a
+// captured graph fixes its grid.
+struct FrameList {
+    const int* act;
+    int n;              // workspace slots: bound of the list and of the grids
+};
+inline FrameList all_frames(int n) { return FrameList{nullptr, n}; }
+__device__ __forceinline__ int fl_count(const int* act, int n) { return act ? act[0] : n; }
+__device__ __forceinline__ int fl_frame(const int* act, int i) { return act ? act[1 + i] : i; }
+// grid extent over frames: every slot when there are few, else `cap` blocks that loop
+inline int grid_frames(int n, int cap) { return n < 1 ? 1 : (n < cap ? n : cap); }
+
 struct LmkArrays {
     const float* base;
 };
 
 // geometry.hip
-int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const float* theta,
-                      const int* mask, float* P0, float* P, float4* vproj, hipStream_t s);
-int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const float* theta,
-                           const int* mask, float* P0, float* P, float4* vproj, float* scratch, hipStream_t s);
+// rt: [slots][12] scratch for the per-frame FP64 rotation (column-major) and translation
+int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
+                      double* rt, float* P0, float* P, float4* vproj, hipStream_t s);
+int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
+                           const float* theta, double* rt, float* P0, float* P, float4* vproj, hipStream_t s);
 
 // raster.hip
 int launch_clear_keys(uint64_t* keys, long long total, hipStream_t s);
-int launch_clear_keys_masked(const FrameGPU* frames, int n, const int* mask, uint64_t* keys, int max_area,
-                             hipStream_t s);
-int launch_raster(const ModelGPU& m, const FrameGPU* frames, int n, const int* mask, const float4* vproj, int ntmp,
+int launch_clear_keys_frames(const FrameGPU* frames, FrameList fl, uint64_t* keys, int max_area, hipStream_t s);
+int launch_raster(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj, int ntmp,
                   uint64_t* keys, hipStream_t s);
-int launch_resolve(const ModelGPU& m, const FrameGPU* frames, int n, const int* mask, const float4* vproj,
+// tcount: [slots][tmax] tile counts, tmax >= resolve_tiles(largest crop)
+int resolve_tiles(int max_area);
+int launch_resolve(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj,
                    const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int max_area,
-                   hipStream_t s);
+                   int* tcount, int tmax, hipStream_t s);
 
 // shade.hip
-int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask, const float* theta,
+int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
                  const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
                  const float* bary, float* img, int max_area, hipStream_t s);
-int launch_blur(const FrameGPU* frames, int n, const int* mask, const float* taps, const float* in, float* tmp,
+int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const float* in, float* tmp,
                 float* out, int max_area, hipStream_t s);
-int launch_grads(const FrameGPU* frames, int n, const int* mask, const int* vis_list, const int* npix,
+int launch_grads(const FrameGPU* frames, FrameList fl, const int* vis_list, const int* npix,
                  const int* cum, const float* blurred, const float2* obs_g, float w, float4* pixg, int4* pnb,
                  float* pcoef, double* obj, int max_area, hipStream_t s);
 int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const float* obs, float* tmp, float* blurred,
                        float2* obs_g, int max_area, hipStream_t s);
 
 // jacobian.hip
-int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask, const float* theta,
+int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
               const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
               const float* bary, const float4* pixg, float* dI, int max_area, hipStream_t s);
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);
-int launch_jtj_f32(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
-                   const int* npix, const int4* pnb, const float* pcoef, const float4* pixg, const float* dI, float w,
-                   float* H, float* g, float* partials, int S, hipStream_t s);
-int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
-                    const int* npix, const int4* pnb, const float* pcoef, const float4* pixg, const float* dI, float w,
-                    float* H, float* g, float* partials, int S, int g_on_mfma, int two_term, hipStream_t s);
+int launch_jtj_f32(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
+                   const float* pcoef, const float4* pixg, const float* dI, float w, float* H, float* g,
+                   float* partials, int S, hipStream_t s);
+int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
+                    const float* pcoef, const float4* pixg, const float* dI, float w, float* H, float* g,
+                    float* partials, int S, int g_on_mfma, int two_term, hipStream_t s);
 
 // solve.hip
-int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask,
+int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                    const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
                    float* H, float* g, hipStream_t s);
 int launch_cholesky(int n, int K, const float* H, const float* g, float* delta, int* status, int mode, const int* mask,
@@ -280,8 +297,10 @@ int launch_ls_propose(int n, int K, int off_tz, float t_min, FitState* st, const
 int launch_ls_decide(int n, int K, FitState* st, const int* evalmask, int* searching, const double* obj,
                      const double* objb, const int* face_ok, float* theta, const float* theta_try,
                      const tdi_fit_options& o, hipStream_t s);
-int launch_face_check(int n, int V, const int* lmk_ids, const float4* vproj, float ref_face_size, const int* mask,
+int launch_face_check(int V, const int* lmk_ids, const float4* vproj, float ref_face_size, FrameList fl,
                       int* face_ok, hipStream_t s);
 int launch_copy_mask(int n, const FitState* st, int* mask, hipStream_t s);
+// act[0] = number of set entries of mask[0..n-1], act[1..] = their indices in increasing order
+int launch_compact_frames(int n, const int* mask, int* act, hipStream_t s);
 
 } // namespace tdi

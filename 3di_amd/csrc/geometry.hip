@@ -40,39 +40,56 @@ __device__ __forceinline__ void store_vertex(const FrameGPU& fr, const double* R
                            (float)Z, 0.f);
 }
 
-__global__ void __launch_bounds__(256) k_synth_gemv(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
-                                                     const float* __restrict__ theta, const int* __restrict__ mask,
-                                                     float* __restrict__ P0, float* __restrict__ P,
-                                                     float4* __restrict__ vproj)
+// FP64 rotation and translation of every listed frame, once per launch (the synthesis
+// workgroups read them instead of evaluating Rodrigues' formula per workgroup).
+__global__ void k_frame_rt(Layout L, const float* __restrict__ theta, const int* __restrict__ act, int n,
+                           double* __restrict__ rt)
 {
-    const int f = blockIdx.y;
-    if (mask && !mask[f]) return;
-    __shared__ double R[9], tau[3];
-    __shared__ float th[MAXK_LDS];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fl_count(act, n)) return;
+    const int f = fl_frame(act, i);
     const float* tf = theta + (size_t)f * L.K;
-    for (int i = threadIdx.x; i < L.K; i += blockDim.x) th[i] = tf[i];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        rodrigues_f64(th + L.off_pose + 3, R);
-        for (int a = 0; a < 3; ++a) tau[a] = th[L.off_pose + a];
-    }
-    __syncthreads();
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= m.V) return;
-    double p[3];
+    double R[9];
+    rodrigues_f64(tf + L.off_pose + 3, R);
+    double* o = rt + (size_t)f * 12;
+    for (int k = 0; k < 9; ++k) o[k] = R[k];
+    for (int a = 0; a < 3; ++a) o[9 + a] = tf[L.off_pose + a];
+}
+
+__global__ void __launch_bounds__(256) k_synth_gemv(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+                                                     const float* __restrict__ theta, const int* __restrict__ act,
+                                                     int n, const double* __restrict__ rt, float* __restrict__ P0,
+                                                     float* __restrict__ P, float4* __restrict__ vproj)
+{
+    __shared__ double R[12];
+    __shared__ float th[MAXK_LDS];
+    const double* tau = R + 9;
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        const float* tf = theta + (size_t)f * L.K;
+        for (int i = threadIdx.x; i < L.K; i += blockDim.x) th[i] = tf[i];
+        if (threadIdx.x < 12) R[threadIdx.x] = rt[(size_t)f * 12 + threadIdx.x];
+        __syncthreads();
+        const int v = blockIdx.x * blockDim.x + threadIdx.x;
+        if (v < m.V) {
+            double p[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int r = 3 * v + a;
-        double s = m.mean[r];
-        const float* e = m.exp_basis + (size_t)r * m.k_exp;
-        for (int k = 0; k < L.nexp; ++k) s = fma((double)e[k], (double)th[L.off_exp + k], s);
-        if (L.nid) {
-            const float* ib = m.id_basis + (size_t)r * m.k_id;
-            for (int k = 0; k < L.nid; ++k) s = fma((double)ib[k], (double)th[L.off_id + k], s);
+            for (int a = 0; a < 3; ++a) {
+                const int r = 3 * v + a;
+                double s = m.mean[r];
+                const float* e = m.exp_basis + (size_t)r * m.k_exp;
+                for (int k = 0; k < L.nexp; ++k) s = fma((double)e[k], (double)th[L.off_exp + k], s);
+                if (L.nid) {
+                    const float* ib = m.id_basis + (size_t)r * m.k_id;
+                    for (int k = 0; k < L.nid; ++k) s = fma((double)ib[k], (double)th[L.off_id + k], s);
+                }
+                p[a] = s;
+            }
+            store_vertex(frames[f], R, tau, p, (size_t)f * m.V + v, P0, P, vproj);
         }
-        p[a] = s;
+        __syncthreads();
     }
-    store_vertex(frames[f], R, tau, p, (size_t)f * m.V + v, P0, P, vproj);
 }
 
 // FP64 MFMA operand layout, v_mfma_f64_16x16x4f64 (one double per lane for A and B):
@@ -84,43 +101,37 @@ constexpr int S64_FR = 64;      // 4 waves x 16 frames
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) k_synth_mfma64(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, int n,
-                                                       const float* __restrict__ theta, const int* __restrict__ mask,
-                                                       float* __restrict__ P0, float* __restrict__ P,
-                                                       float4* __restrict__ vproj)
+// Columns are positions in the frame list: workgroup column group y covers list entries
+// 64y .. 64y+63, so a launch over a short list only runs its first column group(s).
+__global__ void __launch_bounds__(256) k_synth_mfma64(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+                                                       const float* __restrict__ theta, const int* __restrict__ act,
+                                                       int n, const double* __restrict__ rt, float* __restrict__ P0,
+                                                       float* __restrict__ P, float4* __restrict__ vproj)
 {
     const int row0 = blockIdx.x * S64_ROWS;
-    const int f0 = blockIdx.y * S64_FR;
+    const int c0 = blockIdx.y * S64_FR;
+    const int nact = fl_count(act, n);
+    if (c0 >= nact) return;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     __shared__ double sacc[S64_ROWS][S64_FR + 1];
     __shared__ double sR[S64_FR][12];
-    __shared__ int sact[S64_FR];
-    __shared__ int sany;
+    __shared__ int sfr[S64_FR];
     if (tid < S64_FR) {
-        const int f = f0 + tid;
-        const int a = (f < n) && (!mask || mask[f]);
-        sact[tid] = a;
-        if (a) {
-            const float* tf = theta + (size_t)f * L.K;
-            rodrigues_f64(tf + L.off_pose + 3, sR[tid]);
-            sR[tid][9] = tf[L.off_pose + 0];
-            sR[tid][10] = tf[L.off_pose + 1];
-            sR[tid][11] = tf[L.off_pose + 2];
-        }
+        const int c = c0 + tid;
+        const int f = c < nact ? fl_frame(act, c) : -1;
+        sfr[tid] = f;
+        if (f >= 0)
+            for (int k = 0; k < 12; ++k) sR[tid][k] = rt[(size_t)f * 12 + k];
     }
-    if (tid == 0) sany = 0;
     __syncthreads();
-    if (tid < S64_FR && sact[tid]) sany = 1;
-    __syncthreads();
-    if (!sany) return;
 
     const int Ke = L.nexp;
     const int rowsTotal = 3 * m.V;
     {
         v4d acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
         const int i = lane & 15, kk = lane >> 4;
-        const int fj = f0 + wave * 16 + i;
-        const bool frok = fj < n;
+        const int fj = sfr[wave * 16 + i];
+        const bool frok = fj >= 0;
         const float* bcol = theta + (size_t)(frok ? fj : 0) * L.K + L.off_exp;
         const float* arow[3];
         bool rowok[3];
@@ -149,8 +160,8 @@ __global__ void __launch_bounds__(256) k_synth_mfma64(ModelGPU m, Layout L, cons
     for (int e = tid; e < 16 * S64_FR; e += blockDim.x) {
         const int vl = e & 15, j = e >> 4;
         const int v = row0 / 3 + vl;
-        const int f = f0 + j;
-        if (v >= m.V || !sact[j]) continue;
+        const int f = sfr[j];
+        if (v >= m.V || f < 0) continue;
         double p[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) p[a] = (double)m.mean[3 * v + a] + sacc[3 * vl + a][j];
@@ -160,25 +171,37 @@ __global__ void __launch_bounds__(256) k_synth_mfma64(ModelGPU m, Layout L, cons
 
 } // namespace
 
-int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const float* theta,
-                      const int* mask, float* P0, float* P, float4* vproj, hipStream_t s)
+static int launch_frame_rt(const Layout& L, FrameList fl, const float* theta, double* rt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_frame_rt, dim3((fl.n + 255) / 256 > 0 ? (fl.n + 255) / 256 : 1), dim3(256), 0, s, L, theta,
+                       fl.act, fl.n, rt);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
+                      double* rt, float* P0, float* P, float4* vproj, hipStream_t s)
 {
     if (L.K > MAXK_LDS) {
         set_error("synthesize: K=%d exceeds %d", L.K, MAXK_LDS);
         return TDI_ERR_ARG;
     }
-    dim3 grid((m.V + 255) / 256, n);
-    hipLaunchKernelGGL(k_synth_gemv, grid, dim3(256), 0, s, m, L, frames, theta, mask, P0, P, vproj);
+    int rc = launch_frame_rt(L, fl, theta, rt, s);
+    if (rc) return rc;
+    dim3 grid((m.V + 255) / 256, grid_frames(fl.n, 64));
+    hipLaunchKernelGGL(k_synth_gemv, grid, dim3(256), 0, s, m, L, frames, theta, fl.act, fl.n, rt, P0, P, vproj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
 
-int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const float* theta,
-                           const int* mask, float* P0, float* P, float4* vproj, float* /*scratch*/, hipStream_t s)
+int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
+                           const float* theta, double* rt, float* P0, float* P, float4* vproj, hipStream_t s)
 {
-    if (L.nid) return launch_synthesize(m, L, frames, n, theta, mask, P0, P, vproj, s);
-    dim3 grid((3 * m.V + S64_ROWS - 1) / S64_ROWS, (n + S64_FR - 1) / S64_FR);
-    hipLaunchKernelGGL(k_synth_mfma64, grid, dim3(256), 0, s, m, L, frames, n, theta, mask, P0, P, vproj);
+    if (L.nid) return launch_synthesize(m, L, frames, fl, theta, rt, P0, P, vproj, s);
+    int rc = launch_frame_rt(L, fl, theta, rt, s);
+    if (rc) return rc;
+    dim3 grid((3 * m.V + S64_ROWS - 1) / S64_ROWS, (fl.n + S64_FR - 1) / S64_FR);
+    hipLaunchKernelGGL(k_synth_mfma64, grid, dim3(256), 0, s, m, L, frames, theta, fl.act, fl.n, rt, P0, P, vproj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

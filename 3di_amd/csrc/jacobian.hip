@@ -51,17 +51,18 @@ __device__ inline void load_frame_rot(const Layout& L, const FrameGPU& fr, const
 }
 
 __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
-                                             const int* __restrict__ mask, const float* __restrict__ theta,
+                                             const int* __restrict__ act, int n, const float* __restrict__ theta,
                                              const float* __restrict__ P0, const float* __restrict__ P,
                                              const int* __restrict__ vis, const int* __restrict__ npix,
                                              const unsigned long long* __restrict__ keys,
                                              const float2* __restrict__ bary, const float4* __restrict__ pixg,
                                              float* __restrict__ dI)
 {
-    const int f = blockIdx.y;
-    if (mask && !mask[f]) return;
     __shared__ FrameRot sr;
     __shared__ float stex[512];
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const int f = fl_frame(act, yi);
     const FrameGPU fr = frames[f];
     const float* tf = theta + (size_t)f * L.K;
     if (threadIdx.x == 0) load_frame_rot(L, fr, tf, sr);
@@ -165,6 +166,8 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
             out[j] = val;
         }
     }
+    __syncthreads();   // sr and stex are rewritten by the next iteration
+    }
 }
 
 // Per-pixel coefficients for the normalised-gradient Jacobian.
@@ -213,23 +216,25 @@ constexpr int MAXKP = 128;
 __device__ __forceinline__ int tri_index(int I, int J, int T) { return I * T - (I * (I - 1)) / 2 + (J - I); }
 
 __global__ void __launch_bounds__(256) k_jtj_f32(Layout L, const FrameGPU* __restrict__ frames,
-                                                  const int* __restrict__ mask, const int* __restrict__ vis,
+                                                  const int* __restrict__ act, int n,
                                                   const int* __restrict__ npix, const int4* __restrict__ pnb,
                                                   const float* __restrict__ pcoef,
                                                   const float4* __restrict__ pixg, const float* __restrict__ dI,
                                                   int S, int KP, float* __restrict__ Hp, float* __restrict__ gp)
 {
-    const int f = blockIdx.y, s = blockIdx.x;
-    if (mask && !mask[f]) return;
+    const int s = blockIdx.x;
+    __shared__ float sJ[2 * F32_CH][MAXKP + 1];
+    __shared__ float sgs[2 * F32_CH];
+    __shared__ PixCoef spc[F32_CH];
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const int f = fl_frame(act, yi);
     const FrameGPU fr = frames[f];
     const int N = npix[f];
     const int K = L.K;
     const int T = KP / 4;
     const int ntiles = T * (T + 1) / 2;
     const int p0 = (int)((long long)N * s / S), p1 = (int)((long long)N * (s + 1) / S);
-    __shared__ float sJ[2 * F32_CH][MAXKP + 1];
-    __shared__ float sgs[2 * F32_CH];
-    __shared__ PixCoef spc[F32_CH];
     const float* dIf = dI + (size_t)fr.pix_off * K;
     const int tid = threadIdx.x;
     float acc[2][16];
@@ -291,16 +296,19 @@ __global__ void __launch_bounds__(256) k_jtj_f32(Layout L, const FrameGPU* __res
         for (int e = 0; e < 16; ++e) Hout[(size_t)tile * 16 + e] = acc[u][e];
     }
     if (tid < KP) gp[((size_t)f * S + s) * KP + tid] = gacc;
+    __syncthreads();
+    }
 }
 
 // Sum the S partials; tiles are 4x4 (f32 kernel) or 16x16 (MFMA kernel), upper triangle.
-__global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ mask, int K, int KP, int TS, int S,
+__global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ act, int n, int K, int KP, int TS, int S,
                                                      const float* __restrict__ Hp, const float* __restrict__ gp,
                                                      float w, float* __restrict__ H, float* __restrict__ g,
                                                      int g_from_tiles)
 {
-    const int f = blockIdx.y;
-    if (mask && !mask[f]) return;
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const int f = fl_frame(act, yi);
     const int T = KP / TS;
     const int ntiles = T * (T + 1) / 2;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K * K + K; e += gridDim.x * blockDim.x) {
@@ -326,15 +334,14 @@ __global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ mask
             g[(size_t)f * K + j] = -w * sum;
         }
     }
+    }
 }
 
 // ------------------------------------------------------------------ BF16 split MFMA J^T J
 // J is split into BF16 parts: x = hi + mid + lo (three terms, each rounded to nearest even),
 // and J^T J is accumulated in FP32 from the six products that matter down to 2^-24:
 // hh + hm + mh + hl + lh + mm.  With nsplit == 2 only hi + lo and hh + hl + lh are used.
-constexpr int MF_CH = 32;            // pixels per chunk -> 64 rows
-constexpr int MF_ROWS = 2 * MF_CH;
-constexpr int MF_STRIDE = MF_ROWS + 8; // bf16 elements per column in LDS (padding vs bank conflicts)
+constexpr int JTJ_WAVES_PER_SPLIT = 4;   // one-wave workgroups per J^T J split (register kernel)
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -348,124 +355,123 @@ __device__ __forceinline__ unsigned short bf16_rne(float x)
 
 __device__ __forceinline__ float bf16_f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-__global__ void __launch_bounds__(256) k_jtj_mfma(Layout L, const FrameGPU* __restrict__ frames,
-                                                   const int* __restrict__ mask, const int* __restrict__ vis,
-                                                   const int* __restrict__ npix, const int4* __restrict__ pnb,
-                                                   const float* __restrict__ pcoef,
-                                                   const float4* __restrict__ pixg, const float* __restrict__ dI,
-                                                   int S, int KP, int nsplit, float* __restrict__ Hp,
-                                                   float* __restrict__ gp)
+// ------------------------------------------------------------------ register-direct MFMA J^T J
+// One wavefront per workgroup.  The wave walks its pixel range eight pixels (16 rows of
+// [J | r]) at a time.  For v_mfma_f32_16x16x16bf16_1k lane l supplies rows 4*(l/16)..+3 of
+// column l%16 of a 16-column block, for the A operand (J^T) and the B operand (J) alike, so
+// each lane computes exactly the J entries it feeds to the matrix cores: two pixels
+// (Jx and Jy rows of each) times one column of every block.  The entries are split into BF16
+// parts in registers and every upper-triangle tile is accumulated from registers; nothing is
+// staged through LDS and the loop has no barrier.  Accumulator layout: lane l, element r ->
+// C[4*(l/16) + r][l%16].
+template <int T>
+__global__ void __launch_bounds__(64) k_jtj_reg(Layout L, const FrameGPU* __restrict__ frames,
+                                                 const int* __restrict__ act, int n, const int* __restrict__ npix,
+                                                 const int4* __restrict__ pnb, const float* __restrict__ pcoef,
+                                                 const float4* __restrict__ pixg, const float* __restrict__ dI, int S,
+                                                 int nsplit, float* __restrict__ Hp, float* __restrict__ gp)
 {
-    const int f = blockIdx.y, s = blockIdx.x;
-    if (mask && !mask[f]) return;
+    constexpr int NT = T * (T + 1) / 2;
+    const int s = blockIdx.x;
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const int f = fl_frame(act, yi);
     const FrameGPU fr = frames[f];
     const int N = npix[f];
     const int K = L.K;
-    const int T = KP / 16;
-    const int ntiles = T * (T + 1) / 2;
     const int p0 = (int)((long long)N * s / S), p1 = (int)((long long)N * (s + 1) / S);
-    extern __shared__ unsigned short smem[];
-    unsigned short* sJh = smem;                         // [KP][MF_STRIDE]
-    unsigned short* sJm = smem + KP * MF_STRIDE;        // [KP][MF_STRIDE]
-    unsigned short* sJl = smem + 2 * KP * MF_STRIDE;    // [KP][MF_STRIDE] (three-term split only)
-    __shared__ PixCoef spc[MF_CH];
+    const int lane = threadIdx.x & 63, grp = lane >> 4, col = lane & 15;
     const float* dIf = dI + (size_t)fr.pix_off * K;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int MAXT = 7;            // KP <= 112 -> at most 28 tiles, 7 per wave
-    v4f acc[MAXT];
-    int tI[MAXT], tJ[MAXT];
-    float gacc[2] = {0.f, 0.f};        // J^T r (FP32) for columns lane and lane + 64
-    __shared__ float sg[4][128];
+    v4f acc[NT];
 #pragma unroll
-    for (int u = 0; u < MAXT; ++u) {
-        const int tile = wave + 4 * u;
-        int I = 0, rem = tile;
-        while (I < T && rem >= T - I) { rem -= T - I; ++I; }
-        tI[u] = I;
-        tJ[u] = I + rem;
-        acc[u] = (v4f){0.f, 0.f, 0.f, 0.f};
-    }
-    for (int c0 = p0; c0 < p1; c0 += MF_CH) {
-        if (tid < MF_CH) {
-            const int pos = c0 + tid;
-            spc[tid] = pix_coef(fr, pnb, pcoef, pixg, pos < p1 ? pos : N, N);
-        }
-        __syncthreads();
-        // staging: wave w takes pixels w, w+4, ...; lanes run over the columns, so the four
-        // neighbour rows of dI are read with unit stride
-#pragma unroll 2
-        for (int q = 0; q < MF_CH / 4; ++q) {
-            const int p = wave + 4 * q;
-            const PixCoef pc = spc[p];
+    for (int t = 0; t < NT; ++t) acc[t] = (v4f){0.f, 0.f, 0.f, 0.f};
+    float gacc[T];
 #pragma unroll
-            for (int hh = 0; hh < 2; ++hh) {
-                const int j = lane + 64 * hh;
-                if (j < KP) {
-                    float jx = 0.f, jy = 0.f;
-                    if (pc.valid) {
-                        if (j < K) {
-                            jrow(pc, dIf, K, j, jx, jy);
-                            gacc[hh] = fmaf(jx, pc.gsx, fmaf(jy, pc.gsy, gacc[hh]));   // J^T r in FP32
-                        } else if (j == K) {
-                            jx = pc.gsx;
-                            jy = pc.gsy;
-                        }
+    for (int c = 0; c < T; ++c) gacc[c] = 0.f;
+
+    for (int kb = p0; kb < p1; kb += 8) {
+        float jv[T][4];                 // rows: (pixel a: Jx, Jy), (pixel b: Jx, Jy)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int pos = kb + 2 * grp + q;
+            if (pos < p1) {
+                const long long o = fr.pix_off + pos;
+                const int4 nb = pnb[o];
+                const float c = pcoef[o];
+                const float4 gg = pixg[o];
+                const float dx = 2.f * gg.x, dy = 2.f * gg.y;
+                const float* rl = dIf + (size_t)nb.x * K;
+                const float* rr = dIf + (size_t)nb.y * K;
+                const float* ra = dIf + (size_t)nb.z * K;
+                const float* rb = dIf + (size_t)nb.w * K;
+#pragma unroll
+                for (int cb = 0; cb < T; ++cb) {
+                    const int j = 16 * cb + col;
+                    float vx = 0.f, vy = 0.f;
+                    if (j < K) {
+                        const float dh = rr[j] - rl[j];
+                        const float dv = rb[j] - ra[j];
+                        const float sv = dy * dh - dx * dv;
+                        vx = c * dy * sv;
+                        vy = -c * dx * sv;
+                        gacc[cb] = fmaf(vx, gg.z, fmaf(vy, gg.w, gacc[cb]));   // J^T r in FP32
+                    } else if (j == K) {
+                        vx = gg.z;
+                        vy = gg.w;
                     }
-                    const unsigned short hx = bf16_rne(jx), hy = bf16_rne(jy);
-                    const float rx = jx - bf16_f(hx), ry = jy - bf16_f(hy);
-                    const unsigned short mx = bf16_rne(rx), my = bf16_rne(ry);
-                    const int o = j * MF_STRIDE + 2 * p;
-                    sJh[o] = hx;
-                    sJh[o + 1] = hy;
-                    sJm[o] = mx;
-                    sJm[o + 1] = my;
-                    if (nsplit == 3) {
-                        sJl[o] = bf16_rne(rx - bf16_f(mx));
-                        sJl[o + 1] = bf16_rne(ry - bf16_f(my));
-                    }
+                    jv[cb][2 * q] = vx;
+                    jv[cb][2 * q + 1] = vy;
                 }
+            } else {
+#pragma unroll
+                for (int cb = 0; cb < T; ++cb) jv[cb][2 * q] = jv[cb][2 * q + 1] = 0.f;
             }
         }
-        __syncthreads();
-        const int col = lane & 15, kq = 4 * (lane >> 4);
+        v4s fh[T], fm[T], fl[T];
 #pragma unroll
-        for (int ks = 0; ks < MF_ROWS; ks += 16) {
+        for (int cb = 0; cb < T; ++cb) {
 #pragma unroll
-            for (int u = 0; u < MAXT; ++u) {
-                if (wave + 4 * u >= ntiles) continue;
-                const int ca = (tI[u] * 16 + col) * MF_STRIDE + ks + kq;
-                const int cb = (tJ[u] * 16 + col) * MF_STRIDE + ks + kq;
-                const v4s ah = *(const v4s*)(sJh + ca), am = *(const v4s*)(sJm + ca);
-                const v4s bh = *(const v4s*)(sJh + cb), bm = *(const v4s*)(sJm + cb);
-                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[u], 0, 0, 0);
-                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bm, acc[u], 0, 0, 0);
-                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, acc[u], 0, 0, 0);
+            for (int r = 0; r < 4; ++r) {
+                const float x = jv[cb][r];
+                const unsigned short h = bf16_rne(x);
+                const float rx = x - bf16_f(h);
+                const unsigned short mdl = bf16_rne(rx);
+                fh[cb][r] = (short)h;
+                fm[cb][r] = (short)mdl;
+                fl[cb][r] = (short)bf16_rne(rx - bf16_f(mdl));
+            }
+        }
+        int t = 0;
+#pragma unroll
+        for (int I = 0; I < T; ++I) {
+#pragma unroll
+            for (int J = I; J < T; ++J, ++t) {
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fh[I], fh[J], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fh[I], fm[J], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fm[I], fh[J], acc[t], 0, 0, 0);
                 if (nsplit == 3) {
-                    const v4s al = *(const v4s*)(sJl + ca), bl = *(const v4s*)(sJl + cb);
-                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[u], 0, 0, 0);
-                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[u], 0, 0, 0);
-                    acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm, acc[u], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fh[I], fl[J], acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fl[I], fh[J], acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fm[I], fm[J], acc[t], 0, 0, 0);
                 }
             }
         }
-        __syncthreads();
     }
-    // J^T r partials: sum the four waves' column sums
-    sg[wave][lane] = gacc[0];
-    sg[wave][lane + 64] = gacc[1];
-    __syncthreads();
-    if (tid < KP) gp[((size_t)f * S + s) * KP + tid] = sg[0][tid] + sg[1][tid] + sg[2][tid] + sg[3][tid];
-    // accumulator layout of 16x16x16: lane l, element r -> C[4*(l/16) + r][l%16]
-    float* Hout = Hp + ((size_t)f * S + s) * ntiles * 256;
+    // J^T r: sum the four row groups of each column, lanes 0..15 write
+    const int KP = 16 * T;
 #pragma unroll
-    for (int u = 0; u < MAXT; ++u) {
-        const int tile = wave + 4 * u;
-        if (tile >= ntiles) continue;
+    for (int cb = 0; cb < T; ++cb) {
+        float v = gacc[cb];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (grp == 0) gp[((size_t)f * S + s) * KP + 16 * cb + col] = v;
+    }
+    float* Hout = Hp + ((size_t)f * S + s) * NT * 256;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 4 * (lane >> 4) + r, cc = lane & 15;
-            Hout[(size_t)tile * 256 + row * 16 + cc] = acc[u][r];
-        }
+    for (int tt = 0; tt < NT; ++tt) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Hout[(size_t)tt * 256 + (4 * grp + r) * 16 + col] = acc[tt][r];
+    }
     }
 }
 
@@ -482,7 +488,7 @@ int jtj_splits(int n, int per_frame_pixels_hint)
     return S;
 }
 
-int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask, const float* theta,
+int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
               const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
               const float* bary, const float4* pixg, float* dI, int max_area, hipStream_t s)
 {
@@ -491,10 +497,10 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n,
         return TDI_ERR_ARG;
     }
     int bx = (max_area / 2 + 15) / 16;
-    if (bx > 2048) bx = 2048;
+    if (bx > 512) bx = 512;
     if (bx < 1) bx = 1;
-    hipLaunchKernelGGL(k_dI, dim3(bx, n), dim3(256), 0, s, m, L, frames, mask, theta, P0, P, vis_list, npix,
-                       (const unsigned long long*)keys, (const float2*)bary, pixg, dI);
+    hipLaunchKernelGGL(k_dI, dim3(bx, grid_frames(fl.n, 64)), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0,
+                       P, vis_list, npix, (const unsigned long long*)keys, (const float2*)bary, pixg, dI);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -502,18 +508,20 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n,
 size_t jtj_partials_floats(int n, int S, int K, bool mfma)
 {
     if (mfma) {
+        // the register kernel runs JTJ_WAVES_PER_SPLIT one-wave workgroups per split
         const int KP = ((K + 1 + 15) / 16) * 16;
         const int T = KP / 16;
-        return (size_t)n * S * (T * (T + 1) / 2) * 256 + (size_t)n * S * KP;
+        const size_t SP = (size_t)S * JTJ_WAVES_PER_SPLIT;
+        return (size_t)n * SP * (T * (T + 1) / 2) * 256 + (size_t)n * SP * KP;
     }
     const int KP = ((K + 3) / 4) * 4;
     const int T = KP / 4;
     return (size_t)n * S * (T * (T + 1) / 2) * 16 + (size_t)n * S * KP;
 }
 
-int launch_jtj_f32(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
-                   const int* npix, const int4* pnb, const float* pcoef, const float4* pixg, const float* dI, float w,
-                   float* H, float* g, float* partials, int S, hipStream_t s)
+int launch_jtj_f32(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
+                   const float* pcoef, const float4* pixg, const float* dI, float w, float* H, float* g,
+                   float* partials, int S, hipStream_t s)
 {
     const int K = L.K;
     const int KP = ((K + 3) / 4) * 4;
@@ -522,19 +530,21 @@ int launch_jtj_f32(const Layout& L, const FrameGPU* frames, int n, const int* ma
         return TDI_ERR_ARG;
     }
     const int T = KP / 4;
+    const int n = fl.n;
     float* Hp = partials;
     float* gp = partials + (size_t)n * S * (T * (T + 1) / 2) * 16;
-    hipLaunchKernelGGL(k_jtj_f32, dim3(S, n), dim3(256), 0, s, L, frames, mask, vis_list, npix, pnb, pcoef, pixg, dI, S,
+    const int gy = grid_frames(n, 1 << 16);
+    hipLaunchKernelGGL(k_jtj_f32, dim3(S, gy), dim3(256), 0, s, L, frames, fl.act, n, npix, pnb, pcoef, pixg, dI, S,
                        KP, Hp, gp);
     int bx = (K * K + K + 255) / 256;
-    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, n), dim3(256), 0, s, mask, K, KP, 4, S, Hp, gp, w, H, g, 0);
+    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, gy), dim3(256), 0, s, fl.act, n, K, KP, 4, S, Hp, gp, w, H, g, 0);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
 
-int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, int n, const int* mask, const int* vis_list,
-                    const int* npix, const int4* pnb, const float* pcoef, const float4* pixg, const float* dI, float w,
-                    float* H, float* g, float* partials, int S, int g_on_mfma, int two_term, hipStream_t s)
+int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
+                    const float* pcoef, const float4* pixg, const float* dI, float w, float* H, float* g,
+                    float* partials, int S, int g_on_mfma, int two_term, hipStream_t s)
 {
     const int K = L.K;
     const int KP = ((K + 1 + 15) / 16) * 16;
@@ -543,13 +553,24 @@ int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, int n, const int* m
         return TDI_ERR_ARG;
     }
     const int nsplit = two_term ? 2 : 3;
-    const size_t lds = (size_t)nsplit * KP * MF_STRIDE * sizeof(unsigned short);
     const int T = KP / 16;
-    float* gp = partials + (size_t)n * S * (T * (T + 1) / 2) * 256;
-    hipLaunchKernelGGL(k_jtj_mfma, dim3(S, n), dim3(256), lds, s, L, frames, mask, vis_list, npix, pnb, pcoef, pixg, dI,
-                       S, KP, nsplit, partials, gp);
+    const int n = fl.n;
+    const int SP = S * JTJ_WAVES_PER_SPLIT;
+    float* gp = partials + (size_t)n * SP * (T * (T + 1) / 2) * 256;
+    const int gy = grid_frames(n, 1 << 16);     // a row of split waves per frame: J^T J needs the occupancy
+    const dim3 grid(SP, gy);
+    const int* act = fl.act;
+    switch (T) {
+    case 1: hipLaunchKernelGGL(k_jtj_reg<1>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
+    case 2: hipLaunchKernelGGL(k_jtj_reg<2>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
+    case 3: hipLaunchKernelGGL(k_jtj_reg<3>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
+    case 4: hipLaunchKernelGGL(k_jtj_reg<4>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
+    case 5: hipLaunchKernelGGL(k_jtj_reg<5>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
+    case 6: hipLaunchKernelGGL(k_jtj_reg<6>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
+    default: hipLaunchKernelGGL(k_jtj_reg<7>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
+    }
     int bx = (K * K + K + 255) / 256;
-    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, n), dim3(256), 0, s, mask, K, KP, 16, S, partials, gp, w, H, g,
+    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, gy), dim3(256), 0, s, act, n, K, KP, 16, SP, partials, gp, w, H, g,
                        g_on_mfma);
     TDI_HIP(hipGetLastError());
     return TDI_OK;

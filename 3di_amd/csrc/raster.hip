@@ -14,13 +14,13 @@
 //               The winner per pixel is therefore unique and deterministic: minimum depth,
 //               then lowest triangle id (the reference keeps every exact-depth tie and lets
 //               them race in the scatter, funcs.cu:816,1027).
-//   k_resolve — one workgroup per frame walks the crop in column-major order, 4 pixels per
-//               lane; three wavefront ballots (one per bit of the lane's 0..4 count) and
-//               popcounts give each occupied pixel its rank, which is at once the
+//   k_resolve_count/k_resolve_write — the crop in column-major order, 1024-pixel tiles, 4
+//               pixels per lane; three wavefront ballots (one per bit of the lane's 0..4
+//               count) and popcounts give each occupied pixel its rank, which is at once the
 //               inclusive occupancy prefix sum (the reference's cumM0), the position in the
 //               sorted visible-pixel list (the reference's sort_by_key output) and the slot
-//               of its barycentric weights.  No sort is needed because the crop is walked in
-//               pixel-index order.
+//               of its barycentric weights.  No sort is needed because pixels are ranked in
+//               pixel-index order; tile offsets come from a first counting pass.
 #include "common.h"
 
 namespace tdi {
@@ -34,105 +34,159 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, long long n, uint64_t v)
 }
 
 __global__ void __launch_bounds__(256) k_raster(const int* __restrict__ tris, int NT, int V,
-                                                 const FrameGPU* __restrict__ frames, const int* __restrict__ mask,
-                                                 const float4* __restrict__ vproj, int ntmp,
+                                                 const FrameGPU* __restrict__ frames, const int* __restrict__ act,
+                                                 int n, const float4* __restrict__ vproj, int ntmp,
                                                  unsigned long long* __restrict__ keys)
 {
-    const int f = blockIdx.y;
-    if (mask && !mask[f]) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= NT) return;
-    const FrameGPU fr = frames[f];
-    const float4* vp = vproj + (size_t)f * V;
     const int ib = tris[3 * t], ic = tris[3 * t + 1], ia = tris[3 * t + 2];
-    const float4 B = vp[ib], Cc = vp[ic], A = vp[ia];
-    const float xmin = floorf(fminf(fminf(B.x, Cc.x), A.x));
-    const float xmax = ceilf(fmaxf(fmaxf(B.x, Cc.x), A.x));
-    const float ymin = floorf(fminf(fminf(B.y, Cc.y), A.y));
-    const float ymax = ceilf(fmaxf(fmaxf(B.y, Cc.y), A.y));
-    if (!(xmin == xmin) || !(xmax == xmax) || !(ymin == ymin) || !(ymax == ymax)) return;
-    // clip the scan to the crop window [x0, x0+W) x [y0, y0+H)
-    const float xlo = fmaxf(xmin, (float)fr.x0), xhi = fminf(xmax, (float)(fr.x0 + fr.W));
-    const float ylo = fmaxf(ymin, (float)fr.y0), yhi = fminf(ymax, (float)(fr.y0 + fr.H));
-    if (!(xlo < xhi) || !(ylo < yhi)) return;
-    const unsigned long long key = ((unsigned long long)orderable_bits(B.z) << 32) | (unsigned)t;
-    unsigned long long* kf = keys + fr.pix_off;
-    int covered = 0;
-    for (int x = (int)xlo; (float)x < xhi; ++x) {
-        for (int y = (int)ylo; (float)y < yhi; ++y) {
-            float al, be;
-            if (!cover_rn(B.x, B.y, Cc.x, Cc.y, A.x, A.y, x, y, &al, &be)) continue;
-            if (ntmp > 0 && covered >= ntmp) continue;
-            ++covered;
-            atomicMin(kf + (size_t)(x - fr.x0) * fr.H + (y - fr.y0), key);
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        const float4* vp = vproj + (size_t)f * V;
+        const float4 B = vp[ib], Cc = vp[ic], A = vp[ia];
+        const float xmin = floorf(fminf(fminf(B.x, Cc.x), A.x));
+        const float xmax = ceilf(fmaxf(fmaxf(B.x, Cc.x), A.x));
+        const float ymin = floorf(fminf(fminf(B.y, Cc.y), A.y));
+        const float ymax = ceilf(fmaxf(fmaxf(B.y, Cc.y), A.y));
+        if (!(xmin == xmin) || !(xmax == xmax) || !(ymin == ymin) || !(ymax == ymax)) continue;
+        // clip the scan to the crop window [x0, x0+W) x [y0, y0+H)
+        const float xlo = fmaxf(xmin, (float)fr.x0), xhi = fminf(xmax, (float)(fr.x0 + fr.W));
+        const float ylo = fmaxf(ymin, (float)fr.y0), yhi = fminf(ymax, (float)(fr.y0 + fr.H));
+        if (!(xlo < xhi) || !(ylo < yhi)) continue;
+        const unsigned long long key = ((unsigned long long)orderable_bits(B.z) << 32) | (unsigned)t;
+        unsigned long long* kf = keys + fr.pix_off;
+        int covered = 0;
+        for (int x = (int)xlo; (float)x < xhi; ++x) {
+            for (int y = (int)ylo; (float)y < yhi; ++y) {
+                float al, be;
+                if (!cover_rn(B.x, B.y, Cc.x, Cc.y, A.x, A.y, x, y, &al, &be)) continue;
+                if (ntmp > 0 && covered >= ntmp) continue;
+                ++covered;
+                atomicMin(kf + (size_t)(x - fr.x0) * fr.H + (y - fr.y0), key);
+            }
         }
     }
 }
 
-constexpr int RES_THREADS = 1024;
+// Resolve in two passes over 1024-pixel tiles so that every tile of every listed frame is a
+// workgroup of its own (a frame's crop is not walked by one workgroup in sequence):
+//   k_resolve_count: occupied pixels per tile -> tcount[f][tile];
+//   k_resolve_write: tile offset = sum of the preceding tiles' counts, then the ballot
+//                    ranks inside the tile give cum, the sorted visible list and the slot
+//                    of the barycentrics; the last tile writes the frame's pixel count.
+constexpr int RES_THREADS = 256;
 constexpr int RES_PER_THREAD = 4;
 constexpr int RES_TILE = RES_THREADS * RES_PER_THREAD;
+constexpr int RES_NW = RES_THREADS / 64;
 
-__global__ void __launch_bounds__(RES_THREADS) k_resolve(const int* __restrict__ tris, int V,
-                                                          const FrameGPU* __restrict__ frames,
-                                                          const int* __restrict__ mask,
-                                                          const float4* __restrict__ vproj,
-                                                          const unsigned long long* __restrict__ keys,
-                                                          int* __restrict__ cum, int* __restrict__ vis,
-                                                          int* __restrict__ npix, float2* __restrict__ bary)
+// per-thread count (0..4) of this thread's four pixels and their keys
+__device__ __forceinline__ int tile_keys(const unsigned long long* __restrict__ kf, int k0, int A,
+                                         unsigned long long* key)
 {
-    const int f = blockIdx.x;
-    if (mask && !mask[f]) return;
-    const FrameGPU fr = frames[f];
-    const float4* vp = vproj + (size_t)f * V;
-    const long long base = fr.pix_off;
-    const int A = fr.W * fr.H;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int NW = RES_THREADS / 64;
-    __shared__ int swave[NW];
-    int running = 0;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int tile = 0; tile < A; tile += RES_TILE) {
-        const int k0 = tile + RES_PER_THREAD * tid;     // this thread's 4 consecutive pixels
-        unsigned long long key[RES_PER_THREAD];
-        int cnt = 0;
+    int cnt = 0;
 #pragma unroll
-        for (int q = 0; q < RES_PER_THREAD; ++q) {
-            key[q] = (k0 + q < A) ? keys[base + k0 + q] : KEY_EMPTY;
-            cnt += key[q] != KEY_EMPTY;
-        }
-        // wave prefix of the per-thread counts (0..4) from three ballots, one per count bit
-        const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
-        const int before = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
-        if (lane == 0) swave[wave] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
-        __syncthreads();
-        int woff = 0, total = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            woff += (w < wave) ? swave[w] : 0;
-            total += swave[w];
-        }
-        int pos = running + woff + before;
-#pragma unroll
-        for (int q = 0; q < RES_PER_THREAD; ++q) {
-            const int k = k0 + q;
-            if (k >= A) break;
-            const bool occ = key[q] != KEY_EMPTY;
-            cum[base + k] = pos + (occ ? 1 : 0);
-            if (occ) {
-                vis[base + pos] = k;
-                const int t = (int)(unsigned)(key[q] & 0xffffffffull);
-                const float4 B = vp[tris[3 * t]], Cc = vp[tris[3 * t + 1]], Av = vp[tris[3 * t + 2]];
-                float al, be;
-                cover_rn(B.x, B.y, Cc.x, Cc.y, Av.x, Av.y, fr.x0 + k / fr.H, fr.y0 + k % fr.H, &al, &be);
-                bary[base + k] = make_float2(al, be);
-                ++pos;
-            }
-        }
-        running += total;
-        __syncthreads();
+    for (int q = 0; q < RES_PER_THREAD; ++q) {
+        key[q] = (k0 + q < A) ? kf[k0 + q] : KEY_EMPTY;
+        cnt += key[q] != KEY_EMPTY;
     }
-    if (tid == 0) npix[f] = running;
+    return cnt;
+}
+
+__global__ void __launch_bounds__(RES_THREADS) k_resolve_count(const FrameGPU* __restrict__ frames,
+                                                                const int* __restrict__ act, int n,
+                                                                const unsigned long long* __restrict__ keys,
+                                                                int* __restrict__ tcount, int tmax)
+{
+    __shared__ int swave[RES_NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        const int A = fr.W * fr.H;
+        const int ntile = (A + RES_TILE - 1) / RES_TILE;
+        for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+            unsigned long long key[RES_PER_THREAD];
+            const int cnt = tile_keys(keys + fr.pix_off, tile * RES_TILE + RES_PER_THREAD * tid, A, key);
+            const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
+            if (lane == 0) swave[wave] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+            __syncthreads();
+            if (tid == 0) {
+                int t = 0;
+#pragma unroll
+                for (int w = 0; w < RES_NW; ++w) t += swave[w];
+                tcount[(size_t)f * tmax + tile] = t;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(RES_THREADS) k_resolve_write(const int* __restrict__ tris, int V,
+                                                                const FrameGPU* __restrict__ frames,
+                                                                const int* __restrict__ act, int n,
+                                                                const float4* __restrict__ vproj,
+                                                                const unsigned long long* __restrict__ keys,
+                                                                const int* __restrict__ tcount, int tmax,
+                                                                int* __restrict__ cum, int* __restrict__ vis,
+                                                                int* __restrict__ npix, float2* __restrict__ bary)
+{
+    __shared__ int swave[RES_NW];
+    __shared__ int soff[RES_NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        const float4* vp = vproj + (size_t)f * V;
+        const long long base = fr.pix_off;
+        const int A = fr.W * fr.H;
+        const int ntile = (A + RES_TILE - 1) / RES_TILE;
+        for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+            // offset of this tile: the preceding tiles' counts
+            int part = 0;
+            for (int j = tid; j < tile; j += RES_THREADS) part += tcount[(size_t)f * tmax + j];
+            for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+            if (lane == 0) soff[wave] = part;
+            unsigned long long key[RES_PER_THREAD];
+            const int k0 = tile * RES_TILE + RES_PER_THREAD * tid;
+            const int cnt = tile_keys(keys + base, k0, A, key);
+            const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
+            const int before = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
+            if (lane == 0) swave[wave] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+            __syncthreads();
+            int off = 0, woff = 0, total = 0;
+#pragma unroll
+            for (int w = 0; w < RES_NW; ++w) {
+                off += soff[w];
+                woff += (w < wave) ? swave[w] : 0;
+                total += swave[w];
+            }
+            int pos = off + woff + before;
+#pragma unroll
+            for (int q = 0; q < RES_PER_THREAD; ++q) {
+                const int k = k0 + q;
+                if (k >= A) break;
+                const bool occ = key[q] != KEY_EMPTY;
+                cum[base + k] = pos + (occ ? 1 : 0);
+                if (occ) {
+                    vis[base + pos] = k;
+                    const int t = (int)(unsigned)(key[q] & 0xffffffffull);
+                    const float4 B = vp[tris[3 * t]], Cc = vp[tris[3 * t + 1]], Av = vp[tris[3 * t + 2]];
+                    float al, be;
+                    cover_rn(B.x, B.y, Cc.x, Cc.y, Av.x, Av.y, fr.x0 + k / fr.H, fr.y0 + k % fr.H, &al, &be);
+                    bary[base + k] = make_float2(al, be);
+                    ++pos;
+                }
+            }
+            if (tile == ntile - 1 && tid == 0) npix[f] = off + total;
+            __syncthreads();
+        }
+    }
 }
 
 } // namespace
@@ -147,43 +201,58 @@ int launch_clear_keys(uint64_t* keys, long long total, hipStream_t s)
 }
 
 namespace {
-__global__ void __launch_bounds__(256) k_clear_keys_masked(const FrameGPU* __restrict__ frames,
-                                                            const int* __restrict__ mask, uint64_t* __restrict__ keys)
+__global__ void __launch_bounds__(256) k_clear_keys_frames(const FrameGPU* __restrict__ frames,
+                                                            const int* __restrict__ act, int n,
+                                                            uint64_t* __restrict__ keys)
 {
-    const int f = blockIdx.y;
-    if (mask && !mask[f]) return;
-    const FrameGPU fr = frames[f];
-    const int A = fr.W * fr.H;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) keys[fr.pix_off + k] = KEY_EMPTY;
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const FrameGPU fr = frames[fl_frame(act, yi)];
+        const int A = fr.W * fr.H;
+        for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x)
+            keys[fr.pix_off + k] = KEY_EMPTY;
+    }
 }
 } // namespace
 
-int launch_clear_keys_masked(const FrameGPU* frames, int n, const int* mask, uint64_t* keys, int max_area, hipStream_t s)
+int launch_clear_keys_frames(const FrameGPU* frames, FrameList fl, uint64_t* keys, int max_area, hipStream_t s)
 {
     int bx = (max_area + 255) / 256;
-    if (bx > 1024) bx = 1024;
+    if (bx > 256) bx = 256;
     if (bx < 1) bx = 1;
-    hipLaunchKernelGGL(k_clear_keys_masked, dim3(bx, n), dim3(256), 0, s, frames, mask, keys);
+    hipLaunchKernelGGL(k_clear_keys_frames, dim3(bx, grid_frames(fl.n, 64)), dim3(256), 0, s, frames, fl.act, fl.n,
+                       keys);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
 
-int launch_raster(const ModelGPU& m, const FrameGPU* frames, int n, const int* mask, const float4* vproj, int ntmp,
+int launch_raster(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj, int ntmp,
                   uint64_t* keys, hipStream_t s)
 {
-    dim3 grid((m.NT + 255) / 256, n);
-    hipLaunchKernelGGL(k_raster, grid, dim3(256), 0, s, m.tris, m.NT, m.V, frames, mask, vproj, ntmp,
+    dim3 grid((m.NT + 255) / 256, grid_frames(fl.n, 64));
+    hipLaunchKernelGGL(k_raster, grid, dim3(256), 0, s, m.tris, m.NT, m.V, frames, fl.act, fl.n, vproj, ntmp,
                        (unsigned long long*)keys);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
 
-int launch_resolve(const ModelGPU& m, const FrameGPU* frames, int n, const int* mask, const float4* vproj,
-                   const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int /*max_area*/,
-                   hipStream_t s)
+int resolve_tiles(int max_area) { return (max_area + RES_TILE - 1) / RES_TILE; }
+
+int launch_resolve(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj,
+                   const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int max_area,
+                   int* tcount, int tmax, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_resolve, dim3(n), dim3(RES_THREADS), 0, s, m.tris, m.V, frames, mask, vproj,
-                       (const unsigned long long*)keys, cum, vis_list, npix, (float2*)bary);
+    int bx = resolve_tiles(max_area);
+    if (bx > tmax) {
+        set_error("resolve: crop of %d pixels needs %d tiles > %d", max_area, bx, tmax);
+        return TDI_ERR_ARG;
+    }
+    if (bx < 1) bx = 1;
+    const dim3 grid(bx, grid_frames(fl.n, 64));
+    hipLaunchKernelGGL(k_resolve_count, grid, dim3(RES_THREADS), 0, s, frames, fl.act, fl.n,
+                       (const unsigned long long*)keys, tcount, tmax);
+    hipLaunchKernelGGL(k_resolve_write, grid, dim3(RES_THREADS), 0, s, m.tris, m.V, frames, fl.act, fl.n, vproj,
+                       (const unsigned long long*)keys, tcount, tmax, cum, vis_list, npix, (float2*)bary);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

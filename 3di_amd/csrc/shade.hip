@@ -21,26 +21,29 @@ struct Taps {
 
 namespace {
 
-__global__ void __launch_bounds__(256) k_clear_crops(const FrameGPU* __restrict__ frames, const int* __restrict__ mask,
-                                                      float* __restrict__ img)
+__global__ void __launch_bounds__(256) k_clear_crops(const FrameGPU* __restrict__ frames, const int* __restrict__ act,
+                                                      int n, float* __restrict__ img)
 {
-    const int f = blockIdx.y;
-    if (mask && !mask[f]) return;
-    const FrameGPU fr = frames[f];
-    const int A = fr.W * fr.H;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) img[fr.pix_off + k] = 0.f;
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const FrameGPU fr = frames[fl_frame(act, yi)];
+        const int A = fr.W * fr.H;
+        for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x)
+            img[fr.pix_off + k] = 0.f;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
-                                                const int* __restrict__ mask, const float* __restrict__ theta,
+                                                const int* __restrict__ act, int n, const float* __restrict__ theta,
                                                 const float* __restrict__ P0, const float* __restrict__ P,
                                                 const int* __restrict__ vis, const int* __restrict__ npix,
                                                 const unsigned long long* __restrict__ keys,
                                                 const float2* __restrict__ bary, float* __restrict__ img)
 {
-    const int f = blockIdx.y;
-    if (mask && !mask[f]) return;
     __shared__ float sR[9], sdl[3], sL, stex[512];
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const int f = fl_frame(act, yi);
     const float* tf = theta + (size_t)f * L.K;
     const FrameGPU fr = frames[f];
     if (threadIdx.x == 0) {
@@ -90,15 +93,17 @@ __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const Frame
         const float rho = al * rb + be * rc + ga * ra;
         img[base + k] = rho + sL * rho * Id;
     }
+    __syncthreads();   // the frame's shared constants are rewritten by the next iteration
+    }
 }
 
 // 15-tap separable blur, zero outside the crop.  dir 0 = along y (contiguous), 1 = along x.
-__global__ void __launch_bounds__(256) k_blur(const FrameGPU* __restrict__ frames, const int* __restrict__ mask,
+__global__ void __launch_bounds__(256) k_blur(const FrameGPU* __restrict__ frames, const int* __restrict__ act, int n,
                                                Taps taps, const float* __restrict__ in, float* __restrict__ out, int dir)
 {
-    const int f = blockIdx.y;
-    if (mask && !mask[f]) return;
-    const FrameGPU fr = frames[f];
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const FrameGPU fr = frames[fl_frame(act, yi)];
     const int W = fr.W, H = fr.H, A = W * H;
     const float* src = in + fr.pix_off;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
@@ -119,6 +124,7 @@ __global__ void __launch_bounds__(256) k_blur(const FrameGPU* __restrict__ frame
         }
         out[fr.pix_off + k] = s;
     }
+    }
 }
 
 __device__ __forceinline__ void nbr_index(int k, int W, int H, int& kr, int& kl, int& ka, int& kb)
@@ -133,15 +139,17 @@ __device__ __forceinline__ void nbr_index(int k, int W, int H, int& kr, int& kl,
 // Gradients of the blurred rendered crop at every visible pixel, the objective
 // -w sum g^_r.g^_s accumulated in FP64, and the per-pixel records (gx, gy, g^s_x, g^s_y),
 // neighbour positions and Jacobian coefficient used by the normal equations.
-__global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ frames, const int* __restrict__ mask,
+__global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ frames, const int* __restrict__ act, int n,
                                                 const int* __restrict__ vis, const int* __restrict__ npix,
                                                 const int* __restrict__ cum,
                                                 const float* __restrict__ blurred, const float2* __restrict__ obs_g,
                                                 float w, float4* __restrict__ pixg, int4* __restrict__ pnb,
                                                 float* __restrict__ pcoef, double* __restrict__ obj)
 {
-    const int f = blockIdx.y;
-    if (mask && !mask[f]) return;
+    __shared__ double sw[4];
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const int f = fl_frame(act, yi);
     const FrameGPU fr = frames[f];
     const int N = npix[f];
     const long long base = fr.pix_off;
@@ -164,7 +172,6 @@ __global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ fram
     }
     // block reduction (wave shuffles, then LDS)
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-    __shared__ double sw[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) sw[wave] = acc;
     __syncthreads();
@@ -172,37 +179,40 @@ __global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ fram
         const double s = sw[0] + sw[1] + sw[2] + sw[3];
         if (s != 0.0) atomicAdd(obj + f, -(double)w * s);
     }
+    __syncthreads();   // sw is rewritten by the next iteration
+    }
 }
 
 // Observation: blur once, then normalised gradients at every crop pixel.
-__global__ void __launch_bounds__(256) k_obs_grad(const FrameGPU* __restrict__ frames, const float* __restrict__ blurred,
+__global__ void __launch_bounds__(256) k_obs_grad(const FrameGPU* __restrict__ frames, int n, const float* __restrict__ blurred,
                                                    float2* __restrict__ obs_g)
 {
-    const int f = blockIdx.y;
-    const FrameGPU fr = frames[f];
-    const int A = fr.W * fr.H;
-    const float* S = blurred + fr.pix_off;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
-        int kr, kl, ka, kb;
-        nbr_index(k, fr.W, fr.H, kr, kl, ka, kb);
-        const float sx = 0.5f * (S[kr] - S[kl]);
-        const float sy = 0.5f * (S[kb] - S[ka]);
-        const float hs = 1e-16f + sqrtf(4.f * sx * sx + 4.f * sy * sy);
-        obs_g[fr.obs_off + k] = make_float2(2.f * sx / hs, 2.f * sy / hs);
+    for (int f = blockIdx.y; f < n; f += gridDim.y) {
+        const FrameGPU fr = frames[f];
+        const int A = fr.W * fr.H;
+        const float* S = blurred + fr.pix_off;
+        for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+            int kr, kl, ka, kb;
+            nbr_index(k, fr.W, fr.H, kr, kl, ka, kb);
+            const float sx = 0.5f * (S[kr] - S[kl]);
+            const float sy = 0.5f * (S[kb] - S[ka]);
+            const float hs = 1e-16f + sqrtf(4.f * sx * sx + 4.f * sy * sy);
+            obs_g[fr.obs_off + k] = make_float2(2.f * sx / hs, 2.f * sy / hs);
+        }
     }
 }
 
 inline dim3 crop_grid(int max_area, int n)
 {
     int bx = (max_area + 255) / 256;
-    if (bx > 1024) bx = 1024;
+    if (bx > 256) bx = 256;
     if (bx < 1) bx = 1;
-    return dim3(bx, n);
+    return dim3(bx, grid_frames(n, 64));
 }
 
 } // namespace
 
-int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask, const float* theta,
+int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
                  const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
                  const float* bary, float* img, int max_area, hipStream_t s)
 {
@@ -210,30 +220,30 @@ int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int
         set_error("shade: k_tex %d > 512", L.ntex);
         return TDI_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_clear_crops, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, img);
-    hipLaunchKernelGGL(k_shade, crop_grid(max_area, n), dim3(256), 0, s, m, L, frames, mask, theta, P0, P, vis_list,
-                       npix, (const unsigned long long*)keys, (const float2*)bary, img);
+    hipLaunchKernelGGL(k_clear_crops, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, img);
+    hipLaunchKernelGGL(k_shade, crop_grid(max_area, fl.n), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0, P,
+                       vis_list, npix, (const unsigned long long*)keys, (const float2*)bary, img);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
 
-int launch_blur(const FrameGPU* frames, int n, const int* mask, const float* taps, const float* in, float* tmp,
+int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const float* in, float* tmp,
                 float* out, int max_area, hipStream_t s)
 {
     Taps t;
     for (int i = 0; i < 15; ++i) t.t[i] = taps[i];
-    hipLaunchKernelGGL(k_blur, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, t, in, tmp, 0);
-    hipLaunchKernelGGL(k_blur, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, t, tmp, out, 1);
+    hipLaunchKernelGGL(k_blur, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, t, in, tmp, 0);
+    hipLaunchKernelGGL(k_blur, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, t, tmp, out, 1);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
 
-int launch_grads(const FrameGPU* frames, int n, const int* mask, const int* vis_list, const int* npix,
+int launch_grads(const FrameGPU* frames, FrameList fl, const int* vis_list, const int* npix,
                  const int* cum, const float* blurred, const float2* obs_g, float w, float4* pixg, int4* pnb,
                  float* pcoef, double* obj, int max_area, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_grads, crop_grid(max_area, n), dim3(256), 0, s, frames, mask, vis_list, npix, cum, blurred,
-                       obs_g, w, pixg, pnb, pcoef, obj);
+    hipLaunchKernelGGL(k_grads, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, vis_list, npix, cum,
+                       blurred, obs_g, w, pixg, pnb, pcoef, obj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -241,9 +251,9 @@ int launch_grads(const FrameGPU* frames, int n, const int* mask, const int* vis_
 int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const float* obs, float* tmp, float* blurred,
                        float2* obs_g, int max_area, hipStream_t s)
 {
-    int rc = launch_blur(frames, n, nullptr, taps, obs, tmp, blurred, max_area, s);
+    int rc = launch_blur(frames, all_frames(n), taps, obs, tmp, blurred, max_area, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_obs_grad, crop_grid(max_area, n), dim3(256), 0, s, frames, blurred, obs_g);
+    hipLaunchKernelGGL(k_obs_grad, crop_grid(max_area, n), dim3(256), 0, s, frames, n, blurred, obs_g);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

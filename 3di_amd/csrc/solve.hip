@@ -27,13 +27,14 @@ constexpr int BMAXK = 112;
 // mode 0: objective only -> obj[f] = objb[f] = barrier value (FP64)
 // mode 1: derivatives only -> H[f] += barrier Hessian, g[f] += barrier gradient
 __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
-                                                  const int* __restrict__ mask, const float* __restrict__ theta,
+                                                  const int* __restrict__ act, int n, const float* __restrict__ theta,
                                                   LmkArrays lm, int use_lmk, int mode, double* __restrict__ obj,
                                                   double* __restrict__ objb, float* __restrict__ H,
                                                   float* __restrict__ g)
 {
-    const int f = blockIdx.x;
-    if (mask && !mask[f]) return;
+    // one workgroup per listed frame (the grid has a workgroup per slot)
+    if ((int)blockIdx.x >= fl_count(act, n)) return;
+    const int f = fl_frame(act, blockIdx.x);
     const int K = L.K;
     const FrameGPU fr = frames[f];
     const float* tf = theta + (size_t)f * K;
@@ -232,28 +233,11 @@ __global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict
     const int tid = threadIdx.x;
     const int np = K * (K + 1) / 2;
     if (tid == 0) { sflag[0] = 0.0; sflagf[0] = 0.f; }
-    if (mode == 0) {
-        double* A = dsm;
-        for (int e = tid; e < np; e += blockDim.x) {
-            int i = (int)((sqrtf(8.f * e + 1.f) - 1.f) * 0.5f);
-            while ((i + 1) * (i + 2) / 2 <= e) ++i;
-            while (i * (i + 1) / 2 > e) --i;
-            const int j = e - i * (i + 1) / 2;
-            A[e] = (double)Hf[(size_t)i * K + j];
-        }
-        for (int i = tid; i < K; i += blockDim.x) sb[i] = -(double)gf[i];
-        __syncthreads();
-        if (!chol_packed<double>(A, K, sflag)) {
-            if (tid == 0) status[f] = TDI_NOT_PD;
-            return;
-        }
-        chol_solve_packed<double>(A, K, sb);
-        for (int i = tid; i < K; i += blockDim.x) delta[(size_t)f * K + i] = (float)sb[i];
-        if (tid == 0) status[f] = TDI_OK;
-    } else {
+    if (mode != 0) {
         // FP32 factorisation of the equilibrated matrix D^-1/2 H D^-1/2 (D = diag H), which
         // removes the scale spread between pose and coefficient columns, then one FP64
-        // residual-refinement step on the original system.
+        // residual-refinement step on the original system.  A frame whose scaled matrix is
+        // not positive definite in FP32 falls through to the FP64 factorisation below.
         float* A = (float*)dsm;
         __shared__ float sdinv[CMAXK];
         for (int i = tid; i < K; i += blockDim.x) {
@@ -270,39 +254,57 @@ __global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict
         }
         for (int i = tid; i < K; i += blockDim.x) sbf[i] = -gf[i] * sdinv[i];
         __syncthreads();
-        if (!chol_packed<float>(A, K, sflagf)) {
-            if (tid == 0) status[f] = TDI_NOT_PD;
+        if (chol_packed<float>(A, K, sflagf)) {
+            chol_solve_packed<float>(A, K, sbf);
+            // x0 = D^-1/2 y
+            for (int i = tid; i < K; i += blockDim.x) sbf[i] *= sdinv[i];
+            __syncthreads();
+            // refinement: r = -g - H x0 in FP64, correction solved with the same scaled factor
+            for (int i = tid; i < K; i += blockDim.x) {
+                double r = -(double)gf[i];
+                for (int k = 0; k < K; ++k) r -= (double)Hf[(size_t)i * K + k] * (double)sbf[k];
+                sb[i] = r * (double)sdinv[i];
+            }
+            __syncthreads();
+            for (int i = 0; i < K; ++i) {
+                if (tid == 0) sb[i] /= (double)A[pk(i, i)];
+                __syncthreads();
+                const double yi = sb[i];
+                for (int k = i + 1 + tid; k < K; k += blockDim.x) sb[k] -= (double)A[pk(k, i)] * yi;
+                __syncthreads();
+            }
+            for (int i = K - 1; i >= 0; --i) {
+                if (tid == 0) sb[i] /= (double)A[pk(i, i)];
+                __syncthreads();
+                const double xi = sb[i];
+                for (int k = tid; k < i; k += blockDim.x) sb[k] -= (double)A[pk(i, k)] * xi;
+                __syncthreads();
+            }
+            for (int i = tid; i < K; i += blockDim.x)
+                delta[(size_t)f * K + i] = (float)((double)sbf[i] + sb[i] * (double)sdinv[i]);
+            if (tid == 0) status[f] = TDI_OK;
             return;
         }
-        chol_solve_packed<float>(A, K, sbf);
-        // x0 = D^-1/2 y
-        for (int i = tid; i < K; i += blockDim.x) sbf[i] *= sdinv[i];
         __syncthreads();
-        // refinement: r = -g - H x0 in FP64, correction solved with the same scaled factor
-        for (int i = tid; i < K; i += blockDim.x) {
-            double r = -(double)gf[i];
-            for (int k = 0; k < K; ++k) r -= (double)Hf[(size_t)i * K + k] * (double)sbf[k];
-            sb[i] = r * (double)sdinv[i];
-        }
-        __syncthreads();
-        for (int i = 0; i < K; ++i) {
-            if (tid == 0) sb[i] /= (double)A[pk(i, i)];
-            __syncthreads();
-            const double yi = sb[i];
-            for (int k = i + 1 + tid; k < K; k += blockDim.x) sb[k] -= (double)A[pk(k, i)] * yi;
-            __syncthreads();
-        }
-        for (int i = K - 1; i >= 0; --i) {
-            if (tid == 0) sb[i] /= (double)A[pk(i, i)];
-            __syncthreads();
-            const double xi = sb[i];
-            for (int k = tid; k < i; k += blockDim.x) sb[k] -= (double)A[pk(i, k)] * xi;
-            __syncthreads();
-        }
-        for (int i = tid; i < K; i += blockDim.x)
-            delta[(size_t)f * K + i] = (float)((double)sbf[i] + sb[i] * (double)sdinv[i]);
-        if (tid == 0) status[f] = TDI_OK;
     }
+    // FP64 factorisation and solve (mode 0, and the fallback of mode 1)
+    double* A = dsm;
+    for (int e = tid; e < np; e += blockDim.x) {
+        int i = (int)((sqrtf(8.f * e + 1.f) - 1.f) * 0.5f);
+        while ((i + 1) * (i + 2) / 2 <= e) ++i;
+        while (i * (i + 1) / 2 > e) --i;
+        const int j = e - i * (i + 1) / 2;
+        A[e] = (double)Hf[(size_t)i * K + j];
+    }
+    for (int i = tid; i < K; i += blockDim.x) sb[i] = -(double)gf[i];
+    __syncthreads();
+    if (!chol_packed<double>(A, K, sflag)) {
+        if (tid == 0) status[f] = TDI_NOT_PD;
+        return;
+    }
+    chol_solve_packed<double>(A, K, sb);
+    for (int i = tid; i < K; i += blockDim.x) delta[(size_t)f * K + i] = (float)sb[i];
+    if (tid == 0) status[f] = TDI_OK;
 }
 
 // ------------------------------------------------------------------ line-search state machine
@@ -413,12 +415,12 @@ __global__ void k_ls_decide(int n, int K, FitState* __restrict__ st, const int* 
 
 // "Face too large" test of compute_nonrigid_shape2 (renderer.cu:836-857) on projected
 // landmark vertices 0, 9 and 40.
-__global__ void k_face_check(int n, int V, const int* __restrict__ lmk_ids, const float4* __restrict__ vproj,
-                             float ref_face_size, const int* __restrict__ mask, int* __restrict__ face_ok)
+__global__ void k_face_check(int V, const int* __restrict__ lmk_ids, const float4* __restrict__ vproj,
+                             float ref_face_size, const int* __restrict__ act, int n, int* __restrict__ face_ok)
 {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= n) return;
-    if (mask && !mask[f]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fl_count(act, n)) return;
+    const int f = fl_frame(act, i);
     if (ref_face_size <= 0.f) { face_ok[f] = 1; return; }
     const float4* vp = vproj + (size_t)f * V;
     const float4 a = vp[lmk_ids[0]], b = vp[lmk_ids[9]], c = vp[lmk_ids[40]];
@@ -433,9 +435,41 @@ __global__ void k_copy_mask(int n, const FitState* __restrict__ st, int* __restr
     if (f < n) mask[f] = st[f].active;
 }
 
+// Stream compaction of a mask into a frame list, one 1024-thread workgroup: per 1024-entry
+// tile a wavefront ballot and popcount give each set entry its rank within the wave, the
+// wave totals in LDS give the wave offsets, and a running count carries across tiles.
+constexpr int CMP_THREADS = 1024;
+
+__global__ void __launch_bounds__(CMP_THREADS) k_compact_frames(int n, const int* __restrict__ mask,
+                                                                  int* __restrict__ act)
+{
+    constexpr int NW = CMP_THREADS / 64;
+    __shared__ int swave[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int running = 0;
+    for (int t0 = 0; t0 < n; t0 += CMP_THREADS) {
+        const int f = t0 + tid;
+        const int set = (f < n) && mask[f];
+        const unsigned long long b = __ballot(set);
+        if (lane == 0) swave[wave] = __popcll(b);
+        __syncthreads();
+        int off = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            off += (w < wave) ? swave[w] : 0;
+            total += swave[w];
+        }
+        if (set) act[1 + running + off + __popcll(b & lt)] = f;
+        running += total;
+        __syncthreads();
+    }
+    if (tid == 0) act[0] = running;
+}
+
 } // namespace
 
-int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const int* mask,
+int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                    const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
                    float* H, float* g, hipStream_t s)
 {
@@ -443,8 +477,8 @@ int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, i
         set_error("barrier: K=%d > %d", L.K, BMAXK);
         return TDI_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_barrier, dim3(n), dim3(256), 0, s, m, L, frames, mask, theta, lm, use_landmarks, mode, obj,
-                       objb, H, g);
+    hipLaunchKernelGGL(k_barrier, dim3(fl.n > 0 ? fl.n : 1), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, lm,
+                       use_landmarks, mode, obj, objb, H, g);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -456,7 +490,7 @@ int launch_cholesky(int n, int K, const float* H, const float* g, float* delta, 
         set_error("cholesky: K=%d > %d", K, CMAXK);
         return TDI_ERR_ARG;
     }
-    const size_t lds = (size_t)K * (K + 1) / 2 * (mode == 0 ? sizeof(double) : sizeof(float));
+    const size_t lds = (size_t)K * (K + 1) / 2 * sizeof(double);   // FP64 fallback of mode 1 included
     hipLaunchKernelGGL(k_cholesky, dim3(n), dim3(256), lds, s, K, H, g, delta, status, mask, mode);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
@@ -490,11 +524,11 @@ int launch_ls_decide(int n, int K, FitState* st, const int* evalmask, int* searc
     return TDI_OK;
 }
 
-int launch_face_check(int n, int V, const int* lmk_ids, const float4* vproj, float ref_face_size, const int* mask,
+int launch_face_check(int V, const int* lmk_ids, const float4* vproj, float ref_face_size, FrameList fl,
                       int* face_ok, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_face_check, dim3((n + 255) / 256), dim3(256), 0, s, n, V, lmk_ids, vproj, ref_face_size, mask,
-                       face_ok);
+    hipLaunchKernelGGL(k_face_check, dim3((fl.n + 255) / 256 > 0 ? (fl.n + 255) / 256 : 1), dim3(256), 0, s, V,
+                       lmk_ids, vproj, ref_face_size, fl.act, fl.n, face_ok);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -502,6 +536,13 @@ int launch_face_check(int n, int V, const int* lmk_ids, const float4* vproj, flo
 int launch_copy_mask(int n, const FitState* st, int* mask, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy_mask, dim3((n + 255) / 256), dim3(256), 0, s, n, st, mask);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_compact_frames(int n, const int* mask, int* act, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_compact_frames, dim3(1), dim3(CMP_THREADS), 0, s, n, mask, act);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

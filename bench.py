@@ -107,8 +107,20 @@ def sample_accuracy(w, O, sc, init, nc, th_o, ro, th_batched, plain, threads, td
     g1 = g1.cpu().numpy()
     step = _per_frame_rms(th_o1, th0, sl)
     one = _per_frame_rms(g1, th_o1, sl)
+    # the same difference in the energy norm of the oracle's H at the start, relative to the
+    # step: it weighs each direction by what the images determine
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        Hs = list(ex.map(lambda i: sc.normal_equations(i, th0[i])["H"], range(nc)))
+    en = []
+    for i in range(nc):
+        d = g1[i].astype(np.float64) - th_o1[i]
+        st = th_o1[i] - th0[i]
+        en.append(float(np.sqrt(max(d @ Hs[i] @ d, 0.0)) / max(np.sqrt(max(st @ Hs[i] @ st, 0.0)), 1e-30)))
+    en = np.array(en)
     acc["one_iteration"] = dict(median_step_rms=float(np.median(step)), median_frame_rms=float(np.median(one)),
-                                max_frame_rms=float(one.max()), frames_within_1e4=int(np.sum(one <= 1e-4)))
+                                max_frame_rms=float(one.max()), frames_within_1e4=int(np.sum(one <= 1e-4)),
+                                median_energy_rel=float(np.median(en)), max_energy_rel=float(en.max()))
     # visibility buffers at the start point
     xyz, xyp = b1.synthesize(init[:nc])
     keys, _ = b1.rasterize(xyz, xyp)
@@ -126,7 +138,8 @@ def sample_accuracy(w, O, sc, init, nc, th_o, ro, th_batched, plain, threads, td
     acc["oracle_stop_flags"] = stops
     log(f"[bench] accuracy on {nc} frames: whole fit {acc['whole_fit']['frames_within_1e4']}/{nc} within 1e-4 "
         f"(oracle floor {acc['oracle_floor']['frames_within_1e4']}/{nc}), one iteration "
-        f"{acc['one_iteration']['frames_within_1e4']}/{nc} (median {acc['one_iteration']['median_frame_rms']:.1e}), "
+        f"{acc['one_iteration']['frames_within_1e4']}/{nc} (median {acc['one_iteration']['median_frame_rms']:.1e}, "
+        f"energy-norm median {acc['one_iteration']['median_energy_rel']:.1e}), "
         f"visibility bit-exact {exact}/{nc}")
     return acc
 
@@ -142,6 +155,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (reference kernel for scaling rocprof FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="HBM bytes per launch of the dominant kernel, measured with rocprof counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the fitting step shards by frames with no exchange, "
@@ -215,9 +232,22 @@ def main():
                            unit=unit, frac=round(float(ach / peak), 5), frames=int(n_chunk)))
         log(f"[bench] stage {name:24s} {t:8.3f} ms  {ach:9.2f} {unit} ({100 * ach / peak:.1f}% of peak)")
     dom = max(stages, key=lambda s: s["ms"])
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        tj = json.load(open(args.traffic_json))
+        if tj.get("stage") == dom["stage"] and tj.get("frames_per_launch") == int(n_chunk):
+            traffic = tj.get("bytes_per_launch")
     roofline = dict(kernel=dom["stage"], bound=dom["bound"], achieved=dom["achieved"], peak=dom["peak"],
-                    unit=dom["unit"], frac=dom["frac"], traffic=None,
+                    unit=dom["unit"], frac=dom["frac"], traffic=traffic,
                     launch_ms=dom["ms"], frames_per_launch=int(n_chunk))
+    if args.copy_probe:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        b_ = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b_.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copy probe: 1 GiB read + 1 GiB written")
+        del a, b_
 
     # FP64 oracle on a prefix: host baseline and parameter error of the batched fit
     cpu = None
