@@ -78,6 +78,7 @@ struct tdi_ctx {
     int device = 0;
     tdi_fit_options opts{};
     Layout L{};
+    Layout Lx{};       // L with the light appended when it is not an unknown (light sub-fits)
     ModelGPU m{};
     float taps[15];
     Alloc mem;
@@ -211,12 +212,17 @@ int tdi_ctx_create(const tdi_model_desc* md, const tdi_fit_options* opts, int de
     m.tris = (const int*)up(md->triangles, sizeof(int) * 3 * (size_t)NT);
     m.lmk_ids = (const int*)up(md->landmark_ids, sizeof(int) * NLMK);
     const Layout& L = c->L;
-    std::vector<float> lb(L.K, -INFINITY), ub(L.K, INFINITY);
+    c->Lx = L;
+    if (!L.nlight) {
+        c->Lx.nlight = 4;
+        c->Lx.K = L.K + 4;
+    }
+    std::vector<float> lb(c->Lx.K, -INFINITY), ub(c->Lx.K, INFINITY);
     for (int k = 0; k < L.ntex; ++k) { lb[L.off_tex + k] = md->tex_lb[k]; ub[L.off_tex + k] = md->tex_ub[k]; }
     for (int k = 0; k < L.nid; ++k) { lb[L.off_id + k] = md->id_lb[k]; ub[L.off_id + k] = md->id_ub[k]; }
     for (int k = 0; k < L.nexp; ++k) { lb[L.off_exp + k] = md->exp_lb[k]; ub[L.off_exp + k] = md->exp_ub[k]; }
-    m.lb = (const float*)up(lb.data(), sizeof(float) * L.K);
-    m.ub = (const float*)up(ub.data(), sizeof(float) * L.K);
+    m.lb = (const float*)up(lb.data(), sizeof(float) * c->Lx.K);
+    m.ub = (const float*)up(ub.data(), sizeof(float) * c->Lx.K);
     for (void* p : c->mem.ptrs)
         if (!p) { c->mem.release(); delete c; set_error("hipMalloc failed for the model"); return TDI_ERR_NOMEM; }
     if (hipDeviceSynchronize() != hipSuccess) { c->mem.release(); delete c; set_error("upload failed"); return TDI_ERR_HIP; }
@@ -241,7 +247,7 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     tdi_ctx* c = b->ctx;
     Workspace& w = b->ws;
     Alloc& A = b->mem;
-    const int V = c->m.V, K = c->L.K;
+    const int V = c->m.V, K = c->Lx.K;   // room for the light unknowns of the light sub-fits
     w.C = C;
     w.cap = cap;
     w.max_area = b->max_area;
@@ -437,25 +443,27 @@ extern "C" int tdi_batch_destroy(tdi_batch* b)
 // ---------------------------------------------------------------------------- evaluation
 // Objective at theta for the frames in `mask`: synthesis, face check, raster, resolve,
 // barrier objective, shading, blur, gradients (Renderer::render + evaluate_objective_function).
-static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s)
+static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s,
+                       const Layout* Lp = nullptr)
 {
     tdi_ctx* c = b->ctx;
+    const Layout& L = Lp ? *Lp : c->L;
     Workspace& w = b->ws;
     int rc;
     if (mixed & TDI_MIXED_SYNTH_MFMA)
-        rc = launch_synthesize_gemm(c->m, c->L, w.frames, fl, theta, w.rt, w.P0, w.P, w.vproj, s);
+        rc = launch_synthesize_gemm(c->m, L, w.frames, fl, theta, w.rt, w.P0, w.P, w.vproj, s);
     else
-        rc = launch_synthesize(c->m, c->L, w.frames, fl, theta, w.rt, w.P0, w.P, w.vproj, s);
+        rc = launch_synthesize(c->m, L, w.frames, fl, theta, w.rt, w.P0, w.P, w.vproj, s);
     if (rc) return rc;
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
     if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.max_area, s))) return rc;
     if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
     if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, s)))
         return rc;
-    if ((rc = launch_barrier(c->m, c->L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
+    if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
         return rc;
-    if ((rc = launch_shade(c->m, c->L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
+    if ((rc = launch_shade(c->m, L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
                            w.max_area, s)))
         return rc;
     if ((rc = launch_blur(w.frames, fl, c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
@@ -463,30 +471,33 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
                         w.pcoef, w.obj, w.max_area, s);
 }
 
-static int launch_normal_eq(tdi_batch* b, FrameList fl, int mixed, hipStream_t s)
+static int launch_normal_eq(tdi_batch* b, FrameList fl, int mixed, hipStream_t s, const Layout* Lp = nullptr)
 {
     tdi_ctx* c = b->ctx;
+    const Layout& L = Lp ? *Lp : c->L;
     Workspace& w = b->ws;
     if (mixed & TDI_MIXED_JTJ_BF16)
-        return launch_jtj_mfma(c->L, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H,
+        return launch_jtj_mfma(L, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H,
                                w.g, w.partials, w.S, (mixed & TDI_MIXED_JTR_BF16) ? 1 : 0,
                                (mixed & TDI_MIXED_TWO_TERM) ? 1 : 0, s);
-    return launch_jtj_f32(c->L, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
+    return launch_jtj_f32(L, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
                           w.partials, w.S, s);
 }
 
 // Objective, gradient and Gauss-Newton matrix at theta (get_obj_hessian_and_gradient_multiframe).
-static int eval_full(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s)
+static int eval_full(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s,
+                     const Layout* Lp = nullptr)
 {
     tdi_ctx* c = b->ctx;
+    const Layout& L = Lp ? *Lp : c->L;
     Workspace& w = b->ws;
-    int rc = eval_render(b, fl, theta, mixed, s);
+    int rc = eval_render(b, fl, theta, mixed, s, &L);
     if (rc) return rc;
-    if ((rc = launch_dI(c->m, c->L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
+    if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
                         w.max_area, s)))
         return rc;
-    if ((rc = launch_normal_eq(b, fl, mixed, s))) return rc;
-    return launch_barrier(c->m, c->L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj, w.objb,
+    if ((rc = launch_normal_eq(b, fl, mixed, s, &L))) return rc;
+    return launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj, w.objb,
                           w.H, w.g, s);
 }
 
@@ -759,6 +770,142 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
         std::memcpy(theta_host + (size_t)i * K, th.data(), sizeof(float) * K);
     }
     return TDI_OK;
+}
+
+// ---------------------------------------------------------------------------- light sub-fits
+// fit_3DMM_lambdas (model_fitter.cu:981-1073) and fit_3DMM_Lintensity (model_fitter.cu:1084-1180)
+// one frame at a time, host-driven like the reference.  Shape and pose stay at theta; the
+// light is appended to the unknowns (layout Lx), so the light block of the joint normal
+// equations is w J^T J and -w J^T g^_s of the light columns (the barrier does not involve
+// the light): divided by w they are the reference's JTJ_lambda / dG_dtheta_lambda
+// (derivative_computer.cu:228-312) and their L counterparts (318-400).  The objective
+// -sum g^_r.g^_s (render_for_illumination_only, renderer.cu:737-791) is the data part of
+// the evaluation divided by w.  lambda: 3x3 FP64 Cholesky, Armijo 0.5/0.5, stop below
+// t = 0.01 or -d.g < 3e-4; L: d = -g/H, Armijo 0.3/0.5, stop below t = 0.1 or -d.g < 2.5e-3.
+static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, float* Lint_host,
+                     tdi_fit_report* reports, bool intensity)
+{
+    tdi_ctx* c = b->ctx;
+    if (c->L.nlight) {
+        set_error("light sub-fits need a context whose light is not an unknown (fit_light = 0)");
+        return TDI_ERR_ARG;
+    }
+    const Layout& Lx = c->Lx;
+    const int K = c->L.K, Kx = Lx.K, ol = Lx.off_light;
+    const double w = c->opts.data_weight;
+    const double alpha = intensity ? 0.3 : 0.5, beta = 0.5, t_min = intensity ? 0.1 : 0.01,
+                 dec_tol = intensity ? 0.0025 : 0.0003;
+    const int nd = intensity ? 1 : 3, j0 = intensity ? ol + 3 : ol;
+    hipStream_t s = 0;
+    Workspace& W = b->ws;
+    std::vector<float> x(Kx), xt(Kx), Hh((size_t)Kx * Kx), gh(Kx);
+    for (int i = 0; i < b->n; ++i) {
+        int rc = ws_load(b, i, 1, s);
+        if (rc) return rc;
+        tdi_fit_report& rep = reports[i];
+        std::memset(&rep, 0, sizeof(rep));
+        std::memcpy(x.data(), theta_host + (size_t)i * K, sizeof(float) * K);
+        for (int a = 0; a < 3; ++a) x[ol + a] = lam_host[3 * i + a];
+        x[ol + 3] = Lint_host[i];
+        bool terminate = false;
+        int outer = 0, ls = 0;
+        for (int it = 0; it < 10000; ++it) {
+            if (terminate) break;
+            outer++;
+            TDI_HIP(hipMemcpyAsync(W.theta, x.data(), sizeof(float) * Kx, hipMemcpyHostToDevice, s));
+            if ((rc = eval_full(b, all_frames(1), W.theta, 0, s, &Lx))) return rc;
+            double ob[2];
+            int np = 0;
+            TDI_HIP(hipMemcpyAsync(Hh.data(), W.H, sizeof(float) * Kx * Kx, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(gh.data(), W.g, sizeof(float) * Kx, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(&ob[0], W.obj, sizeof(double), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(&ob[1], W.objb, sizeof(double), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(&np, W.npix, sizeof(int), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipStreamSynchronize(s));
+            rep.n_pixels = np;
+            const double obj0 = (ob[0] - ob[1]) / w;
+            double g[3] = {0, 0, 0}, d[3] = {0, 0, 0};
+            for (int a = 0; a < nd; ++a) g[a] = gh[j0 + a] / w;
+            if (intensity) {
+                d[0] = -g[0] / (Hh[(size_t)j0 * Kx + j0] / w);     // compute_Lintensity_search_dir
+            } else {
+                // 3x3 lower Cholesky in FP64 (Solver(3), solver.h:70-105): d = -H^-1 g
+                double A[9];
+                for (int a = 0; a < 3; ++a)
+                    for (int bb = 0; bb < 3; ++bb) A[a * 3 + bb] = Hh[(size_t)(j0 + a) * Kx + j0 + bb] / w;
+                bool pd = true;
+                for (int jj = 0; jj < 3 && pd; ++jj) {
+                    double dd = A[jj * 3 + jj];
+                    for (int k = 0; k < jj; ++k) dd -= A[jj * 3 + k] * A[jj * 3 + k];
+                    if (!(dd > 0)) { pd = false; break; }
+                    dd = std::sqrt(dd);
+                    A[jj * 3 + jj] = dd;
+                    for (int ii = jj + 1; ii < 3; ++ii) {
+                        double v = A[ii * 3 + jj];
+                        for (int k = 0; k < jj; ++k) v -= A[ii * 3 + k] * A[jj * 3 + k];
+                        A[ii * 3 + jj] = v / dd;
+                    }
+                }
+                if (!pd) { rep.stop |= TDI_STOP_NOT_PD; break; }
+                double y[3];
+                for (int ii = 0; ii < 3; ++ii) {
+                    double v = -g[ii];
+                    for (int k = 0; k < ii; ++k) v -= A[ii * 3 + k] * y[k];
+                    y[ii] = v / A[ii * 3 + ii];
+                }
+                for (int ii = 2; ii >= 0; --ii) {
+                    double v = y[ii];
+                    for (int k = ii + 1; k < 3; ++k) v -= A[k * 3 + ii] * d[k];
+                    d[ii] = v / A[ii * 3 + ii];
+                }
+            }
+            double dg = 0;
+            for (int a = 0; a < nd; ++a) dg += d[a] * g[a];
+            double t = 1.0;
+            int inner = 0;
+            while (inner < 1000) {
+                if (t < t_min) { terminate = true; rep.stop |= TDI_STOP_STEP; break; }
+                xt = x;
+                for (int a = 0; a < nd; ++a) xt[j0 + a] = (float)((double)x[j0 + a] + t * d[a]);
+                TDI_HIP(hipMemcpyAsync(W.theta_try, xt.data(), sizeof(float) * Kx, hipMemcpyHostToDevice, s));
+                if ((rc = eval_render(b, all_frames(1), W.theta_try, 0, s, &Lx))) return rc;
+                TDI_HIP(hipMemcpyAsync(&ob[0], W.obj, sizeof(double), hipMemcpyDeviceToHost, s));
+                TDI_HIP(hipMemcpyAsync(&ob[1], W.objb, sizeof(double), hipMemcpyDeviceToHost, s));
+                TDI_HIP(hipStreamSynchronize(s));
+                inner++;
+                ls++;
+                if (-dg < dec_tol) { terminate = true; rep.stop |= TDI_STOP_DECREMENT; break; }
+                if ((ob[0] - ob[1]) / w < obj0 + alpha * t * dg) {
+                    x = xt;
+                    break;
+                }
+                t *= beta;
+            }
+        }
+        TDI_HIP(hipMemcpyAsync(W.theta, x.data(), sizeof(float) * Kx, hipMemcpyHostToDevice, s));
+        if ((rc = eval_render(b, all_frames(1), W.theta, 0, s, &Lx))) return rc;
+        double ob[2];
+        TDI_HIP(hipMemcpyAsync(&ob[0], W.obj, sizeof(double), hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipMemcpyAsync(&ob[1], W.objb, sizeof(double), hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipStreamSynchronize(s));
+        rep.objective = (float)((ob[0] - ob[1]) / w);
+        rep.iters = outer;
+        rep.ls_evals = ls;
+        rep.status = (rep.stop & TDI_STOP_NOT_PD) ? TDI_NOT_PD : TDI_OK;
+        for (int a = 0; a < 3; ++a) lam_host[3 * i + a] = x[ol + a];
+        Lint_host[i] = x[ol + 3];
+    }
+    return TDI_OK;
+}
+
+int tdi_fit_lambdas(tdi_batch* b, const float* theta, float* lam, float* Lint, tdi_fit_report* reports)
+{
+    return light_fit(b, theta, lam, Lint, reports, false);
+}
+
+int tdi_fit_lintensity(tdi_batch* b, const float* theta, float* lam, float* Lint, tdi_fit_report* reports)
+{
+    return light_fit(b, theta, lam, Lint, reports, true);
 }
 
 } // extern "C"

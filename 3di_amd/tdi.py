@@ -55,6 +55,8 @@ def lib():
         L.tdi_stage_times.argtypes = [_vp, _vp, C.c_int32, C.c_int32, _vp]
         L.tdi_last_pixel_counts.argtypes = [_vp, _vp]
         L.tdi_workspace_frames.argtypes = [_vp]
+        L.tdi_fit_lambdas.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.tdi_fit_lintensity.argtypes = [_vp, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -210,6 +212,22 @@ class Batch:
         reps = (abi.tdi_fit_report * self.n)()
         _check(lib().tdi_fit_plain(self.h, th.ctypes.data, C.addressof(reps)))
         return th, list(reps)
+
+    def _light_fit(self, fn, theta, lam, Lint):
+        th = np.ascontiguousarray(np.array(theta, np.float32).reshape(self.n, self.K))
+        lam = np.ascontiguousarray(np.array(lam, np.float32).reshape(self.n, 3))
+        Li = np.ascontiguousarray(np.array(Lint, np.float32).reshape(self.n))
+        reps = (abi.tdi_fit_report * self.n)()
+        _check(fn(self.h, th.ctypes.data, lam.ctypes.data, Li.ctypes.data, C.addressof(reps)))
+        return lam, Li, list(reps)
+
+    def fit_lambdas(self, theta, lam, Lint):
+        """fit_3DMM_lambdas (model_fitter.cu:981-1073) on every frame: returns (lam, Lint, reports)."""
+        return self._light_fit(lib().tdi_fit_lambdas, theta, lam, Lint)
+
+    def fit_lintensity(self, theta, lam, Lint):
+        """fit_3DMM_Lintensity (model_fitter.cu:1084-1180) on every frame."""
+        return self._light_fit(lib().tdi_fit_lintensity, theta, lam, Lint)
 
     def fit_batched(self, theta):
         """theta: device tensor [n, K], updated in place.  Returns the per-frame reports."""

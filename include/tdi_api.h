@@ -212,6 +212,19 @@ int tdi_fit_plain(tdi_batch* b, float* theta, tdi_fit_report* reports);
  * iterations captured in a hipGraph.  theta: [n][K] DEVICE, in/out; reports: HOST [n]. */
 int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream);
 
+/* Light sub-fits, one frame at a time (host arrays), for a context whose light is not an
+ * unknown (fit_light = 0): shape and pose stay at theta [n][K]; lam [n][3] and Lint [n] are
+ * updated in place.
+ *   tdi_fit_lambdas    replaces fit_3DMM_lambdas (model_fitter.cu:981-1073; normal equations
+ *                      DerivativeComputer::compute_hessian_and_gradient_for_lambda,
+ *                      derivative_computer.cu:228-312): 3x3 Gauss-Newton on lambda.
+ *   tdi_fit_lintensity replaces fit_3DMM_Lintensity (model_fitter.cu:1084-1180;
+ *                      compute_hessian_and_gradient_for_Lintensity, derivative_computer.cu:318-400):
+ *                      scalar Newton on L.
+ * reports[i].objective is -sum g^_r.g^_s at the result (no data weight, no barrier). */
+int tdi_fit_lambdas(tdi_batch* b, const float* theta, float* lam, float* Lint, tdi_fit_report* reports);
+int tdi_fit_lintensity(tdi_batch* b, const float* theta, float* lam, float* Lint, tdi_fit_report* reports);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,9 @@ def lib():
         L.orc_cholesky_solve.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_fit.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, C.c_void_p, C.c_int]
         L.orc_max_threads.restype = C.c_int
+        L.orc_fit_light.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_void_p, C.c_int,
+                                    C.c_int]
+        L.orc_fit_light.restype = C.c_int
         _lib = L
     return _lib
 
@@ -158,6 +161,29 @@ class Scene:
         reps = (abi.tdi_fit_report * n)()
         lib().orc_fit(self.md.ref, n, self.fd.ptr(0), C.byref(self.opts), p(th), reps, n_threads)
         return th, list(reps)
+
+
+def _light_fit(scene, theta, lam, Lint, which, n_threads):
+    n = len(scene.frames)
+    th = np.ascontiguousarray(np.array(theta, np.float64).reshape(n, scene.K))
+    lam = np.ascontiguousarray(np.array(lam, np.float64).reshape(n, 3))
+    Li = np.ascontiguousarray(np.array(Lint, np.float64).reshape(n))
+    reps = (abi.tdi_fit_report * n)()
+    rc = lib().orc_fit_light(scene.md.ref, n, scene.fd.ptr(0), C.byref(scene.opts), p(th), p(lam), p(Li), reps, which,
+                             n_threads)
+    if rc:
+        raise ValueError("light sub-fits need options with fit_light = 0")
+    return lam, Li, list(reps)
+
+
+def fit_lambdas(scene, theta, lam, Lint, n_threads=0):
+    """fit_3DMM_lambdas on every frame of `scene` (light position, shape and pose fixed)."""
+    return _light_fit(scene, theta, lam, Lint, 0, n_threads)
+
+
+def fit_lintensity(scene, theta, lam, Lint, n_threads=0):
+    """fit_3DMM_Lintensity on every frame of `scene`."""
+    return _light_fit(scene, theta, lam, Lint, 1, n_threads)
 
 
 def cholesky_solve(H, g):

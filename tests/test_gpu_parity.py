@@ -308,3 +308,36 @@ def test_fit_batched_vs_oracle(scene, gpu):
     obj_g = np.array([r.objective for r in repg])
     obj_o = np.array([r.objective for r in repo])
     assert np.all(obj_g < obj_o + 0.01 * np.abs(obj_o))
+
+
+@pytest.mark.parametrize("which", ["lambdas", "lintensity"])
+def test_light_fit_vs_oracle(scene, which):
+    """fit_3DMM_lambdas / fit_3DMM_Lintensity on the GPU (FP32 J^T J, FP64 3x3 solve) against
+    the FP64 oracle from the same FP32 starting light: same number of iterations and
+    line-search evaluations, light within 1e-3 relative."""
+    if scene.opts.fit_light:
+        pytest.skip("light sub-fits apply when the light is not a joint unknown")
+    _say(f"case: light sub-fit {which}")
+    rng = np.random.default_rng(21)
+    th = _f32(scene.truth())
+    lam_t = np.stack([f.lam for f in scene.frames]).astype(np.float64)
+    lam0 = _f32(lam_t + rng.normal(0, 12.0, lam_t.shape))
+    L0 = _f32(np.array([f.Lint for f in scene.frames]) * (1.0 + rng.normal(0, 0.1, len(scene.frames))))
+    orc = scene.oracle()
+    ctx = tdi.Context(scene.model, scene.opts)
+    b = tdi.Batch(ctx, scene.frames)
+    if which == "lambdas":
+        lo, Lo, ro = O.fit_lambdas(orc, th, lam0, L0)
+        lg, Lg, rg = b.fit_lambdas(th, lam0, L0)
+    else:
+        lo, Lo, ro = O.fit_lintensity(orc, th, lam0, L0)
+        lg, Lg, rg = b.fit_lintensity(th, lam0, L0)
+    b.close()
+    ctx.close()
+    _say(f"  oracle lam {lo.tolist()} L {Lo.tolist()} iters {[r.iters for r in ro]} ls {[r.ls_evals for r in ro]}")
+    _say(f"  gpu    lam {lg.tolist()} L {Lg.tolist()} iters {[r.iters for r in rg]} ls {[r.ls_evals for r in rg]}")
+    assert [r.status for r in rg] == [abi.TDI_OK] * len(rg)
+    assert [(r.iters, r.ls_evals, r.stop) for r in rg] == [(r.iters, r.ls_evals, r.stop) for r in ro]
+    assert np.abs(lg - lo).max() <= 1e-3 * np.abs(lo).max()
+    assert np.abs(Lg - Lo).max() <= 1e-3 * np.abs(Lo).max()
+    assert np.allclose([r.objective for r in rg], [r.objective for r in ro], rtol=1e-4)
