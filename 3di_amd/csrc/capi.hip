@@ -898,6 +898,20 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
     return TDI_OK;
 }
 
+int32_t tdi_batch_size(tdi_batch* b) { return b ? b->n : 0; }
+
+int32_t tdi_batch_unknowns(tdi_batch* b) { return b ? b->ctx->L.K : 0; }
+
+int tdi_batch_set_light(tdi_batch* b, const float* lam, const float* Lint)
+{
+    if (!b || !lam || !Lint) { set_error("set_light: null argument"); return TDI_ERR_ARG; }
+    for (int i = 0; i < b->n; ++i) {
+        for (int a = 0; a < 3; ++a) b->hframes[i].lam[a] = lam[3 * i + a];
+        b->hframes[i].Lint = Lint[i];
+    }
+    return TDI_OK;
+}
+
 int tdi_fit_lambdas(tdi_batch* b, const float* theta, float* lam, float* Lint, tdi_fit_report* reports)
 {
     return light_fit(b, theta, lam, Lint, reports, false);

@@ -1,0 +1,298 @@
+// tdi_host.cpp — C++ host layer above the C ABI of libtdi.so.
+//
+// Restates the reference's host-side code around the per-frame fitting step:
+//   * VideoOutput (video_fitter.cu:2603-2828): per-frame results of several fits, averaged
+//     (expressions over the NRES fits, poses over all fits; the illumination keeps the first
+//     fit) and written as text by write_2d_vector (funcs.h:112-124): every value with
+//     setprecision(7) followed by one space, one row per frame.  save_expressions,
+//     save_poses and save_illums write the frames t < max(frame id), so the last frame is not
+//     written (video_fitter.cu:2754,2787,2821); save_expressions_all writes t <= max.
+//   * compute_euler_angles (rotation_header.cu:421-449): yaw, pitch, roll of R in FP32.
+//   * fit_to_video_frame (video_fitter.cu:554-615): L = 0.005 and a lambda fit per frame,
+//     the shape fit, and with TWOSTAGE_ILLUM two more lambda + shape rounds.
+//   * the config-file keys of this path (config.cu:300-560).
+// Only the C ABI of include/tdi_api.h is called; the batched shape fit takes a device copy of
+// the unknowns, made with the HIP runtime.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/tdi_api.h"
+#include "../../include/tdi_host.h"
+
+namespace tdi_host {
+
+// write_2d_vector (funcs.h:112-124): the row width is that of the first row.
+static void write_2d_vector(const std::string& path, const std::vector<std::vector<float>>& rows)
+{
+    std::ofstream out(path.c_str());
+    for (size_t i = 0; i < rows.size(); ++i) {
+        for (size_t j = 0; j < rows[0].size(); ++j)
+            out << std::setprecision(std::numeric_limits<float>::digits10 + 1) << rows[i][j] << " ";
+        out << std::endl;
+    }
+}
+
+// R (column-major, R[col*3+row]) of the axis-angle vector u, evaluated in FP64 and rounded
+// to FP32 like the rotation the GPU path uses (R = I + sin(t)[n]x + (1-cos(t))[n]x^2).
+static void rotation_f32(const float* uf, float* R)
+{
+    const double u[3] = {uf[0], uf[1], uf[2]};
+    const double th = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double n[3] = {u[0] / th, u[1] / th, u[2] / th};
+    const double N[9] = {0, n[2], -n[1], -n[2], 0, n[0], n[1], -n[0], 0};
+    const double s = std::sin(th), c1 = 1.0 - std::cos(th);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            double nn = 0;
+            for (int k = 0; k < 3; ++k) nn += N[k * 3 + r] * N[c * 3 + k];
+            R[c * 3 + r] = (float)(c1 * nn + s * N[c * 3 + r] + (r == c ? 1.0 : 0.0));
+        }
+}
+
+void compute_euler_angles(const float* R, float& yaw, float& pitch, float& roll)
+{
+    const float cosine_for_pitch = std::sqrt(R[5] * R[5] + R[8] * R[8]);
+    if (!(cosine_for_pitch < 1e-6f)) {
+        roll = std::atan2(R[1], R[0]);
+        yaw = std::atan2(-R[2], cosine_for_pitch);
+        pitch = std::atan2(R[5], R[8]);
+    } else {
+        roll = std::atan2(-R[7], R[4]);
+        yaw = std::atan2(-R[2], cosine_for_pitch);
+        pitch = 0.0f;
+    }
+}
+
+class VideoOutput {
+public:
+    VideoOutput(int k_exp, int nres) : Kepsilon(k_exp), nres_coefs(nres) {}
+
+    void add_exp_coeffs(size_t frame_id, const std::vector<float>& e)
+    {
+        exp_coefs[frame_id].push_back(e);
+        abs_frame_ids.insert(frame_id);
+    }
+    void add_pose(size_t frame_id, const std::vector<float>& p) { poses[frame_id].push_back(p); }
+    void add_illum(size_t frame_id, const std::vector<float>& il)
+    {
+        if (illum_coeffs.find(frame_id) == illum_coeffs.end()) illum_coeffs.insert(std::make_pair(frame_id, il));
+    }
+
+    std::vector<float> compute_avg_exp_coeffs(size_t frame_id, bool ignore_first)
+    {
+        auto it = exp_coefs.find(frame_id);
+        if (it == exp_coefs.end()) return std::vector<float>(Kepsilon, NAN);
+        std::vector<float> avgs(Kepsilon, 0);
+        const int K = (int)it->second.size();
+        const int k0 = ignore_first ? 1 : 0;
+        for (int k = k0; k < K; ++k)
+            for (int ei = 0; ei < Kepsilon; ++ei) avgs[ei] += it->second[k][ei] / ((float)(K - k0));
+        return avgs;
+    }
+    std::vector<float> combine_exp_coeffs(size_t frame_id)
+    {
+        auto it = exp_coefs.find(frame_id);
+        if (it == exp_coefs.end()) return std::vector<float>((size_t)Kepsilon * nres_coefs, NAN);
+        std::vector<float> exps;
+        for (int k = 0; k < nres_coefs; ++k)
+            for (int ei = 0; ei < Kepsilon; ++ei)
+                exps.push_back(k >= (int)it->second.size() ? NAN : it->second[k][ei]);
+        return exps;
+    }
+    std::vector<float> compute_avg_pose(size_t frame_id)
+    {
+        auto it = poses.find(frame_id);
+        if (it == poses.end()) return std::vector<float>(6, NAN);
+        std::vector<float> avgs(6, 0);
+        const int K = (int)it->second.size();
+        for (int k = 0; k < K; ++k)
+            for (int ei = 0; ei < 6; ++ei) avgs[ei] += it->second[k][ei] / ((float)K);
+        return avgs;
+    }
+    std::vector<float> get_illum(size_t frame_id)
+    {
+        auto it = illum_coeffs.find(frame_id);
+        return it == illum_coeffs.end() ? std::vector<float>(4, NAN) : it->second;
+    }
+
+    int max_frame_id() const { return abs_frame_ids.empty() ? -1 : (int)*abs_frame_ids.rbegin(); }
+
+    void save_expressions(const std::string& path, bool ignore_first)
+    {
+        std::vector<std::vector<float>> rows;
+        for (int t = 0; t < max_frame_id(); ++t) rows.push_back(compute_avg_exp_coeffs(t, ignore_first));
+        write_2d_vector(path, rows);
+    }
+    void save_expressions_all(const std::string& path)
+    {
+        std::vector<std::vector<float>> rows;
+        for (int t = 0; t <= max_frame_id(); ++t) rows.push_back(combine_exp_coeffs(t));
+        write_2d_vector(path, rows);
+    }
+    void save_poses(const std::string& path, bool with_euler)
+    {
+        std::vector<std::vector<float>> rows;
+        for (int t = 0; t < max_frame_id(); ++t) {
+            std::vector<float> p = compute_avg_pose(t);
+            if (with_euler) {
+                float R[9], yaw, pitch, roll;
+                rotation_f32(&p[3], R);
+                compute_euler_angles(R, yaw, pitch, roll);
+                p.push_back(yaw);
+                p.push_back(pitch);
+                p.push_back(roll);
+            }
+            rows.push_back(p);
+        }
+        write_2d_vector(path, rows);
+    }
+    void save_illums(const std::string& path)
+    {
+        std::vector<std::vector<float>> rows;
+        for (int t = 0; t < max_frame_id(); ++t) rows.push_back(get_illum(t));
+        write_2d_vector(path, rows);
+    }
+
+private:
+    int Kepsilon, nres_coefs;
+    std::map<size_t, std::vector<std::vector<float>>> exp_coefs, poses;
+    std::map<size_t, std::vector<float>> illum_coeffs;
+    std::set<size_t> abs_frame_ids;
+};
+
+// ---------------------------------------------------------------------------- fits
+// fit_3DMM_shape_rigid (model_fitter.cu:121-375) for every frame of the batch.
+static int fit_3DMM_shape_rigid(tdi_batch* b, int n, int K, bool batched, float* theta, tdi_fit_report* reports)
+{
+    if (!batched) return tdi_fit_plain(b, theta, reports);
+    float* d = nullptr;
+    const size_t bytes = sizeof(float) * (size_t)n * K;
+    if (hipMalloc((void**)&d, bytes) != hipSuccess) return TDI_ERR_NOMEM;
+    int rc = TDI_OK;
+    if (hipMemcpy(d, theta, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = TDI_ERR_HIP;
+    if (!rc) rc = tdi_fit_batched(b, d, reports, nullptr);
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = TDI_ERR_HIP;
+    if (!rc && hipMemcpy(theta, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = TDI_ERR_HIP;
+    (void)hipFree(d);
+    return rc;
+}
+
+// fit_3DMM_lambdas (model_fitter.cu:981-1073) followed by the light update the next shape
+// fit reads.
+static int fit_3DMM_lambdas(tdi_batch* b, int n, const float* theta, float* lam, float* Lint)
+{
+    std::vector<tdi_fit_report> r(n);
+    int rc = tdi_fit_lambdas(b, theta, lam, Lint, r.data());
+    if (rc) return rc;
+    return tdi_batch_set_light(b, lam, Lint);
+}
+
+} // namespace tdi_host
+
+// ---------------------------------------------------------------------------- C entry points
+struct tdi_video_output {
+    tdi_host::VideoOutput vo;
+    int k_exp;
+};
+
+extern "C" {
+
+tdi_video_output* tdi_vo_create(int32_t k_exp, int32_t nres_coefs)
+{
+    if (k_exp <= 0 || nres_coefs <= 0) return nullptr;
+    return new tdi_video_output{tdi_host::VideoOutput(k_exp, nres_coefs), k_exp};
+}
+
+void tdi_vo_destroy(tdi_video_output* vo) { delete vo; }
+
+int tdi_vo_add(tdi_video_output* vo, int64_t frame_id, const float* exp, const float* pose, const float* illum)
+{
+    if (!vo || !exp || frame_id < 0) return TDI_ERR_ARG;
+    vo->vo.add_exp_coeffs((size_t)frame_id, std::vector<float>(exp, exp + vo->k_exp));
+    if (pose) vo->vo.add_pose((size_t)frame_id, std::vector<float>(pose, pose + 6));
+    if (illum) vo->vo.add_illum((size_t)frame_id, std::vector<float>(illum, illum + 4));
+    return TDI_OK;
+}
+
+int tdi_vo_save(tdi_video_output* vo, int32_t kind, const char* path, int32_t flag)
+{
+    if (!vo || !path) return TDI_ERR_ARG;
+    if (vo->vo.max_frame_id() < 0) return TDI_ERR_ARG;
+    switch (kind) {
+    case 0: vo->vo.save_expressions(path, flag != 0); break;
+    case 1: vo->vo.save_expressions_all(path); break;
+    case 2: vo->vo.save_poses(path, flag != 0); break;
+    case 3: vo->vo.save_illums(path); break;
+    default: return TDI_ERR_ARG;
+    }
+    return TDI_OK;
+}
+
+void tdi_euler_angles(const float* u, float* ypr)
+{
+    float R[9];
+    tdi_host::rotation_f32(u, R);
+    tdi_host::compute_euler_angles(R, ypr[0], ypr[1], ypr[2]);
+}
+
+// OpenCV FileStorage YAML subset: "KEY: value" lines, '#' comments, a "%YAML" header and "---".
+int tdi_read_config(const char* path, tdi_fit_options* opts, int32_t* twostage_illum, int32_t* nres_coefs)
+{
+    std::ifstream in(path);
+    if (!in || !opts) return TDI_ERR_ARG;
+    int two = 0, nres = 1;
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line = line.substr(0, hash);
+        const size_t colon = line.find(':');
+        if (colon == std::string::npos || line.empty() || line[0] == '%') continue;
+        std::string key = line.substr(0, colon), val = line.substr(colon + 1);
+        key.erase(0, key.find_first_not_of(" \t"));
+        key.erase(key.find_last_not_of(" \t\r") + 1);
+        val.erase(0, val.find_first_not_of(" \t\""));
+        val.erase(val.find_last_not_of(" \t\r\"") + 1);
+        if (key.empty() || val.empty()) continue;
+        const double v = std::atof(val.c_str());
+        if (key == "KERNEL_SIGMA" && v > 0) opts->blur_sigma = (float)v;       // config.cu:553-556
+        else if (key == "REF_FACE_SIZE") opts->ref_face_size = (float)v;
+        else if (key == "NITERS_SUCCESS") opts->niters_success = (int)v;
+        else if (key == "TWOSTAGE_ILLUM") two = (int)v;
+        else if (key == "NRES_COEFS") nres = (int)v;
+    }
+    if (twostage_illum) *twostage_illum = two;
+    if (nres_coefs) *nres_coefs = nres;
+    return TDI_OK;
+}
+
+int tdi_fit_video_frames(tdi_batch* b, int32_t twostage_illum, int32_t batched, float* theta, float* lam,
+                         float* Lint, tdi_fit_report* reports)
+{
+    if (!b || !theta || !lam || !Lint || !reports) return TDI_ERR_ARG;
+    const int n = tdi_batch_size(b), K = tdi_batch_unknowns(b);
+    // video_fitter.cu:558-565: L = 0.005, then the lambda fit of every frame
+    for (int i = 0; i < n; ++i) Lint[i] = 0.005f;
+    int rc = tdi_batch_set_light(b, lam, Lint);
+    if (!rc) rc = tdi_host::fit_3DMM_lambdas(b, n, theta, lam, Lint);
+    if (!rc) rc = tdi_host::fit_3DMM_shape_rigid(b, n, K, batched != 0, theta, reports);
+    // TWOSTAGE_ILLUM (video_fitter.cu:577-615): lambda + shape, twice more
+    for (int round = 0; !rc && twostage_illum && round < 2; ++round) {
+        rc = tdi_host::fit_3DMM_lambdas(b, n, theta, lam, Lint);
+        if (!rc) rc = tdi_host::fit_3DMM_shape_rigid(b, n, K, batched != 0, theta, reports);
+    }
+    return rc;
+}
+
+} // extern "C"

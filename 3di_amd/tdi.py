@@ -56,6 +56,9 @@ def lib():
         L.tdi_last_pixel_counts.argtypes = [_vp, _vp]
         L.tdi_workspace_frames.argtypes = [_vp]
         L.tdi_fit_lambdas.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.tdi_batch_set_light.argtypes = [_vp, _vp, _vp]
+        L.tdi_batch_size.argtypes = [_vp]
+        L.tdi_batch_unknowns.argtypes = [_vp]
         L.tdi_fit_lintensity.argtypes = [_vp, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
@@ -220,6 +223,12 @@ class Batch:
         reps = (abi.tdi_fit_report * self.n)()
         _check(fn(self.h, th.ctypes.data, lam.ctypes.data, Li.ctypes.data, C.addressof(reps)))
         return lam, Li, list(reps)
+
+    def set_light(self, lam, Lint):
+        """Replace every frame's fixed light (used by shape fits when the light is not an unknown)."""
+        la = np.ascontiguousarray(np.array(lam, np.float32).reshape(self.n, 3))
+        Li = np.ascontiguousarray(np.array(Lint, np.float32).reshape(self.n))
+        _check(lib().tdi_batch_set_light(self.h, la.ctypes.data, Li.ctypes.data))
 
     def fit_lambdas(self, theta, lam, Lint):
         """fit_3DMM_lambdas (model_fitter.cu:981-1073) on every frame: returns (lam, Lint, reports)."""

@@ -1,0 +1,132 @@
+"""Python binding of the C++ host layer (libtdi_host.so, include/tdi_host.h).
+
+Mirrors the reference's host-side pieces around the per-frame fit: ``VideoOutput``
+(video_fitter.cu:2603-2828) with its .expressions / .poses / .illums writers,
+``euler_angles`` (RotationComputer::compute_euler_angles, rotation_header.cu:421-449),
+``read_config`` (config.cu:300-560, the keys of this path) and ``fit_video_frames``
+(the per-frame sequence of VideoFitter::fit_to_video_frame, video_fitter.cu:554-615).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import importlib
+import os
+
+import numpy as np
+
+abi = importlib.import_module("3di_amd.abi")
+tdi = importlib.import_module("3di_amd.tdi")
+
+HOST_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtdi_host.so")
+_lib = None
+_vp = C.c_void_p
+
+EXPRESSIONS, EXPRESSIONS_ALL, POSES, ILLUMS = 0, 1, 2, 3
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(HOST_LIB):
+            raise tdi.TdiError(f"{HOST_LIB} is missing: build it with `python 3di_amd/build.py`")
+        tdi.lib()                     # libtdi.so first (the host layer links against it)
+        L = C.CDLL(HOST_LIB)
+        L.tdi_vo_create.restype = _vp
+        L.tdi_vo_create.argtypes = [C.c_int32, C.c_int32]
+        L.tdi_vo_destroy.argtypes = [_vp]
+        L.tdi_vo_add.argtypes = [_vp, C.c_int64, _vp, _vp, _vp]
+        L.tdi_vo_save.argtypes = [_vp, C.c_int32, C.c_char_p, C.c_int32]
+        L.tdi_euler_angles.argtypes = [_vp, _vp]
+        L.tdi_read_config.argtypes = [C.c_char_p, _vp, _vp, _vp]
+        L.tdi_fit_video_frames.argtypes = [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp]
+        _lib = L
+    return _lib
+
+
+def _f32p(a, n):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, np.float32)
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+class VideoOutput:
+    """Per-frame results of several fits and the reference's output files."""
+
+    def __init__(self, k_exp: int, nres_coefs: int = 1):
+        self.k_exp = k_exp
+        self.h = lib().tdi_vo_create(k_exp, nres_coefs)
+        if not self.h:
+            raise ValueError("k_exp and nres_coefs must be positive")
+
+    def add(self, frame_id: int, exp, pose=None, illum=None):
+        e = _f32p(exp, self.k_exp)
+        p = _f32p(pose, 6)
+        il = _f32p(illum, 4)
+        rc = lib().tdi_vo_add(self.h, int(frame_id), e.ctypes.data, None if p is None else p.ctypes.data,
+                              None if il is None else il.ctypes.data)
+        if rc:
+            raise ValueError("bad frame")
+
+    def save(self, kind: int, path: str, flag: int = 0):
+        if lib().tdi_vo_save(self.h, kind, os.fsencode(path), flag):
+            raise ValueError("nothing to save or unknown kind")
+
+    def save_expressions(self, path, ignore_first=False):
+        self.save(EXPRESSIONS, path, int(ignore_first))
+
+    def save_expressions_all(self, path):
+        self.save(EXPRESSIONS_ALL, path)
+
+    def save_poses(self, path, with_euler=True):
+        self.save(POSES, path, int(with_euler))
+
+    def save_illums(self, path):
+        self.save(ILLUMS, path)
+
+    def close(self):
+        if self.h:
+            lib().tdi_vo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def euler_angles(u) -> np.ndarray:
+    """(yaw, pitch, roll) of the rotation with axis-angle vector u."""
+    uu = np.ascontiguousarray(u, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().tdi_euler_angles(uu.ctypes.data, out.ctypes.data)
+    return out
+
+
+def read_config(path: str, opts=None):
+    """Applies a 3DI config file to fit options; returns (opts, twostage_illum, nres_coefs)."""
+    o = opts if opts is not None else abi.default_options()
+    two = C.c_int32(0)
+    nres = C.c_int32(1)
+    if lib().tdi_read_config(os.fsencode(path), C.addressof(o), C.addressof(two), C.addressof(nres)):
+        raise FileNotFoundError(path)
+    return o, two.value, nres.value
+
+
+def fit_video_frames(batch, theta, lam, twostage_illum: bool = False, batched: bool = False):
+    """fit_to_video_frame's sequence for every frame of `batch` (a tdi.Batch whose light is not
+    an unknown): L = 0.005, lambda fit, shape fit (+ two more rounds with twostage_illum).
+    Returns (theta, lam, Lint, reports of the last shape fit)."""
+    n, K = batch.n, batch.K
+    th = np.ascontiguousarray(np.array(theta, np.float32).reshape(n, K))
+    la = np.ascontiguousarray(np.array(lam, np.float32).reshape(n, 3))
+    Li = np.zeros(n, np.float32)
+    reps = (abi.tdi_fit_report * n)()
+    rc = lib().tdi_fit_video_frames(batch.h, int(twostage_illum), int(batched), th.ctypes.data, la.ctypes.data,
+                                    Li.ctypes.data, C.addressof(reps))
+    if rc:
+        raise tdi.TdiError(f"tdi_fit_video_frames: {rc}: {tdi.lib().tdi_last_error().decode()}")
+    return th, la, Li, list(reps)

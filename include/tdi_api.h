@@ -212,6 +212,15 @@ int tdi_fit_plain(tdi_batch* b, float* theta, tdi_fit_report* reports);
  * iterations captured in a hipGraph.  theta: [n][K] DEVICE, in/out; reports: HOST [n]. */
 int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream);
 
+/* Frames in the batch and unknowns per frame. */
+int32_t tdi_batch_size(tdi_batch* b);
+int32_t tdi_batch_unknowns(tdi_batch* b);
+
+/* Replace the fixed light (lambda [n][3], L [n]) of every frame of the batch: the light a
+ * shape fit uses when the light is not an unknown (the reference's ov.lambdas/ov.Lintensity
+ * between its sub-fits, video_fitter.cu:554-615). */
+int tdi_batch_set_light(tdi_batch* b, const float* lam, const float* Lint);
+
 /* Light sub-fits, one frame at a time (host arrays), for a context whose light is not an
  * unknown (fit_light = 0): shape and pose stay at theta [n][K]; lam [n][3] and Lint [n] are
  * updated in place.

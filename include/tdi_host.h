@@ -1,0 +1,55 @@
+/*
+ * tdi_host.h — C entry points of the C++ host layer (3di_amd/host/tdi_host.cpp).
+ *
+ * The host layer sits above the C ABI of include/tdi_api.h and restates the reference's
+ * host-side code for the per-frame fitting step: the orchestration of one frame's fits
+ * (VideoFitter::fit_to_video_frame, video_fitter.cu:432-648), the output files
+ * (VideoOutput, video_fitter.cu:2603-2828, written by write_2d_vector, funcs.h:112-124),
+ * Euler angles of a pose (RotationComputer::compute_euler_angles, rotation_header.cu:421-449)
+ * and the config-file keys that reach this path (config.cu:300-560).  It is C++; these
+ * functions exist for bindings (ctypes) and tests.  Library: 3di_amd/libtdi_host.so.
+ */
+#ifndef TDI_HOST_H
+#define TDI_HOST_H
+
+#include <stdint.h>
+#include "tdi_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- VideoOutput (video_fitter.h:38-75, video_fitter.cu:2603-2828) ---- */
+typedef struct tdi_video_output tdi_video_output;
+
+/* k_exp: expression coefficients per fit; nres_coefs: columns groups of save_expressions_all
+ * (config NRES_COEFS). */
+tdi_video_output* tdi_vo_create(int32_t k_exp, int32_t nres_coefs);
+void tdi_vo_destroy(tdi_video_output* vo);
+/* One fit of frame `frame_id` (add_exp_coeffs, add_pose, add_illum; pose = tau(3), u(3);
+ * illum = lambda(3), L).  pose / illum may be NULL (FINETUNE_ONLY adds expressions only). */
+int tdi_vo_add(tdi_video_output* vo, int64_t frame_id, const float* exp, const float* pose, const float* illum);
+/* kind 0: save_expressions (flag = ignore_first), 1: save_expressions_all,
+ * 2: save_poses (flag = append yaw, pitch, roll), 3: save_illums. */
+int tdi_vo_save(tdi_video_output* vo, int32_t kind, const char* path, int32_t flag);
+
+/* yaw, pitch, roll of the rotation with axis-angle vector u (compute_euler_angles). */
+void tdi_euler_angles(const float* u, float* yaw_pitch_roll);
+
+/* Reads the keys of a 3DI config file (OpenCV YAML "KEY: value") that act on this path into
+ * opts (KERNEL_SIGMA -> blur_sigma, REF_FACE_SIZE -> ref_face_size, NITERS_SUCCESS ->
+ * niters_success) and returns TWOSTAGE_ILLUM and NRES_COEFS (defaults 0 and 1, config.cu). */
+int tdi_read_config(const char* path, tdi_fit_options* opts, int32_t* twostage_illum, int32_t* nres_coefs);
+
+/* The per-frame sequence of fit_to_video_frame (video_fitter.cu:554-615) for every frame of the
+ * batch: L = 0.005, lambda fit, shape fit; with twostage_illum two more lambda + shape rounds.
+ * theta [n][K], lam [n][3], Lint [n] are host arrays updated in place; reports are those of
+ * the last shape fit.  batched = 0 runs tdi_fit_plain, 1 tdi_fit_batched. */
+int tdi_fit_video_frames(tdi_batch* b, int32_t twostage_illum, int32_t batched, float* theta, float* lam,
+                         float* Lint, tdi_fit_report* reports);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

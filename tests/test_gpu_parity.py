@@ -341,3 +341,38 @@ def test_light_fit_vs_oracle(scene, which):
     assert np.abs(lg - lo).max() <= 1e-3 * np.abs(lo).max()
     assert np.abs(Lg - Lo).max() <= 1e-3 * np.abs(Lo).max()
     assert np.allclose([r.objective for r in rg], [r.objective for r in ro], rtol=1e-4)
+
+
+def test_fit_video_frames_sequence(scene):
+    """The host layer's fit_to_video_frame sequence (L = 0.005, lambda fit, shape fit, and two
+    more rounds with TWOSTAGE_ILLUM) equals the same calls made one by one, bit for bit, and
+    its first lambda fit matches the oracle's."""
+    if scene.opts.fit_light:
+        pytest.skip("the sequence applies when the light is not a joint unknown")
+    _say("case: fit_video_frames sequence")
+    video = importlib.import_module("3di_amd.video")
+    n = len(scene.frames)
+    th0 = _f32(scene.init())
+    rng = np.random.default_rng(5)
+    lam0 = _f32(np.stack([f.lam for f in scene.frames]) + rng.normal(0, 10.0, (n, 3)))
+    ctx = tdi.Context(scene.model, scene.opts)
+    b = tdi.Batch(ctx, scene.frames)
+    th_a, lam_a, L_a, rep_a = video.fit_video_frames(b, th0, lam0, twostage_illum=True)
+    b.close()
+    b = tdi.Batch(ctx, scene.frames)
+    L = np.full(n, 0.005, np.float32)
+    lam, th = lam0.astype(np.float32), th0.astype(np.float32)
+    b.set_light(lam, L)
+    lam_first = None
+    for _ in range(3):
+        lam, L, _ = b.fit_lambdas(th, lam, L)
+        if lam_first is None:
+            lam_first = lam.copy()
+        b.set_light(lam, L)
+        th, rep = b.fit_plain(th)
+    b.close()
+    ctx.close()
+    assert np.array_equal(th_a, th) and np.array_equal(lam_a, lam) and np.array_equal(L_a, L)
+    assert [r.stop for r in rep_a] == [r.stop for r in rep]
+    lo, _, _ = O.fit_lambdas(scene.oracle(), th0, lam0, np.full(n, 0.005))
+    assert np.abs(lam_first - lo).max() <= 1e-3 * np.abs(lo).max()
