@@ -313,34 +313,47 @@ def test_fit_batched_vs_oracle(scene, gpu):
 @pytest.mark.parametrize("which", ["lambdas", "lintensity"])
 def test_light_fit_vs_oracle(scene, which):
     """fit_3DMM_lambdas / fit_3DMM_Lintensity on the GPU (FP32 J^T J, FP64 3x3 solve) against
-    the FP64 oracle from the same FP32 starting light: same number of iterations and
-    line-search evaluations, light within 1e-3 relative."""
+    the FP64 oracle from the same FP32 starting light.  The illumination objective is nearly
+    flat in the light, so these fits, like whole shape fits, turn round-off into different
+    line-search decisions; the GPU result must lie within the oracle's own spread (its fits from
+    three starts 1e-5 relative away, floor 1e-3 relative) and reach an objective no worse than
+    the oracle's by more than 1e-3 relative."""
     if scene.opts.fit_light:
         pytest.skip("light sub-fits apply when the light is not a joint unknown")
     _say(f"case: light sub-fit {which}")
     rng = np.random.default_rng(21)
+    n = len(scene.frames)
     th = _f32(scene.truth())
     lam_t = np.stack([f.lam for f in scene.frames]).astype(np.float64)
     lam0 = _f32(lam_t + rng.normal(0, 12.0, lam_t.shape))
-    L0 = _f32(np.array([f.Lint for f in scene.frames]) * (1.0 + rng.normal(0, 0.1, len(scene.frames))))
+    L0 = _f32(np.array([f.Lint for f in scene.frames]) * (1.0 + rng.normal(0, 0.1, n)))
     orc = scene.oracle()
+    fit_o = O.fit_lambdas if which == "lambdas" else O.fit_lintensity
+    lo, Lo, ro = fit_o(orc, th, lam0, L0)
+    spread_l = np.zeros(n)
+    spread_L = np.zeros(n)
+    for _ in range(3):
+        s_l = 1.0 + 1e-5 * rng.choice([-1.0, 1.0], lam0.shape)
+        s_L = 1.0 + 1e-5 * rng.choice([-1.0, 1.0], L0.shape)
+        lp, Lp, _ = fit_o(orc, th, lam0 * s_l, L0 * s_L)
+        spread_l = np.maximum(spread_l, np.abs(lp - lo).max(axis=1))
+        spread_L = np.maximum(spread_L, np.abs(Lp - Lo))
     ctx = tdi.Context(scene.model, scene.opts)
     b = tdi.Batch(ctx, scene.frames)
     if which == "lambdas":
-        lo, Lo, ro = O.fit_lambdas(orc, th, lam0, L0)
         lg, Lg, rg = b.fit_lambdas(th, lam0, L0)
     else:
-        lo, Lo, ro = O.fit_lintensity(orc, th, lam0, L0)
         lg, Lg, rg = b.fit_lintensity(th, lam0, L0)
     b.close()
     ctx.close()
     _say(f"  oracle lam {lo.tolist()} L {Lo.tolist()} iters {[r.iters for r in ro]} ls {[r.ls_evals for r in ro]}")
     _say(f"  gpu    lam {lg.tolist()} L {Lg.tolist()} iters {[r.iters for r in rg]} ls {[r.ls_evals for r in rg]}")
-    assert [r.status for r in rg] == [abi.TDI_OK] * len(rg)
-    assert [(r.iters, r.ls_evals, r.stop) for r in rg] == [(r.iters, r.ls_evals, r.stop) for r in ro]
-    assert np.abs(lg - lo).max() <= 1e-3 * np.abs(lo).max()
-    assert np.abs(Lg - Lo).max() <= 1e-3 * np.abs(Lo).max()
-    assert np.allclose([r.objective for r in rg], [r.objective for r in ro], rtol=1e-4)
+    _say(f"  oracle spread lam {spread_l.tolist()} L {spread_L.tolist()}")
+    assert [r.status for r in rg] == [abi.TDI_OK] * n
+    for i in range(n):
+        assert np.abs(lg[i] - lo[i]).max() <= max(3 * spread_l[i], 1e-3 * np.abs(lo[i]).max()), i
+        assert abs(Lg[i] - Lo[i]) <= max(3 * spread_L[i], 1e-3 * abs(Lo[i])), i
+        assert rg[i].objective <= ro[i].objective + 1e-3 * abs(ro[i].objective), i
 
 
 def test_fit_video_frames_sequence(scene):
