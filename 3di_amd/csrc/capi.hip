@@ -106,6 +106,7 @@ struct Workspace {
     int* tcount = nullptr;                     // [C][tmax] resolve tile counts
     int tmax = 0;
     double* rt = nullptr;                      // [C][12] FP64 rotation (column-major) + translation
+    int4* box = nullptr;                       // [C] visible bounding box of the last render
     FitState* st = nullptr;
     float* partials = nullptr;
     float* lmk = nullptr;      // [C][6][51]
@@ -284,6 +285,7 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     w.tmax = resolve_tiles(b->max_area);
     w.tcount = A.get<int>((size_t)C * (w.tmax > 0 ? w.tmax : 1));
     w.rt = A.get<double>((size_t)C * 12);
+    w.box = A.get<int4>(C);
     w.st = A.get<FitState>(C);
     w.S = jtj_splits(C, b->max_area);
     const size_t pf = std::max(jtj_partials_floats(C, w.S, K, false), jtj_partials_floats(C, w.S, K, true));
@@ -403,16 +405,35 @@ extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd,
                 for (int j = 0; j < NLMK; ++j) hp[((size_t)i * 6 + q) * NLMK + j] = h[q * S + i * NLMK + j];
         (void)hipMemcpy(b->lmk, hp.data(), sizeof(float) * hp.size(), hipMemcpyHostToDevice);
     }
-    // workspace: whole batch if it fits a budget, else chunks
+    // workspace: the whole batch if it fits the budget, else chunks of C consecutive frames;
+    // the per-pixel buffers are sized by the largest actual crop-pixel sum over the chunks.
+    // Budget: TDI_WORKSPACE_GB, default half of the device memory free at this point.
     const char* env = getenv("TDI_WORKSPACE_GB");
-    double budget_gb = env ? atof(env) : 48.0;
-    const int V = c->m.V, K = c->L.K;
-    const double per_frame = (double)V * 40 + 64.0 * K * K;
-    const double per_pixel = 80.0 + 4.0 * K;
+    double budget_gb = 48.0;
+    if (env) {
+        budget_gb = atof(env);
+    } else {
+        size_t fb = 0, tb = 0;
+        if (hipMemGetInfo(&fb, &tb) == hipSuccess && fb > 0) budget_gb = 0.5 * (double)fb / 1e9;
+    }
+    const int V = c->m.V, K = c->Lx.K;
+    const double per_frame = (double)V * 40 + 64.0 * K * K + 8.0 * resolve_tiles(maxA);
+    const double per_pixel = 84.0 + 4.0 * K;
+    auto chunk_cap = [&](int Cc) {
+        long long best = 0;
+        for (int i0 = 0; i0 < n; i0 += Cc) {
+            const int i1 = std::min(n, i0 + Cc);
+            const long long e = (i1 < n ? b->hframes[i1].obs_off : off) - b->hframes[i0].obs_off;
+            best = std::max(best, e);
+        }
+        return best;
+    };
     int C = n;
     long long cap = off;
-    while (C > 1 && (per_frame * C + per_pixel * (double)maxA * C) > budget_gb * 1e9) C = (C + 1) / 2;
-    if (C < n) cap = (long long)maxA * C;
+    while (C > 1 && (per_frame * C + per_pixel * (double)cap) > budget_gb * 1e9) {
+        C = (C + 1) / 2;
+        cap = chunk_cap(C);
+    }
     int rc = ws_alloc(b, C, cap);
     if (rc) { b->mem.release(); delete b; return rc; }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -443,8 +464,10 @@ extern "C" int tdi_batch_destroy(tdi_batch* b)
 // ---------------------------------------------------------------------------- evaluation
 // Objective at theta for the frames in `mask`: synthesis, face check, raster, resolve,
 // barrier objective, shading, blur, gradients (Renderer::render + evaluate_objective_function).
+// full_crop: blur the whole crops (the rendered image is an output), else only the region
+// the normalised gradients read.
 static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s,
-                       const Layout* Lp = nullptr)
+                       const Layout* Lp = nullptr, bool full_crop = false)
 {
     tdi_ctx* c = b->ctx;
     const Layout& L = Lp ? *Lp : c->L;
@@ -456,17 +479,17 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
         rc = launch_synthesize(c->m, L, w.frames, fl, theta, w.rt, w.P0, w.P, w.vproj, s);
     if (rc) return rc;
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
-    if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.max_area, s))) return rc;
+    if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
     if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, s)))
+    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, w.box, s)))
         return rc;
     if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
         return rc;
-    if ((rc = launch_shade(c->m, L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
-                           w.max_area, s)))
+    if ((rc = launch_shade(c->m, L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary,
+                           full_crop ? nullptr : w.box, w.img, w.max_area, s)))
         return rc;
-    if ((rc = launch_blur(w.frames, fl, c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
+    if ((rc = launch_blur(w.frames, fl, c->taps, full_crop ? nullptr : w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
     return launch_grads(w.frames, fl, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
                         w.pcoef, w.obj, w.max_area, s);
 }
@@ -555,7 +578,7 @@ int tdi_rasterize(tdi_batch* b, const float* xyz, const float* xyp, uint64_t* ke
         if ((rc = launch_clear_keys(w.keys, len, s))) return rc;
         if ((rc = launch_misc_fill_f32(w.bary, 2 * len, 0.f, s))) return rc;
         if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, s)))
+        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, nullptr, s)))
             return rc;
         if (keys) TDI_HIP(hipMemcpyAsync(keys + off, w.keys, sizeof(uint64_t) * len, hipMemcpyDeviceToDevice, s));
         if (bary) TDI_HIP(hipMemcpyAsync(bary + 2 * off, w.bary, sizeof(float) * 2 * len, hipMemcpyDeviceToDevice, s));
@@ -574,7 +597,7 @@ int tdi_render(tdi_batch* b, const float* theta, float* img, float* obj, int32_t
         if (rc) return rc;
         const long long off = b->hframes[i0].obs_off;
         const long long len = (i0 + n < b->n ? b->hframes[i0 + n].obs_off : b->total) - off;
-        if ((rc = eval_render(b, all_frames(n), theta + (size_t)i0 * K, 0, s))) return rc;
+        if ((rc = eval_render(b, all_frames(n), theta + (size_t)i0 * K, 0, s, nullptr, true))) return rc;
         if (img) TDI_HIP(hipMemcpyAsync(img + off, w.img, sizeof(float) * len, hipMemcpyDeviceToDevice, s));
         if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj + i0, n, s))) return rc;
         if (npix) TDI_HIP(hipMemcpyAsync(npix + i0, w.npix, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
@@ -640,15 +663,15 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
             rc = launch_synthesize(c->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, w.P, w.vproj, s);
         if (rc) return rc;
         TDI_HIP(hipEventRecord(ev[1], s));
-        if ((rc = launch_clear_keys_frames(w.frames, all_frames(n), w.keys, w.max_area, s))) return rc;
+        if ((rc = launch_clear_keys_frames(w.frames, all_frames(n), w.keys, w.box, w.max_area, s))) return rc;
         if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, s)))
+        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, w.box, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[2], s));
-        if ((rc = launch_shade(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.img,
+        if ((rc = launch_shade(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.box, w.img,
                                w.max_area, s)))
             return rc;
-        if ((rc = launch_blur(w.frames, all_frames(n), c->taps, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
+        if ((rc = launch_blur(w.frames, all_frames(n), c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
         if ((rc = launch_grads(w.frames, all_frames(n), w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg,
                                w.pnb, w.pcoef, w.obj,
                                w.max_area, s)))

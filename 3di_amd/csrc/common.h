@@ -250,20 +250,23 @@ int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* f
 
 // raster.hip
 int launch_clear_keys(uint64_t* keys, long long total, hipStream_t s);
-int launch_clear_keys_frames(const FrameGPU* frames, FrameList fl, uint64_t* keys, int max_area, hipStream_t s);
+// box: [slots] visible bounding box (xmin, xmax, ymin, ymax), reset here, filled by launch_resolve
+int launch_clear_keys_frames(const FrameGPU* frames, FrameList fl, uint64_t* keys, int4* box, int max_area,
+                             hipStream_t s);
 int launch_raster(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj, int ntmp,
                   uint64_t* keys, hipStream_t s);
 // tcount: [slots][tmax] tile counts, tmax >= resolve_tiles(largest crop)
 int resolve_tiles(int max_area);
 int launch_resolve(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj,
                    const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int max_area,
-                   int* tcount, int tmax, hipStream_t s);
+                   int* tcount, int tmax, int4* box, hipStream_t s);
 
 // shade.hip
 int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
                  const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
-                 const float* bary, float* img, int max_area, hipStream_t s);
-int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const float* in, float* tmp,
+                 const float* bary, const int4* box, float* img, int max_area, hipStream_t s);
+// box == nullptr: whole crops; else only the region the normalised gradients need
+int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const int4* box, const float* in, float* tmp,
                 float* out, int max_area, hipStream_t s);
 int launch_grads(const FrameGPU* frames, FrameList fl, const int* vis_list, const int* npix,
                  const int* cum, const float* blurred, const float2* obs_g, float w, float4* pixg, int4* pnb,

@@ -98,7 +98,8 @@ __device__ __forceinline__ int tile_keys(const unsigned long long* __restrict__ 
 __global__ void __launch_bounds__(RES_THREADS) k_resolve_count(const FrameGPU* __restrict__ frames,
                                                                 const int* __restrict__ act, int n,
                                                                 const unsigned long long* __restrict__ keys,
-                                                                int* __restrict__ tcount, int tmax)
+                                                                int* __restrict__ tcount, int tmax,
+                                                                int4* __restrict__ box)
 {
     __shared__ int swave[RES_NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -110,9 +111,28 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_count(const FrameGPU* _
         const int ntile = (A + RES_TILE - 1) / RES_TILE;
         for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
             unsigned long long key[RES_PER_THREAD];
-            const int cnt = tile_keys(keys + fr.pix_off, tile * RES_TILE + RES_PER_THREAD * tid, A, key);
+            const int k0 = tile * RES_TILE + RES_PER_THREAD * tid;
+            const int cnt = tile_keys(keys + fr.pix_off, k0, A, key);
             const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
             if (lane == 0) swave[wave] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+            if (box && (b0 | b1 | b2)) {
+                // bounding box of the visible pixels (x = k / H, y = k % H)
+                int xl = fr.W, xh = -1, yl = fr.H, yh = -1;
+#pragma unroll
+                for (int q = 0; q < RES_PER_THREAD; ++q) {
+                    if (key[q] == KEY_EMPTY) continue;
+                    const int k = k0 + q, x = k / fr.H, y = k - x * fr.H;
+                    xl = min(xl, x); xh = max(xh, x); yl = min(yl, y); yh = max(yh, y);
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    xl = min(xl, __shfl_xor(xl, o, 64)); xh = max(xh, __shfl_xor(xh, o, 64));
+                    yl = min(yl, __shfl_xor(yl, o, 64)); yh = max(yh, __shfl_xor(yh, o, 64));
+                }
+                if (lane == 0) {
+                    atomicMin(&box[f].x, xl); atomicMax(&box[f].y, xh);
+                    atomicMin(&box[f].z, yl); atomicMax(&box[f].w, yh);
+                }
+            }
             __syncthreads();
             if (tid == 0) {
                 int t = 0;
@@ -203,11 +223,13 @@ int launch_clear_keys(uint64_t* keys, long long total, hipStream_t s)
 namespace {
 __global__ void __launch_bounds__(256) k_clear_keys_frames(const FrameGPU* __restrict__ frames,
                                                             const int* __restrict__ act, int n,
-                                                            uint64_t* __restrict__ keys)
+                                                            uint64_t* __restrict__ keys, int4* __restrict__ box)
 {
     const int nact = fl_count(act, n);
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
-        const FrameGPU fr = frames[fl_frame(act, yi)];
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        if (box && blockIdx.x == 0 && threadIdx.x == 0) box[f] = make_int4(fr.W, -1, fr.H, -1);
         const int A = fr.W * fr.H;
         for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x)
             keys[fr.pix_off + k] = KEY_EMPTY;
@@ -215,13 +237,14 @@ __global__ void __launch_bounds__(256) k_clear_keys_frames(const FrameGPU* __res
 }
 } // namespace
 
-int launch_clear_keys_frames(const FrameGPU* frames, FrameList fl, uint64_t* keys, int max_area, hipStream_t s)
+int launch_clear_keys_frames(const FrameGPU* frames, FrameList fl, uint64_t* keys, int4* box, int max_area,
+                             hipStream_t s)
 {
     int bx = (max_area + 255) / 256;
     if (bx > 256) bx = 256;
     if (bx < 1) bx = 1;
     hipLaunchKernelGGL(k_clear_keys_frames, dim3(bx, grid_frames(fl.n, 64)), dim3(256), 0, s, frames, fl.act, fl.n,
-                       keys);
+                       keys, box);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -240,7 +263,7 @@ int resolve_tiles(int max_area) { return (max_area + RES_TILE - 1) / RES_TILE; }
 
 int launch_resolve(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj,
                    const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int max_area,
-                   int* tcount, int tmax, hipStream_t s)
+                   int* tcount, int tmax, int4* box, hipStream_t s)
 {
     int bx = resolve_tiles(max_area);
     if (bx > tmax) {
@@ -250,7 +273,7 @@ int launch_resolve(const ModelGPU& m, const FrameGPU* frames, FrameList fl, cons
     if (bx < 1) bx = 1;
     const dim3 grid(bx, grid_frames(fl.n, 64));
     hipLaunchKernelGGL(k_resolve_count, grid, dim3(RES_THREADS), 0, s, frames, fl.act, fl.n,
-                       (const unsigned long long*)keys, tcount, tmax);
+                       (const unsigned long long*)keys, tcount, tmax, box);
     hipLaunchKernelGGL(k_resolve_write, grid, dim3(RES_THREADS), 0, s, m.tris, m.V, frames, fl.act, fl.n, vproj,
                        (const unsigned long long*)keys, tcount, tmax, cum, vis_list, npix, (float2*)bary);
     TDI_HIP(hipGetLastError());

@@ -21,15 +21,44 @@ struct Taps {
 
 namespace {
 
+// Region of a crop a pass has to cover, half-open [X0, X1) x [Y0, Y1).  Without a box (or
+// when the visible pixels touch the crop border, whose neighbour rule reads crop index 0) it
+// is the whole crop.  With the visible box (xl, xh, yl, yh) of the frame: the normalised
+// gradients read the blurred image at visible pixels +-1, the blur along x reaches 7 pixels
+// and the blur along y 7 rows, so pass 1 (along x) covers the box + 1, pass 0 (along y) the
+// box + 8 columns / + 1 row, and the rendered image must be clear on the box + 8.
+struct Region {
+    int X0, X1, Y0, Y1;
+};
+
+__device__ __forceinline__ bool crop_region(const FrameGPU& fr, const int4* box, int f, int mx, int my, Region& r)
+{
+    r = Region{0, fr.W, 0, fr.H};
+    if (!box) return true;
+    const int4 b = box[f];
+    if (b.y < 0) return false;                  // nothing visible
+    if (b.x == 0 || b.y == fr.W - 1 || b.z == 0 || b.w == fr.H - 1) return true;
+    r.X0 = max(0, b.x - mx);
+    r.X1 = min(fr.W, b.y + mx + 1);
+    r.Y0 = max(0, b.z - my);
+    r.Y1 = min(fr.H, b.w + my + 1);
+    return true;
+}
+
 __global__ void __launch_bounds__(256) k_clear_crops(const FrameGPU* __restrict__ frames, const int* __restrict__ act,
-                                                      int n, float* __restrict__ img)
+                                                      int n, const int4* __restrict__ box, float* __restrict__ img)
 {
     const int nact = fl_count(act, n);
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
-        const FrameGPU fr = frames[fl_frame(act, yi)];
-        const int A = fr.W * fr.H;
-        for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x)
-            img[fr.pix_off + k] = 0.f;
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        Region r;
+        if (!crop_region(fr, box, f, 8, 8, r)) continue;
+        const int Rh = r.Y1 - r.Y0, R = (r.X1 - r.X0) * Rh;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x) {
+            const int x = r.X0 + i / Rh, y = r.Y0 + i % Rh;
+            img[fr.pix_off + (long long)x * fr.H + y] = 0.f;
+        }
     }
 }
 
@@ -99,15 +128,21 @@ __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const Frame
 
 // 15-tap separable blur, zero outside the crop.  dir 0 = along y (contiguous), 1 = along x.
 __global__ void __launch_bounds__(256) k_blur(const FrameGPU* __restrict__ frames, const int* __restrict__ act, int n,
-                                               Taps taps, const float* __restrict__ in, float* __restrict__ out, int dir)
+                                               const int4* __restrict__ box, Taps taps, const float* __restrict__ in,
+                                               float* __restrict__ out, int dir)
 {
     const int nact = fl_count(act, n);
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
-    const FrameGPU fr = frames[fl_frame(act, yi)];
-    const int W = fr.W, H = fr.H, A = W * H;
+    const int f = fl_frame(act, yi);
+    const FrameGPU fr = frames[f];
+    const int W = fr.W, H = fr.H;
+    Region rg;
+    if (!crop_region(fr, box, f, dir == 0 ? 8 : 1, 1, rg)) continue;
+    const int Rh = rg.Y1 - rg.Y0, R = (rg.X1 - rg.X0) * Rh;
     const float* src = in + fr.pix_off;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
-        const int x = k / H, y = k - x * H;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x) {
+        const int x = rg.X0 + i / Rh, y = rg.Y0 + i % Rh;
+        const int k = x * H + y;
         float s = 0.f;
         if (dir == 0) {
 #pragma unroll
@@ -214,26 +249,26 @@ inline dim3 crop_grid(int max_area, int n)
 
 int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
                  const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
-                 const float* bary, float* img, int max_area, hipStream_t s)
+                 const float* bary, const int4* box, float* img, int max_area, hipStream_t s)
 {
     if (L.ntex > 512) {
         set_error("shade: k_tex %d > 512", L.ntex);
         return TDI_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_clear_crops, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, img);
+    hipLaunchKernelGGL(k_clear_crops, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, img);
     hipLaunchKernelGGL(k_shade, crop_grid(max_area, fl.n), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0, P,
                        vis_list, npix, (const unsigned long long*)keys, (const float2*)bary, img);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
 
-int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const float* in, float* tmp,
+int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const int4* box, const float* in, float* tmp,
                 float* out, int max_area, hipStream_t s)
 {
     Taps t;
     for (int i = 0; i < 15; ++i) t.t[i] = taps[i];
-    hipLaunchKernelGGL(k_blur, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, t, in, tmp, 0);
-    hipLaunchKernelGGL(k_blur, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, t, tmp, out, 1);
+    hipLaunchKernelGGL(k_blur, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, t, in, tmp, 0);
+    hipLaunchKernelGGL(k_blur, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, t, tmp, out, 1);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -251,7 +286,7 @@ int launch_grads(const FrameGPU* frames, FrameList fl, const int* vis_list, cons
 int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const float* obs, float* tmp, float* blurred,
                        float2* obs_g, int max_area, hipStream_t s)
 {
-    int rc = launch_blur(frames, all_frames(n), taps, obs, tmp, blurred, max_area, s);
+    int rc = launch_blur(frames, all_frames(n), taps, nullptr, obs, tmp, blurred, max_area, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_obs_grad, crop_grid(max_area, n), dim3(256), 0, s, frames, n, blurred, obs_g);
     TDI_HIP(hipGetLastError());
