@@ -356,14 +356,32 @@ __device__ __forceinline__ unsigned short bf16_rne(float x)
 __device__ __forceinline__ float bf16_f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
 // ------------------------------------------------------------------ register-direct MFMA J^T J
-// One wavefront per workgroup.  The wave walks its pixel range eight pixels (16 rows of
-// [J | r]) at a time.  For v_mfma_f32_16x16x16bf16_1k lane l supplies rows 4*(l/16)..+3 of
-// column l%16 of a 16-column block, for the A operand (J^T) and the B operand (J) alike, so
-// each lane computes exactly the J entries it feeds to the matrix cores: two pixels
-// (Jx and Jy rows of each) times one column of every block.  The entries are split into BF16
-// parts in registers and every upper-triangle tile is accumulated from registers; nothing is
-// staged through LDS and the loop has no barrier.  Accumulator layout: lane l, element r ->
-// C[4*(l/16) + r][l%16].
+// The two rows the reference forms per pixel are parallel: Jx = c dy s, Jy = -c dx s with
+// s = dy (dI[r] - dI[l]) - dx (dI[b] - dI[a]) (newfuncs.cu:417-463), so
+//   Jx Jx^T + Jy Jy^T = c^2 (dx^2 + dy^2) s s^T   and   Jx g^s_x + Jy g^s_y = c (g^s_x dy - g^s_y dx) s.
+// One row per pixel, u = c |d| s (|d| = sqrt(dx^2 + dy^2)), gives the same normal equations
+// with half the rows.  J^T r is accumulated in FP32 from q s, q = c (g^s_x dy - g^s_y dx); the
+// augmented column K of [u | r] carries r = q / (c |d|) for the MFMA form of J^T r.
+//
+// One wavefront per workgroup walks its pixel range sixteen pixels at a time (16 rows of
+// [u | r]).  For v_mfma_f32_16x16x16bf16_1k lane l supplies rows 4*(l/16)..+3 of column l%16 of
+// a 16-column block, for the A operand (u^T) and the B operand (u) alike, so each lane computes
+// exactly the entries it feeds to the matrix cores: four pixels times one column of every
+// block.  The entries are split into BF16 parts (hi/mid/lo) with the packed round-to-nearest
+// conversion, two rows per instruction, which is the operand layout; every upper-triangle tile
+// is accumulated from registers.  Accumulator layout: lane l, element r -> C[4*(l/16) + r][l%16].
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b)
+{
+    const bf16x2 v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(unsigned, v);
+}
+__device__ __forceinline__ float lo_f(unsigned p) { return __uint_as_float(p << 16); }
+__device__ __forceinline__ float hi_f(unsigned p) { return __uint_as_float(p & 0xffff0000u); }
+
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
+
 template <int T>
 __global__ void __launch_bounds__(64) k_jtj_reg(Layout L, const FrameGPU* __restrict__ frames,
                                                  const int* __restrict__ act, int n, const int* __restrict__ npix,
@@ -389,17 +407,21 @@ __global__ void __launch_bounds__(64) k_jtj_reg(Layout L, const FrameGPU* __rest
 #pragma unroll
     for (int c = 0; c < T; ++c) gacc[c] = 0.f;
 
-    for (int kb = p0; kb < p1; kb += 8) {
-        float jv[T][4];                 // rows: (pixel a: Jx, Jy), (pixel b: Jx, Jy)
+    for (int kb = p0; kb < p1; kb += 16) {
+        float uv[T][4];                 // rows: this lane's four pixels
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int pos = kb + 2 * grp + q;
+        for (int q = 0; q < 4; ++q) {
+            const int pos = kb + 4 * grp + q;
             if (pos < p1) {
                 const long long o = fr.pix_off + pos;
                 const int4 nb = pnb[o];
                 const float c = pcoef[o];
                 const float4 gg = pixg[o];
                 const float dx = 2.f * gg.x, dy = 2.f * gg.y;
+                const float dn = sqrtf(dx * dx + dy * dy);
+                const float wgt = c * dn;                          // row scale c |d|
+                const float qg = c * (gg.z * dy - gg.w * dx);      // J^T r weight
+                const float rK = dn > 0.f ? (gg.z * dy - gg.w * dx) / dn : 0.f;
                 const float* rl = dIf + (size_t)nb.x * K;
                 const float* rr = dIf + (size_t)nb.y * K;
                 const float* ra = dIf + (size_t)nb.z * K;
@@ -407,39 +429,40 @@ __global__ void __launch_bounds__(64) k_jtj_reg(Layout L, const FrameGPU* __rest
 #pragma unroll
                 for (int cb = 0; cb < T; ++cb) {
                     const int j = 16 * cb + col;
-                    float vx = 0.f, vy = 0.f;
+                    float v = 0.f;
                     if (j < K) {
-                        const float dh = rr[j] - rl[j];
-                        const float dv = rb[j] - ra[j];
-                        const float sv = dy * dh - dx * dv;
-                        vx = c * dy * sv;
-                        vy = -c * dx * sv;
-                        gacc[cb] = fmaf(vx, gg.z, fmaf(vy, gg.w, gacc[cb]));   // J^T r in FP32
+                        const float sv = dy * (rr[j] - rl[j]) - dx * (rb[j] - ra[j]);
+                        v = wgt * sv;
+                        gacc[cb] = fmaf(qg, sv, gacc[cb]);          // J^T r in FP32
                     } else if (j == K) {
-                        vx = gg.z;
-                        vy = gg.w;
+                        v = rK;
                     }
-                    jv[cb][2 * q] = vx;
-                    jv[cb][2 * q + 1] = vy;
+                    uv[cb][q] = v;
                 }
             } else {
 #pragma unroll
-                for (int cb = 0; cb < T; ++cb) jv[cb][2 * q] = jv[cb][2 * q + 1] = 0.f;
+                for (int cb = 0; cb < T; ++cb) uv[cb][q] = 0.f;
             }
         }
+        // BF16 parts, two rows per packed conversion: x = hi + mid + lo
         v4s fh[T], fm[T], fl[T];
 #pragma unroll
         for (int cb = 0; cb < T; ++cb) {
+            v2u h, m, l;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float x = jv[cb][r];
-                const unsigned short h = bf16_rne(x);
-                const float rx = x - bf16_f(h);
-                const unsigned short mdl = bf16_rne(rx);
-                fh[cb][r] = (short)h;
-                fm[cb][r] = (short)mdl;
-                fl[cb][r] = (short)bf16_rne(rx - bf16_f(mdl));
+            for (int pr = 0; pr < 2; ++pr) {
+                const float a = uv[cb][2 * pr], b = uv[cb][2 * pr + 1];
+                const unsigned ph = pack_bf16(a, b);
+                const float ra = a - lo_f(ph), rb = b - hi_f(ph);
+                const unsigned pm = pack_bf16(ra, rb);
+                const unsigned pl = pack_bf16(ra - lo_f(pm), rb - hi_f(pm));
+                h[pr] = ph;
+                m[pr] = pm;
+                l[pr] = pl;
             }
+            fh[cb] = __builtin_bit_cast(v4s, h);
+            fm[cb] = __builtin_bit_cast(v4s, m);
+            fl[cb] = __builtin_bit_cast(v4s, l);
         }
         int t = 0;
 #pragma unroll
