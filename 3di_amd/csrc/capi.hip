@@ -474,9 +474,9 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
     Workspace& w = b->ws;
     int rc;
     if (mixed & TDI_MIXED_SYNTH_MFMA)
-        rc = launch_synthesize_gemm(c->m, L, w.frames, fl, theta, w.rt, w.P0, w.P, w.vproj, s);
+        rc = launch_synthesize_gemm(c->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
     else
-        rc = launch_synthesize(c->m, L, w.frames, fl, theta, w.rt, w.P0, w.P, w.vproj, s);
+        rc = launch_synthesize(c->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
     if (rc) return rc;
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
     if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
@@ -516,7 +516,7 @@ static int eval_full(tdi_batch* b, FrameList fl, const float* theta, int mixed, 
     Workspace& w = b->ws;
     int rc = eval_render(b, fl, theta, mixed, s, &L);
     if (rc) return rc;
-    if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
+    if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
                         w.max_area, s)))
         return rc;
     if ((rc = launch_normal_eq(b, fl, mixed, s, &L))) return rc;
@@ -658,9 +658,9 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
     for (int r = 0; r < reps; ++r) {
         TDI_HIP(hipEventRecord(ev[0], s));
         if (mixed & TDI_MIXED_SYNTH_MFMA)
-            rc = launch_synthesize_gemm(c->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, w.P, w.vproj, s);
+            rc = launch_synthesize_gemm(c->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, nullptr, w.vproj, s);
         else
-            rc = launch_synthesize(c->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, w.P, w.vproj, s);
+            rc = launch_synthesize(c->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, nullptr, w.vproj, s);
         if (rc) return rc;
         TDI_HIP(hipEventRecord(ev[1], s));
         if ((rc = launch_clear_keys_frames(w.frames, all_frames(n), w.keys, w.box, w.max_area, s))) return rc;
@@ -681,7 +681,7 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
                                  w.objb, w.H, w.g, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[4], s));
-        if ((rc = launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
+        if ((rc = launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
                             w.max_area, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[5], s));

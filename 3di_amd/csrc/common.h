@@ -242,7 +242,8 @@ struct LmkArrays {
 };
 
 // geometry.hip
-// rt: [slots][12] scratch for the per-frame FP64 rotation (column-major) and translation
+// rt: [slots][12] scratch for the per-frame FP64 rotation (column-major) and translation;
+// P (camera-space vertices) may be null: the fit itself only needs P0 and the projections
 int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
                       double* rt, float* P0, float* P, float4* vproj, hipStream_t s);
 int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
@@ -276,7 +277,7 @@ int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const f
 
 // jacobian.hip
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-              const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
+              const float* P0, const int* vis_list, const int* npix, const uint64_t* keys,
               const float* bary, const float4* pixg, float* dI, int max_area, hipStream_t s);
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);

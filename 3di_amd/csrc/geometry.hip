@@ -33,9 +33,11 @@ __device__ __forceinline__ void store_vertex(const FrameGPU& fr, const double* R
     const double X = R[0] * p[0] + R[3] * p[1] + R[6] * p[2] + tau[0];
     const double Y = R[1] * p[0] + R[4] * p[1] + R[7] * p[2] + tau[1];
     const double Z = R[2] * p[0] + R[5] * p[1] + R[8] * p[2] + tau[2];
-    P[3 * o + 0] = (float)X;
-    P[3 * o + 1] = (float)Y;
-    P[3 * o + 2] = (float)Z;
+    if (P) {                               // camera-space vertices only when asked for
+        P[3 * o + 0] = (float)X;
+        P[3 * o + 1] = (float)Y;
+        P[3 * o + 2] = (float)Z;
+    }
     vproj[o] = make_float4((float)((double)fr.phix * X / Z + (double)fr.cx), (float)((double)fr.phiy * Y / Z + (double)fr.cy),
                            (float)Z, 0.f);
 }

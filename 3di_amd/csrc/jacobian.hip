@@ -52,7 +52,7 @@ __device__ inline void load_frame_rot(const Layout& L, const FrameGPU& fr, const
 
 __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                              const int* __restrict__ act, int n, const float* __restrict__ theta,
-                                             const float* __restrict__ P0, const float* __restrict__ P,
+                                             const float* __restrict__ P0,
                                              const int* __restrict__ vis, const int* __restrict__ npix,
                                              const unsigned long long* __restrict__ keys,
                                              const float2* __restrict__ bary, const float4* __restrict__ pixg,
@@ -74,7 +74,6 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
     const int wave = uniform(threadIdx.x >> 6);
     const int sub = lane >> 4, l16 = lane & 15;      // four pixels per wavefront, 16 lanes each
     const float* P0f = P0 + (size_t)f * m.V * 3;
-    const float* Pf = P + (size_t)f * m.V * 3;
     const int K = L.K;
     for (int pos0 = (blockIdx.x * 4 + wave) * 4; pos0 < N; pos0 += gridDim.x * 16) {
         const int pos = pos0 + sub;
@@ -86,10 +85,10 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
         const int vb = m.tris[3 * t], vc = m.tris[3 * t + 1], va = m.tris[3 * t + 2];
         float v[3], p0[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            v[a] = al * Pf[3 * vb + a] + be * Pf[3 * vc + a] + ga * Pf[3 * va + a];
-            p0[a] = al * P0f[3 * vb + a] + be * P0f[3 * vc + a] + ga * P0f[3 * va + a];
-        }
+        for (int a = 0; a < 3; ++a) p0[a] = al * P0f[3 * vb + a] + be * P0f[3 * vc + a] + ga * P0f[3 * va + a];
+        // camera-space surface point R p0 + tau (R column-major)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = sr.R[a] * p0[0] + sr.R[3 + a] * p0[1] + sr.R[6 + a] * p0[2] + sr.tau[a];
         const float4 gg = pixg[base + pos];
         const float iz = 1.f / v[2];
         // dI = c . dv with c = (-gx phix / vz, -gy phiy / vz, (gx phix vx + gy phiy vy) / vz^2)
@@ -512,7 +511,7 @@ int jtj_splits(int n, int per_frame_pixels_hint)
 }
 
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-              const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
+              const float* P0, const int* vis_list, const int* npix, const uint64_t* keys,
               const float* bary, const float4* pixg, float* dI, int max_area, hipStream_t s)
 {
     if (L.ntex > 512) {
@@ -523,7 +522,7 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameL
     if (bx > 512) bx = 512;
     if (bx < 1) bx = 1;
     hipLaunchKernelGGL(k_dI, dim3(bx, grid_frames(fl.n, 64)), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0,
-                       P, vis_list, npix, (const unsigned long long*)keys, (const float2*)bary, pixg, dI);
+                       vis_list, npix, (const unsigned long long*)keys, (const float2*)bary, pixg, dI);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

@@ -151,6 +151,7 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_write(const int* __rest
                                                                 const float4* __restrict__ vproj,
                                                                 const unsigned long long* __restrict__ keys,
                                                                 const int* __restrict__ tcount, int tmax,
+                                                                const int4* __restrict__ box,
                                                                 int* __restrict__ cum, int* __restrict__ vis,
                                                                 int* __restrict__ npix, float2* __restrict__ bary)
 {
@@ -166,7 +167,22 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_write(const int* __rest
         const long long base = fr.pix_off;
         const int A = fr.W * fr.H;
         const int ntile = (A + RES_TILE - 1) / RES_TILE;
+        // With the visible box known, only the tiles over its columns +-1 are ranked: the
+        // neighbour lookups read cum there and at crop index 0 (the border rule); the last
+        // tile always runs because it writes the pixel count.
+        int t_lo = 0, t_hi = ntile - 1;
+        if (box) {
+            const int4 bb = box[f];
+            if (bb.y < 0) {
+                t_lo = 1;
+                t_hi = 0;
+            } else {
+                t_lo = (max(bb.x - 1, 0) * fr.H) / RES_TILE;
+                t_hi = (min(bb.y + 2, fr.W) * fr.H - 1) / RES_TILE;
+            }
+        }
         for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+            if (tile != 0 && tile != ntile - 1 && (tile < t_lo || tile > t_hi)) continue;
             // offset of this tile: the preceding tiles' counts
             int part = 0;
             for (int j = tid; j < tile; j += RES_THREADS) part += tcount[(size_t)f * tmax + j];
@@ -275,7 +291,7 @@ int launch_resolve(const ModelGPU& m, const FrameGPU* frames, FrameList fl, cons
     hipLaunchKernelGGL(k_resolve_count, grid, dim3(RES_THREADS), 0, s, frames, fl.act, fl.n,
                        (const unsigned long long*)keys, tcount, tmax, box);
     hipLaunchKernelGGL(k_resolve_write, grid, dim3(RES_THREADS), 0, s, m.tris, m.V, frames, fl.act, fl.n, vproj,
-                       (const unsigned long long*)keys, tcount, tmax, cum, vis_list, npix, (float2*)bary);
+                       (const unsigned long long*)keys, tcount, tmax, box, cum, vis_list, npix, (float2*)bary);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

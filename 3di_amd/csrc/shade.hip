@@ -54,11 +54,9 @@ __global__ void __launch_bounds__(256) k_clear_crops(const FrameGPU* __restrict_
         const FrameGPU fr = frames[f];
         Region r;
         if (!crop_region(fr, box, f, 8, 8, r)) continue;
-        const int Rh = r.Y1 - r.Y0, R = (r.X1 - r.X0) * Rh;
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x) {
-            const int x = r.X0 + i / Rh, y = r.Y0 + i % Rh;
-            img[fr.pix_off + (long long)x * fr.H + y] = 0.f;
-        }
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+        for (int x = r.X0 + blockIdx.x * nw + wave; x < r.X1; x += gridDim.x * nw)
+            for (int y = r.Y0 + lane; y < r.Y1; y += 64) img[fr.pix_off + (long long)x * fr.H + y] = 0.f;
     }
 }
 
@@ -126,39 +124,54 @@ __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const Frame
     }
 }
 
-// 15-tap separable blur, zero outside the crop.  dir 0 = along y (contiguous), 1 = along x.
+// One pass of the 15-tap separable blur over a frame's region: one wavefront per column x, its
+// lanes over the rows y (column-major crops, so a wavefront's loads and stores are contiguous).
+// dir 0 blurs along y, dir 1 along x; samples outside the crop count as zero.
 __global__ void __launch_bounds__(256) k_blur(const FrameGPU* __restrict__ frames, const int* __restrict__ act, int n,
                                                const int4* __restrict__ box, Taps taps, const float* __restrict__ in,
                                                float* __restrict__ out, int dir)
 {
     const int nact = fl_count(act, n);
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
-    const int f = fl_frame(act, yi);
-    const FrameGPU fr = frames[f];
-    const int W = fr.W, H = fr.H;
-    Region rg;
-    if (!crop_region(fr, box, f, dir == 0 ? 8 : 1, 1, rg)) continue;
-    const int Rh = rg.Y1 - rg.Y0, R = (rg.X1 - rg.X0) * Rh;
-    const float* src = in + fr.pix_off;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < R; i += gridDim.x * blockDim.x) {
-        const int x = rg.X0 + i / Rh, y = rg.Y0 + i % Rh;
-        const int k = x * H + y;
-        float s = 0.f;
-        if (dir == 0) {
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        const int W = fr.W, H = fr.H;
+        Region rg;
+        if (!crop_region(fr, box, f, dir == 0 ? 8 : 1, 1, rg)) continue;
+        const float* src = in + fr.pix_off;
+        float* dst = out + fr.pix_off;
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+        for (int x = rg.X0 + blockIdx.x * nw + wave; x < rg.X1; x += gridDim.x * nw) {
+            const float* col = src + (size_t)x * H;
+            const bool xin = x >= 7 && x + 7 < W;
+            for (int y = rg.Y0 + lane; y < rg.Y1; y += 64) {
+                float s = 0.f;
+                if (dir == 0) {
+                    if (y >= 7 && y + 7 < H) {
 #pragma unroll
-            for (int j = -7; j <= 7; ++j) {
-                const int yy = y + j;
-                if (yy >= 0 && yy < H) s = fmaf(taps.t[7 - j], src[k + j], s);
-            }
-        } else {
+                        for (int j = -7; j <= 7; ++j) s = fmaf(taps.t[7 - j], col[y + j], s);
+                    } else {
 #pragma unroll
-            for (int j = -7; j <= 7; ++j) {
-                const int xx = x + j;
-                if (xx >= 0 && xx < W) s = fmaf(taps.t[7 - j], src[k + j * H], s);
+                        for (int j = -7; j <= 7; ++j) {
+                            const int yy = y + j;
+                            if (yy >= 0 && yy < H) s = fmaf(taps.t[7 - j], col[yy], s);
+                        }
+                    }
+                } else {
+                    if (xin) {
+#pragma unroll
+                        for (int j = -7; j <= 7; ++j) s = fmaf(taps.t[7 - j], col[(long long)j * H + y], s);
+                    } else {
+#pragma unroll
+                        for (int j = -7; j <= 7; ++j) {
+                            const int xx = x + j;
+                            if (xx >= 0 && xx < W) s = fmaf(taps.t[7 - j], col[(long long)j * H + y], s);
+                        }
+                    }
+                }
+                dst[(size_t)x * H + y] = s;
             }
         }
-        out[fr.pix_off + k] = s;
-    }
     }
 }
 
