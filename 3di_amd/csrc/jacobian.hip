@@ -17,13 +17,14 @@
 //                 independent gather chains (triangle -> vertices -> basis rows) are in flight
 //                 per wave; basis rows are read with unit stride across the 16 lanes.
 // k_jtj_f32     : FP32 J^T J / J^T r from LDS-staged J rows, 4x4 register tiles (plain path).
-// k_jtj_mfma    : J staged in LDS as BF16 parts (hi/mid/lo), [J | r]^T [J | r] on
-//                 v_mfma_f32_16x16x16bf16_1k with FP32 accumulation (six products, or three
-//                 for the two-term split).
-//                 The residual r = g^s rides along as an extra column, so J^T r comes out of
-//                 the same MFMA tiles; J^T r is also accumulated in FP32 while J is staged, and
-//                 the caller picks which one feeds the gradient (the GN fixed point depends on
-//                 g alone, so FP32 J^T r keeps batched trajectories on the FP64 ones).
+// k_jtj_lds     : one row u = c |d| s per pixel (the reference's two rows are parallel), formed
+//                 from branch-free gathers of dI, split into BF16 parts (hi/mid/lo) in LDS in
+//                 the MFMA operand layout, [u | r]^T [u | r] on v_mfma_f32_16x16x16bf16_1k with
+//                 FP32 accumulation (six products, or three for the two-term split).
+//                 The residual rides along as column K, so J^T r comes out of the same MFMA
+//                 tiles; J^T r is also accumulated in FP32, and the caller picks which one feeds
+//                 the gradient (the GN fixed point depends on g alone, so FP32 J^T r keeps
+//                 batched trajectories on the FP64 ones).
 // k_jtj_reduce  : sums the per-split partial tiles, scales by w, writes a dense symmetric H.
 #include "common.h"
 
@@ -339,36 +340,12 @@ __global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ act,
 // ------------------------------------------------------------------ BF16 split MFMA J^T J
 // J is split into BF16 parts: x = hi + mid + lo (three terms, each rounded to nearest even),
 // and J^T J is accumulated in FP32 from the six products that matter down to 2^-24:
-// hh + hm + mh + hl + lh + mm.  With nsplit == 2 only hi + lo and hh + hl + lh are used.
-constexpr int JTJ_WAVES_PER_SPLIT = 4;   // one-wave workgroups per J^T J split (register kernel)
+// hh + hm + mh + hl + lh + mm.  With nsplit == 2 only hi + mid and hh + hm + mh are used.
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ unsigned short bf16_rne(float x)
-{
-    unsigned u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-
-__device__ __forceinline__ float bf16_f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-
-// ------------------------------------------------------------------ register-direct MFMA J^T J
-// The two rows the reference forms per pixel are parallel: Jx = c dy s, Jy = -c dx s with
-// s = dy (dI[r] - dI[l]) - dx (dI[b] - dI[a]) (newfuncs.cu:417-463), so
-//   Jx Jx^T + Jy Jy^T = c^2 (dx^2 + dy^2) s s^T   and   Jx g^s_x + Jy g^s_y = c (g^s_x dy - g^s_y dx) s.
-// One row per pixel, u = c |d| s (|d| = sqrt(dx^2 + dy^2)), gives the same normal equations
-// with half the rows.  J^T r is accumulated in FP32 from q s, q = c (g^s_x dy - g^s_y dx); the
-// augmented column K of [u | r] carries r = q / (c |d|) for the MFMA form of J^T r.
-//
-// One wavefront per workgroup walks its pixel range sixteen pixels at a time (16 rows of
-// [u | r]).  For v_mfma_f32_16x16x16bf16_1k lane l supplies rows 4*(l/16)..+3 of column l%16 of
-// a 16-column block, for the A operand (u^T) and the B operand (u) alike, so each lane computes
-// exactly the entries it feeds to the matrix cores: four pixels times one column of every
-// block.  The entries are split into BF16 parts (hi/mid/lo) with the packed round-to-nearest
-// conversion, two rows per instruction, which is the operand layout; every upper-triangle tile
-// is accumulated from registers.  Accumulator layout: lane l, element r -> C[4*(l/16) + r][l%16].
+// BF16 parts with the packed round-to-nearest-even conversion, two values per instruction.
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ unsigned pack_bf16(float a, float b)
@@ -379,121 +356,276 @@ __device__ __forceinline__ unsigned pack_bf16(float a, float b)
 __device__ __forceinline__ float lo_f(unsigned p) { return __uint_as_float(p << 16); }
 __device__ __forceinline__ float hi_f(unsigned p) { return __uint_as_float(p & 0xffff0000u); }
 
-typedef unsigned v2u __attribute__((ext_vector_type(2)));
+// ------------------------------------------------------------------ LDS-staged MFMA J^T J
+// The two rows the reference forms per pixel are parallel: Jx = c dy s, Jy = -c dx s with
+// s = dy (dI[r] - dI[l]) - dx (dI[b] - dI[a]) (newfuncs.cu:417-463), so
+//   Jx Jx^T + Jy Jy^T = c^2 (dx^2 + dy^2) s s^T   and   Jx g^s_x + Jy g^s_y = c (g^s_x dy - g^s_y dx) s.
+// One row per pixel, u = c |d| s (|d| = sqrt(dx^2 + dy^2)), gives the same normal equations
+// with half the rows.  Column K of [u | r] carries r = (g^s_x dy - g^s_y dx) / |d|, so J^T r also
+// comes out of the MFMA tiles; it is accumulated in FP32 beside them as well, and the caller
+// picks which one feeds the gradient.
+//
+// The gathers of dI are organised so that they never wait on one another.  A 256-thread
+// workgroup takes 64 pixels per step:
+//   1. 64 threads load each pixel's neighbour positions and row weights into LDS;
+//   2. all threads form u for (4-pixel quad, column) items: 16 gathered dI values per item, no
+//      branches (columns j >= K read column K-1 and are replaced by selects, a pixel past the
+//      range has zero weights), then write the hi/mid/lo parts to LDS in the MFMA operand
+//      layout (lane l of a 16-column block holds pixels 4*(l/16)..+3 of column l%16);
+//   3. each wavefront runs v_mfma_f32_16x16x16bf16_1k over its rows of upper-triangle tiles.
+// Rows of tiles are dealt to the four wavefronts at compile time, largest first, each to the
+// least-loaded wavefront (T = 6: 6/5/5/5 tiles).  J^T r is accumulated in FP32 per item.
+constexpr int JL_PX = 64;      // pixels per step
+constexpr int JL_WAVES = 4;
+
+constexpr int jl_row_owner(int T, int I)
+{
+    int load[JL_WAVES] = {0, 0, 0, 0};
+    for (int r = 0; r < T; ++r) {
+        int best = 0;
+        for (int w = 1; w < JL_WAVES; ++w)
+            if (load[w] < load[best]) best = w;
+        if (r == I) return best;
+        load[best] += T - r;
+    }
+    return 0;
+}
+constexpr int jl_wave_tiles(int T, int W)
+{
+    int c = 0;
+    for (int I = 0; I < T; ++I)
+        if (jl_row_owner(T, I) == W) c += T - I;
+    return c;
+}
+constexpr int jl_max_tiles(int T)
+{
+    int m = 0;
+    for (int w = 0; w < JL_WAVES; ++w) m = jl_wave_tiles(T, w) > m ? jl_wave_tiles(T, w) : m;
+    return m;
+}
+constexpr int jl_first_row(int T, int W)
+{
+    for (int I = 0; I < T; ++I)
+        if (jl_row_owner(T, I) == W) return I;
+    return T;
+}
+// slot of tile (I, J) among the tiles of its wavefront
+constexpr int jl_slot(int T, int I, int J)
+{
+    int c = 0;
+    const int w = jl_row_owner(T, I);
+    for (int r = 0; r < I; ++r)
+        if (jl_row_owner(T, r) == w) c += T - r;
+    return c + (J - I);
+}
+
+// Tiles (I, J >= I) of row I on the wavefront that owns the row; I is a template parameter so
+// the ownership test and the accumulator slots are compile-time constants.
+template <int T, int W, int I>
+__device__ __forceinline__ void jl_rows(const v4s* fh, const v4s* fm, const v4s* fl, v4f* acc, int nsplit)
+{
+    if constexpr (I < T) {
+        if constexpr (jl_row_owner(T, I) == W) {
+            constexpr int base = jl_slot(T, I, I);
+#pragma unroll
+            for (int J = I; J < T; ++J) {
+                v4f& a = acc[base + J - I];
+                a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fh[I], fh[J], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fh[I], fm[J], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fm[I], fh[J], a, 0, 0, 0);
+                if (nsplit == 3) {
+                    a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fh[I], fl[J], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fl[I], fh[J], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fm[I], fm[J], a, 0, 0, 0);
+                }
+            }
+        }
+        jl_rows<T, W, I + 1>(fh, fm, fl, acc, nsplit);
+    }
+}
+
+template <int T, int W>
+__device__ __forceinline__ void jl_mfma(const uint2* __restrict__ sfrag, v4f* acc, int lane, int nsplit)
+{
+    constexpr int B0 = jl_first_row(T, W);
+    if constexpr (B0 < T) {
+#pragma unroll
+        for (int ks = 0; ks < JL_PX / 16; ++ks) {
+            v4s fh[T], fm[T], fl[T];
+#pragma unroll
+            for (int b = B0; b < T; ++b) {
+                const uint2* q = sfrag + ((ks * T + b) * 3) * 64 + lane;
+                fh[b] = __builtin_bit_cast(v4s, q[0]);
+                fm[b] = __builtin_bit_cast(v4s, q[64]);
+                fl[b] = __builtin_bit_cast(v4s, q[128]);
+            }
+            jl_rows<T, W, B0>(fh, fm, fl, acc, nsplit);
+        }
+    }
+}
+
+template <int T, int W, int I>
+__device__ __forceinline__ void jl_store_rows(const v4f* acc, float* __restrict__ Hout, int grp, int col)
+{
+    if constexpr (I < T) {
+        if constexpr (jl_row_owner(T, I) == W) {
+            constexpr int base = jl_slot(T, I, I);
+#pragma unroll
+            for (int J = I; J < T; ++J) {
+                const v4f a = acc[base + J - I];
+                float* o = Hout + (size_t)tri_index(I, J, T) * 256;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) o[(4 * grp + r) * 16 + col] = a[r];
+            }
+        }
+        jl_store_rows<T, W, I + 1>(acc, Hout, grp, col);
+    }
+}
+
+template <int T, int W>
+__device__ __forceinline__ void jl_store(const v4f* acc, float* __restrict__ Hout, int lane)
+{
+    jl_store_rows<T, W, 0>(acc, Hout, lane >> 4, lane & 15);
+}
 
 template <int T>
-__global__ void __launch_bounds__(64) k_jtj_reg(Layout L, const FrameGPU* __restrict__ frames,
-                                                 const int* __restrict__ act, int n, const int* __restrict__ npix,
-                                                 const int4* __restrict__ pnb, const float* __restrict__ pcoef,
-                                                 const float4* __restrict__ pixg, const float* __restrict__ dI, int S,
-                                                 int nsplit, float* __restrict__ Hp, float* __restrict__ gp)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_jtj_lds(Layout L, const FrameGPU* __restrict__ frames,
+                                                  const int* __restrict__ act, int n, const int* __restrict__ npix,
+                                                  const int4* __restrict__ pnb, const float* __restrict__ pcoef,
+                                                  const float4* __restrict__ pixg, const float* __restrict__ dI, int S,
+                                                  int nsplit, float* __restrict__ Hp, float* __restrict__ gp)
 {
+    constexpr int KP = 16 * T;
     constexpr int NT = T * (T + 1) / 2;
+    constexpr int NITEM = (JL_PX / 4) * KP;
+    constexpr int R = (NITEM + 255) / 256;
+    constexpr int MT = jl_max_tiles(T);
+    __shared__ uint2 sfrag[(JL_PX / 16) * T * 3 * 64];     // [kstep][block][part][lane]
+    __shared__ float4 scf[JL_PX];                          // (c|d| dy, -c|d| dx, q dy, -q dx)
+    __shared__ int4 snb[JL_PX];
+    __shared__ float srk[JL_PX];
+    __shared__ float sg[KP];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int s = blockIdx.x;
     const int nact = fl_count(act, n);
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
-    const int f = fl_frame(act, yi);
-    const FrameGPU fr = frames[f];
-    const int N = npix[f];
-    const int K = L.K;
-    const int p0 = (int)((long long)N * s / S), p1 = (int)((long long)N * (s + 1) / S);
-    const int lane = threadIdx.x & 63, grp = lane >> 4, col = lane & 15;
-    const float* dIf = dI + (size_t)fr.pix_off * K;
-    v4f acc[NT];
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        const int N = npix[f];
+        const int K = L.K;
+        const int p0 = (int)((long long)N * s / S), p1 = (int)((long long)N * (s + 1) / S);
+        const float* dIf = dI + (size_t)fr.pix_off * K;
+        v4f acc[MT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = (v4f){0.f, 0.f, 0.f, 0.f};
-    float gacc[T];
+        for (int t = 0; t < MT; ++t) acc[t] = (v4f){0.f, 0.f, 0.f, 0.f};
+        float gacc[R];
 #pragma unroll
-    for (int c = 0; c < T; ++c) gacc[c] = 0.f;
-
-    for (int kb = p0; kb < p1; kb += 16) {
-        float uv[T][4];                 // rows: this lane's four pixels
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int pos = kb + 4 * grp + q;
-            if (pos < p1) {
-                const long long o = fr.pix_off + pos;
-                const int4 nb = pnb[o];
-                const float c = pcoef[o];
-                const float4 gg = pixg[o];
-                const float dx = 2.f * gg.x, dy = 2.f * gg.y;
-                const float dn = sqrtf(dx * dx + dy * dy);
-                const float wgt = c * dn;                          // row scale c |d|
-                const float qg = c * (gg.z * dy - gg.w * dx);      // J^T r weight
-                const float rK = dn > 0.f ? (gg.z * dy - gg.w * dx) / dn : 0.f;
-                const float* rl = dIf + (size_t)nb.x * K;
-                const float* rr = dIf + (size_t)nb.y * K;
-                const float* ra = dIf + (size_t)nb.z * K;
-                const float* rb = dIf + (size_t)nb.w * K;
-#pragma unroll
-                for (int cb = 0; cb < T; ++cb) {
-                    const int j = 16 * cb + col;
-                    float v = 0.f;
-                    if (j < K) {
-                        const float sv = dy * (rr[j] - rl[j]) - dx * (rb[j] - ra[j]);
-                        v = wgt * sv;
-                        gacc[cb] = fmaf(qg, sv, gacc[cb]);          // J^T r in FP32
-                    } else if (j == K) {
-                        v = rK;
-                    }
-                    uv[cb][q] = v;
+        for (int r = 0; r < R; ++r) gacc[r] = 0.f;
+        for (int kb = p0; kb < p1; kb += JL_PX) {
+            if (tid < JL_PX) {
+                const int pos = kb + tid;
+                int4 nb = make_int4(0, 0, 0, 0);
+                float4 cf = make_float4(0.f, 0.f, 0.f, 0.f);
+                float rk = 0.f;
+                if (pos < p1) {
+                    const long long o = fr.pix_off + pos;
+                    nb = pnb[o];
+                    const float c = pcoef[o];
+                    const float4 gg = pixg[o];
+                    const float dx = 2.f * gg.x, dy = 2.f * gg.y;
+                    const float dn = sqrtf(dx * dx + dy * dy);
+                    const float wgt = c * dn;                       // row scale c |d|
+                    const float e = gg.z * dy - gg.w * dx;
+                    const float qg = c * e;                         // J^T r weight
+                    cf = make_float4(wgt * dy, -wgt * dx, qg * dy, -qg * dx);
+                    rk = dn > 0.f ? e / dn : 0.f;
                 }
-            } else {
-#pragma unroll
-                for (int cb = 0; cb < T; ++cb) uv[cb][q] = 0.f;
+                snb[tid] = nb;
+                scf[tid] = cf;
+                srk[tid] = rk;
             }
-        }
-        // BF16 parts, two rows per packed conversion: x = hi + mid + lo
-        v4s fh[T], fm[T], fl[T];
+            __syncthreads();
+            // every gather of the step first, then the arithmetic
+            float ld[R][16];
 #pragma unroll
-        for (int cb = 0; cb < T; ++cb) {
-            v2u h, m, l;
+            for (int r = 0; r < R; ++r) {
+                const int it = min(tid + 256 * r, NITEM - 1);
+                const int pq = it / KP, j = it - (it / KP) * KP;
+                const int jj = min(j, K - 1);
 #pragma unroll
-            for (int pr = 0; pr < 2; ++pr) {
-                const float a = uv[cb][2 * pr], b = uv[cb][2 * pr + 1];
-                const unsigned ph = pack_bf16(a, b);
-                const float ra = a - lo_f(ph), rb = b - hi_f(ph);
-                const unsigned pm = pack_bf16(ra, rb);
-                const unsigned pl = pack_bf16(ra - lo_f(pm), rb - hi_f(pm));
-                h[pr] = ph;
-                m[pr] = pm;
-                l[pr] = pl;
-            }
-            fh[cb] = __builtin_bit_cast(v4s, h);
-            fm[cb] = __builtin_bit_cast(v4s, m);
-            fl[cb] = __builtin_bit_cast(v4s, l);
-        }
-        int t = 0;
-#pragma unroll
-        for (int I = 0; I < T; ++I) {
-#pragma unroll
-            for (int J = I; J < T; ++J, ++t) {
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fh[I], fh[J], acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fh[I], fm[J], acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fm[I], fh[J], acc[t], 0, 0, 0);
-                if (nsplit == 3) {
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fh[I], fl[J], acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fl[I], fh[J], acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fm[I], fm[J], acc[t], 0, 0, 0);
+                for (int q = 0; q < 4; ++q) {
+                    const int4 nb = snb[4 * pq + q];
+                    ld[r][4 * q + 0] = dIf[(size_t)nb.x * K + jj];
+                    ld[r][4 * q + 1] = dIf[(size_t)nb.y * K + jj];
+                    ld[r][4 * q + 2] = dIf[(size_t)nb.z * K + jj];
+                    ld[r][4 * q + 3] = dIf[(size_t)nb.w * K + jj];
                 }
             }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int it = tid + 256 * r;
+                const bool item = NITEM % 256 == 0 || it < NITEM;
+                const int itc = min(it, NITEM - 1);
+                const int pq = itc / KP, j = itc - (itc / KP) * KP;
+                // 0/1 multipliers instead of selects keep this straight-line code
+                const float ma = j < K ? 1.f : 0.f, mk = j == K ? 1.f : 0.f;
+                float u[4];
+                float gs = 0.f;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 cf = scf[4 * pq + q];
+                    const float rk = srk[4 * pq + q];
+                    const float dh = ld[r][4 * q + 1] - ld[r][4 * q + 0];
+                    const float dv = ld[r][4 * q + 3] - ld[r][4 * q + 2];
+                    u[q] = ma * (cf.x * dh + cf.y * dv) + mk * rk;
+                    gs += cf.z * dh + cf.w * dv;
+                }
+                gacc[r] = fmaf(item ? ma : 0.f, gs, gacc[r]);
+                // hi/mid/lo parts, two rows per packed conversion
+                const unsigned ph0 = pack_bf16(u[0], u[1]), ph1 = pack_bf16(u[2], u[3]);
+                const float r0 = u[0] - lo_f(ph0), r1 = u[1] - hi_f(ph0);
+                const float r2 = u[2] - lo_f(ph1), r3 = u[3] - hi_f(ph1);
+                const unsigned pm0 = pack_bf16(r0, r1), pm1 = pack_bf16(r2, r3);
+                const uint2 h = make_uint2(ph0, ph1);
+                const uint2 m = make_uint2(pm0, pm1);
+                const uint2 l = make_uint2(pack_bf16(r0 - lo_f(pm0), r1 - hi_f(pm0)),
+                                           pack_bf16(r2 - lo_f(pm1), r3 - hi_f(pm1)));
+                const int ks = pq >> 2, lp = ((pq & 3) << 4) | (j & 15), cb = j >> 4;
+                uint2* dst = sfrag + ((ks * T + cb) * 3) * 64 + lp;
+                if (item) {
+                    dst[0] = h;
+                    dst[64] = m;
+                    dst[128] = l;
+                }
+            }
+            __syncthreads();
+            switch (wave) {
+            case 0: jl_mfma<T, 0>(sfrag, acc, lane, nsplit); break;
+            case 1: jl_mfma<T, 1>(sfrag, acc, lane, nsplit); break;
+            case 2: jl_mfma<T, 2>(sfrag, acc, lane, nsplit); break;
+            default: jl_mfma<T, 3>(sfrag, acc, lane, nsplit); break;
+            }
+            __syncthreads();   // the next step rewrites the LDS staging
         }
-    }
-    // J^T r: sum the four row groups of each column, lanes 0..15 write
-    const int KP = 16 * T;
+        float* Hout = Hp + ((size_t)f * S + s) * NT * 256;
+        switch (wave) {
+        case 0: jl_store<T, 0>(acc, Hout, lane); break;
+        case 1: jl_store<T, 1>(acc, Hout, lane); break;
+        case 2: jl_store<T, 2>(acc, Hout, lane); break;
+        default: jl_store<T, 3>(acc, Hout, lane); break;
+        }
+        // J^T r: the items of one column meet in LDS
+        for (int j = tid; j < KP; j += 256) sg[j] = 0.f;
+        __syncthreads();
 #pragma unroll
-    for (int cb = 0; cb < T; ++cb) {
-        float v = gacc[cb];
-        v += __shfl_xor(v, 16, 64);
-        v += __shfl_xor(v, 32, 64);
-        if (grp == 0) gp[((size_t)f * S + s) * KP + 16 * cb + col] = v;
-    }
-    float* Hout = Hp + ((size_t)f * S + s) * NT * 256;
-#pragma unroll
-    for (int tt = 0; tt < NT; ++tt) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Hout[(size_t)tt * 256 + (4 * grp + r) * 16 + col] = acc[tt][r];
-    }
+        for (int r = 0; r < R; ++r) {
+            const int it = tid + 256 * r;
+            if (NITEM % 256 == 0 || it < NITEM) atomicAdd(&sg[it % KP], gacc[r]);
+        }
+        __syncthreads();
+        for (int j = tid; j < KP; j += 256) gp[((size_t)f * S + s) * KP + j] = sg[j];
+        __syncthreads();   // sg is cleared again for the next frame
     }
 }
 
@@ -530,11 +662,9 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameL
 size_t jtj_partials_floats(int n, int S, int K, bool mfma)
 {
     if (mfma) {
-        // the register kernel runs JTJ_WAVES_PER_SPLIT one-wave workgroups per split
         const int KP = ((K + 1 + 15) / 16) * 16;
         const int T = KP / 16;
-        const size_t SP = (size_t)S * JTJ_WAVES_PER_SPLIT;
-        return (size_t)n * SP * (T * (T + 1) / 2) * 256 + (size_t)n * SP * KP;
+        return (size_t)n * S * (T * (T + 1) / 2) * 256 + (size_t)n * S * KP;
     }
     const int KP = ((K + 3) / 4) * 4;
     const int T = KP / 4;
@@ -577,22 +707,21 @@ int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, FrameList fl, const
     const int nsplit = two_term ? 2 : 3;
     const int T = KP / 16;
     const int n = fl.n;
-    const int SP = S * JTJ_WAVES_PER_SPLIT;
-    float* gp = partials + (size_t)n * SP * (T * (T + 1) / 2) * 256;
-    const int gy = grid_frames(n, 1 << 16);     // a row of split waves per frame: J^T J needs the occupancy
-    const dim3 grid(SP, gy);
+    float* gp = partials + (size_t)n * S * (T * (T + 1) / 2) * 256;
+    const int gy = grid_frames(n, 1 << 16);
+    const dim3 grid(S, gy);
     const int* act = fl.act;
     switch (T) {
-    case 1: hipLaunchKernelGGL(k_jtj_reg<1>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
-    case 2: hipLaunchKernelGGL(k_jtj_reg<2>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
-    case 3: hipLaunchKernelGGL(k_jtj_reg<3>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
-    case 4: hipLaunchKernelGGL(k_jtj_reg<4>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
-    case 5: hipLaunchKernelGGL(k_jtj_reg<5>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
-    case 6: hipLaunchKernelGGL(k_jtj_reg<6>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
-    default: hipLaunchKernelGGL(k_jtj_reg<7>, grid, dim3(64), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, SP, nsplit, partials, gp); break;
+    case 1: hipLaunchKernelGGL(k_jtj_lds<1>, grid, dim3(256), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, S, nsplit, partials, gp); break;
+    case 2: hipLaunchKernelGGL(k_jtj_lds<2>, grid, dim3(256), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, S, nsplit, partials, gp); break;
+    case 3: hipLaunchKernelGGL(k_jtj_lds<3>, grid, dim3(256), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, S, nsplit, partials, gp); break;
+    case 4: hipLaunchKernelGGL(k_jtj_lds<4>, grid, dim3(256), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, S, nsplit, partials, gp); break;
+    case 5: hipLaunchKernelGGL(k_jtj_lds<5>, grid, dim3(256), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, S, nsplit, partials, gp); break;
+    case 6: hipLaunchKernelGGL(k_jtj_lds<6>, grid, dim3(256), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, S, nsplit, partials, gp); break;
+    default: hipLaunchKernelGGL(k_jtj_lds<7>, grid, dim3(256), 0, s, L, frames, act, n, npix, pnb, pcoef, pixg, dI, S, nsplit, partials, gp); break;
     }
     int bx = (K * K + K + 255) / 256;
-    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, gy), dim3(256), 0, s, act, n, K, KP, 16, SP, partials, gp, w, H, g,
+    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, gy), dim3(256), 0, s, act, n, K, KP, 16, S, partials, gp, w, H, g,
                        g_on_mfma);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
