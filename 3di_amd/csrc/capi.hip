@@ -97,6 +97,8 @@ struct Workspace {
     float4* pixg = nullptr;
     int4* pnb = nullptr;
     float* pcoef = nullptr;
+    int4* pvert = nullptr;       // per visible position: vertices of its triangle, crop index
+    float2* pbary = nullptr;     // per visible position: (alpha, beta)
     float* dI = nullptr;
     float *theta = nullptr, *theta_try = nullptr, *delta = nullptr, *H = nullptr, *g = nullptr;
     double *obj = nullptr, *objb = nullptr;
@@ -266,6 +268,8 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     w.pixg = A.get<float4>(cap);
     w.pnb = A.get<int4>(cap);
     w.pcoef = A.get<float>(cap);
+    w.pvert = A.get<int4>(cap);
+    w.pbary = A.get<float2>(cap);
     w.dI = A.get<float>((size_t)cap * K);
     w.theta = A.get<float>((size_t)C * K);
     w.theta_try = A.get<float>((size_t)C * K);
@@ -418,7 +422,7 @@ extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd,
     }
     const int V = c->m.V, K = c->Lx.K;
     const double per_frame = (double)V * 40 + 64.0 * K * K + 8.0 * resolve_tiles(maxA);
-    const double per_pixel = 84.0 + 4.0 * K;
+    const double per_pixel = 108.0 + 4.0 * K;
     auto chunk_cap = [&](int Cc) {
         long long best = 0;
         for (int i0 = 0; i0 < n; i0 += Cc) {
@@ -481,17 +485,45 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
     if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
     if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, w.box, s)))
+    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
         return rc;
     if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
         return rc;
-    if ((rc = launch_shade(c->m, L, w.frames, fl, theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary,
+    if ((rc = launch_shade(c->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix,
                            full_crop ? nullptr : w.box, w.img, w.max_area, s)))
         return rc;
     if ((rc = launch_blur(w.frames, fl, c->taps, full_crop ? nullptr : w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
     return launch_grads(w.frames, fl, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
                         w.pcoef, w.obj, w.max_area, s);
+}
+
+// Objective only, for the line search (the F(theta + t delta) of fit_3DMM_shape_rigid's Armijo
+// loop, model_fitter.cu:219-260): the same synthesis, raster and per-pixel terms as
+// eval_render, but shading and the objective work from the keys over the visible box, so
+// there is no visible list, no scatter and no per-pixel Jacobian records.
+static int eval_objective(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s,
+                          const Layout* Lp = nullptr)
+{
+    tdi_ctx* c = b->ctx;
+    const Layout& L = Lp ? *Lp : c->L;
+    Workspace& w = b->ws;
+    int rc;
+    if (mixed & TDI_MIXED_SYNTH_MFMA)
+        rc = launch_synthesize_gemm(c->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
+    else
+        rc = launch_synthesize(c->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
+    if (rc) return rc;
+    if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
+    if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
+    if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+    if ((rc = launch_visible_box(w.frames, fl, w.keys, w.max_area, w.tcount, w.tmax, w.box, s))) return rc;
+    if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
+                             w.objb, w.H, w.g, s)))
+        return rc;
+    if ((rc = launch_shade_keys(c->m, L, w.frames, fl, theta, w.P0, w.vproj, w.keys, w.box, w.img, s))) return rc;
+    if ((rc = launch_blur(w.frames, fl, c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
+    return launch_objective(w.frames, fl, w.keys, w.box, w.blurred, b->obs_g, c->opts.data_weight, w.obj, s);
 }
 
 static int launch_normal_eq(tdi_batch* b, FrameList fl, int mixed, hipStream_t s, const Layout* Lp = nullptr)
@@ -516,7 +548,7 @@ static int eval_full(tdi_batch* b, FrameList fl, const float* theta, int mixed, 
     Workspace& w = b->ws;
     int rc = eval_render(b, fl, theta, mixed, s, &L);
     if (rc) return rc;
-    if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
+    if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
                         w.max_area, s)))
         return rc;
     if ((rc = launch_normal_eq(b, fl, mixed, s, &L))) return rc;
@@ -578,7 +610,7 @@ int tdi_rasterize(tdi_batch* b, const float* xyz, const float* xyp, uint64_t* ke
         if ((rc = launch_clear_keys(w.keys, len, s))) return rc;
         if ((rc = launch_misc_fill_f32(w.bary, 2 * len, 0.f, s))) return rc;
         if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, nullptr, s)))
+        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, nullptr, s)))
             return rc;
         if (keys) TDI_HIP(hipMemcpyAsync(keys + off, w.keys, sizeof(uint64_t) * len, hipMemcpyDeviceToDevice, s));
         if (bary) TDI_HIP(hipMemcpyAsync(bary + 2 * off, w.bary, sizeof(float) * 2 * len, hipMemcpyDeviceToDevice, s));
@@ -665,10 +697,10 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
         TDI_HIP(hipEventRecord(ev[1], s));
         if ((rc = launch_clear_keys_frames(w.frames, all_frames(n), w.keys, w.box, w.max_area, s))) return rc;
         if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.max_area, w.tcount, w.tmax, w.box, s)))
+        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[2], s));
-        if ((rc = launch_shade(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.P, w.vis, w.npix, w.keys, w.bary, w.box, w.img,
+        if ((rc = launch_shade(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix, w.box, w.img,
                                w.max_area, s)))
             return rc;
         if ((rc = launch_blur(w.frames, all_frames(n), c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
@@ -681,7 +713,7 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
                                  w.objb, w.H, w.g, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[4], s));
-        if ((rc = launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.vis, w.npix, w.keys, w.bary, w.pixg, w.dI,
+        if ((rc = launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
                             w.max_area, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[5], s));
@@ -761,7 +793,7 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
                 for (int k = 0; k < K; ++k) tt[k] = th[k] + t * delta[k];
                 if (tt[off_tz] < 0.f) break;
                 TDI_HIP(hipMemcpyAsync(w.theta_try, tt.data(), sizeof(float) * K, hipMemcpyHostToDevice, s));
-                if ((rc = eval_render(b, all_frames(1), w.theta_try, 0, s))) return rc;
+                if ((rc = eval_objective(b, all_frames(1), w.theta_try, 0, s))) return rc;
                 double objt = 0, objb = 0;
                 int fok = 1;
                 TDI_HIP(hipMemcpyAsync(&objt, w.obj, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -781,7 +813,7 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
             }
         }
         if (!terminate && !(rep.stop & (TDI_STOP_NAN | TDI_STOP_NOT_PD))) rep.stop |= TDI_STOP_MAXITER;
-        if ((rc = eval_render(b, all_frames(1), w.theta, 0, s))) return rc;
+        if ((rc = eval_objective(b, all_frames(1), w.theta, 0, s))) return rc;
         double fin = 0;
         TDI_HIP(hipMemcpyAsync(&fin, w.obj, sizeof(double), hipMemcpyDeviceToHost, s));
         TDI_HIP(hipStreamSynchronize(s));
@@ -891,7 +923,7 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
                 xt = x;
                 for (int a = 0; a < nd; ++a) xt[j0 + a] = (float)((double)x[j0 + a] + t * d[a]);
                 TDI_HIP(hipMemcpyAsync(W.theta_try, xt.data(), sizeof(float) * Kx, hipMemcpyHostToDevice, s));
-                if ((rc = eval_render(b, all_frames(1), W.theta_try, 0, s, &Lx))) return rc;
+                if ((rc = eval_objective(b, all_frames(1), W.theta_try, 0, s, &Lx))) return rc;
                 TDI_HIP(hipMemcpyAsync(&ob[0], W.obj, sizeof(double), hipMemcpyDeviceToHost, s));
                 TDI_HIP(hipMemcpyAsync(&ob[1], W.objb, sizeof(double), hipMemcpyDeviceToHost, s));
                 TDI_HIP(hipStreamSynchronize(s));
@@ -906,7 +938,7 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
             }
         }
         TDI_HIP(hipMemcpyAsync(W.theta, x.data(), sizeof(float) * Kx, hipMemcpyHostToDevice, s));
-        if ((rc = eval_render(b, all_frames(1), W.theta, 0, s, &Lx))) return rc;
+        if ((rc = eval_objective(b, all_frames(1), W.theta, 0, s, &Lx))) return rc;
         double ob[2];
         TDI_HIP(hipMemcpyAsync(&ob[0], W.obj, sizeof(double), hipMemcpyDeviceToHost, s));
         TDI_HIP(hipMemcpyAsync(&ob[1], W.objb, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -972,7 +1004,7 @@ static int record_fit(tdi_batch* b, hipStream_t s)
                                         w.evalmask, s)))
                 return rc;
             if ((rc = launch_compact_frames(n, w.evalmask, w.act_e, s))) return rc;
-            if ((rc = eval_render(b, FrameList{w.act_e, w.C}, w.theta_try, o.mixed_flags, s))) return rc;
+            if ((rc = eval_objective(b, FrameList{w.act_e, w.C}, w.theta_try, o.mixed_flags, s))) return rc;
             if ((rc = launch_ls_decide(n, K, w.st, w.evalmask, w.searching, w.obj, w.objb, w.face_ok, w.theta,
                                        w.theta_try, o, s)))
                 return rc;

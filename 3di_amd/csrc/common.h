@@ -258,14 +258,26 @@ int launch_raster(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const
                   uint64_t* keys, hipStream_t s);
 // tcount: [slots][tmax] tile counts, tmax >= resolve_tiles(largest crop)
 int resolve_tiles(int max_area);
+// pvert/pbary: per visible position (vb, vc, va, crop index k) and (alpha, beta), so that the
+// per-pixel kernels after resolve reach the vertices in one load
 int launch_resolve(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj,
-                   const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int max_area,
-                   int* tcount, int tmax, int4* box, hipStream_t s);
+                   const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int4* pvert,
+                   float2* pbary, int max_area, int* tcount, int tmax, int4* box, hipStream_t s);
+
+// visible box (and tile counts) only, for the passes that work from the keys
+int launch_visible_box(const FrameGPU* frames, FrameList fl, const uint64_t* keys, int max_area, int* tcount,
+                       int tmax, int4* box, hipStream_t s);
 
 // shade.hip
 int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-                 const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
-                 const float* bary, const int4* box, float* img, int max_area, hipStream_t s);
+                 const float* P0, const int4* pvert, const float2* pbary, const int* npix, const int4* box,
+                 float* img, int max_area, hipStream_t s);
+// shading and objective straight from the keys over the visible box (line-search evaluations)
+int launch_shade_keys(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
+                      const float* P0, const float4* vproj, const uint64_t* keys, const int4* box, float* img,
+                      hipStream_t s);
+int launch_objective(const FrameGPU* frames, FrameList fl, const uint64_t* keys, const int4* box,
+                     const float* blurred, const float2* obs_g, float w, double* obj, hipStream_t s);
 // box == nullptr: whole crops; else only the region the normalised gradients need
 int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const int4* box, const float* in, float* tmp,
                 float* out, int max_area, hipStream_t s);
@@ -277,8 +289,8 @@ int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const f
 
 // jacobian.hip
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-              const float* P0, const int* vis_list, const int* npix, const uint64_t* keys,
-              const float* bary, const float4* pixg, float* dI, int max_area, hipStream_t s);
+              const float* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
+              float* dI, int max_area, hipStream_t s);
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);
 int launch_jtj_f32(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,

@@ -51,39 +51,48 @@ __device__ inline void load_frame_rot(const Layout& L, const FrameGPU& fr, const
     }
 }
 
+constexpr int DI_PX_PER_WG = 256;   // pixels per workgroup and frame in k_dI
+
 __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                              const int* __restrict__ act, int n, const float* __restrict__ theta,
-                                             const float* __restrict__ P0,
-                                             const int* __restrict__ vis, const int* __restrict__ npix,
-                                             const unsigned long long* __restrict__ keys,
-                                             const float2* __restrict__ bary, const float4* __restrict__ pixg,
-                                             float* __restrict__ dI)
+                                             const float* __restrict__ P0, const int4* __restrict__ pvert,
+                                             const float2* __restrict__ pbary, const int* __restrict__ npix,
+                                             const float4* __restrict__ pixg, float* __restrict__ dI)
 {
     __shared__ FrameRot sr;
     __shared__ float stex[512];
     const int nact = fl_count(act, n);
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
     const int f = fl_frame(act, yi);
+    const int N = npix[f];
+    // The visible list runs column by column, so a contiguous range of positions is a vertical
+    // strip of the face.  Strip c goes to the workgroups with blockIdx.x % 8 == c, which the
+    // dispatcher places on one XCD (workgroups are dealt round-robin to the 8 XCDs and gridDim.x
+    // is a multiple of 8): each XCD's L2 then holds the basis rows of one strip of the mesh.
+    // A strip is shared by as many workgroups as give each about DI_PX_PER_WG pixels; the
+    // others leave before the frame set-up.
+    const int strip = blockIdx.x & 7, sb = blockIdx.x >> 3;
+    const int nsb = max(1, min((int)(gridDim.x >> 3), (N + 8 * DI_PX_PER_WG - 1) / (8 * DI_PX_PER_WG)));
+    if (sb >= nsb) continue;
     const FrameGPU fr = frames[f];
     const float* tf = theta + (size_t)f * L.K;
     if (threadIdx.x == 0) load_frame_rot(L, fr, tf, sr);
     for (int q = threadIdx.x; q < L.ntex; q += blockDim.x) stex[q] = tf[L.off_tex + q];
     __syncthreads();
-    const int N = npix[f];
     const long long base = fr.pix_off;
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const int sub = lane >> 4, l16 = lane & 15;      // four pixels per wavefront, 16 lanes each
     const float* P0f = P0 + (size_t)f * m.V * 3;
     const int K = L.K;
-    for (int pos0 = (blockIdx.x * 4 + wave) * 4; pos0 < N; pos0 += gridDim.x * 16) {
+    const int q0 = (int)((long long)N * strip / 8), q1 = (int)((long long)N * (strip + 1) / 8);
+    for (int pos0 = q0 + (sb * 4 + wave) * 4; pos0 < q1; pos0 += nsb * 16) {
         const int pos = pos0 + sub;
-        if (pos >= N) continue;
-        const int k = vis[base + pos];
-        const int t = (int)(unsigned)(keys[base + k] & 0xffffffffull);
-        const float2 ab = bary[base + k];
+        if (pos >= q1) continue;
+        const int4 pv = pvert[base + pos];
+        const float2 ab = pbary[base + pos];
         const float al = ab.x, be = ab.y, ga = gamma_rn(al, be);
-        const int vb = m.tris[3 * t], vc = m.tris[3 * t + 1], va = m.tris[3 * t + 2];
+        const int vb = pv.x, vc = pv.y, va = pv.z;
         float v[3], p0[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) p0[a] = al * P0f[3 * vb + a] + be * P0f[3 * vc + a] + ga * P0f[3 * va + a];
@@ -128,17 +137,41 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
             rn[2] = sr.R[2] * nx + sr.R[5] * ny + sr.R[8] * nz;
         }
         float* out = dI + (size_t)(base + pos) * K;
-        for (int j = l16; j < K; j += 16) {
+        // expression columns, 80 per group: every basis gather of the group is issued before
+        // the arithmetic (clamped columns, masked stores), so the group costs one round trip
+        if (L.nexp) {
+            const int ke = m.k_exp;
+            const float* Eb = m.exp_basis + (size_t)(3 * vb) * ke;   // rows 3v, 3v+1, 3v+2
+            const float* Ec = m.exp_basis + (size_t)(3 * vc) * ke;
+            const float* Ea = m.exp_basis + (size_t)(3 * va) * ke;
+            for (int c0 = 0; c0 < L.nexp; c0 += 80) {
+                float eb[5][3], ec[5][3], ea[5][3];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    const int c = min(c0 + 16 * i + l16, L.nexp - 1);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        eb[i][a] = Eb[(size_t)a * ke + c];
+                        ec[i][a] = Ec[(size_t)a * ke + c];
+                        ea[i][a] = Ea[(size_t)a * ke + c];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    const int c = c0 + 16 * i + l16;
+                    const float ex = al * eb[i][0] + be * ec[i][0] + ga * ea[i][0];
+                    const float ey = al * eb[i][1] + be * ec[i][1] + ga * ea[i][1];
+                    const float ez = al * eb[i][2] + be * ec[i][2] + ga * ea[i][2];
+                    if (c < L.nexp) out[L.off_exp + c] = q0 * ex + q1 * ey + q2 * ez;
+                }
+            }
+        }
+        // the other columns: albedo and identity (before the expression block), pose and light
+        const int n_lo = L.off_exp, n_other = L.off_exp + (K - L.off_pose);
+        for (int o = l16; o < n_other; o += 16) {
+            const int j = o < n_lo ? o : L.off_pose + (o - n_lo);
             float val;
-            if (j >= L.off_exp && j < L.off_pose) {
-                const int c = j - L.off_exp;
-                const float* E = m.exp_basis;
-                const int ke = m.k_exp;
-                const float ex = al * E[(size_t)(3 * vb) * ke + c] + be * E[(size_t)(3 * vc) * ke + c] + ga * E[(size_t)(3 * va) * ke + c];
-                const float ey = al * E[(size_t)(3 * vb + 1) * ke + c] + be * E[(size_t)(3 * vc + 1) * ke + c] + ga * E[(size_t)(3 * va + 1) * ke + c];
-                const float ez = al * E[(size_t)(3 * vb + 2) * ke + c] + be * E[(size_t)(3 * vc + 2) * ke + c] + ga * E[(size_t)(3 * va + 2) * ke + c];
-                val = q0 * ex + q1 * ey + q2 * ez;
-            } else if (j >= L.off_pose && j < L.off_pose + 3) {
+            if (j >= L.off_pose && j < L.off_pose + 3) {
                 val = (j == L.off_pose) ? c0 : (j == L.off_pose + 1) ? c1 : c2;
             } else if (j >= L.off_pose + 3 && j < L.off_light) {
                 const float* d = sr.dR + 9 * (j - L.off_pose - 3);
@@ -366,8 +399,8 @@ __device__ __forceinline__ float hi_f(unsigned p) { return __uint_as_float(p & 0
 // picks which one feeds the gradient.
 //
 // The gathers of dI are organised so that they never wait on one another.  A 256-thread
-// workgroup takes 64 pixels per step:
-//   1. 64 threads load each pixel's neighbour positions and row weights into LDS;
+// workgroup takes 32 pixels per step:
+//   1. 32 threads load each pixel's neighbour positions and row weights into LDS;
 //   2. all threads form u for (4-pixel quad, column) items: 16 gathered dI values per item, no
 //      branches (columns j >= K read column K-1 and are replaced by selects, a pixel past the
 //      range has zero weights), then write the hi/mid/lo parts to LDS in the MFMA operand
@@ -375,7 +408,7 @@ __device__ __forceinline__ float hi_f(unsigned p) { return __uint_as_float(p & 0
 //   3. each wavefront runs v_mfma_f32_16x16x16bf16_1k over its rows of upper-triangle tiles.
 // Rows of tiles are dealt to the four wavefronts at compile time, largest first, each to the
 // least-loaded wavefront (T = 6: 6/5/5/5 tiles).  J^T r is accumulated in FP32 per item.
-constexpr int JL_PX = 64;      // pixels per step
+constexpr int JL_PX = 32;      // pixels per step
 constexpr int JL_WAVES = 4;
 
 constexpr int jl_row_owner(int T, int I)
@@ -489,7 +522,7 @@ __device__ __forceinline__ void jl_store(const v4f* acc, float* __restrict__ Hou
 }
 
 template <int T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_jtj_lds(Layout L, const FrameGPU* __restrict__ frames,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T >= 7 ? 2 : 3))) k_jtj_lds(Layout L, const FrameGPU* __restrict__ frames,
                                                   const int* __restrict__ act, int n, const int* __restrict__ npix,
                                                   const int4* __restrict__ pnb, const float* __restrict__ pcoef,
                                                   const float4* __restrict__ pixg, const float* __restrict__ dI, int S,
@@ -643,18 +676,20 @@ int jtj_splits(int n, int per_frame_pixels_hint)
 }
 
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-              const float* P0, const int* vis_list, const int* npix, const uint64_t* keys,
-              const float* bary, const float4* pixg, float* dI, int max_area, hipStream_t s)
+              const float* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
+              float* dI, int max_area, hipStream_t s)
 {
     if (L.ntex > 512) {
         set_error("dI: k_tex %d > 512", L.ntex);
         return TDI_ERR_ARG;
     }
-    int bx = (max_area / 2 + 15) / 16;
-    if (bx > 512) bx = 512;
-    if (bx < 1) bx = 1;
-    hipLaunchKernelGGL(k_dI, dim3(bx, grid_frames(fl.n, 64)), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0,
-                       vis_list, npix, (const unsigned long long*)keys, (const float2*)bary, pixg, dI);
+    // 8 strips per frame, each shared by up to nsb workgroups (sized for a half-visible crop);
+    // one frame per grid row
+    int nsb = (max_area / 2 + 8 * DI_PX_PER_WG - 1) / (8 * DI_PX_PER_WG);
+    if (nsb > 64) nsb = 64;
+    if (nsb < 1) nsb = 1;
+    hipLaunchKernelGGL(k_dI, dim3(8 * nsb, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0,
+                       pvert, pbary, npix, pixg, dI);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

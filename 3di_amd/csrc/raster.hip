@@ -153,7 +153,8 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_write(const int* __rest
                                                                 const int* __restrict__ tcount, int tmax,
                                                                 const int4* __restrict__ box,
                                                                 int* __restrict__ cum, int* __restrict__ vis,
-                                                                int* __restrict__ npix, float2* __restrict__ bary)
+                                                                int* __restrict__ npix, float2* __restrict__ bary,
+                                                                int4* __restrict__ pvert, float2* __restrict__ pbary)
 {
     __shared__ int swave[RES_NW];
     __shared__ int soff[RES_NW];
@@ -212,10 +213,13 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_write(const int* __rest
                 if (occ) {
                     vis[base + pos] = k;
                     const int t = (int)(unsigned)(key[q] & 0xffffffffull);
-                    const float4 B = vp[tris[3 * t]], Cc = vp[tris[3 * t + 1]], Av = vp[tris[3 * t + 2]];
+                    const int vb = tris[3 * t], vc = tris[3 * t + 1], va = tris[3 * t + 2];
+                    const float4 B = vp[vb], Cc = vp[vc], Av = vp[va];
                     float al, be;
                     cover_rn(B.x, B.y, Cc.x, Cc.y, Av.x, Av.y, fr.x0 + k / fr.H, fr.y0 + k % fr.H, &al, &be);
                     bary[base + k] = make_float2(al, be);
+                    pvert[base + pos] = make_int4(vb, vc, va, k);
+                    pbary[base + pos] = make_float2(al, be);
                     ++pos;
                 }
             }
@@ -277,9 +281,24 @@ int launch_raster(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const
 
 int resolve_tiles(int max_area) { return (max_area + RES_TILE - 1) / RES_TILE; }
 
+int launch_visible_box(const FrameGPU* frames, FrameList fl, const uint64_t* keys, int max_area, int* tcount,
+                       int tmax, int4* box, hipStream_t s)
+{
+    int bx = resolve_tiles(max_area);
+    if (bx > tmax) {
+        set_error("resolve: crop of %d pixels needs %d tiles > %d", max_area, bx, tmax);
+        return TDI_ERR_ARG;
+    }
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(k_resolve_count, dim3(bx, grid_frames(fl.n, 64)), dim3(RES_THREADS), 0, s, frames, fl.act, fl.n,
+                       (const unsigned long long*)keys, tcount, tmax, box);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
 int launch_resolve(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj,
-                   const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int max_area,
-                   int* tcount, int tmax, int4* box, hipStream_t s)
+                   const uint64_t* keys, int* cum, int* vis_list, int* npix, float* bary, int4* pvert,
+                   float2* pbary, int max_area, int* tcount, int tmax, int4* box, hipStream_t s)
 {
     int bx = resolve_tiles(max_area);
     if (bx > tmax) {
@@ -291,7 +310,8 @@ int launch_resolve(const ModelGPU& m, const FrameGPU* frames, FrameList fl, cons
     hipLaunchKernelGGL(k_resolve_count, grid, dim3(RES_THREADS), 0, s, frames, fl.act, fl.n,
                        (const unsigned long long*)keys, tcount, tmax, box);
     hipLaunchKernelGGL(k_resolve_write, grid, dim3(RES_THREADS), 0, s, m.tris, m.V, frames, fl.act, fl.n, vproj,
-                       (const unsigned long long*)keys, tcount, tmax, box, cum, vis_list, npix, (float2*)bary);
+                       (const unsigned long long*)keys, tcount, tmax, box, cum, vis_list, npix, (float2*)bary,
+                       pvert, pbary);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

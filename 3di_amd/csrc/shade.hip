@@ -60,67 +60,129 @@ __global__ void __launch_bounds__(256) k_clear_crops(const FrameGPU* __restrict_
     }
 }
 
+// Per-frame shading constants: R (column-major), lambda - tau, L, the albedo coefficients.
+struct ShadeFrame {
+    float R[9], dl[3], L;
+};
+
+__device__ __forceinline__ void shade_frame_setup(const Layout& L, const FrameGPU& fr, const float* tf, ShadeFrame& sf)
+{
+    rodrigues_f32(tf + L.off_pose + 3, sf.R, nullptr);
+    float lam[3] = {fr.lam[0], fr.lam[1], fr.lam[2]};
+    float Li = fr.Lint;
+    if (L.nlight) {
+        lam[0] = tf[L.off_light]; lam[1] = tf[L.off_light + 1]; lam[2] = tf[L.off_light + 2];
+        Li = tf[L.off_light + 3];
+    }
+    for (int a = 0; a < 3; ++a) sf.dl[a] = lam[a] - tf[L.off_pose + a];
+    sf.L = Li;
+}
+
+// I = rho (1 + L Id) at barycentric (al, be) of the triangle (vb, vc, va).
+__device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L, const float* __restrict__ P0f,
+                                             const ShadeFrame& sf, const float* stex, int vb, int vc, int va,
+                                             float al, float be)
+{
+    const float ga = gamma_rn(al, be);
+    const float* B0 = P0f + 3 * vb;
+    const float* C0 = P0f + 3 * vc;
+    const float* A0 = P0f + 3 * va;
+    const float e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
+    const float e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
+    float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const float inv = 1.f / sqrtf(nx * nx + ny * ny + nz * nz);
+    nx *= inv; ny *= inv; nz *= inv;
+    const float px = al * B0[0] + be * C0[0] + ga * A0[0];
+    const float py = al * B0[1] + be * C0[1] + ga * A0[1];
+    const float pz = al * B0[2] + be * C0[2] + ga * A0[2];
+    // R^T (lambda - tau)
+    const float lx = sf.R[0] * sf.dl[0] + sf.R[1] * sf.dl[1] + sf.R[2] * sf.dl[2];
+    const float ly = sf.R[3] * sf.dl[0] + sf.R[4] * sf.dl[1] + sf.R[5] * sf.dl[2];
+    const float lz = sf.R[6] * sf.dl[0] + sf.R[7] * sf.dl[1] + sf.R[8] * sf.dl[2];
+    const float Id = nx * lx + ny * ly + nz * lz - (nx * px + ny * py + nz * pz);
+    float rb = m.tex_mean[vb], rc = m.tex_mean[vc], ra = m.tex_mean[va];
+    for (int q = 0; q < L.ntex; ++q) {
+        rb = fmaf(m.tex_basis[(size_t)vb * m.k_tex + q], stex[q], rb);
+        rc = fmaf(m.tex_basis[(size_t)vc * m.k_tex + q], stex[q], rc);
+        ra = fmaf(m.tex_basis[(size_t)va * m.k_tex + q], stex[q], ra);
+    }
+    const float rho = al * rb + be * rc + ga * ra;
+    return rho + sf.L * rho * Id;
+}
+
+// Shading of the visible list (pvert/pbary from the resolve pass), scattered into the crop.
 __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                                 const int* __restrict__ act, int n, const float* __restrict__ theta,
-                                                const float* __restrict__ P0, const float* __restrict__ P,
-                                                const int* __restrict__ vis, const int* __restrict__ npix,
-                                                const unsigned long long* __restrict__ keys,
-                                                const float2* __restrict__ bary, float* __restrict__ img)
+                                                const float* __restrict__ P0, const int4* __restrict__ pvert,
+                                                const float2* __restrict__ pbary, const int* __restrict__ npix,
+                                                float* __restrict__ img)
 {
-    __shared__ float sR[9], sdl[3], sL, stex[512];
+    __shared__ ShadeFrame sf;
+    __shared__ float stex[512];
     const int nact = fl_count(act, n);
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
-    const int f = fl_frame(act, yi);
-    const float* tf = theta + (size_t)f * L.K;
-    const FrameGPU fr = frames[f];
-    if (threadIdx.x == 0) {
-        rodrigues_f32(tf + L.off_pose + 3, sR, nullptr);
-        float lam[3] = {fr.lam[0], fr.lam[1], fr.lam[2]};
-        float Li = fr.Lint;
-        if (L.nlight) {
-            lam[0] = tf[L.off_light]; lam[1] = tf[L.off_light + 1]; lam[2] = tf[L.off_light + 2];
-            Li = tf[L.off_light + 3];
+        const int f = fl_frame(act, yi);
+        const int N = npix[f];
+        if (blockIdx.x * blockDim.x >= (unsigned)N) continue;
+        const float* tf = theta + (size_t)f * L.K;
+        const FrameGPU fr = frames[f];
+        if (threadIdx.x == 0) shade_frame_setup(L, fr, tf, sf);
+        for (int k = threadIdx.x; k < L.ntex; k += blockDim.x) stex[k] = tf[L.off_tex + k];
+        __syncthreads();
+        const long long base = fr.pix_off;
+        const float* P0f = P0 + (size_t)f * m.V * 3;
+        for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += gridDim.x * blockDim.x) {
+            const int4 pv = pvert[base + pos];
+            const float2 ab = pbary[base + pos];
+            img[base + pv.w] = shade_pixel(m, L, P0f, sf, stex, pv.x, pv.y, pv.z, ab.x, ab.y);
         }
-        for (int a = 0; a < 3; ++a) sdl[a] = lam[a] - tf[L.off_pose + a];
-        sL = Li;
+        __syncthreads();   // the frame's shared constants are rewritten by the next iteration
     }
-    for (int k = threadIdx.x; k < L.ntex; k += blockDim.x) stex[k] = tf[L.off_tex + k];
-    __syncthreads();
-    const int N = npix[f];
-    const long long base = fr.pix_off;
-    const float* P0f = P0 + (size_t)f * m.V * 3;
-    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += gridDim.x * blockDim.x) {
-        const int k = vis[base + pos];
-        const int t = (int)(unsigned)(keys[base + k] & 0xffffffffull);
-        const float2 ab = bary[base + k];
-        const float al = ab.x, be = ab.y, ga = gamma_rn(al, be);
-        const int vb = m.tris[3 * t], vc = m.tris[3 * t + 1], va = m.tris[3 * t + 2];
-        const float* B0 = P0f + 3 * vb;
-        const float* C0 = P0f + 3 * vc;
-        const float* A0 = P0f + 3 * va;
-        const float e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
-        const float e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
-        float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-        const float inv = 1.f / sqrtf(nx * nx + ny * ny + nz * nz);
-        nx *= inv; ny *= inv; nz *= inv;
-        const float px = al * B0[0] + be * C0[0] + ga * A0[0];
-        const float py = al * B0[1] + be * C0[1] + ga * A0[1];
-        const float pz = al * B0[2] + be * C0[2] + ga * A0[2];
-        // R^T (lambda - tau)
-        const float lx = sR[0] * sdl[0] + sR[1] * sdl[1] + sR[2] * sdl[2];
-        const float ly = sR[3] * sdl[0] + sR[4] * sdl[1] + sR[5] * sdl[2];
-        const float lz = sR[6] * sdl[0] + sR[7] * sdl[1] + sR[8] * sdl[2];
-        const float Id = nx * lx + ny * ly + nz * lz - (nx * px + ny * py + nz * pz);
-        float rb = m.tex_mean[vb], rc = m.tex_mean[vc], ra = m.tex_mean[va];
-        for (int q = 0; q < L.ntex; ++q) {
-            rb = fmaf(m.tex_basis[(size_t)vb * m.k_tex + q], stex[q], rb);
-            rc = fmaf(m.tex_basis[(size_t)vc * m.k_tex + q], stex[q], rc);
-            ra = fmaf(m.tex_basis[(size_t)va * m.k_tex + q], stex[q], ra);
-        }
-        const float rho = al * rb + be * rc + ga * ra;
-        img[base + k] = rho + sL * rho * Id;
-    }
-    __syncthreads();   // the frame's shared constants are rewritten by the next iteration
+}
+
+// Shading straight from the depth|triangle keys over the frame's region (visible box + 8):
+// the barycentrics are recomputed with the coverage test the rasteriser used, so the image is
+// the one the visible-list path renders; pixels without a triangle are cleared.  Used by the
+// line search, which needs the objective only.
+__global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+                                                     const int* __restrict__ act, int n, const float* __restrict__ theta,
+                                                     const float* __restrict__ P0, const float4* __restrict__ vproj,
+                                                     const unsigned long long* __restrict__ keys,
+                                                     const int4* __restrict__ box, float* __restrict__ img)
+{
+    __shared__ ShadeFrame sf;
+    __shared__ float stex[512];
+    const int nact = fl_count(act, n);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        Region r;
+        if (!crop_region(fr, box, f, 8, 8, r)) continue;
+        if (r.X0 + (int)blockIdx.x * nw >= r.X1) continue;
+        const float* tf = theta + (size_t)f * L.K;
+        if (threadIdx.x == 0) shade_frame_setup(L, fr, tf, sf);
+        for (int k = threadIdx.x; k < L.ntex; k += blockDim.x) stex[k] = tf[L.off_tex + k];
+        __syncthreads();
+        const long long base = fr.pix_off;
+        const float* P0f = P0 + (size_t)f * m.V * 3;
+        const float4* vp = vproj + (size_t)f * m.V;
+        for (int x = r.X0 + blockIdx.x * nw + wave; x < r.X1; x += gridDim.x * nw)
+            for (int y = r.Y0 + lane; y < r.Y1; y += 64) {
+                const int k = x * fr.H + y;
+                const unsigned long long key = keys[base + k];
+                float v = 0.f;
+                if (key != KEY_EMPTY) {
+                    const int t = (int)(unsigned)(key & 0xffffffffull);
+                    const int vb = m.tris[3 * t], vc = m.tris[3 * t + 1], va = m.tris[3 * t + 2];
+                    const float4 B = vp[vb], Cc = vp[vc], Av = vp[va];
+                    float al, be;
+                    cover_rn(B.x, B.y, Cc.x, Cc.y, Av.x, Av.y, fr.x0 + x, fr.y0 + y, &al, &be);
+                    v = shade_pixel(m, L, P0f, sf, stex, vb, vc, va, al, be);
+                }
+                img[base + k] = v;
+            }
+        __syncthreads();   // the frame's shared constants are rewritten by the next iteration
     }
 }
 
@@ -184,6 +246,16 @@ __device__ __forceinline__ void nbr_index(int k, int W, int H, int& kr, int& kl,
     kb = (y < H - 1) ? k + 1 : 0;
 }
 
+// Objective term g^_r . g^_s of one rendered pixel, with the gradient (gx, gy) and h.
+__device__ __forceinline__ double pixel_term(const float* __restrict__ I, int kr, int kl, int ka, int kb, float2 gs,
+                                             float& gx, float& gy, float& h)
+{
+    gx = 0.5f * (I[kr] - I[kl]);
+    gy = 0.5f * (I[kb] - I[ka]);
+    h = 1e-16f + sqrtf(4.f * gx * gx + 4.f * gy * gy);
+    return (double)(2.f * gx / h) * gs.x + (double)(2.f * gy / h) * gs.y;
+}
+
 // Gradients of the blurred rendered crop at every visible pixel, the objective
 // -w sum g^_r.g^_s accumulated in FP64, and the per-pixel records (gx, gy, g^s_x, g^s_y),
 // neighbour positions and Jacobian coefficient used by the normal equations.
@@ -207,11 +279,9 @@ __global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ fram
         const int k = vis[base + pos];
         int kr, kl, ka, kb;
         nbr_index(k, fr.W, fr.H, kr, kl, ka, kb);
-        const float gx = 0.5f * (I[kr] - I[kl]);
-        const float gy = 0.5f * (I[kb] - I[ka]);
-        const float h = 1e-16f + sqrtf(4.f * gx * gx + 4.f * gy * gy);
+        float gx, gy, h;
         const float2 gs = obs_g[fr.obs_off + k];
-        acc += (double)(2.f * gx / h) * gs.x + (double)(2.f * gy / h) * gs.y;
+        acc += pixel_term(I, kr, kl, ka, kb, gs, gx, gy, h);
         pixg[base + pos] = make_float4(gx, gy, gs.x, gs.y);
         // per-pixel record for the normalised-gradient Jacobian: compacted neighbour
         // positions and c = 1 / (0.001 + h^3) (newfuncs.cu:446)
@@ -232,6 +302,46 @@ __global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ fram
 }
 
 // Observation: blur once, then normalised gradients at every crop pixel.
+// The data objective -w sum g^_r . g^_s alone, over the pixels of the visible box whose key
+// holds a triangle (the pixels of the visible list), without the per-pixel records.
+__global__ void __launch_bounds__(256) k_objective(const FrameGPU* __restrict__ frames, const int* __restrict__ act,
+                                                    int n, const unsigned long long* __restrict__ keys,
+                                                    const int4* __restrict__ box, const float* __restrict__ blurred,
+                                                    const float2* __restrict__ obs_g, float w,
+                                                    double* __restrict__ obj)
+{
+    __shared__ double sw[4];
+    const int nact = fl_count(act, n);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        Region r;
+        if (!crop_region(fr, box, f, 0, 0, r)) continue;
+        if (r.X0 + (int)blockIdx.x * nw >= r.X1) continue;
+        const long long base = fr.pix_off;
+        const float* I = blurred + base;
+        double acc = 0.0;
+        for (int x = r.X0 + blockIdx.x * nw + wave; x < r.X1; x += gridDim.x * nw)
+            for (int y = r.Y0 + lane; y < r.Y1; y += 64) {
+                const int k = x * fr.H + y;
+                if (keys[base + k] == KEY_EMPTY) continue;
+                int kr, kl, ka, kb;
+                nbr_index(k, fr.W, fr.H, kr, kl, ka, kb);
+                float gx, gy, h;
+                acc += pixel_term(I, kr, kl, ka, kb, obs_g[fr.obs_off + k], gx, gy, h);
+            }
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if (lane == 0) sw[wave] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double sum = sw[0] + sw[1] + sw[2] + sw[3];
+            if (sum != 0.0) atomicAdd(obj + f, -(double)w * sum);
+        }
+        __syncthreads();   // sw is rewritten by the next iteration
+    }
+}
+
 __global__ void __launch_bounds__(256) k_obs_grad(const FrameGPU* __restrict__ frames, int n, const float* __restrict__ blurred,
                                                    float2* __restrict__ obs_g)
 {
@@ -250,6 +360,28 @@ __global__ void __launch_bounds__(256) k_obs_grad(const FrameGPU* __restrict__ f
     }
 }
 
+// Column kernels: workgroups of four wavefronts, one column per wavefront at a time; one
+// frame per grid row, 8 workgroups per frame for large batches and up to 128 (one column per
+// wavefront) when there are few frames.
+inline dim3 col_grid(int n)
+{
+    int bx = n > 0 ? 4096 / n : 128;
+    bx = bx < 8 ? 8 : (bx > 128 ? 128 : bx);
+    return dim3(bx, grid_frames(n, 1 << 16));
+}
+
+// Visible-list kernels: workgroups of 256 threads over a frame's visible list (the ones past
+// the frame's visible count leave at once), 16 per frame for large batches and up to one per
+// 256 pixels of half the largest crop when there are few frames; one frame per grid row.
+inline dim3 list_grid(int max_area, int n)
+{
+    int bx = (max_area / 2 + 255) / 256;
+    const int cap = n > 0 ? std::max(16, 8192 / n) : 256;
+    if (bx > cap) bx = cap;
+    if (bx < 1) bx = 1;
+    return dim3(bx, grid_frames(n, 1 << 16));
+}
+
 inline dim3 crop_grid(int max_area, int n)
 {
     int bx = (max_area + 255) / 256;
@@ -261,16 +393,39 @@ inline dim3 crop_grid(int max_area, int n)
 } // namespace
 
 int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-                 const float* P0, const float* P, const int* vis_list, const int* npix, const uint64_t* keys,
-                 const float* bary, const int4* box, float* img, int max_area, hipStream_t s)
+                 const float* P0, const int4* pvert, const float2* pbary, const int* npix, const int4* box,
+                 float* img, int max_area, hipStream_t s)
 {
     if (L.ntex > 512) {
         set_error("shade: k_tex %d > 512", L.ntex);
         return TDI_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_clear_crops, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, img);
-    hipLaunchKernelGGL(k_shade, crop_grid(max_area, fl.n), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0, P,
-                       vis_list, npix, (const unsigned long long*)keys, (const float2*)bary, img);
+    hipLaunchKernelGGL(k_clear_crops, col_grid(fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, img);
+    hipLaunchKernelGGL(k_shade, list_grid(max_area, fl.n), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0,
+                       pvert, pbary, npix, img);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_shade_keys(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
+                      const float* P0, const float4* vproj, const uint64_t* keys, const int4* box, float* img,
+                      hipStream_t s)
+{
+    if (L.ntex > 512) {
+        set_error("shade: k_tex %d > 512", L.ntex);
+        return TDI_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_shade_keys, col_grid(fl.n), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0, vproj,
+                       (const unsigned long long*)keys, box, img);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_objective(const FrameGPU* frames, FrameList fl, const uint64_t* keys, const int4* box,
+                     const float* blurred, const float2* obs_g, float w, double* obj, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_objective, col_grid(fl.n), dim3(256), 0, s, frames, fl.act, fl.n,
+                       (const unsigned long long*)keys, box, blurred, obs_g, w, obj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -280,8 +435,8 @@ int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const i
 {
     Taps t;
     for (int i = 0; i < 15; ++i) t.t[i] = taps[i];
-    hipLaunchKernelGGL(k_blur, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, t, in, tmp, 0);
-    hipLaunchKernelGGL(k_blur, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, t, tmp, out, 1);
+    hipLaunchKernelGGL(k_blur, col_grid(fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, t, in, tmp, 0);
+    hipLaunchKernelGGL(k_blur, col_grid(fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, t, tmp, out, 1);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -290,7 +445,7 @@ int launch_grads(const FrameGPU* frames, FrameList fl, const int* vis_list, cons
                  const int* cum, const float* blurred, const float2* obs_g, float w, float4* pixg, int4* pnb,
                  float* pcoef, double* obj, int max_area, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_grads, crop_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, vis_list, npix, cum,
+    hipLaunchKernelGGL(k_grads, list_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, vis_list, npix, cum,
                        blurred, obs_g, w, pixg, pnb, pcoef, obj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
