@@ -12,10 +12,10 @@
 // at the same 78.6 TFLOP/s, so this costs little next to the rest of an iteration.
 //
 //   k_synth_gemv   — one frame per grid row (the plain path): thread per vertex.
-//   k_synth_mfma64 — many frames: [3V x K] x [K x B] on v_mfma_f64_16x16x4f64; a block
-//                    covers 16 vertices (48 basis rows) x 64 frames, each basis value is
-//                    read once per 64 frames; the epilogue fuses mean, rotation,
-//                    translation and projection.
+//   k_synth_tile   — many frames: [3V x K] x [K x B] on v_mfma_f64_16x16x4f64; a workgroup
+//                    owns 16 vertices, stages their basis rows in LDS once and walks every
+//                    frame of the list; the epilogue fuses mean, rotation, translation and
+//                    projection from registers.
 #include "common.h"
 
 namespace tdi {
@@ -98,76 +98,88 @@ __global__ void __launch_bounds__(256) k_synth_gemv(ModelGPU m, Layout L, const 
 //   A[i][k]: lane = i + 16*k          B[k][j]: lane = j + 16*k
 // Accumulator: lane l, element r (0..3) holds C[l/16 + 4*r][l % 16] (measured on gfx950;
 // the FP32/BF16 16x16 forms use row 4*(l/16) + r instead).
-constexpr int S64_ROWS = 48;    // 16 vertices
-constexpr int S64_FR = 64;      // 4 waves x 16 frames
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// Columns are positions in the frame list: workgroup column group y covers list entries
-// 64y .. 64y+63, so a launch over a short list only runs its first column group(s).
-__global__ void __launch_bounds__(256) k_synth_mfma64(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
-                                                       const float* __restrict__ theta, const int* __restrict__ act,
-                                                       int n, const double* __restrict__ rt, float* __restrict__ P0,
-                                                       float* __restrict__ P, float4* __restrict__ vproj)
+// Vertex tiles for v_mfma_f64_16x16x4f64 arranged so that a lane's accumulator holds one
+// vertex: tile row i = 4 c + q is coordinate c (x, y, z; c = 3 is a zero row) of vertex q of a
+// 4-vertex row block, so lane l, element c holds coordinate c of vertex l/16 for frame l%16.
+// A workgroup owns SY_VERTS vertices: their basis rows are staged in LDS (FP32, converted to
+// FP64 on the way into the MFMA) once; then each wavefront walks its own groups of 16 list
+// entries with no further barrier.  A lane loads the coefficients of its frame for its four
+// k residues up front (one batch of independent loads), and the epilogue (mean, FP64
+// rotation, translation, projection) runs from registers.
+constexpr int SY_RB = 4;                   // 4-vertex row blocks per wavefront
+constexpr int SY_VERTS = 4 * SY_RB;        // vertices per workgroup
+constexpr int SY_KMAX = 80;                // expression coefficients held in LDS (K_exp <= 80)
+
+__global__ void __launch_bounds__(256) k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+                                                     const float* __restrict__ theta, const int* __restrict__ act,
+                                                     int n, const double* __restrict__ rt, float* __restrict__ P0,
+                                                     float* __restrict__ P, float4* __restrict__ vproj)
 {
-    const int row0 = blockIdx.x * S64_ROWS;
-    const int c0 = blockIdx.y * S64_FR;
-    const int nact = fl_count(act, n);
-    if (c0 >= nact) return;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    __shared__ double sacc[S64_ROWS][S64_FR + 1];
-    __shared__ double sR[S64_FR][12];
-    __shared__ int sfr[S64_FR];
-    if (tid < S64_FR) {
-        const int c = c0 + tid;
-        const int f = c < nact ? fl_frame(act, c) : -1;
-        sfr[tid] = f;
-        if (f >= 0)
-            for (int k = 0; k < 12; ++k) sR[tid][k] = rt[(size_t)f * 12 + k];
+    __shared__ float sA[SY_RB][SY_KMAX][16];          // [row block][k][tile row]
+    __shared__ float smean[SY_VERTS * 3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int v0 = blockIdx.x * SY_VERTS;
+    const int Ke = L.nexp, ke = m.k_exp;
+    const int K4 = (Ke + 3) & ~3;
+    // basis rows of the workgroup's vertices, zero padded (coordinate 3, k >= Ke, v >= V)
+    for (int e = tid; e < SY_RB * K4 * 16; e += blockDim.x) {
+        const int k = e % K4, i = (e / K4) & 15, rb = (e / K4) >> 4;
+        const int c = i >> 2, v = v0 + 4 * rb + (i & 3);
+        float a = 0.f;
+        if (c < 3 && k < Ke && v < m.V) a = m.exp_basis[(size_t)(3 * v + c) * ke + k];
+        sA[rb][k][i] = a;
+    }
+    for (int e = tid; e < SY_VERTS * 3; e += blockDim.x) {
+        const int v = v0 + e / 3;
+        smean[e] = v < m.V ? m.mean[3 * v0 + e] : 0.f;
     }
     __syncthreads();
-
-    const int Ke = L.nexp;
-    const int rowsTotal = 3 * m.V;
-    {
-        v4d acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-        const int i = lane & 15, kk = lane >> 4;
-        const int fj = sfr[wave * 16 + i];
-        const bool frok = fj >= 0;
-        const float* bcol = theta + (size_t)(frok ? fj : 0) * L.K + L.off_exp;
-        const float* arow[3];
-        bool rowok[3];
+    const int nact = fl_count(act, n);
+    const int ngroups = Ke > 0 ? (nact + 15) >> 4 : 0;   // Ke == 0 is handled by k_synth_gemv
+    const int j = lane & 15, kq = lane >> 4;
+    for (int g = wave; g < ngroups; g += 4) {
+        const int c = 16 * g + j;
+        const bool live = c < nact;
+        const int f = fl_frame(act, live ? c : 0);
+        // this lane's B entries: coefficient kq + 4 q of its frame, every load issued
+        // unconditionally (clamped to a valid column) so the batch has no branches.  Nothing
+        // needs zeroing: the A rows past K_exp are zero, and an output column depends on its
+        // own frame's B column only, so lanes without a live frame just skip the epilogue.
+        const float* tb = theta + (size_t)f * L.K + L.off_exp;
+        float bv[SY_KMAX / 4];
 #pragma unroll
-        for (int rt = 0; rt < 3; ++rt) {
-            const int row = row0 + 16 * rt + i;
-            rowok[rt] = row < rowsTotal;
-            arow[rt] = m.exp_basis + (size_t)(rowok[rt] ? row : 0) * m.k_exp;
-        }
-        for (int k = 0; k < Ke; k += 4) {
-            const int kx = k + kk;
-            const double b = (frok && kx < Ke) ? (double)bcol[kx] : 0.0;
+        for (int q = 0; q < SY_KMAX / 4; ++q) bv[q] = tb[min(kq + 4 * q, Ke - 1)];
+        v4d acc[SY_RB];
 #pragma unroll
-            for (int rt = 0; rt < 3; ++rt) {
-                const double a = (rowok[rt] && kx < Ke) ? (double)arow[rt][kx] : 0.0;
-                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rt], 0, 0, 0);
+        for (int rb = 0; rb < SY_RB; ++rb) acc[rb] = (v4d){0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < SY_KMAX / 4; ++q) {
+            if (4 * q < K4) {
+                const double b = (double)bv[q];
+#pragma unroll
+                for (int rb = 0; rb < SY_RB; ++rb)
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)sA[rb][4 * q + kq][j], b, acc[rb], 0, 0, 0);
             }
         }
+        if (live) {
+            const FrameGPU fr = frames[f];
+            double R[12];
 #pragma unroll
-        for (int rt = 0; rt < 3; ++rt)
+            for (int q = 0; q < 12; ++q) R[q] = rt[(size_t)f * 12 + q];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sacc[16 * rt + (lane >> 4) + 4 * r][wave * 16 + (lane & 15)] = acc[rt][r];
-    }
-    __syncthreads();
-    // epilogue: 16 vertices x 64 frames over 256 threads
-    for (int e = tid; e < 16 * S64_FR; e += blockDim.x) {
-        const int vl = e & 15, j = e >> 4;
-        const int v = row0 / 3 + vl;
-        const int f = sfr[j];
-        if (v >= m.V || f < 0) continue;
-        double p[3];
+            for (int rb = 0; rb < SY_RB; ++rb) {
+                const int vl = 4 * rb + kq, v = v0 + vl;
+                if (v < m.V) {
+                    double p[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) p[a] = (double)m.mean[3 * v + a] + sacc[3 * vl + a][j];
-        store_vertex(frames[f], sR[j], sR[j] + 9, p, (size_t)f * m.V + v, P0, P, vproj);
+                    for (int a = 0; a < 3; ++a) p[a] = (double)smean[3 * vl + a] + acc[rb][a];
+                    store_vertex(fr, R, R + 9, p, (size_t)f * m.V + v, P0, P, vproj);
+                }
+            }
+        }
     }
 }
 
@@ -202,8 +214,9 @@ int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* f
     if (L.nid) return launch_synthesize(m, L, frames, fl, theta, rt, P0, P, vproj, s);
     int rc = launch_frame_rt(L, fl, theta, rt, s);
     if (rc) return rc;
-    dim3 grid((3 * m.V + S64_ROWS - 1) / S64_ROWS, (fl.n + S64_FR - 1) / S64_FR);
-    hipLaunchKernelGGL(k_synth_mfma64, grid, dim3(256), 0, s, m, L, frames, theta, fl.act, fl.n, rt, P0, P, vproj);
+    if (L.nexp > SY_KMAX || L.nexp == 0) return launch_synthesize(m, L, frames, fl, theta, rt, P0, P, vproj, s);
+    const dim3 grid((m.V + SY_VERTS - 1) / SY_VERTS);
+    hipLaunchKernelGGL(k_synth_tile, grid, dim3(256), 0, s, m, L, frames, theta, fl.act, fl.n, rt, P0, P, vproj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
