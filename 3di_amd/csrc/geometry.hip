@@ -104,21 +104,23 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // Vertex tiles for v_mfma_f64_16x16x4f64 arranged so that a lane's accumulator holds one
 // vertex: tile row i = 4 c + q is coordinate c (x, y, z; c = 3 is a zero row) of vertex q of a
 // 4-vertex row block, so lane l, element c holds coordinate c of vertex l/16 for frame l%16.
-// A workgroup owns SY_VERTS vertices: their basis rows are staged in LDS (FP32, converted to
-// FP64 on the way into the MFMA) once; then each wavefront walks its own groups of 16 list
-// entries with no further barrier.  A lane loads the coefficients of its frame for its four
-// k residues up front (one batch of independent loads), and the epilogue (mean, FP64
-// rotation, translation, projection) runs from registers.
+// A workgroup owns SY_VERTS vertices: their basis rows are staged in LDS (as FP64) once; then
+// each wavefront walks its own groups of 16 list entries with no further barrier.  A lane
+// loads the coefficients of its frame for its four k residues up front (one batch of
+// independent loads), and the epilogue (mean, FP64 rotation, translation, projection) runs
+// from registers.
 constexpr int SY_RB = 4;                   // 4-vertex row blocks per wavefront
 constexpr int SY_VERTS = 4 * SY_RB;        // vertices per workgroup
 constexpr int SY_KMAX = 80;                // expression coefficients held in LDS (K_exp <= 80)
 
-__global__ void __launch_bounds__(256) k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
-                                                     const float* __restrict__ theta, const int* __restrict__ act,
-                                                     int n, const double* __restrict__ rt, float* __restrict__ P0,
-                                                     float* __restrict__ P, float4* __restrict__ vproj)
+// Three wavefronts per SIMD (register budget 168) hide the coefficient loads of one behind the
+// MFMAs of the others.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const float* __restrict__ theta,
+             const int* __restrict__ act, int n, const double* __restrict__ rt, float* __restrict__ P0,
+             float* __restrict__ P, float4* __restrict__ vproj)
 {
-    __shared__ float sA[SY_RB][SY_KMAX][16];          // [row block][k][tile row]
+    __shared__ double sA[SY_RB][SY_KMAX][16];          // [row block][k][tile row]
     __shared__ float smean[SY_VERTS * 3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int v0 = blockIdx.x * SY_VERTS;
@@ -128,8 +130,8 @@ __global__ void __launch_bounds__(256) k_synth_tile(ModelGPU m, Layout L, const 
     for (int e = tid; e < SY_RB * K4 * 16; e += blockDim.x) {
         const int k = e % K4, i = (e / K4) & 15, rb = (e / K4) >> 4;
         const int c = i >> 2, v = v0 + 4 * rb + (i & 3);
-        float a = 0.f;
-        if (c < 3 && k < Ke && v < m.V) a = m.exp_basis[(size_t)(3 * v + c) * ke + k];
+        double a = 0.0;
+        if (c < 3 && k < Ke && v < m.V) a = (double)m.exp_basis[(size_t)(3 * v + c) * ke + k];
         sA[rb][k][i] = a;
     }
     for (int e = tid; e < SY_VERTS * 3; e += blockDim.x) {
@@ -161,7 +163,7 @@ __global__ void __launch_bounds__(256) k_synth_tile(ModelGPU m, Layout L, const 
                 const double b = (double)bv[q];
 #pragma unroll
                 for (int rb = 0; rb < SY_RB; ++rb)
-                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)sA[rb][4 * q + kq][j], b, acc[rb], 0, 0, 0);
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[rb][4 * q + kq][j], b, acc[rb], 0, 0, 0);
             }
         }
         if (live) {
