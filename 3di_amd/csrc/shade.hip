@@ -6,7 +6,7 @@
 //       Id = n.(R^T (lambda - tau)) - n.p0, I = rho (1 + L Id), rho barycentric albedo;
 //   fill_tex_im1 (funcs.cu:800-817): scatter I into the crop (zero elsewhere);
 //   convolutionRowsGPU/ColumnsGPU (newfuncs.cu:539-715): separable 15-tap Gaussian,
-//       zero boundary;
+//       zero boundary (here both passes in one LDS-tiled kernel);
 //   fill_grads (funcs.cu:842-897): g = (I[x+1]-I[x-1], I[y+1]-I[y-1]) / 2 with crop-border
 //       neighbours reading crop index 0, h = 1e-16 + sqrt(4 gx^2 + 4 gy^2), g^ = 2 g / h,
 //       the same for the blurred observation;
@@ -24,9 +24,9 @@ namespace {
 // Region of a crop a pass has to cover, half-open [X0, X1) x [Y0, Y1).  Without a box (or
 // when the visible pixels touch the crop border, whose neighbour rule reads crop index 0) it
 // is the whole crop.  With the visible box (xl, xh, yl, yh) of the frame: the normalised
-// gradients read the blurred image at visible pixels +-1, the blur along x reaches 7 pixels
-// and the blur along y 7 rows, so pass 1 (along x) covers the box + 1, pass 0 (along y) the
-// box + 8 columns / + 1 row, and the rendered image must be clear on the box + 8.
+// gradients read the blurred image at visible pixels +-1 and the blur reaches 7 pixels along
+// x and y, so the blurred output covers the box + 1 and the rendered image must be clear on
+// the box + 8.
 struct Region {
     int X0, X1, Y0, Y1;
 };
@@ -186,57 +186,6 @@ __global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const 
     }
 }
 
-// One pass of the 15-tap separable blur over a frame's region: one wavefront per column x, its
-// lanes over the rows y (column-major crops, so a wavefront's loads and stores are contiguous).
-// dir 0 blurs along y, dir 1 along x; samples outside the crop count as zero.
-__global__ void __launch_bounds__(256) k_blur(const FrameGPU* __restrict__ frames, const int* __restrict__ act, int n,
-                                               const int4* __restrict__ box, Taps taps, const float* __restrict__ in,
-                                               float* __restrict__ out, int dir)
-{
-    const int nact = fl_count(act, n);
-    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
-        const int f = fl_frame(act, yi);
-        const FrameGPU fr = frames[f];
-        const int W = fr.W, H = fr.H;
-        Region rg;
-        if (!crop_region(fr, box, f, dir == 0 ? 8 : 1, 1, rg)) continue;
-        const float* src = in + fr.pix_off;
-        float* dst = out + fr.pix_off;
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-        for (int x = rg.X0 + blockIdx.x * nw + wave; x < rg.X1; x += gridDim.x * nw) {
-            const float* col = src + (size_t)x * H;
-            const bool xin = x >= 7 && x + 7 < W;
-            for (int y = rg.Y0 + lane; y < rg.Y1; y += 64) {
-                float s = 0.f;
-                if (dir == 0) {
-                    if (y >= 7 && y + 7 < H) {
-#pragma unroll
-                        for (int j = -7; j <= 7; ++j) s = fmaf(taps.t[7 - j], col[y + j], s);
-                    } else {
-#pragma unroll
-                        for (int j = -7; j <= 7; ++j) {
-                            const int yy = y + j;
-                            if (yy >= 0 && yy < H) s = fmaf(taps.t[7 - j], col[yy], s);
-                        }
-                    }
-                } else {
-                    if (xin) {
-#pragma unroll
-                        for (int j = -7; j <= 7; ++j) s = fmaf(taps.t[7 - j], col[(long long)j * H + y], s);
-                    } else {
-#pragma unroll
-                        for (int j = -7; j <= 7; ++j) {
-                            const int xx = x + j;
-                            if (xx >= 0 && xx < W) s = fmaf(taps.t[7 - j], col[(long long)j * H + y], s);
-                        }
-                    }
-                }
-                dst[(size_t)x * H + y] = s;
-            }
-        }
-    }
-}
-
 __device__ __forceinline__ void nbr_index(int k, int W, int H, int& kr, int& kl, int& ka, int& kb)
 {
     const int x = k / H, y = k - x * H;
@@ -244,6 +193,68 @@ __device__ __forceinline__ void nbr_index(int k, int W, int H, int& kr, int& kl,
     kl = (x > 0) ? k - H : 0;
     ka = (y > 0) ? k - 1 : 0;
     kb = (y < H - 1) ? k + 1 : 0;
+}
+
+// Both passes of the blur in one kernel, over tiles of the output region (the visible box + 1,
+// or the whole crop): a 32-column x 64-row output tile loads its input with a 7-pixel halo
+// into LDS (zero outside the crop), blurs along y into an LDS intermediate over the tile's
+// 46 columns, then along x.  The taps are applied in the order of the two-pass kernel
+// (j = -7..7, FMA) of separate row and column passes.
+constexpr int BT_X = 32, BT_Y = 64;
+constexpr int BT_IX = BT_X + 14, BT_IY = BT_Y + 14;
+
+__global__ void __launch_bounds__(256) k_blur_tiles(const FrameGPU* __restrict__ frames, const int* __restrict__ act,
+                                                     int n, const int4* __restrict__ box, Taps taps,
+                                                     const float* __restrict__ in, float* __restrict__ out)
+{
+    __shared__ float simg[BT_IX][BT_IY + 1];
+    __shared__ float stmp[BT_IX][BT_Y + 1];
+    const int nact = fl_count(act, n);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        Region rg;
+        if (!crop_region(fr, box, f, 1, 1, rg)) continue;
+        const int W = fr.W, H = fr.H;
+        const int ntx = (rg.X1 - rg.X0 + BT_X - 1) / BT_X, nty = (rg.Y1 - rg.Y0 + BT_Y - 1) / BT_Y;
+        const float* src = in + fr.pix_off;
+        float* dst = out + fr.pix_off;
+        for (int t = blockIdx.x; t < ntx * nty; t += gridDim.x) {
+            const int x0 = rg.X0 + (t % ntx) * BT_X, y0 = rg.Y0 + (t / ntx) * BT_Y;
+            // input tile: columns x0-7 .. x0+BT_X+6, rows y0-7 .. y0+BT_Y+6, zero outside the crop
+            for (int c = wave; c < BT_IX; c += 4) {
+                const int x = x0 - 7 + c;
+                for (int r = lane; r < BT_IY; r += 64) {
+                    const int y = y0 - 7 + r;
+                    simg[c][r] = (x >= 0 && x < W && y >= 0 && y < H) ? src[(size_t)x * H + y] : 0.f;
+                }
+            }
+            __syncthreads();
+            // along y, for the rows of the tile over all its input columns (zero outside the crop)
+            for (int c = wave; c < BT_IX; c += 4) {
+                const int x = x0 - 7 + c;
+                float s = 0.f;
+                if (x >= 0 && x < W) {
+#pragma unroll
+                    for (int j = -7; j <= 7; ++j) s = fmaf(taps.t[7 - j], simg[c][lane + 7 + j], s);
+                }
+                stmp[c][lane] = s;
+            }
+            __syncthreads();
+            // along x
+            for (int c = wave; c < BT_X; c += 4) {
+                const int x = x0 + c, y = y0 + lane;
+                if (x < rg.X1 && y < rg.Y1) {
+                    float s = 0.f;
+#pragma unroll
+                    for (int j = -7; j <= 7; ++j) s = fmaf(taps.t[7 - j], stmp[c + 7 + j][lane], s);
+                    dst[(size_t)x * H + y] = s;
+                }
+            }
+            __syncthreads();   // simg/stmp are rewritten by the next tile
+        }
+    }
 }
 
 // Objective term g^_r . g^_s of one rendered pixel, with the gradient (gx, gy) and h.
@@ -435,8 +446,8 @@ int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const i
 {
     Taps t;
     for (int i = 0; i < 15; ++i) t.t[i] = taps[i];
-    hipLaunchKernelGGL(k_blur, col_grid(fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, t, in, tmp, 0);
-    hipLaunchKernelGGL(k_blur, col_grid(fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, t, tmp, out, 1);
+    (void)tmp;   // the fused kernel keeps the intermediate in LDS
+    hipLaunchKernelGGL(k_blur_tiles, col_grid(fl.n), dim3(256), 0, s, frames, fl.act, fl.n, box, t, in, out);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
