@@ -205,13 +205,21 @@ int tdi_ctx_create(const tdi_model_desc* md, const tdi_fit_options* opts, int de
         if (d && src) (void)hipMemcpy(d, src, bytes, hipMemcpyHostToDevice);
         return d;
     };
+    // bases with rows padded to a multiple of 16 columns (zeros), see ModelGPU
+    auto up_rows = [&](const float* src, size_t rows, int cols, int ld) -> const float* {
+        std::vector<float> h(rows * (size_t)ld, 0.f);
+        for (size_t r = 0; r < rows; ++r) std::memcpy(h.data() + r * ld, src + r * cols, sizeof(float) * cols);
+        return (const float*)up(h.data(), sizeof(float) * h.size());
+    };
+    auto ld16 = [](int k) { return (k + 15) & ~15; };
     ModelGPU& m = c->m;
     m.V = V; m.NT = NT; m.k_id = md->k_id; m.k_tex = md->k_tex; m.k_exp = md->k_exp;
+    m.id_ld = ld16(md->k_id); m.exp_ld = ld16(md->k_exp); m.tex_ld = ld16(md->k_tex);
     m.mean = (const float*)up(md->mean_shape, sizeof(float) * 3 * V);
-    m.exp_basis = md->k_exp ? (const float*)up(md->exp_basis, sizeof(float) * 3 * V * (size_t)md->k_exp) : nullptr;
-    m.id_basis = md->k_id ? (const float*)up(md->id_basis, sizeof(float) * 3 * V * (size_t)md->k_id) : nullptr;
+    m.exp_basis = md->k_exp ? up_rows(md->exp_basis, 3 * (size_t)V, md->k_exp, m.exp_ld) : nullptr;
+    m.id_basis = md->k_id ? up_rows(md->id_basis, 3 * (size_t)V, md->k_id, m.id_ld) : nullptr;
     m.tex_mean = (const float*)up(md->tex_mean, sizeof(float) * V);
-    m.tex_basis = md->k_tex ? (const float*)up(md->tex_basis, sizeof(float) * V * (size_t)md->k_tex) : nullptr;
+    m.tex_basis = md->k_tex ? up_rows(md->tex_basis, (size_t)V, md->k_tex, m.tex_ld) : nullptr;
     m.tris = (const int*)up(md->triangles, sizeof(int) * 3 * (size_t)NT);
     m.lmk_ids = (const int*)up(md->landmark_ids, sizeof(int) * NLMK);
     const Layout& L = c->L;

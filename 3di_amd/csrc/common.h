@@ -40,11 +40,13 @@ inline Layout make_layout(const tdi_model_desc* m, const tdi_fit_options* o)
 // Model resident in HBM (Renderer's device arrays, renderer.cu:28-72).
 struct ModelGPU {
     int V, NT, k_id, k_tex, k_exp;
+    int id_ld, exp_ld, tex_ld; // row strides of the bases: the column counts rounded up to 16, so
+                               // that a 16-lane row segment is one aligned 64-byte sector
     const float* mean;       // [V][3]
-    const float* id_basis;   // [3V][k_id]
-    const float* exp_basis;  // [3V][k_exp]
+    const float* id_basis;   // [3V][id_ld]
+    const float* exp_basis;  // [3V][exp_ld]
     const float* tex_mean;   // [V]
-    const float* tex_basis;  // [V][k_tex]
+    const float* tex_basis;  // [V][tex_ld]
     const int* tris;         // [NT][3]
     const int* lmk_ids;      // [51]
     const float* lb;         // [K] lower bounds of the unknowns (-inf = unbounded)

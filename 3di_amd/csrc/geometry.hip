@@ -80,10 +80,10 @@ __global__ void __launch_bounds__(256) k_synth_gemv(ModelGPU m, Layout L, const 
             for (int a = 0; a < 3; ++a) {
                 const int r = 3 * v + a;
                 double s = m.mean[r];
-                const float* e = m.exp_basis + (size_t)r * m.k_exp;
+                const float* e = m.exp_basis + (size_t)r * m.exp_ld;
                 for (int k = 0; k < L.nexp; ++k) s = fma((double)e[k], (double)th[L.off_exp + k], s);
                 if (L.nid) {
-                    const float* ib = m.id_basis + (size_t)r * m.k_id;
+                    const float* ib = m.id_basis + (size_t)r * m.id_ld;
                     for (int k = 0; k < L.nid; ++k) s = fma((double)ib[k], (double)th[L.off_id + k], s);
                 }
                 p[a] = s;
@@ -124,7 +124,7 @@ k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const fl
     __shared__ float smean[SY_VERTS * 3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int v0 = blockIdx.x * SY_VERTS;
-    const int Ke = L.nexp, ke = m.k_exp;
+    const int Ke = L.nexp, ke = m.exp_ld;
     const int K4 = (Ke + 3) & ~3;
     // basis rows of the workgroup's vertices, zero padded (coordinate 3, k >= Ke, v >= V)
     for (int e = tid; e < SY_RB * K4 * 16; e += blockDim.x) {

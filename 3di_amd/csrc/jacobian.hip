@@ -127,9 +127,9 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
             Id = nx * lx + ny * ly + nz * lz - (nx * p0[0] + ny * p0[1] + nz * p0[2]);
             float rb = m.tex_mean[vb], rc = m.tex_mean[vc], ra = m.tex_mean[va];
             for (int q = 0; q < L.ntex; ++q) {
-                rb = fmaf(m.tex_basis[(size_t)vb * m.k_tex + q], stex[q], rb);
-                rc = fmaf(m.tex_basis[(size_t)vc * m.k_tex + q], stex[q], rc);
-                ra = fmaf(m.tex_basis[(size_t)va * m.k_tex + q], stex[q], ra);
+                rb = fmaf(m.tex_basis[(size_t)vb * m.tex_ld + q], stex[q], rb);
+                rc = fmaf(m.tex_basis[(size_t)vc * m.tex_ld + q], stex[q], rc);
+                ra = fmaf(m.tex_basis[(size_t)va * m.tex_ld + q], stex[q], ra);
             }
             rho = al * rb + be * rc + ga * ra;
             rn[0] = sr.R[0] * nx + sr.R[3] * ny + sr.R[6] * nz;
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
         // expression columns, 80 per group: every basis gather of the group is issued before
         // the arithmetic (clamped columns, masked stores), so the group costs one round trip
         if (L.nexp) {
-            const int ke = m.k_exp;
+            const int ke = m.exp_ld;
             const float* Eb = m.exp_basis + (size_t)(3 * vb) * ke;   // rows 3v, 3v+1, 3v+2
             const float* Ec = m.exp_basis + (size_t)(3 * vc) * ke;
             const float* Ea = m.exp_basis + (size_t)(3 * va) * ke;
@@ -185,15 +185,15 @@ __global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU
             } else if (j >= L.off_id) {
                 const int c = j - L.off_id;
                 const float* E = m.id_basis;
-                const int ki = m.k_id;
+                const int ki = m.id_ld;
                 const float ex = al * E[(size_t)(3 * vb) * ki + c] + be * E[(size_t)(3 * vc) * ki + c] + ga * E[(size_t)(3 * va) * ki + c];
                 const float ey = al * E[(size_t)(3 * vb + 1) * ki + c] + be * E[(size_t)(3 * vc + 1) * ki + c] + ga * E[(size_t)(3 * va + 1) * ki + c];
                 const float ez = al * E[(size_t)(3 * vb + 2) * ki + c] + be * E[(size_t)(3 * vc + 2) * ki + c] + ga * E[(size_t)(3 * va + 2) * ki + c];
                 val = q0 * ex + q1 * ey + q2 * ez;
             } else {
                 const int c = j - L.off_tex;
-                const float tt = al * m.tex_basis[(size_t)vb * m.k_tex + c] + be * m.tex_basis[(size_t)vc * m.k_tex + c]
-                               + ga * m.tex_basis[(size_t)va * m.k_tex + c];
+                const float tt = al * m.tex_basis[(size_t)vb * m.tex_ld + c] + be * m.tex_basis[(size_t)vc * m.tex_ld + c]
+                               + ga * m.tex_basis[(size_t)va * m.tex_ld + c];
                 val = tt + sr.L * Id * tt;
             }
             out[j] = val;

@@ -102,9 +102,9 @@ __device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L,
     const float Id = nx * lx + ny * ly + nz * lz - (nx * px + ny * py + nz * pz);
     float rb = m.tex_mean[vb], rc = m.tex_mean[vc], ra = m.tex_mean[va];
     for (int q = 0; q < L.ntex; ++q) {
-        rb = fmaf(m.tex_basis[(size_t)vb * m.k_tex + q], stex[q], rb);
-        rc = fmaf(m.tex_basis[(size_t)vc * m.k_tex + q], stex[q], rc);
-        ra = fmaf(m.tex_basis[(size_t)va * m.k_tex + q], stex[q], ra);
+        rb = fmaf(m.tex_basis[(size_t)vb * m.tex_ld + q], stex[q], rb);
+        rc = fmaf(m.tex_basis[(size_t)vc * m.tex_ld + q], stex[q], rc);
+        ra = fmaf(m.tex_basis[(size_t)va * m.tex_ld + q], stex[q], ra);
     }
     const float rho = al * rb + be * rc + ga * ra;
     return rho + sf.L * rho * Id;

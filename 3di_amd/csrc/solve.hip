@@ -68,8 +68,8 @@ __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const Fra
             for (int a = 0; a < 3; ++a) {
                 const int r = 3 * v + a;
                 float s = m.mean[r];
-                for (int k = 0; k < L.nexp; ++k) s = fmaf(m.exp_basis[(size_t)r * m.k_exp + k], tf[L.off_exp + k], s);
-                for (int k = 0; k < L.nid; ++k) s = fmaf(m.id_basis[(size_t)r * m.k_id + k], tf[L.off_id + k], s);
+                for (int k = 0; k < L.nexp; ++k) s = fmaf(m.exp_basis[(size_t)r * m.exp_ld + k], tf[L.off_exp + k], s);
+                for (int k = 0; k < L.nid; ++k) s = fmaf(m.id_basis[(size_t)r * m.id_ld + k], tf[L.off_id + k], s);
                 p[a] = s;
                 sp[i][a] = s;
             }
@@ -112,12 +112,12 @@ __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const Fra
                 } else if (j >= L.off_exp && j < L.off_pose) {
                     const int v = m.lmk_ids[i], c = j - L.off_exp;
                     float e3[3];
-                    for (int a = 0; a < 3; ++a) e3[a] = m.exp_basis[(size_t)(3 * v + a) * m.k_exp + c];
+                    for (int a = 0; a < 3; ++a) e3[a] = m.exp_basis[(size_t)(3 * v + a) * m.exp_ld + c];
                     for (int a = 0; a < 3; ++a) dv[a] = sR[a] * e3[0] + sR[3 + a] * e3[1] + sR[6 + a] * e3[2];
                 } else if (j >= L.off_id && j < L.off_exp) {
                     const int v = m.lmk_ids[i], c = j - L.off_id;
                     float e3[3];
-                    for (int a = 0; a < 3; ++a) e3[a] = m.id_basis[(size_t)(3 * v + a) * m.k_id + c];
+                    for (int a = 0; a < 3; ++a) e3[a] = m.id_basis[(size_t)(3 * v + a) * m.id_ld + c];
                     for (int a = 0; a < 3; ++a) dv[a] = sR[a] * e3[0] + sR[3 + a] * e3[1] + sR[6 + a] * e3[2];
                 } else {
                     geo = false;
