@@ -51,12 +51,17 @@ def stage_work(w, K, npix, n):
     raster_bytes = float(np.sum(8 * A + 8 * A + 4 * A + NT * (12 + 3 * 16) + 8 * P + 12 * P))
     shade_bytes = float(np.sum(4 * A + P * (4 + 8 + 8 + 3 * 12 + 3 * 4 + 4) + 16 * A + P * (4 + 4 * 4 + 8 + 16)))
     barrier_flops = 2.0 * 51 * 3 * Ke * n
-    dI_bytes = float(np.sum(P * (4 + 8 + 8 + 3 * 24 + 16 + 9 * Ke * 4 + 4 * K)))
+    # dI at the HBM level: per rendered pixel its vertex ids, barycentrics and image gradient
+    # (16 + 8 + 16 B) and the K-float row written; per frame the vertex positions once; per
+    # launch the expression basis once (it stays in the caches).  The 9 basis rows each pixel
+    # gathers (9 * Ke * 4 B) go through L2/MALL and are reported apart as gather_GBs.
+    dI_bytes = float(np.sum(P * (16 + 8 + 16 + 4 * K)) + n * 12.0 * V + 12.0 * V * Ke)
+    dI_gather = float(np.sum(P * 9 * Ke * 4))
     jtj_flops = float(np.sum(4.0 * K * K * P))
     chol_flops = n * K ** 3 / 3.0
     return [("synthesis", "mfma", synth_flops, "fp64"), ("raster", "hbm", raster_bytes, None),
             ("shade+blur+grads", "hbm", shade_bytes, None), ("barrier", "mfma", barrier_flops, "fp32"),
-            ("jacobian dI", "hbm", dI_bytes, None), ("JtJ (BF16 split MFMA)", "mfma", jtj_flops, "bf16"),
+            ("jacobian dI", "hbm", dI_bytes, None, dI_gather), ("JtJ (BF16 split MFMA)", "mfma", jtj_flops, "bf16"),
             ("cholesky", "mfma", chol_flops, "fp32")]
 
 
@@ -219,7 +224,8 @@ def main():
     npix = batch.pixel_counts()
     work = stage_work(w, K, npix, n_chunk)
     stages = []
-    for (name, bound, amount, prec), t in zip(work, ms):
+    for wk, t in zip(work, ms):
+        name, bound, amount, prec = wk[:4]
         if bound == "hbm":
             ach = amount / (t * 1e-3) / 1e9
             peak = PEAK_HBM_GBS
@@ -230,13 +236,19 @@ def main():
             unit = "TFLOP/s"
         stages.append(dict(stage=name, ms=round(float(t), 4), bound=bound, achieved=round(float(ach), 3), peak=peak,
                            unit=unit, frac=round(float(ach / peak), 5), frames=int(n_chunk)))
-        log(f"[bench] stage {name:24s} {t:8.3f} ms  {ach:9.2f} {unit} ({100 * ach / peak:.1f}% of peak)")
+        extra = ""
+        if len(wk) > 4:
+            stages[-1]["gather_GBs"] = round(float(wk[4] / (t * 1e-3) / 1e9), 1)
+            extra = f", basis gathers {stages[-1]['gather_GBs']:.0f} GB/s through L2/MALL"
+        log(f"[bench] stage {name:24s} {t:8.3f} ms  {ach:9.2f} {unit} ({100 * ach / peak:.1f}% of peak{extra})")
     dom = max(stages, key=lambda s: s["ms"])
     traffic = None
     if os.path.exists(args.traffic_json):
-        tj = json.load(open(args.traffic_json))
-        if tj.get("stage") == dom["stage"] and tj.get("frames_per_launch") == int(n_chunk):
-            traffic = tj.get("bytes_per_launch")
+        # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the
+        # kernels profiled at this launch size
+        for tj in json.load(open(args.traffic_json)).get("kernels", []):
+            if tj.get("stage") == dom["stage"] and tj.get("frames_per_launch") == int(n_chunk):
+                traffic = tj.get("bytes_per_launch")
     roofline = dict(kernel=dom["stage"], bound=dom["bound"], achieved=dom["achieved"], peak=dom["peak"],
                     unit=dom["unit"], frac=dom["frac"], traffic=traffic,
                     launch_ms=dom["ms"], frames_per_launch=int(n_chunk))
