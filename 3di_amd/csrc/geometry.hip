@@ -123,6 +123,7 @@ k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const fl
     __shared__ double sA[SY_RB][SY_KMAX][16];          // [row block][k][tile row]
     __shared__ float smean[SY_VERTS * 3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (fl_count(act, n) == 0) return;       // an emptied list: skip the basis staging
     const int v0 = blockIdx.x * SY_VERTS;
     const int Ke = L.nexp, ke = m.exp_ld;
     const int K4 = (Ke + 3) & ~3;
