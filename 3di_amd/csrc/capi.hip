@@ -102,9 +102,12 @@ struct Workspace {
     float* dI = nullptr;
     float *theta = nullptr, *theta_try = nullptr, *delta = nullptr, *H = nullptr, *g = nullptr;
     double *obj = nullptr, *objb = nullptr;
+    double* objp = nullptr;    // [C][256] per-workgroup partials of the data objective
     int *npix = nullptr, *status = nullptr, *face_ok = nullptr, *mask = nullptr, *searching = nullptr,
         *evalmask = nullptr;
     int *act_m = nullptr, *act_e = nullptr;   // frame lists of mask / evalmask ([0] = count)
+    int *mask_r = nullptr, *mask_u = nullptr;  // active frames with a stale / reusable render
+    int *act_r = nullptr, *act_u = nullptr;    // their frame lists
     int* tcount = nullptr;                     // [C][tmax] resolve tile counts
     int tmax = 0;
     double* rt = nullptr;                      // [C][12] FP64 rotation (column-major) + translation
@@ -286,10 +289,15 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     w.g = A.get<float>((size_t)C * K);
     w.obj = A.get<double>(C);
     w.objb = A.get<double>(C);
+    w.objp = A.get<double>((size_t)C * 256);
     w.npix = A.get<int>(C);
     w.status = A.get<int>(C);
     w.face_ok = A.get<int>(C);
     w.mask = A.get<int>(C);
+    w.mask_r = A.get<int>(C);
+    w.mask_u = A.get<int>(C);
+    w.act_r = A.get<int>(C + 1);
+    w.act_u = A.get<int>(C + 1);
     w.searching = A.get<int>(C);
     w.evalmask = A.get<int>(C);
     w.act_m = A.get<int>(C + 1);
@@ -503,7 +511,7 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
         return rc;
     if ((rc = launch_blur(w.frames, fl, c->taps, full_crop ? nullptr : w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
     return launch_grads(w.frames, fl, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
-                        w.pcoef, w.obj, w.max_area, s);
+                        w.pcoef, w.objp, w.obj, w.max_area, s);
 }
 
 // Objective only, for the line search (the F(theta + t delta) of fit_3DMM_shape_rigid's Armijo
@@ -531,7 +539,7 @@ static int eval_objective(tdi_batch* b, FrameList fl, const float* theta, int mi
         return rc;
     if ((rc = launch_shade_keys(c->m, L, w.frames, fl, theta, w.P0, w.vproj, w.keys, w.box, w.img, s))) return rc;
     if ((rc = launch_blur(w.frames, fl, c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
-    return launch_objective(w.frames, fl, w.keys, w.box, w.blurred, b->obs_g, c->opts.data_weight, w.obj, s);
+    return launch_objective(w.frames, fl, w.keys, w.box, w.blurred, b->obs_g, c->opts.data_weight, w.objp, w.obj, s);
 }
 
 static int launch_normal_eq(tdi_batch* b, FrameList fl, int mixed, hipStream_t s, const Layout* Lp = nullptr)
@@ -547,21 +555,47 @@ static int launch_normal_eq(tdi_batch* b, FrameList fl, int mixed, hipStream_t s
                           w.partials, w.S, s);
 }
 
-// Objective, gradient and Gauss-Newton matrix at theta (get_obj_hessian_and_gradient_multiframe).
-static int eval_full(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s,
-                     const Layout* Lp = nullptr)
+// The visible list, barrier value, per-pixel records and objective of frames whose render of
+// theta is already in the workspace (synthesis, keys, visible box, shaded and blurred images).
+static int eval_reused(tdi_batch* b, FrameList fl, const float* theta, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    const Layout& L = c->L;
+    Workspace& w = b->ws;
+    int rc;
+    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary,
+                             w.max_area, w.tcount, w.tmax, w.box, s)))
+        return rc;
+    if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj, w.objb,
+                             w.H, w.g, s)))
+        return rc;
+    return launch_grads(w.frames, fl, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
+                        w.pcoef, w.objp, w.obj, w.max_area, s);
+}
+
+// dI, the normal equations and the barrier's gradient and Hessian terms, after a render.
+static int eval_jacobian(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s,
+                         const Layout* Lp = nullptr)
 {
     tdi_ctx* c = b->ctx;
     const Layout& L = Lp ? *Lp : c->L;
     Workspace& w = b->ws;
-    int rc = eval_render(b, fl, theta, mixed, s, &L);
-    if (rc) return rc;
+    int rc;
     if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
                         w.max_area, s)))
         return rc;
     if ((rc = launch_normal_eq(b, fl, mixed, s, &L))) return rc;
     return launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj, w.objb,
                           w.H, w.g, s);
+}
+
+// Objective, gradient and Gauss-Newton matrix at theta (get_obj_hessian_and_gradient_multiframe).
+static int eval_full(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s,
+                     const Layout* Lp = nullptr)
+{
+    int rc = eval_render(b, fl, theta, mixed, s, Lp);
+    if (rc) return rc;
+    return eval_jacobian(b, fl, theta, mixed, s, Lp);
 }
 
 // active mask for real frames of the current chunk
@@ -713,8 +747,7 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
             return rc;
         if ((rc = launch_blur(w.frames, all_frames(n), c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
         if ((rc = launch_grads(w.frames, all_frames(n), w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg,
-                               w.pnb, w.pcoef, w.obj,
-                               w.max_area, s)))
+                               w.pnb, w.pcoef, w.objp, w.obj, w.max_area, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[3], s));
         if ((rc = launch_barrier(c->m, c->L, w.frames, all_frames(n), theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
@@ -1002,7 +1035,16 @@ static int record_fit(tdi_batch* b, hipStream_t s)
     for (int it = 0; it < o.max_iters; ++it) {
         if ((rc = launch_copy_mask(n, w.st, w.mask, s))) return rc;
         if ((rc = launch_compact_frames(n, w.mask, w.act_m, s))) return rc;
-        if ((rc = eval_full(b, FrameList{w.act_m, w.C}, w.theta, o.mixed_flags, s))) return rc;
+        // A frame whose last line-search evaluation was accepted has the render of its new theta
+        // in its own workspace slots (later rounds render only the frames still searching): its
+        // synthesis, keys, visible box, shaded and blurred images are reused.  The others (the
+        // first iteration, a search that ended on tz < 0 or on the round cap) are rendered.
+        if ((rc = launch_split_fresh(n, w.st, w.mask_r, w.mask_u, s))) return rc;
+        if ((rc = launch_compact_frames(n, w.mask_r, w.act_r, s))) return rc;
+        if ((rc = launch_compact_frames(n, w.mask_u, w.act_u, s))) return rc;
+        if ((rc = eval_render(b, FrameList{w.act_r, w.C}, w.theta, o.mixed_flags, s))) return rc;
+        if ((rc = eval_reused(b, FrameList{w.act_u, w.C}, w.theta, s))) return rc;
+        if ((rc = eval_jacobian(b, FrameList{w.act_m, w.C}, w.theta, o.mixed_flags, s))) return rc;
         if ((rc = launch_cholesky(n, K, w.H, w.g, w.delta, w.status, (o.mixed_flags & TDI_MIXED_CHOL_F32) ? 1 : 0,
                                   w.mask, s)))
             return rc;

@@ -72,6 +72,7 @@ struct FitState {
     int stop;                // tdi_stop flags
     int status;              // tdi_status
     int iters, outer, ls_evals, inner;
+    int fresh;               // 1 when the frame's render in the workspace is that of its theta
     float t;                 // current step
     float dg;                // delta . g
     double obj0;             // F(theta) of the current iteration
@@ -278,14 +279,15 @@ int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, Fra
 int launch_shade_keys(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
                       const float* P0, const float4* vproj, const uint64_t* keys, const int4* box, float* img,
                       hipStream_t s);
+// objp: [frames][256] per-workgroup partials of the data objective (reduced in order into obj)
 int launch_objective(const FrameGPU* frames, FrameList fl, const uint64_t* keys, const int4* box,
-                     const float* blurred, const float2* obs_g, float w, double* obj, hipStream_t s);
+                     const float* blurred, const float2* obs_g, float w, double* objp, double* obj, hipStream_t s);
 // box == nullptr: whole crops; else only the region the normalised gradients need
 int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const int4* box, const float* in, float* tmp,
                 float* out, int max_area, hipStream_t s);
 int launch_grads(const FrameGPU* frames, FrameList fl, const int* vis_list, const int* npix,
                  const int* cum, const float* blurred, const float2* obs_g, float w, float4* pixg, int4* pnb,
-                 float* pcoef, double* obj, int max_area, hipStream_t s);
+                 float* pcoef, double* objp, double* obj, int max_area, hipStream_t s);
 int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const float* obs, float* tmp, float* blurred,
                        float2* obs_g, int max_area, hipStream_t s);
 
@@ -318,6 +320,7 @@ int launch_ls_decide(int n, int K, FitState* st, const int* evalmask, int* searc
 int launch_face_check(int V, const int* lmk_ids, const float4* vproj, float ref_face_size, FrameList fl,
                       int* face_ok, hipStream_t s);
 int launch_copy_mask(int n, const FitState* st, int* mask, hipStream_t s);
+int launch_split_fresh(int n, const FitState* st, int* mask_r, int* mask_u, hipStream_t s);
 // act[0] = number of set entries of mask[0..n-1], act[1..] = their indices in increasing order
 int launch_compact_frames(int n, const int* mask, int* act, hipStream_t s);
 

@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T >= 7
     __shared__ float4 scf[JL_PX];                          // (c|d| dy, -c|d| dx, q dy, -q dx)
     __shared__ int4 snb[JL_PX];
     __shared__ float srk[JL_PX];
-    __shared__ float sg[KP];
+    __shared__ float sg[NITEM];                             // J^T r partial of every item
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int s = blockIdx.x;
@@ -648,17 +648,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T >= 7
         case 2: jl_store<T, 2>(acc, Hout, lane); break;
         default: jl_store<T, 3>(acc, Hout, lane); break;
         }
-        // J^T r: the items of one column meet in LDS
-        for (int j = tid; j < KP; j += 256) sg[j] = 0.f;
-        __syncthreads();
+        // J^T r: each item's partial in its own slot, then every column summed over its items
+        // in item order (a fixed order, unlike LDS atomics)
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int it = tid + 256 * r;
-            if (NITEM % 256 == 0 || it < NITEM) atomicAdd(&sg[it % KP], gacc[r]);
-        }
+        for (int r = 0; r < R; ++r)
+            if (NITEM % 256 == 0 || tid + 256 * r < NITEM) sg[tid + 256 * r] = gacc[r];
         __syncthreads();
-        for (int j = tid; j < KP; j += 256) gp[((size_t)f * S + s) * KP + j] = sg[j];
-        __syncthreads();   // sg is cleared again for the next frame
+        for (int j = tid; j < KP; j += 256) {
+            float v = 0.f;
+            for (int it = j; it < NITEM; it += KP) v += sg[it];
+            gp[((size_t)f * S + s) * KP + j] = v;
+        }
+        __syncthreads();   // sg is rewritten for the next frame
     }
 }
 

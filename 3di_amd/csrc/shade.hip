@@ -19,6 +19,10 @@ struct Taps {
     float t[15];
 };
 
+// Workgroups per frame of the objective kernels: each writes its partial sum into its own
+// slot of a [frames][OBJ_PARTS] buffer, reduced in order by k_obj_reduce.
+constexpr int OBJ_PARTS = 256;
+
 namespace {
 
 // Region of a crop a pass has to cover, half-open [X0, X1) x [Y0, Y1).  Without a box (or
@@ -275,7 +279,7 @@ __global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ fram
                                                 const int* __restrict__ cum,
                                                 const float* __restrict__ blurred, const float2* __restrict__ obs_g,
                                                 float w, float4* __restrict__ pixg, int4* __restrict__ pnb,
-                                                float* __restrict__ pcoef, double* __restrict__ obj)
+                                                float* __restrict__ pcoef, double* __restrict__ objp)
 {
     __shared__ double sw[4];
     const int nact = fl_count(act, n);
@@ -304,10 +308,7 @@ __global__ void __launch_bounds__(256) k_grads(const FrameGPU* __restrict__ fram
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) sw[wave] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const double s = sw[0] + sw[1] + sw[2] + sw[3];
-        if (s != 0.0) atomicAdd(obj + f, -(double)w * s);
-    }
+    if (threadIdx.x == 0) objp[(size_t)f * OBJ_PARTS + blockIdx.x] = -(double)w * (sw[0] + sw[1] + sw[2] + sw[3]);
     __syncthreads();   // sw is rewritten by the next iteration
     }
 }
@@ -319,7 +320,7 @@ __global__ void __launch_bounds__(256) k_objective(const FrameGPU* __restrict__ 
                                                     int n, const unsigned long long* __restrict__ keys,
                                                     const int4* __restrict__ box, const float* __restrict__ blurred,
                                                     const float2* __restrict__ obs_g, float w,
-                                                    double* __restrict__ obj)
+                                                    double* __restrict__ objp)
 {
     __shared__ double sw[4];
     const int nact = fl_count(act, n);
@@ -328,8 +329,10 @@ __global__ void __launch_bounds__(256) k_objective(const FrameGPU* __restrict__ 
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
         Region r;
-        if (!crop_region(fr, box, f, 0, 0, r)) continue;
-        if (r.X0 + (int)blockIdx.x * nw >= r.X1) continue;
+        if (!crop_region(fr, box, f, 0, 0, r) || r.X0 + (int)blockIdx.x * nw >= r.X1) {
+            if (threadIdx.x == 0) objp[(size_t)f * OBJ_PARTS + blockIdx.x] = 0.0;
+            continue;
+        }
         const long long base = fr.pix_off;
         const float* I = blurred + base;
         double acc = 0.0;
@@ -345,12 +348,22 @@ __global__ void __launch_bounds__(256) k_objective(const FrameGPU* __restrict__ 
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
         if (lane == 0) sw[wave] = acc;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const double sum = sw[0] + sw[1] + sw[2] + sw[3];
-            if (sum != 0.0) atomicAdd(obj + f, -(double)w * sum);
-        }
+        if (threadIdx.x == 0) objp[(size_t)f * OBJ_PARTS + blockIdx.x] = -(double)w * (sw[0] + sw[1] + sw[2] + sw[3]);
         __syncthreads();   // sw is rewritten by the next iteration
     }
+}
+
+// obj[f] += the workgroup partials of the data objective, summed in workgroup order, so that
+// the objective of a frame does not depend on the order in which workgroups finish.
+__global__ void k_obj_reduce(const int* __restrict__ act, int n, int nparts, const double* __restrict__ objp,
+                             double* __restrict__ obj)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fl_count(act, n)) return;
+    const int f = fl_frame(act, i);
+    double s = 0.0;
+    for (int b = 0; b < nparts; ++b) s += objp[(size_t)f * OBJ_PARTS + b];
+    obj[f] += s;
 }
 
 __global__ void __launch_bounds__(256) k_obs_grad(const FrameGPU* __restrict__ frames, int n, const float* __restrict__ blurred,
@@ -387,7 +400,7 @@ inline dim3 col_grid(int n)
 inline dim3 list_grid(int max_area, int n)
 {
     int bx = (max_area / 2 + 255) / 256;
-    const int cap = n > 0 ? std::max(16, 8192 / n) : 256;
+    const int cap = n > 0 ? std::min(OBJ_PARTS, std::max(16, 8192 / n)) : OBJ_PARTS;
     if (bx > cap) bx = cap;
     if (bx < 1) bx = 1;
     return dim3(bx, grid_frames(n, 1 << 16));
@@ -433,10 +446,12 @@ int launch_shade_keys(const ModelGPU& m, const Layout& L, const FrameGPU* frames
 }
 
 int launch_objective(const FrameGPU* frames, FrameList fl, const uint64_t* keys, const int4* box,
-                     const float* blurred, const float2* obs_g, float w, double* obj, hipStream_t s)
+                     const float* blurred, const float2* obs_g, float w, double* objp, double* obj, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_objective, col_grid(fl.n), dim3(256), 0, s, frames, fl.act, fl.n,
-                       (const unsigned long long*)keys, box, blurred, obs_g, w, obj);
+    const dim3 grid = col_grid(fl.n);
+    hipLaunchKernelGGL(k_objective, grid, dim3(256), 0, s, frames, fl.act, fl.n,
+                       (const unsigned long long*)keys, box, blurred, obs_g, w, objp);
+    hipLaunchKernelGGL(k_obj_reduce, dim3((fl.n + 255) / 256), dim3(256), 0, s, fl.act, fl.n, (int)grid.x, objp, obj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -454,10 +469,12 @@ int launch_blur(const FrameGPU* frames, FrameList fl, const float* taps, const i
 
 int launch_grads(const FrameGPU* frames, FrameList fl, const int* vis_list, const int* npix,
                  const int* cum, const float* blurred, const float2* obs_g, float w, float4* pixg, int4* pnb,
-                 float* pcoef, double* obj, int max_area, hipStream_t s)
+                 float* pcoef, double* objp, double* obj, int max_area, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_grads, list_grid(max_area, fl.n), dim3(256), 0, s, frames, fl.act, fl.n, vis_list, npix, cum,
-                       blurred, obs_g, w, pixg, pnb, pcoef, obj);
+    const dim3 grid = list_grid(max_area, fl.n);
+    hipLaunchKernelGGL(k_grads, grid, dim3(256), 0, s, frames, fl.act, fl.n, vis_list, npix, cum,
+                       blurred, obs_g, w, pixg, pnb, pcoef, objp);
+    hipLaunchKernelGGL(k_obj_reduce, dim3((fl.n + 255) / 256), dim3(256), 0, s, fl.act, fl.n, (int)grid.x, objp, obj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

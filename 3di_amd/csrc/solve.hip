@@ -318,6 +318,7 @@ __global__ void k_ls_init(int n, int K, FitState* __restrict__ st, const int* __
     searching[f] = 0;
     if (!s.active) return;
     s.iters++;
+    s.fresh = 1;              // the full evaluation just rendered theta
     if (chol_status[f] != TDI_OK) {
         s.active = 0;
         s.stop |= TDI_STOP_NOT_PD;
@@ -404,6 +405,7 @@ __global__ void k_ls_decide(int n, int K, FitState* __restrict__ st, const int* 
             }
             if (s.inner >= o.max_ls) s.searching = 0;
         }
+        s.fresh = acc;        // the trial's render is in the workspace; it is theta's if accepted
         searching[f] = s.searching;
         st[f] = s;
         accept = acc;
@@ -433,6 +435,17 @@ __global__ void k_copy_mask(int n, const FitState* __restrict__ st, int* __restr
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f < n) mask[f] = st[f].active;
+}
+
+// Active frames whose workspace render is stale (mask_r) or that of their theta (mask_u).
+__global__ void k_split_fresh(int n, const FitState* __restrict__ st, int* __restrict__ mask_r,
+                              int* __restrict__ mask_u)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    const FitState s = st[f];
+    mask_r[f] = s.active && !s.fresh;
+    mask_u[f] = s.active && s.fresh;
 }
 
 // Stream compaction of a mask into a frame list, one 1024-thread workgroup: per 1024-entry
@@ -536,6 +549,13 @@ int launch_face_check(int V, const int* lmk_ids, const float4* vproj, float ref_
 int launch_copy_mask(int n, const FitState* st, int* mask, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy_mask, dim3((n + 255) / 256), dim3(256), 0, s, n, st, mask);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_split_fresh(int n, const FitState* st, int* mask_r, int* mask_u, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_split_fresh, dim3((n + 255) / 256), dim3(256), 0, s, n, st, mask_r, mask_u);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

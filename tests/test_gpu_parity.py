@@ -310,6 +310,22 @@ def test_fit_batched_vs_oracle(scene, gpu):
     assert np.all(obj_g < obj_o + 0.01 * np.abs(obj_o))
 
 
+def test_batched_fit_is_reproducible(scene, gpu):
+    """Two batched fits from the same start give bit-identical parameters and reports: every
+    floating-point sum on the path has a fixed order (per-workgroup partials reduced in
+    workgroup order, no floating-point atomics), so the discrete decisions of the fit (Armijo
+    accept/reject, stop tests) cannot flip between runs."""
+    _say("case: batched fit reproducibility")
+    th0 = _f32(scene.init()).astype(np.float32)
+    outs = []
+    for _ in range(2):
+        thg = torch.from_numpy(th0.copy()).cuda()
+        thg, rep = gpu.fit_batched(thg)
+        outs.append((thg.cpu().numpy(), [(r.iters, r.stop, r.ls_evals, r.objective) for r in rep]))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+
+
 @pytest.mark.parametrize("which", ["lambdas", "lintensity"])
 def test_light_fit_vs_oracle(scene, which):
     """fit_3DMM_lambdas / fit_3DMM_Lintensity on the GPU against the FP64 oracle from the same
