@@ -53,7 +53,8 @@ __device__ inline void load_frame_rot(const Layout& L, const FrameGPU& fr, const
 
 constexpr int DI_PX_PER_WG = 256;   // pixels per workgroup and frame in k_dI
 
-__global__ void __launch_bounds__(256) k_dI(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+// Four wavefronts per SIMD (register budget 128): more pixels' gathers in flight per CU.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_dI(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                              const int* __restrict__ act, int n, const float* __restrict__ theta,
                                              const float* __restrict__ P0, const int4* __restrict__ pvert,
                                              const float2* __restrict__ pbary, const int* __restrict__ npix,
@@ -522,7 +523,7 @@ __device__ __forceinline__ void jl_store(const v4f* acc, float* __restrict__ Hou
 }
 
 template <int T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T >= 7 ? 2 : 3))) k_jtj_lds(Layout L, const FrameGPU* __restrict__ frames,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T >= 7 ? 2 : 4))) k_jtj_lds(Layout L, const FrameGPU* __restrict__ frames,
                                                   const int* __restrict__ act, int n, const int* __restrict__ npix,
                                                   const int4* __restrict__ pnb, const float* __restrict__ pcoef,
                                                   const float4* __restrict__ pixg, const float* __restrict__ dI, int S,

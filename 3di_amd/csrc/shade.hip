@@ -82,15 +82,35 @@ __device__ __forceinline__ void shade_frame_setup(const Layout& L, const FrameGP
     sf.L = Li;
 }
 
+// Per-pixel inputs of the shading: the triangle's model-space corners and corner albedos.
+struct ShadeIn {
+    float b[3], c[3], a[3];
+    float rb, rc, ra;
+};
+
+__device__ __forceinline__ void shade_load(const ModelGPU& m, const float* __restrict__ P0f, int vb, int vc, int va,
+                                           ShadeIn& in)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        in.b[i] = P0f[3 * vb + i];
+        in.c[i] = P0f[3 * vc + i];
+        in.a[i] = P0f[3 * va + i];
+    }
+    in.rb = m.tex_mean[vb];
+    in.rc = m.tex_mean[vc];
+    in.ra = m.tex_mean[va];
+}
+
 // I = rho (1 + L Id) at barycentric (al, be) of the triangle (vb, vc, va).
-__device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L, const float* __restrict__ P0f,
-                                             const ShadeFrame& sf, const float* stex, int vb, int vc, int va,
-                                             float al, float be)
+__device__ __forceinline__ float shade_eval(const ModelGPU& m, const Layout& L, const ShadeFrame& sf,
+                                            const float* stex, int vb, int vc, int va, const ShadeIn& in,
+                                            float al, float be)
 {
     const float ga = gamma_rn(al, be);
-    const float* B0 = P0f + 3 * vb;
-    const float* C0 = P0f + 3 * vc;
-    const float* A0 = P0f + 3 * va;
+    const float* B0 = in.b;
+    const float* C0 = in.c;
+    const float* A0 = in.a;
     const float e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
     const float e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
     float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
@@ -104,7 +124,7 @@ __device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L,
     const float ly = sf.R[3] * sf.dl[0] + sf.R[4] * sf.dl[1] + sf.R[5] * sf.dl[2];
     const float lz = sf.R[6] * sf.dl[0] + sf.R[7] * sf.dl[1] + sf.R[8] * sf.dl[2];
     const float Id = nx * lx + ny * ly + nz * lz - (nx * px + ny * py + nz * pz);
-    float rb = m.tex_mean[vb], rc = m.tex_mean[vc], ra = m.tex_mean[va];
+    float rb = in.rb, rc = in.rc, ra = in.ra;
     for (int q = 0; q < L.ntex; ++q) {
         rb = fmaf(m.tex_basis[(size_t)vb * m.tex_ld + q], stex[q], rb);
         rc = fmaf(m.tex_basis[(size_t)vc * m.tex_ld + q], stex[q], rc);
@@ -112,6 +132,15 @@ __device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L,
     }
     const float rho = al * rb + be * rc + ga * ra;
     return rho + sf.L * rho * Id;
+}
+
+__device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L, const float* __restrict__ P0f,
+                                             const ShadeFrame& sf, const float* stex, int vb, int vc, int va,
+                                             float al, float be)
+{
+    ShadeIn in;
+    shade_load(m, P0f, vb, vc, va, in);
+    return shade_eval(m, L, sf, stex, vb, vc, va, in, al, be);
 }
 
 // Shading of the visible list (pvert/pbary from the resolve pass), scattered into the crop.
