@@ -259,6 +259,38 @@ def test_one_iteration_batched_vs_oracle(scene):
     ctx.close()
 
 
+def _chunked_batch(ctx, frames):
+    """A batch whose workspace budget holds one frame: the batched fit then runs chunk by
+    chunk (one graph replay per chunk, frames loaded into the workspace in turn)."""
+    import os
+    old = os.environ.get("TDI_WORKSPACE_GB")
+    os.environ["TDI_WORKSPACE_GB"] = "1e-6"
+    try:
+        b = tdi.Batch(ctx, frames)
+    finally:
+        if old is None:
+            del os.environ["TDI_WORKSPACE_GB"]
+        else:
+            os.environ["TDI_WORKSPACE_GB"] = old
+    assert b.workspace_frames() == 1 < len(frames)
+    return b
+
+
+def test_one_iteration_batched_chunked_vs_oracle(scene):
+    """The batched path with one frame per workspace chunk: same step as the oracle's."""
+    _say("case: one GN iteration, batched path in one-frame chunks vs oracle")
+    opts = _one_iteration_opts(scene)
+    th0 = _f32(scene.init())
+    orc = scene.oracle(opts)
+    tho, repo = orc.fit(th0)
+    ctx = tdi.Context(scene.model, opts)
+    b = _chunked_batch(ctx, scene.frames)
+    thg, repg = b.fit_batched(torch.from_numpy(th0.astype(np.float32)).cuda())
+    _check_one_iteration(scene, th0, tho, repo, thg.cpu().numpy().astype(np.float64), repg, "batched, chunked")
+    b.close()
+    ctx.close()
+
+
 def _oracle_floor(scene, th0):
     """The oracle's own spread: the largest pose/expression RMS between its fit from th0 and
     its fits from three starts 1e-6 away (random signs).  The rasterised objective is
@@ -308,6 +340,26 @@ def test_fit_batched_vs_oracle(scene, gpu):
     obj_g = np.array([r.objective for r in repg])
     obj_o = np.array([r.objective for r in repo])
     assert np.all(obj_g < obj_o + 0.01 * np.abs(obj_o))
+
+
+def test_fit_batched_chunked_vs_oracle(scene):
+    """Whole batched fit in one-frame workspace chunks, judged like the single-chunk fit."""
+    _say("case: batched fit in one-frame chunks vs oracle")
+    th0 = _f32(scene.init())
+    tho, repo, floor = _oracle_floor(scene, th0)
+    ctx = tdi.Context(scene.model, scene.opts)
+    b = _chunked_batch(ctx, scene.frames)
+    thg, repg = b.fit_batched(torch.from_numpy(th0.astype(np.float32)).cuda())
+    thg = thg.cpu().numpy().astype(np.float64)
+    rms = _pose_exp_rms(thg, tho, scene.lay)
+    err_o = _pose_exp_rms(tho, scene.truth(), scene.lay)
+    err_g = _pose_exp_rms(thg, scene.truth(), scene.lay)
+    _say(f"  chunked batched GPU vs FP64 oracle RMS {rms:.2e} (oracle self-spread {floor:.2e}); error vs truth "
+         f"gpu {err_g:.3f} oracle {err_o:.3f}")
+    assert rms <= max(3 * floor, 1e-3)
+    assert err_g < 1.5 * err_o + 0.05
+    b.close()
+    ctx.close()
 
 
 def test_batched_fit_is_reproducible(scene, gpu):
