@@ -128,12 +128,25 @@ k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const fl
     const int Ke = L.nexp, ke = m.exp_ld;
     const int K4 = (Ke + 3) & ~3;
     // basis rows of the workgroup's vertices, zero padded (coordinate 3, k >= Ke, v >= V)
-    for (int e = tid; e < SY_RB * K4 * 16; e += blockDim.x) {
+    // (a fixed count of loads per thread, issued unconditionally from clamped indices and zeroed
+    // by a select, so that they are in flight together)
+    constexpr int NST = SY_RB * SY_KMAX * 16 / 256;
+    float av[NST];
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int e = tid + 256 * q;
         const int k = e % K4, i = (e / K4) & 15, rb = (e / K4) >> 4;
         const int c = i >> 2, v = v0 + 4 * rb + (i & 3);
-        double a = 0.0;
-        if (c < 3 && k < Ke && v < m.V) a = (double)m.exp_basis[(size_t)(3 * v + c) * ke + k];
-        sA[rb][k][i] = a;
+        av[q] = m.exp_basis[(size_t)(3 * min(v, m.V - 1) + min(c, 2)) * ke + min(k, Ke - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int e = tid + 256 * q;
+        if (e < SY_RB * K4 * 16) {
+            const int k = e % K4, i = (e / K4) & 15, rb = (e / K4) >> 4;
+            const int c = i >> 2, v = v0 + 4 * rb + (i & 3);
+            sA[rb][k][i] = (c < 3 && k < Ke && v < m.V) ? (double)av[q] : 0.0;
+        }
     }
     for (int e = tid; e < SY_VERTS * 3; e += blockDim.x) {
         const int v = v0 + e / 3;

@@ -49,17 +49,46 @@ __device__ __forceinline__ bool crop_region(const FrameGPU& fr, const int4* box,
     return true;
 }
 
+// Column kernels run on a 1-D grid of nb workgroups.  The workgroups per frame follow the
+// length of the frame list, read on the device: 8 per frame when the list is long, up to 128
+// (one column per wavefront) when it is short, so that the last line-search rounds, whose
+// lists hold few frames, spread each frame over more wavefronts.  Slots of fpb consecutive
+// workgroups walk the list.
+constexpr int COL_MIN = 8, COL_MAX = 128;
+
+struct ColMap {
+    int fpb, nslots, slot, sub;
+};
+
+__device__ __forceinline__ int col_fpb(int nb, int nact)
+{
+    const int fpb = nact > 0 ? nb / nact : COL_MAX;
+    return fpb < COL_MIN ? COL_MIN : (fpb > COL_MAX ? COL_MAX : fpb);
+}
+
+__device__ __forceinline__ ColMap col_map(int nact)
+{
+    ColMap c;
+    c.fpb = col_fpb(gridDim.x, nact);
+    c.nslots = gridDim.x / c.fpb;
+    c.slot = blockIdx.x / c.fpb;
+    c.sub = blockIdx.x - c.slot * c.fpb;
+    return c;
+}
+
 __global__ void __launch_bounds__(256) k_clear_crops(const FrameGPU* __restrict__ frames, const int* __restrict__ act,
                                                       int n, const int4* __restrict__ box, float* __restrict__ img)
 {
     const int nact = fl_count(act, n);
-    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const ColMap cm = col_map(nact);
+    if (cm.slot >= cm.nslots) return;
+    for (int yi = cm.slot; yi < nact; yi += cm.nslots) {
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
         Region r;
         if (!crop_region(fr, box, f, 8, 8, r)) continue;
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-        for (int x = r.X0 + blockIdx.x * nw + wave; x < r.X1; x += gridDim.x * nw)
+        for (int x = r.X0 + cm.sub * nw + wave; x < r.X1; x += cm.fpb * nw)
             for (int y = r.Y0 + lane; y < r.Y1; y += 64) img[fr.pix_off + (long long)x * fr.H + y] = 0.f;
     }
 }
@@ -187,12 +216,14 @@ __global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const 
     __shared__ float stex[512];
     const int nact = fl_count(act, n);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const ColMap cm = col_map(nact);
+    if (cm.slot >= cm.nslots) return;
+    for (int yi = cm.slot; yi < nact; yi += cm.nslots) {
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
         Region r;
         if (!crop_region(fr, box, f, 8, 8, r)) continue;
-        if (r.X0 + (int)blockIdx.x * nw >= r.X1) continue;
+        if (r.X0 + (int)cm.sub * nw >= r.X1) continue;
         const float* tf = theta + (size_t)f * L.K;
         if (threadIdx.x == 0) shade_frame_setup(L, fr, tf, sf);
         for (int k = threadIdx.x; k < L.ntex; k += blockDim.x) stex[k] = tf[L.off_tex + k];
@@ -200,7 +231,7 @@ __global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const 
         const long long base = fr.pix_off;
         const float* P0f = P0 + (size_t)f * m.V * 3;
         const float4* vp = vproj + (size_t)f * m.V;
-        for (int x = r.X0 + blockIdx.x * nw + wave; x < r.X1; x += gridDim.x * nw)
+        for (int x = r.X0 + cm.sub * nw + wave; x < r.X1; x += cm.fpb * nw)
             for (int y = r.Y0 + lane; y < r.Y1; y += 64) {
                 const int k = x * fr.H + y;
                 const unsigned long long key = keys[base + k];
@@ -244,7 +275,9 @@ __global__ void __launch_bounds__(256) k_blur_tiles(const FrameGPU* __restrict__
     __shared__ float stmp[BT_IX][BT_Y + 1];
     const int nact = fl_count(act, n);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const ColMap cm = col_map(nact);
+    if (cm.slot >= cm.nslots) return;
+    for (int yi = cm.slot; yi < nact; yi += cm.nslots) {
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
         Region rg;
@@ -253,7 +286,7 @@ __global__ void __launch_bounds__(256) k_blur_tiles(const FrameGPU* __restrict__
         const int ntx = (rg.X1 - rg.X0 + BT_X - 1) / BT_X, nty = (rg.Y1 - rg.Y0 + BT_Y - 1) / BT_Y;
         const float* src = in + fr.pix_off;
         float* dst = out + fr.pix_off;
-        for (int t = blockIdx.x; t < ntx * nty; t += gridDim.x) {
+        for (int t = cm.sub; t < ntx * nty; t += cm.fpb) {
             const int x0 = rg.X0 + (t % ntx) * BT_X, y0 = rg.Y0 + (t / ntx) * BT_Y;
             // input tile: columns x0-7 .. x0+BT_X+6, rows y0-7 .. y0+BT_Y+6, zero outside the crop
             for (int c = wave; c < BT_IX; c += 4) {
@@ -354,18 +387,20 @@ __global__ void __launch_bounds__(256) k_objective(const FrameGPU* __restrict__ 
     __shared__ double sw[4];
     const int nact = fl_count(act, n);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+    const ColMap cm = col_map(nact);
+    if (cm.slot >= cm.nslots) return;
+    for (int yi = cm.slot; yi < nact; yi += cm.nslots) {
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
         Region r;
-        if (!crop_region(fr, box, f, 0, 0, r) || r.X0 + (int)blockIdx.x * nw >= r.X1) {
-            if (threadIdx.x == 0) objp[(size_t)f * OBJ_PARTS + blockIdx.x] = 0.0;
+        if (!crop_region(fr, box, f, 0, 0, r) || r.X0 + (int)cm.sub * nw >= r.X1) {
+            if (threadIdx.x == 0) objp[(size_t)f * OBJ_PARTS + cm.sub] = 0.0;
             continue;
         }
         const long long base = fr.pix_off;
         const float* I = blurred + base;
         double acc = 0.0;
-        for (int x = r.X0 + blockIdx.x * nw + wave; x < r.X1; x += gridDim.x * nw)
+        for (int x = r.X0 + cm.sub * nw + wave; x < r.X1; x += cm.fpb * nw)
             for (int y = r.Y0 + lane; y < r.Y1; y += 64) {
                 const int k = x * fr.H + y;
                 if (keys[base + k] == KEY_EMPTY) continue;
@@ -377,21 +412,25 @@ __global__ void __launch_bounds__(256) k_objective(const FrameGPU* __restrict__ 
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
         if (lane == 0) sw[wave] = acc;
         __syncthreads();
-        if (threadIdx.x == 0) objp[(size_t)f * OBJ_PARTS + blockIdx.x] = -(double)w * (sw[0] + sw[1] + sw[2] + sw[3]);
+        if (threadIdx.x == 0) objp[(size_t)f * OBJ_PARTS + cm.sub] = -(double)w * (sw[0] + sw[1] + sw[2] + sw[3]);
         __syncthreads();   // sw is rewritten by the next iteration
     }
 }
 
 // obj[f] += the workgroup partials of the data objective, summed in workgroup order, so that
 // the objective of a frame does not depend on the order in which workgroups finish.
-__global__ void k_obj_reduce(const int* __restrict__ act, int n, int nparts, const double* __restrict__ objp,
-                             double* __restrict__ obj)
+// nparts: partials per frame; col_nb > 0: a column kernel of col_nb workgroups wrote them (as
+// many per frame as col_map gave it).
+__global__ void k_obj_reduce(const int* __restrict__ act, int n, int nparts, int col_nb,
+                             const double* __restrict__ objp, double* __restrict__ obj)
 {
+    const int nact = fl_count(act, n);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= fl_count(act, n)) return;
+    if (i >= nact) return;
     const int f = fl_frame(act, i);
+    const int np = col_nb > 0 ? col_fpb(col_nb, nact) : nparts;
     double s = 0.0;
-    for (int b = 0; b < nparts; ++b) s += objp[(size_t)f * OBJ_PARTS + b];
+    for (int b = 0; b < np; ++b) s += objp[(size_t)f * OBJ_PARTS + b];
     obj[f] += s;
 }
 
@@ -413,14 +452,14 @@ __global__ void __launch_bounds__(256) k_obs_grad(const FrameGPU* __restrict__ f
     }
 }
 
-// Column kernels: workgroups of four wavefronts, one column per wavefront at a time; one
-// frame per grid row, 8 workgroups per frame for large batches and up to 128 (one column per
-// wavefront) when there are few frames.
+// Column kernels: workgroups of four wavefronts, one column per wavefront at a time; 8
+// workgroups per slot of the list capacity for large batches, up to 128 for few frames (the
+// per-frame split is then chosen on the device, see col_map).
 inline dim3 col_grid(int n)
 {
-    int bx = n > 0 ? 4096 / n : 128;
-    bx = bx < 8 ? 8 : (bx > 128 ? 128 : bx);
-    return dim3(bx, grid_frames(n, 1 << 16));
+    int bx = n > 0 ? 4096 / n : COL_MAX;
+    bx = bx < COL_MIN ? COL_MIN : (bx > COL_MAX ? COL_MAX : bx);
+    return dim3(bx * (n > 0 ? n : 1));
 }
 
 // Visible-list kernels: workgroups of 256 threads over a frame's visible list (the ones past
@@ -480,7 +519,7 @@ int launch_objective(const FrameGPU* frames, FrameList fl, const uint64_t* keys,
     const dim3 grid = col_grid(fl.n);
     hipLaunchKernelGGL(k_objective, grid, dim3(256), 0, s, frames, fl.act, fl.n,
                        (const unsigned long long*)keys, box, blurred, obs_g, w, objp);
-    hipLaunchKernelGGL(k_obj_reduce, dim3((fl.n + 255) / 256), dim3(256), 0, s, fl.act, fl.n, (int)grid.x, objp, obj);
+    hipLaunchKernelGGL(k_obj_reduce, dim3((fl.n + 255) / 256), dim3(256), 0, s, fl.act, fl.n, 0, (int)grid.x, objp, obj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
@@ -503,7 +542,7 @@ int launch_grads(const FrameGPU* frames, FrameList fl, const int* vis_list, cons
     const dim3 grid = list_grid(max_area, fl.n);
     hipLaunchKernelGGL(k_grads, grid, dim3(256), 0, s, frames, fl.act, fl.n, vis_list, npix, cum,
                        blurred, obs_g, w, pixg, pnb, pcoef, objp);
-    hipLaunchKernelGGL(k_obj_reduce, dim3((fl.n + 255) / 256), dim3(256), 0, s, fl.act, fl.n, (int)grid.x, objp, obj);
+    hipLaunchKernelGGL(k_obj_reduce, dim3((fl.n + 255) / 256), dim3(256), 0, s, fl.act, fl.n, (int)grid.x, 0, objp, obj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
