@@ -8,9 +8,10 @@
 // and DerivativeComputer sorts/scans it (derivative_computer.cu:43-72).
 //
 // Here:
-//   k_raster  — thread per triangle, bounding box clipped to the frame's crop, exact
-//               reference coverage arithmetic (cover_rn, no FMA contraction), and a single
-//               64-bit atomicMin of (orderable depth << 32 | triangle id) per covered pixel.
+//   k_raster  — thread per triangle, bounding box clipped to the frame's crop, the reference's
+//               coverage decision (cover_rn's FP32 sequence, no FMA contraction; the divisions
+//               only for samples next to an edge, raster_frame), and a single 64-bit
+//               atomicMin of (orderable depth << 32 | triangle id) per covered pixel.
 //               The winner per pixel is therefore unique and deterministic: minimum depth,
 //               then lowest triangle id (the reference keeps every exact-depth tie and lets
 //               them race in the scatter, funcs.cu:816,1027).
@@ -33,6 +34,70 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, long long n, uint64_t v)
         p[i] = v;
 }
 
+// Coverage of the sample points of one triangle in one frame.  The decision is the one
+// cover_rn makes: alpha > 0, beta > 0 (the sign pre-test, exact) and RN(RN(na/den) +
+// RN(nb/den)) <= 1.  The sum is first formed from na * rcp(den) and nb * rcp(den): v_rcp_f32
+// is within 1 ulp, so that sum is within 1.5 * 2^-22 * (alpha + beta) of the correctly rounded
+// one, and only samples whose approximate sum lies within 2^-18 of 1 (or with den outside
+// [2^-60, 2^60]) take the two IEEE divisions.  The samples are scanned x-major in one loop, so
+// a wavefront runs max(nx * ny) trips rather than nested maxima.
+__device__ __forceinline__ void raster_frame(const FrameGPU& fr, float4 B, float4 Cc, float4 A, int t, int ntmp,
+                                             unsigned long long* __restrict__ keys)
+{
+#pragma clang fp contract(off)
+    const float xmin = floorf(fminf(fminf(B.x, Cc.x), A.x));
+    const float xmax = ceilf(fmaxf(fmaxf(B.x, Cc.x), A.x));
+    const float ymin = floorf(fminf(fminf(B.y, Cc.y), A.y));
+    const float ymax = ceilf(fmaxf(fmaxf(B.y, Cc.y), A.y));
+    if (!(xmin == xmin) || !(xmax == xmax) || !(ymin == ymin) || !(ymax == ymax)) return;
+    // clip the scan to the crop window [x0, x0+W) x [y0, y0+H); the clipped bounds are integers
+    const float xlo = fmaxf(xmin, (float)fr.x0), xhi = fminf(xmax, (float)(fr.x0 + fr.W));
+    const float ylo = fmaxf(ymin, (float)fr.y0), yhi = fminf(ymax, (float)(fr.y0 + fr.H));
+    if (!(xlo < xhi) || !(ylo < yhi)) return;
+    const int xl = (int)xlo, yl = (int)ylo, ny = (int)yhi - yl;
+    const int nxy = ((int)xhi - xl) * ny;
+    const unsigned long long key = ((unsigned long long)orderable_bits(B.z) << 32) | (unsigned)t;
+    unsigned long long* kf = keys + fr.pix_off;
+    const float v0x = B.x - A.x, v1x = Cc.x - A.x;
+    const float v0y = B.y - A.y, v1y = Cc.y - A.y;
+    const float d00 = v0x * v0x + v0y * v0y;
+    const float d01 = v0x * v1x + v0y * v1y;
+    const float d11 = v1x * v1x + v1y * v1y;
+    const float den = d00 * d11 - d01 * d01;
+    const float q = __builtin_amdgcn_rcpf(den);
+    const bool den_ok = fabsf(den) >= 0x1p-60f && fabsf(den) <= 0x1p60f;
+    const bool dpos = den > 0.f, dneg = den < 0.f;
+    int covered = 0;
+    int x = xl, y = yl;
+    int off = (xl - fr.x0) * fr.H + (yl - fr.y0);
+    for (int i = 0; i < nxy; ++i) {
+        const float v2x = (float)x - A.x, v2y = (float)y - A.y;
+        const float d20 = v0x * v2x + v0y * v2y;
+        const float d21 = v1x * v2x + v1y * v2y;
+        const float na = d11 * d20 - d01 * d21;
+        const float nb = d00 * d21 - d01 * d20;
+        const bool pa = (na > 0.f && dpos) || (na < 0.f && dneg);
+        const bool pb = (nb > 0.f && dpos) || (nb < 0.f && dneg);
+        const float s1 = na * q + nb * q;
+        bool cov = pa && pb && s1 < 1.f;
+        if (pa && pb && !(den_ok && fabsf(s1 - 1.f) > 0x1p-18f)) {
+            const float a = na / den, b = nb / den;
+            cov = a > 0.f && b > 0.f && (a + b <= 1.f);
+        }
+        if (cov && (ntmp <= 0 || covered < ntmp)) {
+            ++covered;
+            atomicMin(kf + off, key);
+        }
+        ++y;
+        ++off;
+        if (y - yl == ny) {
+            y = yl;
+            ++x;
+            off += fr.H - ny;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_raster(const int* __restrict__ tris, int NT, int V,
                                                  const FrameGPU* __restrict__ frames, const int* __restrict__ act,
                                                  int n, const float4* __restrict__ vproj, int ntmp,
@@ -46,28 +111,7 @@ __global__ void __launch_bounds__(256) k_raster(const int* __restrict__ tris, in
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
         const float4* vp = vproj + (size_t)f * V;
-        const float4 B = vp[ib], Cc = vp[ic], A = vp[ia];
-        const float xmin = floorf(fminf(fminf(B.x, Cc.x), A.x));
-        const float xmax = ceilf(fmaxf(fmaxf(B.x, Cc.x), A.x));
-        const float ymin = floorf(fminf(fminf(B.y, Cc.y), A.y));
-        const float ymax = ceilf(fmaxf(fmaxf(B.y, Cc.y), A.y));
-        if (!(xmin == xmin) || !(xmax == xmax) || !(ymin == ymin) || !(ymax == ymax)) continue;
-        // clip the scan to the crop window [x0, x0+W) x [y0, y0+H)
-        const float xlo = fmaxf(xmin, (float)fr.x0), xhi = fminf(xmax, (float)(fr.x0 + fr.W));
-        const float ylo = fmaxf(ymin, (float)fr.y0), yhi = fminf(ymax, (float)(fr.y0 + fr.H));
-        if (!(xlo < xhi) || !(ylo < yhi)) continue;
-        const unsigned long long key = ((unsigned long long)orderable_bits(B.z) << 32) | (unsigned)t;
-        unsigned long long* kf = keys + fr.pix_off;
-        int covered = 0;
-        for (int x = (int)xlo; (float)x < xhi; ++x) {
-            for (int y = (int)ylo; (float)y < yhi; ++y) {
-                float al, be;
-                if (!cover_rn(B.x, B.y, Cc.x, Cc.y, A.x, A.y, x, y, &al, &be)) continue;
-                if (ntmp > 0 && covered >= ntmp) continue;
-                ++covered;
-                atomicMin(kf + (size_t)(x - fr.x0) * fr.H + (y - fr.y0), key);
-            }
-        }
+        raster_frame(fr, vp[ib], vp[ic], vp[ia], t, ntmp, keys);
     }
 }
 
