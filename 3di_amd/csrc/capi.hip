@@ -339,7 +339,8 @@ static int ws_load(tdi_batch* b, int i0, int n, hipStream_t s)
     w.n_cur = n;
     TDI_HIP(hipMemcpyAsync(w.frames, w.hframes.data(), sizeof(FrameGPU) * w.C, hipMemcpyHostToDevice, s));
     TDI_HIP(hipMemcpyAsync(w.lmk, b->lmk + (size_t)i0 * 6 * NLMK, sizeof(float) * 6 * NLMK * n, hipMemcpyDeviceToDevice, s));
-    return TDI_OK;
+    // the keys left by the previous chunk lie anywhere in the new crops
+    return launch_box_full(w.frames, w.C, w.box, s);
 }
 
 static LmkArrays lmk_arrays(const Workspace& w)
@@ -500,7 +501,7 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
     if (rc) return rc;
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
     if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
-    if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+    if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, w.box, s))) return rc;
     if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
         return rc;
     if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
@@ -532,7 +533,7 @@ static int eval_objective(tdi_batch* b, FrameList fl, const float* theta, int mi
     if (rc) return rc;
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
     if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
-    if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+    if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, w.box, s))) return rc;
     if ((rc = launch_visible_box(w.frames, fl, w.keys, w.max_area, w.tcount, w.tmax, w.box, s))) return rc;
     if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
@@ -651,8 +652,8 @@ int tdi_rasterize(tdi_batch* b, const float* xyz, const float* xyp, uint64_t* ke
         if ((rc = launch_misc_pack(n, V, xyz + (size_t)i0 * V * 3, xyp + (size_t)i0 * V * 2, w.vproj, s))) return rc;
         if ((rc = launch_clear_keys(w.keys, len, s))) return rc;
         if ((rc = launch_misc_fill_f32(w.bary, 2 * len, 0.f, s))) return rc;
-        if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
-        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, nullptr, s)))
+        if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, w.box, s))) return rc;
+        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
             return rc;
         if (keys) TDI_HIP(hipMemcpyAsync(keys + off, w.keys, sizeof(uint64_t) * len, hipMemcpyDeviceToDevice, s));
         if (bary) TDI_HIP(hipMemcpyAsync(bary + 2 * off, w.bary, sizeof(float) * 2 * len, hipMemcpyDeviceToDevice, s));
@@ -738,7 +739,7 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
         if (rc) return rc;
         TDI_HIP(hipEventRecord(ev[1], s));
         if ((rc = launch_clear_keys_frames(w.frames, all_frames(n), w.keys, w.box, w.max_area, s))) return rc;
-        if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, s))) return rc;
+        if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, w.box, s))) return rc;
         if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[2], s));

@@ -258,7 +258,8 @@ int launch_clear_keys(uint64_t* keys, long long total, hipStream_t s);
 int launch_clear_keys_frames(const FrameGPU* frames, FrameList fl, uint64_t* keys, int4* box, int max_area,
                              hipStream_t s);
 int launch_raster(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj, int ntmp,
-                  uint64_t* keys, hipStream_t s);
+                  uint64_t* keys, int4* box, hipStream_t s);
+int launch_box_full(const FrameGPU* frames, int n, int4* box, hipStream_t s);
 // tcount: [slots][tmax] tile counts, tmax >= resolve_tiles(largest crop)
 int resolve_tiles(int max_area);
 // pvert/pbary: per visible position (vb, vc, va, crop index k) and (alpha, beta), so that the

@@ -101,7 +101,7 @@ __device__ __forceinline__ void raster_frame(const FrameGPU& fr, float4 B, float
 __global__ void __launch_bounds__(256) k_raster(const int* __restrict__ tris, int NT, int V,
                                                  const FrameGPU* __restrict__ frames, const int* __restrict__ act,
                                                  int n, const float4* __restrict__ vproj, int ntmp,
-                                                 unsigned long long* __restrict__ keys)
+                                                 unsigned long long* __restrict__ keys, int4* __restrict__ box)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= NT) return;
@@ -110,6 +110,8 @@ __global__ void __launch_bounds__(256) k_raster(const int* __restrict__ tris, in
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
+        // empty box for the visible-box or resolve pass that follows (it grows it with atomics)
+        if (box && t == 0) box[f] = make_int4(fr.W, -1, fr.H, -1);
         const float4* vp = vproj + (size_t)f * V;
         raster_frame(fr, vp[ib], vp[ic], vp[ia], t, ntmp, keys);
     }
@@ -285,27 +287,49 @@ int launch_clear_keys(uint64_t* keys, long long total, hipStream_t s)
 }
 
 namespace {
+// Clears the keys a frame's previous raster can have set.  Every raster is followed by the
+// visible-box pass (or resolve), so the keys outside box[f] are already empty and only the
+// box is cleared; batch chunk loads set box[f] to the whole crop.  Without a box the whole
+// crop is cleared.  The box itself is reset by the next k_raster, after this clear has read it.
 __global__ void __launch_bounds__(256) k_clear_keys_frames(const FrameGPU* __restrict__ frames,
                                                             const int* __restrict__ act, int n,
-                                                            uint64_t* __restrict__ keys, int4* __restrict__ box)
+                                                            uint64_t* __restrict__ keys, const int4* __restrict__ box)
 {
     const int nact = fl_count(act, n);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
-        if (box && blockIdx.x == 0 && threadIdx.x == 0) box[f] = make_int4(fr.W, -1, fr.H, -1);
-        const int A = fr.W * fr.H;
-        for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x)
-            keys[fr.pix_off + k] = KEY_EMPTY;
+        int4 bx = box ? box[f] : make_int4(0, fr.W - 1, 0, fr.H - 1);
+        bx.x = max(bx.x, 0); bx.y = min(bx.y, fr.W - 1);
+        bx.z = max(bx.z, 0); bx.w = min(bx.w, fr.H - 1);
+        uint64_t* kf = keys + fr.pix_off;
+        for (int x = bx.x + blockIdx.x * nw + wave; x <= bx.y; x += gridDim.x * nw)
+            for (int y = bx.z + lane; y <= bx.w; y += 64) kf[(long long)x * fr.H + y] = KEY_EMPTY;
     }
 }
+
+// box[f] = the whole crop, for every slot of a freshly loaded chunk (their keys are stale).
+__global__ void k_box_full(const FrameGPU* __restrict__ frames, int n, int4* __restrict__ box)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < n) box[f] = make_int4(0, frames[f].W - 1, 0, frames[f].H - 1);
+}
 } // namespace
+
+int launch_box_full(const FrameGPU* frames, int n, int4* box, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_box_full, dim3((n + 255) / 256), dim3(256), 0, s, frames, n, box);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
 
 int launch_clear_keys_frames(const FrameGPU* frames, FrameList fl, uint64_t* keys, int4* box, int max_area,
                              hipStream_t s)
 {
-    int bx = (max_area + 255) / 256;
-    if (bx > 256) bx = 256;
+    // a workgroup clears four crop columns at a time
+    int bx = (int)(sqrtf((float)max_area) + 3) / 4;
+    if (bx > 64) bx = 64;
     if (bx < 1) bx = 1;
     hipLaunchKernelGGL(k_clear_keys_frames, dim3(bx, grid_frames(fl.n, 64)), dim3(256), 0, s, frames, fl.act, fl.n,
                        keys, box);
@@ -314,11 +338,11 @@ int launch_clear_keys_frames(const FrameGPU* frames, FrameList fl, uint64_t* key
 }
 
 int launch_raster(const ModelGPU& m, const FrameGPU* frames, FrameList fl, const float4* vproj, int ntmp,
-                  uint64_t* keys, hipStream_t s)
+                  uint64_t* keys, int4* box, hipStream_t s)
 {
     dim3 grid((m.NT + 255) / 256, grid_frames(fl.n, 64));
     hipLaunchKernelGGL(k_raster, grid, dim3(256), 0, s, m.tris, m.NT, m.V, frames, fl.act, fl.n, vproj, ntmp,
-                       (unsigned long long*)keys);
+                       (unsigned long long*)keys, box);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
