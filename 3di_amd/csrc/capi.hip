@@ -343,6 +343,12 @@ static int ws_load(tdi_batch* b, int i0, int n, hipStream_t s)
     return launch_box_full(w.frames, w.C, w.box, s);
 }
 
+// Mean crop area of the workspace's chunk (its crop pixels over its frame slots).
+static int mean_crop(const Workspace& w)
+{
+    return (int)(w.cap / (w.C > 0 ? w.C : 1));
+}
+
 static LmkArrays lmk_arrays(const Workspace& w)
 {
     LmkArrays a;
@@ -583,7 +589,7 @@ static int eval_jacobian(tdi_batch* b, FrameList fl, const float* theta, int mix
     Workspace& w = b->ws;
     int rc;
     if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
-                        w.max_area, s)))
+                        mean_crop(w), s)))
         return rc;
     if ((rc = launch_normal_eq(b, fl, mixed, s, &L))) return rc;
     return launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj, w.objb,
@@ -756,7 +762,7 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
             return rc;
         TDI_HIP(hipEventRecord(ev[4], s));
         if ((rc = launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
-                            w.max_area, s)))
+                            mean_crop(w), s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[5], s));
         if ((rc = launch_normal_eq(b, all_frames(n), mixed, s))) return rc;

@@ -295,7 +295,7 @@ int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const f
 // jacobian.hip
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
               const float* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
-              float* dI, int max_area, hipStream_t s);
+              float* dI, int mean_area, hipStream_t s);
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);
 int launch_jtj_f32(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,

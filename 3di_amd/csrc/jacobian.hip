@@ -679,15 +679,16 @@ int jtj_splits(int n, int per_frame_pixels_hint)
 
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
               const float* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
-              float* dI, int max_area, hipStream_t s)
+              float* dI, int mean_area, hipStream_t s)
 {
     if (L.ntex > 512) {
         set_error("dI: k_tex %d > 512", L.ntex);
         return TDI_ERR_ARG;
     }
-    // 8 strips per frame, each shared by up to nsb workgroups (sized for a half-visible crop);
-    // one frame per grid row
-    int nsb = (max_area / 2 + 8 * DI_PX_PER_WG - 1) / (8 * DI_PX_PER_WG);
+    // 8 strips per frame, each shared by up to nsb workgroups; one frame per grid row
+    // (sized for a quarter-visible mean crop: larger frames loop, and few workgroups of a grid
+    // row find no pixels)
+    int nsb = (mean_area / 4 + 8 * DI_PX_PER_WG - 1) / (8 * DI_PX_PER_WG);
     if (nsb > 64) nsb = 64;
     if (nsb < 1) nsb = 1;
     hipLaunchKernelGGL(k_dI, dim3(8 * nsb, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0,
