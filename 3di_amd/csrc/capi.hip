@@ -720,6 +720,18 @@ int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, flo
     return launch_cholesky(n, K, H, g, delta, status, mode, nullptr, (hipStream_t)stream);
 }
 
+int tdi_ortho_init(tdi_ctx* c, int32_t n, const float* lmk, const float* cams, float* x_orth, float* pose,
+                   float* angles, int32_t* info, void* stream)
+{
+    if (!c || n < 0 || (n > 0 && (!lmk || !cams || !x_orth || !pose || !info))) {
+        set_error("tdi_ortho_init: null argument or negative frame count");
+        return TDI_ERR_ARG;
+    }
+    if (!c->m.lmk_ids) { set_error("tdi_ortho_init: the model has no landmark ids"); return TDI_ERR_ARG; }
+    (void)hipSetDevice(c->device);
+    return launch_ortho_init(c->m, n, lmk, cams, x_orth, pose, angles, info, (hipStream_t)stream);
+}
+
 // Stage timers: one full evaluation (the work of one GN iteration before the solve) with
 // HIP events around each stage, on the batch's stream.  Mirrors the reference's compile-time
 // MEASURE_TIME timers (model_fitter.cu:43-58, derivative_computer.cu:32-37,141-172).

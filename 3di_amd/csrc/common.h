@@ -293,6 +293,10 @@ int launch_obs_prepare(const FrameGPU* frames, int n, const float* taps, const f
                        float2* obs_g, int max_area, hipStream_t s);
 
 // jacobian.hip
+// init.hip
+int launch_ortho_init(const ModelGPU& m, int n, const float* lmk, const float* cams, float* x_orth, float* pose,
+                      float* angles, int* info, hipStream_t s);
+
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
               const float* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
               float* dI, int mean_area, hipStream_t s);

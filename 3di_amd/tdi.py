@@ -43,6 +43,7 @@ def lib():
         L.tdi_batch_create.argtypes = [_vp, C.c_int32, _vp, C.POINTER(_vp)]
         L.tdi_batch_destroy.argtypes = [_vp]
         L.tdi_rotation.argtypes = [C.c_int32, _vp, _vp, _vp, _vp]
+        L.tdi_ortho_init.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_synthesize.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.tdi_rasterize.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_render.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
@@ -254,6 +255,31 @@ def cholesky_solve(H, g, mode: int = 0):
     _check(lib().tdi_cholesky_solve(n, K, _dptr(H.contiguous()), _dptr(g.contiguous()), _dptr(delta), _dptr(status),
                                     mode, _stream()))
     return delta, status
+
+
+NLANDMARKS = 51   # TDI_NLANDMARKS
+
+
+def ortho_init(ctx: Context, lmk, cams):
+    """Orthographic landmark initialisation and pose seeding of n frames on the GPU
+    (tdi_ortho_init).  lmk: [n, 2, 51] image-pixel landmarks (x row, y row); cams: [n, 4]
+    (phix, phiy, cx, cy).  Returns device tensors x_orth [n, 7], pose [n, 6] (tau, u),
+    angles [n, 3] (yaw, pitch, roll, degrees) and info [n, 4] (iterations, refit iterations,
+    refit from the mirrored pose, status)."""
+    torch = _torch()
+    lmk = lmk.float().contiguous()
+    cams = cams.float().contiguous()
+    n = lmk.shape[0]
+    if tuple(lmk.shape[1:]) != (2, NLANDMARKS) or tuple(cams.shape) != (n, 4):
+        raise ValueError("ortho_init: lmk must be [n, 2, 51] and cams [n, 4]")
+    dev = dict(device="cuda", dtype=torch.float32)
+    x_orth = torch.empty((n, 7), **dev)
+    pose = torch.empty((n, 6), **dev)
+    angles = torch.empty((n, 3), **dev)
+    info = torch.empty((n, 4), device="cuda", dtype=torch.int32)
+    _check(lib().tdi_ortho_init(ctx.h, n, _dptr(lmk), _dptr(cams), _dptr(x_orth), _dptr(pose), _dptr(angles),
+                                _dptr(info), _stream()))
+    return dict(x_orth=x_orth, pose=pose, angles=angles, info=info)
 
 
 def rotation(u):

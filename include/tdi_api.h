@@ -189,6 +189,21 @@ int tdi_normal_equations_mixed(tdi_batch* b, const float* theta, float* H, float
 int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, float* delta,
                        int32_t* status, int32_t mode, void* stream);
 
+/* Landmark-driven initialisation of n frames, one wavefront per frame in FP64 —
+ * OrthographicInitializer::fit_model (orthographic_initializer.cu:243-377): a 7-parameter
+ * Gauss-Newton fit of x_orth = (tau_x, tau_y, sigma_x, sigma_y, u) to the mean-centred
+ * landmarks against the model's mean shape at its 51 landmark vertices; then, as in
+ * Logbarrier_Initializer::initialize_with_orthographic_t (logbarrier_initializer.cu:1862-1914),
+ * a refit from the mirrored pose when sigma_x < 0 and the perspective pose of initialize_kernel
+ * (logbarrier_initializer.cu:31-46).  Device pointers: lmk [n][2][51] (x row, then y row, image
+ * pixels); cams [n][4] (phix, phiy, cx, cy); outputs x_orth [n][7], pose [n][6] = (tau_x,
+ * tau_y, tau_z, u), angles [n][3] = (yaw, pitch, roll) in degrees of the last line-search
+ * rotation (may be NULL), info [n][4] = (outer iterations, refit iterations, refit from the
+ * mirrored pose, tdi_status).  A frame whose 7x7 normal matrix is not positive definite stops
+ * with TDI_NOT_PD (the reference carries on with the failed factorisation). */
+int tdi_ortho_init(tdi_ctx* ctx, int32_t n, const float* lmk, const float* cams, float* x_orth, float* pose,
+                   float* angles, int32_t* info, void* stream);
+
 /* Frames per workspace chunk (the batch is processed in chunks of this many frames). */
 int tdi_workspace_frames(tdi_batch* b);
 

@@ -55,6 +55,10 @@ def lib():
         L.orc_fit_light.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_void_p, C.c_int,
                                     C.c_int]
         L.orc_fit_light.restype = C.c_int
+        L.orc_ortho_residuals.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_rotvec_from_euler.argtypes = [_dp, _dp]
+        L.orc_euler_angles.argtypes = [_dp, _dp]
+        L.orc_ortho_init.argtypes = [C.c_int, _fp, _fp, _dp, _dp, _dp, _dp, _ip]
         _lib = L
     return _lib
 
@@ -191,3 +195,39 @@ def cholesky_solve(H, g):
     d = np.zeros(K)
     rc = lib().orc_cholesky_solve(K, p(_d(H)), p(_d(g)), p(d))
     return rc, d
+
+
+def ortho_residuals(x, xlb, ylb, p0L, jacobian=True):
+    """Residuals [102] and Jacobian [102, 7] of the orthographic landmark model at x."""
+    e = np.zeros(102)
+    J = np.zeros(102 * 7) if jacobian else None
+    lib().orc_ortho_residuals(p(_d(x)), p(_d(xlb)), p(_d(ylb)), p(_d(p0L)), p(e), p(J))
+    return (e, J.reshape(102, 7)) if jacobian else e
+
+
+def rotvec_from_euler(e):
+    """Rotation vector of Rz(e[2]) Ry(e[1]) Rx(e[0])."""
+    r = np.zeros(3)
+    lib().orc_rotvec_from_euler(p(_d(e)), p(r))
+    return r
+
+
+def euler_angles(R):
+    """(yaw, pitch, roll) in radians of a rotation matrix R (3x3, as a numpy matrix)."""
+    ypr = np.zeros(3)
+    lib().orc_euler_angles(p(_d(np.asarray(R).T.reshape(-1))), p(ypr))
+    return ypr
+
+
+def ortho_init(lmk, cams, p0L):
+    """Orthographic initialisation of n frames: lmk [n, 2, 51], cams [n, 4] (phix, phiy, cx, cy),
+    p0L [51, 3].  Returns x_orth [n, 7], pose [n, 6], angles [n, 3] (degrees), info [n, 4]."""
+    lmk = np.ascontiguousarray(lmk, np.float32)
+    cams = np.ascontiguousarray(cams, np.float32)
+    n = lmk.shape[0]
+    xo = np.zeros((n, 7))
+    pose = np.zeros((n, 6))
+    ang = np.zeros((n, 3))
+    info = np.zeros((n, 4), np.int32)
+    lib().orc_ortho_init(n, p(lmk, _fp), p(cams, _fp), p(_d(p0L)), p(xo), p(pose), p(ang), p(info, _ip))
+    return dict(x_orth=xo, pose=pose, angles=ang, info=info)
