@@ -261,6 +261,22 @@ def main():
         log("[bench] copy probe: 1 GiB read + 1 GiB written")
         del a, b_
 
+    # landmark-driven initialisation of every frame (orthographic fit + pose seed, tdi_ortho_init)
+    lmk = torch.from_numpy(np.stack([np.stack([f.lmk_x, f.lmk_y]) for f in w.frames]).astype(np.float32)).cuda()
+    cams = torch.from_numpy(np.array([[f.phix, f.phiy, f.cx, f.cy] for f in w.frames], np.float32)).cuda()
+    tdi.ortho_init(ctx, lmk, cams)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        init_out = tdi.ortho_init(ctx, lmk, cams)
+    e1.record()
+    torch.cuda.synchronize()
+    init_ms = e0.elapsed_time(e1) / 5
+    landmark_init = dict(frames=args.frames, ms=round(init_ms, 4), frames_per_s=round(args.frames / (init_ms * 1e-3), 1),
+                         mean_outer_iterations=round(float(init_out["info"][:, 0].float().mean()), 2))
+    log(f"[bench] landmark initialisation: {args.frames} frames in {init_ms:.3f} ms")
+
     # FP64 oracle on a prefix: host baseline and parameter error of the batched fit
     cpu = None
     accuracy = None
@@ -278,6 +294,11 @@ def main():
                           f"{w.opts.max_iters} GN iterations, {dtc:.1f} s")
         log(f"[bench] oracle: {nc} frames in {dtc:.1f}s on {threads} threads")
         accuracy = sample_accuracy(w, O, sc, init, nc, th_o, ro, th_batched, plain, threads, tdi, ctx)
+        p0L = w.model.mean_shape.astype(np.float64)[w.model.landmark_ids.astype(np.int64)]
+        t0 = time.perf_counter()
+        O.ortho_init(lmk.cpu().numpy(), cams.cpu().numpy(), p0L)
+        landmark_init["cpu_frames_per_s"] = round(args.frames / (time.perf_counter() - t0), 1)
+        landmark_init["cpu_cores"] = threads
 
     out = {
         "metric": METRIC,
@@ -304,6 +325,7 @@ def main():
         "accuracy_vs_fp64": accuracy,
         "batched_stop_flags_note": "1 max iterations, 2 decrement test, 4 step below t_min, 8 NaN, 16 not PD",
         "stages": stages,
+        "landmark_init": landmark_init,
     }
     print(json.dumps(out), flush=True)
     batch.close()
