@@ -508,7 +508,7 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
     if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
     if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, w.box, s))) return rc;
-    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
+    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, nullptr, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
         return rc;
     if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
@@ -570,7 +570,7 @@ static int eval_reused(tdi_batch* b, FrameList fl, const float* theta, hipStream
     const Layout& L = c->L;
     Workspace& w = b->ws;
     int rc;
-    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary,
+    if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, nullptr, w.pvert, w.pbary,
                              w.max_area, w.tcount, w.tmax, w.box, s)))
         return rc;
     if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj, w.objb,
@@ -758,7 +758,7 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
         TDI_HIP(hipEventRecord(ev[1], s));
         if ((rc = launch_clear_keys_frames(w.frames, all_frames(n), w.keys, w.box, w.max_area, s))) return rc;
         if ((rc = launch_raster(c->m, w.frames, all_frames(n), w.vproj, c->opts.ntmp_cap, w.keys, w.box, s))) return rc;
-        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, w.bary, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
+        if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, nullptr, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[2], s));
         if ((rc = launch_shade(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix, w.box, w.img,

@@ -250,6 +250,27 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_write(const int* __rest
                 total += swave[w];
             }
             int pos = off + woff + before;
+            // the four pixels' gathers first (triangle corners, then their projected vertices),
+            // so that they are in flight together; then the coverage arithmetic and the stores
+            int tv[RES_PER_THREAD][3];
+            float4 tB[RES_PER_THREAD], tC[RES_PER_THREAD], tA[RES_PER_THREAD];
+#pragma unroll
+            for (int q = 0; q < RES_PER_THREAD; ++q) {
+                tv[q][0] = tv[q][1] = tv[q][2] = 0;
+                if (key[q] != KEY_EMPTY) {
+                    const int t = (int)(unsigned)(key[q] & 0xffffffffull);
+                    tv[q][0] = tris[3 * t];
+                    tv[q][1] = tris[3 * t + 1];
+                    tv[q][2] = tris[3 * t + 2];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RES_PER_THREAD; ++q)
+                if (key[q] != KEY_EMPTY) {
+                    tB[q] = vp[tv[q][0]];
+                    tC[q] = vp[tv[q][1]];
+                    tA[q] = vp[tv[q][2]];
+                }
 #pragma unroll
             for (int q = 0; q < RES_PER_THREAD; ++q) {
                 const int k = k0 + q;
@@ -258,13 +279,11 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_write(const int* __rest
                 cum[base + k] = pos + (occ ? 1 : 0);
                 if (occ) {
                     vis[base + pos] = k;
-                    const int t = (int)(unsigned)(key[q] & 0xffffffffull);
-                    const int vb = tris[3 * t], vc = tris[3 * t + 1], va = tris[3 * t + 2];
-                    const float4 B = vp[vb], Cc = vp[vc], Av = vp[va];
                     float al, be;
-                    cover_rn(B.x, B.y, Cc.x, Cc.y, Av.x, Av.y, fr.x0 + k / fr.H, fr.y0 + k % fr.H, &al, &be);
-                    bary[base + k] = make_float2(al, be);
-                    pvert[base + pos] = make_int4(vb, vc, va, k);
+                    cover_rn(tB[q].x, tB[q].y, tC[q].x, tC[q].y, tA[q].x, tA[q].y, fr.x0 + k / fr.H, fr.y0 + k % fr.H,
+                             &al, &be);
+                    if (bary) bary[base + k] = make_float2(al, be);
+                    pvert[base + pos] = make_int4(tv[q][0], tv[q][1], tv[q][2], k);
                     pbary[base + pos] = make_float2(al, be);
                     ++pos;
                 }
