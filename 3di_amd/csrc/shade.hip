@@ -232,19 +232,47 @@ __global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const 
         const float* P0f = P0 + (size_t)f * m.V * 3;
         const float4* vp = vproj + (size_t)f * m.V;
         for (int x = r.X0 + cm.sub * nw + wave; x < r.X1; x += cm.fpb * nw)
-            for (int y = r.Y0 + lane; y < r.Y1; y += 64) {
-                const int k = x * fr.H + y;
-                const unsigned long long key = keys[base + k];
-                float v = 0.f;
-                if (key != KEY_EMPTY) {
-                    const int t = (int)(unsigned)(key & 0xffffffffull);
-                    const int vb = m.tris[3 * t], vc = m.tris[3 * t + 1], va = m.tris[3 * t + 2];
-                    const float4 B = vp[vb], Cc = vp[vc], Av = vp[va];
-                    float al, be;
-                    cover_rn(B.x, B.y, Cc.x, Cc.y, Av.x, Av.y, fr.x0 + x, fr.y0 + y, &al, &be);
-                    v = shade_pixel(m, L, P0f, sf, stex, vb, vc, va, al, be);
+            for (int y0 = r.Y0 + lane; y0 < r.Y1; y0 += 128) {
+                // two rows per lane: the gathers of both (keys, triangle corners, then vertices
+                // and albedo) are issued before the arithmetic, each under its pixel's coverage
+                const int y1 = y0 + 64;
+                const bool h1 = y1 < r.Y1;
+                const int k0 = x * fr.H + y0, k1 = x * fr.H + y1;
+                const unsigned long long key0 = keys[base + k0];
+                const unsigned long long key1 = h1 ? keys[base + k1] : KEY_EMPTY;
+                const bool c0 = key0 != KEY_EMPTY, c1 = key1 != KEY_EMPTY;
+                int v0[3] = {0, 0, 0}, v1[3] = {0, 0, 0};
+                if (c0) {
+                    const int t = (int)(unsigned)(key0 & 0xffffffffull);
+                    v0[0] = m.tris[3 * t]; v0[1] = m.tris[3 * t + 1]; v0[2] = m.tris[3 * t + 2];
                 }
-                img[base + k] = v;
+                if (c1) {
+                    const int t = (int)(unsigned)(key1 & 0xffffffffull);
+                    v1[0] = m.tris[3 * t]; v1[1] = m.tris[3 * t + 1]; v1[2] = m.tris[3 * t + 2];
+                }
+                float4 B0, C0, A0, B1, C1, A1;
+                ShadeIn in0, in1;
+                if (c0) {
+                    B0 = vp[v0[0]]; C0 = vp[v0[1]]; A0 = vp[v0[2]];
+                    shade_load(m, P0f, v0[0], v0[1], v0[2], in0);
+                }
+                if (c1) {
+                    B1 = vp[v1[0]]; C1 = vp[v1[1]]; A1 = vp[v1[2]];
+                    shade_load(m, P0f, v1[0], v1[1], v1[2], in1);
+                }
+                float s0 = 0.f, s1 = 0.f;
+                if (c0) {
+                    float al, be;
+                    cover_rn(B0.x, B0.y, C0.x, C0.y, A0.x, A0.y, fr.x0 + x, fr.y0 + y0, &al, &be);
+                    s0 = shade_eval(m, L, sf, stex, v0[0], v0[1], v0[2], in0, al, be);
+                }
+                if (c1) {
+                    float al, be;
+                    cover_rn(B1.x, B1.y, C1.x, C1.y, A1.x, A1.y, fr.x0 + x, fr.y0 + y1, &al, &be);
+                    s1 = shade_eval(m, L, sf, stex, v1[0], v1[1], v1[2], in1, al, be);
+                }
+                img[base + k0] = s0;
+                if (h1) img[base + k1] = s1;
             }
         __syncthreads();   // the frame's shared constants are rewritten by the next iteration
     }
