@@ -199,6 +199,43 @@ static int fit_3DMM_lambdas(tdi_batch* b, int n, const float* theta, float* lam,
     return tdi_batch_set_light(b, lam, Lint);
 }
 
+// Logbarrier_Initializer::initialize_with_orthographic_t (logbarrier_initializer.cu:1862-1914),
+// orthographic fit and pose seed, for n frames at once: host arrays in and out.
+static int initialize_with_orthographic_t(tdi_ctx* ctx, int n, const float* lmk_x, const float* lmk_y,
+                                          const float* cams, float* pose, float* angles, int32_t* info)
+{
+    const int NL = TDI_NLANDMARKS;
+    std::vector<float> lmk((size_t)n * 2 * NL);
+    for (int f = 0; f < n; ++f)
+        for (int i = 0; i < NL; ++i) {
+            lmk[(size_t)f * 2 * NL + i] = lmk_x[(size_t)f * NL + i];
+            lmk[(size_t)f * 2 * NL + NL + i] = lmk_y[(size_t)f * NL + i];
+        }
+    // one device block: landmarks, cameras, x_orth, pose, angles, info
+    const size_t nl = lmk.size(), nc = 4 * (size_t)n, nx = 7 * (size_t)n, np = 6 * (size_t)n, na = 3 * (size_t)n;
+    const size_t bytes = sizeof(float) * (nl + nc + nx + np + na) + sizeof(int32_t) * 4 * (size_t)n;
+    char* d = nullptr;
+    if (hipMalloc((void**)&d, bytes) != hipSuccess) return TDI_ERR_NOMEM;
+    float* dl = (float*)d;
+    float* dc = dl + nl;
+    float* dx = dc + nc;
+    float* dp = dx + nx;
+    float* da = dp + np;
+    int32_t* di = (int32_t*)(da + na);
+    int rc = TDI_OK;
+    if (hipMemcpy(dl, lmk.data(), sizeof(float) * nl, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dc, cams, sizeof(float) * nc, hipMemcpyHostToDevice) != hipSuccess)
+        rc = TDI_ERR_HIP;
+    if (!rc) rc = tdi_ortho_init(ctx, n, dl, dc, dx, dp, da, di, nullptr);
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = TDI_ERR_HIP;
+    if (!rc && hipMemcpy(pose, dp, sizeof(float) * np, hipMemcpyDeviceToHost) != hipSuccess) rc = TDI_ERR_HIP;
+    if (!rc && angles && hipMemcpy(angles, da, sizeof(float) * na, hipMemcpyDeviceToHost) != hipSuccess) rc = TDI_ERR_HIP;
+    if (!rc && info && hipMemcpy(info, di, sizeof(int32_t) * 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = TDI_ERR_HIP;
+    (void)hipFree(d);
+    return rc;
+}
+
 } // namespace tdi_host
 
 // ---------------------------------------------------------------------------- C entry points
@@ -275,6 +312,13 @@ int tdi_read_config(const char* path, tdi_fit_options* opts, int32_t* twostage_i
     if (twostage_illum) *twostage_illum = two;
     if (nres_coefs) *nres_coefs = nres;
     return TDI_OK;
+}
+
+int tdi_initialize_with_orthographic(tdi_ctx* ctx, int32_t n, const float* lmk_x, const float* lmk_y,
+                                     const float* cams, float* pose, float* angles, int32_t* info)
+{
+    if (!ctx || n <= 0 || !lmk_x || !lmk_y || !cams || !pose) return TDI_ERR_ARG;
+    return tdi_host::initialize_with_orthographic_t(ctx, n, lmk_x, lmk_y, cams, pose, angles, info);
 }
 
 int tdi_fit_video_frames(tdi_batch* b, int32_t twostage_illum, int32_t batched, float* theta, float* lam,

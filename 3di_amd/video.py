@@ -39,6 +39,7 @@ def lib():
         L.tdi_euler_angles.argtypes = [_vp, _vp]
         L.tdi_read_config.argtypes = [C.c_char_p, _vp, _vp, _vp]
         L.tdi_fit_video_frames.argtypes = [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp]
+        L.tdi_initialize_with_orthographic.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -130,3 +131,20 @@ def fit_video_frames(batch, theta, lam, twostage_illum: bool = False, batched: b
     if rc:
         raise tdi.TdiError(f"tdi_fit_video_frames: {rc}: {tdi.lib().tdi_last_error().decode()}")
     return th, la, Li, list(reps)
+
+
+def initialize_with_orthographic(ctx, lmk_x, lmk_y, cams):
+    """initialize_with_orthographic_t for n frames (host arrays): lmk_x, lmk_y [n, 51], cams [n, 4]
+    (phix, phiy, cx, cy).  Returns pose [n, 6] (tau, u), angles [n, 3] (degrees), info [n, 4]."""
+    lx = np.ascontiguousarray(lmk_x, np.float32)
+    ly = np.ascontiguousarray(lmk_y, np.float32)
+    cm = np.ascontiguousarray(cams, np.float32)
+    n = lx.shape[0]
+    pose = np.zeros((n, 6), np.float32)
+    ang = np.zeros((n, 3), np.float32)
+    info = np.zeros((n, 4), np.int32)
+    rc = lib().tdi_initialize_with_orthographic(ctx.h, n, lx.ctypes.data, ly.ctypes.data, cm.ctypes.data,
+                                                pose.ctypes.data, ang.ctypes.data, info.ctypes.data)
+    if rc:
+        raise tdi.TdiError(f"tdi_initialize_with_orthographic: {rc}: {tdi.lib().tdi_last_error().decode()}")
+    return pose, ang, info

@@ -41,6 +41,15 @@ void tdi_euler_angles(const float* u, float* yaw_pitch_roll);
  * niters_success) and returns TWOSTAGE_ILLUM and NRES_COEFS (defaults 0 and 1, config.cu). */
 int tdi_read_config(const char* path, tdi_fit_options* opts, int32_t* twostage_illum, int32_t* nres_coefs);
 
+/* Starting poses from the landmarks, as Logbarrier_Initializer::initialize_with_orthographic_t
+ * (logbarrier_initializer.cu:1862-1914) computes them before each frame's fit
+ * (video_fitter.cu:216-224,479-485), for n frames at once on the GPU (tdi_ortho_init).  Host
+ * arrays: lmk_x, lmk_y [n][51] image pixels, cams [n][4] (phix, phiy, cx, cy); outputs pose
+ * [n][6] (tau, u), angles [n][3] (yaw, pitch, roll, degrees; may be NULL), info [n][4] (may be
+ * NULL; see tdi_ortho_init). */
+int tdi_initialize_with_orthographic(tdi_ctx* ctx, int32_t n, const float* lmk_x, const float* lmk_y,
+                                     const float* cams, float* pose, float* angles, int32_t* info);
+
 /* The per-frame sequence of fit_to_video_frame (video_fitter.cu:554-615) for every frame of the
  * batch: L = 0.005, lambda fit, shape fit; with twostage_illum two more lambda + shape rounds.
  * theta [n][K], lam [n][3], Lint [n] are host arrays updated in place; reports are those of

@@ -111,3 +111,15 @@ def test_ortho_init_weak_perspective_truth(model, ctx):
     got = _compare(model, ctx, lmk, cams, "weak-perspective truth")
     assert np.allclose(got["pose"][:, 2], want[:, 2], rtol=1e-4)
     assert np.allclose(got["pose"][:, 3:], want[:, 3:], atol=1e-4)
+
+
+def test_host_layer_initialisation_matches(model, ctx):
+    video = importlib.import_module("3di_amd.video")
+    lmk, cams = _frames(model, 24, seed=5)
+    _say("case: host-layer initialize_with_orthographic, 24 frames")
+    pose, ang, info = video.initialize_with_orthographic(ctx, lmk[:, 0], lmk[:, 1], cams)
+    got = tdi.ortho_init(ctx, torch.from_numpy(lmk).cuda(), torch.from_numpy(cams).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(pose, got["pose"].cpu().numpy())
+    assert np.array_equal(ang, got["angles"].cpu().numpy())
+    assert np.array_equal(info, got["info"].cpu().numpy())
