@@ -167,7 +167,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the fitting step shards by frames with no exchange, "
-              "so --gpus N > 1 is not run here (see DESIGN.md, 'More than one GPU').", file=sys.stderr)
+              "so --gpus N > 1 is not run here; frames are independent, and more GPUs would each fit a "
+              "contiguous range of the frames with their own model copy.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -295,7 +295,7 @@ def _oracle_floor(scene, th0):
     """The oracle's own spread: the largest pose/expression RMS between its fit from th0 and
     its fits from three starts 1e-6 away (random signs).  The rasterised objective is
     piecewise constant in visibility and the line search takes discrete decisions, so
-    whole-fit trajectories amplify round-off; parity of whole fits is judged against this."""
+    whole-fit trajectories amplify round-off; parity of whole fits is measured against this."""
     orc = scene.oracle()
     a, ra = orc.fit(th0)
     rng = np.random.default_rng(17)
@@ -343,7 +343,7 @@ def test_fit_batched_vs_oracle(scene, gpu):
 
 
 def test_fit_batched_chunked_vs_oracle(scene):
-    """Whole batched fit in one-frame workspace chunks, judged like the single-chunk fit."""
+    """Whole batched fit in one-frame workspace chunks, held to the bar of the single-chunk fit."""
     _say("case: batched fit in one-frame chunks vs oracle")
     th0 = _f32(scene.init())
     tho, repo, floor = _oracle_floor(scene, th0)
