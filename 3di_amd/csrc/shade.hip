@@ -28,9 +28,9 @@ namespace {
 // Region of a crop a pass has to cover, half-open [X0, X1) x [Y0, Y1).  Without a box (or
 // when the visible pixels touch the crop border, whose neighbour rule reads crop index 0) it
 // is the whole crop.  With the visible box (xl, xh, yl, yh) of the frame: the normalised
-// gradients read the blurred image at visible pixels +-1 and the blur reaches 7 pixels along
-// x and y, so the blurred output covers the box + 1 and the rendered image must be clear on
-// the box + 8.
+// gradients read the blurred image at visible pixels +-1, so the blurred output covers the
+// box + 1.  The rendered image is written on the box only (cleared where no triangle is); the
+// blur, which reaches 7 pixels along x and y, reads zero outside it.
 struct Region {
     int X0, X1, Y0, Y1;
 };
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) k_clear_crops(const FrameGPU* __restrict_
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
         Region r;
-        if (!crop_region(fr, box, f, 8, 8, r)) continue;
+        if (!crop_region(fr, box, f, 0, 0, r)) continue;
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
         for (int x = r.X0 + cm.sub * nw + wave; x < r.X1; x += cm.fpb * nw)
             for (int y = r.Y0 + lane; y < r.Y1; y += 64) img[fr.pix_off + (long long)x * fr.H + y] = 0.f;
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const Frame
     }
 }
 
-// Shading straight from the depth|triangle keys over the frame's region (visible box + 8):
+// Shading straight from the depth|triangle keys over the frame's visible box:
 // the barycentrics are recomputed with the coverage test the rasteriser used, so the image is
 // the one the visible-list path renders; pixels without a triangle are cleared.  Used by the
 // line search, which needs the objective only.
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const 
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
         Region r;
-        if (!crop_region(fr, box, f, 8, 8, r)) continue;
+        if (!crop_region(fr, box, f, 0, 0, r)) continue;
         if (r.X0 + (int)cm.sub * nw >= r.X1) continue;
         const float* tf = theta + (size_t)f * L.K;
         if (threadIdx.x == 0) shade_frame_setup(L, fr, tf, sf);
@@ -308,8 +308,10 @@ __global__ void __launch_bounds__(256) k_blur_tiles(const FrameGPU* __restrict__
     for (int yi = cm.slot; yi < nact; yi += cm.nslots) {
         const int f = fl_frame(act, yi);
         const FrameGPU fr = frames[f];
-        Region rg;
+        Region rg, rs;
         if (!crop_region(fr, box, f, 1, 1, rg)) continue;
+        // the shaded region: the rendered image is written there only and is zero around it
+        crop_region(fr, box, f, 0, 0, rs);
         const int W = fr.W, H = fr.H;
         const int ntx = (rg.X1 - rg.X0 + BT_X - 1) / BT_X, nty = (rg.Y1 - rg.Y0 + BT_Y - 1) / BT_Y;
         const float* src = in + fr.pix_off;
@@ -321,7 +323,7 @@ __global__ void __launch_bounds__(256) k_blur_tiles(const FrameGPU* __restrict__
                 const int x = x0 - 7 + c;
                 for (int r = lane; r < BT_IY; r += 64) {
                     const int y = y0 - 7 + r;
-                    simg[c][r] = (x >= 0 && x < W && y >= 0 && y < H) ? src[(size_t)x * H + y] : 0.f;
+                    simg[c][r] = (x >= rs.X0 && x < rs.X1 && y >= rs.Y0 && y < rs.Y1) ? src[(size_t)x * H + y] : 0.f;
                 }
             }
             __syncthreads();
@@ -432,8 +434,9 @@ __global__ void __launch_bounds__(256) k_objective(const FrameGPU* __restrict__ 
             for (int y = r.Y0 + lane; y < r.Y1; y += 64) {
                 const int k = x * fr.H + y;
                 if (keys[base + k] == KEY_EMPTY) continue;
-                int kr, kl, ka, kb;
-                nbr_index(k, fr.W, fr.H, kr, kl, ka, kb);
+                // neighbour indices as nbr_index gives them, from the loop's x and y
+                const int kr = (x < fr.W - 1) ? k + fr.H : 0, kl = (x > 0) ? k - fr.H : 0;
+                const int ka = (y > 0) ? k - 1 : 0, kb = (y < fr.H - 1) ? k + 1 : 0;
                 float gx, gy, h;
                 acc += pixel_term(I, kr, kl, ka, kb, obs_g[fr.obs_off + k], gx, gy, h);
             }
