@@ -11,6 +11,8 @@ import importlib
 import os
 import re
 
+import ctypes as C
+
 import numpy as np
 import pytest
 from scipy.spatial.transform import Rotation
@@ -110,3 +112,13 @@ def test_read_config():
     assert o.data_weight == d.data_weight and o.max_iters == d.max_iters
     with pytest.raises(FileNotFoundError):
         video.read_config(os.path.join(ROOT, "tests", "golden", "no_such_config.txt"))
+
+
+def test_initialize_with_orthographic_rejects_bad_arguments():
+    # argument checks come before any device call (TDI_ERR_ARG = 1)
+    L = video.lib()
+    buf = np.zeros(51 * 4, np.float32)
+    p = buf.ctypes.data
+    assert L.tdi_initialize_with_orthographic(None, 1, p, p, p, p, None, None) == 1
+    assert L.tdi_initialize_with_orthographic(C.c_void_p(1), 0, p, p, p, p, None, None) == 1
+    assert L.tdi_initialize_with_orthographic(C.c_void_p(1), 1, None, p, p, p, None, None) == 1
