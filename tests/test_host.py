@@ -7,11 +7,10 @@ unambiguous; it follows the reference's rules (setprecision(7), one space after 
 rows t < max(frame id) -- the last frame is not written -- except save_expressions_all, which
 writes t <= max; missing frames are NaN rows; expressions average the NRES fits, poses all
 fits, illumination keeps the first)."""
+import ctypes as C
 import importlib
 import os
 import re
-
-import ctypes as C
 
 import numpy as np
 import pytest
