@@ -1,0 +1,191 @@
+// frames.hip — frame preparation: 8-bit BGR video frames -> the grey FP32 crops the fit observes.
+//
+// One pass per crop pixel replaces the reference's per-frame host chain
+// InputData::get_resized_frame (input_data.cu:44-53: cvtColor BGR2GRAY, convertTo CV_32F, /255,
+// cv::resize by the camera's resize coefficient, copyMakeBorder with zeros at the bottom and right)
+// followed by the crop at (x0_short, y0_short) and its transpose to column-major
+// (video_fitter.cu:537-543).  The frames stay resident in HBM; only the crop's footprint in the
+// source frame is read.  OpenCV's published semantics are kept operation for operation:
+//   - grey = (1868 B + 9617 G + 4899 R + 2^13) >> 14, the BT.601 weights in 14-bit fixed point;
+//   - value = float(grey) * float(1/255), the scale of convertTo;
+//   - resized size round-half-even(src * coef); when it equals the source size the frame is
+//     copied; otherwise INTER_LINEAR with half-pixel centres: f = float((d + 0.5) / coef - 0.5),
+//     s = floor(f), weight f - s; columns left of 0 or at/after the last take the edge pixel with
+//     weight 0, rows are clamped individually and keep their weights; the horizontal pass first,
+//     each pass a*(1-w) + b*w with both products and the sum rounded (contraction is off
+//     in this file's device code: no fused multiply-add).
+#include <cmath>
+
+#include "common.h"
+
+namespace tdi {
+
+namespace {
+
+// A workgroup prepares a 64 x 64 tile of one crop in two phases.  Phase 1: each lane owns a crop
+// column and walks 16 rows, so the 64 lanes of a wavefront read neighbouring source pixels of the
+// same two source rows (a few cache lines per load instead of one per lane); the values go to an
+// LDS tile.  Phase 2: each lane owns a crop row and writes 16 columns, so every wavefront store is
+// 64 consecutive floats of one column-major crop column.
+constexpr int PREP_TILE = 64;
+constexpr int PREP_WAVES = 4;
+
+struct PrepFrame {
+    long long src;     // byte offset of the frame
+    long long dst;     // float offset of the crop
+    double scale;      // 1 / coef
+    int x0, y0, w, h;  // crop in resized-frame pixels
+    int dw, dh;        // resized frame size
+    int copy;          // resized size equals the source size: plain copy
+    int tiles_y;       // ceil(h / PREP_TILE)
+};
+
+__device__ __forceinline__ float grey_at(const uint8_t* __restrict__ f, int sw, int x, int y)
+{
+#pragma clang fp contract(off)
+    const uint8_t* p = f + 3ll * ((long long)y * sw + x);
+    const int g = (1868 * (int)p[0] + 9617 * (int)p[1] + 4899 * (int)p[2] + 8192) >> 14;
+    return (float)g * (float)(1.0 / 255.0);
+}
+
+__device__ __forceinline__ float lerp_rn(float a, float b, float w)
+{
+#pragma clang fp contract(off)
+    return a * (1.f - w) + b * w;
+}
+
+__global__ __launch_bounds__(256) void k_prepare_crops(int n, const PrepFrame* __restrict__ fr,
+                                                       const int* __restrict__ tile_start,
+                                                       const uint8_t* __restrict__ bgr, int sw, int sh,
+                                                       float* __restrict__ out)
+{
+#pragma clang fp contract(off)   // every product and sum rounded, as the restated operation sequence
+    __shared__ float tile[PREP_TILE][PREP_TILE + 1];
+    // frame of this tile: the last f with tile_start[f] <= blockIdx.x
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const PrepFrame F = fr[lo];
+    const int t = b - tile_start[lo];
+    const int c0 = (t / F.tiles_y) * PREP_TILE, r0t = (t % F.tiles_y) * PREP_TILE;
+    const int lane = (int)threadIdx.x % PREP_TILE, wave = (int)threadIdx.x / PREP_TILE;
+    const uint8_t* src = bgr + F.src;
+
+    // phase 1: lane = column
+    {
+        const int c = c0 + lane;
+        const int X = F.x0 + c;
+        int sx = 0, sx1 = 0;
+        float fx = 0.f;
+        if (!F.copy && X < F.dw) {
+            fx = (float)(((double)X + 0.5) * F.scale - 0.5);
+            sx = (int)floorf(fx);
+            fx -= (float)sx;
+            if (sx < 0) { sx = 0; fx = 0.f; }
+            if (sx >= sw - 1) { sx = sw - 1; fx = 0.f; }
+            sx1 = min(sx + 1, sw - 1);
+        }
+        for (int k = wave; k < PREP_TILE; k += PREP_WAVES) {
+            const int r = r0t + k;
+            if (c >= F.w || r >= F.h) continue;
+            const int Y = F.y0 + r;
+            float v = 0.f;
+            if (X < F.dw && Y < F.dh) {
+                if (F.copy) {
+                    v = grey_at(src, sw, X, Y);
+                } else {
+                    float fy = (float)(((double)Y + 0.5) * F.scale - 0.5);
+                    const int sy = (int)floorf(fy);
+                    fy -= (float)sy;
+                    const int y0 = min(max(sy, 0), sh - 1), y1 = min(max(sy + 1, 0), sh - 1);
+                    const float h0 = lerp_rn(grey_at(src, sw, sx, y0), grey_at(src, sw, sx1, y0), fx);
+                    const float h1 = lerp_rn(grey_at(src, sw, sx, y1), grey_at(src, sw, sx1, y1), fx);
+                    v = lerp_rn(h0, h1, fy);
+                }
+            }
+            tile[lane][k] = v;
+        }
+    }
+    __syncthreads();
+    // phase 2: lane = row
+    const int r = r0t + lane;
+    if (r >= F.h) return;
+    for (int k = wave; k < PREP_TILE; k += PREP_WAVES) {
+        const int c = c0 + k;
+        if (c >= F.w) break;
+        out[F.dst + (long long)c * F.h + r] = tile[k][lane];
+    }
+}
+
+} // namespace
+
+} // namespace tdi
+
+using namespace tdi;
+
+extern "C" int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
+                                 const float* coef, const int32_t* crop, float* out, void* stream)
+{
+    if (n < 0 || (n > 0 && (!bgr || !coef || !crop || !out)) || src_w < 1 || src_h < 1 ||
+        frame_stride < 3ll * src_w * src_h) {
+        set_error("tdi_prepare_crops: null argument, negative frame count or bad frame geometry");
+        return TDI_ERR_ARG;
+    }
+    if (n == 0) return TDI_OK;
+    std::vector<PrepFrame> fr(n);
+    std::vector<int> tile_start(n);
+    long long dst = 0, tiles = 0;
+    for (int i = 0; i < n; ++i) {
+        const float k = coef[i];
+        const int* q = crop + 4 * i;
+        if (!(k > 0.f) || !std::isfinite(k) || q[0] < 0 || q[1] < 0 || q[2] < 1 || q[3] < 1) {
+            set_error("tdi_prepare_crops: frame %d: coefficient must be positive and finite, crop origin "
+                      "non-negative and crop size positive", i);
+            return TDI_ERR_ARG;
+        }
+        PrepFrame& F = fr[i];
+        F.src = (long long)i * frame_stride;
+        F.dst = dst;
+        F.scale = 1.0 / (double)k;
+        F.x0 = q[0]; F.y0 = q[1]; F.w = q[2]; F.h = q[3];
+        const double dw = std::nearbyint((double)src_w * (double)k), dh = std::nearbyint((double)src_h * (double)k);
+        if (dw < 1.0 || dh < 1.0 || dw > 1e9 || dh > 1e9) {
+            set_error("tdi_prepare_crops: frame %d: the resized frame would be empty or too large", i);
+            return TDI_ERR_ARG;
+        }
+        F.dw = (int)dw; F.dh = (int)dh;
+        F.copy = (F.dw == src_w && F.dh == src_h);
+        F.tiles_y = (F.h + PREP_TILE - 1) / PREP_TILE;
+        tile_start[i] = (int)tiles;
+        tiles += (long long)F.tiles_y * ((F.w + PREP_TILE - 1) / PREP_TILE);
+        dst += (long long)F.w * F.h;
+        if (tiles > (1ll << 24) - 1) { set_error("tdi_prepare_crops: too many crop pixels for one call"); return TDI_ERR_ARG; }
+    }
+    hipStream_t s = (hipStream_t)stream;
+    PrepFrame* dfr = nullptr;
+    int* dts = nullptr;
+    if (hipMalloc(&dfr, sizeof(PrepFrame) * n) != hipSuccess || hipMalloc(&dts, sizeof(int) * n) != hipSuccess) {
+        (void)hipFree(dfr);
+        set_error("tdi_prepare_crops: descriptor allocation failed");
+        return TDI_ERR_NOMEM;
+    }
+    int rc = TDI_OK;
+    if (hipMemcpyAsync(dfr, fr.data(), sizeof(PrepFrame) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dts, tile_start.data(), sizeof(int) * n, hipMemcpyHostToDevice, s) != hipSuccess) {
+        set_error("tdi_prepare_crops: descriptor upload failed");
+        rc = TDI_ERR_HIP;
+    } else {
+        hipLaunchKernelGGL(k_prepare_crops, dim3((unsigned)tiles), dim3(PREP_TILE * PREP_WAVES), 0, s, n, dfr, dts, bgr,
+                           src_w, src_h, out);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { set_error("tdi_prepare_crops: %s", hipGetErrorString(e)); rc = TDI_ERR_HIP; }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dfr);
+    (void)hipFree(dts);
+    return rc;
+}

@@ -236,6 +236,111 @@ static int initialize_with_orthographic_t(tdi_ctx* ctx, int n, const float* lmk_
     return rc;
 }
 
+// ---------------------------------------------------------------------------- landmark files
+// read2DVectorFromFile_unknown_size (funcs.h:352-413): the first line's count of numbers (with
+// "nan" counted as a number) fixes the columns; the file's line count, at most 32767, the rows;
+// the values are then read in order, one that does not parse reading as NaN and skipping three
+// characters (the text "nan").  LandmarkData::fill_xpypvec (input_data.cu:105-121) takes
+// x_i = row[2i], y_i = row[2i+1].
+static int count_numbers(std::string line)
+{
+    for (size_t at = line.find("nan"); at != std::string::npos; at = line.find("nan", at + 1)) line.replace(at, 3, "0");
+    std::istringstream in(line);
+    int cnt = 0;
+    float v;
+    while (in >> v) ++cnt;
+    return cnt;
+}
+
+static int read_landmark_rows(const char* path, std::vector<std::vector<float>>& rows)
+{
+    std::ifstream shape(path);
+    if (!shape.is_open()) return TDI_ERR_ARG;
+    std::string line;
+    std::getline(shape, line);
+    const int cols = count_numbers(line);
+    int nrows = 1;
+    while (nrows < 32767 && std::getline(shape, line)) ++nrows;
+    shape.close();
+    if (cols < 2 * TDI_NLANDMARKS) return TDI_ERR_ARG;
+    std::ifstream in(path);
+    rows.assign(nrows, std::vector<float>());
+    for (int r = 0; r < nrows; ++r)
+        for (int c = 0; c < cols; ++c) {
+            float v;
+            if (!(in >> v)) {
+                rows[r].push_back(std::numeric_limits<float>::quiet_NaN());
+                in.clear();
+                in.ignore(3);
+                continue;
+            }
+            rows[r].push_back(v);
+        }
+    return TDI_OK;
+}
+
+// ---------------------------------------------------------------------------- frame geometry
+// LandmarkData::get_face_size (input_data.cu:124-138): extents of the landmarks truncated to int.
+static float landmark_face_size(const float* x, const float* y)
+{
+    const int NL = TDI_NLANDMARKS;
+    const int x0 = (int)*std::min_element(x, x + NL), x1 = (int)*std::max_element(x, x + NL);
+    const int y0 = (int)*std::min_element(y, y + NL), y1 = (int)*std::max_element(y, y + NL);
+    return (float)std::max(x1 - x0, y1 - y0);
+}
+
+// compute_face_size (funcs.cu:44-53): the larger extent of the landmarks, in float.
+static float compute_face_size(const float* x, const float* y)
+{
+    const int NL = TDI_NLANDMARKS;
+    const float w = *std::max_element(x, x + NL) - *std::min_element(x, x + NL);
+    const float h = *std::max_element(y, y + NL) - *std::min_element(y, y + NL);
+    return std::max(w, h);
+}
+
+// Renderer::set_x0_short_y0_short (renderer.cu:585-606): the crop origin is the landmarks' top
+// left corner less a margin of 1/1.5 of the face's larger extent (pad) or 20 pixels, clamped at 0
+// and truncated to an unsigned short.
+static void set_x0_short_y0_short(const float* x, const float* y, bool pad, int32_t* x0, int32_t* y0)
+{
+    const int NL = TDI_NLANDMARKS;
+    const float xmin = *std::min_element(x, x + NL), xmax = *std::max_element(x, x + NL);
+    const float ymin = *std::min_element(y, y + NL), ymax = *std::max_element(y, y + NL);
+    if (pad) {
+        const float sz = std::max(xmax - xmin, ymax - ymin);
+        *x0 = (uint16_t)std::max<float>(xmin - sz / 1.5, 0);
+        *y0 = (uint16_t)std::max<float>(ymin - sz / 1.5, 0);
+    } else {
+        *x0 = (uint16_t)std::max<float>(xmin - 20, 0);
+        *y0 = (uint16_t)std::max<float>(ymin - 20, 0);
+    }
+}
+
+// One frame of the video loop before its fit: the in-frame test (video_fitter.cu:180-186), the
+// resize coefficient ref_face_size / face size (video_fitter.cu:357-360) applied to the camera
+// (Camera::update_camera, camera.h:162-174) and to the landmarks (get_resized_landmarks,
+// input_data.cu:33-41), the crop origin of the resized landmarks and their face size
+// (video_fitter.cu:466).
+static int frame_geometry(const float* x, const float* y, const float* cam, int fw, int fh, float ref_face_size,
+                          bool pad, int cw, int ch, float* coef, float* xo, float* yo, float* cam_out, int32_t* crop,
+                          float* face_size)
+{
+    const int NL = TDI_NLANDMARKS;
+    const float minx = *std::min_element(x, x + NL), maxx = *std::max_element(x, x + NL);
+    const float miny = *std::min_element(y, y + NL), maxy = *std::max_element(y, y + NL);
+    if (!(minx > 0.0f && miny > 0.0f && maxx < fw && maxy < fh)) return TDI_OUT_OF_FRAME;
+    const float k = ref_face_size / landmark_face_size(x, y);
+    if (!(k > 0.f) || !std::isfinite(k)) return TDI_OUT_OF_FRAME;
+    *coef = k;
+    for (int i = 0; i < 4; ++i) cam_out[i] = cam[i] * k;
+    for (int i = 0; i < NL; ++i) { xo[i] = k * x[i]; yo[i] = k * y[i]; }
+    set_x0_short_y0_short(xo, yo, pad, crop, crop + 1);
+    crop[2] = cw;
+    crop[3] = ch;
+    *face_size = compute_face_size(xo, yo);
+    return TDI_OK;
+}
+
 } // namespace tdi_host
 
 // ---------------------------------------------------------------------------- C entry points
@@ -337,6 +442,48 @@ int tdi_fit_video_frames(tdi_batch* b, int32_t twostage_illum, int32_t batched, 
         if (!rc) rc = tdi_host::fit_3DMM_shape_rigid(b, n, K, batched != 0, theta, reports);
     }
     return rc;
+}
+
+int tdi_read_landmarks(const char* path, int32_t max_frames, float* lmk_x, float* lmk_y, int32_t* n_frames)
+{
+    if (!path || !n_frames || (lmk_x && (!lmk_y || max_frames < 0))) return TDI_ERR_ARG;
+    std::vector<std::vector<float>> rows;
+    const int rc = tdi_host::read_landmark_rows(path, rows);
+    if (rc) return rc;
+    const int NL = TDI_NLANDMARKS;
+    const int n = lmk_x ? std::min<int>(max_frames, (int)rows.size()) : (int)rows.size();
+    for (int t = 0; lmk_x && t < n; ++t)
+        for (int i = 0; i < NL; ++i) {
+            lmk_x[(size_t)t * NL + i] = rows[t][2 * i];
+            lmk_y[(size_t)t * NL + i] = rows[t][2 * i + 1];
+        }
+    *n_frames = n;
+    return TDI_OK;
+}
+
+int tdi_frame_geometry(int32_t n, const float* lmk_x, const float* lmk_y, const float* cams, int32_t frame_w,
+                       int32_t frame_h, float ref_face_size, int32_t pad, int32_t crop_w, int32_t crop_h, float* coef,
+                       float* lmk_x_out, float* lmk_y_out, float* cams_out, int32_t* crop, float* face_size,
+                       int32_t* status)
+{
+    if (n < 0 || (n > 0 && (!lmk_x || !lmk_y || !cams || !coef || !lmk_x_out || !lmk_y_out || !cams_out || !crop ||
+                            !face_size || !status)) ||
+        frame_w < 1 || frame_h < 1 || crop_w < 1 || crop_h < 1 || !(ref_face_size > 0.f))
+        return TDI_ERR_ARG;
+    const int NL = TDI_NLANDMARKS;
+    for (int f = 0; f < n; ++f) {
+        const size_t o = (size_t)f * NL;
+        status[f] = tdi_host::frame_geometry(lmk_x + o, lmk_y + o, cams + 4 * (size_t)f, frame_w, frame_h, ref_face_size,
+                                             pad != 0, crop_w, crop_h, coef + f, lmk_x_out + o, lmk_y_out + o,
+                                             cams_out + 4 * (size_t)f, crop + 4 * (size_t)f, face_size + f);
+        if (status[f]) {
+            coef[f] = 0.f;
+            face_size[f] = 0.f;
+            for (int i = 0; i < 4; ++i) { cams_out[4 * (size_t)f + i] = 0.f; crop[4 * (size_t)f + i] = 0; }
+            for (int i = 0; i < NL; ++i) lmk_x_out[o + i] = lmk_y_out[o + i] = 0.f;
+        }
+    }
+    return TDI_OK;
 }
 
 } // extern "C"

@@ -61,6 +61,7 @@ def lib():
         L.tdi_batch_size.argtypes = [_vp]
         L.tdi_batch_unknowns.argtypes = [_vp]
         L.tdi_fit_lintensity.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.tdi_prepare_crops.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -280,6 +281,28 @@ def ortho_init(ctx: Context, lmk, cams):
     _check(lib().tdi_ortho_init(ctx.h, n, _dptr(lmk), _dptr(cams), _dptr(x_orth), _dptr(pose), _dptr(angles),
                                 _dptr(info), _stream()))
     return dict(x_orth=x_orth, pose=pose, angles=angles, info=info)
+
+
+def prepare_crops(frames, coef, crop):
+    """Grey FP32 crops of resized 8-bit BGR frames on the GPU (tdi_prepare_crops).  frames: uint8
+    device tensor [n, H, W, 3] (B, G, R); coef: [n] resize coefficients; crop: [n, 4] (x0, y0,
+    width, height) in resized-frame pixels.  Returns a float32 device tensor with the crops
+    concatenated, each column-major (frame i at the prefix sum of width*height)."""
+    import numpy as np
+    torch = _torch()
+    if frames.dtype != torch.uint8 or frames.dim() != 4 or frames.shape[3] != 3 or not frames.is_cuda:
+        raise ValueError("prepare_crops: frames must be a uint8 CUDA tensor [n, H, W, 3]")
+    frames = frames.contiguous()
+    n, H, W = (int(v) for v in frames.shape[:3])
+    coef = np.ascontiguousarray(coef, np.float32).reshape(-1)
+    crop = np.ascontiguousarray(crop, np.int32).reshape(-1, 4)
+    if coef.shape[0] != n or crop.shape[0] != n:
+        raise ValueError("prepare_crops: coef must be [n] and crop [n, 4]")
+    out = torch.empty(int((crop[:, 2].astype(np.int64) * crop[:, 3]).sum()), device=frames.device,
+                      dtype=torch.float32)
+    _check(lib().tdi_prepare_crops(n, _dptr(frames), W, H, H * W * 3, coef.ctypes.data, crop.ctypes.data,
+                                   _dptr(out), _stream()))
+    return out
 
 
 def rotation(u):

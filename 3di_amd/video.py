@@ -40,6 +40,9 @@ def lib():
         L.tdi_read_config.argtypes = [C.c_char_p, _vp, _vp, _vp]
         L.tdi_fit_video_frames.argtypes = [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp]
         L.tdi_initialize_with_orthographic.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_read_landmarks.argtypes = [C.c_char_p, C.c_int32, _vp, _vp, _vp]
+        L.tdi_frame_geometry.argtypes = [C.c_int32, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_float, C.c_int32,
+                                         C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -148,3 +151,59 @@ def initialize_with_orthographic(ctx, lmk_x, lmk_y, cams):
     if rc:
         raise tdi.TdiError(f"tdi_initialize_with_orthographic: {rc}: {tdi.lib().tdi_last_error().decode()}")
     return pose, ang, info
+
+
+def read_landmarks(path: str):
+    """Landmark text file -> (lmk_x, lmk_y) [n, 51] float32 (LandmarkData::init_from_txtfile)."""
+    n = C.c_int32(0)
+    rc = lib().tdi_read_landmarks(path.encode(), 0, None, None, C.byref(n))
+    if rc:
+        raise tdi.TdiError(f"tdi_read_landmarks: {rc}: cannot read {path}")
+    lx = np.zeros((n.value, tdi.NLANDMARKS), np.float32)
+    ly = np.zeros_like(lx)
+    rc = lib().tdi_read_landmarks(path.encode(), n.value, lx.ctypes.data, ly.ctypes.data, C.byref(n))
+    if rc:
+        raise tdi.TdiError(f"tdi_read_landmarks: {rc}: cannot read {path}")
+    return lx[:n.value], ly[:n.value]
+
+
+def frame_geometry(lmk_x, lmk_y, cams, frame_w: int, frame_h: int, ref_face_size: float, pad: bool = True,
+                   crop_size=(256, 256)):
+    """Per-frame resize coefficient, resized landmarks and camera, crop and face size of n frames
+    (tdi_frame_geometry).  Returns a dict of host arrays: coef [n], lmk_x, lmk_y [n, 51],
+    cams [n, 4], crop [n, 4], face_size [n], status [n]."""
+    lx = np.ascontiguousarray(lmk_x, np.float32)
+    ly = np.ascontiguousarray(lmk_y, np.float32)
+    cm = np.ascontiguousarray(cams, np.float32)
+    n = lx.shape[0]
+    if lx.shape != (n, tdi.NLANDMARKS) or ly.shape != lx.shape or cm.shape != (n, 4):
+        raise ValueError("frame_geometry: lmk_x, lmk_y must be [n, 51] and cams [n, 4]")
+    out = dict(coef=np.zeros(n, np.float32), lmk_x=np.zeros_like(lx), lmk_y=np.zeros_like(ly),
+               cams=np.zeros_like(cm), crop=np.zeros((n, 4), np.int32), face_size=np.zeros(n, np.float32),
+               status=np.zeros(n, np.int32))
+    rc = lib().tdi_frame_geometry(n, lx.ctypes.data, ly.ctypes.data, cm.ctypes.data, int(frame_w), int(frame_h),
+                                  float(ref_face_size), int(bool(pad)), int(crop_size[0]), int(crop_size[1]),
+                                  out["coef"].ctypes.data, out["lmk_x"].ctypes.data, out["lmk_y"].ctypes.data,
+                                  out["cams"].ctypes.data, out["crop"].ctypes.data, out["face_size"].ctypes.data,
+                                  out["status"].ctypes.data)
+    if rc:
+        raise tdi.TdiError(f"tdi_frame_geometry: {rc}")
+    return out
+
+
+def prepare_frames(frames, lmk_x, lmk_y, cams, ref_face_size: float, pad: bool = True, crop_size=(256, 256)):
+    """The frame preparation of the video loop for n frames: frame_geometry on the host, then the
+    grey crops of the frames that passed (status 0) on the GPU (tdi.prepare_crops).  frames: uint8
+    CUDA tensor [n, H, W, 3] (B, G, R).  Returns the geometry dict with "obs", the crops of the
+    frames that passed, concatenated column-major in frame order, and "frames", their indices."""
+    n, H, W = (int(v) for v in frames.shape[:3])
+    g = frame_geometry(lmk_x, lmk_y, cams, W, H, ref_face_size, pad, crop_size)
+    keep = np.flatnonzero(g["status"] == 0)
+    g["frames"] = keep
+    if keep.size:
+        import torch
+        sel = frames if keep.size == n else frames[torch.from_numpy(keep).to(frames.device)]
+        g["obs"] = tdi.prepare_crops(sel, g["coef"][keep], g["crop"][keep])
+    else:
+        g["obs"] = None
+    return g

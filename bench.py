@@ -11,7 +11,10 @@ Also measured in the same run:
   * the FP64 OpenMP oracle on a prefix of the same frames on the host cores (cpu_baseline),
     which also gives the batched path's parameter error against FP64;
   * per-stage times with HIP events on the first workspace chunk -> the dominant kernel's
-    roofline entry.
+    roofline entry;
+  * the landmark-driven initialisation (tdi_ortho_init) of every frame;
+  * the frame preparation (tdi_prepare_crops): 8-bit BGR 1280x720 frames in HBM -> the
+    workload's grey crops, against the oracle on a prefix (time and bit-exactness).
 
 Usage: python bench.py [--steps K] [--warmup W] [--frames N] [--gpus 1]
 Prints one JSON line.
@@ -278,6 +281,38 @@ def main():
                          mean_outer_iterations=round(float(init_out["info"][:, 0].float().mean()), 2))
     log(f"[bench] landmark initialisation: {args.frames} frames in {init_ms:.3f} ms")
 
+    # frame preparation (tdi_prepare_crops): 8-bit BGR 1280x720 frames resident in HBM -> the
+    # workload's crop sizes, resize coefficients U(0.5, 1.5)
+    sw, sh = 1280, 720
+    rng = np.random.default_rng(0x3D1 + 7)
+    pcoef = rng.uniform(0.5, 1.5, args.frames).astype(np.float32)
+    pcrop = np.zeros((args.frames, 4), np.int32)
+    for i, f in enumerate(w.frames):
+        dw, dh = int(np.rint(sw * float(pcoef[i]))), int(np.rint(sh * float(pcoef[i])))
+        pcrop[i] = (rng.integers(0, max(dw - f.width, 1)), rng.integers(0, max(dh - f.height, 1)), f.width, f.height)
+    src = torch.randint(0, 256, (args.frames, sh, sw, 3), dtype=torch.uint8, device="cuda")
+    tdi.prepare_crops(src, pcoef, pcrop)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        crops = tdi.prepare_crops(src, pcoef, pcrop)
+    e1.record()
+    torch.cuda.synchronize()
+    prep_ms = e0.elapsed_time(e1) / 5
+    # algorithmic bytes: the crops written plus the source footprint of each crop read once
+    k = pcoef.astype(np.float64)
+    fx0 = np.clip(np.floor((pcrop[:, 0] + 0.5) / k - 0.5), 0, sw - 1)
+    fx1 = np.clip(np.floor((pcrop[:, 0] + pcrop[:, 2] - 0.5) / k - 0.5) + 1, 0, sw - 1)
+    fy0 = np.clip(np.floor((pcrop[:, 1] + 0.5) / k - 0.5), 0, sh - 1)
+    fy1 = np.clip(np.floor((pcrop[:, 1] + pcrop[:, 3] - 0.5) / k - 0.5) + 1, 0, sh - 1)
+    prep_bytes = float((4.0 * pcrop[:, 2] * pcrop[:, 3]).sum() + (3.0 * (fx1 - fx0 + 1) * (fy1 - fy0 + 1)).sum())
+    frame_prep = dict(frames=args.frames, source=f"{sw}x{sh} BGR8", ms=round(prep_ms, 4),
+                      frames_per_s=round(args.frames / (prep_ms * 1e-3), 1), algorithmic_bytes=int(prep_bytes),
+                      achieved_GBs=round(prep_bytes / (prep_ms * 1e-3) / 1e9, 1), peak_GBs=8000.0,
+                      frac=round(prep_bytes / (prep_ms * 1e-3) / 8e12, 4),
+                      note="ms per call includes the per-call descriptor upload and stream synchronisation")
+    log(f"[bench] frame preparation: {args.frames} frames in {prep_ms:.3f} ms")
+
     # FP64 oracle on a prefix: host baseline and parameter error of the batched fit
     cpu = None
     accuracy = None
@@ -300,6 +335,15 @@ def main():
         O.ortho_init(lmk.cpu().numpy(), cams.cpu().numpy(), p0L)
         landmark_init["cpu_frames_per_s"] = round(args.frames / (time.perf_counter() - t0), 1)
         landmark_init["cpu_cores"] = threads
+        npf = min(256, args.frames)
+        host_src = src[:npf].cpu().numpy()
+        t0 = time.perf_counter()
+        want = O.prepare_crops(host_src, pcoef[:npf], pcrop[:npf])
+        frame_prep["cpu_frames_per_s"] = round(npf / (time.perf_counter() - t0), 1)
+        frame_prep["cpu_sample"] = f"first {npf} frames, oracle on {os.environ.get('OMP_NUM_THREADS', 'all')} OpenMP threads"
+        got = crops[:want.size].cpu().numpy()
+        frame_prep["bitexact_vs_oracle"] = bool(np.array_equal(got.view(np.uint32), want.view(np.uint32)))
+    del src, crops
 
     out = {
         "metric": METRIC,
@@ -327,6 +371,7 @@ def main():
         "batched_stop_flags_note": "1 max iterations, 2 decrement test, 4 step below t_min, 8 NaN, 16 not PD",
         "stages": stages,
         "landmark_init": landmark_init,
+        "frame_prep": frame_prep,
     }
     print(json.dumps(out), flush=True)
     batch.close()

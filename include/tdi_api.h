@@ -213,6 +213,19 @@ int tdi_workspace_frames(tdi_batch* b);
  * Mirrors the reference's MEASURE_TIME stage timers (model_fitter.cu:43-58). */
 int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed_flags, int32_t reps, float ms[7]);
 
+/* ---- Frame preparation: replaces InputData::get_resized_frame (input_data.cu:44-53; grey
+ *      conversion, /255, cv::resize by Camera::resize_coef, zero border at the bottom and right)
+ *      and the crop + transpose of VideoFitter::fit_to_video_frame (video_fitter.cu:537-543).
+ * bgr: DEVICE, n frames of src_h rows x src_w pixels x 3 bytes (B, G, R: OpenCV's CV_8UC3
+ * layout), frame i at bgr + i*frame_stride bytes (frame_stride >= 3*src_w*src_h).
+ * coef: HOST [n] resize coefficients; crop: HOST [n][4] (x0, y0, width, height) in pixels of the
+ * resized frame (Renderer::set_x0_short_y0_short, renderer.cu:585-606, gives x0, y0).
+ * out: DEVICE, the grey crops, column-major, frame i at the exclusive prefix sum of
+ * width*height: the raw observation that tdi_frame_desc.obs takes.  Pixels beyond the resized
+ * frame are 0.  Returns when the crops are written. */
+int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
+                      const float* coef, const int32_t* crop, float* out, void* stream);
+
 /* Rendered pixel counts of the batch's last evaluation (host array of n). */
 int tdi_last_pixel_counts(tdi_batch* b, int32_t* npix);
 

@@ -57,6 +57,31 @@ int tdi_initialize_with_orthographic(tdi_ctx* ctx, int32_t n, const float* lmk_x
 int tdi_fit_video_frames(tdi_batch* b, int32_t twostage_illum, int32_t batched, float* theta, float* lam,
                          float* Lint, tdi_fit_report* reports);
 
+/* ---- Frame preparation (host side; the pixels are tdi_prepare_crops in tdi_api.h) ---- */
+
+/* Landmark text file, one frame per line "x0 y0 x1 y1 ... x50 y50", as
+ * LandmarkData::init_from_txtfile reads it (input_data.cu:98-121, read2DVectorFromFile_unknown_size,
+ * funcs.h:352-413): the first line's count of numbers fixes the columns (at least 102), at most
+ * 32767 lines, a value that does not parse reads as NaN.  With lmk_x == NULL only *n_frames is
+ * set; otherwise lmk_x, lmk_y [max_frames][51] receive the first min(max_frames, lines) frames. */
+int tdi_read_landmarks(const char* path, int32_t max_frames, float* lmk_x, float* lmk_y, int32_t* n_frames);
+
+/* Per-frame geometry of the video loop before each fit, for n frames: the landmarks-inside-the-frame
+ * test (video_fitter.cu:180-186), the resize coefficient ref_face_size / face size with the face
+ * size of LandmarkData::get_face_size (input_data.cu:124-138; video_fitter.cu:357-360), the camera
+ * scaled by it (Camera::update_camera, camera.h:162-174), the resized landmarks
+ * (InputData::get_resized_landmarks, input_data.cu:33-41), the crop origin
+ * (Renderer::set_x0_short_y0_short, renderer.cu:585-606; pad = PAD_RENDERING) and the face size
+ * of the resized landmarks (compute_face_size, funcs.cu:44-53).  Host arrays: lmk_x, lmk_y [n][51]
+ * and cams [n][4] (phix, phiy, cx, cy) of the unresized frame of frame_w x frame_h pixels.
+ * Outputs: coef [n], resized landmarks [n][51], cams_out [n][4], crop [n][4] = (x0, y0, crop_w,
+ * crop_h) (the reference's crop is DIMX x DIMY = 256 x 256, constants.h:55-56), face_size [n],
+ * status [n] = TDI_OK or TDI_OUT_OF_FRAME (outputs of such a frame are zero). */
+int tdi_frame_geometry(int32_t n, const float* lmk_x, const float* lmk_y, const float* cams, int32_t frame_w,
+                       int32_t frame_h, float ref_face_size, int32_t pad, int32_t crop_w, int32_t crop_h, float* coef,
+                       float* lmk_x_out, float* lmk_y_out, float* cams_out, int32_t* crop, float* face_size,
+                       int32_t* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,9 @@ def lib():
         L.orc_rotvec_from_euler.argtypes = [_dp, _dp]
         L.orc_euler_angles.argtypes = [_dp, _dp]
         L.orc_ortho_init.argtypes = [C.c_int, _fp, _fp, _dp, _dp, _dp, _dp, _ip]
+        L.orc_prepare_crops.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_longlong, _fp, _ip, _fp]
+        L.orc_resized_size.argtypes = [C.c_int, C.c_float]
+        L.orc_resized_size.restype = C.c_int
         _lib = L
     return _lib
 
@@ -231,3 +234,20 @@ def ortho_init(lmk, cams, p0L):
     info = np.zeros((n, 4), np.int32)
     lib().orc_ortho_init(n, p(lmk, _fp), p(cams, _fp), p(_d(p0L)), p(xo), p(pose), p(ang), p(info, _ip))
     return dict(x_orth=xo, pose=pose, angles=ang, info=info)
+
+
+def prepare_crops(frames, coef, crop):
+    """Grey crops of resized 8-bit BGR frames: frames [n, sh, sw, 3] uint8, coef [n], crop [n, 4]
+    (x0, y0, width, height).  Returns the crops concatenated, each column-major (width*height)."""
+    frames = np.ascontiguousarray(frames, np.uint8)
+    coef = np.ascontiguousarray(coef, np.float32)
+    crop = np.ascontiguousarray(crop, np.int32)
+    n, sh, sw = frames.shape[:3]
+    out = np.zeros(int((crop[:, 2].astype(np.int64) * crop[:, 3]).sum()), np.float32)
+    lib().orc_prepare_crops(n, frames.ctypes.data, sw, sh, sh * sw * 3, p(coef, _fp), p(crop, _ip), p(out, _fp))
+    return out
+
+
+def resized_size(n, coef):
+    """cv::resize's destination size for a source of n pixels at scale coef."""
+    return lib().orc_resized_size(int(n), float(coef))

@@ -8,7 +8,8 @@
  * Everything is computed in double precision except the rasteriser's coverage test,
  * which follows the reference's FP32 operation sequence (funcs.cu:950-979) so that
  * triangle-id/visibility buffers can be compared bit for bit with the GPU given
- * identical FP32 projected vertices.  Build with -ffp-contract=off (oracle/Makefile).
+ * identical FP32 projected vertices, and the frame preparation, which follows OpenCV's
+ * fixed-point grey conversion and FP32 bilinear resize.  Build with -ffp-contract=off (oracle/Makefile).
  *
  * Intended divergences from the reference, shared with the HIP library:
  *   - visibility: packed (depth|triangle-id) minimum, so exact-depth ties resolve to the
@@ -21,7 +22,8 @@
  *     reference (funcs.cu:313), is clamped to sorted position 0.
  *
  * Parity status of each function is documented in the header of tests/test_oracle.py (the
- * landmark initialisation in tests/test_init_oracle.py).
+ * landmark initialisation in tests/test_init_oracle.py, the frame preparation in
+ * tests/test_frames_oracle.py).
  */
 #include <cmath>
 #include <cstdint>
@@ -984,9 +986,95 @@ void ortho_init_one(const float* lx, const float* ly, const float* cam, const do
     ang[0] = yaw * r2d; ang[1] = pitch * r2d; ang[2] = roll * r2d;
 }
 
+// ---------------------------------------------------------------- frame preparation
+// InputData::get_resized_frame (input_data.cu:44-53) and the crop + transpose of
+// VideoFitter::fit_to_video_frame (video_fitter.cu:537-543), on 8-bit BGR frames.  The steps are
+// OpenCV 4 calls (a dependency absent here), restated from OpenCV's published semantics in the
+// FP32 / fixed-point arithmetic OpenCV uses, so this function is not FP64:
+//   cvtColor(BGR2GRAY) on 8-bit data:  grey = (1868 B + 9617 G + 4899 R + 2^13) >> 14;
+//   convertTo(CV_32F) and /255:        float(grey) * float(1/255.)  (convertTo's scale);
+//   resize(fx = fy = coef, INTER_LINEAR): destination size round-half-even(src * coef); equal
+//     sizes copy; otherwise each destination column d has f = float((d + 0.5) * (1/coef) - 0.5),
+//     s = floor(f), w = f - s, with (s, w) = (0, 0) for s < 0 and (src-1, 0) for s >= src-1; each
+//     row likewise, but rows s and s+1 are clamped to the image individually and keep w;
+//     horizontal pass, then vertical, each a*(1-w) + b*w in FP32 with every operation rounded;
+//   copyMakeBorder(0, DIMY, 0, DIMX, zero): pixels past the resized frame are 0;
+//   the crop at (x0, y0), stored column-major.
+int prep_resized_size(int n, float coef) { return (int)std::nearbyint((double)n * (double)coef); }
+
+inline float prep_grey(const uint8_t* frame, int sw, int x, int y)
+{
+    const uint8_t* p = frame + 3 * ((size_t)y * sw + x);
+    const int g = (1868 * (int)p[0] + 9617 * (int)p[1] + 4899 * (int)p[2] + 8192) >> 14;
+    return (float)g * (float)(1.0 / 255.0);
+}
+
+inline float prep_lerp(float a, float b, float w)
+{
+    const float wa = 1.f - w;
+    const float pa = a * wa;
+    const float pb = b * w;
+    return pa + pb;
+}
+
+void prep_crop(const uint8_t* frame, int sw, int sh, float coef, const int32_t* crop, float* out)
+{
+    const int x0 = crop[0], y0 = crop[1], W = crop[2], H = crop[3];
+    const int dw = prep_resized_size(sw, coef), dh = prep_resized_size(sh, coef);
+    const bool copy = dw == sw && dh == sh;
+    const double scale = 1.0 / (double)coef;
+    for (int c = 0; c < W; ++c) {
+        const int X = x0 + c;
+        int sx = 0;
+        float wx = 0.f;
+        if (!copy && X < dw) {
+            const double t = ((double)X + 0.5) * scale;
+            float f = (float)(t - 0.5);
+            sx = (int)std::floor(f);
+            wx = f - (float)sx;
+            if (sx < 0) { sx = 0; wx = 0.f; }
+            if (sx >= sw - 1) { sx = sw - 1; wx = 0.f; }
+        }
+        const int sx1 = std::min(sx + 1, sw - 1);
+        for (int r = 0; r < H; ++r) {
+            const int Y = y0 + r;
+            float v = 0.f;
+            if (X < dw && Y < dh) {
+                if (copy) {
+                    v = prep_grey(frame, sw, X, Y);
+                } else {
+                    const double t = ((double)Y + 0.5) * scale;
+                    float f = (float)(t - 0.5);
+                    const int sy = (int)std::floor(f);
+                    const float wy = f - (float)sy;
+                    const int r0 = std::min(std::max(sy, 0), sh - 1), r1 = std::min(std::max(sy + 1, 0), sh - 1);
+                    const float h0 = prep_lerp(prep_grey(frame, sw, sx, r0), prep_grey(frame, sw, sx1, r0), wx);
+                    const float h1 = prep_lerp(prep_grey(frame, sw, sx, r1), prep_grey(frame, sw, sx1, r1), wx);
+                    v = prep_lerp(h0, h1, wy);
+                }
+            }
+            out[(size_t)c * H + r] = v;
+        }
+    }
+}
+
 } // namespace
 
 extern "C" {
+
+// frames: n frames of [sh][sw][3] bytes, frame_stride bytes apart; coef [n]; crop [n][4] =
+// (x0, y0, width, height); out: the crops column-major, frame i at the prefix sum of width*height.
+void orc_prepare_crops(int n, const uint8_t* frames, int sw, int sh, long long frame_stride, const float* coef,
+                       const int32_t* crop, float* out)
+{
+    std::vector<size_t> off(n + 1, 0);
+    for (int i = 0; i < n; ++i) off[i + 1] = off[i] + (size_t)crop[4 * i + 2] * crop[4 * i + 3];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < n; ++i)
+        prep_crop(frames + (size_t)i * frame_stride, sw, sh, coef[i], crop + 4 * i, out + off[i]);
+}
+
+int orc_resized_size(int n, float coef) { return prep_resized_size(n, coef); }
 
 void orc_ortho_residuals(const double* x, const double* xlb, const double* ylb, const double* p0L, double* e,
                          double* J)

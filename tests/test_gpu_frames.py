@@ -1,0 +1,82 @@
+"""GPU parity of the frame preparation (tdi_prepare_crops) against the oracle: grey crops of
+resized 8-bit BGR frames, bit for bit.  Both sides perform the same FP32 operations in the same
+order (the device code is built without multiply-add contraction for this kernel), so every
+crop pixel must be identical.  Cases: random coefficients from 0.3 to 3 (down- and upscaling),
+the copy case (destination size equal to the source size), coefficient 1, crops that reach past
+the resized frame into the zero border, one-pixel crops, and the host-layer chain
+(landmark geometry, then crops) on synthetic frames.
+"""
+import importlib
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+tdi = importlib.import_module("3di_amd.tdi")
+video = importlib.import_module("3di_amd.video")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _case(rng, n, sh, sw):
+    frames = rng.integers(0, 256, (n, sh, sw, 3), dtype=np.uint8)
+    coef = rng.uniform(0.3, 3.0, n).astype(np.float32)
+    coef[0], coef[1] = 1.0, np.float32(1.0 + 0.2 / sw)      # coefficient 1, and an equal-size copy
+    crop = np.zeros((n, 4), np.int32)
+    for i in range(n):
+        dw, dh = O.resized_size(sw, coef[i]), O.resized_size(sh, coef[i])
+        crop[i, 0], crop[i, 1] = rng.integers(0, max(dw - 8, 1)), rng.integers(0, max(dh - 8, 1))
+        crop[i, 2], crop[i, 3] = rng.integers(1, 400), rng.integers(1, 400)
+    crop[2, 2:] = 1
+    return frames, coef, crop
+
+
+@pytest.mark.parametrize("seed,n,sh,sw", [(1, 24, 120, 160), (2, 16, 481, 643)])
+def test_prepare_crops_bitexact(seed, n, sh, sw):
+    rng = np.random.default_rng(seed)
+    frames, coef, crop = _case(rng, n, sh, sw)
+    want = O.prepare_crops(frames, coef, crop)
+    _say(f"case: prepare_crops seed {seed}, {n} frames {sw}x{sh}, {want.size} crop pixels")
+    got = tdi.prepare_crops(torch.from_numpy(frames).cuda(), coef, crop)
+    torch.cuda.synchronize()
+    got = got.cpu().numpy()
+    bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+    assert bad.size == 0, f"{bad.size} pixels differ, first at {bad[0]}: {got[bad[0]]} vs {want[bad[0]]}"
+
+
+def test_prepare_crops_rejects_bad_arguments():
+    frames = torch.zeros((2, 8, 8, 3), dtype=torch.uint8, device="cuda")
+    with pytest.raises(tdi.TdiError):
+        tdi.prepare_crops(frames, [1.0, 0.0], [[0, 0, 4, 4], [0, 0, 4, 4]])      # coefficient 0
+    with pytest.raises(tdi.TdiError):
+        tdi.prepare_crops(frames, [1.0, 1.0], [[0, 0, 4, 4], [-1, 0, 4, 4]])     # negative origin
+    with pytest.raises(tdi.TdiError):
+        tdi.prepare_crops(frames, [1.0, 1e-3], [[0, 0, 4, 4], [0, 0, 4, 4]])     # empty resized frame
+
+
+def test_host_layer_prepare_frames():
+    rng = np.random.default_rng(5)
+    n, sh, sw = 12, 360, 480
+    frames = rng.integers(0, 256, (n, sh, sw, 3), dtype=np.uint8)
+    cx, cy = rng.uniform(150, 330, n), rng.uniform(120, 240, n)
+    size = rng.uniform(60, 140, n)
+    base = rng.uniform(-0.5, 0.5, (51, 2))
+    lx = (cx[:, None] + size[:, None] * base[None, :, 0]).astype(np.float32)
+    ly = (cy[:, None] + size[:, None] * base[None, :, 1]).astype(np.float32)
+    lx[3, 0] = -5.0                                          # one frame with a landmark outside
+    cams = np.tile(np.array([[900, 900, sw / 2, sh / 2]], np.float32), (n, 1))
+    _say(f"case: host-layer prepare_frames, {n} frames {sw}x{sh}")
+    g = video.prepare_frames(torch.from_numpy(frames).cuda(), lx, ly, cams, 100.0, pad=True, crop_size=(192, 160))
+    torch.cuda.synchronize()
+    keep = g["frames"]
+    assert list(np.flatnonzero(g["status"])) == [3] and keep.size == n - 1
+    want = O.prepare_crops(frames[keep], g["coef"][keep], g["crop"][keep])
+    assert np.array_equal(g["obs"].cpu().numpy().view(np.uint32), want.view(np.uint32))
