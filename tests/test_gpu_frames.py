@@ -3,7 +3,8 @@ resized 8-bit BGR frames, bit for bit.  Both sides perform the same FP32 operati
 order (the device code is built without multiply-add contraction for this kernel), so every
 crop pixel must be identical.  Cases: random coefficients from 0.3 to 3 (down- and upscaling),
 the copy case (destination size equal to the source size), coefficient 1, crops that reach past
-the resized frame into the zero border, one-pixel crops, and the host-layer chain
+the resized frame into the zero border, one-pixel crops, one-pixel-wide and one-pixel-high
+sources, a crop entirely past the resized frame, and the host-layer chain
 (landmark geometry, then crops) on synthetic frames.
 """
 import importlib
@@ -50,6 +51,23 @@ def test_prepare_crops_bitexact(seed, n, sh, sw):
     got = got.cpu().numpy()
     bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
     assert bad.size == 0, f"{bad.size} pixels differ, first at {bad[0]}: {got[bad[0]]} vs {want[bad[0]]}"
+
+
+def test_prepare_crops_edge_cases():
+    # one-pixel-wide and one-pixel-high sources, a crop entirely past the resized frame, and a
+    # large upscale
+    rng = np.random.default_rng(9)
+    for sh, sw in ((1, 37), (29, 1), (1, 1)):
+        frames = rng.integers(0, 256, (3, sh, sw, 3), dtype=np.uint8)
+        coef = np.array([2.5, 7.0, 1.0], np.float32)
+        crop = np.array([[0, 0, 20, 9], [3, 2, 40, 30], [sw + 2, sh + 2, 5, 4]], np.int32)
+        want = O.prepare_crops(frames, coef, crop)
+        _say(f"case: prepare_crops edge source {sw}x{sh}")
+        got = tdi.prepare_crops(torch.from_numpy(frames).cuda(), coef, crop)
+        torch.cuda.synchronize()
+        got = got.cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        assert (got[-20:] == 0).all()
 
 
 def test_prepare_crops_rejects_bad_arguments():
