@@ -404,11 +404,23 @@ extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd,
             set_error("observation staging allocation failed");
             return TDI_ERR_NOMEM;
         }
-        for (int i = 0; i < n; ++i)
-            (void)hipMemcpy(raw + b->hframes[i].obs_off, fd[i].obs, sizeof(float) * fd[i].width * fd[i].height,
-                            hipMemcpyHostToDevice);
+        // an observation is host memory or, as tdi_prepare_crops leaves it, device memory
+        int rc = TDI_OK;
+        for (int i = 0; i < n && !rc; ++i) {
+            hipPointerAttribute_t at{};
+            hipMemcpyKind kind = hipMemcpyHostToDevice;
+            if (hipPointerGetAttributes(&at, fd[i].obs) == hipSuccess && at.type == hipMemoryTypeDevice)
+                kind = hipMemcpyDeviceToDevice;
+            else
+                (void)hipGetLastError();
+            if (hipMemcpy(raw + b->hframes[i].obs_off, fd[i].obs, sizeof(float) * fd[i].width * fd[i].height, kind) !=
+                hipSuccess) {
+                set_error("frame %d: observation copy failed", i);
+                rc = TDI_ERR_HIP;
+            }
+        }
         (void)hipMemcpy(dfr, b->hframes.data(), sizeof(FrameGPU) * n, hipMemcpyHostToDevice);
-        int rc = launch_obs_prepare(dfr, n, c->taps, raw, tmp, blr, b->obs_g, maxA, s);
+        if (!rc) rc = launch_obs_prepare(dfr, n, c->taps, raw, tmp, blr, b->obs_g, maxA, s);
         (void)hipDeviceSynchronize();
         (void)hipFree(raw); (void)hipFree(tmp); (void)hipFree(blr);
         if (rc) { b->mem.release(); delete b; return rc; }

@@ -75,7 +75,8 @@ typedef struct tdi_frame_desc {
     float phix, phiy, cx, cy;          /* pinhole intrinsics in image pixels (camera.h:15-60)   */
     int32_t x0, y0, width, height;     /* crop origin and size in image pixels                   */
     float face_size;                   /* compute_face_size of the landmarks (funcs.cu:44-53)    */
-    const float* obs;                  /* [width][height] raw grey crop, column-major            */
+    const float* obs;                  /* [width][height] raw grey crop, column-major; host or
+                                          device memory (tdi_prepare_crops output)               */
     const float* lmk_x;                /* [51] detected landmarks, image pixels                  */
     const float* lmk_y;
     const float* box_lx;               /* [51] landmark box, face-size units                     */

@@ -98,3 +98,40 @@ def test_host_layer_prepare_frames():
     assert list(np.flatnonzero(g["status"])) == [3] and keep.size == n - 1
     want = O.prepare_crops(frames[keep], g["coef"][keep], g["crop"][keep])
     assert np.array_equal(g["obs"].cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
+def test_batch_accepts_device_observations():
+    """tdi_batch_create reads tdi_frame_desc.obs from device memory as well as from host memory
+    (the crops of tdi_prepare_crops need not leave the GPU): both batches give the same bits."""
+    import ctypes as C
+    from scenes import make_scene
+    abi = importlib.import_module("3di_amd.abi")
+    sc = make_scene(grid=48, k_exp=12, crops=(96, 128, 80), seed=11)
+    _say("case: batch from device observations, 3 frames")
+    ctx = tdi.Context(sc.model, sc.opts)
+    try:
+        host = tdi.Batch(ctx, sc.frames)
+        dev_obs = [torch.from_numpy(np.ascontiguousarray(f.obs, np.float32).reshape(-1)).cuda() for f in sc.frames]
+        fd = abi.FrameDescs(sc.frames)
+        for i, t in enumerate(dev_obs):
+            fd.arr[i].obs = C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_float))
+        h = C.c_void_p()
+        rc = tdi.lib().tdi_batch_create(ctx.h, len(sc.frames), C.addressof(fd.arr), C.byref(h))
+        assert rc == 0, tdi.lib().tdi_last_error()
+        dev = object.__new__(tdi.Batch)
+        dev.ctx, dev.frames, dev.fd, dev.h = ctx, list(sc.frames), fd, h
+        dev.areas, dev.offsets = host.areas, host.offsets
+        th = sc.init().astype(np.float32)
+        _, obj_h, npix_h = host.render(th)
+        _, obj_d, npix_d = dev.render(th)
+        torch.cuda.synchronize()
+        assert np.array_equal(obj_h.cpu().numpy(), obj_d.cpu().numpy())
+        assert np.array_equal(npix_h.cpu().numpy(), npix_d.cpu().numpy())
+        H_h, g_h, _ = host.normal_equations(th)[:3]
+        H_d, g_d, _ = dev.normal_equations(th)[:3]
+        torch.cuda.synchronize()
+        assert np.array_equal(g_h.cpu().numpy(), g_d.cpu().numpy())
+        dev.close()
+        host.close()
+    finally:
+        ctx.close()
