@@ -9,8 +9,8 @@
 //   - grey = (1868 B + 9617 G + 4899 R + 2^13) >> 14, the BT.601 weights in 14-bit fixed point;
 //   - value = float(grey) * float(1/255), the scale of convertTo;
 //   - resized size round-half-even(src * coef); when it equals the source size the frame is
-//     copied; otherwise INTER_LINEAR with half-pixel centres: f = float((d + 0.5) / coef - 0.5),
-//     s = floor(f), weight f - s; columns left of 0 or at/after the last take the edge pixel with
+//     copied; otherwise INTER_LINEAR with half-pixel centres: f = float((d + 0.5) * (1/coef) - 0.5)
+//     evaluated in double, s = floor(f), weight f - s; columns left of 0 or at/after the last take the edge pixel with
 //     weight 0, rows are clamped individually and keep their weights; the horizontal pass first,
 //     each pass a*(1-w) + b*w with both products and the sum rounded (contraction is off
 //     in this file's device code: no fused multiply-add).
