@@ -10,10 +10,10 @@
 //   - value = float(grey) * float(1/255), the scale of convertTo;
 //   - resized size round-half-even(src * coef); when it equals the source size the frame is
 //     copied; otherwise INTER_LINEAR with half-pixel centres: f = float((d + 0.5) * (1/coef) - 0.5)
-//     evaluated in double, s = floor(f), weight f - s; columns left of 0 or at/after the last take the edge pixel with
-//     weight 0, rows are clamped individually and keep their weights; the horizontal pass first,
-//     each pass a*(1-w) + b*w with both products and the sum rounded (contraction is off
-//     in this file's device code: no fused multiply-add).
+//     evaluated in double, s = floor(f), weight f - s; columns left of 0 or at/after the last
+//     take the edge pixel with weight 0, rows are clamped individually and keep their weights;
+//     the horizontal pass first, each pass a*(1-w) + b*w with both products and the sum rounded
+//     (contraction is off in this file's device code: no fused multiply-add).
 #include <cmath>
 
 #include "common.h"
@@ -162,7 +162,10 @@ extern "C" int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, i
         tile_start[i] = (int)tiles;
         tiles += (long long)F.tiles_y * ((F.w + PREP_TILE - 1) / PREP_TILE);
         dst += (long long)F.w * F.h;
-        if (tiles > (1ll << 24) - 1) { set_error("tdi_prepare_crops: too many crop pixels for one call"); return TDI_ERR_ARG; }
+        if (tiles > (1ll << 24) - 1) {
+            set_error("tdi_prepare_crops: too many crop pixels for one call");
+            return TDI_ERR_ARG;
+        }
     }
     hipStream_t s = (hipStream_t)stream;
     PrepFrame* dfr = nullptr;

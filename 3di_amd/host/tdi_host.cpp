@@ -473,9 +473,10 @@ int tdi_frame_geometry(int32_t n, const float* lmk_x, const float* lmk_y, const 
     const int NL = TDI_NLANDMARKS;
     for (int f = 0; f < n; ++f) {
         const size_t o = (size_t)f * NL;
-        status[f] = tdi_host::frame_geometry(lmk_x + o, lmk_y + o, cams + 4 * (size_t)f, frame_w, frame_h, ref_face_size,
-                                             pad != 0, crop_w, crop_h, coef + f, lmk_x_out + o, lmk_y_out + o,
-                                             cams_out + 4 * (size_t)f, crop + 4 * (size_t)f, face_size + f);
+        status[f] = tdi_host::frame_geometry(lmk_x + o, lmk_y + o, cams + 4 * (size_t)f, frame_w, frame_h,
+                                             ref_face_size, pad != 0, crop_w, crop_h, coef + f, lmk_x_out + o,
+                                             lmk_y_out + o, cams_out + 4 * (size_t)f, crop + 4 * (size_t)f,
+                                             face_size + f);
         if (status[f]) {
             coef[f] = 0.f;
             face_size[f] = 0.f;
