@@ -185,3 +185,29 @@ def test_frame_geometry_known_answers():
     # a landmark outside the frame: the frame is skipped
     g = video.frame_geometry(lx, ly, cams, 170, 480, 200.0)
     assert g["status"][0] == 6 and g["coef"][0] == 0
+
+
+def test_prepare_crops_rejects_bad_arguments_before_any_device_call():
+    tdi = importlib.import_module("3di_amd.tdi")
+    import ctypes as C
+    L = tdi.lib()
+    fake = C.c_void_p(1)            # never dereferenced: validation fails first
+    good_crop = np.array([[0, 0, 8, 8]], np.int32)
+    cases = [([0.0], good_crop, 16), ([np.inf], good_crop, 16),          # coefficient
+             ([1.0], np.array([[-1, 0, 8, 8]], np.int32), 16),                 # crop origin
+             ([1.0], np.array([[0, 0, 0, 8]], np.int32), 16),                  # crop size
+             ([1.0], good_crop, 0),                                            # source width
+             ([1e-4], good_crop, 16)]                                          # empty resized frame
+    for coef, crop, sw in cases:
+        c = np.array(coef, np.float32)
+        rc = L.tdi_prepare_crops(1, fake, sw, 16, 3 * 16 * 16, c.ctypes.data, crop.ctypes.data, fake, None)
+        assert rc == 1, (coef, crop, sw)    # TDI_ERR_ARG
+    assert L.tdi_prepare_crops(0, None, 16, 16, 768, None, None, None, None) == 0
+
+
+def test_host_frame_functions_reject_bad_arguments(tmp_path):
+    _host_or_skip()
+    with pytest.raises(Exception):
+        video.read_landmarks(str(tmp_path / "missing.txt"))
+    with pytest.raises(Exception):
+        video.frame_geometry(np.zeros((1, 51)), np.zeros((1, 51)), np.zeros((1, 4)), 640, 480, 0.0)
