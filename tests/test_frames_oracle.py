@@ -11,7 +11,8 @@ semantics.  How each part is pinned:
   - resize: the known 1x2 -> 1x4 upsampling [0, 1] -> [0, .25, .75, 1] in both directions; the
     copy when the destination size equals the source size; linear images reproduced inside the
     frame (bilinear interpolation is exact on them); an independent float64 bilinear with
-    half-pixel centres and clamped edges agrees to 1e-6 on random frames;
+    half-pixel centres and clamped edges agrees to 1e-6 on random frames (sources of 1-97 pixels
+    a side, scales 0.2-6, crops reaching past the frame);
   - crop layout: column-major, origin (x0, y0), zero beyond the resized frame.
 Left open ("parity unpinned" against OpenCV itself): OpenCV builds whose SIMD paths fuse the
 multiply-adds of the resize round differently in the last bit, and the exact 2x downscale
@@ -211,3 +212,29 @@ def test_host_frame_functions_reject_bad_arguments(tmp_path):
         video.read_landmarks(str(tmp_path / "missing.txt"))
     with pytest.raises(Exception):
         video.frame_geometry(np.zeros((1, 51)), np.zeros((1, 51)), np.zeros((1, 4)), 640, 480, 0.0)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_sources_and_scales_agree_with_a_float64_bilinear(seed):
+    # sources from 1 pixel to 97 pixels a side, scales 0.2-6, crops anywhere (also past the frame)
+    rng = np.random.default_rng(100 + seed)
+    sh, sw = (int(v) for v in rng.integers(1, 98, 2))
+    n = 4
+    fr = rng.integers(0, 256, (n, sh, sw, 3), dtype=np.uint8)
+    coef = np.exp(rng.uniform(np.log(0.2), np.log(6.0), n)).astype(f32)
+    coef = np.where([O.resized_size(sw, c) >= 1 and O.resized_size(sh, c) >= 1 for c in coef], coef, f32(1.0))
+    crop = np.stack([rng.integers(0, 3 * sw, n), rng.integers(0, 3 * sh, n), rng.integers(1, 90, n),
+                     rng.integers(1, 90, n)], 1).astype(np.int32)
+    got = _crops(fr, coef, crop)
+    for i in range(n):
+        dw, dh = O.resized_size(sw, coef[i]), O.resized_size(sh, coef[i])
+        grey = _grey(fr[i]) / 255.0
+        if dw == sw and dh == sh:           # equal sizes: a copy
+            want = np.zeros((crop[i, 3], crop[i, 2]))
+            x0, y0, w, h = crop[i]
+            ys, xs = slice(y0, min(y0 + h, sh)), slice(x0, min(x0 + w, sw))
+            part = grey[ys, xs]
+            want[: part.shape[0], : part.shape[1]] = part
+        else:
+            want = _bilinear64(grey, float(coef[i]), *crop[i])
+        assert np.allclose(got[i], want, atol=1e-6), (seed, i)
