@@ -4,7 +4,8 @@ order (the device code is built without multiply-add contraction for this kernel
 crop pixel must be identical.  Cases: random coefficients from 0.3 to 3 (down- and upscaling),
 the copy case (destination size equal to the source size), coefficient 1, crops that reach past
 the resized frame into the zero border, one-pixel crops, one-pixel-wide and one-pixel-high
-sources, a crop entirely past the resized frame, and the host-layer chain
+sources, a crop entirely past the resized frame, random sources of 1-97 pixels a side at scales
+0.2-6, and the host-layer chain
 (landmark geometry, then crops) on synthetic frames.
 """
 import importlib
@@ -135,3 +136,22 @@ def test_batch_accepts_device_observations():
         host.close()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_prepare_crops_random_sources_bitexact(seed):
+    # sources from 1 to 97 pixels a side, scales 0.2-6, crops anywhere (also past the frame)
+    rng = np.random.default_rng(100 + seed)
+    sh, sw = (int(v) for v in rng.integers(1, 98, 2))
+    n = 8
+    frames = rng.integers(0, 256, (n, sh, sw, 3), dtype=np.uint8)
+    coef = np.exp(rng.uniform(np.log(0.2), np.log(6.0), n)).astype(np.float32)
+    coef = np.where([O.resized_size(sw, c) >= 1 and O.resized_size(sh, c) >= 1 for c in coef], coef,
+                    np.float32(1.0)).astype(np.float32)
+    crop = np.stack([rng.integers(0, 3 * sw, n), rng.integers(0, 3 * sh, n), rng.integers(1, 90, n),
+                     rng.integers(1, 90, n)], 1).astype(np.int32)
+    want = O.prepare_crops(frames, coef, crop)
+    _say(f"case: prepare_crops random seed {seed}, {n} frames {sw}x{sh}")
+    got = tdi.prepare_crops(torch.from_numpy(frames).cuda(), coef, crop)
+    torch.cuda.synchronize()
+    assert np.array_equal(got.cpu().numpy().view(np.uint32), want.view(np.uint32))
