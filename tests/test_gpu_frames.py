@@ -5,8 +5,7 @@ crop pixel must be identical.  Cases: random coefficients from 0.3 to 3 (down- a
 the copy case (destination size equal to the source size), coefficient 1, crops that reach past
 the resized frame into the zero border, one-pixel crops, one-pixel-wide and one-pixel-high
 sources, a crop entirely past the resized frame, random sources of 1-97 pixels a side at scales
-0.2-6, and the host-layer chain
-(landmark geometry, then crops) on synthetic frames.
+0.2-6, and the host-layer chain (landmark geometry, then crops) on synthetic frames.
 """
 import importlib
 import sys
