@@ -172,6 +172,10 @@ int tdi_ctx_create(const tdi_model_desc* md, const tdi_fit_options* opts, int de
         set_error("model: empty mesh or missing arrays");
         return TDI_ERR_ARG;
     }
+    if (opts->fit_light < 0 || opts->fit_light > TDI_LIGHT_SH2) {
+        set_error("fit_light %d: 0, TDI_LIGHT_POINT or TDI_LIGHT_SH2", opts->fit_light);
+        return TDI_ERR_ARG;
+    }
     if ((opts->fit_exp && (md->k_exp <= 0 || !md->exp_basis || !md->exp_lb || !md->exp_ub)) ||
         (opts->fit_id && (md->k_id <= 0 || !md->id_basis || !md->id_lb || !md->id_ub)) ||
         (opts->fit_tex && (md->k_tex <= 0 || !md->tex_basis || !md->tex_lb || !md->tex_ub))) {

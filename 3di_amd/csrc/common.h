@@ -27,7 +27,7 @@ inline Layout make_layout(const tdi_model_desc* m, const tdi_fit_options* o)
     L.ntex = o->fit_tex ? m->k_tex : 0;
     L.nid = o->fit_id ? m->k_id : 0;
     L.nexp = o->fit_exp ? m->k_exp : 0;
-    L.nlight = o->fit_light ? 4 : 0;
+    L.nlight = o->fit_light == TDI_LIGHT_SH2 ? 9 : (o->fit_light ? 4 : 0);
     L.off_tex = 0;
     L.off_id = L.ntex;
     L.off_exp = L.off_id + L.nid;
@@ -127,6 +127,18 @@ __device__ __forceinline__ bool cover_rn(float bx, float by, float cx, float cy,
     return a > 0.f && b > 0.f && (a + b <= 1.f);
 }
 
+// The second-order light terms (TDI_LIGHT_SH2) of the camera-space unit normal n:
+// Y = (nx ny, ny nz, nx nz, nx^2 - ny^2, 3 nz^2 - 1).  Y_k alone (s = e_k) is the Jacobian
+// column of s_k divided by the albedo.
+__device__ __forceinline__ void sh2_terms(const float* n, float* Y)
+{
+    Y[0] = n[0] * n[1];
+    Y[1] = n[1] * n[2];
+    Y[2] = n[0] * n[2];
+    Y[3] = n[0] * n[0] - n[1] * n[1];
+    Y[4] = 3.f * n[2] * n[2] - 1.f;
+}
+
 // 1 - alpha - beta rounded as the reference does ((1 - alpha) - beta in FP32).
 __device__ __forceinline__ float gamma_rn(float al, float be)
 {
@@ -192,7 +204,37 @@ __device__ inline void rodrigues_f32(const float* u, float* R, float* dR)
 }
 
 // Rodrigues rotation in FP64 (used for the vertex transform so that projected vertices come
-// out correctly rounded to FP32).
+// out correctly rounded to FP32), and its derivatives dR/du_i in FP64 (same formula as
+// rodrigues_f32) for the barrier's landmark Jacobian.
+__device__ inline void rodrigues_f64_d(const float* uf, double* R, double* dR)
+{
+    const double u[3] = {uf[0], uf[1], uf[2]};
+    const double th = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double N[9] = {0, u[2] / th, -u[1] / th, -u[2] / th, 0, u[0] / th, u[1] / th, -u[0] / th, 0};
+    const double s = sin(th), c1 = 1.0 - cos(th);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; ++k) acc += N[k * 3 + r] * N[c * 3 + k];
+            R[c * 3 + r] = c1 * acc + s * N[c * 3 + r] + (r == c ? 1.0 : 0.0);
+        }
+    const double U[9] = {0, u[2], -u[1], -u[2], 0, u[0], u[1], -u[0], 0};
+    const double inv = 1.0 / (th * th);
+    for (int i = 0; i < 3; ++i) {
+        const double e0 = -R[i * 3 + 0] + (i == 0), e1 = -R[i * 3 + 1] + (i == 1), e2 = -R[i * 3 + 2] + (i == 2);
+        const double w0 = u[1] * e2 - u[2] * e1, w1 = u[2] * e0 - u[0] * e2, w2 = u[0] * e1 - u[1] * e0;
+        const double A[9] = {u[i] * U[0], u[i] * U[1] + w2, u[i] * U[2] - w1,
+                             u[i] * U[3] - w2, u[i] * U[4], u[i] * U[5] + w0,
+                             u[i] * U[6] + w1, u[i] * U[7] - w0, u[i] * U[8]};
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) {
+                double acc = 0.0;
+                for (int k = 0; k < 3; ++k) acc += A[k * 3 + r] * R[c * 3 + k];
+                dR[9 * i + c * 3 + r] = acc * inv;
+            }
+    }
+}
+
 __device__ inline void rodrigues_f64(const float* uf, double* R)
 {
     const double u[3] = {uf[0], uf[1], uf[2]};

@@ -6,7 +6,8 @@
 //   ..._dI_dbeta (383-403): dI/dtheta_j = -(gx dWx/dtheta_j + gy dWy/dtheta_j) for the
 //   geometric unknowns (W = pinhole projection of v = a P_b + b P_c + g P_a), and
 //   T_j (1 + L Id) for albedo; the light columns reuse dI_dlambda = L rho R n and
-//   dI_dLintensity = rho Id (newfuncs.cu:151-155);
+//   dI_dLintensity = rho Id (newfuncs.cu:151-155); with nine light unknowns (a, L, s) the
+//   columns are rho R n, rho Id (lambda = 0) and rho Y_k(R n) (sh2_terms);
 //   fill_optimization_auxiliary_variables_phase2_new (newfuncs.cu:417-463):
 //   J_x = c dy s, J_y = -c dx s, s = dy (dI[r]-dI[l]) - dx (dI[b]-dI[a]), c = 1/(0.001+h^3),
 //   with the compacted-neighbour positions of fill_krels/fill_krels2 (funcs.cu:288-340);
@@ -36,19 +37,25 @@ __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirs
 
 struct FrameRot {
     float R[9], dR[27], tau[3], lam[3], L;
+    float a[3], sh[5];   // directional and second-order light terms (zero unless nine light unknowns)
 };
 
 __device__ inline void load_frame_rot(const Layout& L, const FrameGPU& fr, const float* tf, FrameRot& s)
 {
     rodrigues_f32(tf + L.off_pose + 3, s.R, s.dR);
     for (int a = 0; a < 3; ++a) s.tau[a] = tf[L.off_pose + a];
-    if (L.nlight) {
+    if (L.nlight == 4) {
         for (int a = 0; a < 3; ++a) s.lam[a] = tf[L.off_light + a];
+        s.L = tf[L.off_light + 3];
+    } else if (L.nlight == 9) {
+        for (int a = 0; a < 3; ++a) s.lam[a] = 0.f;   // Id = -(R n).v_c; a enters as (R n).a
         s.L = tf[L.off_light + 3];
     } else {
         for (int a = 0; a < 3; ++a) s.lam[a] = fr.lam[a];
         s.L = fr.Lint;
     }
+    for (int k = 0; k < 3; ++k) s.a[k] = L.nlight == 9 ? tf[L.off_light + k] : 0.f;
+    for (int k = 0; k < 5; ++k) s.sh[k] = L.nlight == 9 ? tf[L.off_light + 4 + k] : 0.f;
 }
 
 constexpr int DI_PX_PER_WG = 256;   // pixels per workgroup and frame in k_dI
@@ -111,7 +118,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const float q1 = sr.R[3] * c0 + sr.R[4] * c1 + sr.R[5] * c2;
         const float q2 = sr.R[6] * c0 + sr.R[7] * c1 + sr.R[8] * c2;
         // shading quantities for albedo and light columns
-        float rho = 0.f, Id = 0.f, rn[3] = {0.f, 0.f, 0.f};
+        float rho = 0.f, Id = 0.f, rn[3] = {0.f, 0.f, 0.f}, Y[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, S = 0.f;
         if (L.ntex || L.nlight) {
             const float* B0 = P0f + 3 * vb;
             const float* C0 = P0f + 3 * vc;
@@ -136,6 +143,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             rn[0] = sr.R[0] * nx + sr.R[3] * ny + sr.R[6] * nz;
             rn[1] = sr.R[1] * nx + sr.R[4] * ny + sr.R[7] * nz;
             rn[2] = sr.R[2] * nx + sr.R[5] * ny + sr.R[8] * nz;
+            if (L.nlight == 9) {
+                sh2_terms(rn, Y);
+                S = rn[0] * sr.a[0] + rn[1] * sr.a[1] + rn[2] * sr.a[2];
+#pragma unroll
+                for (int k = 0; k < 5; ++k) S += sr.sh[k] * Y[k];
+            }
         }
         float* out = dI + (size_t)(base + pos) * K;
         // expression columns, 80 per group: every basis gather of the group is issued before
@@ -182,7 +195,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 val = c0 * dvx + c1 * dvy + c2 * dvz;
             } else if (j >= L.off_light) {
                 const int c = j - L.off_light;
-                val = (c < 3) ? sr.L * rho * rn[c] : rho * Id;
+                val = (c < 3) ? (L.nlight == 9 ? rho : sr.L * rho) * rn[c] : (c == 3) ? rho * Id : rho * Y[min(c - 4, 4)];
             } else if (j >= L.off_id) {
                 const int c = j - L.off_id;
                 const float* E = m.id_basis;
@@ -195,7 +208,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 const int c = j - L.off_tex;
                 const float tt = al * m.tex_basis[(size_t)vb * m.tex_ld + c] + be * m.tex_basis[(size_t)vc * m.tex_ld + c]
                                + ga * m.tex_basis[(size_t)va * m.tex_ld + c];
-                val = tt + sr.L * Id * tt;
+                val = tt + sr.L * Id * tt + S * tt;
             }
             out[j] = val;
         }
@@ -334,7 +347,7 @@ __global__ void __launch_bounds__(256) k_jtj_f32(Layout L, const FrameGPU* __res
     }
 }
 
-// Sum the S partials; tiles are 4x4 (f32 kernel) or 16x16 (MFMA kernel), upper triangle.
+// Sum the S partials in FP64; tiles are 4x4 (f32 kernel) or 16x16 (MFMA kernel), upper triangle.
 __global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ act, int n, int K, int KP, int TS, int S,
                                                      const float* __restrict__ Hp, const float* __restrict__ gp,
                                                      float w, float* __restrict__ H, float* __restrict__ g,
@@ -351,12 +364,12 @@ __global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ act,
             if (a / TS > b / TS) { const int t = a; a = b; b = t; }
             const int tile = tri_index(a / TS, b / TS, T);
             const int off = (a % TS) * TS + (b % TS);
-            float sum = 0.f;
+            double sum = 0.0;
             for (int s = 0; s < S; ++s) sum += Hp[(((size_t)f * S + s) * ntiles + tile) * TS * TS + off];
-            H[(size_t)f * K * K + e] = w * sum;
+            H[(size_t)f * K * K + e] = (float)(w * sum);
         } else {
             const int j = e - K * K;
-            float sum = 0.f;
+            double sum = 0.0;
             if (g_from_tiles) {
                 // J^T r lives in column K of the augmented product: entry (j, K)
                 const int tile = tri_index(j / TS, K / TS, T);
@@ -365,7 +378,7 @@ __global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ act,
             } else {
                 for (int s = 0; s < S; ++s) sum += gp[((size_t)f * S + s) * KP + j];
             }
-            g[(size_t)f * K + j] = -w * sum;
+            g[(size_t)f * K + j] = (float)(-w * sum);
         }
     }
     }

@@ -4,6 +4,8 @@
 //   fill_diffuse_component_and_populate_texture_and_shape (newfuncs.cu:18-167):
 //       n = normalise((P0_b - P0_a) x (P0_c - P0_a)), p0 = a P0_b + b P0_c + g P0_a,
 //       Id = n.(R^T (lambda - tau)) - n.p0, I = rho (1 + L Id), rho barycentric albedo;
+//       with nine light unknowns (a, L, s) the factor is 1 + (R n).a + L Id + sum_k s_k Y_k(R n)
+//       with lambda = 0 in Id (sh2_terms);
 //   fill_tex_im1 (funcs.cu:800-817): scatter I into the crop (zero elsewhere);
 //   convolutionRowsGPU/ColumnsGPU (newfuncs.cu:539-715): separable 15-tap Gaussian,
 //       zero boundary (here both passes in one LDS-tiled kernel);
@@ -93,9 +95,13 @@ __global__ void __launch_bounds__(256) k_clear_crops(const FrameGPU* __restrict_
     }
 }
 
-// Per-frame shading constants: R (column-major), lambda - tau, L, the albedo coefficients.
+// Per-frame shading constants: R (column-major), lambda - tau, L; with nine light unknowns
+// (nsh = 5) lambda = 0, la = R^T a (the directional term in model space) and the second-order
+// terms s.
 struct ShadeFrame {
     float R[9], dl[3], L;
+    float la[3], sh[5];
+    int nsh;
 };
 
 __device__ __forceinline__ void shade_frame_setup(const Layout& L, const FrameGPU& fr, const float* tf, ShadeFrame& sf)
@@ -103,12 +109,21 @@ __device__ __forceinline__ void shade_frame_setup(const Layout& L, const FrameGP
     rodrigues_f32(tf + L.off_pose + 3, sf.R, nullptr);
     float lam[3] = {fr.lam[0], fr.lam[1], fr.lam[2]};
     float Li = fr.Lint;
-    if (L.nlight) {
+    sf.nsh = L.nlight == 9 ? 5 : 0;
+    if (L.nlight == 4) {
         lam[0] = tf[L.off_light]; lam[1] = tf[L.off_light + 1]; lam[2] = tf[L.off_light + 2];
+        Li = tf[L.off_light + 3];
+    } else if (sf.nsh) {
+        lam[0] = lam[1] = lam[2] = 0.f;
         Li = tf[L.off_light + 3];
     }
     for (int a = 0; a < 3; ++a) sf.dl[a] = lam[a] - tf[L.off_pose + a];
     sf.L = Li;
+    for (int k = 0; k < 5; ++k) sf.sh[k] = sf.nsh ? tf[L.off_light + 4 + k] : 0.f;
+    for (int r = 0; r < 3; ++r)
+        sf.la[r] = sf.nsh ? sf.R[3 * r] * tf[L.off_light] + sf.R[3 * r + 1] * tf[L.off_light + 1]
+                                + sf.R[3 * r + 2] * tf[L.off_light + 2]
+                          : 0.f;
 }
 
 // Per-pixel inputs of the shading: the triangle's model-space corners and corner albedos.
@@ -160,7 +175,15 @@ __device__ __forceinline__ float shade_eval(const ModelGPU& m, const Layout& L, 
         ra = fmaf(m.tex_basis[(size_t)va * m.tex_ld + q], stex[q], ra);
     }
     const float rho = al * rb + be * rc + ga * ra;
-    return rho + sf.L * rho * Id;
+    if (!sf.nsh) return rho + sf.L * rho * Id;
+    const float nc[3] = {sf.R[0] * nx + sf.R[3] * ny + sf.R[6] * nz, sf.R[1] * nx + sf.R[4] * ny + sf.R[7] * nz,
+                         sf.R[2] * nx + sf.R[5] * ny + sf.R[8] * nz};
+    float Y[5];
+    sh2_terms(nc, Y);
+    float S = nx * sf.la[0] + ny * sf.la[1] + nz * sf.la[2];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) S += sf.sh[k] * Y[k];
+    return rho + sf.L * rho * Id + rho * S;
 }
 
 __device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L, const float* __restrict__ P0f,

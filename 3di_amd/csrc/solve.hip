@@ -24,8 +24,12 @@ namespace {
 
 constexpr int BMAXK = 112;
 
-// mode 0: objective only -> obj[f] = objb[f] = barrier value (FP64)
+// mode 0: objective only -> obj[f] = objb[f] = barrier value
 // mode 1: derivatives only -> H[f] += barrier Hessian, g[f] += barrier gradient
+// Everything is FP64 (rotation and its derivatives, landmark positions, constraint values,
+// weights, projected-landmark derivatives, sums); the FP32 H and g receive one rounding each.
+// Dynamic LDS: the projected-landmark derivatives dx, dy as double [51][K] each, scaled by the
+// square roots of the Gauss-Newton weights, so that the landmark block is sum_i ex ex^T + ey ey^T.
 __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                                   const int* __restrict__ act, int n, const float* __restrict__ theta,
                                                   LmkArrays lm, int use_lmk, int mode, double* __restrict__ obj,
@@ -38,63 +42,65 @@ __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const Fra
     const int K = L.K;
     const FrameGPU fr = frames[f];
     const float* tf = theta + (size_t)f * K;
-    __shared__ float sR[9], sdR[27];
-    __shared__ float sv[NLMK][3], sp[NLMK][3];
-    __shared__ float swx[NLMK], swy[NLMK], sgx[NLMK], sgy[NLMK];
-    __shared__ float sdx[NLMK][BMAXK + 1], sdy[NLMK][BMAXK + 1];
+    extern __shared__ double sdyn[];
+    double* sdx = sdyn;                    // [NLMK][K], sqrt(wx_i) dx_i/dtheta
+    double* sdy = sdyn + (size_t)NLMK * K; // [NLMK][K]
+    __shared__ double sR[9], sdR[27];
+    __shared__ double sv[NLMK][3], sp[NLMK][3];
+    __shared__ double swx[NLMK], swy[NLMK], sgx[NLMK], sgy[NLMK];
     __shared__ double sred[4];
     const int tid = threadIdx.x;
-    if (tid == 0) rodrigues_f32(tf + L.off_pose + 3, sR, sdR);
+    if (tid == 0) rodrigues_f64_d(tf + L.off_pose + 3, sR, sdR);
     __syncthreads();
     double acc = 0.0;
     // coefficient bounds (only finite bounds take part)
     for (int j = tid; j < K; j += blockDim.x) {
-        const float lb = m.lb[j], ub = m.ub[j];
-        if (!(ub < INFINITY) || !(lb > -INFINITY)) continue;
-        const float x = tf[j];
-        const float fub = x - ub, flb = lb - x;
+        const float lbf = m.lb[j], ubf = m.ub[j];
+        if (!(ubf < INFINITY) || !(lbf > -INFINITY)) continue;
+        const double x = tf[j], lb = lbf, ub = ubf;
+        const double fub = x - ub, flb = lb - x;
         if (mode == 0) {
-            acc += -log(-(double)fub) - log(-(double)flb);
+            acc += -log(-fub) - log(-flb);
         } else {
-            g[(size_t)f * K + j] += -1.f / fub + 1.f / flb;
-            const float a = -1.f / (fub + 1e-12f), b = 1.f / (flb + 1e-12f);
-            H[(size_t)f * K * K + (size_t)j * K + j] += a * a + b * b;
+            g[(size_t)f * K + j] += (float)(-1.0 / fub + 1.0 / flb);
+            const double a = -1.0 / (fub + 1e-12), b = 1.0 / (flb + 1e-12);
+            H[(size_t)f * K * K + (size_t)j * K + j] += (float)(a * a + b * b);
         }
     }
     if (use_lmk) {
         if (tid < NLMK) {
             const int i = tid, v = m.lmk_ids[i];
-            float p[3];
+            double p[3];
             for (int a = 0; a < 3; ++a) {
                 const int r = 3 * v + a;
-                float s = m.mean[r];
-                for (int k = 0; k < L.nexp; ++k) s = fmaf(m.exp_basis[(size_t)r * m.exp_ld + k], tf[L.off_exp + k], s);
-                for (int k = 0; k < L.nid; ++k) s = fmaf(m.id_basis[(size_t)r * m.id_ld + k], tf[L.off_id + k], s);
+                double s = m.mean[r];
+                for (int k = 0; k < L.nexp; ++k) s = fma((double)m.exp_basis[(size_t)r * m.exp_ld + k], (double)tf[L.off_exp + k], s);
+                for (int k = 0; k < L.nid; ++k) s = fma((double)m.id_basis[(size_t)r * m.id_ld + k], (double)tf[L.off_id + k], s);
                 p[a] = s;
                 sp[i][a] = s;
             }
-            const float vx = sR[0] * p[0] + sR[3] * p[1] + sR[6] * p[2] + tf[L.off_pose + 0];
-            const float vy = sR[1] * p[0] + sR[4] * p[1] + sR[7] * p[2] + tf[L.off_pose + 1];
-            const float vz = sR[2] * p[0] + sR[5] * p[1] + sR[8] * p[2] + tf[L.off_pose + 2];
+            const double vx = sR[0] * p[0] + sR[3] * p[1] + sR[6] * p[2] + tf[L.off_pose + 0];
+            const double vy = sR[1] * p[0] + sR[4] * p[1] + sR[7] * p[2] + tf[L.off_pose + 1];
+            const double vz = sR[2] * p[0] + sR[5] * p[1] + sR[8] * p[2] + tf[L.off_pose + 2];
             sv[i][0] = vx; sv[i][1] = vy; sv[i][2] = vz;
-            const float xproj = fr.phix * vx / vz + fr.cx;
-            const float yproj = fr.phiy * vy / vz + fr.cy;
+            const double xproj = fr.phix * vx / vz + fr.cx;
+            const double yproj = fr.phiy * vy / vz + fr.cy;
             const float* L6 = lm.base + (size_t)f * 6 * NLMK;
-            const float lx = L6[i], ly = L6[NLMK + i];
-            const float blx = L6[2 * NLMK + i], bux = L6[3 * NLMK + i];
-            const float bly = L6[4 * NLMK + i], buy = L6[5 * NLMK + i];
-            const float fs = fr.face_size;
-            const float Gxm = xproj - (lx + blx * fs);
-            const float Gym = yproj - (ly + bly * fs);
-            const float Gxp = (lx + bux * fs) - xproj;
-            const float Gyp = (ly + buy * fs) - yproj;
+            const double lx = L6[i], ly = L6[NLMK + i];
+            const double blx = L6[2 * NLMK + i], bux = L6[3 * NLMK + i];
+            const double bly = L6[4 * NLMK + i], buy = L6[5 * NLMK + i];
+            const double fs = fr.face_size;
+            const double Gxm = xproj - (lx + blx * fs);
+            const double Gym = yproj - (ly + bly * fs);
+            const double Gxp = (lx + bux * fs) - xproj;
+            const double Gyp = (ly + buy * fs) - yproj;
             if (mode == 0) {
-                acc += -log((double)Gxm) - log((double)Gym) - log((double)Gxp) - log((double)Gyp);
+                acc += -log(Gxm) - log(Gym) - log(Gxp) - log(Gyp);
             } else {
-                swx[i] = 1.f / (Gxm * Gxm) + 1.f / (Gxp * Gxp);
-                swy[i] = 1.f / (Gym * Gym) + 1.f / (Gyp * Gyp);
-                sgx[i] = -1.f / Gxm + 1.f / Gxp;
-                sgy[i] = -1.f / Gym + 1.f / Gyp;
+                swx[i] = 1.0 / (Gxm * Gxm) + 1.0 / (Gxp * Gxp);
+                swy[i] = 1.0 / (Gym * Gym) + 1.0 / (Gyp * Gyp);
+                sgx[i] = -1.0 / Gxm + 1.0 / Gxp;
+                sgy[i] = -1.0 / Gym + 1.0 / Gyp;
             }
         }
         __syncthreads();
@@ -102,46 +108,57 @@ __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const Fra
             // projected-landmark gradients dx[i][j], dy[i][j]
             for (int e = tid; e < NLMK * K; e += blockDim.x) {
                 const int i = e / K, j = e - i * K;
-                float dv[3] = {0.f, 0.f, 0.f};
+                double dv[3] = {0.0, 0.0, 0.0};
                 bool geo = true;
                 if (j >= L.off_pose && j < L.off_pose + 3) {
-                    dv[j - L.off_pose] = 1.f;
+                    dv[j - L.off_pose] = 1.0;
                 } else if (j >= L.off_pose + 3 && j < L.off_light) {
-                    const float* d = sdR + 9 * (j - L.off_pose - 3);
+                    const double* d = sdR + 9 * (j - L.off_pose - 3);
                     for (int a = 0; a < 3; ++a) dv[a] = d[a] * sp[i][0] + d[3 + a] * sp[i][1] + d[6 + a] * sp[i][2];
                 } else if (j >= L.off_exp && j < L.off_pose) {
                     const int v = m.lmk_ids[i], c = j - L.off_exp;
-                    float e3[3];
+                    double e3[3];
                     for (int a = 0; a < 3; ++a) e3[a] = m.exp_basis[(size_t)(3 * v + a) * m.exp_ld + c];
                     for (int a = 0; a < 3; ++a) dv[a] = sR[a] * e3[0] + sR[3 + a] * e3[1] + sR[6 + a] * e3[2];
                 } else if (j >= L.off_id && j < L.off_exp) {
                     const int v = m.lmk_ids[i], c = j - L.off_id;
-                    float e3[3];
+                    double e3[3];
                     for (int a = 0; a < 3; ++a) e3[a] = m.id_basis[(size_t)(3 * v + a) * m.id_ld + c];
                     for (int a = 0; a < 3; ++a) dv[a] = sR[a] * e3[0] + sR[3 + a] * e3[1] + sR[6 + a] * e3[2];
                 } else {
                     geo = false;
                 }
-                float dxj = 0.f, dyj = 0.f;
+                double dxj = 0.0, dyj = 0.0;
                 if (geo) {
-                    const float vx = sv[i][0], vy = sv[i][1], vz = sv[i][2];
+                    const double vx = sv[i][0], vy = sv[i][1], vz = sv[i][2];
                     dxj = fr.phix * (dv[0] * vz - dv[2] * vx) / (vz * vz);
                     dyj = fr.phiy * (dv[1] * vz - dv[2] * vy) / (vz * vz);
                 }
-                sdx[i][j] = dxj;
-                sdy[i][j] = dyj;
+                // gradient: sum_i gx_i dx_i; the Hessian uses sqrt(w) dx
+                sdx[(size_t)i * K + j] = dxj;
+                sdy[(size_t)i * K + j] = dyj;
             }
             __syncthreads();
             for (int j = tid; j < K; j += blockDim.x) {
-                float s = 0.f;
-                for (int i = 0; i < NLMK; ++i) s += sgx[i] * sdx[i][j] + sgy[i] * sdy[i][j];
-                g[(size_t)f * K + j] += s;
+                double s = 0.0;
+                for (int i = 0; i < NLMK; ++i) s += sgx[i] * sdx[(size_t)i * K + j] + sgy[i] * sdy[(size_t)i * K + j];
+                g[(size_t)f * K + j] += (float)s;
             }
+            __syncthreads();
+            for (int e = tid; e < NLMK * K; e += blockDim.x) {
+                const int i = e / K;
+                sdx[e] *= sqrt(swx[i]);
+                sdy[e] *= sqrt(swy[i]);
+            }
+            __syncthreads();
             for (int e = tid; e < K * K; e += blockDim.x) {
                 const int a = e / K, b = e - a * K;
-                float s = 0.f;
-                for (int i = 0; i < NLMK; ++i) s += swx[i] * sdx[i][a] * sdx[i][b] + swy[i] * sdy[i][a] * sdy[i][b];
-                H[(size_t)f * K * K + e] += s;
+                if (b < a) continue;
+                double s = 0.0;
+                for (int i = 0; i < NLMK; ++i)
+                    s += sdx[(size_t)i * K + a] * sdx[(size_t)i * K + b] + sdy[(size_t)i * K + a] * sdy[(size_t)i * K + b];
+                H[(size_t)f * K * K + e] += (float)s;
+                if (b != a) H[(size_t)f * K * K + (size_t)b * K + a] += (float)s;
             }
         }
     }
@@ -490,7 +507,15 @@ int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, F
         set_error("barrier: K=%d > %d", L.K, BMAXK);
         return TDI_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_barrier, dim3(fl.n > 0 ? fl.n : 1), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, lm,
+    const size_t lds = mode == 1 && use_landmarks ? 2 * sizeof(double) * NLMK * L.K : 0;
+    if (lds > 64 * 1024) {
+        // above the default dynamic-LDS limit: allow up to what K <= BMAXK needs (91 KB), once per process
+        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(k_barrier),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                           (int)(2 * sizeof(double) * NLMK * BMAXK));
+        TDI_HIP(attr);
+    }
+    hipLaunchKernelGGL(k_barrier, dim3(fl.n > 0 ? fl.n : 1), dim3(256), lds, s, m, L, frames, fl.act, fl.n, theta, lm,
                        use_landmarks, mode, obj, objb, H, g);
     TDI_HIP(hipGetLastError());
     return TDI_OK;

@@ -288,14 +288,25 @@ def rotvec_from_euler(yaw, pitch, roll) -> np.ndarray:
     return w * th
 
 
-def layout(m: Model, fit_id: bool, fit_tex: bool, fit_exp: bool, fit_light: bool):
-    """Offsets of the unknown blocks (Optimizer.h:71-91 order)."""
+LIGHT_UNKNOWNS = {0: 0, 1: 4, 2: 9}   # fit_light: fixed, point light (lambda, L), + second-order terms
+# second-order light terms of a drawn truth ~ N(0, SH2_SIGMA), start perturbed by N(0, SH2_INIT_PERT)
+SH2_SIGMA = 0.05
+SH2_INIT_PERT = 0.01
+
+
+def light_unknowns(fit_light) -> int:
+    return LIGHT_UNKNOWNS[int(fit_light)]
+
+
+def layout(m: Model, fit_id: bool, fit_tex: bool, fit_exp: bool, fit_light):
+    """Offsets of the unknown blocks (Optimizer.h:71-91 order).  fit_light: 0 (fixed), 1 (the
+    reference's point light, 4 unknowns) or 2 (with the second-order terms, 9 unknowns)."""
     ntex = m.k_tex if fit_tex else 0
     nid = m.k_id if fit_id else 0
     nexp = m.k_exp if fit_exp else 0
     off = {"tex": 0, "id": ntex, "exp": ntex + nid, "pose": ntex + nid + nexp}
     off["light"] = off["pose"] + 6
-    off["K"] = off["light"] + (4 if fit_light else 0)
+    off["K"] = off["light"] + light_unknowns(fit_light)
     return off
 
 
@@ -331,9 +342,13 @@ def draw_truth(m: Model, rng, lay: dict, pose_deg: float, tz_range=(400.0, 600.0
     th[lay["pose"]:lay["pose"] + 6] = pose
     lam = LAMBDA0.astype(np.float64) + rng.normal(0, 20.0, 3)
     Lint = L0 * rng.uniform(0.8, 1.2)
-    if fit_light:
-        th[lay["light"]:lay["light"] + 3] = lam
+    nl = lay["K"] - lay["light"]
+    if nl:
+        # the point light (lambda, L); with nine light unknowns written as (a = L lambda, L)
+        th[lay["light"]:lay["light"] + 3] = lam * (Lint if nl == 9 else 1.0)
         th[lay["light"] + 3] = Lint
+    if nl == 9:
+        th[lay["light"] + 4:lay["light"] + 9] = rng.normal(0, SH2_SIGMA, 5)
     init = th.copy()
     # start from a perturbed expression/identity, a perturbed pose and a perturbed light,
     # at roughly the one-pixel accuracy of the reference's landmark-driven initialisation
@@ -347,9 +362,13 @@ def draw_truth(m: Model, rng, lay: dict, pose_deg: float, tz_range=(400.0, 600.0
     p = lay["pose"]
     init[p:p + 3] += rng.normal(0, [0.3, 0.3, 1.0])
     init[p + 3:p + 6] += rng.normal(0, np.deg2rad(0.2), 3)
-    if fit_light:
-        init[lay["light"]:lay["light"] + 3] = lam + rng.normal(0, LIGHT_INIT_PERT[0], 3)
-        init[lay["light"] + 3] = Lint * (1.0 + rng.normal(0, LIGHT_INIT_PERT[1]))
+    if nl:
+        lam_i = lam + rng.normal(0, LIGHT_INIT_PERT[0], 3)
+        L_i = Lint * (1.0 + rng.normal(0, LIGHT_INIT_PERT[1]))
+        init[lay["light"]:lay["light"] + 3] = lam_i * (L_i if nl == 9 else 1.0)
+        init[lay["light"] + 3] = L_i
+    if nl == 9:
+        init[lay["light"] + 4:lay["light"] + 9] += rng.normal(0, SH2_INIT_PERT, 5)
     return Truth(theta=th, init=init, lam=lam, Lint=Lint)
 
 

@@ -3,8 +3,9 @@
 cfg2 (the batched per-frame stage): identity and albedo drawn once on the 231x231-grid
 model (53,361 vertices, 105,800 triangles, K_id = K_tex = 199, K_exp = 79) and folded into
 the means; every frame has its own crop side S ~ U{96..384}, focal length 800*S/192, pose
-within +-30 deg, tz in 400..600 mm, expression ~ N(0,1) clipped to +-3, and its own light
-(lambda, L) which is fitted jointly (6 pose + 79 expression + 4 light = 89 unknowns).
+within +-30 deg, tz in 400..600 mm, expression ~ N(0,1) clipped to +-3, and its own
+light, fitted jointly: the point light (lambda, L) and the five second-order terms
+(6 pose + 79 expression + 9 light = 94 unknowns).
 Observations are the GPU render of the truth plus N(0, 0.02) noise; starting points are
 perturbed truths made landmark-feasible.
 """
@@ -51,7 +52,7 @@ def cfg2_model(grid: int = 231, k_exp: int = 79, k_id: int = 199, k_tex: int = 1
     return _cfg1_cache[key]
 
 
-def cfg2(n_frames: int = 4096, iters: int = 12, grid: int = 231, k_exp: int = 79, fit_light: bool = True,
+def cfg2(n_frames: int = 4096, iters: int = 12, grid: int = 231, k_exp: int = 79, fit_light: int = 2,
          noise: float = 0.02, log=print) -> Workload:
     t0 = time.time()
     m = cfg2_model(grid, k_exp) if grid == 231 and k_exp == 79 else cfg2_model(grid, k_exp, 0, 0)
@@ -91,5 +92,5 @@ def cfg2(n_frames: int = 4096, iters: int = 12, grid: int = 231, k_exp: int = 79
         f"({time.time() - t0:.1f}s)")
     return Workload(model=m, frames=frames, truths=truths, opts=opts, lay=lay,
                     name=f"cfg2: {n_frames} frames, {grid}x{grid} grid ({m.n_vertices} V, {m.n_triangles} T), "
-                         f"K={lay['K']} (6 pose + {k_exp} expression + {4 if fit_light else 0} light), "
+                         f"K={lay['K']} (6 pose + {k_exp} expression + {synth.light_unknowns(fit_light)} light), "
                          f"crops U{{96..384}}, {iters} GN iterations")

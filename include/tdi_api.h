@@ -4,8 +4,8 @@
  * The drop-in boundary.  The reference has no FFI for this path: its fitting
  * step is the C++ library `open3DI` (src/CMakeLists.txt:41) whose entry points
  * take cuBLAS/cuSOLVER handles and raw device pointers (src/lib/model_fitter.h:29-79).
- * This header is the thin, type-clean layer beneath the C++ host mirror
- * (3di_amd/csrc/host/model_fitter.h): plain pointers, explicit sizes, int status
+ * This header is the thin, type-clean layer beneath the C++ host layer
+ * (include/tdi_host.h, 3di_amd/host/tdi_host.cpp): plain pointers, explicit sizes, int status
  * codes, a hipStream_t passed as void*, no exceptions across the ABI, no torch
  * types.  Each entry point names the reference interface it replaces.
  *
@@ -17,7 +17,20 @@
  *     (the reference transposes its crop, video_fitter.cu:542-549);
  *   - the per-frame unknown vector follows OptimizationVariables
  *     (Optimizer.h:71-91, T=1): [beta | alpha | epsilon | tau_x tau_y tau_z u1 u2 u3 | light],
- *     a block present only when its fit_* flag is set; light = (lambda1..3, L).
+ *     a block present only when its fit_* flag is set; light = (lambda1..3, L) with
+ *     fit_light = TDI_LIGHT_POINT, or (a1..3, L, s1..s5) with fit_light = TDI_LIGHT_SH2.
+ *
+ * Shading (newfuncs.cu:122-137), with n_c = R n the camera-space unit normal and v_c the
+ * camera-space surface point:
+ *   TDI_LIGHT_POINT (the reference): I = rho (1 + L n_c.(lambda - v_c));
+ *   TDI_LIGHT_SH2: I = rho (1 + n_c.a - L n_c.v_c + sum_k s_k Y_k(n_c)), with
+ *     Y_1 = nx ny, Y_2 = ny nz, Y_3 = nx nz, Y_4 = nx^2 - ny^2, Y_5 = 3 nz^2 - 1
+ *   the five second-order spherical-harmonic terms.  With s = 0 this is the reference's light
+ *   with a = L lambda: the same model written linearly in its unknowns.  Fitted jointly, the
+ *   reference's (lambda, L) is degenerate, since a distant light (|lambda| -> inf, L -> 0 with
+ *   L lambda fixed) shades like a directional one and the normal matrix becomes singular;
+ *   (a, L) has no such direction.  The constant term stays 1: the normalised-gradient data
+ *   term is blind to the scale of the image.
  */
 #ifndef TDI_API_H_
 #define TDI_API_H_
@@ -89,7 +102,10 @@ typedef struct tdi_frame_desc {
 
 /* Fit options.  Defaults (tdi_default_options) are the reference's constants. */
 typedef struct tdi_fit_options {
-    int32_t fit_id, fit_tex, fit_exp, fit_light;  /* unknown blocks                                 */
+    int32_t fit_id, fit_tex, fit_exp;  /* unknown blocks                                           */
+    int32_t fit_light;          /* 0: light fixed per frame (tdi_frame_desc.lambda, Lintensity);
+                                   TDI_LIGHT_POINT: (lambda, L) are 4 unknowns;
+                                   TDI_LIGHT_SH2: (lambda, L, s1..s5) are 9 unknowns             */
     int32_t max_iters;          /* GN iterations (model_fitter.cu:163: 100)                        */
     int32_t max_ls;             /* line-search evaluations per iteration (model_fitter.cu:224: 100) */
     int32_t ntmp_cap;           /* pixels kept per triangle, 0 = all (constants.h:19 uses 4)       */
@@ -108,6 +124,9 @@ typedef struct tdi_fit_options {
                                    MFMA tiles (else FP32), 16 = two-term hi/lo split of J
                                    (else three-term hi/mid/lo).  Default 7.               */
 } tdi_fit_options;
+
+#define TDI_LIGHT_POINT 1
+#define TDI_LIGHT_SH2 2
 
 #define TDI_MIXED_SYNTH_MFMA 1
 #define TDI_MIXED_JTJ_BF16 2

@@ -52,7 +52,7 @@ Layout make_layout(const tdi_model_desc* m, const tdi_fit_options* o)
     L.ntex = o->fit_tex ? m->k_tex : 0;
     L.nid = o->fit_id ? m->k_id : 0;
     L.nexp = o->fit_exp ? m->k_exp : 0;
-    L.nlight = o->fit_light ? 4 : 0;
+    L.nlight = o->fit_light == TDI_LIGHT_SH2 ? 9 : (o->fit_light ? 4 : 0);
     L.off_tex = 0;
     L.off_id = L.ntex;
     L.off_exp = L.off_id + L.nid;
@@ -272,7 +272,21 @@ struct Geometry {
     std::vector<double> xp, yp;
     std::vector<double> rho;        // per-vertex albedo
     double lam[3], Lint, tau[3];
+    double a[3], sh[5];             // directional and second-order light terms (TDI_LIGHT_SH2)
 };
+
+// Second-order terms of the camera-space unit normal n (the light of TDI_LIGHT_SH2,
+// include/tdi_api.h): Y = (nx ny, ny nz, nx nz, nx^2 - ny^2, 3 nz^2 - 1).  That light is
+// I = rho (1 + n_c.a - L n_c.v_c + sum_k s_k Y_k(n_c)): the reference's point light
+// (newfuncs.cu:122-137) written linearly in a = L lambda, plus the second-order terms.
+void sh2_terms(const double* n, double* Y)
+{
+    Y[0] = n[0] * n[1];
+    Y[1] = n[1] * n[2];
+    Y[2] = n[0] * n[2];
+    Y[3] = n[0] * n[0] - n[1] * n[1];
+    Y[4] = 3 * n[2] * n[2] - 1;
+}
 
 void geometry(const Frame& F, const double* th, bool need_dR, Geometry& G)
 {
@@ -282,13 +296,18 @@ void geometry(const Frame& F, const double* th, bool need_dR, Geometry& G)
     const double* u = th + L.off_pose + 3;
     rodrigues(u, G.R, need_dR ? G.dR : nullptr);
     for (int i = 0; i < 3; ++i) G.tau[i] = th[L.off_pose + i];
-    if (L.nlight) {
+    if (L.nlight == 4) {
         for (int i = 0; i < 3; ++i) G.lam[i] = th[L.off_light + i];
+        G.Lint = th[L.off_light + 3];
+    } else if (L.nlight == 9) {
+        for (int i = 0; i < 3; ++i) G.lam[i] = 0.0;     // L Id = -L n_c.v_c
         G.Lint = th[L.off_light + 3];
     } else {
         for (int i = 0; i < 3; ++i) G.lam[i] = F.f->lambda[i];
         G.Lint = F.f->Lintensity;
     }
+    for (int k = 0; k < 3; ++k) G.a[k] = L.nlight == 9 ? th[L.off_light + k] : 0.0;
+    for (int k = 0; k < 5; ++k) G.sh[k] = L.nlight == 9 ? th[L.off_light + 4 + k] : 0.0;
     G.P0.assign((size_t)V * 3, 0);
     G.P.assign((size_t)V * 3, 0);
     G.xp.resize(V); G.yp.resize(V);
@@ -458,7 +477,7 @@ void evaluate(const Frame& F, const double* th, bool derivs, Eval& E, double* dI
 
     // shading (fill_diffuse_component_and_populate_texture_and_shape, newfuncs.cu:18-167)
     std::vector<double> I(A, 0.0);
-    struct Px { double al, be, ga, v[3], p0[3], nrm[3], rho, Id; int tb, tc, ta; };
+    struct Px { double al, be, ga, v[3], p0[3], nrm[3], rho, Id, Y[5], S; int tb, tc, ta; };
     std::vector<Px> px(N);
     for (int n = 0; n < N; ++n) {
         const int k = E.vis[n];
@@ -484,7 +503,13 @@ void evaluate(const Frame& F, const double* th, bool derivs, Eval& E, double* dI
         q.Id = q.nrm[0] * rl[0] + q.nrm[1] * rl[1] + q.nrm[2] * rl[2]
              - (q.nrm[0] * q.p0[0] + q.nrm[1] * q.p0[1] + q.nrm[2] * q.p0[2]);
         q.rho = q.al * G.rho[q.tb] + q.be * G.rho[q.tc] + q.ga * G.rho[q.ta];
-        I[k] = q.rho + G.Lint * q.rho * q.Id;
+        // second-order light terms of the camera-space normal R n
+        double nc[3];
+        rot(G.R, q.nrm, nc);
+        sh2_terms(nc, q.Y);
+        q.S = nc[0] * G.a[0] + nc[1] * G.a[1] + nc[2] * G.a[2];
+        for (int j = 0; j < 5; ++j) q.S += G.sh[j] * q.Y[j];
+        I[k] = q.rho + G.Lint * q.rho * q.Id + q.rho * q.S;
     }
     E.img.assign(A, 0.0);
     blur(I.data(), E.img.data(), W, Hh, F.taps);
@@ -552,13 +577,15 @@ void evaluate(const Frame& F, const double* th, bool derivs, Eval& E, double* dI
         for (int k = 0; k < L.ntex; ++k) {
             double t = q.al * m->tex_basis[(size_t)q.tb * m->k_tex + k] + q.be * m->tex_basis[(size_t)q.tc * m->k_tex + k]
                      + q.ga * m->tex_basis[(size_t)q.ta * m->k_tex + k];
-            row[L.off_tex + k] = t + G.Lint * q.Id * t;     // fill_optimization_dI_dbeta (newfuncs.cu:383-403)
+            row[L.off_tex + k] = t + G.Lint * q.Id * t + q.S * t;  // fill_optimization_dI_dbeta (newfuncs.cu:383-403)
         }
         if (L.nlight) {
             double rn[3];
             rot(G.R, q.nrm, rn);
-            for (int a = 0; a < 3; ++a) row[L.off_light + a] = G.Lint * q.rho * rn[a];   // newfuncs.cu:151-153 x R
+            for (int a = 0; a < 3; ++a)                                                   // newfuncs.cu:151-153 x R
+                row[L.off_light + a] = (L.nlight == 9 ? 1.0 : G.Lint) * q.rho * rn[a];
             row[L.off_light + 3] = q.rho * q.Id;                                          // newfuncs.cu:155
+            for (int j = 0; j < L.nlight - 4; ++j) row[L.off_light + 4 + j] = q.rho * q.Y[j];
         }
     }
 

@@ -36,6 +36,8 @@ SCENES = {
     # the benchmark's block structure: light fitted jointly (pose + expression + 4 light
     # unknowns), more bases, larger and uneven crops, wider poses
     "light": dict(grid=64, k_exp=40, fit_light=True, crops=(160, 224, 120), pose_deg=25.0, seed=13, cfg=10),
+    # nine light unknowns: the point light and the second-order terms
+    "sh2": dict(grid=64, k_exp=40, fit_light=2, crops=(160, 224, 120), pose_deg=25.0, seed=13, cfg=12),
 }
 
 

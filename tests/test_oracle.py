@@ -99,10 +99,11 @@ def geometry_np(m, lay, th, fr):
     return P0, P, R, xp, yp
 
 
-@pytest.fixture(scope="module")
-def full_scene():
-    """Every unknown block present: texture, identity, expression, pose, light."""
-    return make_scene(grid=28, k_exp=6, k_id=4, k_tex=4, fit_id=True, fit_tex=True, fit_light=True,
+@pytest.fixture(scope="module", params=[1, 2], ids=["point_light", "sh2_light"])
+def full_scene(request):
+    """Every unknown block present: texture, identity, expression, pose, light; the light as
+    the reference's point light (4 unknowns) or with the second-order terms (9 unknowns)."""
+    return make_scene(grid=28, k_exp=6, k_id=4, k_tex=4, fit_id=True, fit_tex=True, fit_light=request.param,
                       crops=(72,), pose_deg=12.0, noise=0.02, iters=8, seed=5, cfg=22)
 
 
@@ -247,12 +248,32 @@ def test_shading_camera_space_form(full_scene):
     # exactly, so the point is interpolated in model space and then moved by (R, tau)
     p0 = al[:, None] * P0[b] + be[:, None] * P0[c] + ga[:, None] * P0[a]
     v = p0 @ R.T + th[lay["pose"]:lay["pose"] + 3]
-    lam = th[lay["light"]:lay["light"] + 3]
     L = th[lay["light"] + 3]
-    Id = np.sum((n @ R.T) * (lam[None, :] - v), axis=1)
+    # nine light unknowns carry a = L lambda
+    lam = th[lay["light"]:lay["light"] + 3] / (L if sc.K - lay["light"] == 9 else 1.0)
+    nc = n @ R.T
+    Id = np.sum(nc * (lam[None, :] - v), axis=1)
     rho_v = m.tex_mean.astype(np.float64) + m.tex_basis.astype(np.float64) @ th[lay["tex"]:lay["id"]]
     rho = al * rho_v[b] + be * rho_v[c] + ga * rho_v[a]
-    expect = rho * (1 + L * Id)
+    S = 0.0
+    if sc.K - lay["light"] == 9:
+        # second-order terms: real spherical harmonics of degree 2 of the camera-space normal,
+        # up to their constant factors (scipy's sph_harm_y, with the normal's polar angle and
+        # azimuth), combined with the unknowns s1..s5
+        from scipy.special import sph_harm_y
+        pol = np.arccos(np.clip(nc[:, 2], -1, 1))
+        azi = np.arctan2(nc[:, 1], nc[:, 0])
+        Y = lambda mm: sph_harm_y(2, mm, pol, azi)
+        # Y_2^{+-2} = c2 (nx +- i ny)^2, Y_2^{+-1} = -+2 c2 (nx +- i ny) nz, Y_2^0 = c0 (3 nz^2 - 1)
+        c2, c0 = np.sqrt(15 / (32 * np.pi)), np.sqrt(5 / (16 * np.pi))
+        terms = [(Y(2) - Y(-2)).imag / (4 * c2),       # nx ny
+                 -(Y(-1) + Y(1)).imag / (4 * c2),      # ny nz
+                 (Y(-1) - Y(1)).real / (4 * c2),       # nx nz
+                 (Y(2) + Y(-2)).real / (2 * c2),       # nx^2 - ny^2
+                 Y(0).real / c0]                       # 3 nz^2 - 1
+        s5 = th[lay["light"] + 4:lay["light"] + 9]
+        S = sum(s5[k] * terms[k] for k in range(5))
+    expect = rho * (1 + L * Id + S)
     assert np.abs(r["raw"][vis] - expect).max() < 1e-10
 
 
@@ -305,7 +326,7 @@ def test_jacobian_columns_fd(full_scene):
             xm1, ym1 = proj(th - e)
             fd = -(gx * (xp1 - xm1) + gy * (yp1 - ym1)) / (2 * h)
         else:   # texture and light enter the unblurred render linearly
-            h = 1e-3 if j == lay["light"] + 3 else 0.5
+            h = 1e-3 if j == lay["light"] + 3 else (0.05 if j > lay["light"] + 3 else 0.5)
             e[j] = h
             fd = (orc.render(0, th + e)["raw"][vis] - orc.render(0, th - e)["raw"][vis]) / (2 * h)
         scale = np.abs(fd).max()
