@@ -14,6 +14,7 @@
 //     take the edge pixel with weight 0, rows are clamped individually and keep their weights;
 //     the horizontal pass first, each pass a*(1-w) + b*w with both products and the sum rounded
 //     (contraction is off in this file's device code: no fused multiply-add).
+#include <climits>
 #include <cmath>
 
 #include "common.h"
@@ -144,6 +145,11 @@ extern "C" int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, i
         if (!(k > 0.f) || !std::isfinite(k) || q[0] < 0 || q[1] < 0 || q[2] < 1 || q[3] < 1) {
             set_error("tdi_prepare_crops: frame %d: coefficient must be positive and finite, crop origin "
                       "non-negative and crop size positive", i);
+            return TDI_ERR_ARG;
+        }
+        if (q[0] > INT_MAX - q[2] || q[1] > INT_MAX - q[3]) {
+            // the kernel forms x0 + column and y0 + row in 32 bits
+            set_error("tdi_prepare_crops: frame %d: crop reaches past 2^31 - 1 pixels", i);
             return TDI_ERR_ARG;
         }
         PrepFrame& F = fr[i];

@@ -197,6 +197,8 @@ def test_prepare_crops_rejects_bad_arguments_before_any_device_call():
     cases = [([0.0], good_crop, 16), ([np.inf], good_crop, 16),          # coefficient
              ([1.0], np.array([[-1, 0, 8, 8]], np.int32), 16),                 # crop origin
              ([1.0], np.array([[0, 0, 0, 8]], np.int32), 16),                  # crop size
+             ([1.0], np.array([[2**31 - 8, 0, 8, 8]], np.int32), 16),          # x0 + w past 2^31 - 1
+             ([1.0], np.array([[0, 2**31 - 2, 8, 8]], np.int32), 16),          # y0 + h past 2^31 - 1
              ([1.0], good_crop, 0),                                            # source width
              ([1e-4], good_crop, 16)]                                          # empty resized frame
     for coef, crop, sw in cases:
