@@ -16,6 +16,8 @@
 //     (contraction is off in this file's device code: no fused multiply-add).
 #include <climits>
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 #include "common.h"
 
@@ -123,22 +125,22 @@ __global__ __launch_bounds__(256) void k_prepare_crops(int n, const PrepFrame* _
 
 } // namespace
 
-} // namespace tdi
+namespace {
 
-using namespace tdi;
-
-extern "C" int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
-                                 const float* coef, const int32_t* crop, float* out, void* stream)
+// Checks the arguments and builds the per-frame descriptors and tile offsets on the host.
+int prepare_setup(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
+                  const float* coef, const int32_t* crop, float* out, std::vector<PrepFrame>& fr,
+                  std::vector<int>& tile_start, long long& tiles)
 {
     if (n < 0 || (n > 0 && (!bgr || !coef || !crop || !out)) || src_w < 1 || src_h < 1 ||
         frame_stride < 3ll * src_w * src_h) {
         set_error("tdi_prepare_crops: null argument, negative frame count or bad frame geometry");
         return TDI_ERR_ARG;
     }
-    if (n == 0) return TDI_OK;
-    std::vector<PrepFrame> fr(n);
-    std::vector<int> tile_start(n);
-    long long dst = 0, tiles = 0;
+    fr.resize(n);
+    tile_start.resize(n);
+    long long dst = 0;
+    tiles = 0;
     for (int i = 0; i < n; ++i) {
         const float k = coef[i];
         const int* q = crop + 4 * i;
@@ -173,14 +175,15 @@ extern "C" int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, i
             return TDI_ERR_ARG;
         }
     }
-    hipStream_t s = (hipStream_t)stream;
-    PrepFrame* dfr = nullptr;
-    int* dts = nullptr;
-    if (hipMalloc(&dfr, sizeof(PrepFrame) * n) != hipSuccess || hipMalloc(&dts, sizeof(int) * n) != hipSuccess) {
-        (void)hipFree(dfr);
-        set_error("tdi_prepare_crops: descriptor allocation failed");
-        return TDI_ERR_NOMEM;
-    }
+    return TDI_OK;
+}
+
+// Uploads the descriptors to dfr/dts (device memory), runs the kernel and waits for it: the
+// host descriptor arrays are the sources of the asynchronous copies.
+int prepare_run(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, float* out,
+                const std::vector<PrepFrame>& fr, const std::vector<int>& tile_start, long long tiles,
+                PrepFrame* dfr, int* dts, hipStream_t s)
+{
     int rc = TDI_OK;
     if (hipMemcpyAsync(dfr, fr.data(), sizeof(PrepFrame) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(dts, tile_start.data(), sizeof(int) * n, hipMemcpyHostToDevice, s) != hipSuccess) {
@@ -194,7 +197,86 @@ extern "C" int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, i
         if (e != hipSuccess) { set_error("tdi_prepare_crops: %s", hipGetErrorString(e)); rc = TDI_ERR_HIP; }
     }
     (void)hipStreamSynchronize(s);
+    return rc;
+}
+
+// The descriptors of tdi_prepare_crops_noalloc: PrepFrame[n] then int[n], n <= TDI_PREP_NOALLOC_MAX_FRAMES,
+// one copy per GPU for the life of the process.
+constexpr size_t PREP_DESC_BYTES = ((sizeof(PrepFrame) * TDI_PREP_NOALLOC_MAX_FRAMES + 255) / 256) * 256;
+constexpr size_t PREP_SCRATCH_BYTES = PREP_DESC_BYTES + sizeof(int) * TDI_PREP_NOALLOC_MAX_FRAMES;
+__device__ __align__(256) unsigned char scratch[PREP_SCRATCH_BYTES];
+
+// Device address of `scratch` on each device, looked up once per device with that device current
+// (the array's host-side name is not a device address).
+constexpr int PREP_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[PREP_MAX_DEVICES] = {};
+
+int scratch_for_current_device(void** ptr)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= PREP_MAX_DEVICES) {
+        set_error("tdi_prepare_crops_noalloc: no usable current device");
+        return TDI_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("tdi_prepare_crops_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            return TDI_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return TDI_OK;
+}
+
+} // namespace
+
+} // namespace tdi
+
+using namespace tdi;
+
+extern "C" int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
+                                 const float* coef, const int32_t* crop, float* out, void* stream)
+{
+    std::vector<PrepFrame> fr;
+    std::vector<int> tile_start;
+    long long tiles = 0;
+    int rc = prepare_setup(n, bgr, src_w, src_h, frame_stride, coef, crop, out, fr, tile_start, tiles);
+    if (rc || n == 0) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    PrepFrame* dfr = nullptr;
+    int* dts = nullptr;
+    if (hipMalloc(&dfr, sizeof(PrepFrame) * n) != hipSuccess || hipMalloc(&dts, sizeof(int) * n) != hipSuccess) {
+        (void)hipFree(dfr);
+        set_error("tdi_prepare_crops: descriptor allocation failed");
+        return TDI_ERR_NOMEM;
+    }
+    rc = prepare_run(n, bgr, src_w, src_h, out, fr, tile_start, tiles, dfr, dts, s);
     (void)hipFree(dfr);
     (void)hipFree(dts);
     return rc;
+}
+
+extern "C" int tdi_prepare_crops_noalloc(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h,
+                                         int64_t frame_stride, const float* coef, const int32_t* crop, float* out,
+                                         void* stream)
+{
+    if (n > TDI_PREP_NOALLOC_MAX_FRAMES) {
+        set_error("tdi_prepare_crops_noalloc: %d frames, at most %d per call", n, TDI_PREP_NOALLOC_MAX_FRAMES);
+        return TDI_ERR_ARG;
+    }
+    std::vector<PrepFrame> fr;
+    std::vector<int> tile_start;
+    long long tiles = 0;
+    int rc = prepare_setup(n, bgr, src_w, src_h, frame_stride, coef, crop, out, fr, tile_start, tiles);
+    if (rc || n == 0) return rc;
+    void* base = nullptr;
+    if ((rc = scratch_for_current_device(&base))) return rc;
+    PrepFrame* dfr = reinterpret_cast<PrepFrame*>(base);
+    int* dts = reinterpret_cast<int*>(static_cast<unsigned char*>(base) + PREP_DESC_BYTES);
+    return prepare_run(n, bgr, src_w, src_h, out, fr, tile_start, tiles, dfr, dts, (hipStream_t)stream);
 }

@@ -62,6 +62,7 @@ def lib():
         L.tdi_batch_unknowns.argtypes = [_vp]
         L.tdi_fit_lintensity.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
+        L.tdi_prepare_crops_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -283,11 +284,13 @@ def ortho_init(ctx: Context, lmk, cams):
     return dict(x_orth=x_orth, pose=pose, angles=angles, info=info)
 
 
-def prepare_crops(frames, coef, crop):
+def prepare_crops(frames, coef, crop, out=None, noalloc: bool = False):
     """Grey FP32 crops of resized 8-bit BGR frames on the GPU (tdi_prepare_crops).  frames: uint8
     device tensor [n, H, W, 3] (B, G, R); coef: [n] resize coefficients; crop: [n, 4] (x0, y0,
     width, height) in resized-frame pixels.  Returns a float32 device tensor with the crops
-    concatenated, each column-major (frame i at the prefix sum of width*height)."""
+    concatenated, each column-major (frame i at the prefix sum of width*height); `out`, when
+    given, is that tensor.  noalloc: call tdi_prepare_crops_noalloc, which allocates no device
+    memory (at most TDI_PREP_NOALLOC_MAX_FRAMES frames)."""
     import numpy as np
     torch = _torch()
     if frames.dtype != torch.uint8 or frames.dim() != 4 or frames.shape[3] != 3 or not frames.is_cuda:
@@ -298,10 +301,13 @@ def prepare_crops(frames, coef, crop):
     crop = np.ascontiguousarray(crop, np.int32).reshape(-1, 4)
     if coef.shape[0] != n or crop.shape[0] != n:
         raise ValueError("prepare_crops: coef must be [n] and crop [n, 4]")
-    out = torch.empty(int((crop[:, 2].astype(np.int64) * crop[:, 3]).sum()), device=frames.device,
-                      dtype=torch.float32)
-    _check(lib().tdi_prepare_crops(n, _dptr(frames), W, H, H * W * 3, coef.ctypes.data, crop.ctypes.data,
-                                   _dptr(out), _stream()))
+    total = int((crop[:, 2].astype(np.int64) * crop[:, 3]).sum())
+    if out is None:
+        out = torch.empty(total, device=frames.device, dtype=torch.float32)
+    elif out.dtype != torch.float32 or not out.is_cuda or out.numel() < total:
+        raise ValueError("prepare_crops: out must be a float32 CUDA tensor of at least the crops' size")
+    fn = lib().tdi_prepare_crops_noalloc if noalloc else lib().tdi_prepare_crops
+    _check(fn(n, _dptr(frames), W, H, H * W * 3, coef.ctypes.data, crop.ctypes.data, _dptr(out), _stream()))
     return out
 
 

@@ -246,6 +246,17 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed_flags, int32
 int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
                       const float* coef, const int32_t* crop, float* out, void* stream);
 
+/* tdi_prepare_crops without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as tdi_prepare_crops, at
+ * most TDI_PREP_NOALLOC_MAX_FRAMES frames per call (TDI_ERR_ARG beyond, before any device call).
+ * The per-frame descriptors go to a fixed device array of the library (under 1 MB per GPU, held
+ * until the process ends) instead of a per-call allocation.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (from several host threads or streams) race. */
+#define TDI_PREP_NOALLOC_MAX_FRAMES 16384
+int tdi_prepare_crops_noalloc(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
+                              const float* coef, const int32_t* crop, float* out, void* stream);
+
 /* Rendered pixel counts of the batch's last evaluation (host array of n). */
 int tdi_last_pixel_counts(tdi_batch* b, int32_t* npix);
 

@@ -19,11 +19,14 @@ multiply-adds of the resize round differently in the last bit, and the exact 2x 
 (coef 0.5), for which OpenCV switches to its area-averaging path.
 """
 import importlib
+import os
 
 import numpy as np
 import pytest
 
 import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 f32 = np.float32
 
@@ -206,6 +209,18 @@ def test_prepare_crops_rejects_bad_arguments_before_any_device_call():
         rc = L.tdi_prepare_crops(1, fake, sw, 16, 3 * 16 * 16, c.ctypes.data, crop.ctypes.data, fake, None)
         assert rc == 1, (coef, crop, sw)    # TDI_ERR_ARG
     assert L.tdi_prepare_crops(0, None, 16, 16, 768, None, None, None, None) == 0
+    # the allocation-free variant checks the same arguments, and its frame limit, first
+    for coef, crop, sw in cases:
+        c = np.array(coef, np.float32)
+        rc = L.tdi_prepare_crops_noalloc(1, fake, sw, 16, 3 * 16 * 16, c.ctypes.data, crop.ctypes.data, fake, None)
+        assert rc == 1, (coef, crop, sw)
+    nmax = 16384                    # TDI_PREP_NOALLOC_MAX_FRAMES
+    assert "TDI_PREP_NOALLOC_MAX_FRAMES %d" % nmax in open(os.path.join(ROOT, "include", "tdi_api.h")).read()
+    c = np.ones(nmax + 1, np.float32)
+    crops = np.tile(good_crop, (nmax + 1, 1))
+    rc = L.tdi_prepare_crops_noalloc(nmax + 1, fake, 16, 16, 768, c.ctypes.data, crops.ctypes.data, fake, None)
+    assert rc == 1 and b"at most" in L.tdi_last_error()
+    assert L.tdi_prepare_crops_noalloc(0, None, 16, 16, 768, None, None, None, None) == 0
 
 
 def test_host_frame_functions_reject_bad_arguments(tmp_path):
