@@ -88,7 +88,8 @@ struct Workspace {
     int C = 0;                 // frame slots
     long long cap = 0;         // crop pixels
     FrameGPU* frames = nullptr;
-    float *P0 = nullptr, *P = nullptr;
+    double* P0 = nullptr;      // [C][V][3] model-space vertices (FP64)
+    float* P = nullptr;
     float4* vproj = nullptr;
     uint64_t* keys = nullptr;
     int *cum = nullptr, *vis = nullptr;
@@ -100,7 +101,8 @@ struct Workspace {
     int4* pvert = nullptr;       // per visible position: vertices of its triangle, crop index
     float2* pbary = nullptr;     // per visible position: (alpha, beta)
     float* dI = nullptr;
-    float *theta = nullptr, *theta_try = nullptr, *delta = nullptr, *H = nullptr, *g = nullptr;
+    float *theta = nullptr, *theta_try = nullptr, *delta = nullptr, *g = nullptr;
+    double* H = nullptr;       // [C][K][K] normal matrices (FP64)
     double *obj = nullptr, *objb = nullptr;
     double* objp = nullptr;    // [C][256] per-workgroup partials of the data objective
     int *npix = nullptr, *status = nullptr, *face_ok = nullptr, *mask = nullptr, *searching = nullptr,
@@ -270,7 +272,7 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     w.cap = cap;
     w.max_area = b->max_area;
     w.frames = A.get<FrameGPU>(C);
-    w.P0 = A.get<float>((size_t)C * V * 3);
+    w.P0 = A.get<double>((size_t)C * V * 3);
     w.P = A.get<float>((size_t)C * V * 3);
     w.vproj = A.get<float4>((size_t)C * V);
     w.keys = A.get<uint64_t>(cap);
@@ -289,7 +291,7 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     w.theta = A.get<float>((size_t)C * K);
     w.theta_try = A.get<float>((size_t)C * K);
     w.delta = A.get<float>((size_t)C * K);
-    w.H = A.get<float>((size_t)C * K * K);
+    w.H = A.get<double>((size_t)C * K * K);
     w.g = A.get<float>((size_t)C * K);
     w.obj = A.get<double>(C);
     w.objb = A.get<double>(C);
@@ -683,7 +685,8 @@ int tdi_rasterize(tdi_batch* b, const float* xyz, const float* xyp, uint64_t* ke
     return TDI_OK;
 }
 
-int tdi_render(tdi_batch* b, const float* theta, float* img, float* obj, int32_t* npix, void* stream)
+int tdi_render(tdi_batch* b, const float* theta, float* img, float* blurred, float* obj, int32_t* npix,
+               void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
     Workspace& w = b->ws;
@@ -696,6 +699,7 @@ int tdi_render(tdi_batch* b, const float* theta, float* img, float* obj, int32_t
         const long long len = (i0 + n < b->n ? b->hframes[i0 + n].obs_off : b->total) - off;
         if ((rc = eval_render(b, all_frames(n), theta + (size_t)i0 * K, 0, s, nullptr, true))) return rc;
         if (img) TDI_HIP(hipMemcpyAsync(img + off, w.img, sizeof(float) * len, hipMemcpyDeviceToDevice, s));
+        if (blurred) TDI_HIP(hipMemcpyAsync(blurred + off, w.blurred, sizeof(float) * len, hipMemcpyDeviceToDevice, s));
         if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj + i0, n, s))) return rc;
         if (npix) TDI_HIP(hipMemcpyAsync(npix + i0, w.npix, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
     }
@@ -712,7 +716,7 @@ static int normal_eq(tdi_batch* b, const float* theta, float* H, float* g, float
         int rc = ws_load(b, i0, n, s);
         if (rc) return rc;
         if ((rc = eval_full(b, all_frames(n), theta + (size_t)i0 * K, mixed ? TDI_MIXED_JTJ_BF16 : 0, s))) return rc;
-        if (H) TDI_HIP(hipMemcpyAsync(H + (size_t)i0 * K * K, w.H, sizeof(float) * n * K * K, hipMemcpyDeviceToDevice, s));
+        if (H && (rc = launch_misc_f64_to_f32(w.H, H + (size_t)i0 * K * K, n * K * K, s))) return rc;
         if (g) TDI_HIP(hipMemcpyAsync(g + (size_t)i0 * K, w.g, sizeof(float) * n * K, hipMemcpyDeviceToDevice, s));
         if (obj && (rc = launch_misc_f64_to_f32(w.obj, obj + i0, n, s))) return rc;
     }
@@ -929,7 +933,8 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
     const int nd = intensity ? 1 : 3, j0 = intensity ? ol + 3 : ol;
     hipStream_t s = 0;
     Workspace& W = b->ws;
-    std::vector<float> x(Kx), xt(Kx), Hh((size_t)Kx * Kx), gh(Kx);
+    std::vector<float> x(Kx), xt(Kx), gh(Kx);
+    std::vector<double> Hh((size_t)Kx * Kx);
     for (int i = 0; i < b->n; ++i) {
         int rc = ws_load(b, i, 1, s);
         if (rc) return rc;
@@ -947,7 +952,7 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
             if ((rc = eval_full(b, all_frames(1), W.theta, 0, s, &Lx))) return rc;
             double ob[2];
             int np = 0;
-            TDI_HIP(hipMemcpyAsync(Hh.data(), W.H, sizeof(float) * Kx * Kx, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(Hh.data(), W.H, sizeof(double) * Kx * Kx, hipMemcpyDeviceToHost, s));
             TDI_HIP(hipMemcpyAsync(gh.data(), W.g, sizeof(float) * Kx, hipMemcpyDeviceToHost, s));
             TDI_HIP(hipMemcpyAsync(&ob[0], W.obj, sizeof(double), hipMemcpyDeviceToHost, s));
             TDI_HIP(hipMemcpyAsync(&ob[1], W.objb, sizeof(double), hipMemcpyDeviceToHost, s));

@@ -290,9 +290,9 @@ struct LmkArrays {
 // rt: [slots][12] scratch for the per-frame FP64 rotation (column-major) and translation;
 // P (camera-space vertices) may be null: the fit itself only needs P0 and the projections
 int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-                      double* rt, float* P0, float* P, float4* vproj, hipStream_t s);
+                      double* rt, double* P0, float* P, float4* vproj, hipStream_t s);
 int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
-                           const float* theta, double* rt, float* P0, float* P, float4* vproj, hipStream_t s);
+                           const float* theta, double* rt, double* P0, float* P, float4* vproj, hipStream_t s);
 
 // raster.hip
 int launch_clear_keys(uint64_t* keys, long long total, hipStream_t s);
@@ -316,11 +316,11 @@ int launch_visible_box(const FrameGPU* frames, FrameList fl, const uint64_t* key
 
 // shade.hip
 int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-                 const float* P0, const int4* pvert, const float2* pbary, const int* npix, const int4* box,
+                 const double* P0, const int4* pvert, const float2* pbary, const int* npix, const int4* box,
                  float* img, int max_area, hipStream_t s);
 // shading and objective straight from the keys over the visible box (line-search evaluations)
 int launch_shade_keys(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-                      const float* P0, const float4* vproj, const uint64_t* keys, const int4* box, float* img,
+                      const double* P0, const float4* vproj, const uint64_t* keys, const int4* box, float* img,
                       hipStream_t s);
 // objp: [frames][256] per-workgroup partials of the data objective (reduced in order into obj)
 int launch_objective(const FrameGPU* frames, FrameList fl, const uint64_t* keys, const int4* box,
@@ -340,22 +340,25 @@ int launch_ortho_init(const ModelGPU& m, int n, const float* lmk, const float* c
                       float* angles, int* info, hipStream_t s);
 
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-              const float* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
+              const double* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
               float* dI, int mean_area, hipStream_t s);
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);
 int launch_jtj_f32(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
-                   const float* pcoef, const float4* pixg, const float* dI, float w, float* H, float* g,
+                   const float* pcoef, const float4* pixg, const float* dI, float w, double* H, float* g,
                    float* partials, int S, hipStream_t s);
 int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
-                    const float* pcoef, const float4* pixg, const float* dI, float w, float* H, float* g,
+                    const float* pcoef, const float4* pixg, const float* dI, float w, double* H, float* g,
                     float* partials, int S, int g_on_mfma, int two_term, hipStream_t s);
 
 // solve.hip
 int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                    const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
-                   float* H, float* g, hipStream_t s);
+                   double* H, float* g, hipStream_t s);
+// H: the normal matrices, FP32 (the building-block ABI) or FP64 (the fits' workspace)
 int launch_cholesky(int n, int K, const float* H, const float* g, float* delta, int* status, int mode, const int* mask,
+                    hipStream_t s);
+int launch_cholesky(int n, int K, const double* H, const float* g, float* delta, int* status, int mode, const int* mask,
                     hipStream_t s);
 int launch_ls_init(int n, int K, FitState* st, const int* chol_status, const float* delta, const float* g,
                    const double* obj, int* searching, hipStream_t s);

@@ -25,11 +25,13 @@ namespace {
 constexpr int MAXK_LDS = 1024;
 
 __device__ __forceinline__ void store_vertex(const FrameGPU& fr, const double* R, const double* tau, const double* p,
-                                             size_t o, float* P0, float* P, float4* vproj)
+                                             size_t o, double* P0, float* P, float4* vproj)
 {
-    P0[3 * o + 0] = (float)p[0];
-    P0[3 * o + 1] = (float)p[1];
-    P0[3 * o + 2] = (float)p[2];
+    // model-space vertices stay FP64: the shading normal is the cross product of edges of
+    // about a millimetre between coordinates of about 100 mm, which FP32 would round to 1e-5
+    P0[3 * o + 0] = p[0];
+    P0[3 * o + 1] = p[1];
+    P0[3 * o + 2] = p[2];
     const double X = R[0] * p[0] + R[3] * p[1] + R[6] * p[2] + tau[0];
     const double Y = R[1] * p[0] + R[4] * p[1] + R[7] * p[2] + tau[1];
     const double Z = R[2] * p[0] + R[5] * p[1] + R[8] * p[2] + tau[2];
@@ -60,7 +62,7 @@ __global__ void k_frame_rt(Layout L, const float* __restrict__ theta, const int*
 
 __global__ void __launch_bounds__(256) k_synth_gemv(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                                      const float* __restrict__ theta, const int* __restrict__ act,
-                                                     int n, const double* __restrict__ rt, float* __restrict__ P0,
+                                                     int n, const double* __restrict__ rt, double* __restrict__ P0,
                                                      float* __restrict__ P, float4* __restrict__ vproj)
 {
     __shared__ double R[12];
@@ -117,7 +119,7 @@ constexpr int SY_KMAX = 80;                // expression coefficients held in LD
 // MFMAs of the others.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const float* __restrict__ theta,
-             const int* __restrict__ act, int n, const double* __restrict__ rt, float* __restrict__ P0,
+             const int* __restrict__ act, int n, const double* __restrict__ rt, double* __restrict__ P0,
              float* __restrict__ P, float4* __restrict__ vproj)
 {
     __shared__ double sA[SY_RB][SY_KMAX][16];          // [row block][k][tile row]
@@ -210,7 +212,7 @@ static int launch_frame_rt(const Layout& L, FrameList fl, const float* theta, do
 }
 
 int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-                      double* rt, float* P0, float* P, float4* vproj, hipStream_t s)
+                      double* rt, double* P0, float* P, float4* vproj, hipStream_t s)
 {
     if (L.K > MAXK_LDS) {
         set_error("synthesize: K=%d exceeds %d", L.K, MAXK_LDS);
@@ -225,7 +227,7 @@ int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames
 }
 
 int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
-                           const float* theta, double* rt, float* P0, float* P, float4* vproj, hipStream_t s)
+                           const float* theta, double* rt, double* P0, float* P, float4* vproj, hipStream_t s)
 {
     if (L.nid) return launch_synthesize(m, L, frames, fl, theta, rt, P0, P, vproj, s);
     int rc = launch_frame_rt(L, fl, theta, rt, s);

@@ -63,7 +63,7 @@ constexpr int DI_PX_PER_WG = 256;   // pixels per workgroup and frame in k_dI
 // Four wavefronts per SIMD (register budget 128): more pixels' gathers in flight per CU.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_dI(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                              const int* __restrict__ act, int n, const float* __restrict__ theta,
-                                             const float* __restrict__ P0, const int4* __restrict__ pvert,
+                                             const double* __restrict__ P0, const int4* __restrict__ pvert,
                                              const float2* __restrict__ pbary, const int* __restrict__ npix,
                                              const float4* __restrict__ pixg, float* __restrict__ dI)
 {
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const int sub = lane >> 4, l16 = lane & 15;      // four pixels per wavefront, 16 lanes each
-    const float* P0f = P0 + (size_t)f * m.V * 3;
+    const double* P0f = P0 + (size_t)f * m.V * 3;
     const int K = L.K;
     const int q0 = (int)((long long)N * strip / 8), q1 = (int)((long long)N * (strip + 1) / 8);
     for (int pos0 = q0 + (sb * 4 + wave) * 4; pos0 < q1; pos0 += nsb * 16) {
@@ -102,8 +102,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const float al = ab.x, be = ab.y, ga = gamma_rn(al, be);
         const int vb = pv.x, vc = pv.y, va = pv.z;
         float v[3], p0[3];
+        double p0d[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) p0[a] = al * P0f[3 * vb + a] + be * P0f[3 * vc + a] + ga * P0f[3 * va + a];
+        for (int a = 0; a < 3; ++a) {
+            p0d[a] = (double)al * P0f[3 * vb + a] + (double)be * P0f[3 * vc + a] + (double)ga * P0f[3 * va + a];
+            p0[a] = (float)p0d[a];
+        }
         // camera-space surface point R p0 + tau (R column-major)
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = sr.R[a] * p0[0] + sr.R[3 + a] * p0[1] + sr.R[6 + a] * p0[2] + sr.tau[a];
@@ -120,19 +124,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         // shading quantities for albedo and light columns
         float rho = 0.f, Id = 0.f, rn[3] = {0.f, 0.f, 0.f}, Y[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, S = 0.f;
         if (L.ntex || L.nlight) {
-            const float* B0 = P0f + 3 * vb;
-            const float* C0 = P0f + 3 * vc;
-            const float* A0 = P0f + 3 * va;
-            const float e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
-            const float e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
-            float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-            const float inv = 1.f / sqrtf(nx * nx + ny * ny + nz * nz);
-            nx *= inv; ny *= inv; nz *= inv;
-            const float d0 = sr.lam[0] - sr.tau[0], d1 = sr.lam[1] - sr.tau[1], d2 = sr.lam[2] - sr.tau[2];
-            const float lx = sr.R[0] * d0 + sr.R[1] * d1 + sr.R[2] * d2;
-            const float ly = sr.R[3] * d0 + sr.R[4] * d1 + sr.R[5] * d2;
-            const float lz = sr.R[6] * d0 + sr.R[7] * d1 + sr.R[8] * d2;
-            Id = nx * lx + ny * ly + nz * lz - (nx * p0[0] + ny * p0[1] + nz * p0[2]);
+            // normal and Id in FP64 from the FP64 corners (edges of about a millimetre)
+            const double* B0 = P0f + 3 * vb;
+            const double* C0 = P0f + 3 * vc;
+            const double* A0 = P0f + 3 * va;
+            const double e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
+            const double e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
+            double ndx = e1y * e2z - e1z * e2y, ndy = e1z * e2x - e1x * e2z, ndz = e1x * e2y - e1y * e2x;
+            const double inv = 1.0 / sqrt(ndx * ndx + ndy * ndy + ndz * ndz);
+            ndx *= inv; ndy *= inv; ndz *= inv;
+            const double d0 = (double)sr.lam[0] - sr.tau[0], d1 = (double)sr.lam[1] - sr.tau[1],
+                         d2 = (double)sr.lam[2] - sr.tau[2];
+            const double lx = sr.R[0] * d0 + sr.R[1] * d1 + sr.R[2] * d2;
+            const double ly = sr.R[3] * d0 + sr.R[4] * d1 + sr.R[5] * d2;
+            const double lz = sr.R[6] * d0 + sr.R[7] * d1 + sr.R[8] * d2;
+            Id = (float)(ndx * lx + ndy * ly + ndz * lz - (ndx * p0d[0] + ndy * p0d[1] + ndz * p0d[2]));
+            const float nx = (float)ndx, ny = (float)ndy, nz = (float)ndz;
             float rb = m.tex_mean[vb], rc = m.tex_mean[vc], ra = m.tex_mean[va];
             for (int q = 0; q < L.ntex; ++q) {
                 rb = fmaf(m.tex_basis[(size_t)vb * m.tex_ld + q], stex[q], rb);
@@ -347,10 +354,11 @@ __global__ void __launch_bounds__(256) k_jtj_f32(Layout L, const FrameGPU* __res
     }
 }
 
-// Sum the S partials in FP64; tiles are 4x4 (f32 kernel) or 16x16 (MFMA kernel), upper triangle.
+// Sum the S partials in FP64 into the FP64 normal matrix; tiles are 4x4 (f32 kernel) or 16x16
+// (MFMA kernel), upper triangle.
 __global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ act, int n, int K, int KP, int TS, int S,
                                                      const float* __restrict__ Hp, const float* __restrict__ gp,
-                                                     float w, float* __restrict__ H, float* __restrict__ g,
+                                                     float w, double* __restrict__ H, float* __restrict__ g,
                                                      int g_from_tiles)
 {
     const int nact = fl_count(act, n);
@@ -366,7 +374,7 @@ __global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ act,
             const int off = (a % TS) * TS + (b % TS);
             double sum = 0.0;
             for (int s = 0; s < S; ++s) sum += Hp[(((size_t)f * S + s) * ntiles + tile) * TS * TS + off];
-            H[(size_t)f * K * K + e] = (float)(w * sum);
+            H[(size_t)f * K * K + e] = (double)w * sum;
         } else {
             const int j = e - K * K;
             double sum = 0.0;
@@ -696,7 +704,7 @@ int jtj_splits(int n, int per_frame_pixels_hint)
 }
 
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-              const float* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
+              const double* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
               float* dI, int mean_area, hipStream_t s)
 {
     if (L.ntex > 512) {
@@ -728,7 +736,7 @@ size_t jtj_partials_floats(int n, int S, int K, bool mfma)
 }
 
 int launch_jtj_f32(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
-                   const float* pcoef, const float4* pixg, const float* dI, float w, float* H, float* g,
+                   const float* pcoef, const float4* pixg, const float* dI, float w, double* H, float* g,
                    float* partials, int S, hipStream_t s)
 {
     const int K = L.K;
@@ -751,7 +759,7 @@ int launch_jtj_f32(const Layout& L, const FrameGPU* frames, FrameList fl, const 
 }
 
 int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
-                    const float* pcoef, const float4* pixg, const float* dI, float w, float* H, float* g,
+                    const float* pcoef, const float4* pixg, const float* dI, float w, double* H, float* g,
                     float* partials, int S, int g_on_mfma, int two_term, hipStream_t s)
 {
     const int K = L.K;

@@ -126,13 +126,13 @@ __device__ __forceinline__ void shade_frame_setup(const Layout& L, const FrameGP
                           : 0.f;
 }
 
-// Per-pixel inputs of the shading: the triangle's model-space corners and corner albedos.
+// Per-pixel inputs of the shading: the triangle's model-space corners (FP64) and corner albedos.
 struct ShadeIn {
-    float b[3], c[3], a[3];
+    double b[3], c[3], a[3];
     float rb, rc, ra;
 };
 
-__device__ __forceinline__ void shade_load(const ModelGPU& m, const float* __restrict__ P0f, int vb, int vc, int va,
+__device__ __forceinline__ void shade_load(const ModelGPU& m, const double* __restrict__ P0f, int vb, int vc, int va,
                                            ShadeIn& in)
 {
 #pragma unroll
@@ -146,47 +146,51 @@ __device__ __forceinline__ void shade_load(const ModelGPU& m, const float* __res
     in.ra = m.tex_mean[va];
 }
 
-// I = rho (1 + L Id) at barycentric (al, be) of the triangle (vb, vc, va).
+// I = rho (1 + L Id) at barycentric (al, be) of the triangle (vb, vc, va).  The geometry (normal,
+// surface point, Id) and the shading factor are FP64 from the FP64 corners, so the rendered
+// value is the FP64 value of the same formula rounded once (the weights al, be, ga are the
+// rasteriser's FP32 values, as in the reference).
 __device__ __forceinline__ float shade_eval(const ModelGPU& m, const Layout& L, const ShadeFrame& sf,
                                             const float* stex, int vb, int vc, int va, const ShadeIn& in,
                                             float al, float be)
 {
-    const float ga = gamma_rn(al, be);
-    const float* B0 = in.b;
-    const float* C0 = in.c;
-    const float* A0 = in.a;
-    const float e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
-    const float e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
-    float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-    const float inv = 1.f / sqrtf(nx * nx + ny * ny + nz * nz);
+    const float gaf = gamma_rn(al, be);
+    const double ald = al, bed = be, gad = gaf;
+    const double* B0 = in.b;
+    const double* C0 = in.c;
+    const double* A0 = in.a;
+    const double e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
+    const double e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
+    double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
     nx *= inv; ny *= inv; nz *= inv;
-    const float px = al * B0[0] + be * C0[0] + ga * A0[0];
-    const float py = al * B0[1] + be * C0[1] + ga * A0[1];
-    const float pz = al * B0[2] + be * C0[2] + ga * A0[2];
+    const double px = ald * B0[0] + bed * C0[0] + gad * A0[0];
+    const double py = ald * B0[1] + bed * C0[1] + gad * A0[1];
+    const double pz = ald * B0[2] + bed * C0[2] + gad * A0[2];
     // R^T (lambda - tau)
-    const float lx = sf.R[0] * sf.dl[0] + sf.R[1] * sf.dl[1] + sf.R[2] * sf.dl[2];
-    const float ly = sf.R[3] * sf.dl[0] + sf.R[4] * sf.dl[1] + sf.R[5] * sf.dl[2];
-    const float lz = sf.R[6] * sf.dl[0] + sf.R[7] * sf.dl[1] + sf.R[8] * sf.dl[2];
-    const float Id = nx * lx + ny * ly + nz * lz - (nx * px + ny * py + nz * pz);
+    const double lx = (double)sf.R[0] * sf.dl[0] + (double)sf.R[1] * sf.dl[1] + (double)sf.R[2] * sf.dl[2];
+    const double ly = (double)sf.R[3] * sf.dl[0] + (double)sf.R[4] * sf.dl[1] + (double)sf.R[5] * sf.dl[2];
+    const double lz = (double)sf.R[6] * sf.dl[0] + (double)sf.R[7] * sf.dl[1] + (double)sf.R[8] * sf.dl[2];
+    const double Id = nx * lx + ny * ly + nz * lz - (nx * px + ny * py + nz * pz);
     float rb = in.rb, rc = in.rc, ra = in.ra;
     for (int q = 0; q < L.ntex; ++q) {
         rb = fmaf(m.tex_basis[(size_t)vb * m.tex_ld + q], stex[q], rb);
         rc = fmaf(m.tex_basis[(size_t)vc * m.tex_ld + q], stex[q], rc);
         ra = fmaf(m.tex_basis[(size_t)va * m.tex_ld + q], stex[q], ra);
     }
-    const float rho = al * rb + be * rc + ga * ra;
-    if (!sf.nsh) return rho + sf.L * rho * Id;
-    const float nc[3] = {sf.R[0] * nx + sf.R[3] * ny + sf.R[6] * nz, sf.R[1] * nx + sf.R[4] * ny + sf.R[7] * nz,
-                         sf.R[2] * nx + sf.R[5] * ny + sf.R[8] * nz};
-    float Y[5];
-    sh2_terms(nc, Y);
-    float S = nx * sf.la[0] + ny * sf.la[1] + nz * sf.la[2];
+    const double rho = ald * rb + bed * rc + gad * ra;
+    if (!sf.nsh) return (float)(rho + sf.L * rho * Id);
+    const double nc[3] = {sf.R[0] * nx + sf.R[3] * ny + sf.R[6] * nz, sf.R[1] * nx + sf.R[4] * ny + sf.R[7] * nz,
+                          sf.R[2] * nx + sf.R[5] * ny + sf.R[8] * nz};
+    const double Y[5] = {nc[0] * nc[1], nc[1] * nc[2], nc[0] * nc[2], nc[0] * nc[0] - nc[1] * nc[1],
+                         3.0 * nc[2] * nc[2] - 1.0};
+    double S = nx * sf.la[0] + ny * sf.la[1] + nz * sf.la[2];
 #pragma unroll
     for (int k = 0; k < 5; ++k) S += sf.sh[k] * Y[k];
-    return rho + sf.L * rho * Id + rho * S;
+    return (float)(rho + sf.L * rho * Id + rho * S);
 }
 
-__device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L, const float* __restrict__ P0f,
+__device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L, const double* __restrict__ P0f,
                                              const ShadeFrame& sf, const float* stex, int vb, int vc, int va,
                                              float al, float be)
 {
@@ -198,7 +202,7 @@ __device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L,
 // Shading of the visible list (pvert/pbary from the resolve pass), scattered into the crop.
 __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                                 const int* __restrict__ act, int n, const float* __restrict__ theta,
-                                                const float* __restrict__ P0, const int4* __restrict__ pvert,
+                                                const double* __restrict__ P0, const int4* __restrict__ pvert,
                                                 const float2* __restrict__ pbary, const int* __restrict__ npix,
                                                 float* __restrict__ img)
 {
@@ -215,7 +219,7 @@ __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const Frame
         for (int k = threadIdx.x; k < L.ntex; k += blockDim.x) stex[k] = tf[L.off_tex + k];
         __syncthreads();
         const long long base = fr.pix_off;
-        const float* P0f = P0 + (size_t)f * m.V * 3;
+        const double* P0f = P0 + (size_t)f * m.V * 3;
         for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += gridDim.x * blockDim.x) {
             const int4 pv = pvert[base + pos];
             const float2 ab = pbary[base + pos];
@@ -231,7 +235,7 @@ __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const Frame
 // line search, which needs the objective only.
 __global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                                      const int* __restrict__ act, int n, const float* __restrict__ theta,
-                                                     const float* __restrict__ P0, const float4* __restrict__ vproj,
+                                                     const double* __restrict__ P0, const float4* __restrict__ vproj,
                                                      const unsigned long long* __restrict__ keys,
                                                      const int4* __restrict__ box, float* __restrict__ img)
 {
@@ -252,7 +256,7 @@ __global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const 
         for (int k = threadIdx.x; k < L.ntex; k += blockDim.x) stex[k] = tf[L.off_tex + k];
         __syncthreads();
         const long long base = fr.pix_off;
-        const float* P0f = P0 + (size_t)f * m.V * 3;
+        const double* P0f = P0 + (size_t)f * m.V * 3;
         const float4* vp = vproj + (size_t)f * m.V;
         for (int x = r.X0 + cm.sub * nw + wave; x < r.X1; x += cm.fpb * nw)
             for (int y0 = r.Y0 + lane; y0 < r.Y1; y0 += 128) {
@@ -539,7 +543,7 @@ inline dim3 crop_grid(int max_area, int n)
 } // namespace
 
 int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-                 const float* P0, const int4* pvert, const float2* pbary, const int* npix, const int4* box,
+                 const double* P0, const int4* pvert, const float2* pbary, const int* npix, const int4* box,
                  float* img, int max_area, hipStream_t s)
 {
     if (L.ntex > 512) {
@@ -554,7 +558,7 @@ int launch_shade(const ModelGPU& m, const Layout& L, const FrameGPU* frames, Fra
 }
 
 int launch_shade_keys(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
-                      const float* P0, const float4* vproj, const uint64_t* keys, const int4* box, float* img,
+                      const double* P0, const float4* vproj, const uint64_t* keys, const int4* box, float* img,
                       hipStream_t s)
 {
     if (L.ntex > 512) {

@@ -27,13 +27,13 @@ constexpr int BMAXK = 112;
 // mode 0: objective only -> obj[f] = objb[f] = barrier value
 // mode 1: derivatives only -> H[f] += barrier Hessian, g[f] += barrier gradient
 // Everything is FP64 (rotation and its derivatives, landmark positions, constraint values,
-// weights, projected-landmark derivatives, sums); the FP32 H and g receive one rounding each.
+// weights, projected-landmark derivatives, sums, H); the FP32 g receives one rounding.
 // Dynamic LDS: the projected-landmark derivatives dx, dy as double [51][K] each, scaled by the
 // square roots of the Gauss-Newton weights, so that the landmark block is sum_i ex ex^T + ey ey^T.
 __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                                   const int* __restrict__ act, int n, const float* __restrict__ theta,
                                                   LmkArrays lm, int use_lmk, int mode, double* __restrict__ obj,
-                                                  double* __restrict__ objb, float* __restrict__ H,
+                                                  double* __restrict__ objb, double* __restrict__ H,
                                                   float* __restrict__ g)
 {
     // one workgroup per listed frame (the grid has a workgroup per slot)
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const Fra
         } else {
             g[(size_t)f * K + j] += (float)(-1.0 / fub + 1.0 / flb);
             const double a = -1.0 / (fub + 1e-12), b = 1.0 / (flb + 1e-12);
-            H[(size_t)f * K * K + (size_t)j * K + j] += (float)(a * a + b * b);
+            H[(size_t)f * K * K + (size_t)j * K + j] += a * a + b * b;
         }
     }
     if (use_lmk) {
@@ -157,8 +157,8 @@ __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const Fra
                 double s = 0.0;
                 for (int i = 0; i < NLMK; ++i)
                     s += sdx[(size_t)i * K + a] * sdx[(size_t)i * K + b] + sdy[(size_t)i * K + a] * sdy[(size_t)i * K + b];
-                H[(size_t)f * K * K + e] += (float)s;
-                if (b != a) H[(size_t)f * K * K + (size_t)b * K + a] += (float)s;
+                H[(size_t)f * K * K + e] += s;
+                if (b != a) H[(size_t)f * K * K + (size_t)b * K + a] += s;
             }
         }
     }
@@ -234,7 +234,8 @@ constexpr int CMAXK = 112;
 // mode 0: FP64 factorisation and solve.  mode 1: FP32 factorisation of the equilibrated
 // matrix and solve, then one FP64 residual-refinement step (r = -g - H x computed in FP64,
 // correction solved with the FP32 factor).
-__global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict__ H, const float* __restrict__ g,
+template <typename TH>
+__global__ void __launch_bounds__(256) k_cholesky(int K, const TH* __restrict__ H, const float* __restrict__ g,
                                                    float* __restrict__ delta, int* __restrict__ status,
                                                    const int* __restrict__ mask, int mode)
 {
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict
     __shared__ double sb[CMAXK];
     __shared__ float sbf[CMAXK];
     __shared__ float sflagf[1];
-    const float* Hf = H + (size_t)f * K * K;
+    const TH* Hf = H + (size_t)f * K * K;
     const float* gf = g + (size_t)f * K;
     const int tid = threadIdx.x;
     const int np = K * (K + 1) / 2;
@@ -258,7 +259,7 @@ __global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict
         float* A = (float*)dsm;
         __shared__ float sdinv[CMAXK];
         for (int i = tid; i < K; i += blockDim.x) {
-            const float d = Hf[(size_t)i * K + i];
+            const float d = (float)Hf[(size_t)i * K + i];
             sdinv[i] = d > 0.f ? rsqrtf(d) : 1.f;
         }
         __syncthreads();
@@ -267,7 +268,7 @@ __global__ void __launch_bounds__(256) k_cholesky(int K, const float* __restrict
             while ((i + 1) * (i + 2) / 2 <= e) ++i;
             while (i * (i + 1) / 2 > e) --i;
             const int j = e - i * (i + 1) / 2;
-            A[e] = Hf[(size_t)i * K + j] * sdinv[i] * sdinv[j];
+            A[e] = (float)Hf[(size_t)i * K + j] * sdinv[i] * sdinv[j];
         }
         for (int i = tid; i < K; i += blockDim.x) sbf[i] = -gf[i] * sdinv[i];
         __syncthreads();
@@ -501,7 +502,7 @@ __global__ void __launch_bounds__(CMP_THREADS) k_compact_frames(int n, const int
 
 int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                    const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
-                   float* H, float* g, hipStream_t s)
+                   double* H, float* g, hipStream_t s)
 {
     if (L.K > BMAXK) {
         set_error("barrier: K=%d > %d", L.K, BMAXK);
@@ -521,17 +522,30 @@ int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, F
     return TDI_OK;
 }
 
-int launch_cholesky(int n, int K, const float* H, const float* g, float* delta, int* status, int mode, const int* mask,
-                    hipStream_t s)
+template <typename TH>
+static int launch_cholesky_t(int n, int K, const TH* H, const float* g, float* delta, int* status, int mode,
+                             const int* mask, hipStream_t s)
 {
     if (K > CMAXK) {
         set_error("cholesky: K=%d > %d", K, CMAXK);
         return TDI_ERR_ARG;
     }
     const size_t lds = (size_t)K * (K + 1) / 2 * sizeof(double);   // FP64 fallback of mode 1 included
-    hipLaunchKernelGGL(k_cholesky, dim3(n), dim3(256), lds, s, K, H, g, delta, status, mask, mode);
+    hipLaunchKernelGGL(k_cholesky<TH>, dim3(n), dim3(256), lds, s, K, H, g, delta, status, mask, mode);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
+}
+
+int launch_cholesky(int n, int K, const float* H, const float* g, float* delta, int* status, int mode, const int* mask,
+                    hipStream_t s)
+{
+    return launch_cholesky_t(n, K, H, g, delta, status, mode, mask, s);
+}
+
+int launch_cholesky(int n, int K, const double* H, const float* g, float* delta, int* status, int mode, const int* mask,
+                    hipStream_t s)
+{
+    return launch_cholesky_t(n, K, H, g, delta, status, mode, mask, s);
 }
 
 int launch_ls_init(int n, int K, FitState* st, const int* chol_status, const float* delta, const float* g,

@@ -46,7 +46,7 @@ def lib():
         L.tdi_ortho_init.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_synthesize.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.tdi_rasterize.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
-        L.tdi_render.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_render.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_normal_equations.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_normal_equations_mixed.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_cholesky_solve.argtypes = [C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, C.c_int32, _vp]
@@ -177,14 +177,16 @@ class Batch:
                                    _stream()))
         return keys, bary
 
-    def render(self, theta):
+    def render(self, theta, blurred: bool = False):
+        """(img, obj, npix), or (img, blurred image, obj, npix) with blurred=True."""
         torch = _torch()
         th = self._theta(theta)
         img = torch.empty(int(self.offsets[-1]), device="cuda", dtype=torch.float32)
+        blr = torch.empty(int(self.offsets[-1]), device="cuda", dtype=torch.float32) if blurred else None
         obj = torch.empty(self.n, device="cuda", dtype=torch.float32)
         npix = torch.empty(self.n, device="cuda", dtype=torch.int32)
-        _check(lib().tdi_render(self.h, _dptr(th), _dptr(img), _dptr(obj), _dptr(npix), _stream()))
-        return img, obj, npix
+        _check(lib().tdi_render(self.h, _dptr(th), _dptr(img), _dptr(blr), _dptr(obj), _dptr(npix), _stream()))
+        return (img, blr, obj, npix) if blurred else (img, obj, npix)
 
     def normal_equations(self, theta, mixed: bool = False):
         torch = _torch()

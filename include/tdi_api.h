@@ -190,9 +190,12 @@ int tdi_synthesize(tdi_batch* b, const float* theta, float* xyz, float* xyp, voi
 int tdi_rasterize(tdi_batch* b, const float* xyz, const float* xyp, uint64_t* keys, float* bary, void* stream);
 
 /* Render + objective at theta: Renderer::render (renderer.cu:635-730) plus the barrier
- * objective (logbarrier_initializer.cu:1921-2081).  img: [sum W*H] blurred rendered crop
- * (may be NULL); obj: [n]; npix: [n] rendered pixel counts. */
-int tdi_render(tdi_batch* b, const float* theta, float* img, float* obj, int32_t* npix, void* stream);
+ * objective (logbarrier_initializer.cu:1921-2081).  img: [sum W*H] rendered crop before the
+ * blur (fill_tex_im1, funcs.cu:800-817), blurred: [sum W*H] the same after the 15-tap blur
+ * (convolutionRowsGPU/ColumnsGPU, newfuncs.cu:539-715), both zero-padded crops in the batch's
+ * crop order (either may be NULL); obj: [n]; npix: [n] rendered pixel counts. */
+int tdi_render(tdi_batch* b, const float* theta, float* img, float* blurred, float* obj, int32_t* npix,
+               void* stream);
 
 /* Normal equations at theta, FP32 accumulation: H [n][K][K], g [n][K], obj [n] —
  * get_obj_hessian_and_gradient_multiframe + DerivativeComputer::compute_hessian_and_gradient
