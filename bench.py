@@ -2,14 +2,16 @@
 """Benchmark of the batched mixed-precision per-frame 3DMM fit on one MI355X.
 
 One step = a complete fit (12 Gauss-Newton iterations with Armijo line search) of every
-frame of the cfg2 workload (4096 synthetic frames, crops U{96..384}, 89 unknowns per frame),
+frame of the cfg2 workload (4096 synthetic frames, crops U{96..384}, 94 unknowns per frame:
+6 pose + 79 expression + 9 light),
 inputs resident in HBM, through the batched path (device-resident loop in a hipGraph).
 
 Also measured in the same run:
   * the plain path (one frame at a time, host-driven loop, FP32 J^T J, FP64 Cholesky) on a
     prefix of the same frames -> frames/s and the speed-up of the batched path;
   * the FP64 OpenMP oracle on a prefix of the same frames on the host cores (cpu_baseline),
-    which also gives the batched path's parameter error against FP64;
+    which also gives the batched and plain paths' parameter errors against FP64, per block of
+    unknowns and per unknown, for one Gauss-Newton iteration and for the whole fit;
   * per-stage times with HIP events on the first workspace chunk -> the dominant kernel's
     roofline entry;
   * the landmark-driven initialisation (tdi_ortho_init) of every frame;
@@ -68,41 +70,59 @@ def stage_work(w, K, npix, n):
             ("cholesky", "mfma", chol_flops, "fp32")]
 
 
+BLOCKS = ("expression", "translation_mm", "rotation_rad", "light")
+
+
+def _blocks(lay):
+    return {"expression": slice(lay["exp"], lay["pose"]), "translation_mm": slice(lay["pose"], lay["pose"] + 3),
+            "rotation_rad": slice(lay["pose"] + 3, lay["pose"] + 6), "light": slice(lay["light"], lay["K"])}
+
+
 def _per_frame_rms(a, b, sl):
     d = np.asarray(a, np.float64)[:, sl] - np.asarray(b, np.float64)[:, sl]
     return np.sqrt(np.mean(d * d, axis=1))
 
 
-def sample_accuracy(w, O, sc, init, nc, th_o, ro, th_batched, plain, threads, tdi, ctx):
-    """Parameter agreement with the FP64 oracle on the first nc frames.
+def _table(a, b, lay):
+    """Error of a against b (frames x unknowns): per block of unknowns the RMS over the sample,
+    the median and max per-frame RMS, and the frames within 1e-4; the same for pose +
+    expression together (the metric's quantity)."""
+    out = {}
+    sl = dict(_blocks(lay))
+    sl["pose+expression"] = slice(lay["exp"], lay["pose"] + 6)
+    for k, v in sl.items():
+        if v.stop <= v.start:
+            continue
+        per = _per_frame_rms(a, b, v)
+        out[k] = dict(rms=float(np.sqrt(np.mean(per ** 2))), median_frame_rms=float(np.median(per)),
+                      max_frame_rms=float(per.max()), frames_within_1e4=int(np.sum(per <= 1e-4)))
+    return out
 
-    A whole 12-iteration fit is a chain of discrete decisions (Armijo accept/reject, stop
-    tests, pixels entering or leaving the rendered set), so two correct implementations
-    agree to round-off only while every decision agrees.  Reported side by side:
-      * whole fit: batched (and plain) GPU vs oracle, per-frame RMS of pose + expression;
-      * floor: the oracle against itself started 1e-6 away -- the agreement any
-        implementation can reach on these frames;
-      * one iteration: the first GN step (render, J^T J, Cholesky, line search) of the
-        batched mixed path vs the oracle's, from the same FP32 start;
-      * visibility: triangle-id buffers of the GPU raster vs the oracle's at the start point.
+
+def _per_unknown(a, b):
+    d = np.asarray(a, np.float64) - np.asarray(b, np.float64)
+    return [float("%.3g" % v) for v in np.sqrt(np.mean(d * d, axis=0))]
+
+
+def sample_accuracy(w, O, sc, init, nc, th_o, ro, th_batched, plain, threads, tdi):
+    """Parameter agreement with the FP64 oracle on the first nc frames, per block of unknowns
+    (expression coefficients, translation in mm, rotation vector in rad, light) and per unknown.
+
+    * one iteration: the first Gauss-Newton step (render, J^T J, Cholesky, line search) of the
+      batched mixed path and of the plain path, against the oracle's, from the same FP32 start.
+      This is the arithmetic's agreement; the accuracy gate is taken on it.
+    * whole fit: the 12-iteration fits against the oracle's, beside the oracle against itself
+      started 1e-6 away (its floor).  A whole fit is a chain of discrete decisions (Armijo
+      accept/reject, stop tests, pixels entering or leaving the rendered set) that round-off
+      flips, so whole fits agree to round-off only while every decision agrees; the floor shows
+      how often that happens on these frames for any implementation.
+    * visibility: triangle-id buffers of the GPU raster against the oracle's at the start point.
     """
     import torch
     abi = importlib.import_module("3di_amd.abi")
     lay = w.lay
-    sl = slice(lay["exp"], lay["pose"] + 6)
     th0 = init[:nc].astype(np.float64)
-    acc = {"frames": nc}
-    per = _per_frame_rms(th_batched[:nc], th_o, sl)
-    acc["whole_fit"] = dict(rms=float(np.sqrt(np.mean(per ** 2))), median_frame_rms=float(np.median(per)),
-                            frames_within_1e4=int(np.sum(per <= 1e-4)))
-    th_p, _ = sc.fit(th0 + 1e-6, n_threads=threads)
-    fl = _per_frame_rms(th_p, th_o, sl)
-    acc["oracle_floor"] = dict(median_frame_rms=float(np.median(fl)), frames_within_1e4=int(np.sum(fl <= 1e-4)))
-    if plain is not None:
-        npc = min(nc, plain["frames"])
-        pp = _per_frame_rms(plain["theta"][:npc], th_o[:npc], sl)
-        acc["plain_whole_fit"] = dict(frames=int(npc), median_frame_rms=float(np.median(pp)),
-                                      frames_within_1e4=int(np.sum(pp <= 1e-4)))
+    acc = {"frames": nc, "units": "expression coefficients; translation mm; rotation vector rad; light: a (3), L, s (5)"}
     # one GN iteration from the same start
     o1 = abi.default_options(**{f[0]: getattr(w.opts, f[0]) for f in abi.tdi_fit_options._fields_})
     o1.max_iters = 1
@@ -112,23 +132,18 @@ def sample_accuracy(w, O, sc, init, nc, th_o, ro, th_batched, plain, threads, td
     b1 = tdi.Batch(c1, w.frames[:nc])
     g1 = torch.from_numpy(init[:nc].astype(np.float32)).cuda()
     b1.fit_batched(g1)
-    g1 = g1.cpu().numpy()
-    step = _per_frame_rms(th_o1, th0, sl)
-    one = _per_frame_rms(g1, th_o1, sl)
-    # the same difference in the energy norm of the oracle's H at the start, relative to the
-    # step: it weighs each direction by what the images determine
-    from concurrent.futures import ThreadPoolExecutor
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        Hs = list(ex.map(lambda i: sc.normal_equations(i, th0[i])["H"], range(nc)))
-    en = []
-    for i in range(nc):
-        d = g1[i].astype(np.float64) - th_o1[i]
-        st = th_o1[i] - th0[i]
-        en.append(float(np.sqrt(max(d @ Hs[i] @ d, 0.0)) / max(np.sqrt(max(st @ Hs[i] @ st, 0.0)), 1e-30)))
-    en = np.array(en)
-    acc["one_iteration"] = dict(median_step_rms=float(np.median(step)), median_frame_rms=float(np.median(one)),
-                                max_frame_rms=float(one.max()), frames_within_1e4=int(np.sum(one <= 1e-4)),
-                                median_energy_rel=float(np.median(en)), max_energy_rel=float(en.max()))
+    g1 = g1.cpu().numpy().astype(np.float64)
+    p1, _ = b1.fit_plain(init[:nc])
+    acc["one_iteration"] = dict(step_size=_table(th_o1, th0, lay), batched=_table(g1, th_o1, lay),
+                                plain=_table(p1, th_o1, lay), batched_per_unknown_rms=_per_unknown(g1, th_o1))
+    # whole fits
+    th_f, _ = sc.fit(th0 + 1e-6, n_threads=threads)
+    acc["whole_fit"] = dict(batched=_table(th_batched[:nc], th_o, lay), oracle_floor=_table(th_f, th_o, lay),
+                            error_vs_truth_oracle=_table(th_o, w.truth()[:nc], lay),
+                            batched_per_unknown_rms=_per_unknown(th_batched[:nc], th_o))
+    if plain is not None:
+        npc = min(nc, plain["frames"])
+        acc["whole_fit"]["plain"] = _table(plain["theta"][:npc], th_o[:npc], lay)
     # visibility buffers at the start point
     xyz, xyp = b1.synthesize(init[:nc])
     keys, _ = b1.rasterize(xyz, xyp)
@@ -144,12 +159,41 @@ def sample_accuracy(w, O, sc, init, nc, th_o, ro, th_batched, plain, threads, td
     for r in ro:
         stops[int(r.stop)] = stops.get(int(r.stop), 0) + 1
     acc["oracle_stop_flags"] = stops
-    log(f"[bench] accuracy on {nc} frames: whole fit {acc['whole_fit']['frames_within_1e4']}/{nc} within 1e-4 "
-        f"(oracle floor {acc['oracle_floor']['frames_within_1e4']}/{nc}), one iteration "
-        f"{acc['one_iteration']['frames_within_1e4']}/{nc} (median {acc['one_iteration']['median_frame_rms']:.1e}, "
-        f"energy-norm median {acc['one_iteration']['median_energy_rel']:.1e}), "
-        f"visibility bit-exact {exact}/{nc}")
+    # gate: one GN iteration, median per-frame RMS <= 1e-4 for the expression coefficients and the
+    # rotation vector (rad), and for the translation relative to the depth tau_z (mm / mm): the
+    # unknowns are FP32 as in the reference, and at tau_z ~ 500 mm one FP32 ulp of tau_z is 3e-5 mm,
+    # so an absolute 1e-4 on millimetres would ask for agreement to a few ulps of the depth
+    ob = acc["one_iteration"]["batched"]
+    p = lay["pose"]
+    dt = np.sqrt(np.mean(((g1[:, p:p + 3] - th_o1[:, p:p + 3]) / np.abs(th_o1[:, p + 2:p + 3])) ** 2, axis=1))
+    vals = dict(expression=ob["expression"]["median_frame_rms"], rotation_rad=ob["rotation_rad"]["median_frame_rms"],
+                translation_rel_depth=float(np.median(dt)))
+    acc["gate"] = dict(criterion="one Gauss-Newton iteration of the batched mixed path vs FP64, median per-frame RMS "
+                                 "<= 1e-4 for the expression coefficients, the rotation vector (rad) and the "
+                                 "translation relative to the depth",
+                       values=vals, passed=bool(max(vals.values()) <= 1e-4),
+                       pose_expression_mm_rad_median=ob["pose+expression"]["median_frame_rms"])
+    pe = lambda t: f"{t['pose+expression']['median_frame_rms']:.1e} ({t['pose+expression']['frames_within_1e4']}/{nc})"
+    log(f"[bench] accuracy on {nc} frames, pose+expression median RMS (frames within 1e-4): one iteration batched "
+        f"{pe(acc['one_iteration']['batched'])}, plain {pe(acc['one_iteration']['plain'])}; whole fit batched "
+        f"{pe(acc['whole_fit']['batched'])}, plain {pe(acc['whole_fit']['plain']) if plain else '-'}, oracle floor "
+        f"{pe(acc['whole_fit']['oracle_floor'])}; visibility bit-exact {exact}/{nc}")
+    for k in BLOCKS:
+        if k in acc["one_iteration"]["batched"]:
+            log(f"[bench]   {k:15s} one iteration median {acc['one_iteration']['batched'][k]['median_frame_rms']:.1e} "
+                f"(step {acc['one_iteration']['step_size'][k]['median_frame_rms']:.1e}), whole fit median "
+                f"{acc['whole_fit']['batched'][k]['median_frame_rms']:.1e}")
     return acc
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -158,8 +202,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=4096)
-    ap.add_argument("--plain-frames", type=int, default=64)
-    ap.add_argument("--cpu-frames", type=int, default=32)
+    ap.add_argument("--plain-frames", type=int, default=512)
+    ap.add_argument("--cpu-frames", type=int, default=128)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -253,9 +297,34 @@ def main():
         for tj in json.load(open(args.traffic_json)).get("kernels", []):
             if tj.get("stage") == dom["stage"] and tj.get("frames_per_launch") == int(n_chunk):
                 traffic = tj.get("bytes_per_launch")
-    roofline = dict(kernel=dom["stage"], bound=dom["bound"], achieved=dom["achieved"], peak=dom["peak"],
-                    unit=dom["unit"], frac=dom["frac"], traffic=traffic,
-                    launch_ms=dom["ms"], frames_per_launch=int(n_chunk))
+    # the path's algorithmic work per unit (one GN iteration of one frame), as the scope table
+    # defines it, over the first chunk's frames: BYTES = 8A (observed g^s) + 16A (visibility
+    # write + read) + 24V (projected vertices write + read) + (12 NT + 3 V K_exp 4 B) / B (mesh
+    # and FP32 basis, amortised over the B frames of a launch) + 4K^2 (H out); FLOP = 2*3V*K_exp
+    # (synthesis) + 4K^2 P (J^T J, the reference's two rows per pixel, full square) + 4KP
+    # (J^T r) + K^3/3 (Cholesky).  A = crop pixels, P = rendered pixels.
+    m_ = w.model
+    A_ = np.array([f.width * f.height for f in w.frames[:n_chunk]], np.float64)
+    P_ = npix[:n_chunk].astype(np.float64)
+    V_, NT_, Ke_ = m_.n_vertices, m_.n_triangles, m_.k_exp
+    unit_bytes = float(np.mean(8 * A_ + 16 * A_) + 24 * V_ + (12 * NT_ + 3 * V_ * Ke_ * 4) / n_chunk + 4 * K * K)
+    unit_flops = float(np.mean(2 * 3 * V_ * Ke_ + 4 * K * K * P_ + 4 * K * P_) + K ** 3 / 3)
+    path = dict(unit="one GN iteration of one frame", bytes=round(unit_bytes), flops=round(unit_flops),
+                mean_crop_pixels=round(float(A_.mean())), mean_rendered_pixels=round(float(P_.mean())),
+                frames_per_launch=int(n_chunk))
+    # roofline of the dominant kernel: the path's algorithmic bytes (HBM-bound kernel) or flops
+    # (MFMA-bound kernel) per launch over that kernel's launch time; its own bytes beside it
+    if dom["bound"] == "hbm":
+        ach = unit_bytes * n_chunk / (dom["ms"] * 1e-3) / 1e9
+    else:
+        ach = (4 * K * K * float(P_.mean()) + 4 * K * float(P_.mean())) * n_chunk / (dom["ms"] * 1e-3) / 1e12
+    roofline = dict(kernel=dom["stage"], bound=dom["bound"], achieved=round(ach, 3), peak=dom["peak"],
+                    unit=dom["unit"], frac=round(ach / dom["peak"], 5), traffic=traffic,
+                    launch_ms=dom["ms"], frames_per_launch=int(n_chunk),
+                    achieved_own_work=dom["achieved"], frac_own_work=dom["frac"],
+                    note="achieved = the path's algorithmic work per GN iteration and frame (path) x frames per "
+                         "launch / this kernel's launch time; *_own_work uses the kernel's own algorithmic bytes "
+                         "or flops (stages)")
     if args.copy_probe:
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
         b_ = torch.empty_like(a)
@@ -320,16 +389,18 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as O
         nc = min(args.cpu_frames, args.frames)
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        # the host cores the box gives this GPU's job (OMP_NUM_THREADS); nproc is recorded beside it
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
         sc = O.Scene(w.model, w.frames[:nc], w.opts)
         t0 = time.perf_counter()
         th_o, ro = sc.fit(init[:nc].astype(np.float64), n_threads=threads)
         dtc = time.perf_counter() - t0
         cpu = dict(value=round(nc / dtc, 3), unit="frames/s", cores=threads, kind="port",
                    sample=f"first {nc} frames of the workload, FP64 OpenMP oracle, one frame per thread, "
-                          f"{w.opts.max_iters} GN iterations, {dtc:.1f} s")
+                          f"{w.opts.max_iters} GN iterations, {dtc:.1f} s",
+                   cpu_model=cpu_model(), nproc=os.cpu_count(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
         log(f"[bench] oracle: {nc} frames in {dtc:.1f}s on {threads} threads")
-        accuracy = sample_accuracy(w, O, sc, init, nc, th_o, ro, th_batched, plain, threads, tdi, ctx)
+        accuracy = sample_accuracy(w, O, sc, init, nc, th_o, ro, th_batched, plain, threads, tdi)
         p0L = w.model.mean_shape.astype(np.float64)[w.model.landmark_ids.astype(np.int64)]
         t0 = time.perf_counter()
         O.ortho_init(lmk.cpu().numpy(), cams.cpu().numpy(), p0L)
@@ -360,12 +431,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "frames": args.frames, "unknowns_per_frame": K,
                    "gn_iterations": int(w.opts.max_iters), "mean_iterations_used": float(iters.mean()),
-                   "precision": "FP64 vertex synthesis (f64 MFMA), BF16 hi/mid/lo J^T J on MFMA with FP32 "
-                                "accumulation, FP32 J^T r, equilibrated FP32 Cholesky + 1 FP64 refinement"},
+                   "precision": "FP64 vertex synthesis (f64 MFMA) and shading geometry, FP32 image and per-pixel "
+                                "Jacobian, BF16 hi/mid/lo J^T J on MFMA with FP32 accumulation and FP64 reduction, "
+                                "FP32 J^T r, FP64 barrier and normal matrix, equilibrated FP32 Cholesky + 1 FP64 "
+                                "refinement"},
         "roofline": roofline,
+        "path_work": path,
         "cpu_baseline": cpu,
         "plain_gpu": ({k: v for k, v in plain.items() if k != "theta"} if plain else None),
         "speedup_vs_plain": (round(fps / plain["fps"], 2) if plain else None),
+        "accuracy_gate": (accuracy or {}).get("gate"),
         "batched_stop_flags": {str(k): int(v) for k, v in zip(*np.unique([r.stop for r in reps], return_counts=True))},
         "accuracy_vs_fp64": accuracy,
         "batched_stop_flags_note": "1 max iterations, 2 decrement test, 4 step below t_min, 8 NaN, 16 not PD",
