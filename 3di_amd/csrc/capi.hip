@@ -755,6 +755,64 @@ int tdi_ortho_init(tdi_ctx* c, int32_t n, const float* lmk, const float* cams, f
 // Stage timers: one full evaluation (the work of one GN iteration before the solve) with
 // HIP events around each stage, on the batch's stream.  Mirrors the reference's compile-time
 // MEASURE_TIME timers (model_fitter.cu:43-58, derivative_computer.cu:32-37,141-172).
+int tdi_fit_buffers(tdi_batch* b, const float* theta, int32_t mixed_flags, float* vproj, int32_t* nbr,
+                    int32_t* npix, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    Workspace& w = b->ws;
+    const int K = b->ctx->L.K, V = b->ctx->m.V;
+    for (int i0 = 0; i0 < b->n; i0 += w.C) {
+        const int n = std::min(w.C, b->n - i0);
+        int rc = ws_load(b, i0, n, s);
+        if (rc) return rc;
+        const long long off = b->hframes[i0].obs_off;
+        const long long len = (i0 + n < b->n ? b->hframes[i0 + n].obs_off : b->total) - off;
+        if ((rc = eval_render(b, all_frames(n), theta + (size_t)i0 * K, mixed_flags, s))) return rc;
+        if (vproj)
+            TDI_HIP(hipMemcpyAsync(vproj + (size_t)i0 * V * 4, w.vproj, sizeof(float4) * (size_t)n * V,
+                                   hipMemcpyDeviceToDevice, s));
+        // pnb holds each frame's positions at the frame's crop offset: the same layout as the output
+        if (nbr) TDI_HIP(hipMemcpyAsync(nbr + 4 * off, w.pnb, sizeof(int4) * len, hipMemcpyDeviceToDevice, s));
+        if (npix) TDI_HIP(hipMemcpyAsync(npix + i0, w.npix, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    }
+    return TDI_OK;
+}
+
+int tdi_barrier_terms(tdi_batch* b, const float* theta, double* H, double* g, double* obj, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    Workspace& w = b->ws;
+    tdi_ctx* c = b->ctx;
+    const int K = c->L.K;
+    for (int i0 = 0; i0 < b->n; i0 += w.C) {
+        const int n = std::min(w.C, b->n - i0);
+        int rc = ws_load(b, i0, n, s);
+        if (rc) return rc;
+        const float* th = theta + (size_t)i0 * K;
+        TDI_HIP(hipMemsetAsync(w.H, 0, sizeof(double) * (size_t)n * K * K, s));
+        TDI_HIP(hipMemsetAsync(w.g, 0, sizeof(float) * (size_t)n * K, s));
+        if ((rc = launch_barrier(c->m, c->L, w.frames, all_frames(n), th, lmk_arrays(w), c->opts.use_landmarks, 0,
+                                 w.obj, w.objb, w.H, w.g, s)))
+            return rc;
+        if ((rc = launch_barrier(c->m, c->L, w.frames, all_frames(n), th, lmk_arrays(w), c->opts.use_landmarks, 1,
+                                 w.obj, w.objb, w.H, w.g, s)))
+            return rc;
+        if (H) TDI_HIP(hipMemcpyAsync(H + (size_t)i0 * K * K, w.H, sizeof(double) * (size_t)n * K * K,
+                                      hipMemcpyDeviceToDevice, s));
+        if (g) {
+            std::vector<float> hg((size_t)n * K);
+            std::vector<double> dg((size_t)n * K);
+            TDI_HIP(hipMemcpyAsync(hg.data(), w.g, sizeof(float) * hg.size(), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipStreamSynchronize(s));
+            for (size_t k = 0; k < hg.size(); ++k) dg[k] = hg[k];
+            TDI_HIP(hipMemcpyAsync(g + (size_t)i0 * K, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice, s));
+            TDI_HIP(hipStreamSynchronize(s));
+        }
+        if (obj) TDI_HIP(hipMemcpyAsync(obj + i0, w.objb, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+    return TDI_OK;
+}
+
 // Stages: 0 synthesis, 1 raster (clear+raster+resolve), 2 shade+blur+grads, 3 barrier,
 // 4 dI, 5 J^T J (+reduce), 6 Cholesky.  ms[7] receives the mean per repetition.
 int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t reps, float* ms)

@@ -233,7 +233,8 @@ constexpr int CMAXK = 112;
 
 // mode 0: FP64 factorisation and solve.  mode 1: FP32 factorisation of the equilibrated
 // matrix and solve, then one FP64 residual-refinement step (r = -g - H x computed in FP64,
-// correction solved with the FP32 factor).
+// correction solved with the FP32 factor); when the FP32 factorisation breaks down or the
+// correction exceeds 1e-3 of the solution (an ill-conditioned frame), the FP64 factorisation.
 template <typename TH>
 __global__ void __launch_bounds__(256) k_cholesky(int K, const TH* __restrict__ H, const float* __restrict__ g,
                                                    float* __restrict__ delta, int* __restrict__ status,
@@ -298,10 +299,27 @@ __global__ void __launch_bounds__(256) k_cholesky(int K, const TH* __restrict__ 
                 for (int k = tid; k < i; k += blockDim.x) sb[k] -= (double)A[pk(i, k)] * xi;
                 __syncthreads();
             }
-            for (int i = tid; i < K; i += blockDim.x)
-                delta[(size_t)f * K + i] = (float)((double)sbf[i] + sb[i] * (double)sdinv[i]);
-            if (tid == 0) status[f] = TDI_OK;
-            return;
+            // the correction measures the FP32 solution's error (about cond * 2^-24 of it); past
+            // 1e-3 of the solution one refinement step leaves more than ~1e-6, and the FP64
+            // factorisation below takes over
+            if (tid == 0) {
+                double cn = 0.0, xn = 0.0;
+                for (int i = 0; i < K; ++i) {
+                    const double c = sb[i] * (double)sdinv[i];
+                    cn += c * c;
+                    xn += (double)sbf[i] * (double)sbf[i];
+                }
+                sflag[0] = (cn <= 1e-6 * xn) ? 0.0 : 1.0;
+            }
+            __syncthreads();
+            if (sflag[0] == 0.0) {
+                for (int i = tid; i < K; i += blockDim.x)
+                    delta[(size_t)f * K + i] = (float)((double)sbf[i] + sb[i] * (double)sdinv[i]);
+                if (tid == 0) status[f] = TDI_OK;
+                return;
+            }
+            __syncthreads();
+            if (tid == 0) sflag[0] = 0.0;
         }
         __syncthreads();
     }

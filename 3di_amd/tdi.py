@@ -61,6 +61,8 @@ def lib():
         L.tdi_batch_size.argtypes = [_vp]
         L.tdi_batch_unknowns.argtypes = [_vp]
         L.tdi_fit_lintensity.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.tdi_fit_buffers.argtypes = [_vp, _vp, C.c_int32, _vp, _vp, _vp, _vp]
+        L.tdi_barrier_terms.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         _lib = L
@@ -187,6 +189,30 @@ class Batch:
         npix = torch.empty(self.n, device="cuda", dtype=torch.int32)
         _check(lib().tdi_render(self.h, _dptr(th), _dptr(img), _dptr(blr), _dptr(obj), _dptr(npix), _stream()))
         return (img, blr, obj, npix) if blurred else (img, obj, npix)
+
+    def fit_buffers(self, theta, mixed_flags: int = 1):
+        """The fit's projected vertices [n, V, 4], neighbour positions [sum W*H, 4] (frame i's
+        rendered pixels at offsets[i] .. offsets[i] + npix[i]) and pixel counts (tdi_fit_buffers)."""
+        torch = _torch()
+        th = self._theta(theta)
+        V = self.ctx.model.n_vertices
+        vproj = torch.empty((self.n, V, 4), device="cuda", dtype=torch.float32)
+        nbr = torch.zeros((int(self.offsets[-1]), 4), device="cuda", dtype=torch.int32)
+        npix = torch.empty(self.n, device="cuda", dtype=torch.int32)
+        _check(lib().tdi_fit_buffers(self.h, _dptr(th), int(mixed_flags), _dptr(vproj), _dptr(nbr), _dptr(npix),
+                                     _stream()))
+        return vproj, nbr, npix
+
+    def barrier_terms(self, theta):
+        """(H [n, K, K] float64, g [n, K] float64, obj [n] float64) of the log barrier alone."""
+        torch = _torch()
+        th = self._theta(theta)
+        K = self.K
+        H = torch.empty((self.n, K, K), device="cuda", dtype=torch.float64)
+        g = torch.empty((self.n, K), device="cuda", dtype=torch.float64)
+        obj = torch.empty(self.n, device="cuda", dtype=torch.float64)
+        _check(lib().tdi_barrier_terms(self.h, _dptr(th), _dptr(H), _dptr(g), _dptr(obj), _stream()))
+        return H, g, obj
 
     def normal_equations(self, theta, mixed: bool = False):
         torch = _torch()

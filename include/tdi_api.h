@@ -227,6 +227,26 @@ int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, flo
 int tdi_ortho_init(tdi_ctx* ctx, int32_t n, const float* lmk, const float* cams, float* x_orth, float* pose,
                    float* angles, int32_t* info, void* stream);
 
+/* The integer buffers the fit itself computes, for bit-for-bit checks: a render at theta through
+ * the same launches as a fit with these mixed_flags (TDI_MIXED_SYNTH_MFMA selects the FP64-MFMA
+ * synthesis the batched fit uses), then
+ *   vproj [n][V][4] the projected vertices (x, y in image pixels, camera z, 0) the rasteriser
+ *   reads -- view_transform_3d_pts_and_render_2d (funcs.cu:440-452);
+ *   nbr   [sum W*H][4] per rendered pixel, in the frame's crop order from its crop offset, the
+ *   sorted positions of its left, right, above and below neighbours -- fill_krels/fill_krels2
+ *   (funcs.cu:288-340, derivative_computer.cu:43-72); entries past the frame's pixel count are
+ *   unspecified;
+ *   npix  [n] rendered pixel counts.
+ * Device pointers; any output may be NULL. */
+int tdi_fit_buffers(tdi_batch* b, const float* theta, int32_t mixed_flags, float* vproj, int32_t* nbr,
+                    int32_t* npix, void* stream);
+
+/* The log-barrier terms alone at theta, as the fits add them -- Logbarrier_Initializer::
+ * compute_gradient_and_hessian and evaluate_objective_function (logbarrier_initializer.cu:
+ * 1272-1842, 1921-2081): H [n][K][K] (FP64), g [n][K] (the fits keep g in FP32; returned
+ * widened), obj [n].  Device pointers; any may be NULL. */
+int tdi_barrier_terms(tdi_batch* b, const float* theta, double* H, double* g, double* obj, void* stream);
+
 /* Frames per workspace chunk (the batch is processed in chunks of this many frames). */
 int tdi_workspace_frames(tdi_batch* b);
 

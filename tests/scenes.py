@@ -65,3 +65,32 @@ def make_scene(grid=40, k_exp=10, k_id=0, k_tex=0, fit_id=False, fit_tex=False, 
         frames.append(fr)
         truths.append(tr)
     return Scene(model=m, frames=frames, truths=truths, opts=opts, lay=lay)
+
+
+def cfg2_scene(crops=(96, 140, 180, 220, 260, 300, 340, 384), noise=0.02, iters=12, cfg=2):
+    """Frames of the batched per-frame workload at full size: the 231 x 231-grid model (53,361
+    vertices, 105,800 triangles) with identity and albedo folded in, K_exp = 79, nine light
+    unknowns (94 per frame), crops of the given sides; observations rendered by the oracle."""
+    workloads = importlib.import_module("3di_amd.workloads")
+    m = workloads.cfg2_model(231, 79)
+    lay = synth.layout(m, False, False, True, 2)
+    opts = abi.default_options(max_iters=iters, fit_light=2)
+    frames, truths = [], []
+    for i, S in enumerate(crops):
+        rng = np.random.default_rng(synth.frame_seed(cfg, i))
+        tr = synth.draw_truth(m, rng, lay, 30.0, fit_light=2)
+        fr = synth.make_frame(m, tr, lay, rng, synth.cfg2_focal(S), S, S)
+        raw = O.Scene(m, [fr], opts).render(0, tr.theta)["raw"].astype(np.float32)
+        fr.obs = synth.add_noise(raw, rng, noise)
+        synth.make_feasible(m, tr, lay, fr)
+        frames.append(fr)
+        truths.append(tr)
+    return Scene(model=m, frames=frames, truths=truths, opts=opts, lay=lay)
+
+
+def cfg0_scene(n_frames=2, noise=0.02, iters=20):
+    """The small plain-fit configuration: 100 x 100 grid (10,000 vertices, 19,602 triangles),
+    K_id = K_tex = 40, K_exp = 20, identity, albedo, expression, pose and the point light fitted
+    (110 unknowns), 192 x 192 crops at f = 800 px."""
+    return make_scene(grid=100, k_exp=20, k_id=40, k_tex=40, fit_id=True, fit_tex=True, fit_light=1,
+                      crops=(192,) * n_frames, pose_deg=20.0, noise=noise, iters=iters, seed=synth.BASE_SEED, cfg=0)
