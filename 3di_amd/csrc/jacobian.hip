@@ -595,6 +595,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T >= 7
                     const float qg = c * e;                         // J^T r weight
                     cf = make_float4(wgt * dy, -wgt * dx, qg * dy, -qg * dx);
                     rk = dn > 0.f ? e / dn : 0.f;
+                    // a NaN observation makes e NaN: 0 * NaN would carry it into every column of the
+                    // tiles through the multipliers below, so the tiles' residual column takes 0 and
+                    // the NaN reaches g through the FP32 J^T r (qg), which stops the frame on a NaN
+                    // search direction
+                    if (!(rk == rk)) rk = 0.f;
                 }
                 snb[tid] = nb;
                 scf[tid] = cf;
@@ -623,7 +628,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T >= 7
                 const bool item = NITEM % 256 == 0 || it < NITEM;
                 const int itc = min(it, NITEM - 1);
                 const int pq = itc / KP, j = itc - (itc / KP) * KP;
-                // 0/1 multipliers instead of selects keep this straight-line code
+                // 0/1 multipliers instead of selects keep this straight-line code (rk is finite, see
+                // the pixel set-up)
                 const float ma = j < K ? 1.f : 0.f, mk = j == K ? 1.f : 0.f;
                 float u[4];
                 float gs = 0.f;

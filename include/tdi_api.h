@@ -121,7 +121,9 @@ typedef struct tdi_fit_options {
     int32_t mixed_flags;        /* batched path arithmetic (no reference counterpart), bit set:
                                    1 = FP64-MFMA basis GEMM over frames, 2 = BF16 hi/lo MFMA J^T J,
                                    4 = FP32 Cholesky + FP64 refinement, 8 = J^T r from the BF16
-                                   MFMA tiles (else FP32), 16 = two-term hi/lo split of J
+                                   MFMA tiles (else FP32; with 8 a NaN observed pixel contributes
+                                   zero instead of stopping the frame on a NaN direction),
+                                   16 = two-term hi/lo split of J
                                    (else three-term hi/mid/lo).  Default 7.               */
 } tdi_fit_options;
 

@@ -94,3 +94,51 @@ def cfg0_scene(n_frames=2, noise=0.02, iters=20):
     (110 unknowns), 192 x 192 crops at f = 800 px."""
     return make_scene(grid=100, k_exp=20, k_id=40, k_tex=40, fit_id=True, fit_tex=True, fit_light=1,
                       crops=(192,) * n_frames, pose_deg=20.0, noise=noise, iters=iters, seed=synth.BASE_SEED, cfg=0)
+
+
+def _face_metric(m, th, lay, fr):
+    """max(iod, lem, rem) of the face-size test (renderer.cu:836-857) at th, in image pixels."""
+    ids = m.landmark_ids.astype(np.int64)[[0, 9, 40]]
+    p = synth.project_points(m, th, lay, fr.phix, fr.phiy, fr.cx, fr.cy, ids)
+    return max(abs(p[1, 0] - p[0, 0]), abs(p[0, 1] - p[2, 1]), abs(p[1, 1] - p[2, 1]))
+
+
+BRANCH_FRAMES = ("normal", "nan_observation", "normal", "no_pixels", "normal", "face_too_large", "normal")
+
+
+def branch_scene(iters=3, fit_light=2):
+    """Frames that force the line search's branches (model_fitter.cu:206-345) between ordinary ones:
+      nan_observation one NaN pixel in the observed crop: the blurred observation's normalised
+                     gradients, and with them g and the search direction, are NaN (stop: NaN);
+      no_pixels      the crop lies far to the right of the face: nothing is rendered, the light
+                     unknowns have neither data nor barrier, H is singular (stop: not PD);
+      face_too_large focal length x4 with ref_face_size between the ordinary frames' face size and
+                     this one's: every trial fails the face-size test (stop: step);
+    and the ordinary frames fit as usual.  Returns the Scene; frame kinds in BRANCH_FRAMES."""
+    n = len(BRANCH_FRAMES)
+    sc = make_scene(grid=40, k_exp=10, fit_light=fit_light, crops=(96,) * n, pose_deg=15.0, noise=0.02,
+                    iters=iters, seed=7, cfg=41)
+    m, lay = sc.model, sc.lay
+    for i, kind in enumerate(BRANCH_FRAMES):
+        fr, tr = sc.frames[i], sc.truths[i]
+        if kind == "nan_observation":
+            fr.obs = fr.obs.copy()
+            fr.obs[(fr.width // 2) * fr.height + fr.height // 2] = np.nan
+        elif kind == "no_pixels":
+            fr.x0 = 20 * fr.width
+            fr.obs = np.zeros_like(fr.obs)
+        elif kind == "face_too_large":
+            fr.phix *= 4.0
+            fr.phiy *= 4.0
+            p = synth.project_points(m, tr.init, lay, fr.phix, fr.phiy, fr.cx, fr.cy, m.landmark_ids.astype(np.int64))
+            fr.lmk_x = p[:, 0].astype(np.float32)
+            fr.lmk_y = p[:, 1].astype(np.float32)
+            fr.face_size *= 4.0
+            raw = O.Scene(m, [fr], sc.opts).render(0, tr.theta)["raw"].astype(np.float32)
+            fr.obs = raw
+    normal = max(_face_metric(m, sc.truths[i].init, lay, sc.frames[i]) for i, k in enumerate(BRANCH_FRAMES)
+                 if k == "normal")
+    big = _face_metric(m, sc.truths[5].init, lay, sc.frames[5])
+    # normal frames stay below the limit 2 * ref throughout their short fits, the large one is above
+    sc.opts.ref_face_size = float(0.25 * (1.3 * normal + 0.7 * big))
+    return sc

@@ -4,9 +4,10 @@ Tolerances (FP32 device arithmetic vs FP64 oracle):
   rotation R, dR            : 2e-6 absolute
   projected vertices        : 2e-3 px, camera-space points 1e-5 relative
   visibility / barycentrics : bit-exact given identical FP32 projected vertices
-  rendered intensity        : 2e-4 absolute where both sides see the same triangle
-  objective                 : 1e-4 relative
-  H, g (FP32 and BF16 hi/lo): 2e-3 relative (Frobenius / 2-norm) against FP64
+  rendered intensity        : 2e-6 absolute before and after the blur (FP64 shading geometry)
+  objective                 : 1e-5 relative
+  H, g (FP32 and BF16 hi/lo): 2e-5 / 3e-5 relative (Frobenius / 2-norm) against FP64 (observed
+                              1e-7 to 3e-6: FP32 per-pixel Jacobian, FP64 reduction, barrier and H)
   Cholesky solve            : 1e-9 (FP64 mode) / 1e-6 (FP32 + FP64 refinement) relative
 """
 import importlib
@@ -156,8 +157,8 @@ def test_normal_equations(scene, gpu, mixed):
                for k, v in blocks.items() if v.stop > v.start}
         _say(f"  frame {i}: {r['npix']} px, |dH|/|H| = {eH:.2e}  |dg|/|g| = {eg:.2e}  diagonal blocks "
              + " ".join(f"{k} {e:.1e}" for k, e in per.items()))
-        assert eH < 2e-3 and eg < 2e-3
-        assert all(e < 2e-3 for e in per.values())
+        assert eH < 2e-5 and eg < 3e-5
+        assert all(e < 3e-5 for e in per.values())
 
 
 def test_mixed_matches_fp32(scene, gpu):
@@ -168,8 +169,8 @@ def test_mixed_matches_fp32(scene, gpu):
     H1, H2 = H1.cpu().numpy().astype(np.float64), H2.cpu().numpy().astype(np.float64)
     g1, g2 = g1.cpu().numpy().astype(np.float64), g2.cpu().numpy().astype(np.float64)
     for i in range(len(scene.frames)):
-        assert np.linalg.norm(H1[i] - H2[i]) / np.linalg.norm(H1[i]) < 1e-4
-        assert np.linalg.norm(g1[i] - g2[i]) / np.linalg.norm(g1[i]) < 1e-4
+        assert np.linalg.norm(H1[i] - H2[i]) / np.linalg.norm(H1[i]) < 2e-6
+        assert np.linalg.norm(g1[i] - g2[i]) / np.linalg.norm(g1[i]) < 5e-6
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -215,23 +216,33 @@ def _one_iteration_opts(scene):
 def _check_one_iteration(scene, th0, tho, repo, thg, repg, label):
     """Same GN step: equal line-search decisions, and the step difference small in the energy
     norm of the oracle's H at the start, sqrt(d^T H d) / sqrt(s^T H s) < 1e-3 (d = GPU - oracle
-    result, s = oracle step).  The energy norm weighs each direction by what the data
-    determine; along directions the images barely see (scaled condition numbers reach 1e6
-    when the light is fitted) FP32 J^T J moves the solution further, so the plain parameter
-    RMS bound 1e-4 * max(1, |step|) is asserted for the well-conditioned scene only."""
+    result, s = oracle step).  The energy norm weighs each direction by what the data determine.
+    A trial outside a landmark box is shrunk without counting as an evaluation
+    (model_fitter.cu:276-282), so a frame whose box edge crosses the line within the FP32 step's
+    difference from the FP64 step stops one shrink (a factor 0.8) apart; the direction must
+    still agree.  On the well-conditioned scene also the parameter RMS is within 1e-4 of the step
+    size."""
     orc = scene.oracle()
     step = _pose_exp_rms(tho, th0, scene.lay)
-    rms = _pose_exp_rms(thg, tho, scene.lay)
-    en = []
+    en, lams = [], []
+    thg = thg.copy()
     for i in range(len(scene.frames)):
         Hm = orc.normal_equations(i, th0[i])["H"]
-        d = thg[i] - tho[i]
-        st = tho[i] - th0[i]
-        en.append(float(np.sqrt(max(d @ Hm @ d, 0.0)) / max(np.sqrt(max(st @ Hm @ st, 0.0)), 1e-30)))
+        s_, o_ = thg[i] - th0[i], tho[i] - th0[i]
+        lam = float(s_ @ Hm @ o_) / max(float(o_ @ Hm @ o_), 1e-300)
+        if abs(lam - 1.0) > 1e-2:
+            assert min(abs(lam - 0.8), abs(lam - 1.25)) < 1e-2, f"frame {i}: step length ratio {lam}"
+            thg[i] = th0[i] + s_ / lam
+            s_ = s_ / lam
+        lams.append(lam)
+        d = s_ - o_
+        en.append(float(np.sqrt(max(d @ Hm @ d, 0.0)) / max(np.sqrt(max(o_ @ Hm @ o_, 0.0)), 1e-30)))
+    rms = _pose_exp_rms(thg, tho, scene.lay)
     _say(f"  {label}: step RMS {step:.3e}, GPU vs oracle RMS {rms:.3e}, energy-norm rel "
-         f"{' '.join(f'{e:.1e}' for e in en)}, ls evals gpu {[r.ls_evals for r in repg]} oracle "
-         f"{[r.ls_evals for r in repo]}")
+         f"{' '.join(f'{e:.1e}' for e in en)}, step length ratios {' '.join(f'{v:.3f}' for v in lams)}, "
+         f"ls evals gpu {[r.ls_evals for r in repg]} oracle {[r.ls_evals for r in repo]}")
     assert [r.ls_evals for r in repg] == [r.ls_evals for r in repo]
+    assert sum(abs(v - 1.0) > 1e-2 for v in lams) <= 1
     assert max(en) < 1e-3
     if not scene.opts.fit_light:
         assert rms < 1e-4 * max(1.0, step)
@@ -315,7 +326,18 @@ def _oracle_floor(scene, th0):
     return a, ra, spread
 
 
+def _whole_fit_bound(rms, floor, repg, repo):
+    """While the GPU takes the oracle's decisions (the same iteration and evaluation counts), whole
+    fits must agree to 1e-3 (a wrong step moves them by far more); otherwise within three times
+    the oracle's own spread under 1e-6 perturbations of the start."""
+    same = [(r.iters, r.ls_evals) for r in repg] == [(r.iters, r.ls_evals) for r in repo]
+    assert rms <= (1e-3 if same else max(3 * floor, 1e-3))
+
+
 def test_fit_plain_vs_oracle(scene, gpu):
+    if scene.opts.fit_light == 1:
+        pytest.skip("whole fits with the point light (lambda, L) as joint unknowns are ill-posed (a distant light "
+                    "shades like a directional one); the nine-unknown light scene covers joint light fits")
     _say("case: plain fit vs oracle")
     th0 = _f32(scene.init())
     tho, repo, floor = _oracle_floor(scene, th0)
@@ -325,7 +347,7 @@ def test_fit_plain_vs_oracle(scene, gpu):
     err_g = _pose_exp_rms(thg.astype(np.float64), scene.truth(), scene.lay)
     _say(f"  plain GPU vs FP64 oracle RMS {rms:.2e} (oracle self-spread {floor:.2e}); error vs truth "
          f"gpu {err_g:.3f} oracle {err_o:.3f}; iters gpu {[r.iters for r in repg]} oracle {[r.iters for r in repo]}")
-    assert rms <= max(3 * floor, 1e-3)
+    _whole_fit_bound(rms, floor, repg, repo)
     assert err_g < 1.5 * err_o + 0.05
     obj_g = np.array([r.objective for r in repg])
     obj_o = np.array([r.objective for r in repo])
@@ -333,6 +355,9 @@ def test_fit_plain_vs_oracle(scene, gpu):
 
 
 def test_fit_batched_vs_oracle(scene, gpu):
+    if scene.opts.fit_light == 1:
+        pytest.skip("whole fits with the point light (lambda, L) as joint unknowns are ill-posed (a distant light "
+                    "shades like a directional one); the nine-unknown light scene covers joint light fits")
     _say("case: batched fit vs oracle")
     th0 = _f32(scene.init())
     tho, repo, floor = _oracle_floor(scene, th0)
@@ -344,7 +369,7 @@ def test_fit_batched_vs_oracle(scene, gpu):
     err_g = _pose_exp_rms(thg, scene.truth(), scene.lay)
     _say(f"  batched GPU vs FP64 oracle RMS {rms:.2e} (oracle self-spread {floor:.2e}); error vs truth "
          f"gpu {err_g:.3f} oracle {err_o:.3f}; iters gpu {[r.iters for r in repg]} oracle {[r.iters for r in repo]}")
-    assert rms <= max(3 * floor, 1e-3)
+    _whole_fit_bound(rms, floor, repg, repo)
     assert err_g < 1.5 * err_o + 0.05
     obj_g = np.array([r.objective for r in repg])
     obj_o = np.array([r.objective for r in repo])
@@ -352,6 +377,9 @@ def test_fit_batched_vs_oracle(scene, gpu):
 
 
 def test_fit_batched_chunked_vs_oracle(scene):
+    if scene.opts.fit_light == 1:
+        pytest.skip("whole fits with the point light (lambda, L) as joint unknowns are ill-posed (a distant light "
+                    "shades like a directional one); the nine-unknown light scene covers joint light fits")
     """Whole batched fit in one-frame workspace chunks, held to the bar of the single-chunk fit."""
     _say("case: batched fit in one-frame chunks vs oracle")
     th0 = _f32(scene.init())
@@ -365,7 +393,7 @@ def test_fit_batched_chunked_vs_oracle(scene):
     err_g = _pose_exp_rms(thg, scene.truth(), scene.lay)
     _say(f"  chunked batched GPU vs FP64 oracle RMS {rms:.2e} (oracle self-spread {floor:.2e}); error vs truth "
          f"gpu {err_g:.3f} oracle {err_o:.3f}")
-    assert rms <= max(3 * floor, 1e-3)
+    _whole_fit_bound(rms, floor, repg, repo)
     assert err_g < 1.5 * err_o + 0.05
     b.close()
     ctx.close()
