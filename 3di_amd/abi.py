@@ -48,7 +48,7 @@ class tdi_fit_options(C.Structure):
                 ("use_landmarks", C.c_int32), ("niters_success", C.c_int32),
                 ("data_weight", C.c_float), ("armijo_alpha", C.c_float), ("armijo_beta", C.c_float),
                 ("t_min", C.c_float), ("decrement_tol", C.c_float), ("blur_sigma", C.c_float),
-                ("ref_face_size", C.c_float), ("mixed_flags", C.c_int32)]
+                ("ref_face_size", C.c_float), ("mixed_flags", C.c_int32), ("finetune_coef", C.c_float)]
 
 
 class tdi_fit_report(C.Structure):
@@ -61,7 +61,7 @@ def default_options(**kw) -> tdi_fit_options:
     o = tdi_fit_options(fit_id=0, fit_tex=0, fit_exp=1, fit_light=0, max_iters=100, max_ls=100,
                         ntmp_cap=0, use_landmarks=1, niters_success=6, data_weight=25.0,
                         armijo_alpha=0.4, armijo_beta=0.8, t_min=0.075, decrement_tol=3e-4,
-                        blur_sigma=0.75, ref_face_size=0.0, mixed_flags=7)
+                        blur_sigma=0.75, ref_face_size=0.0, mixed_flags=7, finetune_coef=1.5)
     for k, v in kw.items():
         if not hasattr(o, k):
             raise AttributeError(k)

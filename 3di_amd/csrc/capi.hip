@@ -80,6 +80,7 @@ struct tdi_ctx {
     Layout L{};
     Layout Lx{};       // L with the light appended when it is not an unknown (light sub-fits)
     ModelGPU m{};
+    ModelGPU m_fine{};  // m with the finetune fits' expression bounds (finetune_coef times wider)
     float taps[15];
     Alloc mem;
 };
@@ -135,6 +136,8 @@ struct tdi_batch {
     Workspace ws;
     Alloc mem;
     hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph_fine = nullptr;   // the finetune fit's schedule
+    int finetune = 0;                      // set while a finetune fit runs: bounds and pose hold
     hipStream_t stream = nullptr;     // capture-capable stream owned by the batch (default priority)
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
@@ -160,6 +163,7 @@ void tdi_default_options(tdi_fit_options* o)
     o->blur_sigma = 0.75f;
     o->ref_face_size = 0.f;
     o->mixed_flags = TDI_MIXED_SYNTH_MFMA | TDI_MIXED_JTJ_BF16 | TDI_MIXED_CHOL_F32;
+    o->finetune_coef = 1.5f;
 }
 
 int32_t tdi_num_unknowns(const tdi_model_desc* m, const tdi_fit_options* o) { return make_layout(m, o).K; }
@@ -172,6 +176,10 @@ int tdi_ctx_create(const tdi_model_desc* md, const tdi_fit_options* opts, int de
     if (md->n_vertices <= 0 || md->n_triangles <= 0 || !md->mean_shape || !md->triangles || !md->landmark_ids ||
         !md->tex_mean) {
         set_error("model: empty mesh or missing arrays");
+        return TDI_ERR_ARG;
+    }
+    if (!(opts->finetune_coef > 0.f)) {
+        set_error("finetune_coef %g must be positive", opts->finetune_coef);
         return TDI_ERR_ARG;
     }
     if (opts->fit_light < 0 || opts->fit_light > TDI_LIGHT_SH2) {
@@ -243,6 +251,14 @@ int tdi_ctx_create(const tdi_model_desc* md, const tdi_fit_options* opts, int de
     for (int k = 0; k < L.nexp; ++k) { lb[L.off_exp + k] = md->exp_lb[k]; ub[L.off_exp + k] = md->exp_ub[k]; }
     m.lb = (const float*)up(lb.data(), sizeof(float) * c->Lx.K);
     m.ub = (const float*)up(ub.data(), sizeof(float) * c->Lx.K);
+    // the finetune fits' bounds (logbarrier_initializer.cu:285-286): expression bounds scaled
+    for (int k = 0; k < L.nexp; ++k) {
+        lb[L.off_exp + k] *= opts->finetune_coef;
+        ub[L.off_exp + k] *= opts->finetune_coef;
+    }
+    c->m_fine = m;
+    c->m_fine.lb = (const float*)up(lb.data(), sizeof(float) * c->Lx.K);
+    c->m_fine.ub = (const float*)up(ub.data(), sizeof(float) * c->Lx.K);
     for (void* p : c->mem.ptrs)
         if (!p) { c->mem.release(); delete c; set_error("hipMalloc failed for the model"); return TDI_ERR_NOMEM; }
     if (hipDeviceSynchronize() != hipSuccess) { c->mem.release(); delete c; set_error("upload failed"); return TDI_ERR_HIP; }
@@ -498,6 +514,7 @@ extern "C" int tdi_batch_destroy(tdi_batch* b)
     if (!b) return TDI_OK;
     (void)hipSetDevice(b->ctx->device);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->graph_fine) (void)hipGraphExecDestroy(b->graph_fine);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->ev_in) (void)hipEventDestroy(b->ev_in);
     if (b->ev_out) (void)hipEventDestroy(b->ev_out);
@@ -507,6 +524,10 @@ extern "C" int tdi_batch_destroy(tdi_batch* b)
 }
 
 // ---------------------------------------------------------------------------- evaluation
+// The model whose coefficient bounds the barrier uses: the finetune fits' widened expression
+// bounds while a finetune fit runs (model_fitter.cu:409-410), else the regular ones.
+static const ModelGPU& bounds_model(const tdi_batch* b) { return b->finetune ? b->ctx->m_fine : b->ctx->m; }
+
 // Objective at theta for the frames in `mask`: synthesis, face check, raster, resolve,
 // barrier objective, shading, blur, gradients (Renderer::render + evaluate_objective_function).
 // full_crop: blur the whole crops (the rendered image is an output), else only the region
@@ -528,7 +549,7 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
     if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, w.box, s))) return rc;
     if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, nullptr, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
         return rc;
-    if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
+    if ((rc = launch_barrier(bounds_model(b), L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
         return rc;
     if ((rc = launch_shade(c->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix,
@@ -559,7 +580,7 @@ static int eval_objective(tdi_batch* b, FrameList fl, const float* theta, int mi
     if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
     if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, w.box, s))) return rc;
     if ((rc = launch_visible_box(w.frames, fl, w.keys, w.max_area, w.tcount, w.tmax, w.box, s))) return rc;
-    if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
+    if ((rc = launch_barrier(bounds_model(b), L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
         return rc;
     if ((rc = launch_shade_keys(c->m, L, w.frames, fl, theta, w.P0, w.vproj, w.keys, w.box, w.img, s))) return rc;
@@ -591,8 +612,8 @@ static int eval_reused(tdi_batch* b, FrameList fl, const float* theta, hipStream
     if ((rc = launch_resolve(c->m, w.frames, fl, w.vproj, w.keys, w.cum, w.vis, w.npix, nullptr, w.pvert, w.pbary,
                              w.max_area, w.tcount, w.tmax, w.box, s)))
         return rc;
-    if ((rc = launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj, w.objb,
-                             w.H, w.g, s)))
+    if ((rc = launch_barrier(bounds_model(b), L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
+                             w.objb, w.H, w.g, s)))
         return rc;
     return launch_grads(w.frames, fl, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
                         w.pcoef, w.objp, w.obj, w.max_area, s);
@@ -610,8 +631,8 @@ static int eval_jacobian(tdi_batch* b, FrameList fl, const float* theta, int mix
                         mean_crop(w), s)))
         return rc;
     if ((rc = launch_normal_eq(b, fl, mixed, s, &L))) return rc;
-    return launch_barrier(c->m, L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj, w.objb,
-                          w.H, w.g, s);
+    return launch_barrier(bounds_model(b), L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj,
+                          w.objb, w.H, w.g, s);
 }
 
 // Objective, gradient and Gauss-Newton matrix at theta (get_obj_hessian_and_gradient_multiframe).
@@ -887,11 +908,12 @@ int tdi_last_pixel_counts(tdi_batch* b, int32_t* npix)
 // fit_3DMM_shape_rigid (model_fitter.cu:121-375) one frame at a time: the host owns the
 // loop and reads the objective, the direction and the status back after every step, like
 // the reference's blocking cudaMemcpy pattern; FP32 J^T J, FP64 Cholesky, no graph.
-int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
+static int fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
 {
     tdi_ctx* c = b->ctx;
     const tdi_fit_options& o = c->opts;
     const int K = c->L.K, off_tz = c->L.off_pose + 2;
+    const int hold0 = b->finetune ? c->L.off_pose : 0, hold1 = b->finetune ? c->L.off_pose + 6 : 0;
     hipStream_t s = 0;
     Workspace& w = b->ws;
     std::vector<float> th(K), tt(K), delta(K), g(K);
@@ -928,7 +950,7 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
             int inner = 0;
             while (inner < o.max_ls) {
                 if (t < o.t_min) { terminate = true; rep.stop |= TDI_STOP_STEP; break; }
-                for (int k = 0; k < K; ++k) tt[k] = th[k] + t * delta[k];
+                for (int k = 0; k < K; ++k) tt[k] = (k >= hold0 && k < hold1) ? th[k] : th[k] + t * delta[k];
                 if (tt[off_tz] < 0.f) break;
                 TDI_HIP(hipMemcpyAsync(w.theta_try, tt.data(), sizeof(float) * K, hipMemcpyHostToDevice, s));
                 if ((rc = eval_objective(b, all_frames(1), w.theta_try, 0, s))) return rc;
@@ -963,6 +985,22 @@ int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
         std::memcpy(theta_host + (size_t)i * K, th.data(), sizeof(float) * K);
     }
     return TDI_OK;
+}
+
+int tdi_fit_plain(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
+{
+    if (!b || !theta_host || !reports) { set_error("tdi_fit_plain: null argument"); return TDI_ERR_ARG; }
+    b->finetune = 0;
+    return fit_plain(b, theta_host, reports);
+}
+
+int tdi_fit_plain_finetune(tdi_batch* b, float* theta_host, tdi_fit_report* reports)
+{
+    if (!b || !theta_host || !reports) { set_error("tdi_fit_plain_finetune: null argument"); return TDI_ERR_ARG; }
+    b->finetune = 1;
+    const int rc = fit_plain(b, theta_host, reports);
+    b->finetune = 0;
+    return rc;
 }
 
 // ---------------------------------------------------------------------------- light sub-fits
@@ -1126,6 +1164,8 @@ static int record_fit(tdi_batch* b, hipStream_t s)
     const tdi_fit_options& o = c->opts;
     Workspace& w = b->ws;
     const int n = w.C, K = c->L.K, off_tz = c->L.off_pose + 2;
+    // the finetune fit's trials keep the pose (model_fitter.cu:502)
+    const int hold0 = b->finetune ? c->L.off_pose : 0, hold1 = b->finetune ? c->L.off_pose + 6 : 0;
     // evaluation rounds of the line search: steps t = beta^k >= t_min, capped by max_ls
     int rounds = 0;
     for (double t = 1.0; t >= o.t_min && rounds < o.max_ls; t *= o.armijo_beta) ++rounds;
@@ -1149,7 +1189,7 @@ static int record_fit(tdi_batch* b, hipStream_t s)
         if ((rc = launch_ls_init(n, K, w.st, w.status, w.delta, w.g, w.obj, w.searching, s))) return rc;
         for (int r = 0; r < rounds; ++r) {
             if ((rc = launch_ls_propose(n, K, off_tz, o.t_min, w.st, w.theta, w.delta, w.theta_try, w.searching,
-                                        w.evalmask, s)))
+                                        w.evalmask, hold0, hold1, s)))
                 return rc;
             if ((rc = launch_compact_frames(n, w.evalmask, w.act_e, s))) return rc;
             if ((rc = eval_objective(b, FrameList{w.act_e, w.C}, w.theta_try, o.mixed_flags, s))) return rc;
@@ -1158,13 +1198,13 @@ static int record_fit(tdi_batch* b, hipStream_t s)
                 return rc;
         }
         if ((rc = launch_ls_propose(n, K, off_tz, o.t_min, w.st, w.theta, w.delta, w.theta_try, w.searching, w.evalmask,
-                                    s)))
+                                    hold0, hold1, s)))
             return rc;
     }
     return TDI_OK;
 }
 
-extern "C" int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream)
+static int fit_batched(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream)
 {
     tdi_ctx* c = b->ctx;
     const tdi_fit_options& o = c->opts;
@@ -1191,17 +1231,19 @@ extern "C" int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* repor
         TDI_HIP(hipMemcpyAsync(w.st, hst.data(), sizeof(FitState) * w.C, hipMemcpyHostToDevice, s));
         TDI_HIP(hipMemcpyAsync(w.theta, theta + (size_t)i0 * K, sizeof(float) * n * K, hipMemcpyDeviceToDevice, s));
         if (use_graph) {
-            if (!b->graph) {
+            // one captured schedule per kind of fit (the finetune fit's bounds and pose hold differ)
+            hipGraphExec_t& ge = b->finetune ? b->graph_fine : b->graph;
+            if (!ge) {
                 hipGraph_t g;
                 TDI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 rc = record_fit(b, s);
                 hipError_t ec = hipStreamEndCapture(s, &g);
                 if (rc) return rc;
                 if (ec != hipSuccess) { set_error("graph capture failed: %s", hipGetErrorString(ec)); return TDI_ERR_HIP; }
-                TDI_HIP(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+                TDI_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
                 (void)hipGraphDestroy(g);
             }
-            TDI_HIP(hipGraphLaunch(b->graph, s));
+            TDI_HIP(hipGraphLaunch(ge, s));
         } else {
             if ((rc = record_fit(b, s))) return rc;
         }
@@ -1236,4 +1278,20 @@ extern "C" int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* repor
     TDI_HIP(hipEventRecord(b->ev_out, s));
     TDI_HIP(hipStreamWaitEvent(caller, b->ev_out, 0));
     return TDI_OK;
+}
+
+extern "C" int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream)
+{
+    if (!b || !theta) { set_error("tdi_fit_batched: null argument"); return TDI_ERR_ARG; }
+    b->finetune = 0;
+    return fit_batched(b, theta, reports, stream);
+}
+
+extern "C" int tdi_fit_batched_finetune(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream)
+{
+    if (!b || !theta) { set_error("tdi_fit_batched_finetune: null argument"); return TDI_ERR_ARG; }
+    b->finetune = 1;
+    const int rc = fit_batched(b, theta, reports, stream);
+    b->finetune = 0;
+    return rc;
 }

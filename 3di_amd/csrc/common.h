@@ -362,8 +362,9 @@ int launch_cholesky(int n, int K, const double* H, const float* g, float* delta,
                     hipStream_t s);
 int launch_ls_init(int n, int K, FitState* st, const int* chol_status, const float* delta, const float* g,
                    const double* obj, int* searching, hipStream_t s);
+// unknowns [hold0, hold1) keep theta's values in the trials (hold0 == hold1: none)
 int launch_ls_propose(int n, int K, int off_tz, float t_min, FitState* st, const float* theta, const float* delta,
-                      float* theta_try, int* searching, int* evalmask, hipStream_t s);
+                      float* theta_try, int* searching, int* evalmask, int hold0, int hold1, hipStream_t s);
 int launch_ls_decide(int n, int K, FitState* st, const int* evalmask, int* searching, const double* obj,
                      const double* objb, const int* face_ok, float* theta, const float* theta_try,
                      const tdi_fit_options& o, hipStream_t s);

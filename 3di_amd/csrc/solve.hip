@@ -376,9 +376,12 @@ __global__ void k_ls_init(int n, int K, FitState* __restrict__ st, const int* __
     st[f] = s;
 }
 
+// Trials theta + t delta; unknowns [hold0, hold1) keep theta's value (the finetune fit's pose,
+// model_fitter.cu:502).
 __global__ void k_ls_propose(int n, int K, int off_tz, float t_min, FitState* __restrict__ st,
                              const float* __restrict__ theta, const float* __restrict__ delta,
-                             float* __restrict__ theta_try, int* __restrict__ searching, int* __restrict__ evalmask)
+                             float* __restrict__ theta_try, int* __restrict__ searching, int* __restrict__ evalmask,
+                             int hold0, int hold1)
 {
     const int f = blockIdx.x;
     if (f >= n) return;
@@ -393,7 +396,8 @@ __global__ void k_ls_propose(int n, int K, int off_tz, float t_min, FitState* __
                 s.stop |= TDI_STOP_STEP;
                 searching[f] = 0;
             } else {
-                const float tz = theta[(size_t)f * K + off_tz] + s.t * delta[(size_t)f * K + off_tz];
+                const bool held = off_tz >= hold0 && off_tz < hold1;
+                const float tz = theta[(size_t)f * K + off_tz] + (held ? 0.f : s.t * delta[(size_t)f * K + off_tz]);
                 if (tz < 0.f) {
                     s.searching = 0;
                     searching[f] = 0;
@@ -410,7 +414,8 @@ __global__ void k_ls_propose(int n, int K, int off_tz, float t_min, FitState* __
     if (!go) return;
     const float t = st[f].t;
     for (int k = threadIdx.x; k < K; k += blockDim.x)
-        theta_try[(size_t)f * K + k] = theta[(size_t)f * K + k] + t * delta[(size_t)f * K + k];
+        theta_try[(size_t)f * K + k] = (k >= hold0 && k < hold1) ? theta[(size_t)f * K + k]
+                                                                 : theta[(size_t)f * K + k] + t * delta[(size_t)f * K + k];
 }
 
 __global__ void k_ls_decide(int n, int K, FitState* __restrict__ st, const int* __restrict__ evalmask,
@@ -576,10 +581,10 @@ int launch_ls_init(int n, int K, FitState* st, const int* chol_status, const flo
 }
 
 int launch_ls_propose(int n, int K, int off_tz, float t_min, FitState* st, const float* theta, const float* delta,
-                      float* theta_try, int* searching, int* evalmask, hipStream_t s)
+                      float* theta_try, int* searching, int* evalmask, int hold0, int hold1, hipStream_t s)
 {
     hipLaunchKernelGGL(k_ls_propose, dim3(n), dim3(64), 0, s, n, K, off_tz, t_min, st, theta, delta, theta_try,
-                       searching, evalmask);
+                       searching, evalmask, hold0, hold1);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

@@ -52,6 +52,8 @@ def lib():
         L.tdi_cholesky_solve.argtypes = [C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, C.c_int32, _vp]
         L.tdi_fit_plain.argtypes = [_vp, _vp, _vp]
         L.tdi_fit_batched.argtypes = [_vp, _vp, _vp, _vp]
+        L.tdi_fit_plain_finetune.argtypes = [_vp, _vp, _vp]
+        L.tdi_fit_batched_finetune.argtypes = [_vp, _vp, _vp, _vp]
         L.tdi_blur_taps.argtypes = [C.c_float, _vp]
         L.tdi_stage_times.argtypes = [_vp, _vp, C.c_int32, C.c_int32, _vp]
         L.tdi_last_pixel_counts.argtypes = [_vp, _vp]
@@ -241,10 +243,13 @@ class Batch:
         return out
 
     # -------------------------------------------------------------- fits
-    def fit_plain(self, theta0) -> Tuple[np.ndarray, List]:
+    def fit_plain(self, theta0, finetune: bool = False) -> Tuple[np.ndarray, List]:
+        """fit_3DMM_shape_rigid one frame at a time (tdi_fit_plain), or with finetune=True
+        fit_3DMM_epsilon_finetune (tdi_fit_plain_finetune).  theta0: host [n, K]."""
         th = np.ascontiguousarray(np.array(theta0, np.float32).reshape(self.n, self.K))
         reps = (abi.tdi_fit_report * self.n)()
-        _check(lib().tdi_fit_plain(self.h, th.ctypes.data, C.addressof(reps)))
+        fn = lib().tdi_fit_plain_finetune if finetune else lib().tdi_fit_plain
+        _check(fn(self.h, th.ctypes.data, C.addressof(reps)))
         return th, list(reps)
 
     def _light_fit(self, fn, theta, lam, Lint):
@@ -269,11 +274,13 @@ class Batch:
         """fit_3DMM_Lintensity (model_fitter.cu:1084-1180) on every frame."""
         return self._light_fit(lib().tdi_fit_lintensity, theta, lam, Lint)
 
-    def fit_batched(self, theta):
-        """theta: device tensor [n, K], updated in place.  Returns the per-frame reports."""
+    def fit_batched(self, theta, finetune: bool = False):
+        """theta: device tensor [n, K], updated in place.  Returns the per-frame reports.  finetune:
+        the expression finetune (tdi_fit_batched_finetune) instead of the shape fit."""
         th = self._theta(theta)
         reps = (abi.tdi_fit_report * self.n)()
-        _check(lib().tdi_fit_batched(self.h, _dptr(th), C.addressof(reps), _stream()))
+        fn = lib().tdi_fit_batched_finetune if finetune else lib().tdi_fit_batched
+        _check(fn(self.h, _dptr(th), C.addressof(reps), _stream()))
         return th, list(reps)
 
 

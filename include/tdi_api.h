@@ -125,6 +125,9 @@ typedef struct tdi_fit_options {
                                    zero instead of stopping the frame on a NaN direction),
                                    16 = two-term hi/lo split of J
                                    (else three-term hi/mid/lo).  Default 7.               */
+    float finetune_coef;        /* FINETUNE_COEF (config.cu:100: 1.5): the expression bounds of the
+                                   finetune fits are these times the regular ones
+                                   (logbarrier_initializer.cu:285-286)                              */
 } tdi_fit_options;
 
 #define TDI_LIGHT_POINT 1
@@ -295,6 +298,17 @@ int tdi_fit_plain(tdi_batch* b, float* theta, tdi_fit_report* reports);
  * BF16 hi/lo J^T J on MFMA, FP32 Cholesky + FP64 refinement, per-frame line-search masks,
  * iterations captured in a hipGraph.  theta: [n][K] DEVICE, in/out; reports: HOST [n]. */
 int tdi_fit_batched(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream);
+
+/* Expression finetune: fit_3DMM_epsilon_finetune (model_fitter.cu:397-661).  The same
+ * Gauss-Newton fit with the expression bounds widened by finetune_coef (restored afterwards,
+ * 409-410, 657-658) and line-search trials that move every unknown but the six pose unknowns
+ * (set_xtmp over Ktotal - 6, 502): the normal equations and the decrement test still cover all
+ * of them.  The pose stays at theta's (the reference's trials carry the pose its previous line
+ * search left in ov_linesearch, which is theta's after an accepted step).  _plain: host theta,
+ * one frame at a time as tdi_fit_plain; _batched: device theta, every frame on the device as
+ * tdi_fit_batched. */
+int tdi_fit_plain_finetune(tdi_batch* b, float* theta, tdi_fit_report* reports);
+int tdi_fit_batched_finetune(tdi_batch* b, float* theta, tdi_fit_report* reports, void* stream);
 
 /* Frames in the batch and unknowns per frame. */
 int32_t tdi_batch_size(tdi_batch* b);

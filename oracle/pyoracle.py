@@ -50,7 +50,7 @@ def lib():
         L.orc_barrier.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _dp]
         L.orc_barrier.restype = C.c_double
         L.orc_cholesky_solve.argtypes = [C.c_int, _dp, _dp, _dp]
-        L.orc_fit.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, C.c_void_p, C.c_int]
+        L.orc_fit.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, C.c_void_p, C.c_int, C.c_int]
         L.orc_max_threads.restype = C.c_int
         L.orc_fit_light.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_void_p, C.c_int,
                                     C.c_int]
@@ -162,11 +162,11 @@ class Scene:
         obj = lib().orc_barrier(self.md.ref, self._f(i), C.byref(self.opts), p(_d(theta)), p(g), p(H))
         return obj, g, H.reshape(K, K)
 
-    def fit(self, theta0, n_threads=0, frames=None):
+    def fit(self, theta0, n_threads=0, frames=None, finetune=False):
         n = len(self.frames)
         th = np.ascontiguousarray(np.array(theta0, np.float64).reshape(n, self.K))
         reps = (abi.tdi_fit_report * n)()
-        lib().orc_fit(self.md.ref, n, self.fd.ptr(0), C.byref(self.opts), p(th), reps, n_threads)
+        lib().orc_fit(self.md.ref, n, self.fd.ptr(0), C.byref(self.opts), p(th), reps, n_threads, int(finetune))
         return th, list(reps)
 
 

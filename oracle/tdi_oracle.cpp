@@ -263,6 +263,7 @@ struct Frame {
     Layout L;
     double taps[15];
     std::vector<double> obs_blur;   // blurred observation
+    bool finetune = false;          // fit_3DMM_epsilon_finetune: widened expression bounds, pose held
 };
 
 struct Geometry {
@@ -378,7 +379,17 @@ double barrier(const Frame& F, const double* th, const Geometry& G, double* grad
     };
     bound(L.off_id, L.nid, m->id_lb, m->id_ub);
     bound(L.off_tex, L.ntex, m->tex_lb, m->tex_ub);
-    bound(L.off_exp, L.nexp, m->exp_lb, m->exp_ub);
+    if (F.finetune) {
+        // the finetune fits' bounds: FINETUNE_COEF times the regular ones (logbarrier_initializer.cu:285-286)
+        std::vector<float> lb(L.nexp), ub(L.nexp);
+        for (int k = 0; k < L.nexp; ++k) {
+            lb[k] = m->exp_lb[k] * F.o->finetune_coef;
+            ub[k] = m->exp_ub[k] * F.o->finetune_coef;
+        }
+        bound(L.off_exp, L.nexp, lb.data(), ub.data());
+    } else {
+        bound(L.off_exp, L.nexp, m->exp_lb, m->exp_ub);
+    }
     if (!F.o->use_landmarks) return obj;
 
     const tdi_frame_desc* f = F.f;
@@ -713,7 +724,11 @@ void fit_one(const Frame& F, double* th, tdi_fit_report* rep)
         Eval Es;
         while (inner < o->max_ls) {
             if (t < o->t_min) { terminate = true; rep->stop |= TDI_STOP_STEP; break; }
-            for (int k = 0; k < K; ++k) tt[k] = th[k] + t * delta[k];
+            // fit_3DMM_epsilon_finetune moves all but the six pose unknowns (model_fitter.cu:502)
+            for (int k = 0; k < K; ++k) {
+                const bool held = F.finetune && k >= F.L.off_pose && k < F.L.off_pose + 6;
+                tt[k] = held ? th[k] : th[k] + t * delta[k];
+            }
             if (tt[F.L.off_pose + 2] < 0) break;                       // model_fitter.cu:253-259
             evaluate(F, tt.data(), false, Es);
             if (!Es.face_ok || std::isnan(Es.obj_barrier)) { t *= o->armijo_beta; continue; }
@@ -1229,7 +1244,7 @@ int orc_cholesky_solve(int K, const double* Hm, const double* g, double* delta)
 // Fit n frames independently (OpenMP over frames, dynamic schedule for uneven crops).
 // theta: [n][K] double in/out.  Returns TDI_OK.
 int orc_fit(const tdi_model_desc* m, int n, const tdi_frame_desc* frames, const tdi_fit_options* o,
-            double* theta, tdi_fit_report* reports, int n_threads)
+            double* theta, tdi_fit_report* reports, int n_threads, int finetune)
 {
     const int K = make_layout(m, o).K;
     if (n_threads > 0) omp_set_num_threads(n_threads);
@@ -1237,6 +1252,7 @@ int orc_fit(const tdi_model_desc* m, int n, const tdi_frame_desc* frames, const 
     for (int i = 0; i < n; ++i) {
         Frame F;
         setup_frame(F, m, &frames[i], o);
+        F.finetune = finetune != 0;
         fit_one(F, theta + (size_t)i * K, &reports[i]);
     }
     return TDI_OK;

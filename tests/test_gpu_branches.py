@@ -104,3 +104,44 @@ def test_ordinary_frames_unaffected_by_forced_neighbours():
              f"max |dtheta| {d:.1e}")
         assert (rep_all[i].stop, rep_all[i].ls_evals) == (rep_n[j].stop, rep_n[j].ls_evals)
         assert d < 1e-5
+
+
+@pytest.mark.parametrize("path", ["batched", "plain"])
+def test_finetune_vs_oracle(path):
+    """fit_3DMM_epsilon_finetune on the GPU from a start outside the regular expression bounds
+    and inside the widened ones (as test_oracle_branches): the pose comes back bit for bit, the
+    first iteration takes the oracle's step (energy norm 1e-3) with the same evaluation count,
+    and the regular shape fit leaves theta unchanged, as the oracle's does."""
+    _say(f"case: expression finetune ({path})")
+    sc = scenes.make_scene(grid=40, k_exp=10, fit_light=0, crops=(96, 120, 80), pose_deg=15.0, noise=0.02, iters=1,
+                           seed=7, cfg=43, use_landmarks=0)
+    lay = sc.lay
+    e0, p = lay["exp"], lay["pose"]
+    th0 = np.asarray(sc.init(), np.float32)
+    th0[:, e0] = np.float32(1.2 * sc.model.exp_ub[0])
+    ctx = tdi.Context(sc.model, sc.opts)
+    b = tdi.Batch(ctx, sc.frames)
+
+    def run(finetune):
+        if path == "batched":
+            t, r = b.fit_batched(torch.from_numpy(th0.copy()).cuda(), finetune=finetune)
+            return t.cpu().numpy().astype(np.float64), r
+        t, r = b.fit_plain(th0.copy(), finetune=finetune)
+        return t.astype(np.float64), r
+
+    th_r, _ = run(False)
+    thg, repg = run(True)
+    b.close()
+    ctx.close()
+    assert np.array_equal(th_r, th0.astype(np.float64))
+    assert np.array_equal(thg[:, p:p + 6], th0[:, p:p + 6].astype(np.float64))
+    orc = sc.oracle()
+    tho, repo = orc.fit(th0.astype(np.float64), finetune=True)
+    assert [r.ls_evals for r in repg] == [r.ls_evals for r in repo]
+    for i in range(len(sc.frames)):
+        Hm = orc.normal_equations(i, th0[i].astype(np.float64))["H"]
+        s_, o_ = thg[i] - th0[i], tho[i] - th0[i]
+        d = s_ - o_
+        en = np.sqrt(max(d @ Hm @ d, 0.0)) / np.sqrt(o_ @ Hm @ o_)
+        _say(f"  frame {i}: ls {repg[i].ls_evals}, energy-norm rel {en:.1e}")
+        assert en < 1e-3

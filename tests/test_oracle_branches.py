@@ -23,6 +23,7 @@ import numpy as np
 
 import pyoracle as O
 import scenes
+from scenes import make_scene
 
 abi = importlib.import_module("3di_amd.abi")
 
@@ -76,3 +77,28 @@ def test_decrement_test_on_the_first_trial():
     for i, r in enumerate(reps):
         assert r.stop == 2 and r.ls_evals == 1 and r.iters == 1, (r.stop, r.ls_evals, r.iters)
         assert np.array_equal(th[i], th0[i])
+
+
+def test_finetune_holds_the_pose_and_widens_the_expression_bounds():
+    """fit_3DMM_epsilon_finetune (model_fitter.cu:397-661): trials move every unknown but the pose
+    (set_xtmp over Ktotal - 6, 502), so the pose is returned bit for bit, and the expression
+    bounds are FINETUNE_COEF times wider (409-410).  A start whose first expression coefficient
+    lies at 1.2 times its regular bound (landmark barrier off, so the bounds alone decide
+    feasibility) is outside the shape fit's feasible set -- its objective is NaN, no trial is
+    ever accepted and theta comes back unchanged -- and inside the finetune fit's, which moves
+    the expressions and keeps them within the widened bounds."""
+    sc = make_scene(grid=40, k_exp=10, fit_light=0, crops=(96, 96, 96), pose_deg=15.0, noise=0.02, iters=6,
+                    seed=7, cfg=43, use_landmarks=0)
+    lay = sc.lay
+    e0, p = lay["exp"], lay["pose"]
+    ub = sc.model.exp_ub.astype(np.float64)
+    th0 = np.asarray(sc.init(), np.float32).astype(np.float64)
+    th0[:, e0] = np.float32(1.2 * ub[0])
+    orc = O.Scene(sc.model, sc.frames, sc.opts)
+    th_r, rep_r = orc.fit(th0)
+    assert np.array_equal(th_r, th0)
+    th, reps = orc.fit(th0, finetune=True)
+    assert np.array_equal(th[:, p:p + 6], th0[:, p:p + 6])
+    assert all(r.ls_evals > 0 for r in reps)
+    assert not np.array_equal(th[:, e0:p], th0[:, e0:p])
+    assert np.all(np.abs(th[:, e0:p]) < sc.opts.finetune_coef * ub[:p - e0])
