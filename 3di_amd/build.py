@@ -18,7 +18,7 @@ HEADERS = sorted(glob.glob(os.path.join(HERE, "csrc", "*.h"))) + [os.path.join(R
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HOST_LIB = os.path.join(HERE, "libtdi_host.so")
 HOST_SOURCES = sorted(glob.glob(os.path.join(HERE, "host", "*.cpp")))
-HOST_HEADERS = [os.path.join(ROOT, "include", "tdi_host.h")]
+HOST_HEADERS = [os.path.join(ROOT, "include", "tdi_host.h"), os.path.join(ROOT, "include", "tdi_model_fitter.h")]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # no packed FP32 instructions anywhere in the device code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -30,6 +30,7 @@
 
 #include "../../include/tdi_api.h"
 #include "../../include/tdi_host.h"
+#include "../../include/tdi_model_fitter.h"
 
 namespace tdi_host {
 
@@ -173,30 +174,45 @@ private:
 };
 
 // ---------------------------------------------------------------------------- fits
-// fit_3DMM_shape_rigid (model_fitter.cu:121-375) for every frame of the batch.
-static int fit_3DMM_shape_rigid(tdi_batch* b, int n, int K, bool batched, float* theta, tdi_fit_report* reports)
+// fit_3DMM_shape_rigid (model_fitter.cu:121-375) or fit_3DMM_epsilon_finetune (397-661) for every
+// frame of the batch; the batched path takes a device copy of the unknowns.
+static int shape_fit(tdi_batch* b, int n, int K, bool batched, bool finetune, float* theta, tdi_fit_report* reports)
 {
-    if (!batched) return tdi_fit_plain(b, theta, reports);
+    if (!batched) return finetune ? tdi_fit_plain_finetune(b, theta, reports) : tdi_fit_plain(b, theta, reports);
     float* d = nullptr;
     const size_t bytes = sizeof(float) * (size_t)n * K;
     if (hipMalloc((void**)&d, bytes) != hipSuccess) return TDI_ERR_NOMEM;
     int rc = TDI_OK;
     if (hipMemcpy(d, theta, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = TDI_ERR_HIP;
-    if (!rc) rc = tdi_fit_batched(b, d, reports, nullptr);
+    if (!rc) rc = finetune ? tdi_fit_batched_finetune(b, d, reports, nullptr) : tdi_fit_batched(b, d, reports, nullptr);
     if (!rc && hipDeviceSynchronize() != hipSuccess) rc = TDI_ERR_HIP;
     if (!rc && hipMemcpy(theta, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = TDI_ERR_HIP;
     (void)hipFree(d);
     return rc;
 }
 
-// fit_3DMM_lambdas (model_fitter.cu:981-1073) followed by the light update the next shape
-// fit reads.
-static int fit_3DMM_lambdas(tdi_batch* b, int n, const float* theta, float* lam, float* Lint)
+// fit_3DMM_lambdas / fit_3DMM_Lintensity (model_fitter.cu:981-1180) followed by the light update
+// the next shape fit reads.
+static int light_fit(tdi_batch* b, int n, const float* theta, float* lam, float* Lint, bool intensity,
+                     tdi_fit_report* reports)
 {
     std::vector<tdi_fit_report> r(n);
-    int rc = tdi_fit_lambdas(b, theta, lam, Lint, r.data());
+    tdi_fit_report* rp = reports ? reports : r.data();
+    int rc = intensity ? tdi_fit_lintensity(b, theta, lam, Lint, rp) : tdi_fit_lambdas(b, theta, lam, Lint, rp);
     if (rc) return rc;
     return tdi_batch_set_light(b, lam, Lint);
+}
+
+static int fit_3DMM_lambdas(tdi_batch* b, int n, const float* theta, float* lam, float* Lint)
+{
+    return light_fit(b, n, theta, lam, Lint, false, nullptr);
+}
+
+static bool all_success(const tdi_fit_report* r, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (!r[i].success) return false;
+    return true;
 }
 
 // Logbarrier_Initializer::initialize_with_orthographic_t (logbarrier_initializer.cu:1862-1914),
@@ -435,11 +451,11 @@ int tdi_fit_video_frames(tdi_batch* b, int32_t twostage_illum, int32_t batched, 
     for (int i = 0; i < n; ++i) Lint[i] = 0.005f;
     int rc = tdi_batch_set_light(b, lam, Lint);
     if (!rc) rc = tdi_host::fit_3DMM_lambdas(b, n, theta, lam, Lint);
-    if (!rc) rc = tdi_host::fit_3DMM_shape_rigid(b, n, K, batched != 0, theta, reports);
+    if (!rc) rc = tdi_host::shape_fit(b, n, K, batched != 0, false, theta, reports);
     // TWOSTAGE_ILLUM (video_fitter.cu:577-615): lambda + shape, twice more
     for (int round = 0; !rc && twostage_illum && round < 2; ++round) {
         rc = tdi_host::fit_3DMM_lambdas(b, n, theta, lam, Lint);
-        if (!rc) rc = tdi_host::fit_3DMM_shape_rigid(b, n, K, batched != 0, theta, reports);
+        if (!rc) rc = tdi_host::shape_fit(b, n, K, batched != 0, false, theta, reports);
     }
     return rc;
 }
@@ -483,6 +499,237 @@ int tdi_frame_geometry(int32_t n, const float* lmk_x, const float* lmk_y, const 
             for (int i = 0; i < 4; ++i) { cams_out[4 * (size_t)f + i] = 0.f; crop[4 * (size_t)f + i] = 0; }
             for (int i = 0; i < NL; ++i) lmk_x_out[o + i] = lmk_y_out[o + i] = 0.f;
         }
+    }
+    return TDI_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------- the reference's names
+namespace tdi {
+
+bool fit_3DMM_shape_rigid(FitContext& ctx, float* theta, tdi_fit_report* reports)
+{
+    if (!ctx.batch || !theta || !reports) return false;
+    const int n = tdi_batch_size(ctx.batch), K = tdi_batch_unknowns(ctx.batch);
+    if (tdi_host::shape_fit(ctx.batch, n, K, ctx.batched, false, theta, reports)) return false;
+    return tdi_host::all_success(reports, n);
+}
+
+bool fit_3DMM_epsilon_finetune(FitContext& ctx, float* theta, tdi_fit_report* reports)
+{
+    if (!ctx.batch || !theta || !reports) return false;
+    const int n = tdi_batch_size(ctx.batch), K = tdi_batch_unknowns(ctx.batch);
+    if (tdi_host::shape_fit(ctx.batch, n, K, ctx.batched, true, theta, reports)) return false;
+    return tdi_host::all_success(reports, n);
+}
+
+bool fit_3DMM_lambdas(FitContext& ctx, const float* theta, float* lambdas, float* Lintensity, tdi_fit_report* reports)
+{
+    if (!ctx.batch || !theta || !lambdas || !Lintensity || !reports) return false;
+    const int n = tdi_batch_size(ctx.batch);
+    if (tdi_host::light_fit(ctx.batch, n, theta, lambdas, Lintensity, false, reports)) return false;
+    return tdi_host::all_success(reports, n);
+}
+
+bool fit_3DMM_Lintensity(FitContext& ctx, const float* theta, float* lambdas, float* Lintensity,
+                         tdi_fit_report* reports)
+{
+    if (!ctx.batch || !theta || !lambdas || !Lintensity || !reports) return false;
+    const int n = tdi_batch_size(ctx.batch);
+    if (tdi_host::light_fit(ctx.batch, n, theta, lambdas, Lintensity, true, reports)) return false;
+    return tdi_host::all_success(reports, n);
+}
+
+} // namespace tdi
+
+// ---------------------------------------------------------------------------- per-frame stage of a video
+namespace tdi_host {
+
+// update_shape_single_ref_size's sizes (video_fitter.cu:273-280): REF_EXTRA = 5 (145).
+static std::vector<float> reference_sizes(float R, int nres)
+{
+    const float RE = 5.0f, RE2 = RE / 2.0f;
+    std::vector<float> v({R, R + RE + 5.0f, R + RE + 3.0f, R + RE + 9.0f, R + RE + 7.0f, R + RE - 9.0f,
+                          R + RE - 6.0f, R + RE - 3.0f, R + RE2 + 5.0f, R + RE2 + 3.0f, R + RE2 + 9.0f,
+                          R + RE2 + 7.0f, R + RE2 - 9.0f, R + RE2 - 6.0f, R + RE2 - 3.0f});
+    v.resize(nres);
+    return v;
+}
+
+} // namespace tdi_host
+
+extern "C" {
+
+void tdi_default_video_params(tdi_video_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->ref_face_size = 86.f;      // config.cu:18
+    p->nres_coefs = 4;            // config.cu:41
+    p->twostage_illum = 0;
+    p->finetune_expressions = 0;  // config.cu:95
+    p->pad = 1;                   // config.cu:43
+    p->crop_w = 256;              // constants.h:55-56
+    p->crop_h = 256;
+    p->batched = 1;
+    p->init_pose = 1;
+    p->box_half_width = 0.15f;
+}
+
+int tdi_read_video_config(const char* path, tdi_fit_options* opts, tdi_video_params* p)
+{
+    if (!path || !opts || !p) return TDI_ERR_ARG;
+    int two = p->twostage_illum, nres = p->nres_coefs;
+    int rc = tdi_read_config(path, opts, &two, &nres);
+    if (rc) return rc;
+    std::ifstream in(path);
+    std::string line;
+    bool nres_set = false;
+    while (std::getline(in, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line = line.substr(0, hash);
+        const size_t colon = line.find(':');
+        if (colon == std::string::npos || line.empty() || line[0] == '%') continue;
+        std::string key = line.substr(0, colon), val = line.substr(colon + 1);
+        key.erase(0, key.find_first_not_of(" \t"));
+        key.erase(key.find_last_not_of(" \t\r") + 1);
+        val.erase(0, val.find_first_not_of(" \t\""));
+        val.erase(val.find_last_not_of(" \t\r\"") + 1);
+        if (key.empty() || val.empty()) continue;
+        const double v = std::atof(val.c_str());
+        if (key == "FINETUNE_EXPRESSIONS") p->finetune_expressions = (int)v;   // config.cu:512-513
+        else if (key == "FINETUNE_COEF" && v > 0) opts->finetune_coef = (float)v; // 518-519
+        else if (key == "PAD_RENDERING") p->pad = (int)v;                      // 434-435
+        else if (key == "REF_FACE_SIZE" && v > 0) p->ref_face_size = (float)v; // 383-384
+        else if (key == "NRES_COEFS") nres_set = true;
+    }
+    p->twostage_illum = two;
+    if (nres_set) p->nres_coefs = nres;
+    return TDI_OK;
+}
+
+int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n, const uint8_t* bgr, int32_t src_w,
+                              int32_t src_h, int64_t frame_stride, const float* lmk_x, const float* lmk_y,
+                              const float* cams, const float* boxes, const int64_t* frame_ids, float* theta, float* lam,
+                              float* Lint, tdi_video_output* vo, int32_t* status, tdi_fit_report* reports)
+{
+    if (!ctx || !p || n <= 0 || !bgr || !lmk_x || !lmk_y || !cams || !frame_ids || !theta || !lam || !Lint ||
+        !status || p->nres_coefs < 1 || p->nres_coefs > 15 || !(p->ref_face_size > 0.f))
+        return TDI_ERR_ARG;
+    const int NL = TDI_NLANDMARKS;
+    // pose offsets of the unknowns: the pose follows the coefficient blocks (tdi_api.h); the
+    // light is not an unknown here, so the pose is the last six
+    int K = 0;
+    std::vector<int> ok_idx;
+    // landmark-driven starting pose of every frame (video_fitter.cu:216-224)
+    std::vector<float> pose((size_t)n * 6);
+    std::vector<int32_t> info((size_t)n * 4);
+    int rc = TDI_OK;
+    if (p->init_pose) {
+        rc = tdi_host::initialize_with_orthographic_t(ctx, n, lmk_x, lmk_y, cams, pose.data(), nullptr, info.data());
+        if (rc) return rc;
+    }
+    std::vector<float> coef(n), lx((size_t)n * NL), ly((size_t)n * NL), cam2((size_t)n * 4), fsz(n);
+    std::vector<int32_t> crop((size_t)n * 4), st(n);
+    std::vector<float> box_default(NL), box_neg(NL);
+    for (int i = 0; i < NL; ++i) { box_default[i] = p->box_half_width; box_neg[i] = -p->box_half_width; }
+    for (int i = 0; i < n; ++i) status[i] = TDI_OK;
+    const std::vector<float> sizes = tdi_host::reference_sizes(p->ref_face_size, p->nres_coefs);
+    bool first = true;
+    for (size_t ri = 0; ri < sizes.size(); ++ri) {
+        rc = tdi_frame_geometry(n, lmk_x, lmk_y, cams, src_w, src_h, sizes[ri], p->pad, p->crop_w, p->crop_h,
+                                coef.data(), lx.data(), ly.data(), cam2.data(), crop.data(), fsz.data(), st.data());
+        if (rc) return rc;
+        ok_idx.clear();
+        for (int i = 0; i < n; ++i)
+            if (st[i] == TDI_OK) ok_idx.push_back(i); else status[i] = st[i];
+        const int m = (int)ok_idx.size();
+        if (m == 0) return TDI_OK;
+        // the frames at this size: resized, grey, cropped (InputData::get_resized_frame and the crop of
+        // fit_to_video_frame, video_fitter.cu:537-549), on the device
+        std::vector<float> mcoef(m);
+        std::vector<int32_t> mcrop((size_t)m * 4);
+        long long pixels = 0;
+        for (int j = 0; j < m; ++j) {
+            const int i = ok_idx[j];
+            mcoef[j] = coef[i];
+            for (int q = 0; q < 4; ++q) mcrop[(size_t)j * 4 + q] = crop[(size_t)i * 4 + q];
+            pixels += (long long)crop[(size_t)i * 4 + 2] * crop[(size_t)i * 4 + 3];
+        }
+        // frames of the ok list are not contiguous in bgr: prepare them one run at a time
+        float* dcrops = nullptr;
+        if (hipMalloc((void**)&dcrops, sizeof(float) * (size_t)pixels) != hipSuccess) return TDI_ERR_NOMEM;
+        long long off = 0;
+        for (int j = 0; j < m && !rc; ++j) {
+            const int i = ok_idx[j];
+            rc = tdi_prepare_crops(1, bgr + (size_t)i * frame_stride, src_w, src_h, frame_stride, &mcoef[j],
+                                   &mcrop[(size_t)j * 4], dcrops + off, nullptr);
+            off += (long long)mcrop[(size_t)j * 4 + 2] * mcrop[(size_t)j * 4 + 3];
+        }
+        std::vector<tdi_frame_desc> fd(m);
+        off = 0;
+        for (int j = 0; j < m && !rc; ++j) {
+            const int i = ok_idx[j];
+            tdi_frame_desc& f = fd[j];
+            std::memset(&f, 0, sizeof(f));
+            f.phix = cam2[(size_t)i * 4]; f.phiy = cam2[(size_t)i * 4 + 1];
+            f.cx = cam2[(size_t)i * 4 + 2]; f.cy = cam2[(size_t)i * 4 + 3];
+            f.x0 = mcrop[(size_t)j * 4]; f.y0 = mcrop[(size_t)j * 4 + 1];
+            f.width = mcrop[(size_t)j * 4 + 2]; f.height = mcrop[(size_t)j * 4 + 3];
+            f.face_size = fsz[i];
+            f.obs = dcrops + off;
+            off += (long long)f.width * f.height;
+            f.lmk_x = &lx[(size_t)i * NL];
+            f.lmk_y = &ly[(size_t)i * NL];
+            const float* bx = boxes ? boxes + (size_t)i * 4 * NL : nullptr;
+            f.box_lx = bx ? bx : box_neg.data();
+            f.box_ux = bx ? bx + NL : box_default.data();
+            f.box_ly = bx ? bx + 2 * NL : box_neg.data();
+            f.box_uy = bx ? bx + 3 * NL : box_default.data();
+            for (int a = 0; a < 3; ++a) f.lambda[a] = lam[(size_t)i * 3 + a];
+            f.Lintensity = Lint[i];
+        }
+        tdi_batch* b = nullptr;
+        if (!rc) rc = tdi_batch_create(ctx, m, fd.data(), &b);
+        if (!rc) {
+            K = tdi_batch_unknowns(b);
+            // the per-frame stage's unknowns are the expressions and the pose (the per-frame app fits
+            // neither identity nor albedo, video_from_saved_identity.cu:154-157)
+            if (vo && K - 6 != vo->k_exp) rc = TDI_ERR_ARG;
+        }
+        if (!rc) {
+            std::vector<float> th((size_t)m * K), ml((size_t)m * 3), mL(m);
+            std::vector<tdi_fit_report> rep(m);
+            for (int j = 0; j < m; ++j) {
+                const int i = ok_idx[j];
+                std::memcpy(&th[(size_t)j * K], &theta[(size_t)i * K], sizeof(float) * K);
+                if (first && p->init_pose) std::memcpy(&th[(size_t)j * K + K - 6], &pose[(size_t)i * 6], sizeof(float) * 6);
+                for (int a = 0; a < 3; ++a) ml[(size_t)j * 3 + a] = lam[(size_t)i * 3 + a];
+                mL[j] = Lint[i];
+            }
+            rc = tdi_fit_video_frames(b, p->twostage_illum, p->batched, th.data(), ml.data(), mL.data(), rep.data());
+            // FINETUNE_EXPRESSIONS (video_fitter.cu:627-628)
+            if (!rc && p->finetune_expressions)
+                rc = tdi_host::shape_fit(b, m, K, p->batched != 0, true, th.data(), rep.data());
+            for (int j = 0; !rc && j < m; ++j) {
+                const int i = ok_idx[j];
+                std::memcpy(&theta[(size_t)i * K], &th[(size_t)j * K], sizeof(float) * K);
+                for (int a = 0; a < 3; ++a) lam[(size_t)i * 3 + a] = ml[(size_t)j * 3 + a];
+                Lint[i] = mL[j];
+                status[i] = rep[j].status;
+                if (reports) reports[i] = rep[j];
+                if (vo) {
+                    // update_shape_single_ref_size (video_fitter.cu:365-389): expressions, pose, light
+                    const float il[4] = {ml[(size_t)j * 3], ml[(size_t)j * 3 + 1], ml[(size_t)j * 3 + 2], mL[j]};
+                    tdi_vo_add(vo, frame_ids[i], &th[(size_t)j * K], &th[(size_t)j * K + K - 6], il);
+                }
+            }
+        }
+        if (b) tdi_batch_destroy(b);
+        (void)hipFree(dcrops);
+        if (rc) return rc;
+        first = false;
     }
     return TDI_OK;
 }

@@ -43,6 +43,10 @@ def lib():
         L.tdi_read_landmarks.argtypes = [C.c_char_p, C.c_int32, _vp, _vp, _vp]
         L.tdi_frame_geometry.argtypes = [C.c_int32, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_float, C.c_int32,
                                          C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_default_video_params.argtypes = [_vp]
+        L.tdi_read_video_config.argtypes = [C.c_char_p, _vp, _vp]
+        L.tdi_fit_video_frames_nres.argtypes = [_vp, _vp, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp,
+                                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -100,6 +104,61 @@ class VideoOutput:
             self.close()
         except Exception:
             pass
+
+
+class VideoParams(C.Structure):
+    """tdi_video_params (include/tdi_host.h)."""
+    _fields_ = [("ref_face_size", C.c_float), ("nres_coefs", C.c_int32), ("twostage_illum", C.c_int32),
+                ("finetune_expressions", C.c_int32), ("pad", C.c_int32), ("crop_w", C.c_int32), ("crop_h", C.c_int32),
+                ("batched", C.c_int32), ("init_pose", C.c_int32), ("box_half_width", C.c_float)]
+
+
+def video_params(**kw) -> VideoParams:
+    """The per-frame stage's parameters, the reference's defaults (config.cu) overridden by kw."""
+    p = VideoParams()
+    lib().tdi_default_video_params(C.addressof(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def read_video_config(path: str, opts=None, params=None):
+    """Applies a 3DI config file to fit options and the per-frame stage's parameters."""
+    o = opts if opts is not None else abi.default_options()
+    p = params if params is not None else video_params()
+    if lib().tdi_read_video_config(os.fsencode(path), C.addressof(o), C.addressof(p)):
+        raise FileNotFoundError(path)
+    return o, p
+
+
+def fit_video_frames_nres(ctx, params, frames, lmk_x, lmk_y, cams, theta, lam, frame_ids=None, boxes=None,
+                          vo=None, Lint=None):
+    """The per-frame stage of a video (tdi_fit_video_frames_nres): frames uint8 CUDA tensor
+    [n, H, W, 3] (B, G, R); lmk_x, lmk_y [n, 51], cams [n, 4] of the unresized frames; theta [n, K]
+    and lam [n, 3] the start.  Returns (theta, lam, Lint, status, reports)."""
+    n, H, W = (int(v) for v in frames.shape[:3])
+    frames = frames.contiguous()
+    K = ctx.K
+    th = np.ascontiguousarray(np.array(theta, np.float32).reshape(n, K))
+    la = np.ascontiguousarray(np.array(lam, np.float32).reshape(n, 3))
+    Li = np.ascontiguousarray(np.full(n, 0.005, np.float32) if Lint is None else np.array(Lint, np.float32))
+    lx = np.ascontiguousarray(lmk_x, np.float32)
+    ly = np.ascontiguousarray(lmk_y, np.float32)
+    cm = np.ascontiguousarray(cams, np.float32)
+    ids = np.ascontiguousarray(np.arange(n) if frame_ids is None else frame_ids, np.int64)
+    bx = None if boxes is None else np.ascontiguousarray(boxes, np.float32)
+    st = np.zeros(n, np.int32)
+    reps = (abi.tdi_fit_report * n)()
+    rc = lib().tdi_fit_video_frames_nres(ctx.h, C.addressof(params), n, frames.data_ptr(), W, H, H * W * 3,
+                                         lx.ctypes.data, ly.ctypes.data, cm.ctypes.data,
+                                         None if bx is None else bx.ctypes.data, ids.ctypes.data, th.ctypes.data,
+                                         la.ctypes.data, Li.ctypes.data, None if vo is None else vo.h,
+                                         st.ctypes.data, C.addressof(reps))
+    if rc:
+        raise tdi.TdiError(f"tdi_fit_video_frames_nres: {rc}: {tdi.lib().tdi_last_error().decode()}")
+    return th, la, Li, st, list(reps)
 
 
 def euler_angles(u) -> np.ndarray:

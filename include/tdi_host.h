@@ -57,6 +57,44 @@ int tdi_initialize_with_orthographic(tdi_ctx* ctx, int32_t n, const float* lmk_x
 int tdi_fit_video_frames(tdi_batch* b, int32_t twostage_illum, int32_t batched, float* theta, float* lam,
                          float* Lint, tdi_fit_report* reports);
 
+/* ---- The per-frame stage of a video (VideoFitter::fit_video_frames_auto after landmark
+ *      detection, video_fitter.cu:160-393): for every frame, the landmark-driven starting pose,
+ *      then for each of NRES_COEFS reference face sizes (update_shape_single_ref_size, 355-393)
+ *      the frame resized and cropped at that size and fit_to_video_frame's fits (554-628),
+ *      the results added to a VideoOutput (averaged there over the sizes). ---- */
+typedef struct tdi_video_params {
+    float ref_face_size;        /* REF_FACE_SIZE (config.cu:18: 86)                                 */
+    int32_t nres_coefs;         /* NRES_COEFS (config.cu:41: 4), 1..15                               */
+    int32_t twostage_illum;     /* TWOSTAGE_ILLUM (two more light + shape rounds)                    */
+    int32_t finetune_expressions; /* FINETUNE_EXPRESSIONS (config.cu:95): the expression finetune     */
+    int32_t pad;                /* PAD_RENDERING (config.cu:43: 1)                                   */
+    int32_t crop_w, crop_h;     /* DIMX, DIMY (constants.h:55-56: 256)                               */
+    int32_t batched;            /* 1: the batched fits, 0: frame by frame                            */
+    int32_t init_pose;          /* 1: start each frame's pose at tdi_initialize_with_orthographic's  */
+    float box_half_width;       /* landmark boxes (face-size units) when none are given              */
+} tdi_video_params;
+
+void tdi_default_video_params(tdi_video_params* p);
+
+/* The config keys of the per-frame stage: tdi_read_config's plus FINETUNE_EXPRESSIONS,
+ * FINETUNE_COEF (-> opts.finetune_coef), PAD_RENDERING and REF_FACE_SIZE (both the face-size
+ * test of opts and the base of the reference sizes). */
+int tdi_read_video_config(const char* path, tdi_fit_options* opts, tdi_video_params* p);
+
+/* n frames of one video: bgr DEVICE [n] frames of src_h x src_w x 3 bytes at frame_stride bytes;
+ * lmk_x, lmk_y [n][51] landmarks and cams [n][4] (phix, phiy, cx, cy) of the unresized frames,
+ * host; boxes [n][4][51] landmark boxes (lx, ux, ly, uy; face-size units; NULL = +-box_half_width);
+ * frame_ids [n] the frames' numbers in the video (VideoOutput rows).  The context's light must
+ * not be an unknown (the reference fits it with its own sub-fits).  theta [n][K], lam [n][3],
+ * Lint [n] host, in/out: the start (pose overwritten when init_pose) and the last fit.
+ * status [n]: TDI_OUT_OF_FRAME for a frame whose landmarks leave the frame (skipped, as the
+ * reference's frame loop skips it, video_fitter.cu:187-208), else the last shape fit's status;
+ * reports [n] (may be NULL): the last shape fit's reports.  vo (may be NULL) receives every fit. */
+int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n, const uint8_t* bgr, int32_t src_w,
+                              int32_t src_h, int64_t frame_stride, const float* lmk_x, const float* lmk_y,
+                              const float* cams, const float* boxes, const int64_t* frame_ids, float* theta, float* lam,
+                              float* Lint, tdi_video_output* vo, int32_t* status, tdi_fit_report* reports);
+
 /* ---- Frame preparation (host side; the pixels are tdi_prepare_crops in tdi_api.h) ---- */
 
 /* Landmark text file, one frame per line "x0 y0 x1 y1 ... x50 y50", as

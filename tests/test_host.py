@@ -121,3 +121,37 @@ def test_initialize_with_orthographic_rejects_bad_arguments():
     assert L.tdi_initialize_with_orthographic(None, 1, p, p, p, p, None, None) == 1
     assert L.tdi_initialize_with_orthographic(C.c_void_p(1), 0, p, p, p, p, None, None) == 1
     assert L.tdi_initialize_with_orthographic(C.c_void_p(1), 1, None, p, p, p, None, None) == 1
+
+
+def test_reference_named_cpp_entry_points_link(tmp_path):
+    """A C++ caller written against include/tdi_model_fitter.h (the reference's fitting functions
+    by name, model_fitter.h:29-79) compiles and links against libtdi_host.so."""
+    import shutil
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if not shutil.which("g++"):
+        pytest.skip("no g++")
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        '#include "tdi_model_fitter.h"\n'
+        "bool run(tdi_batch* b, float* th, float* lam, float* L, tdi_fit_report* r) {\n"
+        "    tdi::FitContext c{b, true};\n"
+        "    return tdi::fit_3DMM_lambdas(c, th, lam, L, r) && tdi::fit_3DMM_Lintensity(c, th, lam, L, r) &&\n"
+        "           tdi::fit_3DMM_shape_rigid(c, th, r) && tdi::fit_3DMM_epsilon_finetune(c, th, r);\n"
+        "}\n"
+        "int main(int argc, char**) { return argc > 5 ? (int)run(nullptr, nullptr, nullptr, nullptr, nullptr) : 0; }\n")
+    lib = os.path.join(root, "3di_amd")
+    subprocess.check_call(["g++", "-std=c++17", "-I", os.path.join(root, "include"), str(src), "-L", lib,
+                           "-ltdi_host", "-ltdi", "-Wl,-rpath," + lib, "-Wl,--allow-shlib-undefined",
+                           "-o", str(tmp_path / "caller")])
+
+
+def test_read_video_config(tmp_path):
+    cfg = tmp_path / "c.txt"
+    cfg.write_text("%YAML:1.0\nREF_FACE_SIZE: 90\nNRES_COEFS: 3\nTWOSTAGE_ILLUM: 1\nFINETUNE_EXPRESSIONS: 1\n"
+                   "FINETUNE_COEF: 2.0\nPAD_RENDERING: 0\nKERNEL_SIGMA: 1.0\n")
+    o, p = video.read_video_config(str(cfg))
+    assert (p.ref_face_size, p.nres_coefs, p.twostage_illum, p.finetune_expressions, p.pad) == (90.0, 3, 1, 1, 0)
+    assert o.finetune_coef == 2.0 and o.blur_sigma == 1.0 and o.ref_face_size == 90.0
+    d = video.video_params()
+    assert (d.ref_face_size, d.nres_coefs, d.pad, d.crop_w, d.crop_h) == (86.0, 4, 1, 256, 256)
