@@ -233,11 +233,161 @@ int scratch_for_current_device(void** ptr)
     return TDI_OK;
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Inner-mouth masking (PAINT_INNERMOUTH_BLACK; preprocessing.cu:40-103).  The reference strokes
+// the inner-lip polygon (landmarks 43-50; 43 and 47 moved 3u inwards, u = face size /
+// REF_FACE_SIZE) with 3-pixel lines, traces the outer contour of that ring and blackens every pixel
+// of the mouth box (landmarks 31-50 clipped to the frame) whose signed distance to the contour
+// exceeds u, when the mouth is open by more than 3 REF_FACE_SIZE units (mean of three lip
+// distances).  The traced contour is replaced by its geometry, the polygon moved one pixel outwards:
+// a pixel is blackened when it lies inside the polygon by more than u - 1 (or anywhere inside when
+// u < 1) or outside by less than 1 - u.  The per-frame quantities are FP32 on the host in the
+// reference's order; the per-pixel test is FP64 with squared distances and no contraction.
+// One lane per pixel of the box; the boxes of all frames form one flat list of 256-pixel chunks.
+
+constexpr int MOUTH_CHUNK = 256;
+
+struct MouthFrame {
+    double px[8], py[8];  // polygon in frame pixels
+    double u;             // face size / REF_FACE_SIZE
+    long long src;        // byte offset of the frame
+    int x0, y0, bw, bh;   // box origin and size (pixels)
+    int chunk0;           // first chunk of the frame
+};
+
+__device__ __forceinline__ bool mouth_pixel(const MouthFrame& M, double X, double Y)
+{
+#pragma clang fp contract(off)
+    double best = 1e300;
+    bool inside = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int k1 = (k + 1) & 7;
+        const double ax = M.px[k], ay = M.py[k], bx = M.px[k1], by = M.py[k1];
+        const double ex = bx - ax, ey = by - ay;
+        const double wx = X - ax, wy = Y - ay;
+        const double ee = ex * ex + ey * ey;
+        double t = ee > 0.0 ? (wx * ex + wy * ey) / ee : 0.0;
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        const double dx = wx - t * ex, dy = wy - t * ey;
+        const double d2 = dx * dx + dy * dy;
+        best = d2 < best ? d2 : best;
+        if ((ay > Y) != (by > Y)) {
+            const double xc = ax + (Y - ay) / (by - ay) * ex;
+            inside = (X < xc) != inside;
+        }
+    }
+    const double m = M.u - 1.0;
+    return inside ? (m < 0.0 || best > m * m) : (m < 0.0 && best < m * m);
+}
+
+__global__ __launch_bounds__(MOUTH_CHUNK) void k_paint_innermouth(int n, const MouthFrame* __restrict__ fr,
+                                                                   uint8_t* __restrict__ bgr, int w)
+{
+    const int c = blockIdx.x;
+    int lo = 0, hi = n - 1;  // last frame with chunk0 <= c
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (fr[mid].chunk0 <= c) lo = mid; else hi = mid - 1;
+    }
+    const MouthFrame& M = fr[lo];
+    const long long p = (long long)(c - M.chunk0) * MOUTH_CHUNK + threadIdx.x;
+    if (p >= (long long)M.bw * M.bh) return;
+    const int i = M.y0 + (int)(p / M.bw), j = M.x0 + (int)(p % M.bw);
+    if (mouth_pixel(M, (double)j, (double)i)) {
+        uint8_t* q = bgr + M.src + 3ll * ((long long)i * w + j);
+        q[0] = 0; q[1] = 0; q[2] = 0;
+    }
+}
+
+// Per-frame quantities on the host, FP32 as the reference computes them.  Returns false for a
+// frame that is left as it is (mouth closed, empty box, non-finite landmarks).
+bool mouth_frame(const float* x, const float* y, int w, int h, float ref, MouthFrame& M)
+{
+#pragma clang fp contract(off)
+    float minx = x[0], maxx = x[0], miny = y[0], maxy = y[0];
+    for (int i = 1; i < TDI_NLANDMARKS; ++i) {
+        minx = std::min(minx, x[i]); maxx = std::max(maxx, x[i]);
+        miny = std::min(miny, y[i]); maxy = std::max(maxy, y[i]);
+    }
+    const float face = std::max(maxx - minx, maxy - miny);  // compute_face_size, funcs.cu:44-53
+    const float unit = face / ref;
+    bool finite = std::isfinite(face);
+    for (int i = 31; i < TDI_NLANDMARKS; ++i) finite = finite && std::isfinite(x[i]) && std::isfinite(y[i]);
+    auto dist = [&](int a, int b) { return std::sqrt((x[a] - x[b]) * (x[a] - x[b]) + (y[a] - y[b]) * (y[a] - y[b])); };
+    const float open = (ref / face) * (dist(44, 50) + dist(45, 49) + dist(46, 48)) / 3.0f;
+    if (!(finite && face > 0.f && open > 3.0f)) return false;
+    for (int k = 0; k < 8; ++k) {
+        const float shift = k == 0 ? 3 * unit : (k == 4 ? -(3 * unit) : 0.f);
+        M.px[k] = (double)(x[43 + k] + shift);
+        M.py[k] = (double)y[43 + k];
+    }
+    float bx0 = x[31], bx1 = x[31], by0 = y[31], by1 = y[31];
+    for (int i = 32; i < TDI_NLANDMARKS; ++i) {
+        bx0 = std::min(bx0, x[i]); bx1 = std::max(bx1, x[i]);
+        by0 = std::min(by0, y[i]); by1 = std::max(by1, y[i]);
+    }
+    bx0 = std::min((float)w, std::max(0.f, bx0)); bx1 = std::max(0.f, std::min((float)w, bx1));
+    by0 = std::min((float)h, std::max(0.f, by0)); by1 = std::max(0.f, std::min((float)h, by1));
+    M.u = unit;
+    M.x0 = (int)(unsigned)bx0; M.y0 = (int)(unsigned)by0;
+    M.bw = (int)(unsigned)bx1 - M.x0; M.bh = (int)(unsigned)by1 - M.y0;
+    return M.bw > 0 && M.bh > 0;
+}
+
 } // namespace
 
 } // namespace tdi
 
 using namespace tdi;
+
+extern "C" int tdi_paint_innermouth_black(int32_t n, uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
+                                          const float* lmk_x, const float* lmk_y, float ref_face_size, void* stream)
+{
+    if (n < 0 || (n > 0 && (!bgr || !lmk_x || !lmk_y)) || src_w < 1 || src_h < 1 ||
+        frame_stride < 3ll * src_w * src_h || !(ref_face_size > 0.f) || !std::isfinite(ref_face_size)) {
+        set_error("tdi_paint_innermouth_black: null argument, negative frame count, bad frame geometry or "
+                  "reference face size");
+        return TDI_ERR_ARG;
+    }
+    std::vector<MouthFrame> fr;
+    fr.reserve(n);
+    long long chunks = 0;
+    for (int f = 0; f < n; ++f) {
+        MouthFrame M{};
+        if (!mouth_frame(lmk_x + (size_t)f * TDI_NLANDMARKS, lmk_y + (size_t)f * TDI_NLANDMARKS, src_w, src_h,
+                         ref_face_size, M))
+            continue;
+        M.src = (long long)f * frame_stride;
+        M.chunk0 = (int)chunks;
+        chunks += ((long long)M.bw * M.bh + MOUTH_CHUNK - 1) / MOUTH_CHUNK;
+        if (chunks > INT_MAX) {
+            set_error("tdi_paint_innermouth_black: too many mouth pixels for one call");
+            return TDI_ERR_ARG;
+        }
+        fr.push_back(M);
+    }
+    if (fr.empty()) return TDI_OK;
+    hipStream_t s = (hipStream_t)stream;
+    MouthFrame* dfr = nullptr;
+    if (hipMalloc(&dfr, sizeof(MouthFrame) * fr.size()) != hipSuccess) {
+        set_error("tdi_paint_innermouth_black: descriptor allocation failed");
+        return TDI_ERR_NOMEM;
+    }
+    int rc = TDI_OK;
+    hipError_t e = hipMemcpyAsync(dfr, fr.data(), sizeof(MouthFrame) * fr.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_paint_innermouth, dim3((unsigned)chunks), dim3(MOUTH_CHUNK), 0, s, (int)fr.size(), dfr,
+                           bgr, src_w);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { set_error("tdi_paint_innermouth_black: %s", hipGetErrorString(e)); rc = TDI_ERR_HIP; }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dfr);
+    return rc;
+}
 
 extern "C" int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
                                  const float* coef, const int32_t* crop, float* out, void* stream)

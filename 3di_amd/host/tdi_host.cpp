@@ -575,6 +575,7 @@ void tdi_default_video_params(tdi_video_params* p)
     p->batched = 1;
     p->init_pose = 1;
     p->box_half_width = 0.15f;
+    p->paint_innermouth_black = 1; // config.cu:92
 }
 
 int tdi_read_video_config(const char* path, tdi_fit_options* opts, tdi_video_params* p)
@@ -601,6 +602,7 @@ int tdi_read_video_config(const char* path, tdi_fit_options* opts, tdi_video_par
         if (key == "FINETUNE_EXPRESSIONS") p->finetune_expressions = (int)v;   // config.cu:512-513
         else if (key == "FINETUNE_COEF" && v > 0) opts->finetune_coef = (float)v; // 518-519
         else if (key == "PAD_RENDERING") p->pad = (int)v;                      // 434-435
+        else if (key == "PAINT_INNERMOUTH_BLACK") p->paint_innermouth_black = (int)v; // 509-510
         else if (key == "REF_FACE_SIZE" && v > 0) p->ref_face_size = (float)v; // 383-384
         else if (key == "NRES_COEFS") nres_set = true;
     }
@@ -609,7 +611,7 @@ int tdi_read_video_config(const char* path, tdi_fit_options* opts, tdi_video_par
     return TDI_OK;
 }
 
-int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n, const uint8_t* bgr, int32_t src_w,
+int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n, uint8_t* bgr, int32_t src_w,
                               int32_t src_h, int64_t frame_stride, const float* lmk_x, const float* lmk_y,
                               const float* cams, const float* boxes, const int64_t* frame_ids, float* theta, float* lam,
                               float* Lint, tdi_video_output* vo, int32_t* status, tdi_fit_report* reports)
@@ -636,6 +638,20 @@ int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n
     for (int i = 0; i < NL; ++i) { box_default[i] = p->box_half_width; box_neg[i] = -p->box_half_width; }
     for (int i = 0; i < n; ++i) status[i] = TDI_OK;
     const std::vector<float> sizes = tdi_host::reference_sizes(p->ref_face_size, p->nres_coefs);
+    if (p->paint_innermouth_black) {
+        // the frames whose landmarks lie inside the frame (video_fitter.cu:183-191), painted in one call;
+        // the landmarks of the others are replaced by NaN, which the masking skips
+        rc = tdi_frame_geometry(n, lmk_x, lmk_y, cams, src_w, src_h, sizes[0], p->pad, p->crop_w, p->crop_h,
+                                coef.data(), lx.data(), ly.data(), cam2.data(), crop.data(), fsz.data(), st.data());
+        if (rc) return rc;
+        std::vector<float> px(lmk_x, lmk_x + (size_t)n * NL), py(lmk_y, lmk_y + (size_t)n * NL);
+        for (int i = 0; i < n; ++i)
+            if (st[i] != TDI_OK)
+                for (int k = 0; k < NL; ++k) px[(size_t)i * NL + k] = py[(size_t)i * NL + k] = NAN;
+        rc = tdi_paint_innermouth_black(n, bgr, src_w, src_h, frame_stride, px.data(), py.data(), p->ref_face_size,
+                                        nullptr);
+        if (rc) return rc;
+    }
     bool first = true;
     for (size_t ri = 0; ri < sizes.size(); ++ri) {
         rc = tdi_frame_geometry(n, lmk_x, lmk_y, cams, src_w, src_h, sizes[ri], p->pad, p->crop_w, p->crop_h,

@@ -67,6 +67,8 @@ def lib():
         L.tdi_barrier_terms.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
+        L.tdi_paint_innermouth_black.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, C.c_float,
+                                                 _vp]
         _lib = L
     return _lib
 
@@ -344,6 +346,25 @@ def prepare_crops(frames, coef, crop, out=None, noalloc: bool = False):
     fn = lib().tdi_prepare_crops_noalloc if noalloc else lib().tdi_prepare_crops
     _check(fn(n, _dptr(frames), W, H, H * W * 3, coef.ctypes.data, crop.ctypes.data, _dptr(out), _stream()))
     return out
+
+
+def paint_innermouth_black(frames, lmk_x, lmk_y, ref_face_size: float = 86.0):
+    """Inner-mouth masking of 8-bit BGR frames on the GPU, in place (tdi_paint_innermouth_black).
+    frames: uint8 contiguous device tensor [n, H, W, 3]; lmk_x, lmk_y: [n, 51] frame pixels.
+    Returns frames."""
+    import numpy as np
+    torch = _torch()
+    if (frames.dtype != torch.uint8 or frames.dim() != 4 or frames.shape[3] != 3 or not frames.is_cuda
+            or not frames.is_contiguous()):
+        raise ValueError("paint_innermouth_black: frames must be a contiguous uint8 CUDA tensor [n, H, W, 3]")
+    n, H, W = (int(v) for v in frames.shape[:3])
+    lx = np.ascontiguousarray(lmk_x, np.float32).reshape(-1, 51)
+    ly = np.ascontiguousarray(lmk_y, np.float32).reshape(-1, 51)
+    if lx.shape[0] != n or ly.shape[0] != n:
+        raise ValueError("paint_innermouth_black: landmarks must be [n, 51]")
+    _check(lib().tdi_paint_innermouth_black(n, _dptr(frames), W, H, H * W * 3, lx.ctypes.data, ly.ctypes.data,
+                                            float(ref_face_size), _stream()))
+    return frames
 
 
 def rotation(u):

@@ -110,7 +110,8 @@ class VideoParams(C.Structure):
     """tdi_video_params (include/tdi_host.h)."""
     _fields_ = [("ref_face_size", C.c_float), ("nres_coefs", C.c_int32), ("twostage_illum", C.c_int32),
                 ("finetune_expressions", C.c_int32), ("pad", C.c_int32), ("crop_w", C.c_int32), ("crop_h", C.c_int32),
-                ("batched", C.c_int32), ("init_pose", C.c_int32), ("box_half_width", C.c_float)]
+                ("batched", C.c_int32), ("init_pose", C.c_int32), ("box_half_width", C.c_float),
+                ("paint_innermouth_black", C.c_int32)]
 
 
 def video_params(**kw) -> VideoParams:
@@ -137,9 +138,12 @@ def fit_video_frames_nres(ctx, params, frames, lmk_x, lmk_y, cams, theta, lam, f
                           vo=None, Lint=None):
     """The per-frame stage of a video (tdi_fit_video_frames_nres): frames uint8 CUDA tensor
     [n, H, W, 3] (B, G, R); lmk_x, lmk_y [n, 51], cams [n, 4] of the unresized frames; theta [n, K]
-    and lam [n, 3] the start.  Returns (theta, lam, Lint, status, reports)."""
+    and lam [n, 3] the start.  The frames are painted in place when params.paint_innermouth_black
+    (the inner-mouth masking), so they must be contiguous.  Returns (theta, lam, Lint, status,
+    reports)."""
     n, H, W = (int(v) for v in frames.shape[:3])
-    frames = frames.contiguous()
+    if not frames.is_contiguous():
+        raise ValueError("fit_video_frames_nres: frames must be a contiguous uint8 CUDA tensor [n, H, W, 3]")
     K = ctx.K
     th = np.ascontiguousarray(np.array(theta, np.float32).reshape(n, K))
     la = np.ascontiguousarray(np.array(lam, np.float32).reshape(n, 3))

@@ -285,6 +285,22 @@ int tdi_prepare_crops(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_
 int tdi_prepare_crops_noalloc(int32_t n, const uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
                               const float* coef, const int32_t* crop, float* out, void* stream);
 
+/* Inner-mouth masking of the frames before the crops are taken (config PAINT_INNERMOUTH_BLACK,
+ * default 1; paint_innermouth_black, preprocessing.cu:40-103, called on every frame at
+ * video_fitter.cu:531-535).  bgr: DEVICE, n frames as for tdi_prepare_crops, painted in place;
+ * lmk_x, lmk_y: HOST [n][51] landmarks in frame pixels.  A frame whose mouth is open by more
+ * than 3 ref_face_size units (mean of the distances of landmarks 44-50, 45-49, 46-48 times
+ * ref_face_size / face size) has the pixels of its mouth box (landmarks 31-50, clipped to the
+ * frame) set to 0 in all three channels when they lie inside the inner-lip polygon (landmarks
+ * 43-50, 43 and 47 moved 3u inwards, u = face size / ref_face_size) deeper than u - 1 pixels,
+ * or outside it by less than 1 - u: the reference's contour of the 3-pixel stroke of that
+ * polygon taken as the polygon moved one pixel outwards (an approximation of OpenCV's
+ * rasterised contour; the per-frame quantities keep the reference's FP32 operations).  Frames
+ * with a non-finite landmark among 31-50 are left as they are.  Returns when the frames are
+ * written. */
+int tdi_paint_innermouth_black(int32_t n, uint8_t* bgr, int32_t src_w, int32_t src_h, int64_t frame_stride,
+                               const float* lmk_x, const float* lmk_y, float ref_face_size, void* stream);
+
 /* Rendered pixel counts of the batch's last evaluation (host array of n). */
 int tdi_last_pixel_counts(tdi_batch* b, int32_t* npix);
 

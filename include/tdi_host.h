@@ -72,16 +72,20 @@ typedef struct tdi_video_params {
     int32_t batched;            /* 1: the batched fits, 0: frame by frame                            */
     int32_t init_pose;          /* 1: start each frame's pose at tdi_initialize_with_orthographic's  */
     float box_half_width;       /* landmark boxes (face-size units) when none are given              */
+    int32_t paint_innermouth_black; /* PAINT_INNERMOUTH_BLACK (config.cu:92: 1): tdi_paint_innermouth_black
+                                     * on every frame inside the frame, once, before the sizes loop   */
 } tdi_video_params;
 
 void tdi_default_video_params(tdi_video_params* p);
 
 /* The config keys of the per-frame stage: tdi_read_config's plus FINETUNE_EXPRESSIONS,
- * FINETUNE_COEF (-> opts.finetune_coef), PAD_RENDERING and REF_FACE_SIZE (both the face-size
- * test of opts and the base of the reference sizes). */
+ * FINETUNE_COEF (-> opts.finetune_coef), PAD_RENDERING, PAINT_INNERMOUTH_BLACK and REF_FACE_SIZE
+ * (both the face-size test of opts and the base of the reference sizes). */
 int tdi_read_video_config(const char* path, tdi_fit_options* opts, tdi_video_params* p);
 
-/* n frames of one video: bgr DEVICE [n] frames of src_h x src_w x 3 bytes at frame_stride bytes;
+/* n frames of one video: bgr DEVICE [n] frames of src_h x src_w x 3 bytes at frame_stride bytes
+ * (painted in place when p->paint_innermouth_black, as the reference paints its decoded frames,
+ * video_fitter.cu:189-191);
  * lmk_x, lmk_y [n][51] landmarks and cams [n][4] (phix, phiy, cx, cy) of the unresized frames,
  * host; boxes [n][4][51] landmark boxes (lx, ux, ly, uy; face-size units; NULL = +-box_half_width);
  * frame_ids [n] the frames' numbers in the video (VideoOutput rows).  The context's light must
@@ -90,7 +94,7 @@ int tdi_read_video_config(const char* path, tdi_fit_options* opts, tdi_video_par
  * status [n]: TDI_OUT_OF_FRAME for a frame whose landmarks leave the frame (skipped, as the
  * reference's frame loop skips it, video_fitter.cu:187-208), else the last shape fit's status;
  * reports [n] (may be NULL): the last shape fit's reports.  vo (may be NULL) receives every fit. */
-int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n, const uint8_t* bgr, int32_t src_w,
+int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n, uint8_t* bgr, int32_t src_w,
                               int32_t src_h, int64_t frame_stride, const float* lmk_x, const float* lmk_y,
                               const float* cams, const float* boxes, const int64_t* frame_ids, float* theta, float* lam,
                               float* Lint, tdi_video_output* vo, int32_t* status, tdi_fit_report* reports);

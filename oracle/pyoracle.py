@@ -61,6 +61,8 @@ def lib():
         L.orc_ortho_init.argtypes = [C.c_int, _fp, _fp, _dp, _dp, _dp, _dp, _ip]
         L.orc_prepare_crops.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_longlong, _fp, _ip, _fp]
         L.orc_resized_size.argtypes = [C.c_int, C.c_float]
+        L.orc_paint_innermouth_black.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_longlong, _fp, _fp,
+                                                 C.c_float]
         L.orc_resized_size.restype = C.c_int
         _lib = L
     return _lib
@@ -251,3 +253,14 @@ def prepare_crops(frames, coef, crop):
 def resized_size(n, coef):
     """cv::resize's destination size for a source of n pixels at scale coef."""
     return lib().orc_resized_size(int(n), float(coef))
+
+
+def paint_innermouth_black(frames, lmk_x, lmk_y, ref_face_size=86.0):
+    """The inner-mouth masking of uint8 BGR frames [n, H, W, 3] (a painted copy is returned);
+    lmk_x, lmk_y [n, 51] frame pixels."""
+    fr = np.ascontiguousarray(np.array(frames, np.uint8))
+    n, H, W = fr.shape[:3]
+    lx = np.ascontiguousarray(lmk_x, np.float32)
+    ly = np.ascontiguousarray(lmk_y, np.float32)
+    lib().orc_paint_innermouth_black(n, fr.ctypes.data, W, H, H * W * 3, p(lx, _fp), p(ly, _fp), ref_face_size)
+    return fr

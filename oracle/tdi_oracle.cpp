@@ -1104,6 +1104,89 @@ void prep_crop(const uint8_t* frame, int sw, int sh, float coef, const int32_t* 
 
 extern "C" {
 
+// ---------------------------------------------------------------- inner-mouth masking
+// paint_innermouth_black (preprocessing.cu:40-103), with the traced contour replaced by its
+// geometry: the reference draws the inner-lip polygon (landmarks 43-50, 43 and 47 moved 3u
+// inwards, u = face size / REF_FACE_SIZE) as 3-pixel-wide lines, takes the outer contour of that
+// ring (findContours) and blackens every pixel of the mouth box (landmarks 31-50, clipped to the
+// frame, (uint) bounds) whose signed distance to the contour exceeds u, when the mouth opening
+// (mean of three lip distances, in REF_FACE_SIZE units) exceeds 3.  Here the contour is the
+// polygon moved one pixel outwards: a pixel is blackened when it is inside the polygon by more
+// than u - 1, or outside by less than 1 - u.  Distances are compared squared, in FP64 without
+// contraction (the library's kernel uses the same operations).  A frame with a non-finite
+// landmark among 31-50 (or a non-finite face size) is left as it is; the box is clamped to the
+// frame before its (uint) conversion.
+struct Mouth {
+    double px[8], py[8];
+    double u;
+    int x0, x1, y0, y1;   // pixel box [x0, x1) x [y0, y1)
+    int active;
+};
+
+Mouth mouth_setup(const float* x, const float* y, int w, int h, float ref_face_size)
+{
+    Mouth M{};
+    float minx = x[0], maxx = x[0], miny = y[0], maxy = y[0];
+    for (int i = 1; i < NL; ++i) {
+        minx = std::min(minx, x[i]); maxx = std::max(maxx, x[i]);
+        miny = std::min(miny, y[i]); maxy = std::max(maxy, y[i]);
+    }
+    const float face = std::max(maxx - minx, maxy - miny);       // compute_face_size (funcs.cu:44-53)
+    const float unit = face / ref_face_size;
+    for (int k = 0; k < 8; ++k) {
+        const int i = 43 + k;
+        float off = 0.f;
+        if (i == 43) off += 3 * unit;
+        if (i == 47) off -= 3 * unit;
+        M.px[k] = (double)(x[i] + off);
+        M.py[k] = (double)y[i];
+    }
+    float bx0 = x[31], bx1 = x[31], by0 = y[31], by1 = y[31];
+    for (int i = 32; i < NL; ++i) {
+        bx0 = std::min(bx0, x[i]); bx1 = std::max(bx1, x[i]);
+        by0 = std::min(by0, y[i]); by1 = std::max(by1, y[i]);
+    }
+    bx0 = std::min((float)w, std::max(0.f, bx0)); bx1 = std::max(0.f, std::min((float)w, bx1));
+    by0 = std::min((float)h, std::max(0.f, by0)); by1 = std::max(0.f, std::min((float)h, by1));
+    auto dist = [&](int a, int b) { return std::sqrt((x[a] - x[b]) * (x[a] - x[b]) + (y[a] - y[b]) * (y[a] - y[b])); };
+    const float open = (ref_face_size / face) * (dist(44, 50) + dist(45, 49) + dist(46, 48)) / 3.0f;
+    bool finite = std::isfinite(face);
+    for (int i = 31; i < NL; ++i) finite = finite && std::isfinite(x[i]) && std::isfinite(y[i]);
+    M.u = unit;
+    M.active = finite && face > 0.f && open > 3.0f;
+    if (!M.active) return M;
+    M.x0 = (int)(unsigned)bx0; M.x1 = (int)(unsigned)bx1;
+    M.y0 = (int)(unsigned)by0; M.y1 = (int)(unsigned)by1;
+    return M;
+}
+
+bool mouth_paint(const Mouth& M, int j, int i)
+{
+    const double X = j, Y = i;
+    double best = 1e300;
+    bool inside = false;
+    for (int k = 0; k < 8; ++k) {
+        const int k1 = (k + 1) & 7;
+        const double ax = M.px[k], ay = M.py[k], bx = M.px[k1], by = M.py[k1];
+        const double ex = bx - ax, ey = by - ay;
+        const double wx = X - ax, wy = Y - ay;
+        const double ee = ex * ex + ey * ey;
+        double t = ee > 0.0 ? (wx * ex + wy * ey) / ee : 0.0;
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        const double dx = wx - t * ex, dy = wy - t * ey;
+        const double d2 = dx * dx + dy * dy;
+        if (d2 < best) best = d2;
+        // even-odd crossing of the ray towards +x
+        if ((ay > Y) != (by > Y)) {
+            const double xc = ax + (Y - ay) / (by - ay) * ex;
+            if (X < xc) inside = !inside;
+        }
+    }
+    const double m = M.u - 1.0;
+    if (inside) return m < 0.0 || best > m * m;
+    return m < 0.0 && best < m * m;
+}
+
 // frames: n frames of [sh][sw][3] bytes, frame_stride bytes apart; coef [n]; crop [n][4] =
 // (x0, y0, width, height); out: the crops column-major, frame i at the prefix sum of width*height.
 void orc_prepare_crops(int n, const uint8_t* frames, int sw, int sh, long long frame_stride, const float* coef,
@@ -1117,6 +1200,25 @@ void orc_prepare_crops(int n, const uint8_t* frames, int sw, int sh, long long f
 }
 
 int orc_resized_size(int n, float coef) { return prep_resized_size(n, coef); }
+
+// n frames of [h][w][3] bytes (B, G, R) frame_stride bytes apart, painted in place; lmk_x, lmk_y
+// [n][51] in frame pixels.
+void orc_paint_innermouth_black(int n, uint8_t* frames, int w, int h, long long frame_stride, const float* lmk_x,
+                                const float* lmk_y, float ref_face_size)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int f = 0; f < n; ++f) {
+        const Mouth M = mouth_setup(lmk_x + (size_t)f * NL, lmk_y + (size_t)f * NL, w, h, ref_face_size);
+        if (!M.active) continue;
+        uint8_t* fr = frames + (size_t)f * frame_stride;
+        for (int i = M.y0; i < M.y1; ++i)
+            for (int j = M.x0; j < M.x1; ++j)
+                if (mouth_paint(M, j, i)) {
+                    uint8_t* p = fr + 3 * ((size_t)i * w + j);
+                    p[0] = p[1] = p[2] = 0;
+                }
+    }
+}
 
 void orc_ortho_residuals(const double* x, const double* xlb, const double* ylb, const double* p0L, double* e,
                          double* J)

@@ -154,3 +154,51 @@ def test_prepare_crops_random_sources_bitexact(seed):
     got = tdi.prepare_crops(torch.from_numpy(frames).cuda(), coef, crop)
     torch.cuda.synchronize()
     assert np.array_equal(got.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
+def _mouths(rng, n, w, h):
+    """Random faces' landmarks: a mouth polygon around a random centre (some near or past the frame
+    edges), random scale, lip noise and opening (some closed), landmarks 0 and 1 setting the face
+    size."""
+    base = np.array([(-20, 0), (-10, -10), (0, -10), (10, -10), (20, 0), (10, 10), (0, 10), (-10, 10)], np.float64)
+    X = np.zeros((n, 51), np.float32)
+    Y = np.zeros((n, 51), np.float32)
+    for f in range(n):
+        s = rng.uniform(0.3, 3.0)
+        c = np.array([rng.uniform(-20, w + 20), rng.uniform(-20, h + 20)])
+        P = base * s
+        P[5:8, 1] *= rng.uniform(0.0, 2.0)        # opening of the lower lip
+        P += rng.normal(0, 0.8 * s, P.shape)
+        x = np.full(51, c[0]) + rng.normal(0, 5 * s, 51)
+        y = np.full(51, c[1]) + rng.normal(0, 5 * s, 51)
+        x[43:51], y[43:51] = P[:, 0] + c[0], P[:, 1] + c[1]
+        x[0], x[1] = c[0] - 43 * s * rng.uniform(1, 1.3), c[0] + 43 * s * rng.uniform(1, 1.3)
+        X[f], Y[f] = x, y
+    X[3, 40] = np.nan
+    return X, Y
+
+
+@pytest.mark.parametrize("seed,n,h,w", [(3, 48, 240, 320), (4, 12, 481, 643)])
+def test_paint_innermouth_black_bitexact(seed, n, h, w):
+    rng = np.random.default_rng(seed)
+    frames = rng.integers(1, 256, (n, h, w, 3), dtype=np.uint8)
+    X, Y = _mouths(rng, n, w, h)
+    want = O.paint_innermouth_black(frames, X, Y, 86.0)
+    painted = int((want[..., 0] == 0).sum())
+    _say(f"case: paint_innermouth_black seed {seed}, {n} frames {w}x{h}, {painted} pixels blackened")
+    assert painted > 100
+    dev = torch.from_numpy(frames).cuda()
+    tdi.paint_innermouth_black(dev, X, Y, 86.0)
+    torch.cuda.synchronize()
+    got = dev.cpu().numpy()
+    bad = np.argwhere(got != want)
+    assert bad.size == 0, f"{len(bad)} bytes differ, first {bad[:5].tolist()}"
+
+
+def test_paint_innermouth_black_rejects_bad_arguments():
+    dev = torch.zeros((2, 16, 16, 3), dtype=torch.uint8, device="cuda")
+    X = np.zeros((2, 51), np.float32)
+    with pytest.raises(Exception):
+        tdi.paint_innermouth_black(dev, X, X, 0.0)
+    with pytest.raises(ValueError):
+        tdi.paint_innermouth_black(dev, X[:1], X[:1], 86.0)

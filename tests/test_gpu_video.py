@@ -99,6 +99,9 @@ def test_video_stage_matches_its_steps(tmp_path):
     th, lam, Li, st, reps = video.fit_video_frames_nres(ctx, params, dframes, lx, ly, cams, th0, lam0, vo=vo)
     _say(f"  status {st.tolist()}, stops {[r.stop for r in reps]}")
     assert (st == 0).all()
+    # the frames were painted in place first (PAINT_INNERMOUTH_BLACK, default 1), as the oracle paints them
+    assert params.paint_innermouth_black == 1
+    assert np.array_equal(dframes.cpu().numpy(), O.paint_innermouth_black(frames, lx, ly, 86.0))
     t, la, per_size = _steps(ctx, dframes, lx, ly, cams, th0, lam0, sizes, True)
     assert np.array_equal(th, t) and np.array_equal(lam, la)
     # VideoOutput: expressions averaged over the two sizes (frames t < max(frame id) written)

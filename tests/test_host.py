@@ -149,9 +149,10 @@ def test_reference_named_cpp_entry_points_link(tmp_path):
 def test_read_video_config(tmp_path):
     cfg = tmp_path / "c.txt"
     cfg.write_text("%YAML:1.0\nREF_FACE_SIZE: 90\nNRES_COEFS: 3\nTWOSTAGE_ILLUM: 1\nFINETUNE_EXPRESSIONS: 1\n"
-                   "FINETUNE_COEF: 2.0\nPAD_RENDERING: 0\nKERNEL_SIGMA: 1.0\n")
+                   "FINETUNE_COEF: 2.0\nPAD_RENDERING: 0\nKERNEL_SIGMA: 1.0\nPAINT_INNERMOUTH_BLACK: 0\n")
     o, p = video.read_video_config(str(cfg))
     assert (p.ref_face_size, p.nres_coefs, p.twostage_illum, p.finetune_expressions, p.pad) == (90.0, 3, 1, 1, 0)
+    assert p.paint_innermouth_black == 0 and video.video_params().paint_innermouth_black == 1
     assert o.finetune_coef == 2.0 and o.blur_sigma == 1.0 and o.ref_face_size == 90.0
     d = video.video_params()
     assert (d.ref_face_size, d.nres_coefs, d.pad, d.crop_w, d.crop_h) == (86.0, 4, 1, 256, 256)
