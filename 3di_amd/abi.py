@@ -56,6 +56,14 @@ class tdi_fit_report(C.Structure):
                 ("n_pixels", C.c_int32), ("success", C.c_int32), ("objective", C.c_float), ("pad", C.c_float)]
 
 
+class tdi_lmk_report(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("rounds", C.c_int32), ("accepted", C.c_int32),
+                ("slack0", C.c_float), ("slack", C.c_float)]
+
+
+TDI_LMK_INFEASIBLE = 8
+
+
 def default_options(**kw) -> tdi_fit_options:
     """Reference defaults (config.cu:15-140, model_fitter.cu:137-345)."""
     o = tdi_fit_options(fit_id=0, fit_tex=0, fit_exp=1, fit_light=0, max_iters=100, max_ls=100,

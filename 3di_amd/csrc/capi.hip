@@ -773,6 +773,56 @@ int tdi_ortho_init(tdi_ctx* c, int32_t n, const float* lmk, const float* cams, f
     return launch_ortho_init(c->m, n, lmk, cams, x_orth, pose, angles, info, (hipStream_t)stream);
 }
 
+// The landmark-only fit with slack and its terms: every frame of the batch in one launch.
+static int lmk_slack_run(tdi_batch* b, float* theta, int32_t k_exp, int mode, const double* z, double tau, double* H,
+                         double* g, double* obj, tdi_lmk_report* reports, hipStream_t s, const char* who)
+{
+    if (!b || !theta || (mode == 1 && !z)) { set_error("%s: null argument", who); return TDI_ERR_ARG; }
+    tdi_ctx* c = b->ctx;
+    if (!c->m.lmk_ids) { set_error("%s: the model has no landmark ids", who); return TDI_ERR_ARG; }
+    if (b->n == 0) return TDI_OK;
+    (void)hipSetDevice(c->device);
+    static_assert(sizeof(tdi_lmk_report) == 24, "tdi_lmk_report layout");
+    FrameGPU* dfr = nullptr;
+    tdi_lmk_report* drep = nullptr;
+    int rc = TDI_OK;
+    if (hipMalloc(&dfr, sizeof(FrameGPU) * b->n) != hipSuccess ||
+        hipMalloc(&drep, sizeof(tdi_lmk_report) * b->n) != hipSuccess) {
+        (void)hipFree(dfr);
+        set_error("%s: allocation failed", who);
+        return TDI_ERR_NOMEM;
+    }
+    if (hipMemcpyAsync(dfr, b->hframes.data(), sizeof(FrameGPU) * b->n, hipMemcpyHostToDevice, s) != hipSuccess) {
+        set_error("%s: upload failed", who);
+        rc = TDI_ERR_HIP;
+    }
+    if (!rc) rc = launch_lmk_slack(c->m, c->L, dfr, b->lmk, b->n, theta, k_exp, mode, z, tau, H, g, obj, drep, s);
+    if (!rc && reports &&
+        hipMemcpyAsync(reports, drep, sizeof(tdi_lmk_report) * b->n, hipMemcpyDeviceToHost, s) != hipSuccess) {
+        set_error("%s: report download failed", who);
+        rc = TDI_ERR_HIP;
+    }
+    const hipError_t e = hipStreamSynchronize(s);
+    if (!rc && e != hipSuccess) { set_error("%s: %s", who, hipGetErrorString(e)); rc = TDI_ERR_HIP; }
+    (void)hipFree(dfr);
+    (void)hipFree(drep);
+    return rc;
+}
+
+int tdi_fit_landmarks_slack(tdi_batch* b, float* theta, int32_t k_exp, tdi_lmk_report* reports, void* stream)
+{
+    return lmk_slack_run(b, theta, k_exp, 0, nullptr, 0.0, nullptr, nullptr, nullptr, reports, (hipStream_t)stream,
+                         "tdi_fit_landmarks_slack");
+}
+
+int tdi_landmark_slack_terms(tdi_batch* b, const float* theta, int32_t k_exp, const double* z, double tau, double* H,
+                             double* g, double* obj, void* stream)
+{
+    // mode 1 only reads theta
+    return lmk_slack_run(b, const_cast<float*>(theta), k_exp, 1, z, tau, H, g, obj, nullptr, (hipStream_t)stream,
+                         "tdi_landmark_slack_terms");
+}
+
 // Stage timers: one full evaluation (the work of one GN iteration before the solve) with
 // HIP events around each stage, on the batch's stream.  Mirrors the reference's compile-time
 // MEASURE_TIME timers (model_fitter.cu:43-58, derivative_computer.cu:32-37,141-172).

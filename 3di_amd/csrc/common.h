@@ -206,7 +206,8 @@ __device__ inline void rodrigues_f32(const float* u, float* R, float* dR)
 // Rodrigues rotation in FP64 (used for the vertex transform so that projected vertices come
 // out correctly rounded to FP32), and its derivatives dR/du_i in FP64 (same formula as
 // rodrigues_f32) for the barrier's landmark Jacobian.
-__device__ inline void rodrigues_f64_d(const float* uf, double* R, double* dR)
+template <typename TU>
+__device__ inline void rodrigues_f64_d(const TU* uf, double* R, double* dR)
 {
     const double u[3] = {uf[0], uf[1], uf[2]};
     const double th = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
@@ -235,7 +236,8 @@ __device__ inline void rodrigues_f64_d(const float* uf, double* R, double* dR)
     }
 }
 
-__device__ inline void rodrigues_f64(const float* uf, double* R)
+template <typename TU>
+__device__ inline void rodrigues_f64(const TU* uf, double* R)
 {
     const double u[3] = {uf[0], uf[1], uf[2]};
     const double th = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
@@ -355,6 +357,11 @@ int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, FrameList fl, const
 int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                    const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
                    double* H, float* g, hipStream_t s);
+// lmkfit.hip: the landmark-only fit with slack (mode 0; rep: device tdi_lmk_report[n]) or its
+// terms at zin (mode 1); frames and lmk [n][6][51] cover the n frames of the launch
+int launch_lmk_slack(const ModelGPU& m, const Layout& L, const FrameGPU* frames, const float* lmk, int n,
+                     float* theta, int Kl, int mode, const double* zin, double tau, double* H, double* g,
+                     double* obj, void* rep, hipStream_t s);
 // H: the normal matrices, FP32 (the building-block ABI) or FP64 (the fits' workspace)
 int launch_cholesky(int n, int K, const float* H, const float* g, float* delta, int* status, int mode, const int* mask,
                     hipStream_t s);

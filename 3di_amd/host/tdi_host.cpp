@@ -215,6 +215,20 @@ static bool all_success(const tdi_fit_report* r, int n)
     return true;
 }
 
+// The landmark-only fit with slack of every frame of b (tdi_fit_landmarks_slack) on host theta
+// [n][K], in/out.
+static int landmark_fit(tdi_batch* b, int n, int K, int k_exp, float* theta, tdi_lmk_report* reports)
+{
+    float* d = nullptr;
+    const size_t bytes = sizeof(float) * (size_t)n * K;
+    if (hipMalloc((void**)&d, bytes) != hipSuccess) return TDI_ERR_NOMEM;
+    int rc = hipMemcpy(d, theta, bytes, hipMemcpyHostToDevice) == hipSuccess ? TDI_OK : TDI_ERR_HIP;
+    if (!rc) rc = tdi_fit_landmarks_slack(b, d, k_exp, reports, nullptr);
+    if (!rc && hipMemcpy(theta, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = TDI_ERR_HIP;
+    (void)hipFree(d);
+    return rc;
+}
+
 // Logbarrier_Initializer::initialize_with_orthographic_t (logbarrier_initializer.cu:1862-1914),
 // orthographic fit and pose seed, for n frames at once: host arrays in and out.
 static int initialize_with_orthographic_t(tdi_ctx* ctx, int n, const float* lmk_x, const float* lmk_y,
@@ -541,6 +555,16 @@ bool fit_3DMM_Lintensity(FitContext& ctx, const float* theta, float* lambdas, fl
     return tdi_host::all_success(reports, n);
 }
 
+bool fit_model_with_slack(FitContext& ctx, float* theta, int k_exp, tdi_lmk_report* reports)
+{
+    if (!ctx.batch || !theta || !reports) return false;
+    const int n = tdi_batch_size(ctx.batch), K = tdi_batch_unknowns(ctx.batch);
+    if (tdi_host::landmark_fit(ctx.batch, n, K, k_exp, theta, reports)) return false;
+    for (int i = 0; i < n; ++i)
+        if (reports[i].status != TDI_OK) return false;
+    return true;
+}
+
 } // namespace tdi
 
 // ---------------------------------------------------------------------------- per-frame stage of a video
@@ -576,6 +600,7 @@ void tdi_default_video_params(tdi_video_params* p)
     p->init_pose = 1;
     p->box_half_width = 0.15f;
     p->paint_innermouth_black = 1; // config.cu:92
+    p->lmk_k_exp = 79;             // K_EPSILON_L, config.cu:112
 }
 
 int tdi_read_video_config(const char* path, tdi_fit_options* opts, tdi_video_params* p)
@@ -624,14 +649,7 @@ int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n
     // light is not an unknown here, so the pose is the last six
     int K = 0;
     std::vector<int> ok_idx;
-    // landmark-driven starting pose of every frame (video_fitter.cu:216-224)
-    std::vector<float> pose((size_t)n * 6);
-    std::vector<int32_t> info((size_t)n * 4);
     int rc = TDI_OK;
-    if (p->init_pose) {
-        rc = tdi_host::initialize_with_orthographic_t(ctx, n, lmk_x, lmk_y, cams, pose.data(), nullptr, info.data());
-        if (rc) return rc;
-    }
     std::vector<float> coef(n), lx((size_t)n * NL), ly((size_t)n * NL), cam2((size_t)n * 4), fsz(n);
     std::vector<int32_t> crop((size_t)n * 4), st(n);
     std::vector<float> box_default(NL), box_neg(NL);
@@ -652,7 +670,6 @@ int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n
                                         nullptr);
         if (rc) return rc;
     }
-    bool first = true;
     for (size_t ri = 0; ri < sizes.size(); ++ri) {
         rc = tdi_frame_geometry(n, lmk_x, lmk_y, cams, src_w, src_h, sizes[ri], p->pad, p->crop_w, p->crop_h,
                                 coef.data(), lx.data(), ly.data(), cam2.data(), crop.data(), fsz.data(), st.data());
@@ -720,9 +737,35 @@ int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n
             for (int j = 0; j < m; ++j) {
                 const int i = ok_idx[j];
                 std::memcpy(&th[(size_t)j * K], &theta[(size_t)i * K], sizeof(float) * K);
-                if (first && p->init_pose) std::memcpy(&th[(size_t)j * K + K - 6], &pose[(size_t)i * 6], sizeof(float) * 6);
                 for (int a = 0; a < 3; ++a) ml[(size_t)j * 3 + a] = lam[(size_t)i * 3 + a];
                 mL[j] = Lint[i];
+            }
+            if (p->init_pose) {
+                // fit_to_video_frame (video_fitter.cu:459-486): the orthographic start from this size's
+                // landmarks and camera, then the landmark-only fit with slack from it
+                std::vector<float> mx((size_t)m * NL), my((size_t)m * NL), mc((size_t)m * 4), pose((size_t)m * 6);
+                for (int j = 0; j < m; ++j) {
+                    const int i = ok_idx[j];
+                    std::memcpy(&mx[(size_t)j * NL], &lx[(size_t)i * NL], sizeof(float) * NL);
+                    std::memcpy(&my[(size_t)j * NL], &ly[(size_t)i * NL], sizeof(float) * NL);
+                    std::memcpy(&mc[(size_t)j * 4], &cam2[(size_t)i * 4], sizeof(float) * 4);
+                }
+                rc = tdi_host::initialize_with_orthographic_t(ctx, m, mx.data(), my.data(), mc.data(), pose.data(),
+                                                              nullptr, nullptr);
+                const int kl = std::min(p->lmk_k_exp, K - 6);
+                if (!rc && kl > 0) {
+                    std::vector<float> tt(th);
+                    std::vector<tdi_lmk_report> lrep(m);
+                    for (int j = 0; j < m; ++j)
+                        std::memcpy(&tt[(size_t)j * K + K - 6], &pose[(size_t)j * 6], sizeof(float) * 6);
+                    rc = tdi_host::landmark_fit(b, m, K, kl, tt.data(), lrep.data());
+                    for (int j = 0; !rc && j < m; ++j)
+                        if (lrep[j].status == TDI_OK)
+                            std::memcpy(&th[(size_t)j * K], &tt[(size_t)j * K], sizeof(float) * K);
+                } else if (!rc) {
+                    for (int j = 0; j < m; ++j)
+                        std::memcpy(&th[(size_t)j * K + K - 6], &pose[(size_t)j * 6], sizeof(float) * 6);
+                }
             }
             rc = tdi_fit_video_frames(b, p->twostage_illum, p->batched, th.data(), ml.data(), mL.data(), rep.data());
             // FINETUNE_EXPRESSIONS (video_fitter.cu:627-628)
@@ -745,7 +788,6 @@ int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n
         if (b) tdi_batch_destroy(b);
         (void)hipFree(dcrops);
         if (rc) return rc;
-        first = false;
     }
     return TDI_OK;
 }

@@ -65,6 +65,8 @@ def lib():
         L.tdi_fit_lintensity.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.tdi_fit_buffers.argtypes = [_vp, _vp, C.c_int32, _vp, _vp, _vp, _vp]
         L.tdi_barrier_terms.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_fit_landmarks_slack.argtypes = [_vp, _vp, C.c_int32, _vp, _vp]
+        L.tdi_landmark_slack_terms.argtypes = [_vp, _vp, C.c_int32, _vp, C.c_double, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         L.tdi_paint_innermouth_black.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, C.c_float,
@@ -216,6 +218,30 @@ class Batch:
         g = torch.empty((self.n, K), device="cuda", dtype=torch.float64)
         obj = torch.empty(self.n, device="cuda", dtype=torch.float64)
         _check(lib().tdi_barrier_terms(self.h, _dptr(th), _dptr(H), _dptr(g), _dptr(obj), _stream()))
+        return H, g, obj
+
+    def fit_landmarks_slack(self, theta, k_exp: int):
+        """The landmark-only fit with slack of every frame (tdi_fit_landmarks_slack).  theta:
+        device tensor [n, K], updated in place (pose and the first k_exp expression coefficients of
+        the frames that succeed).  Returns (theta, reports)."""
+        th = self._theta(theta)
+        reps = (abi.tdi_lmk_report * self.n)()
+        _check(lib().tdi_fit_landmarks_slack(self.h, _dptr(th), int(k_exp), C.addressof(reps), _stream()))
+        return th, list(reps)
+
+    def landmark_slack_terms(self, theta, k_exp: int, z, tau: float):
+        """(H [n, k_exp+7, k_exp+7], g [n, k_exp+7], obj [n]) float64 of that fit at z [n, k_exp+7]."""
+        torch = _torch()
+        th = self._theta(theta)
+        nz = int(k_exp) + 7
+        if isinstance(z, np.ndarray):
+            z = torch.from_numpy(np.ascontiguousarray(z, np.float64)).cuda()
+        z = z.double().contiguous().reshape(self.n, nz)
+        H = torch.empty((self.n, nz, nz), device="cuda", dtype=torch.float64)
+        g = torch.empty((self.n, nz), device="cuda", dtype=torch.float64)
+        obj = torch.empty(self.n, device="cuda", dtype=torch.float64)
+        _check(lib().tdi_landmark_slack_terms(self.h, _dptr(th), int(k_exp), _dptr(z), float(tau), _dptr(H), _dptr(g),
+                                              _dptr(obj), _stream()))
         return H, g, obj
 
     def normal_equations(self, theta, mixed: bool = False):

@@ -111,7 +111,7 @@ class VideoParams(C.Structure):
     _fields_ = [("ref_face_size", C.c_float), ("nres_coefs", C.c_int32), ("twostage_illum", C.c_int32),
                 ("finetune_expressions", C.c_int32), ("pad", C.c_int32), ("crop_w", C.c_int32), ("crop_h", C.c_int32),
                 ("batched", C.c_int32), ("init_pose", C.c_int32), ("box_half_width", C.c_float),
-                ("paint_innermouth_black", C.c_int32)]
+                ("paint_innermouth_black", C.c_int32), ("lmk_k_exp", C.c_int32)]
 
 
 def video_params(**kw) -> VideoParams:

@@ -53,7 +53,9 @@ enum tdi_status {
     TDI_NOT_PD = 4,           /* Cholesky met a non-positive pivot (solver.h:86-92)        */
     TDI_NAN = 5,              /* NaN search direction (model_fitter.cu:206-213)            */
     TDI_OUT_OF_FRAME = 6,     /* landmarks outside the image (video_fitter.cu:187-208)     */
-    TDI_TOO_FEW_PIXELS = 7    /* nothing rendered inside the crop                          */
+    TDI_TOO_FEW_PIXELS = 7,   /* nothing rendered inside the crop                          */
+    TDI_LMK_INFEASIBLE = 8    /* the landmark-only fit did not bring its slack below -0.01
+                                 (Logbarrier_Initializer::fit_model returns false)         */
 };
 
 /* Per-frame termination reasons reported by the fits (bit flags). */
@@ -231,6 +233,43 @@ int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, flo
  * with TDI_NOT_PD (the reference carries on with the failed factorisation). */
 int tdi_ortho_init(tdi_ctx* ctx, int32_t n, const float* lmk, const float* cams, float* x_orth, float* pose,
                    float* angles, int32_t* info, void* stream);
+
+/* Landmark-only log-barrier fit with slack, the step between the orthographic start and the
+ * first shape fit of every frame (video_fitter.cu:470-486: Logbarrier_Initializer::
+ * set_minimal_slack, logbarrier_initializer.cu:1041-1090, then fit_model with use_slack,
+ * 1095-1267; the object li_init of video_fitter.cu:30: no identity, k_exp expression
+ * coefficients, T = 1).  Unknowns z = (eps[0..k_exp), tau_x, tau_y, tau_z, u, s); objective
+ *   tau s + sum over the 51 landmarks and 4 box sides of -log G + sum_k -log(ub_k - eps_k) - log(eps_k - lb_k),
+ * G = s + the signed distance of the projected landmark inside its box side (compute_Gs,
+ * logbarrier_initializer.cu:458-518), with the Gauss-Newton Hessian of
+ * compute_gradient_and_hessian (1272-1842).  Start: eps = 0, the pose of theta, s = 5 - min G
+ * at s = 0.  Rounds tau = 100, 1e3, ..., 1e6, each up to 100 Newton steps (Cholesky; a failed
+ * factorisation ends the round) with a backtracking search (t = 1, halved, a step accepted when
+ * the objective decreases, the round ends below t = 0.01); the fit succeeds when s < -0.01 is
+ * seen at the top of an iteration: every landmark then lies 0.01 px inside its box.
+ * One workgroup per frame runs the whole fit in FP64 in one launch.
+ * theta: DEVICE [n][K], in/out: the pose is the start; on success the pose and the first k_exp
+ * expression coefficients are overwritten (the others are not fitted and stay), otherwise theta
+ * is left as it is.  Identity coefficients of theta (if the layout has them) are held fixed.
+ * The expression bounds are the model's.  0 < k_exp <= the layout's expression count, k_exp <= 89.
+ * reports: HOST [n] (may be NULL).  Returns when the fit is done. */
+typedef struct tdi_lmk_report {
+    int32_t status;        /* TDI_OK (s < -0.01 reached) or TDI_LMK_INFEASIBLE         */
+    int32_t iterations;    /* Newton steps computed, over all rounds                   */
+    int32_t rounds;        /* barrier rounds started                                   */
+    int32_t accepted;      /* steps the line search accepted                           */
+    float slack0;          /* the start of s (set_minimal_slack)                       */
+    float slack;           /* s at the end                                             */
+} tdi_lmk_report;
+
+int tdi_fit_landmarks_slack(tdi_batch* b, float* theta, int32_t k_exp, tdi_lmk_report* reports, void* stream);
+
+/* The terms of that fit at given unknowns, for checks: z DEVICE [n][k_exp + 7] (FP64), theta
+ * DEVICE [n][K] (its identity coefficients, if any, enter the landmark positions), tau the
+ * barrier weight; outputs (DEVICE, any may be NULL) H [n][k_exp + 7]^2 (Gauss-Newton Hessian,
+ * FP64), g [n][k_exp + 7], obj [n] (NaN or +inf outside the feasible set). */
+int tdi_landmark_slack_terms(tdi_batch* b, const float* theta, int32_t k_exp, const double* z, double tau, double* H,
+                             double* g, double* obj, void* stream);
 
 /* The integer buffers the fit itself computes, for bit-for-bit checks: a render at theta through
  * the same launches as a fit with these mixed_flags (TDI_MIXED_SYNTH_MFMA selects the FP64-MFMA

@@ -74,6 +74,9 @@ typedef struct tdi_video_params {
     float box_half_width;       /* landmark boxes (face-size units) when none are given              */
     int32_t paint_innermouth_black; /* PAINT_INNERMOUTH_BLACK (config.cu:92: 1): tdi_paint_innermouth_black
                                      * on every frame inside the frame, once, before the sizes loop   */
+    int32_t lmk_k_exp;          /* K_EPSILON_L (config.cu:112: 79): expression coefficients of the
+                                 * landmark-only fit with slack that follows the orthographic start
+                                 * at every size (at most the layout's); 0: no landmark fit        */
 } tdi_video_params;
 
 void tdi_default_video_params(tdi_video_params* p);
@@ -82,8 +85,14 @@ void tdi_default_video_params(tdi_video_params* p);
  * FINETUNE_COEF (-> opts.finetune_coef), PAD_RENDERING, PAINT_INNERMOUTH_BLACK and REF_FACE_SIZE
  * (both the face-size test of opts and the base of the reference sizes). */
 int tdi_read_video_config(const char* path, tdi_fit_options* opts, tdi_video_params* p);
+/* (K_EPSILON_L is set by the reference's model choice, config.cu:300-340, not by a key.) */
 
-/* n frames of one video: bgr DEVICE [n] frames of src_h x src_w x 3 bytes at frame_stride bytes
+/* Per size, as fit_to_video_frame (video_fitter.cu:459-486): with init_pose, the orthographic
+ * start from that size's resized landmarks and camera, then (lmk_k_exp > 0) the landmark-only fit
+ * with slack from it (tdi_fit_landmarks_slack); a frame whose landmark fit succeeds starts its
+ * fits from that pose and expression, the others from theta as it stands (the reference keeps
+ * its previous unknowns; without the landmark fit, from the orthographic pose).
+ * n frames of one video: bgr DEVICE [n] frames of src_h x src_w x 3 bytes at frame_stride bytes
  * (painted in place when p->paint_innermouth_black, as the reference paints its decoded frames,
  * video_fitter.cu:189-191);
  * lmk_x, lmk_y [n][51] landmarks and cams [n][4] (phix, phiy, cx, cy) of the unresized frames,

@@ -41,6 +41,13 @@ bool fit_3DMM_lambdas(FitContext& ctx, const float* theta, float* lambdas, float
 bool fit_3DMM_Lintensity(FitContext& ctx, const float* theta, float* lambdas, float* Lintensity,
                          tdi_fit_report* reports);
 
+/* Logbarrier_Initializer::set_minimal_slack then fit_model with the slack variable
+ * (logbarrier_initializer.cu:1041-1267), as fit_to_video_frame runs them on its landmark object
+ * li_init (video_fitter.cu:479-486): theta [n][K] host, in/out (the pose is the start; a frame
+ * that reaches s < -0.01 gets the fitted pose and first k_exp expression coefficients, the
+ * others keep theta); returns fit_success for every frame (tdi_fit_landmarks_slack). */
+bool fit_model_with_slack(FitContext& ctx, float* theta, int k_exp, tdi_lmk_report* reports);
+
 } // namespace tdi
 
 #endif

@@ -50,6 +50,10 @@ def lib():
         L.orc_barrier.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _dp]
         L.orc_barrier.restype = C.c_double
         L.orc_cholesky_solve.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_fit_landmarks_slack.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_void_p]
+        L.orc_landmark_slack_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp, _dp, C.c_double,
+                                               _dp, _dp]
+        L.orc_landmark_slack_terms.restype = C.c_double
         L.orc_fit.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, C.c_void_p, C.c_int, C.c_int]
         L.orc_max_threads.restype = C.c_int
         L.orc_fit_light.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_void_p, C.c_int,
@@ -163,6 +167,27 @@ class Scene:
         H = np.zeros(K * K)
         obj = lib().orc_barrier(self.md.ref, self._f(i), C.byref(self.opts), p(_d(theta)), p(g), p(H))
         return obj, g, H.reshape(K, K)
+
+    def fit_landmarks_slack(self, theta0, k_exp):
+        """The landmark-only fit with slack of every frame: (theta [n, K] float64, reports)."""
+        n = len(self.frames)
+        th = np.ascontiguousarray(np.array(theta0, np.float64).reshape(n, self.K))
+        reps = (abi.tdi_lmk_report * n)()
+        rc = lib().orc_fit_landmarks_slack(self.md.ref, n, self.fd.ptr(0), C.byref(self.opts), int(k_exp), p(th), reps)
+        if rc:
+            raise ValueError(f"orc_fit_landmarks_slack: error {rc}")
+        return th, list(reps)
+
+    def landmark_slack_terms(self, i, theta, k_exp, z, tau, derivs=True):
+        """(objective, g [k_exp+7], H [k_exp+7, k_exp+7]) of that fit at z for frame i."""
+        nz = int(k_exp) + 7
+        z = _d(z)
+        assert z.size == nz
+        g = np.zeros(nz) if derivs else None
+        H = np.zeros(nz * nz) if derivs else None
+        obj = lib().orc_landmark_slack_terms(self.md.ref, self._f(i), C.byref(self.opts), int(k_exp), p(_d(theta)),
+                                             p(z), float(tau), p(g) if derivs else None, p(H) if derivs else None)
+        return (obj, g, H.reshape(nz, nz)) if derivs else (obj, None, None)
 
     def fit(self, theta0, n_threads=0, frames=None, finetune=False):
         n = len(self.frames)

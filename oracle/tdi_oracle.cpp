@@ -1028,6 +1028,164 @@ void ortho_init_one(const float* lx, const float* ly, const float* cam, const do
     ang[0] = yaw * r2d; ang[1] = pitch * r2d; ang[2] = roll * r2d;
 }
 
+// ---------------------------------------------------------------- landmark-only fit with slack
+// Logbarrier_Initializer::set_minimal_slack (logbarrier_initializer.cu:1041-1090) and fit_model
+// with use_slack (1095-1267) for one frame (T = 1), no identity unknowns (li_init,
+// video_fitter.cu:30), Kl expression coefficients.  Unknowns z = (eps[0..Kl), tau(3), u(3), s).
+// Objective (compute_gradient_and_hessian / evaluate_objective_function, 1272-1842, 1921-2081):
+//   tau_b s + sum_{i, 4 sides} -log G + sum_k -log(ub_k - eps_k) - log(eps_k - lb_k),
+//   G_x- = s + xproj - (xl + blx fs), G_x+ = s + (xl + bux fs) - xproj (same for y) (compute_Gs,
+//   458-518); gradient sum -dG/G over the sides (no epsilon) and tau_b - sum 1/(1e-6 + G) for s;
+//   Gauss-Newton Hessian sum dG dG^T / G^2 (fill_optimization_*_landmark, 587-910), the s column
+//   sum dG / G^2 (fill_for_nabla2F_dsdc), the s diagonal sum 1/(1e-6 + G)^2, the expression
+//   bounds' diagonal 1/(ub-eps+1e-12)^2 + 1/(eps-lb+1e-12)^2 (update_diagonal_of_hessian_wbounds).
+// The landmark positions are the mean shape plus the fixed identity terms of theta plus the
+// first Kl expression columns times eps.
+struct LmkFit {
+    const tdi_model_desc* m;
+    const tdi_frame_desc* f;
+    int Kl = 0, n = 0;
+    double base[NL][3];
+};
+
+void lmk_fit_setup(LmkFit& S, const tdi_model_desc* m, const tdi_frame_desc* f, const Layout& L, const double* th,
+                   int Kl)
+{
+    S.m = m; S.f = f; S.Kl = Kl; S.n = Kl + 7;
+    for (int i = 0; i < NL; ++i) {
+        const int v = m->landmark_ids[i];
+        for (int a = 0; a < 3; ++a) {
+            const int r = 3 * v + a;
+            double x = m->mean_shape[r];
+            for (int k = 0; k < L.nid; ++k) x += (double)m->id_basis[(size_t)r * m->k_id + k] * th[L.off_id + k];
+            S.base[i][a] = x;
+        }
+    }
+}
+
+// Objective at z; with g and H (n and n x n, overwritten) the gradient and Gauss-Newton Hessian.
+// gmin: the least G (may be null).
+double lmk_fit_terms(const LmkFit& S, const double* z, double tau_b, double* g, double* H, double* gmin)
+{
+    const tdi_model_desc* m = S.m;
+    const tdi_frame_desc* f = S.f;
+    const int Kl = S.Kl, n = S.n, np = n - 1;
+    const double s = z[np];
+    double R[9], dR[27];
+    rodrigues(z + Kl + 3, R, g ? dR : nullptr);
+    if (g) { std::fill(g, g + n, 0.0); std::fill(H, H + (size_t)n * n, 0.0); }
+    double obj = tau_b * s, lo = INFINITY;
+    for (int k = 0; k < Kl; ++k) {
+        if (!(m->exp_ub[k] < INFINITY) || !(m->exp_lb[k] > -INFINITY)) continue;   // unbounded: no term
+        const double x = z[k], fub = x - m->exp_ub[k], flb = m->exp_lb[k] - x;
+        obj += -std::log(-fub) - std::log(-flb);
+        if (g) {
+            g[k] += -1.0 / fub + 1.0 / flb;
+            const double a = -1.0 / (fub + 1e-12), b = 1.0 / (flb + 1e-12);
+            H[(size_t)k * n + k] += a * a + b * b;
+        }
+    }
+    std::vector<double> dx(np), dy(np);
+    const double fs = f->face_size;
+    for (int i = 0; i < NL; ++i) {
+        const int v = m->landmark_ids[i];
+        double p[3];
+        for (int a = 0; a < 3; ++a) {
+            double x = S.base[i][a];
+            for (int k = 0; k < Kl; ++k) x += (double)m->exp_basis[(size_t)(3 * v + a) * m->k_exp + k] * z[k];
+            p[a] = x;
+        }
+        double c[3];
+        for (int a = 0; a < 3; ++a) c[a] = R[a] * p[0] + R[3 + a] * p[1] + R[6 + a] * p[2] + z[Kl + a];
+        const double xp = f->phix * c[0] / c[2] + f->cx, yp = f->phiy * c[1] / c[2] + f->cy;
+        const double Gxm = s + xp - (f->lmk_x[i] + f->box_lx[i] * fs);
+        const double Gym = s + yp - (f->lmk_y[i] + f->box_ly[i] * fs);
+        const double Gxp = s + (f->lmk_x[i] + f->box_ux[i] * fs) - xp;
+        const double Gyp = s + (f->lmk_y[i] + f->box_uy[i] * fs) - yp;
+        obj += -std::log(Gxm) - std::log(Gym) - std::log(Gxp) - std::log(Gyp);
+        for (double G : {Gxm, Gym, Gxp, Gyp}) lo = (G != G || lo != lo) ? NAN : std::min(lo, G);   // NaN wins
+        if (!g) continue;
+        for (int j = 0; j < np; ++j) {
+            double dv[3];
+            if (j < Kl) {
+                double e[3];
+                for (int a = 0; a < 3; ++a) e[a] = m->exp_basis[(size_t)(3 * v + a) * m->k_exp + j];
+                for (int a = 0; a < 3; ++a) dv[a] = R[a] * e[0] + R[3 + a] * e[1] + R[6 + a] * e[2];
+            } else if (j < Kl + 3) {
+                dv[0] = dv[1] = dv[2] = 0.0;
+                dv[j - Kl] = 1.0;
+            } else {
+                const double* d = dR + 9 * (j - Kl - 3);
+                for (int a = 0; a < 3; ++a) dv[a] = d[a] * p[0] + d[3 + a] * p[1] + d[6 + a] * p[2];
+            }
+            dx[j] = f->phix * (dv[0] * c[2] - dv[2] * c[0]) / (c[2] * c[2]);
+            dy[j] = f->phiy * (dv[1] * c[2] - dv[2] * c[1]) / (c[2] * c[2]);
+        }
+        const double cx = -1.0 / Gxm + 1.0 / Gxp, cy = -1.0 / Gym + 1.0 / Gyp;
+        const double wx = 1.0 / (Gxm * Gxm) + 1.0 / (Gxp * Gxp), wy = 1.0 / (Gym * Gym) + 1.0 / (Gyp * Gyp);
+        const double hx = 1.0 / (Gxm * Gxm) - 1.0 / (Gxp * Gxp), hy = 1.0 / (Gym * Gym) - 1.0 / (Gyp * Gyp);
+        for (int a = 0; a < np; ++a) {
+            g[a] += cx * dx[a] + cy * dy[a];
+            for (int b = 0; b < np; ++b) H[(size_t)a * n + b] += wx * dx[a] * dx[b] + wy * dy[a] * dy[b];
+            const double hs = hx * dx[a] + hy * dy[a];
+            H[(size_t)a * n + np] += hs;
+            H[(size_t)np * n + a] += hs;
+        }
+        const double q[4] = {1.0 / (1e-6 + Gxm), 1.0 / (1e-6 + Gym), 1.0 / (1e-6 + Gxp), 1.0 / (1e-6 + Gyp)};
+        for (int k = 0; k < 4; ++k) {
+            g[np] -= q[k];
+            H[(size_t)np * n + np] += q[k] * q[k];
+        }
+    }
+    if (g) g[np] += tau_b;
+    if (gmin) *gmin = lo;
+    return obj;
+}
+
+// The whole fit; z in: the pose (z[Kl..Kl+6)); out: the final unknowns.  Returns the report.
+tdi_lmk_report lmk_fit_run(const LmkFit& S, double* z)
+{
+    const int Kl = S.Kl, n = S.n;
+    tdi_lmk_report rep{};
+    for (int k = 0; k < Kl; ++k) z[k] = 0.0;
+    // set_minimal_slack: s = 5 - min G at s = 0
+    z[n - 1] = 0.0;
+    double gmin = 0.0;
+    lmk_fit_terms(S, z, 0.0, nullptr, nullptr, &gmin);
+    z[n - 1] = 5.0 - gmin;
+    rep.slack0 = (float)z[n - 1];
+    std::vector<double> g(n), H((size_t)n * n), d(n), zt(n);
+    double tau_b = 10.0;
+    bool success = false;
+    for (int round = 0; round < 5 && !success; ++round) {
+        tau_b *= 10.0;
+        rep.rounds++;
+        for (int it = 0; it < 100; ++it) {
+            if (z[n - 1] < -0.01) { success = true; break; }
+            const double obj = lmk_fit_terms(S, z, tau_b, g.data(), H.data(), nullptr);
+            rep.iterations++;
+            if (!cholesky_solve(n, H, g.data(), d.data())) break;
+            bool stop = false;
+            double t = 1.0;
+            for (int inner = 0; inner < 100; ++inner) {
+                if (t < 0.01) { stop = true; break; }
+                for (int j = 0; j < n; ++j) zt[j] = z[j] + t * d[j];
+                const double ot = lmk_fit_terms(S, zt.data(), tau_b, nullptr, nullptr, nullptr);
+                if (ot < obj) {
+                    std::copy(zt.begin(), zt.end(), z);
+                    rep.accepted++;
+                    break;
+                }
+                t *= 0.5;
+            }
+            if (stop) break;
+        }
+    }
+    rep.status = success ? TDI_OK : TDI_LMK_INFEASIBLE;
+    rep.slack = (float)z[n - 1];
+    return rep;
+}
+
 // ---------------------------------------------------------------- frame preparation
 // InputData::get_resized_frame (input_data.cu:44-53) and the crop + transpose of
 // VideoFitter::fit_to_video_frame (video_fitter.cu:537-543), on 8-bit BGR frames.  The steps are
@@ -1245,6 +1403,40 @@ void orc_ortho_init(int n, const float* lmk, const float* cams, const double* p0
 }
 
 int orc_num_unknowns(const tdi_model_desc* m, const tdi_fit_options* o) { return make_layout(m, o).K; }
+
+// Landmark-only fit with slack for n frames: theta [n][K] double in/out (the pose is the start; on
+// success the pose and the first Kl expression coefficients are overwritten), reports [n].
+int orc_fit_landmarks_slack(const tdi_model_desc* m, int n, const tdi_frame_desc* frames, const tdi_fit_options* o,
+                            int Kl, double* theta, tdi_lmk_report* reports)
+{
+    const Layout L = make_layout(m, o);
+    if (Kl < 1 || Kl > L.nexp) return TDI_ERR_ARG;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int f = 0; f < n; ++f) {
+        double* th = theta + (size_t)f * L.K;
+        LmkFit S;
+        lmk_fit_setup(S, m, &frames[f], L, th, Kl);
+        std::vector<double> z(S.n);
+        for (int a = 0; a < 6; ++a) z[Kl + a] = th[L.off_pose + a];
+        reports[f] = lmk_fit_run(S, z.data());
+        if (reports[f].status == TDI_OK) {
+            for (int k = 0; k < Kl; ++k) th[L.off_exp + k] = z[k];
+            for (int a = 0; a < 6; ++a) th[L.off_pose + a] = z[Kl + a];
+        }
+    }
+    return TDI_OK;
+}
+
+// The terms of that fit at z [Kl + 7] for one frame (theta supplies fixed identity terms):
+// g [Kl + 7], H [(Kl + 7)^2] (may be null together); returns the objective.
+double orc_landmark_slack_terms(const tdi_model_desc* m, const tdi_frame_desc* f, const tdi_fit_options* o, int Kl,
+                                const double* theta, const double* z, double tau_b, double* g, double* H)
+{
+    const Layout L = make_layout(m, o);
+    LmkFit S;
+    lmk_fit_setup(S, m, f, L, theta, Kl);
+    return lmk_fit_terms(S, z, tau_b, g, H, nullptr);
+}
 
 int orc_blur_taps(double sigma, double* taps) { return blur_taps(sigma, taps); }
 

@@ -50,8 +50,10 @@ def _video(n=3, W=320, H=240, f=700.0):
     return m, lay, opts, np.stack(frames), np.stack(lx), np.stack(ly), cams, truths
 
 
-def _steps(ctx, dframes, lx, ly, cams, th_start, lam0, sizes, finetune):
-    """The stage's steps one by one through the Python bindings; returns the results per size."""
+def _steps(ctx, dframes, lx, ly, cams, th_start, lam0, sizes, finetune, init=None):
+    """The stage's steps one by one through the Python bindings; returns the results per size.
+    init: None (start from theta), "ortho" (the orthographic pose of each size's landmarks) or
+    "slack" (that pose, then the landmark-only fit with slack; frames it fails keep theta)."""
     n = len(th_start)
     t = th_start.copy()
     la = lam0.copy()
@@ -73,6 +75,20 @@ def _steps(ctx, dframes, lx, ly, cams, th_start, lam0, sizes, finetune):
                                    lam=la[i], Lint=0.005))
             off += w_ * h_
         b = tdi.Batch(ctx, fds)
+        if init is not None:
+            K = t.shape[1]
+            pose, _, _ = video.initialize_with_orthographic(ctx, g["lmk_x"], g["lmk_y"], g["cams"])
+            tt = t.copy()
+            tt[:, K - 6:] = pose
+            if init == "ortho":
+                t = tt
+            else:
+                dt, lreps = b.fit_landmarks_slack(torch.from_numpy(tt).cuda(), min(79, K - 6))
+                dt = dt.cpu().numpy()
+                _say(f"  landmark fit: status {[r.status for r in lreps]}, steps {[r.iterations for r in lreps]}")
+                for i, r in enumerate(lreps):
+                    if r.status == 0:
+                        t[i] = dt[i]
         t, la, L_, _ = video.fit_video_frames(b, t, la, twostage_illum=False, batched=True)
         if finetune:
             dt = torch.from_numpy(t.copy()).cuda()
@@ -117,16 +133,20 @@ def test_video_stage_matches_its_steps(tmp_path):
         drot = np.degrees(np.abs(th[i, K - 3:] - tr.theta[K - 3:]).max())
         _say(f"  frame {i}: |dtau| {dpos:.2f} mm, |du| {drot:.3f} deg")
         assert dpos < 20.0 and drot < 10.0
-    # with the landmark-driven starting pose: the same steps after tdi_initialize_with_orthographic
-    # (the synthetic landmarks lie on a smooth cap, so the orthographic start is far from the truth
-    # in pitch; only the orchestration is compared)
+    # with the landmark-driven start: at each size the orthographic pose of the resized landmarks,
+    # then the landmark-only fit with slack (the synthetic landmarks lie on a smooth cap, so the
+    # orthographic start is far from the truth in pitch; only the orchestration is compared)
     params.init_pose = 1
-    params.nres_coefs = 1
+    params.nres_coefs = 2
     params.finetune_expressions = 0
+    assert params.lmk_k_exp == 79
     th1, _, _, st1, _ = video.fit_video_frames_nres(ctx, params, dframes, lx, ly, cams, th0, lam0)
-    pose, _, _ = video.initialize_with_orthographic(ctx, lx, ly, cams)
-    ts = th0.copy()
-    ts[:, K - 6:] = pose
-    t1, _, _ = _steps(ctx, dframes, lx, ly, cams, ts, lam0, sizes[:1], False)
-    ctx.close()
+    t1, _, _ = _steps(ctx, dframes, lx, ly, cams, th0, lam0, sizes, False, init="slack")
     assert np.array_equal(th1, t1)
+    # without the landmark fit: the orthographic pose alone
+    params.lmk_k_exp = 0
+    params.nres_coefs = 1
+    th2, _, _, _, _ = video.fit_video_frames_nres(ctx, params, dframes, lx, ly, cams, th0, lam0)
+    t2, _, _ = _steps(ctx, dframes, lx, ly, cams, th0, lam0, sizes[:1], False, init="ortho")
+    ctx.close()
+    assert np.array_equal(th2, t2)
