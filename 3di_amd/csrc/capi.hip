@@ -532,8 +532,10 @@ static const ModelGPU& bounds_model(const tdi_batch* b) { return b->finetune ? b
 // barrier objective, shading, blur, gradients (Renderer::render + evaluate_objective_function).
 // full_crop: blur the whole crops (the rendered image is an output), else only the region
 // the normalised gradients read.
+// dverts (optional, device [fl.n][V][3]): per-vertex model-space offsets added after the synthesis
+// (all frames of the chunk listed, fl.act == nullptr).
 static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed, hipStream_t s,
-                       const Layout* Lp = nullptr, bool full_crop = false)
+                       const Layout* Lp = nullptr, bool full_crop = false, const float* dverts = nullptr)
 {
     tdi_ctx* c = b->ctx;
     const Layout& L = Lp ? *Lp : c->L;
@@ -544,6 +546,7 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
     else
         rc = launch_synthesize(c->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
     if (rc) return rc;
+    if (dverts && (rc = launch_add_offsets(c->m.V, w.frames, fl.n, w.rt, dverts, w.P0, w.vproj, s))) return rc;
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
     if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
     if ((rc = launch_raster(c->m, w.frames, fl, w.vproj, c->opts.ntmp_cap, w.keys, w.box, s))) return rc;
@@ -725,6 +728,54 @@ int tdi_render(tdi_batch* b, const float* theta, float* img, float* blurred, flo
         if (npix) TDI_HIP(hipMemcpyAsync(npix + i0, w.npix, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
     }
     return TDI_OK;
+}
+
+// The render loss and its adjoint, chunk by chunk; the render (rasterisation included) is
+// recomputed from theta and dverts.
+int tdi_render_loss_grad(tdi_batch* b, const float* theta, const float* dverts, const double* gloss, double* loss,
+                         float* gtheta, float* gverts, void* stream)
+{
+    if (!b || !theta) { set_error("tdi_render_loss_grad: null argument"); return TDI_ERR_ARG; }
+    hipStream_t s = (hipStream_t)stream;
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    const int K = c->L.K, V = c->m.V;
+    if (!c->m.lmk_ids) { set_error("tdi_render_loss_grad: the model has no landmark ids"); return TDI_ERR_ARG; }
+    const bool grad = gtheta || gverts;
+    AdjScratch a{};
+    a.cap = w.cap;
+    const size_t C = (size_t)w.C;
+    const size_t nd = (grad ? (size_t)w.cap + C * V * 3 + C * 15 + C * c->L.nexp + C * c->L.nid : 0) + C;
+    double* blk = nullptr;
+    if (hipMalloc((void**)&blk, sizeof(double) * nd) != hipSuccess) {
+        set_error("tdi_render_loss_grad: scratch allocation of %zu bytes failed", sizeof(double) * nd);
+        return TDI_ERR_NOMEM;
+    }
+    double* q = blk;
+    a.loss = q; q += C;
+    if (grad) {
+        a.dB = q; q += w.cap;
+        a.dP0 = q; q += C * V * 3;
+        a.gpl = q; q += C * 15;
+        a.gexp = q; q += C * c->L.nexp;
+        a.gid = q;
+    }
+    int rc = TDI_OK;
+    for (int i0 = 0; i0 < b->n && !rc; i0 += w.C) {
+        const int n = std::min(w.C, b->n - i0);
+        if ((rc = ws_load(b, i0, n, s))) break;
+        const float* th = theta + (size_t)i0 * K;
+        if ((rc = eval_render(b, all_frames(n), th, 0, s, nullptr, false, dverts ? dverts + (size_t)i0 * V * 3 : nullptr)))
+            break;
+        rc = launch_adjoint(c->m, c->L, w.frames, n, th, w.P0, w.pvert, w.pbary, w.vis, w.npix, w.pixg, w.lmk, c->taps,
+                            w.max_area, w.img, w.tmp, w.blurred, a, gloss ? gloss + i0 : nullptr,
+                            loss ? loss + i0 : nullptr, gtheta ? gtheta + (size_t)i0 * K : nullptr,
+                            gverts ? gverts + (size_t)i0 * V * 3 : nullptr, s);
+    }
+    const hipError_t e = hipStreamSynchronize(s);
+    if (!rc && e != hipSuccess) { set_error("tdi_render_loss_grad: %s", hipGetErrorString(e)); rc = TDI_ERR_HIP; }
+    (void)hipFree(blk);
+    return rc;
 }
 
 static int normal_eq(tdi_batch* b, const float* theta, float* H, float* g, float* obj, void* stream, bool mixed)

@@ -357,6 +357,24 @@ int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, FrameList fl, const
 int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                    const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
                    double* H, float* g, hipStream_t s);
+// adjoint.hip: the render loss and its adjoint (scratch: device buffers of the caller)
+struct AdjScratch {
+    double* dB;        // [cap] dL/d(blurred render)
+    long long cap;
+    double* dP0;       // [n][V][3] dL/d(model-space vertices)
+    double* gpl;       // [n][15] pose (6) and light (9)
+    double* gexp;      // [n][nexp]
+    double* gid;       // [n][nid]
+    double* loss;      // [n]
+};
+int launch_add_offsets(int V, const FrameGPU* frames, int n, const double* rt, const float* dverts, double* P0,
+                       float4* vproj, hipStream_t s);
+int launch_adjoint(const ModelGPU& m, const Layout& L, const FrameGPU* frames, int n, const float* theta,
+                   const double* P0, const int4* pvert, const float2* pbary, const int* vis, const int* npix,
+                   const float4* pixg, const float* lmk, const float* taps, int max_area, float* img, float* tmp,
+                   float* blurred, const AdjScratch& a, const double* gloss, double* loss, float* gtheta,
+                   float* gverts, hipStream_t s);
+
 // lmkfit.hip: the landmark-only fit with slack (mode 0; rep: device tdi_lmk_report[n]) or its
 // terms at zin (mode 1); frames and lmk [n][6][51] cover the n frames of the launch
 int launch_lmk_slack(const ModelGPU& m, const Layout& L, const FrameGPU* frames, const float* lmk, int n,

@@ -66,6 +66,7 @@ def lib():
         L.tdi_fit_buffers.argtypes = [_vp, _vp, C.c_int32, _vp, _vp, _vp, _vp]
         L.tdi_barrier_terms.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_fit_landmarks_slack.argtypes = [_vp, _vp, C.c_int32, _vp, _vp]
+        L.tdi_render_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_landmark_slack_terms.argtypes = [_vp, _vp, C.c_int32, _vp, C.c_double, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
@@ -219,6 +220,29 @@ class Batch:
         obj = torch.empty(self.n, device="cuda", dtype=torch.float64)
         _check(lib().tdi_barrier_terms(self.h, _dptr(th), _dptr(H), _dptr(g), _dptr(obj), _stream()))
         return H, g, obj
+
+    def render_loss(self, theta, dverts=None, gloss=None, grad: bool = True):
+        """The render loss of every frame and, with grad, its adjoint (tdi_render_loss_grad).
+        theta: [n, K]; dverts: per-vertex model-space offsets [n, V, 3] or None; gloss: upstream
+        gradient [n] of each frame's loss (None = 1).  Returns (loss [n] float64, gtheta [n, K]
+        float32 or None, gverts [n, V, 3] float32 or None)."""
+        torch = _torch()
+        th = self._theta(theta)
+        V = self.ctx.model.n_vertices
+        dv = None
+        if dverts is not None:
+            dv = dverts.float().contiguous().reshape(self.n, V, 3)
+        gl = None
+        if gloss is not None:
+            gl = gloss.double().contiguous().reshape(self.n)
+        loss = torch.empty(self.n, device="cuda", dtype=torch.float64)
+        gth = torch.empty((self.n, self.K), device="cuda", dtype=torch.float32) if grad else None
+        gv = torch.empty((self.n, V, 3), device="cuda", dtype=torch.float32) if grad else None
+        _check(lib().tdi_render_loss_grad(self.h, _dptr(th), _dptr(dv) if dv is not None else None,
+                                          _dptr(gl) if gl is not None else None, _dptr(loss),
+                                          _dptr(gth) if gth is not None else None,
+                                          _dptr(gv) if gv is not None else None, _stream()))
+        return loss, gth, gv
 
     def fit_landmarks_slack(self, theta, k_exp: int):
         """The landmark-only fit with slack of every frame (tdi_fit_landmarks_slack).  theta:

@@ -291,6 +291,27 @@ int tdi_fit_buffers(tdi_batch* b, const float* theta, int32_t mixed_flags, float
  * widened), obj [n].  Device pointers; any may be NULL. */
 int tdi_barrier_terms(tdi_batch* b, const float* theta, double* H, double* g, double* obj, void* stream);
 
+/* The render loss and its adjoint (vector-Jacobian product), for differentiation (the repo's
+ * Python driver binds it behind a torch.autograd.Function).  No counterpart entry point in the
+ * reference, whose fit forms the Jacobian itself (newfuncs.cu:173-463, derivative_computer.cu:
+ * 141-172); this is its transpose applied to a vector, without J.  Per frame:
+ *   loss = sum over the visible pixels of |g^_r - g^_s|^2 (the normalised gradients of the
+ *          blurred render and of the blurred observation, fill_grads, funcs.cu:842-897; equal to
+ *          (2/w) times the fit's data objective plus a constant while |g^| = 1)
+ *        + sum over the 51 landmarks of the squared image-pixel distance between the projected
+ *          landmark vertex and the detected landmark;
+ * the model-space vertices are the synthesis plus optional per-vertex offsets dverts.  The
+ * render, rasterisation included, is recomputed from theta (and dverts) on every call; the
+ * gradients hold the visibility (the triangle of each pixel) fixed and are exact derivatives of
+ * the rendered values, with the barycentrics of each pixel centre following the projected
+ * triangle.  Device pointers: theta [n][K]; dverts [n][V][3] (may be NULL); gloss [n] upstream
+ * gradient of each frame's loss (NULL = 1); outputs (any may be NULL) loss [n], gtheta [n][K]
+ * (pose, expression, identity and light unknowns; albedo unknowns get 0), gverts [n][V][3]
+ * (d loss / d model-space vertices).  With gtheta and gverts NULL only the loss is computed.
+ * Returns when done. */
+int tdi_render_loss_grad(tdi_batch* b, const float* theta, const float* dverts, const double* gloss, double* loss,
+                         float* gtheta, float* gverts, void* stream);
+
 /* Frames per workspace chunk (the batch is processed in chunks of this many frames). */
 int tdi_workspace_frames(tdi_batch* b);
 

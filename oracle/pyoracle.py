@@ -50,6 +50,8 @@ def lib():
         L.orc_barrier.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _dp]
         L.orc_barrier.restype = C.c_double
         L.orc_cholesky_solve.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_render_loss.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _ip, C.c_int]
+        L.orc_render_loss.restype = C.c_double
         L.orc_fit_landmarks_slack.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_void_p]
         L.orc_landmark_slack_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp, _dp, C.c_double,
                                                _dp, _dp]
@@ -167,6 +169,15 @@ class Scene:
         H = np.zeros(K * K)
         obj = lib().orc_barrier(self.md.ref, self._f(i), C.byref(self.opts), p(_d(theta)), p(g), p(H))
         return obj, g, H.reshape(K, K)
+
+    def render_loss(self, i, theta, dverts=None, tri=None, frozen=False):
+        """The render loss of frame i (sum over visible pixels of |g^_r - g^_s|^2 plus the squared
+        landmark reprojection errors) at theta with per-vertex offsets dverts [V, 3] (model space).
+        tri: int32 [W*H] triangle per pixel: written (the rasteriser's) unless frozen, read when
+        frozen (visibility held fixed, exact FP64 barycentrics).  Returns the loss."""
+        dv = None if dverts is None else _d(dverts).reshape(-1)
+        return lib().orc_render_loss(self.md.ref, self._f(i), C.byref(self.opts), p(_d(theta)), p(dv),
+                                     p(tri, _ip), int(frozen))
 
     def fit_landmarks_slack(self, theta0, k_exp):
         """The landmark-only fit with slack of every frame: (theta [n, K] float64, reports)."""

@@ -1028,6 +1028,116 @@ void ortho_init_one(const float* lx, const float* ly, const float* cam, const do
     ang[0] = yaw * r2d; ang[1] = pitch * r2d; ang[2] = roll * r2d;
 }
 
+// ---------------------------------------------------------------- render loss (differentiation check)
+// The loss of the adjoint: over the visible pixels, the squared difference of the normalised
+// gradients of the blurred render and of the blurred observation (the data term's residuals,
+// fill_grads, funcs.cu:842-897; sum |g^_r - g^_s|^2 = const - (2 / w) * the fit's data objective
+// when |g^| = 1), plus the squared reprojection error of the 51 landmark vertices in image pixels.
+// The model-space vertices may carry per-vertex offsets dverts [V][3].  With tri == null or
+// frozen == 0 the visibility is the rasteriser's (tri [W*H] receives the triangle per pixel, -1
+// empty, when given); with frozen != 0 the triangle of each pixel is tri's, and the barycentrics
+// are the exact FP64 ones of the pixel centre in the projected triangle (visibility held fixed,
+// the render a smooth function of the vertices).
+double render_loss(const Frame& F, const double* th, const double* dverts, int32_t* tri, int frozen)
+{
+    const tdi_model_desc* m = F.m;
+    const tdi_frame_desc* f = F.f;
+    const int W = f->width, Hh = f->height, V = m->n_vertices;
+    const size_t A = (size_t)W * Hh;
+    Geometry G;
+    geometry(F, th, false, G);
+    if (dverts) {
+        for (int v = 0; v < V; ++v) {
+            double q[3];
+            for (int a = 0; a < 3; ++a) G.P0[3 * v + a] += dverts[3 * v + a];
+            rot(G.R, &G.P0[3 * v], q);
+            for (int a = 0; a < 3; ++a) { q[a] += G.tau[a]; G.P[3 * v + a] = q[a]; }
+            G.xp[v] = f->phix * q[0] / q[2] + f->cx;
+            G.yp[v] = f->phiy * q[1] / q[2] + f->cy;
+            G.xpf[v] = (float)G.xp[v];
+            G.ypf[v] = (float)G.yp[v];
+            G.zf[v] = (float)q[2];
+        }
+    }
+    std::vector<int> tmap(A, -1);
+    std::vector<double> bal(A, 0.0), bbe(A, 0.0), bga(A, 0.0);
+    if (tri && frozen) {
+        for (size_t k = 0; k < A; ++k) {
+            const int t = tri[k];
+            tmap[k] = t;
+            if (t < 0) continue;
+            const int ib = m->triangles[3 * t], ic = m->triangles[3 * t + 1], ia = m->triangles[3 * t + 2];
+            const double px = f->x0 + (double)(k / Hh), py = f->y0 + (double)(k % Hh);
+            const double ux = G.xp[ib] - G.xp[ia], uy = G.yp[ib] - G.yp[ia];
+            const double vx = G.xp[ic] - G.xp[ia], vy = G.yp[ic] - G.yp[ia];
+            const double wx = px - G.xp[ia], wy = py - G.yp[ia];
+            const double D = ux * vy - uy * vx;
+            bal[k] = (wx * vy - wy * vx) / D;
+            bbe[k] = (ux * wy - uy * wx) / D;
+            bga[k] = 1.0 - bal[k] - bbe[k];
+        }
+    } else {
+        std::vector<uint64_t> keys(A);
+        std::vector<float> bary(2 * A);
+        rasterize(m->n_triangles, m->triangles, G.xpf.data(), G.ypf.data(), G.zf.data(), f->x0, f->y0, W, Hh,
+                  F.o->ntmp_cap, keys.data(), bary.data());
+        for (size_t k = 0; k < A; ++k) {
+            if (keys[k] == ~0ull) continue;
+            tmap[k] = (int)(keys[k] & 0xffffffffu);
+            bal[k] = bary[2 * k];
+            bbe[k] = bary[2 * k + 1];
+            bga[k] = (1.0f - bary[2 * k]) - bary[2 * k + 1];   // the renderer's FP32 gamma
+        }
+        if (tri) std::copy(tmap.begin(), tmap.end(), tri);
+    }
+    std::vector<double> I(A, 0.0);
+    double dlt[3] = {G.lam[0] - G.tau[0], G.lam[1] - G.tau[1], G.lam[2] - G.tau[2]}, rl[3];
+    rotT(G.R, dlt, rl);
+    for (size_t k = 0; k < A; ++k) {
+        const int t = tmap[k];
+        if (t < 0) continue;
+        const int ib = m->triangles[3 * t], ic = m->triangles[3 * t + 1], ia = m->triangles[3 * t + 2];
+        const double al = bal[k], be = bbe[k], ga = bga[k];
+        const double* B0 = &G.P0[3 * ib]; const double* C0 = &G.P0[3 * ic]; const double* A0 = &G.P0[3 * ia];
+        double e1[3] = {B0[0] - A0[0], B0[1] - A0[1], B0[2] - A0[2]};
+        double e2[3] = {C0[0] - A0[0], C0[1] - A0[1], C0[2] - A0[2]};
+        double nr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const double nn = std::sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+        double nrm[3], p0[3];
+        for (int a = 0; a < 3; ++a) nrm[a] = nr[a] / nn;
+        for (int a = 0; a < 3; ++a) p0[a] = al * B0[a] + be * C0[a] + ga * A0[a];
+        const double Id = nrm[0] * rl[0] + nrm[1] * rl[1] + nrm[2] * rl[2] - (nrm[0] * p0[0] + nrm[1] * p0[1] + nrm[2] * p0[2]);
+        const double rho = al * G.rho[ib] + be * G.rho[ic] + ga * G.rho[ia];
+        double nc[3], Y[5];
+        rot(G.R, nrm, nc);
+        sh2_terms(nc, Y);
+        double S = nc[0] * G.a[0] + nc[1] * G.a[1] + nc[2] * G.a[2];
+        for (int j = 0; j < 5; ++j) S += G.sh[j] * Y[j];
+        I[k] = rho + G.Lint * rho * Id + rho * S;
+    }
+    std::vector<double> B(A, 0.0);
+    blur(I.data(), B.data(), W, Hh, F.taps);
+    double loss = 0.0;
+    for (size_t k = 0; k < A; ++k) {
+        if (tmap[k] < 0) continue;
+        const int x = (int)(k / Hh), y = (int)(k % Hh);
+        const int kr = (x < W - 1) ? (int)k + Hh : 0, kl = (x > 0) ? (int)k - Hh : 0;
+        const int ka = (y > 0) ? (int)k - 1 : 0, kb = (y < Hh - 1) ? (int)k + 1 : 0;
+        const double gx = 0.5 * (B[kr] - B[kl]), gy = 0.5 * (B[kb] - B[ka]);
+        const double h = 1e-16 + std::sqrt(4 * gx * gx + 4 * gy * gy);
+        const double sx = 0.5 * (F.obs_blur[kr] - F.obs_blur[kl]), sy = 0.5 * (F.obs_blur[kb] - F.obs_blur[ka]);
+        const double hs = 1e-16 + std::sqrt(4 * sx * sx + 4 * sy * sy);
+        const double rx = 2 * gx / h - 2 * sx / hs, ry = 2 * gy / h - 2 * sy / hs;
+        loss += rx * rx + ry * ry;
+    }
+    for (int i = 0; i < NL; ++i) {
+        const int v = m->landmark_ids[i];
+        const double dx = G.xp[v] - f->lmk_x[i], dy = G.yp[v] - f->lmk_y[i];
+        loss += dx * dx + dy * dy;
+    }
+    return loss;
+}
+
 // ---------------------------------------------------------------- landmark-only fit with slack
 // Logbarrier_Initializer::set_minimal_slack (logbarrier_initializer.cu:1041-1090) and fit_model
 // with use_slack (1095-1267) for one frame (T = 1), no identity unknowns (li_init,
@@ -1403,6 +1513,16 @@ void orc_ortho_init(int n, const float* lmk, const float* cams, const double* p0
 }
 
 int orc_num_unknowns(const tdi_model_desc* m, const tdi_fit_options* o) { return make_layout(m, o).K; }
+
+// The render loss of one frame (render_loss above): theta [K], dverts [V][3] (may be null),
+// tri [W*H] (may be null; see render_loss), frozen.
+double orc_render_loss(const tdi_model_desc* m, const tdi_frame_desc* f, const tdi_fit_options* o, const double* th,
+                       const double* dverts, int32_t* tri, int frozen)
+{
+    Frame F;
+    setup_frame(F, m, f, o);
+    return render_loss(F, th, dverts, tri, frozen);
+}
 
 // Landmark-only fit with slack for n frames: theta [n][K] double in/out (the pose is the start; on
 // success the pose and the first Kl expression coefficients are overwritten), reports [n].
