@@ -10,7 +10,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 namespace tdi {
 
@@ -118,7 +120,9 @@ struct Workspace {
     FitState* st = nullptr;
     float* partials = nullptr;
     float* lmk = nullptr;      // [C][6][51]
-    int S = 1;                 // J^T J splits per frame (fixed per workspace)
+    int S = 1;                 // J^T J splits per frame (the heuristic's, or the autotuner's)
+    int S_cap = 1;             // splits the partials buffer holds
+    int di_area = 0;           // the area that sizes k_dI's workgroups per frame (0: the mean crop)
     // chunk bookkeeping (host)
     std::vector<FrameGPU> hframes;
     int n_cur = 0;
@@ -277,6 +281,16 @@ int tdi_ctx_destroy(tdi_ctx* c)
 
 } // extern "C"
 
+// ---------------------------------------------------------------------------- autotuned launch shapes
+// Choices of tdi_batch_autotune, per (unknowns, frame slots, crop pixels, largest crop, mixed
+// flags): a batch created later in the process with the same key starts from them.
+using TuneKey = std::tuple<int, int, long long, int, int>;
+struct TuneChoice {
+    int S, di_area;
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneChoice> g_tuned;
+
 // ---------------------------------------------------------------------------- workspace
 static int ws_alloc(tdi_batch* b, int C, long long cap)
 {
@@ -330,7 +344,9 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     w.box = A.get<int4>(C);
     w.st = A.get<FitState>(C);
     w.S = jtj_splits(C, b->max_area);
-    const size_t pf = std::max(jtj_partials_floats(C, w.S, K, false), jtj_partials_floats(C, w.S, K, true));
+    // room for the autotuner's candidates up to twice the heuristic's split count
+    w.S_cap = std::min(64, 2 * w.S);
+    const size_t pf = std::max(jtj_partials_floats(C, w.S_cap, K, false), jtj_partials_floats(C, w.S_cap, K, true));
     w.partials = A.get<float>(pf);
     w.lmk = A.get<float>((size_t)C * 6 * NLMK);
     for (void* p : A.ptrs)
@@ -369,6 +385,12 @@ static int ws_load(tdi_batch* b, int i0, int n, hipStream_t s)
 static int mean_crop(const Workspace& w)
 {
     return (int)(w.cap / (w.C > 0 ? w.C : 1));
+}
+
+// The area k_dI's grid is sized for: the autotuner's choice, else the mean crop.
+static int dI_area(const Workspace& w)
+{
+    return w.di_area > 0 ? w.di_area : mean_crop(w);
 }
 
 static LmkArrays lmk_arrays(const Workspace& w)
@@ -497,6 +519,14 @@ extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd,
     }
     int rc = ws_alloc(b, C, cap);
     if (rc) { b->mem.release(); delete b; return rc; }
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        const auto it = g_tuned.find(TuneKey(c->L.K, C, cap, b->max_area, c->opts.mixed_flags));
+        if (it != g_tuned.end() && it->second.S <= b->ws.S_cap) {
+            b->ws.S = it->second.S;
+            b->ws.di_area = it->second.di_area;
+        }
+    }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess) {
@@ -631,7 +661,7 @@ static int eval_jacobian(tdi_batch* b, FrameList fl, const float* theta, int mix
     Workspace& w = b->ws;
     int rc;
     if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
-                        mean_crop(w), s)))
+                        dI_area(w), s)))
         return rc;
     if ((rc = launch_normal_eq(b, fl, mixed, s, &L))) return rc;
     return launch_barrier(bounds_model(b), L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj,
@@ -935,6 +965,90 @@ int tdi_barrier_terms(tdi_batch* b, const float* theta, double* H, double* g, do
     return TDI_OK;
 }
 
+// Launch-shape autotuner: on the first workspace chunk, rendered at theta, time the per-pixel
+// Jacobian kernel for four grid sizes (workgroups per frame sized for half, one, two and four
+// times the mean crop) and the J^T J kernel for the split counts 1, 2, 4, ... up to the partials
+// buffer's capacity, `reps` launches each after one untimed launch (HIP events on the batch's
+// stream); keep the fastest of each for this batch, drop its captured fit schedules (they hold
+// the old shapes) and remember the choice for later batches of the same shape.
+int tdi_batch_autotune(tdi_batch* b, const float* theta, int32_t reps, int32_t* choice, float* report)
+{
+    if (!b || !theta || reps < 1) { set_error("tdi_batch_autotune: null argument or reps < 1"); return TDI_ERR_ARG; }
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    hipStream_t s = b->stream;
+    (void)hipSetDevice(c->device);
+    if (b->n == 0) return TDI_OK;
+    const int n = std::min(w.C, b->n);
+    const int mixed = c->opts.mixed_flags;
+    int rc = ws_load(b, 0, n, s);
+    if (!rc) rc = eval_render(b, all_frames(n), theta, mixed, s);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    TDI_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error("tdi_batch_autotune: event creation failed");
+        return TDI_ERR_HIP;
+    }
+    float rep[11];
+    for (float& r : rep) r = -1.f;
+    auto timed = [&](auto&& launch, float& out) -> int {
+        int r = launch();
+        if (r) return r;
+        if (hipEventRecord(e0, s) != hipSuccess) return TDI_ERR_HIP;
+        for (int i = 0; i < reps && !r; ++i) r = launch();
+        if (r) return r;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return TDI_ERR_HIP;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return TDI_ERR_HIP;
+        out = ms / reps;
+        return TDI_OK;
+    };
+    const int a0 = mean_crop(w);
+    const int areas[4] = {std::max(1, a0 / 2), a0, 2 * a0, 4 * a0};
+    int best_area = a0;
+    float best = 1e30f;
+    for (int i = 0; i < 4 && !rc; ++i) {
+        rc = timed([&] { return launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix,
+                                          w.pixg, w.dI, areas[i], s); },
+                   rep[i]);
+        if (!rc && rep[i] < best) { best = rep[i]; best_area = areas[i]; }
+    }
+    const int S0 = w.S;
+    int best_S = S0;
+    best = 1e30f;
+    for (int lg = 0; lg <= 6 && !rc; ++lg) {
+        const int S = 1 << lg;
+        if (S > w.S_cap) break;
+        w.S = S;
+        rc = timed([&] { return launch_normal_eq(b, all_frames(n), mixed, s); }, rep[4 + lg]);
+        if (!rc && rep[4 + lg] < best) { best = rep[4 + lg]; best_S = S; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) { w.S = S0; return rc; }
+    w.S = best_S;
+    w.di_area = best_area;
+    if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    if (b->graph_fine) { (void)hipGraphExecDestroy(b->graph_fine); b->graph_fine = nullptr; }
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tuned[TuneKey(c->L.K, w.C, w.cap, b->max_area, mixed)] = TuneChoice{best_S, best_area};
+    }
+    if (choice) { choice[0] = best_S; choice[1] = best_area; choice[2] = a0; choice[3] = S0; }
+    if (report) std::memcpy(report, rep, sizeof(rep));
+    return TDI_OK;
+}
+
+int tdi_batch_launch_shape(tdi_batch* b, int32_t* shape)
+{
+    if (!b || !shape) { set_error("tdi_batch_launch_shape: null argument"); return TDI_ERR_ARG; }
+    shape[0] = b->ws.S;
+    shape[1] = dI_area(b->ws);
+    return TDI_OK;
+}
+
 // Stages: 0 synthesis, 1 raster (clear+raster+resolve), 2 shade+blur+grads, 3 barrier,
 // 4 dI, 5 J^T J (+reduce), 6 Cholesky.  ms[7] receives the mean per repetition.
 int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t reps, float* ms)
@@ -974,7 +1088,7 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
             return rc;
         TDI_HIP(hipEventRecord(ev[4], s));
         if ((rc = launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
-                            mean_crop(w), s)))
+                            dI_area(w), s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[5], s));
         if ((rc = launch_normal_eq(b, all_frames(n), mixed, s))) return rc;

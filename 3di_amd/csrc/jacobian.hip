@@ -700,7 +700,7 @@ int jtj_splits(int n, int per_frame_pixels_hint)
     // (measured at 4096 frames: 1, 2, 4, 8, 16 splits give 16.5, 14.5, 14.0, 13.7, 14.6 ms);
     // at least ~2 chunks per workgroup, and at most 64 splits (more split the sums of a frame's
     // normal matrix into partials of a few dozen pixels and cost agreement with FP64: the
-    // one-iteration energy-norm median on 32 cfg2 frames went from 1.7e-4 at 64 to 2.7e-4 at 256)
+    // one-iteration energy-norm median on 32 frames of the 4096-frame workload went from 1.7e-4 at 64 to 2.7e-4 at 256)
     int S = (32768 + n - 1) / n;
     const int maxS = per_frame_pixels_hint / 128 + 1;
     if (S > maxS) S = maxS;

@@ -447,5 +447,5 @@ def cfg2_crop_sizes(n: int, seed: int = BASE_SEED + 2) -> np.ndarray:
 
 
 def cfg2_focal(S: int) -> float:
-    """Focal length that keeps the cfg0 face-to-crop proportion (f = 800 px at 192 px)."""
+    """Focal length that keeps the small plain fit's face-to-crop proportion (f = 800 px at 192 px)."""
     return 800.0 * S / 192.0

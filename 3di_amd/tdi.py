@@ -67,6 +67,8 @@ def lib():
         L.tdi_barrier_terms.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_fit_landmarks_slack.argtypes = [_vp, _vp, C.c_int32, _vp, _vp]
         L.tdi_render_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_batch_autotune.argtypes = [_vp, _vp, C.c_int32, _vp, _vp]
+        L.tdi_batch_launch_shape.argtypes = [_vp, _vp]
         L.tdi_landmark_slack_terms.argtypes = [_vp, _vp, C.c_int32, _vp, C.c_double, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
@@ -220,6 +222,24 @@ class Batch:
         obj = torch.empty(self.n, device="cuda", dtype=torch.float64)
         _check(lib().tdi_barrier_terms(self.h, _dptr(th), _dptr(H), _dptr(g), _dptr(obj), _stream()))
         return H, g, obj
+
+    def autotune(self, theta, reps: int = 3) -> dict:
+        """Pick the per-pixel Jacobian grid and the J^T J split count by timing (tdi_batch_autotune)."""
+        th = self._theta(theta)
+        choice = np.zeros(4, np.int32)
+        rep = np.zeros(11, np.float32)
+        _check(lib().tdi_batch_autotune(self.h, _dptr(th), int(reps), choice.ctypes.data, rep.ctypes.data))
+        a0 = int(choice[2])
+        areas = [max(1, a0 // 2), a0, 2 * a0, 4 * a0]
+        return dict(jtj_splits=int(choice[0]), dI_area=int(choice[1]), mean_crop=a0, default_splits=int(choice[3]),
+                    dI_ms={str(a): round(float(t), 4) for a, t in zip(areas, rep[:4])},
+                    jtj_ms={str(1 << i): round(float(t), 4) for i, t in enumerate(rep[4:]) if t >= 0})
+
+    def launch_shape(self):
+        """(J^T J splits per frame, area the per-pixel Jacobian grid is sized for)."""
+        sh = np.zeros(2, np.int32)
+        _check(lib().tdi_batch_launch_shape(self.h, sh.ctypes.data))
+        return int(sh[0]), int(sh[1])
 
     def render_loss(self, theta, dverts=None, gloss=None, grad: bool = True):
         """The render loss of every frame and, with grad, its adjoint (tdi_render_loss_grad).

@@ -1,6 +1,6 @@
 """Synthetic benchmark workloads, rendered by the HIP library itself.
 
-cfg2 (the batched per-frame stage): identity and albedo drawn once on the 231x231-grid
+The batched per-frame stage: identity and albedo drawn once on the 231x231-grid
 model (53,361 vertices, 105,800 triangles, K_id = K_tex = 199, K_exp = 79) and folded into
 the means; every frame has its own crop side S ~ U{96..384}, focal length 800*S/192, pose
 within +-30 deg, tz in 400..600 mm, expression ~ N(0,1) clipped to +-3, and its own
@@ -53,16 +53,18 @@ def cfg2_model(grid: int = 231, k_exp: int = 79, k_id: int = 199, k_tex: int = 1
 
 
 def cfg2(n_frames: int = 4096, iters: int = 12, grid: int = 231, k_exp: int = 79, fit_light: int = 2,
-         noise: float = 0.02, log=print) -> Workload:
+         noise: float = 0.02, log=print, crop: int = 0, config: int = 2) -> Workload:
+    """crop > 0: every crop crop x crop (the differentiation configuration: 256); config: the seed
+    stream of the frames."""
     t0 = time.time()
     m = cfg2_model(grid, k_exp) if grid == 231 and k_exp == 79 else cfg2_model(grid, k_exp, 0, 0)
     log(f"[workload] model {m.n_vertices} vertices, {m.n_triangles} triangles ({time.time() - t0:.1f}s)")
     lay = synth.layout(m, False, False, True, fit_light)
     opts = abi.default_options(max_iters=iters, fit_light=int(fit_light))
-    crops = synth.cfg2_crop_sizes(n_frames)
+    crops = synth.cfg2_crop_sizes(n_frames) if crop <= 0 else np.full(n_frames, crop)
     frames, truths = [], []
     for i in range(n_frames):
-        rng = np.random.default_rng(synth.frame_seed(2, i))
+        rng = np.random.default_rng(synth.frame_seed(config, i))
         S = int(crops[i])
         tr = synth.draw_truth(m, rng, lay, 30.0, fit_light=fit_light)
         fr = synth.make_frame(m, tr, lay, rng, synth.cfg2_focal(S), S, S)
@@ -91,6 +93,6 @@ def cfg2(n_frames: int = 4096, iters: int = 12, grid: int = 231, k_exp: int = 79
     log(f"[workload] observations rendered, mean {float(npix.float().mean()):.0f} face pixels/frame "
         f"({time.time() - t0:.1f}s)")
     return Workload(model=m, frames=frames, truths=truths, opts=opts, lay=lay,
-                    name=f"cfg2: {n_frames} frames, {grid}x{grid} grid ({m.n_vertices} V, {m.n_triangles} T), "
+                    name=f"{'batched per-frame fit' if crop <= 0 else 'render loss and adjoint'}: {n_frames} frames, {grid}x{grid} grid ({m.n_vertices} V, {m.n_triangles} T), "
                          f"K={lay['K']} (6 pose + {k_exp} expression + {synth.light_unknowns(fit_light)} light), "
-                         f"crops U{{96..384}}, {iters} GN iterations")
+                         + (f"crops U{{96..384}}, {iters} GN iterations" if crop <= 0 else f"{crop}x{crop} crops"))

@@ -2,7 +2,7 @@
 """Benchmark of the batched mixed-precision per-frame 3DMM fit on one MI355X.
 
 One step = a complete fit (12 Gauss-Newton iterations with Armijo line search) of every
-frame of the cfg2 workload (4096 synthetic frames, crops U{96..384}, 94 unknowns per frame:
+frame of the batched per-frame workload (4096 synthetic frames, crops U{96..384}, 94 unknowns per frame:
 6 pose + 79 expression + 9 light),
 inputs resident in HBM, through the batched path (device-resident loop in a hipGraph).
 
@@ -14,9 +14,14 @@ Also measured in the same run:
     unknowns and per unknown, for one Gauss-Newton iteration and for the whole fit;
   * per-stage times with HIP events on the first workspace chunk -> the dominant kernel's
     roofline entry;
-  * the landmark-driven initialisation (tdi_ortho_init) of every frame;
+  * the launch shapes of the per-pixel Jacobian and J^T J kernels, picked by the autotuner
+    (tdi_batch_autotune) before the warm-up; the timed fits use them;
+  * the landmark-driven initialisation (tdi_ortho_init) of every frame, then the landmark-only
+    fit with slack (tdi_fit_landmarks_slack) from it, and the oracle's decisions on a prefix;
   * the frame preparation (tdi_prepare_crops): 8-bit BGR 1280x720 frames in HBM -> the
-    workload's grey crops, against the oracle on a prefix (time and bit-exactness).
+    workload's grey crops, against the oracle on a prefix (time and bit-exactness);
+  * the render loss and its adjoint (tdi_render_loss_grad) on 256 frames of 256 x 256 crops:
+    forward, and forward + backward with the rasterisation recomputed.
 
 Usage: python bench.py [--steps K] [--warmup W] [--frames N] [--gpus 1]
 Prints one JSON line.
@@ -207,6 +212,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--diff-frames", type=int, default=256,
+                    help="frames of the render-loss + adjoint measurement (256 x 256 crops); 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for scaling rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -231,6 +239,13 @@ def main():
     th_init = torch.from_numpy(init).cuda()
     th = th_init.clone()
     log(f"[bench] workspace chunk = {batch.workspace_frames()} frames")
+    # launch shapes of the per-pixel Jacobian and J^T J kernels, picked by timing (autotuner)
+    autotune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        autotune = batch.autotune(th_init, reps=3)
+        autotune["seconds"] = round(time.perf_counter() - t0, 2)
+        log(f"[bench] autotune: {autotune}")
 
     for i in range(args.warmup):
         t0 = time.perf_counter()
@@ -403,6 +418,35 @@ def main():
                       note="ms per call includes the per-call descriptor upload and stream synchronisation")
     log(f"[bench] frame preparation: {args.frames} frames in {prep_ms:.3f} ms")
 
+    # render loss + adjoint (forward and backward, the rasterisation recomputed) on frames of
+    # 256 x 256 crops of the same model
+    differentiation = None
+    if args.diff_frames > 0:
+        wd = workloads.cfg2(n_frames=args.diff_frames, crop=256, config=3, log=log)
+        bd = tdi.Batch(ctx, wd.frames)
+        thd = torch.from_numpy(wd.init()).cuda()
+        bd.render_loss(thd)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            lossd, gthd, gvd = bd.render_loss(thd)
+        torch.cuda.synchronize()
+        dms = 1e3 * (time.perf_counter() - t0) / 3
+        t0 = time.perf_counter()
+        for _ in range(3):
+            bd.render_loss(thd, grad=False)
+        torch.cuda.synchronize()
+        fms = 1e3 * (time.perf_counter() - t0) / 3
+        differentiation = dict(workload=wd.name, frames=args.diff_frames, forward_ms=round(fms, 3),
+                               forward_backward_ms=round(dms, 3),
+                               frames_per_s=round(args.diff_frames / (dms * 1e-3), 1),
+                               gradients="pose, expression, light per frame and d loss / d vertex (53,361 x 3) "
+                                         "per frame; the render recomputed in the backward pass",
+                               mean_loss=round(float(lossd.mean()), 4))
+        log(f"[bench] render loss + adjoint: {args.diff_frames} frames in {dms:.2f} ms (forward {fms:.2f} ms)")
+        del gvd
+        bd.close()
+
     # FP64 oracle on a prefix: host baseline and parameter error of the batched fit
     cpu = None
     accuracy = None
@@ -474,6 +518,8 @@ def main():
         "batched_stop_flags_note": "1 max iterations, 2 decrement test, 4 step below t_min, 8 NaN, 16 not PD",
         "stages": stages,
         "landmark_init": landmark_init,
+        "autotune": autotune,
+        "differentiation": differentiation,
         "frame_prep": frame_prep,
     }
     print(json.dumps(out), flush=True)

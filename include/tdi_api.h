@@ -312,6 +312,19 @@ int tdi_barrier_terms(tdi_batch* b, const float* theta, double* H, double* g, do
 int tdi_render_loss_grad(tdi_batch* b, const float* theta, const float* dverts, const double* gloss, double* loss,
                          float* gtheta, float* gverts, void* stream);
 
+/* Launch-shape autotuner (no counterpart in the reference, whose launch shapes are fixed in the
+ * source): on the batch's first workspace chunk rendered at theta (DEVICE [n][K]), times the
+ * per-pixel Jacobian kernel with its grid sized for 1/2, 1, 2 and 4 times the mean crop, and the
+ * J^T J kernel with 1, 2, 4, ... splits per frame (up to twice the default), `reps` launches each,
+ * and keeps the fastest of each for the batch (its captured fit schedules are rebuilt on the next
+ * fit); later batches of the same shape in the process start from the same choice.  choice [4]
+ * (may be NULL): J^T J splits, Jacobian area, mean crop, the default splits; report [11] (may be
+ * NULL): ms per launch for the four areas, then for 1, 2, 4, ..., 64 splits (-1 where not timed). */
+int tdi_batch_autotune(tdi_batch* b, const float* theta, int32_t reps, int32_t* choice, float* report);
+/* The batch's current launch shape: shape[0] J^T J splits per frame, shape[1] the area k_dI's
+ * grid is sized for. */
+int tdi_batch_launch_shape(tdi_batch* b, int32_t* shape);
+
 /* Frames per workspace chunk (the batch is processed in chunks of this many frames). */
 int tdi_workspace_frames(tdi_batch* b);
 
