@@ -357,6 +357,10 @@ int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, FrameList fl, const
 int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                    const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
                    double* H, float* g, hipStream_t s);
+// cholesky_big.hip: FP64 blocked Cholesky solve for K above what one workgroup holds in LDS
+constexpr int CHOL_BIG_MAXK = 4096;
+int launch_cholesky_big(int n, int K, const float* H, const float* g, float* delta, int* status, hipStream_t s);
+
 // adjoint.hip: the render loss and its adjoint (scratch: device buffers of the caller)
 struct AdjScratch {
     double* dB;        // [cap] dL/d(blurred render)

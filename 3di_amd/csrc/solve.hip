@@ -562,6 +562,8 @@ static int launch_cholesky_t(int n, int K, const TH* H, const float* g, float* d
 int launch_cholesky(int n, int K, const float* H, const float* g, float* delta, int* status, int mode, const int* mask,
                     hipStream_t s)
 {
+    // above one workgroup's LDS: the blocked FP64 factorisation (every listed matrix)
+    if (K > CMAXK && !mask) return launch_cholesky_big(n, K, H, g, delta, status, s);
     return launch_cholesky_t(n, K, H, g, delta, status, mode, mask, s);
 }
 

@@ -215,7 +215,10 @@ int tdi_normal_equations_mixed(tdi_batch* b, const float* theta, float* H, float
 /* Batched Cholesky solve of H delta = -g.  mode 0: FP64 factorisation (Solver::solve,
  * solver.h:70-105, cusolverDnDpotrf/Dpotrs); mode 1: FP32 factorisation of the equilibrated
  * matrix D^-1/2 H D^-1/2 + one FP64 residual-refinement step, with the FP64 factorisation
- * as the fallback of a frame whose FP32 factorisation breaks down.  status: [n] tdi_status. */
+ * as the fallback of a frame whose FP32 factorisation breaks down.  status: [n] tdi_status.
+ * K <= 112: one workgroup per matrix in LDS.  112 < K <= 4096 (the joint identity-stage system,
+ * 199 + 199 + 94 per frame): blocked FP64 factorisation of an FP64 copy in HBM (32-column panels,
+ * the trailing update over a grid of tiles), mode ignored. */
 int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, float* delta,
                        int32_t* status, int32_t mode, void* stream);
 
