@@ -5,7 +5,8 @@
 // joint identity-stage system (K up to a few thousand: 199 + 199 + 94 per frame).  Blocked
 // right-looking factorisation in place on an FP64 copy in HBM, 32-column panels:
 //   k_chol_panel   one workgroup per matrix: factor the 32x32 diagonal block in LDS, then the
-//                  panel below it (L21 = A21 L11^-T, one row per thread, L11 in LDS);
+//                  panel below it (L21 = A21 L11^-T, 128 rows at a time staged in LDS, one row
+//                  per thread);
 //   k_chol_update  a grid of 32x32 tiles of the trailing lower triangle per matrix:
 //                  A22 -= L21 L21^T, both 32-row slices of the panel staged in LDS;
 //   k_chol_trsv    one workgroup per matrix: L y = -g, then L^T delta = y (column-oriented,
@@ -66,15 +67,29 @@ __global__ void __launch_bounds__(256) k_chol_panel(int K, int k0, double* __res
         const int i = e / nb, j = e - i * nb;
         if (j <= i) M[(size_t)(k0 + i) * K + k0 + j] = D[i][j];
     }
-    // panel: row r of A21 solves x L11^T = a (forward substitution over the 32 columns, in
-    // place: the thread's own row)
-    for (int r = k0 + nb + tid; r < K; r += blockDim.x) {
-        double* row = M + (size_t)r * K + k0;
-        for (int j = 0; j < nb; ++j) {
-            double s = row[j];
-            for (int q = 0; q < j; ++q) s -= row[q] * D[j][q];
-            row[j] = s / D[j][j];
+    // panel: row r of A21 solves x L11^T = a (forward substitution over the panel's columns),
+    // 128 rows at a time staged in LDS (coalesced loads and stores, LDS operands)
+    __shared__ double Pn[128][CB + 1];
+    for (int r0 = k0 + nb; r0 < K; r0 += 128) {
+        const int nr = min(128, K - r0);
+        for (int e = tid; e < nr * nb; e += blockDim.x) {
+            const int i = e / nb, j = e - i * nb;
+            Pn[i][j] = M[(size_t)(r0 + i) * K + k0 + j];
         }
+        __syncthreads();
+        if (tid < nr) {
+            for (int j = 0; j < nb; ++j) {
+                double s = Pn[tid][j];
+                for (int q = 0; q < j; ++q) s -= Pn[tid][q] * D[j][q];
+                Pn[tid][j] = s / D[j][j];
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < nr * nb; e += blockDim.x) {
+            const int i = e / nb, j = e - i * nb;
+            M[(size_t)(r0 + i) * K + k0 + j] = Pn[i][j];
+        }
+        __syncthreads();
     }
 }
 
@@ -104,6 +119,7 @@ __global__ void __launch_bounds__(256) k_chol_update(int K, int k0, double* __re
     // 256 threads x 4 entries: thread (a, b4) computes row a, columns b4..b4+3 of the tile
     const int a = tid >> 3, b4 = (tid & 7) * 4;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
     for (int c = 0; c < CB; ++c) {
         const double li = Li[a][c];
 #pragma unroll
