@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
@@ -89,6 +90,73 @@ public:
     void add_illum(size_t frame_id, const std::vector<float>& il)
     {
         if (illum_coeffs.find(frame_id) == illum_coeffs.end()) illum_coeffs.insert(std::make_pair(frame_id, il));
+    }
+
+    // Every fit added, in order, so that a long run can be saved and replayed exactly.
+    struct Added {
+        size_t frame_id;
+        std::vector<float> exp, pose, illum;   // pose, illum empty when not given
+    };
+    std::vector<Added> log;
+    size_t dumped = 0;   // entries of `log` already written by dump()
+
+    void add(size_t frame_id, const std::vector<float>& e, const std::vector<float>& p, const std::vector<float>& il)
+    {
+        add_exp_coeffs(frame_id, e);
+        if (!p.empty()) add_pose(frame_id, p);
+        if (!il.empty()) add_illum(frame_id, il);
+        log.push_back(Added{frame_id, e, p, il});
+    }
+
+    // The entries added since the last dump, appended to `path` as text with every float in
+    // hexadecimal (exact): "frame_id n_exp n_pose n_illum values...".
+    bool dump(const std::string& path)
+    {
+        std::ofstream out(path.c_str(), std::ios::app);
+        if (!out) return false;
+        out << std::hexfloat;
+        for (; dumped < log.size(); ++dumped) {
+            const Added& a = log[dumped];
+            out << a.frame_id << ' ' << a.exp.size() << ' ' << a.pose.size() << ' ' << a.illum.size();
+            for (float v : a.exp) out << ' ' << v;
+            for (float v : a.pose) out << ' ' << v;
+            for (float v : a.illum) out << ' ' << v;
+            out << '\n';
+        }
+        return (bool)out;
+    }
+
+    // Replays a dump, entries of frames below `frame_limit` only (the entries count as written).
+    bool load(const std::string& path, size_t frame_limit = (size_t)-1)
+    {
+        std::ifstream in(path.c_str());
+        if (!in) return false;
+        std::string line;
+        while (std::getline(in, line)) {
+            if (line.empty()) continue;
+            std::istringstream ls(line);
+            size_t fid = 0, ne = 0, np = 0, ni = 0;
+            if (!(ls >> fid >> ne >> np >> ni) || ne != (size_t)Kepsilon || (np != 0 && np != 6) || (ni != 0 && ni != 4))
+                return false;
+            std::vector<float> e(ne), p(np), il(ni);
+            for (auto* v : {&e, &p, &il})
+                for (float& x : *v) {
+                    std::string tok;
+                    if (!(ls >> tok)) return false;
+                    x = std::strtof(tok.c_str(), nullptr);
+                }
+            if (fid < frame_limit) add(fid, e, p, il);
+        }
+        dumped = log.size();
+        return true;
+    }
+
+    // Rewrites `path` with every entry of the log.
+    bool rewrite(const std::string& path)
+    {
+        { std::ofstream trunc(path.c_str(), std::ios::trunc); if (!trunc) return false; }
+        dumped = 0;
+        return dump(path);
     }
 
     std::vector<float> compute_avg_exp_coeffs(size_t frame_id, bool ignore_first)
@@ -392,10 +460,22 @@ void tdi_vo_destroy(tdi_video_output* vo) { delete vo; }
 int tdi_vo_add(tdi_video_output* vo, int64_t frame_id, const float* exp, const float* pose, const float* illum)
 {
     if (!vo || !exp || frame_id < 0) return TDI_ERR_ARG;
-    vo->vo.add_exp_coeffs((size_t)frame_id, std::vector<float>(exp, exp + vo->k_exp));
-    if (pose) vo->vo.add_pose((size_t)frame_id, std::vector<float>(pose, pose + 6));
-    if (illum) vo->vo.add_illum((size_t)frame_id, std::vector<float>(illum, illum + 4));
+    vo->vo.add((size_t)frame_id, std::vector<float>(exp, exp + vo->k_exp),
+               pose ? std::vector<float>(pose, pose + 6) : std::vector<float>(),
+               illum ? std::vector<float>(illum, illum + 4) : std::vector<float>());
     return TDI_OK;
+}
+
+int tdi_vo_dump(tdi_video_output* vo, const char* path)
+{
+    if (!vo || !path) return TDI_ERR_ARG;
+    return vo->vo.dump(path) ? TDI_OK : TDI_ERR_ARG;
+}
+
+int tdi_vo_load(tdi_video_output* vo, const char* path)
+{
+    if (!vo || !path) return TDI_ERR_ARG;
+    return vo->vo.load(path) ? TDI_OK : TDI_ERR_ARG;
 }
 
 int tdi_vo_save(tdi_video_output* vo, int32_t kind, const char* path, int32_t flag)
@@ -790,6 +870,145 @@ int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n
         if (rc) return rc;
     }
     return TDI_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------- long videos
+namespace tdi_host {
+
+struct LongState {
+    int n_frames = 0, K = 0, frames_done = 0, chunks_done = 0;
+    std::vector<float> theta, lam, Lint;
+};
+
+static bool read_state(const std::string& path, LongState& st)
+{
+    std::ifstream in(path.c_str());
+    if (!in) return false;
+    std::string tag;
+    int version = 0;
+    if (!(in >> tag >> version) || tag != "tdi_long_video" || version != 1) return false;
+    auto floats = [&](const char* name, size_t n, std::vector<float>& v) {
+        std::string key;
+        if (!(in >> key) || key != name) return false;
+        v.resize(n);
+        for (float& x : v) {
+            std::string tok;
+            if (!(in >> tok)) return false;
+            x = std::strtof(tok.c_str(), nullptr);
+        }
+        return true;
+    };
+    std::string k1, k2, k3, k4;
+    if (!(in >> k1 >> st.n_frames >> k2 >> st.K >> k3 >> st.frames_done >> k4 >> st.chunks_done)) return false;
+    if (k1 != "n_frames" || k2 != "K" || k3 != "frames_done" || k4 != "chunks_done") return false;
+    return floats("theta", (size_t)st.K, st.theta) && floats("lam", 3, st.lam) && floats("Lint", 1, st.Lint);
+}
+
+// Written to a temporary file and renamed over the old one, so that a run stopped while writing
+// leaves the previous state whole.
+static bool write_state(const std::string& path, const LongState& st)
+{
+    const std::string tmp = path + ".tmp";
+    {
+        std::ofstream out(tmp.c_str(), std::ios::trunc);
+        if (!out) return false;
+        out << "tdi_long_video 1\n";
+        out << "n_frames " << st.n_frames << " K " << st.K << " frames_done " << st.frames_done << " chunks_done "
+            << st.chunks_done << "\n" << std::hexfloat;
+        out << "theta";
+        for (float v : st.theta) out << ' ' << v;
+        out << "\nlam";
+        for (float v : st.lam) out << ' ' << v;
+        out << "\nLint";
+        for (float v : st.Lint) out << ' ' << v;
+        out << "\n";
+        if (!out) return false;
+    }
+    return std::rename(tmp.c_str(), path.c_str()) == 0;
+}
+
+} // namespace tdi_host
+
+extern "C" {
+
+int tdi_fit_video_long(tdi_ctx* ctx, const tdi_video_params* p, const tdi_long_video* lv, tdi_frame_source src,
+                       void* user, const float* lmk_x, const float* lmk_y, const float* cams, float* theta, float* lam,
+                       float* Lint, tdi_video_output* vo, int32_t* status, int32_t* frames_done)
+{
+    if (!ctx || !p || !lv || !src || !lmk_x || !lmk_y || !cams || !theta || !lam || !Lint || !vo || !status ||
+        lv->n_frames <= 0 || lv->src_w < 1 || lv->src_h < 1 || lv->chunk_frames < 0 || lv->max_chunks < 0)
+        return TDI_ERR_ARG;
+    const int NL = TDI_NLANDMARKS, n = lv->n_frames;
+    const int K = vo->k_exp + 6;
+    const int64_t stride = (int64_t)lv->src_w * lv->src_h * 3;
+    const int chunk = lv->chunk_frames > 0 ? std::min(lv->chunk_frames, n) : std::min(n, 1024);
+    const std::string state_path = lv->state_path ? lv->state_path : "";
+    const std::string fits_path = state_path.empty() ? "" : state_path + ".fits";
+    tdi_host::LongState st;
+    st.n_frames = n;
+    st.K = K;
+    st.theta.assign(theta, theta + K);
+    st.lam.assign(lam, lam + 3);
+    st.Lint.assign(Lint, Lint + 1);
+    for (int i = 0; i < n; ++i) status[i] = -1;
+    if (lv->resume && !state_path.empty()) {
+        tdi_host::LongState prev;
+        if (tdi_host::read_state(state_path, prev)) {
+            if (prev.n_frames != n || prev.K != K || !vo->vo.log.empty()) return TDI_ERR_ARG;
+            // the fits of the chunks the state records (a chunk whose state was not written is redone)
+            if (!vo->vo.load(fits_path, (size_t)prev.frames_done) || !vo->vo.rewrite(fits_path)) return TDI_ERR_ARG;
+            st = prev;
+        } else {
+            std::remove(fits_path.c_str());   // no state: nothing finished, start afresh
+        }
+    } else if (!state_path.empty()) {
+        std::remove(fits_path.c_str());
+        std::remove(state_path.c_str());
+    }
+    int ran = 0, rc = TDI_OK;
+    std::vector<float> th, ml, mL;
+    std::vector<int64_t> ids;
+    while (st.frames_done < n && (lv->max_chunks == 0 || ran < lv->max_chunks)) {
+        const int first = st.frames_done, m = std::min(chunk, n - first);
+        uint8_t* bgr = const_cast<uint8_t*>(src(user, first, m));
+        if (!bgr) { rc = TDI_ERR_ARG; break; }
+        // every frame of the chunk starts from the carried state (the last fit of the previous chunk)
+        th.resize((size_t)m * K);
+        ml.resize((size_t)m * 3);
+        mL.resize(m);
+        ids.resize(m);
+        for (int j = 0; j < m; ++j) {
+            std::memcpy(&th[(size_t)j * K], st.theta.data(), sizeof(float) * K);
+            std::memcpy(&ml[(size_t)j * 3], st.lam.data(), sizeof(float) * 3);
+            mL[j] = st.Lint[0];
+            ids[j] = first + j;
+        }
+        rc = tdi_fit_video_frames_nres(ctx, p, m, bgr, lv->src_w, lv->src_h, stride, lmk_x + (size_t)first * NL,
+                                       lmk_y + (size_t)first * NL, cams + (size_t)first * 4, nullptr, ids.data(),
+                                       th.data(), ml.data(), mL.data(), vo, status + first, nullptr);
+        if (rc) break;
+        for (int j = m - 1; j >= 0; --j)
+            if (status[first + j] == TDI_OK) {
+                std::memcpy(st.theta.data(), &th[(size_t)j * K], sizeof(float) * K);
+                std::memcpy(st.lam.data(), &ml[(size_t)j * 3], sizeof(float) * 3);
+                st.Lint[0] = mL[j];
+                break;
+            }
+        st.frames_done += m;
+        st.chunks_done += 1;
+        ++ran;
+        if (!state_path.empty() && (!vo->vo.dump(fits_path) || !tdi_host::write_state(state_path, st))) {
+            rc = TDI_ERR_ARG;
+            break;
+        }
+    }
+    std::memcpy(theta, st.theta.data(), sizeof(float) * K);
+    std::memcpy(lam, st.lam.data(), sizeof(float) * 3);
+    Lint[0] = st.Lint[0];
+    if (frames_done) *frames_done = st.frames_done;
+    return rc;
 }
 
 } // extern "C"

@@ -36,6 +36,10 @@ def lib():
         L.tdi_vo_destroy.argtypes = [_vp]
         L.tdi_vo_add.argtypes = [_vp, C.c_int64, _vp, _vp, _vp]
         L.tdi_vo_save.argtypes = [_vp, C.c_int32, C.c_char_p, C.c_int32]
+        L.tdi_vo_dump.argtypes = [_vp, C.c_char_p]
+        L.tdi_vo_load.argtypes = [_vp, C.c_char_p]
+        L.tdi_fit_video_long.argtypes = [_vp, _vp, _vp, FRAME_SOURCE, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         _vp]
         L.tdi_euler_angles.argtypes = [_vp, _vp]
         L.tdi_read_config.argtypes = [C.c_char_p, _vp, _vp, _vp]
         L.tdi_fit_video_frames.argtypes = [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp]
@@ -60,6 +64,16 @@ def _f32p(a, n):
     return a
 
 
+# tdi_frame_source: (user, first, count) -> device pointer to count BGR8 frames
+FRAME_SOURCE = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_int32, C.c_int32)
+
+
+class LongVideo(C.Structure):
+    """tdi_long_video (include/tdi_host.h)."""
+    _fields_ = [("n_frames", C.c_int32), ("src_w", C.c_int32), ("src_h", C.c_int32), ("chunk_frames", C.c_int32),
+                ("max_chunks", C.c_int32), ("state_path", C.c_char_p), ("resume", C.c_int32)]
+
+
 class VideoOutput:
     """Per-frame results of several fits and the reference's output files."""
 
@@ -77,6 +91,16 @@ class VideoOutput:
                               None if il is None else il.ctypes.data)
         if rc:
             raise ValueError("bad frame")
+
+    def dump(self, path: str):
+        """Appends the fits added since the last dump to path (exact, hexadecimal)."""
+        if lib().tdi_vo_dump(self.h, os.fsencode(path)):
+            raise OSError(path)
+
+    def load(self, path: str):
+        """Replays the fits of a dump."""
+        if lib().tdi_vo_load(self.h, os.fsencode(path)):
+            raise ValueError(f"{path}: not a VideoOutput dump of {self.k_exp} expressions")
 
     def save(self, kind: int, path: str, flag: int = 0):
         if lib().tdi_vo_save(self.h, kind, os.fsencode(path), flag):
@@ -163,6 +187,48 @@ def fit_video_frames_nres(ctx, params, frames, lmk_x, lmk_y, cams, theta, lam, f
     if rc:
         raise tdi.TdiError(f"tdi_fit_video_frames_nres: {rc}: {tdi.lib().tdi_last_error().decode()}")
     return th, la, Li, st, list(reps)
+
+
+def fit_video_long(ctx, params, n_frames: int, width: int, height: int, source, lmk_x, lmk_y, cams, theta, lam,
+                   vo, Lint: float = 0.005, chunk_frames: int = 0, max_chunks: int = 0, state_path=None,
+                   resume: bool = False):
+    """The per-frame stage of a long video, chunk by chunk (tdi_fit_video_long).  source(first,
+    count) returns a contiguous uint8 CUDA tensor [count, height, width, 3] of those frames (kept
+    alive until the next call; painted in place).  theta [K], lam [3]: the start.  Returns
+    (theta, lam, Lint, status [n_frames], frames_done)."""
+    K = vo.k_exp + 6
+    keep = {}
+
+    def _src(user, first, count):
+        t = source(int(first), int(count))
+        if (t is None or not t.is_cuda or not t.is_contiguous() or t.dtype != torch_uint8()
+                or tuple(t.shape) != (count, height, width, 3)):
+            return None
+        keep["frames"] = t
+        return t.data_ptr()
+
+    cb = FRAME_SOURCE(_src)
+    lv = LongVideo(n_frames=n_frames, src_w=width, src_h=height, chunk_frames=chunk_frames, max_chunks=max_chunks,
+                   state_path=None if state_path is None else os.fsencode(state_path), resume=int(resume))
+    lx = np.ascontiguousarray(lmk_x, np.float32).reshape(n_frames, 51)
+    ly = np.ascontiguousarray(lmk_y, np.float32).reshape(n_frames, 51)
+    cm = np.ascontiguousarray(cams, np.float32).reshape(n_frames, 4)
+    th = np.ascontiguousarray(np.array(theta, np.float32).reshape(K))
+    la = np.ascontiguousarray(np.array(lam, np.float32).reshape(3))
+    Li = np.array([Lint], np.float32)
+    st = np.zeros(n_frames, np.int32)
+    done = np.zeros(1, np.int32)
+    rc = lib().tdi_fit_video_long(ctx.h, C.addressof(params), C.addressof(lv), cb, None, lx.ctypes.data,
+                                  ly.ctypes.data, cm.ctypes.data, th.ctypes.data, la.ctypes.data, Li.ctypes.data,
+                                  vo.h, st.ctypes.data, done.ctypes.data)
+    if rc:
+        raise tdi.TdiError(f"tdi_fit_video_long: {rc}: {tdi.lib().tdi_last_error().decode()}")
+    return th, la, float(Li[0]), st, int(done[0])
+
+
+def torch_uint8():
+    import torch
+    return torch.uint8
 
 
 def euler_angles(u) -> np.ndarray:

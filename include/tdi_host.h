@@ -108,6 +108,44 @@ int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n
                               const float* cams, const float* boxes, const int64_t* frame_ids, float* theta, float* lam,
                               float* Lint, tdi_video_output* vo, int32_t* status, tdi_fit_report* reports);
 
+/* ---- Long videos: the per-frame stage over more frames than one batch should hold (the
+ *      reference's frame loop of fit_video_frames_auto runs to the end of the video and writes
+ *      its VideoOutput at the end, video_fitter.cu:160-393).  The frames are taken chunk by
+ *      chunk from a source, each chunk runs tdi_fit_video_frames_nres, and every fit goes to vo;
+ *      with a state file the run can stop after any chunk and resume there. ---- */
+
+/* Frames [first, first + count) of the video: a DEVICE pointer to count consecutive BGR8 frames
+ * of src_h x src_w x 3 bytes, valid until the next call; the driver paints them in place
+ * (inner-mouth masking).  NULL aborts the run with TDI_ERR_ARG. */
+typedef const uint8_t* (*tdi_frame_source)(void* user, int32_t first, int32_t count);
+
+typedef struct tdi_long_video {
+    int32_t n_frames;           /* frames of the video                                          */
+    int32_t src_w, src_h;       /* frame size                                                   */
+    int32_t chunk_frames;       /* frames per chunk (0: 1024)                                   */
+    int32_t max_chunks;         /* chunks to run in this call (0: to the end)                   */
+    const char* state_path;     /* state file (NULL: none); the fits so far go to <state>.fits  */
+    int32_t resume;             /* 1: continue after the chunks the state file records          */
+} tdi_long_video;
+
+/* Landmarks lmk_x, lmk_y [n_frames][51] and cams [n_frames][4] of the unresized frames, host.
+ * theta [K], lam [3], Lint [1] (host, in/out): the start of every frame of the first chunk; each
+ * later chunk starts from the last successful fit of the one before (the state carried, and
+ * returned).  status [n_frames]: as tdi_fit_video_frames_nres's for the frames run in this call,
+ * -1 for the others.  vo receives every fit (on resume, first the saved fits of the chunks the
+ * state records; vo must then be empty).  frames_done (may be NULL): frames finished so far.
+ * The state file records the frames done and the carried state, written after each chunk by
+ * renaming a temporary file; <state>.fits holds every fit in exact (hexadecimal) form, so that a
+ * resumed run writes the same output files as an uninterrupted one. */
+int tdi_fit_video_long(tdi_ctx* ctx, const tdi_video_params* p, const tdi_long_video* lv, tdi_frame_source src,
+                       void* user, const float* lmk_x, const float* lmk_y, const float* cams, float* theta, float* lam,
+                       float* Lint, tdi_video_output* vo, int32_t* status, int32_t* frames_done);
+
+/* Every fit added to vo since its last dump, appended to path in exact (hexadecimal) form; and
+ * the fits of such a file replayed into vo. */
+int tdi_vo_dump(tdi_video_output* vo, const char* path);
+int tdi_vo_load(tdi_video_output* vo, const char* path);
+
 /* ---- Frame preparation (host side; the pixels are tdi_prepare_crops in tdi_api.h) ---- */
 
 /* Landmark text file, one frame per line "x0 y0 x1 y1 ... x50 y50", as

@@ -150,3 +150,56 @@ def test_video_stage_matches_its_steps(tmp_path):
     t2, _, _ = _steps(ctx, dframes, lx, ly, cams, th0, lam0, sizes[:1], False, init="ortho")
     ctx.close()
     assert np.array_equal(th2, t2)
+
+
+def test_long_video_chunks_and_resume(tmp_path):
+    _say("case: long-video driver, 6 frames in chunks of 2, stop and resume")
+    m, lay, opts, frames, lx, ly, cams, truths = _video(n=6)
+    n, K = len(truths), lay["K"]
+    H, W = frames.shape[1:3]
+    ctx = tdi.Context(m, opts)
+    dframes = torch.from_numpy(frames).cuda()
+
+    def source(first, count):
+        return dframes[first:first + count]
+
+    params = video.video_params(nres_coefs=2, twostage_illum=0, batched=1, crop_w=128, crop_h=128,
+                                ref_face_size=86.0)
+    th0 = truths[0].init.astype(np.float32)
+    lam0 = np.asarray(synth.LAMBDA0, np.float32)
+    # uninterrupted
+    vo_a = video.VideoOutput(K - 6, 2)
+    th_a, la_a, L_a, st_a, done = video.fit_video_long(ctx, params, n, W, H, source, lx, ly, cams, th0, lam0, vo_a,
+                                                       chunk_frames=2, state_path=str(tmp_path / "a.state"))
+    _say(f"  uninterrupted: status {st_a.tolist()}, frames done {done}")
+    assert done == n and (st_a == 0).all()
+    # one chunk, then a resumed run with a fresh VideoOutput
+    vo_b = video.VideoOutput(K - 6, 2)
+    _, _, _, st_b, done_b = video.fit_video_long(ctx, params, n, W, H, source, lx, ly, cams, th0, lam0, vo_b,
+                                                 chunk_frames=2, max_chunks=1, state_path=str(tmp_path / "b.state"))
+    assert done_b == 2 and (st_b[:2] == 0).all() and (st_b[2:] == -1).all()
+    vo_c = video.VideoOutput(K - 6, 2)
+    th_c, la_c, L_c, st_c, done_c = video.fit_video_long(ctx, params, n, W, H, source, lx, ly, cams, th0, lam0, vo_c,
+                                                         chunk_frames=2, state_path=str(tmp_path / "b.state"),
+                                                         resume=True)
+    assert done_c == n and (st_c[:2] == -1).all() and (st_c[2:] == 0).all()
+    assert np.array_equal(th_c, th_a) and np.array_equal(la_c, la_a) and L_c == L_a
+    # the chunks are tdi_fit_video_frames_nres calls from the carried start
+    vo_d = video.VideoOutput(K - 6, 2)
+    carry = (th0.copy(), lam0.copy(), 0.005)
+    for c in range(0, n, 2):
+        sl = slice(c, c + 2)
+        th, la, Li, st, _ = video.fit_video_frames_nres(
+            ctx, params, dframes[sl], lx[sl], ly[sl], cams[sl], np.tile(carry[0], (2, 1)), np.tile(carry[1], (2, 1)),
+            frame_ids=np.arange(c, c + 2), vo=vo_d, Lint=np.full(2, carry[2], np.float32))
+        j = max(i for i in range(2) if st[i] == 0)
+        carry = (th[j].copy(), la[j].copy(), float(Li[j]))
+    assert np.array_equal(carry[0], th_a)
+    for v, tag in ((vo_a, "a"), (vo_c, "c"), (vo_d, "d")):
+        v.save_expressions_all(str(tmp_path / f"{tag}.ea"))
+        v.save_poses(str(tmp_path / f"{tag}.p"))
+        v.save_illums(str(tmp_path / f"{tag}.i"))
+    for ext in ("ea", "p", "i"):
+        a = (tmp_path / f"a.{ext}").read_bytes()
+        assert a == (tmp_path / f"c.{ext}").read_bytes() and a == (tmp_path / f"d.{ext}").read_bytes()
+    ctx.close()

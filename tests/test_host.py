@@ -156,3 +156,29 @@ def test_read_video_config(tmp_path):
     assert o.finetune_coef == 2.0 and o.blur_sigma == 1.0 and o.ref_face_size == 90.0
     d = video.video_params()
     assert (d.ref_face_size, d.nres_coefs, d.pad, d.crop_w, d.crop_h) == (86.0, 4, 1, 256, 256)
+
+
+def test_video_output_dump_and_load(tmp_path):
+    """VideoOutput's exact dump: replaying it gives byte-identical output files; a second dump
+    appends only the fits added since the first."""
+    rng = np.random.default_rng(8)
+    vo = video.VideoOutput(5, 2)
+    for fid in (0, 1, 1, 2, 3, 3):
+        vo.add(fid, rng.normal(size=5).astype(np.float32), rng.normal(size=6).astype(np.float32),
+               rng.normal(size=4).astype(np.float32))
+    d = tmp_path / "fits.txt"
+    vo.dump(str(d))
+    vo.add(4, rng.normal(size=5).astype(np.float32), rng.normal(size=6).astype(np.float32))
+    vo.dump(str(d))
+    assert len(d.read_text().splitlines()) == 7
+    vo2 = video.VideoOutput(5, 2)
+    vo2.load(str(d))
+    for v, tag in ((vo, "a"), (vo2, "b")):
+        v.save_expressions(str(tmp_path / f"{tag}.e"))
+        v.save_expressions_all(str(tmp_path / f"{tag}.ea"))
+        v.save_poses(str(tmp_path / f"{tag}.p"))
+        v.save_illums(str(tmp_path / f"{tag}.i"))
+    for ext in ("e", "ea", "p", "i"):
+        assert (tmp_path / f"a.{ext}").read_bytes() == (tmp_path / f"b.{ext}").read_bytes()
+    with pytest.raises(ValueError):
+        video.VideoOutput(6, 2).load(str(d))
