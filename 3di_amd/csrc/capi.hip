@@ -125,6 +125,7 @@ struct Workspace {
     int di_area = 0;           // the area that sizes k_dI's workgroups per frame (0: the mean crop)
     // chunk bookkeeping (host)
     std::vector<FrameGPU> hframes;
+    std::vector<int> hlist;    // staging of a frame list (count, then slots)
     int n_cur = 0;
     int max_area = 0;
 };
@@ -1219,15 +1220,101 @@ int tdi_fit_plain_finetune(tdi_batch* b, float* theta_host, tdi_fit_report* repo
 }
 
 // ---------------------------------------------------------------------------- light sub-fits
-// fit_3DMM_lambdas (model_fitter.cu:981-1073) and fit_3DMM_Lintensity (model_fitter.cu:1084-1180)
-// one frame at a time, host-driven like the reference.  Shape and pose stay at theta; the
-// light is appended to the unknowns (layout Lx), so the light block of the joint normal
-// equations is w J^T J and -w J^T g^_s of the light columns (the barrier does not involve
+// fit_3DMM_lambdas (model_fitter.cu:981-1073) and fit_3DMM_Lintensity (model_fitter.cu:1084-1180):
+// every frame takes the reference's own sequence of decisions (its outer iterations, line-search
+// steps and stops), but the evaluations of a workspace chunk run together: one evaluation of the
+// objective, gradient and normal matrix for the frames starting an outer iteration, then rounds
+// of objective evaluations for the frames still in their line search.  Shape and pose stay at
+// theta; the light is appended to the unknowns (layout Lx), so the light block of the joint
+// normal equations is w J^T J and -w J^T g^_s of the light columns (the barrier does not involve
 // the light): divided by w they are the reference's JTJ_lambda / dG_dtheta_lambda
 // (derivative_computer.cu:228-312) and their L counterparts (318-400).  The objective
 // -sum g^_r.g^_s (render_for_illumination_only, renderer.cu:737-791) is the data part of
 // the evaluation divided by w.  lambda: 3x3 FP64 Cholesky, Armijo 0.5/0.5, stop below
 // t = 0.01 or -d.g < 3e-4; L: d = -g/H, Armijo 0.3/0.5, stop below t = 0.1 or -d.g < 2.5e-3.
+namespace {
+
+struct LightFrame {
+    enum Phase { OUTER, SEARCH, DONE };
+    Phase phase = OUTER;
+    int outer = 0, inner = 0;
+    double obj0 = 0, dg = 0, t = 1;
+    double d[3] = {0, 0, 0};
+};
+
+// 3x3 lower Cholesky in FP64 (Solver(3), solver.h:70-105): d = -A^-1 g; false when A is not PD.
+bool solve3(double A[9], const double g[3], double d[3])
+{
+    for (int jj = 0; jj < 3; ++jj) {
+        double dd = A[jj * 3 + jj];
+        for (int k = 0; k < jj; ++k) dd -= A[jj * 3 + k] * A[jj * 3 + k];
+        if (!(dd > 0)) return false;
+        dd = std::sqrt(dd);
+        A[jj * 3 + jj] = dd;
+        for (int ii = jj + 1; ii < 3; ++ii) {
+            double v = A[ii * 3 + jj];
+            for (int k = 0; k < jj; ++k) v -= A[ii * 3 + k] * A[jj * 3 + k];
+            A[ii * 3 + jj] = v / dd;
+        }
+    }
+    double y[3];
+    for (int ii = 0; ii < 3; ++ii) {
+        double v = -g[ii];
+        for (int k = 0; k < ii; ++k) v -= A[ii * 3 + k] * y[k];
+        y[ii] = v / A[ii * 3 + ii];
+    }
+    for (int ii = 2; ii >= 0; --ii) {
+        double v = y[ii];
+        for (int k = ii + 1; k < 3; ++k) v -= A[k * 3 + ii] * d[k];
+        d[ii] = v / A[ii * 3 + ii];
+    }
+    return true;
+}
+
+} // namespace
+
+// Evaluations of the light sub-fits after the first: the shape and pose are those of the last full
+// evaluation of each listed frame's slot, so its synthesis, keys, visible box, visible list and
+// barrier value in the workspace hold; only the shading and what follows it are redone.
+// Objective, gradient and normal matrix (eval_full's shading, blur, normalised gradients and
+// Jacobian stages):
+static int eval_light_full(tdi_batch* b, FrameList fl, const float* theta, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    const Layout& Lx = c->Lx;
+    Workspace& w = b->ws;
+    int rc;
+    if ((rc = launch_obj_from_barrier(fl, w.objb, w.obj, s))) return rc;
+    if ((rc = launch_shade(c->m, Lx, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.box, w.img, w.max_area, s)))
+        return rc;
+    if ((rc = launch_blur(w.frames, fl, c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
+    if ((rc = launch_grads(w.frames, fl, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
+                           w.pcoef, w.objp, w.obj, w.max_area, s)))
+        return rc;
+    return eval_jacobian(b, fl, theta, 0, s, &Lx);
+}
+
+// Objective only (eval_objective's shading, blur and objective stages):
+static int eval_light_objective(tdi_batch* b, FrameList fl, const float* theta, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    int rc;
+    if ((rc = launch_obj_from_barrier(fl, w.objb, w.obj, s))) return rc;
+    if ((rc = launch_shade_keys(c->m, c->Lx, w.frames, fl, theta, w.P0, w.vproj, w.keys, w.box, w.img, s))) return rc;
+    if ((rc = launch_blur(w.frames, fl, c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
+    return launch_objective(w.frames, fl, w.keys, w.box, w.blurred, b->obs_g, c->opts.data_weight, w.objp, w.obj, s);
+}
+
+// The frames of `list` as the device frame list act (count, then slots).
+static int upload_list(Workspace& W, int* act, const std::vector<int>& list, hipStream_t s)
+{
+    W.hlist.assign(1, (int)list.size());
+    W.hlist.insert(W.hlist.end(), list.begin(), list.end());
+    TDI_HIP(hipMemcpyAsync(act, W.hlist.data(), sizeof(int) * W.hlist.size(), hipMemcpyHostToDevice, s));
+    return TDI_OK;
+}
+
 static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, float* Lint_host,
                      tdi_fit_report* reports, bool intensity)
 {
@@ -1244,103 +1331,128 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
     const int nd = intensity ? 1 : 3, j0 = intensity ? ol + 3 : ol;
     hipStream_t s = 0;
     Workspace& W = b->ws;
-    std::vector<float> x(Kx), xt(Kx), gh(Kx);
-    std::vector<double> Hh((size_t)Kx * Kx);
-    for (int i = 0; i < b->n; ++i) {
-        int rc = ws_load(b, i, 1, s);
-        if (rc) return rc;
-        tdi_fit_report& rep = reports[i];
-        std::memset(&rep, 0, sizeof(rep));
-        std::memcpy(x.data(), theta_host + (size_t)i * K, sizeof(float) * K);
-        for (int a = 0; a < 3; ++a) x[ol + a] = lam_host[3 * i + a];
-        x[ol + 3] = Lint_host[i];
-        bool terminate = false;
-        int outer = 0, ls = 0;
-        for (int it = 0; it < 10000; ++it) {
-            if (terminate) break;
-            outer++;
-            TDI_HIP(hipMemcpyAsync(W.theta, x.data(), sizeof(float) * Kx, hipMemcpyHostToDevice, s));
-            if ((rc = eval_full(b, all_frames(1), W.theta, 0, s, &Lx))) return rc;
-            double ob[2];
-            int np = 0;
-            TDI_HIP(hipMemcpyAsync(Hh.data(), W.H, sizeof(double) * Kx * Kx, hipMemcpyDeviceToHost, s));
-            TDI_HIP(hipMemcpyAsync(gh.data(), W.g, sizeof(float) * Kx, hipMemcpyDeviceToHost, s));
-            TDI_HIP(hipMemcpyAsync(&ob[0], W.obj, sizeof(double), hipMemcpyDeviceToHost, s));
-            TDI_HIP(hipMemcpyAsync(&ob[1], W.objb, sizeof(double), hipMemcpyDeviceToHost, s));
-            TDI_HIP(hipMemcpyAsync(&np, W.npix, sizeof(int), hipMemcpyDeviceToHost, s));
+    int rc = TDI_OK;
+    for (int i0 = 0; i0 < b->n; i0 += W.C) {
+        const int n = std::min(W.C, b->n - i0);
+        if ((rc = ws_load(b, i0, n, s))) return rc;
+        std::vector<float> x((size_t)n * Kx), xt;
+        std::vector<LightFrame> st(n);
+        for (int i = 0; i < n; ++i) {
+            tdi_fit_report& rep = reports[i0 + i];
+            std::memset(&rep, 0, sizeof(rep));
+            float* xi = &x[(size_t)i * Kx];
+            std::memcpy(xi, theta_host + (size_t)(i0 + i) * K, sizeof(float) * K);
+            for (int a = 0; a < 3; ++a) xi[ol + a] = lam_host[3 * (i0 + i) + a];
+            xi[ol + 3] = Lint_host[i0 + i];
+        }
+        std::vector<double> Hh, ob0(n), ob1(n);
+        std::vector<float> gh((size_t)n * Kx);
+        std::vector<int> np(n), list;
+        bool first = true;
+        for (;;) {
+            // frames starting an outer iteration (10000 at most)
+            list.clear();
+            for (int i = 0; i < n; ++i) {
+                if (st[i].phase == LightFrame::OUTER && st[i].outer >= 10000) st[i].phase = LightFrame::DONE;
+                if (st[i].phase == LightFrame::OUTER) list.push_back(i);
+            }
+            if (list.empty()) break;
+            TDI_HIP(hipMemcpyAsync(W.theta, x.data(), sizeof(float) * (size_t)n * Kx, hipMemcpyHostToDevice, s));
+            if ((rc = upload_list(W, W.act_e, list, s))) return rc;
+            const FrameList fl{W.act_e, n};
+            // the first evaluation renders every frame of the chunk; the light changes nothing else
+            if ((rc = first ? eval_full(b, fl, W.theta, 0, s, &Lx) : eval_light_full(b, fl, W.theta, s))) return rc;
+            first = false;
+            Hh.resize((size_t)n * Kx * Kx);
+            TDI_HIP(hipMemcpyAsync(Hh.data(), W.H, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(gh.data(), W.g, sizeof(float) * gh.size(), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(ob0.data(), W.obj, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(ob1.data(), W.objb, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(np.data(), W.npix, sizeof(int) * n, hipMemcpyDeviceToHost, s));
             TDI_HIP(hipStreamSynchronize(s));
-            rep.n_pixels = np;
-            const double obj0 = (ob[0] - ob[1]) / w;
-            double g[3] = {0, 0, 0}, d[3] = {0, 0, 0};
-            for (int a = 0; a < nd; ++a) g[a] = gh[j0 + a] / w;
-            if (intensity) {
-                d[0] = -g[0] / (Hh[(size_t)j0 * Kx + j0] / w);     // compute_Lintensity_search_dir
-            } else {
-                // 3x3 lower Cholesky in FP64 (Solver(3), solver.h:70-105): d = -H^-1 g
-                double A[9];
-                for (int a = 0; a < 3; ++a)
-                    for (int bb = 0; bb < 3; ++bb) A[a * 3 + bb] = Hh[(size_t)(j0 + a) * Kx + j0 + bb] / w;
-                bool pd = true;
-                for (int jj = 0; jj < 3 && pd; ++jj) {
-                    double dd = A[jj * 3 + jj];
-                    for (int k = 0; k < jj; ++k) dd -= A[jj * 3 + k] * A[jj * 3 + k];
-                    if (!(dd > 0)) { pd = false; break; }
-                    dd = std::sqrt(dd);
-                    A[jj * 3 + jj] = dd;
-                    for (int ii = jj + 1; ii < 3; ++ii) {
-                        double v = A[ii * 3 + jj];
-                        for (int k = 0; k < jj; ++k) v -= A[ii * 3 + k] * A[jj * 3 + k];
-                        A[ii * 3 + jj] = v / dd;
+            for (int i : list) {
+                LightFrame& f = st[i];
+                tdi_fit_report& rep = reports[i0 + i];
+                f.outer++;
+                rep.n_pixels = np[i];
+                f.obj0 = (ob0[i] - ob1[i]) / w;
+                const double* Hi = &Hh[(size_t)i * Kx * Kx];
+                double g[3] = {0, 0, 0};
+                f.d[0] = f.d[1] = f.d[2] = 0;
+                for (int a = 0; a < nd; ++a) g[a] = gh[(size_t)i * Kx + j0 + a] / w;
+                if (intensity) {
+                    f.d[0] = -g[0] / (Hi[(size_t)j0 * Kx + j0] / w);     // compute_Lintensity_search_dir
+                } else {
+                    double A[9];
+                    for (int a = 0; a < 3; ++a)
+                        for (int bb = 0; bb < 3; ++bb) A[a * 3 + bb] = Hi[(size_t)(j0 + a) * Kx + j0 + bb] / w;
+                    if (!solve3(A, g, f.d)) {
+                        rep.stop |= TDI_STOP_NOT_PD;
+                        f.phase = LightFrame::DONE;
+                        continue;
                     }
                 }
-                if (!pd) { rep.stop |= TDI_STOP_NOT_PD; break; }
-                double y[3];
-                for (int ii = 0; ii < 3; ++ii) {
-                    double v = -g[ii];
-                    for (int k = 0; k < ii; ++k) v -= A[ii * 3 + k] * y[k];
-                    y[ii] = v / A[ii * 3 + ii];
-                }
-                for (int ii = 2; ii >= 0; --ii) {
-                    double v = y[ii];
-                    for (int k = ii + 1; k < 3; ++k) v -= A[k * 3 + ii] * d[k];
-                    d[ii] = v / A[ii * 3 + ii];
-                }
+                f.dg = 0;
+                for (int a = 0; a < nd; ++a) f.dg += f.d[a] * g[a];
+                f.t = 1.0;
+                f.inner = 0;
+                f.phase = LightFrame::SEARCH;
             }
-            double dg = 0;
-            for (int a = 0; a < nd; ++a) dg += d[a] * g[a];
-            double t = 1.0;
-            int inner = 0;
-            while (inner < 1000) {
-                if (t < t_min) { terminate = true; rep.stop |= TDI_STOP_STEP; break; }
+            // the line searches, one objective evaluation per round for every frame still searching
+            for (;;) {
+                list.clear();
                 xt = x;
-                for (int a = 0; a < nd; ++a) xt[j0 + a] = (float)((double)x[j0 + a] + t * d[a]);
-                TDI_HIP(hipMemcpyAsync(W.theta_try, xt.data(), sizeof(float) * Kx, hipMemcpyHostToDevice, s));
-                if ((rc = eval_objective(b, all_frames(1), W.theta_try, 0, s, &Lx))) return rc;
-                TDI_HIP(hipMemcpyAsync(&ob[0], W.obj, sizeof(double), hipMemcpyDeviceToHost, s));
-                TDI_HIP(hipMemcpyAsync(&ob[1], W.objb, sizeof(double), hipMemcpyDeviceToHost, s));
-                TDI_HIP(hipStreamSynchronize(s));
-                inner++;
-                ls++;
-                if (-dg < dec_tol) { terminate = true; rep.stop |= TDI_STOP_DECREMENT; break; }
-                if ((ob[0] - ob[1]) / w < obj0 + alpha * t * dg) {
-                    x = xt;
-                    break;
+                for (int i = 0; i < n; ++i) {
+                    LightFrame& f = st[i];
+                    if (f.phase != LightFrame::SEARCH) continue;
+                    if (f.inner >= 1000) { f.phase = LightFrame::OUTER; continue; }
+                    if (f.t < t_min) {
+                        reports[i0 + i].stop |= TDI_STOP_STEP;
+                        f.phase = LightFrame::DONE;
+                        continue;
+                    }
+                    float* xi = &xt[(size_t)i * Kx];
+                    for (int a = 0; a < nd; ++a) xi[j0 + a] = (float)((double)x[(size_t)i * Kx + j0 + a] + f.t * f.d[a]);
+                    list.push_back(i);
                 }
-                t *= beta;
+                if (list.empty()) break;
+                TDI_HIP(hipMemcpyAsync(W.theta_try, xt.data(), sizeof(float) * xt.size(), hipMemcpyHostToDevice, s));
+                if ((rc = upload_list(W, W.act_e, list, s))) return rc;
+                if ((rc = eval_light_objective(b, FrameList{W.act_e, n}, W.theta_try, s))) return rc;
+                TDI_HIP(hipMemcpyAsync(ob0.data(), W.obj, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                TDI_HIP(hipMemcpyAsync(ob1.data(), W.objb, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                TDI_HIP(hipStreamSynchronize(s));
+                for (int i : list) {
+                    LightFrame& f = st[i];
+                    tdi_fit_report& rep = reports[i0 + i];
+                    f.inner++;
+                    rep.ls_evals++;
+                    if (-f.dg < dec_tol) {
+                        rep.stop |= TDI_STOP_DECREMENT;
+                        f.phase = LightFrame::DONE;
+                    } else if ((ob0[i] - ob1[i]) / w < f.obj0 + alpha * f.t * f.dg) {
+                        std::memcpy(&x[(size_t)i * Kx], &xt[(size_t)i * Kx], sizeof(float) * Kx);
+                        f.phase = LightFrame::OUTER;
+                    } else {
+                        f.t *= beta;
+                    }
+                }
             }
         }
-        TDI_HIP(hipMemcpyAsync(W.theta, x.data(), sizeof(float) * Kx, hipMemcpyHostToDevice, s));
-        if ((rc = eval_objective(b, all_frames(1), W.theta, 0, s, &Lx))) return rc;
-        double ob[2];
-        TDI_HIP(hipMemcpyAsync(&ob[0], W.obj, sizeof(double), hipMemcpyDeviceToHost, s));
-        TDI_HIP(hipMemcpyAsync(&ob[1], W.objb, sizeof(double), hipMemcpyDeviceToHost, s));
+        // the objective at the result, every frame of the chunk
+        TDI_HIP(hipMemcpyAsync(W.theta, x.data(), sizeof(float) * (size_t)n * Kx, hipMemcpyHostToDevice, s));
+        if ((rc = eval_light_objective(b, all_frames(n), W.theta, s))) return rc;
+        TDI_HIP(hipMemcpyAsync(ob0.data(), W.obj, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipMemcpyAsync(ob1.data(), W.objb, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         TDI_HIP(hipStreamSynchronize(s));
-        rep.objective = (float)((ob[0] - ob[1]) / w);
-        rep.iters = outer;
-        rep.ls_evals = ls;
-        rep.status = (rep.stop & TDI_STOP_NOT_PD) ? TDI_NOT_PD : TDI_OK;
-        for (int a = 0; a < 3; ++a) lam_host[3 * i + a] = x[ol + a];
-        Lint_host[i] = x[ol + 3];
+        for (int i = 0; i < n; ++i) {
+            tdi_fit_report& rep = reports[i0 + i];
+            rep.objective = (float)((ob0[i] - ob1[i]) / w);
+            rep.iters = st[i].outer;
+            rep.status = (rep.stop & TDI_STOP_NOT_PD) ? TDI_NOT_PD : TDI_OK;
+            for (int a = 0; a < 3; ++a) lam_host[3 * (i0 + i) + a] = x[(size_t)i * Kx + ol + a];
+            Lint_host[i0 + i] = x[(size_t)i * Kx + ol + 3];
+        }
     }
     return TDI_OK;
 }

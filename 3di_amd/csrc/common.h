@@ -325,6 +325,7 @@ int launch_shade_keys(const ModelGPU& m, const Layout& L, const FrameGPU* frames
                       const double* P0, const float4* vproj, const uint64_t* keys, const int4* box, float* img,
                       hipStream_t s);
 // objp: [frames][256] per-workgroup partials of the data objective (reduced in order into obj)
+int launch_obj_from_barrier(FrameList fl, const double* objb, double* obj, hipStream_t s);
 int launch_objective(const FrameGPU* frames, FrameList fl, const uint64_t* keys, const int4* box,
                      const float* blurred, const float2* obs_g, float w, double* objp, double* obj, hipStream_t s);
 // box == nullptr: whole crops; else only the region the normalised gradients need

@@ -492,6 +492,17 @@ __global__ void k_obj_reduce(const int* __restrict__ act, int n, int nparts, int
     obj[f] += s;
 }
 
+// obj[f] = objb[f] for the listed frames: the objective restarted from the barrier value that the
+// last barrier evaluation left (for evaluations that change only the light).
+__global__ void k_obj_from_barrier(const int* __restrict__ act, int n, const double* __restrict__ objb,
+                                   double* __restrict__ obj)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fl_count(act, n)) return;
+    const int f = fl_frame(act, i);
+    obj[f] = objb[f];
+}
+
 __global__ void __launch_bounds__(256) k_obs_grad(const FrameGPU* __restrict__ frames, int n, const float* __restrict__ blurred,
                                                    float2* __restrict__ obs_g)
 {
@@ -578,6 +589,13 @@ int launch_objective(const FrameGPU* frames, FrameList fl, const uint64_t* keys,
     hipLaunchKernelGGL(k_objective, grid, dim3(256), 0, s, frames, fl.act, fl.n,
                        (const unsigned long long*)keys, box, blurred, obs_g, w, objp);
     hipLaunchKernelGGL(k_obj_reduce, dim3((fl.n + 255) / 256), dim3(256), 0, s, fl.act, fl.n, 0, (int)grid.x, objp, obj);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_obj_from_barrier(FrameList fl, const double* objb, double* obj, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_obj_from_barrier, dim3((fl.n + 255) / 256), dim3(256), 0, s, fl.act, fl.n, objb, obj);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

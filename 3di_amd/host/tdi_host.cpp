@@ -770,15 +770,16 @@ int tdi_fit_video_frames_nres(tdi_ctx* ctx, const tdi_video_params* p, int32_t n
             for (int q = 0; q < 4; ++q) mcrop[(size_t)j * 4 + q] = crop[(size_t)i * 4 + q];
             pixels += (long long)crop[(size_t)i * 4 + 2] * crop[(size_t)i * 4 + 3];
         }
-        // frames of the ok list are not contiguous in bgr: prepare them one run at a time
+        // the ok list's frames, one call per run of consecutive frames in bgr
         float* dcrops = nullptr;
         if (hipMalloc((void**)&dcrops, sizeof(float) * (size_t)pixels) != hipSuccess) return TDI_ERR_NOMEM;
         long long off = 0;
-        for (int j = 0; j < m && !rc; ++j) {
-            const int i = ok_idx[j];
-            rc = tdi_prepare_crops(1, bgr + (size_t)i * frame_stride, src_w, src_h, frame_stride, &mcoef[j],
+        for (int j = 0; j < m && !rc;) {
+            int e = j + 1;
+            while (e < m && ok_idx[e] == ok_idx[e - 1] + 1) ++e;
+            rc = tdi_prepare_crops(e - j, bgr + (size_t)ok_idx[j] * frame_stride, src_w, src_h, frame_stride, &mcoef[j],
                                    &mcrop[(size_t)j * 4], dcrops + off, nullptr);
-            off += (long long)mcrop[(size_t)j * 4 + 2] * mcrop[(size_t)j * 4 + 3];
+            for (; j < e; ++j) off += (long long)mcrop[(size_t)j * 4 + 2] * mcrop[(size_t)j * 4 + 3];
         }
         std::vector<tdi_frame_desc> fd(m);
         off = 0;

@@ -96,3 +96,144 @@ def cfg2(n_frames: int = 4096, iters: int = 12, grid: int = 231, k_exp: int = 79
                     name=f"{'batched per-frame fit' if crop <= 0 else 'render loss and adjoint'}: {n_frames} frames, {grid}x{grid} grid ({m.n_vertices} V, {m.n_triangles} T), "
                          f"K={lay['K']} (6 pose + {k_exp} expression + {synth.light_unknowns(fit_light)} light), "
                          + (f"crops U{{96..384}}, {iters} GN iterations" if crop <= 0 else f"{crop}x{crop} crops"))
+
+
+@dataclasses.dataclass
+class LongVideo:
+    """A synthetic long video: per-frame truths, landmarks and cameras on the host; the frames are
+    rendered on the GPU chunk by chunk by `source` (8-bit BGR, width x height)."""
+    model: object
+    opts: object
+    lay: dict
+    theta: np.ndarray          # [n, K] truths
+    lmk_x: np.ndarray          # [n, 51] image pixels (1 px noise)
+    lmk_y: np.ndarray
+    cams: np.ndarray           # [n, 4]
+    width: int
+    height: int
+    render_ms: list            # time spent rendering frames, per source call
+    call_times: list = dataclasses.field(default_factory=list)   # host clock at each source call
+    source: object = None
+
+    @property
+    def n(self):
+        return self.theta.shape[0]
+
+
+def long_video(n_frames: int = 2048, width: int = 1920, height: int = 1080, face_px=(280.0, 420.0),
+               iters: int = 12, crop: int = 896, noise: float = 0.02, seed: int = 0x3D1 + 4) -> LongVideo:
+    """A face video: the pose follows a random walk (0.5 deg and 2 mm per frame per axis, angles
+    within +-30 deg), the expression an AR(1) process (rho = 0.95), and the steps are redrawn until
+    the 51 landmarks stay 20 px inside the frame with an extent within face_px; the light is fixed
+    at the reference's initial values.  The fit's unknowns: 6 pose + 79 expression (the per-frame
+    stage fits the light in its own sub-fits).  The frames are the render of the truth (crop x crop
+    around the face) over a grey N(0.5, noise) background, + N(0, noise), as 8-bit BGR."""
+    import torch
+    m = cfg2_model(231, 79)
+    lay = synth.layout(m, False, False, True, 0)
+    opts = abi.default_options(max_iters=iters, fit_light=0)
+    K = lay["K"]
+    rng = np.random.default_rng(seed)
+    ids = m.landmark_ids.astype(np.int64)
+    f = 1800.0
+    cx, cy = width / 2.0, height / 2.0
+    ne = lay["pose"] - lay["exp"]
+    ub = m.exp_ub[:ne].astype(np.float64)
+    margin = 20.0
+
+    P0 = m.mean_shape[ids].astype(np.float64)
+    E = m.exp_basis.reshape(-1, 3, m.exp_basis.shape[1])[ids].astype(np.float64)      # [51, 3, K_exp]
+    po = lay["pose"]
+
+    def landmarks(th):
+        P = P0 + E @ th[lay["exp"]:po]
+        u = th[po + 3:po + 6]
+        t = np.linalg.norm(u)
+        n = u / t
+        N = np.array([[0, -n[2], n[1]], [n[2], 0, -n[0]], [-n[1], n[0], 0]])
+        q = P @ (np.eye(3) + np.sin(t) * N + (1 - np.cos(t)) * N @ N).T + th[po:po + 3]
+        return np.stack([f * q[:, 0] / q[:, 2] + cx, f * q[:, 1] / q[:, 2] + cy], 1)
+
+    def inside(lm):
+        ext = max(np.ptp(lm[:, 0]), np.ptp(lm[:, 1]))
+        return (face_px[0] <= ext <= face_px[1] and lm[:, 0].min() > margin and lm[:, 1].min() > margin
+                and lm[:, 0].max() < width - margin and lm[:, 1].max() < height - margin)
+
+    theta = np.zeros((n_frames, K))
+    e = np.clip(rng.standard_normal(ne), -3, 3)
+    ang = np.zeros(3)
+    th = np.zeros(K)
+    th[lay["exp"]:lay["pose"]] = e * (ub / 3.5)
+    th[lay["pose"] + 3:lay["pose"] + 6] = synth.rotvec_from_euler(*ang)
+    # the depth that puts the first face in the middle of face_px
+    th[lay["pose"] + 2] = 1000.0
+    ext = max(np.ptp(landmarks(th), axis=0))
+    th[lay["pose"] + 2] = 1000.0 * ext / (0.5 * (face_px[0] + face_px[1]))
+    rho = 0.95
+    for i in range(n_frames):
+        if i:
+            # an expression and pose step that keeps the face inside the frame at the wanted
+            # size (redrawn otherwise; the frame repeats the previous one if no draw fits)
+            for _ in range(32):
+                e2 = rho * e + np.sqrt(1 - rho * rho) * rng.standard_normal(ne)
+                a2 = np.clip(ang + np.deg2rad(rng.normal(0, 0.5, 3)), -np.deg2rad(30), np.deg2rad(30))
+                cand = th.copy()
+                cand[lay["exp"]:po] = np.clip(e2, -3, 3) * (ub / 3.5)
+                cand[po:po + 3] += rng.normal(0, 2.0, 3)
+                cand[po + 3:po + 6] = synth.rotvec_from_euler(*a2)
+                if inside(landmarks(cand)):
+                    th, ang, e = cand, a2, e2
+                    break
+        theta[i] = th
+    lx = np.zeros((n_frames, 51), np.float32)
+    ly = np.zeros((n_frames, 51), np.float32)
+    for i in range(n_frames):
+        lm = landmarks(theta[i]) + rng.normal(0, 1.0, (51, 2))
+        lx[i], ly[i] = lm[:, 0], lm[:, 1]
+    cams = np.tile(np.array([[f, f, cx, cy]], np.float32), (n_frames, 1))
+    lv = LongVideo(model=m, opts=opts, lay=lay, theta=theta, lmk_x=lx, lmk_y=ly, cams=cams, width=width,
+                   height=height, render_ms=[])
+    state = {}
+    zero_obs = np.zeros(crop * crop, np.float32)      # the renderer ignores the observation
+    frng = np.random.default_rng(seed + 1)
+
+    def source(first: int, count: int):
+        t0 = time.perf_counter()
+        lv.call_times.append(t0)
+        if "ctx" not in state:
+            state["ctx"] = tdi.Context(m, opts)
+        frames = []
+        for i in range(first, first + count):
+            # a crop x crop window around the face, clipped to the frame
+            mx, my = float(lx[i].mean()), float(ly[i].mean())
+            x0 = int(np.clip(round(mx - crop / 2), 0, width - crop))
+            y0 = int(np.clip(round(my - crop / 2), 0, height - crop))
+            fr = synth.make_frame(m, synth.Truth(theta=theta[i], init=theta[i], lam=synth.LAMBDA0,
+                                                 Lint=synth.L0), lay, frng, f, crop, crop)
+            fr.cx, fr.cy, fr.x0, fr.y0 = cx, cy, x0, y0
+            fr.lmk_x, fr.lmk_y, fr.obs = lx[i], ly[i], zero_obs
+            frames.append(fr)
+        b = tdi.Batch(state["ctx"], frames)
+        th = torch.from_numpy(theta[first:first + count].astype(np.float32)).cuda()
+        img, _, _ = b.render(th)
+        b.close()
+        buf = state.get("buf")
+        if buf is None or buf.shape[0] < count:
+            buf = state["buf"] = torch.empty((count, height, width, 3), dtype=torch.uint8, device="cuda")
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed + first)
+        grey = (0.5 + noise * torch.randn((count, height, width), device="cuda", generator=gen))
+        face = img.view(count, crop, crop).transpose(1, 2)        # [count, y, x]
+        noisy = face + noise * torch.randn(face.shape, device="cuda", generator=gen)
+        for j, fr in enumerate(frames):
+            win = grey[j, fr.y0:fr.y0 + crop, fr.x0:fr.x0 + crop]
+            mask = face[j] != 0
+            win[mask] = noisy[j][mask]
+        out = buf[:count]
+        out.copy_((grey.clamp(0, 1) * 255.0 + 0.5).to(torch.uint8).unsqueeze(-1).expand(count, height, width, 3))
+        torch.cuda.synchronize()
+        lv.render_ms.append(1e3 * (time.perf_counter() - t0))
+        return out
+
+    lv.source = source
+    return lv

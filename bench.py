@@ -21,7 +21,10 @@ Also measured in the same run:
   * the frame preparation (tdi_prepare_crops): 8-bit BGR 1280x720 frames in HBM -> the
     workload's grey crops, against the oracle on a prefix (time and bit-exactness);
   * the render loss and its adjoint (tdi_render_loss_grad) on 256 frames of 256 x 256 crops:
-    forward, and forward + backward with the rasterisation recomputed.
+    forward, and forward + backward with the rasterisation recomputed;
+  * a long video (1024 frames of 1920x1080 BGR8, a random-walk pose and AR(1) expressions)
+    through the chunked per-frame stage (tdi_fit_video_long), frames rendered by the frame source
+    chunk by chunk; frames/s of the stage and the fitted poses against the truth.
 
 Usage: python bench.py [--steps K] [--warmup W] [--frames N] [--gpus 1]
 Prints one JSON line.
@@ -215,6 +218,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--diff-frames", type=int, default=256,
                     help="frames of the render-loss + adjoint measurement (256 x 256 crops); 0 skips it")
+    ap.add_argument("--video-frames", type=int, default=1024,
+                    help="frames of the long-video run (1920x1080 BGR8, tdi_fit_video_long); 0 skips it")
+    ap.add_argument("--video-chunk", type=int, default=512, help="frames per chunk of the long-video run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for scaling rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -447,6 +453,59 @@ def main():
         del gvd
         bd.close()
 
+    # a long video through the chunked per-frame stage (tdi_fit_video_long): 1920x1080 BGR8 frames
+    # rendered on the GPU chunk by chunk inside the frame source, NRES_COEFS = 4 reference face
+    # sizes, 256 x 256 crops, landmark fit with slack, light and shape sub-fits per size
+    long_video = None
+    if args.video_frames > 0:
+        import tempfile
+        from scipy.spatial.transform import Rotation
+        video = importlib.import_module("3di_amd.video")
+        lvw = workloads.long_video(n_frames=args.video_frames)
+        ctxv = tdi.Context(lvw.model, lvw.opts)
+        pv = video.video_params()
+        Kv = lvw.lay["K"]
+        with tempfile.TemporaryDirectory() as td:
+            vo = video.VideoOutput(Kv - 6, pv.nres_coefs)
+            th0 = np.zeros(Kv, np.float32)
+            th0[lvw.lay["pose"] + 2] = float(lvw.theta[0, lvw.lay["pose"] + 2])
+            th0[lvw.lay["pose"] + 3] = 1e-6
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, _, stv, donev = video.fit_video_long(
+                ctxv, pv, lvw.n, lvw.width, lvw.height, lvw.source, lvw.lmk_x, lvw.lmk_y, lvw.cams, th0,
+                np.asarray(workloads.synth.LAMBDA0, np.float32), vo, chunk_frames=args.video_chunk,
+                state_path=os.path.join(td, "video.state"))
+            torch.cuda.synchronize()
+            tv = time.perf_counter() - t0
+            gen_s = 1e-3 * float(np.sum(lvw.render_ms))
+            vo.save_poses(os.path.join(td, "poses.txt"), with_euler=False)
+            fitted = np.loadtxt(os.path.join(td, "poses.txt"), ndmin=2)
+            vo.close()
+        po = lvw.lay["pose"]
+        log(f"[bench] long video: {lvw.n} frames in {tv:.1f} s ({gen_s:.1f} s generating the frames)")
+        # the pose file stops before the last frame (save_poses writes frames 0 .. max id - 1)
+        ok = np.flatnonzero(stv[:fitted.shape[0]] == 0)
+        ang = np.array([np.rad2deg(np.linalg.norm(
+            (Rotation.from_rotvec(fitted[i, 3:6]).inv()
+             * Rotation.from_rotvec(lvw.theta[i, po + 3:po + 6])).as_rotvec())) for i in ok])
+        dz = np.abs(fitted[ok, 2] - lvw.theta[ok, po + 2])
+        long_video = dict(frames=lvw.n, frames_done=donev, frame_size=f"{lvw.width}x{lvw.height} BGR8",
+                          chunk_frames=args.video_chunk, nres_coefs=int(pv.nres_coefs),
+                          crop=f"{pv.crop_w}x{pv.crop_h}", gn_iterations=int(lvw.opts.max_iters),
+                          seconds_total=round(tv, 2), seconds_generating_frames=round(gen_s, 2),
+                          frames_per_s=round(lvw.n / (tv - gen_s), 2),
+                          frames_ok=int(ok.size),
+                          pose_angle_error_deg=dict(median=round(float(np.median(ang)), 3),
+                                                    p95=round(float(np.percentile(ang, 95)), 3)),
+                          depth_error_mm=dict(median=round(float(np.median(dz)), 2),
+                                              p95=round(float(np.percentile(dz, 95)), 2)),
+                          note="frames/s excludes the time the frame source spends rendering the frames; "
+                               "fitted poses are the VideoOutput's per-frame averages over the NRES_COEFS sizes")
+        log(f"[bench] long video: {long_video['frames_per_s']} frames/s, "
+            f"median pose error {long_video['pose_angle_error_deg']['median']} deg")
+        ctxv.close()
+
     # FP64 Cholesky solve of one joint identity-stage system (K = 199 + 199 + 10 x 94 = 1338)
     rngc = np.random.default_rng(1338)
     Ac = rngc.normal(size=(1338, 1338)) / np.sqrt(1338)
@@ -540,6 +599,7 @@ def main():
         "autotune": autotune,
         "differentiation": differentiation,
         "large_k_cholesky": large_k_cholesky,
+        "long_video": long_video,
         "frame_prep": frame_prep,
     }
     print(json.dumps(out), flush=True)
