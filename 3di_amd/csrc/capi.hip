@@ -83,6 +83,7 @@ struct tdi_ctx {
     Layout Lx{};       // L with the light appended when it is not an unknown (light sub-fits)
     ModelGPU m{};
     ModelGPU m_fine{};  // m with the finetune fits' expression bounds (finetune_coef times wider)
+    ModelGPU m_joint{}; // m without the albedo and identity bounds (the joint fit adds them once)
     float taps[15];
     Alloc mem;
 };
@@ -143,6 +144,13 @@ struct tdi_batch {
     hipGraphExec_t graph = nullptr;
     hipGraphExec_t graph_fine = nullptr;   // the finetune fit's schedule
     int finetune = 0;                      // set while a finetune fit runs: bounds and pose hold
+    int joint = 0;                         // set while the identity stage's joint fit evaluates
+    // scratch of the normal equations above 111 unknowns (allocated on first use): the Jacobian
+    // rows U, the J^T J partials and the barrier's landmark-derivative rows
+    float* big_U = nullptr;
+    float* big_partials = nullptr;
+    double* big_D = nullptr;
+    int big_S = 0;
     hipStream_t stream = nullptr;     // capture-capable stream owned by the batch (default priority)
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
@@ -264,6 +272,12 @@ int tdi_ctx_create(const tdi_model_desc* md, const tdi_fit_options* opts, int de
     c->m_fine = m;
     c->m_fine.lb = (const float*)up(lb.data(), sizeof(float) * c->Lx.K);
     c->m_fine.ub = (const float*)up(ub.data(), sizeof(float) * c->Lx.K);
+    // the identity stage's frames: regular expression bounds, no albedo or identity bounds
+    for (int k = 0; k < L.nexp; ++k) { lb[L.off_exp + k] = md->exp_lb[k]; ub[L.off_exp + k] = md->exp_ub[k]; }
+    for (int k = 0; k < L.ntex + L.nid; ++k) { lb[k] = -INFINITY; ub[k] = INFINITY; }
+    c->m_joint = m;
+    c->m_joint.lb = (const float*)up(lb.data(), sizeof(float) * c->Lx.K);
+    c->m_joint.ub = (const float*)up(ub.data(), sizeof(float) * c->Lx.K);
     for (void* p : c->mem.ptrs)
         if (!p) { c->mem.release(); delete c; set_error("hipMalloc failed for the model"); return TDI_ERR_NOMEM; }
     if (hipDeviceSynchronize() != hipSuccess) { c->mem.release(); delete c; set_error("upload failed"); return TDI_ERR_HIP; }
@@ -549,6 +563,9 @@ extern "C" int tdi_batch_destroy(tdi_batch* b)
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->ev_in) (void)hipEventDestroy(b->ev_in);
     if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+    (void)hipFree(b->big_U);
+    (void)hipFree(b->big_partials);
+    (void)hipFree(b->big_D);
     b->mem.release();
     delete b;
     return TDI_OK;
@@ -556,8 +573,41 @@ extern "C" int tdi_batch_destroy(tdi_batch* b)
 
 // ---------------------------------------------------------------------------- evaluation
 // The model whose coefficient bounds the barrier uses: the finetune fits' widened expression
-// bounds while a finetune fit runs (model_fitter.cu:409-410), else the regular ones.
-static const ModelGPU& bounds_model(const tdi_batch* b) { return b->finetune ? b->ctx->m_fine : b->ctx->m; }
+// bounds while a finetune fit runs (model_fitter.cu:409-410), the bounds without the shared
+// albedo and identity blocks while the joint fit evaluates its frames, else the regular ones.
+static const ModelGPU& bounds_model(const tdi_batch* b)
+{
+    return b->joint ? b->ctx->m_joint : b->finetune ? b->ctx->m_fine : b->ctx->m;
+}
+
+constexpr int SMALL_JTJ_MAXK = 111;   // the LDS-staged J^T J kernels and k_barrier's LDS rows
+
+// Scratch of the normal equations above SMALL_JTJ_MAXK unknowns, allocated once per batch.
+static int big_scratch(tdi_batch* b)
+{
+    if (b->big_U) return TDI_OK;
+    const Workspace& w = b->ws;
+    const int K = b->ctx->L.K;
+    int S = w.max_area / 512 + 1;    // pixel splits per frame: about 512 pixels of a full crop or more each
+    if (S > 16) S = 16;
+    const size_t nu = (size_t)w.cap * jtj_big_row_stride(K), np = jtj_big_partials_floats(w.C, S, K);
+    const size_t nd = (size_t)w.C * 2 * NLMK * K;
+    if (hipMalloc((void**)&b->big_U, sizeof(float) * nu) != hipSuccess ||
+        hipMalloc((void**)&b->big_partials, sizeof(float) * np) != hipSuccess ||
+        hipMalloc((void**)&b->big_D, sizeof(double) * nd) != hipSuccess) {
+        (void)hipFree(b->big_U);
+        (void)hipFree(b->big_partials);
+        (void)hipFree(b->big_D);
+        b->big_U = nullptr;
+        b->big_partials = nullptr;
+        b->big_D = nullptr;
+        set_error("normal equations at K=%d: %zu bytes of scratch not available", K,
+                  sizeof(float) * (nu + np) + sizeof(double) * nd);
+        return TDI_ERR_NOMEM;
+    }
+    b->big_S = S;
+    return TDI_OK;
+}
 
 // Objective at theta for the frames in `mask`: synthesis, face check, raster, resolve,
 // barrier objective, shading, blur, gradients (Renderer::render + evaluate_objective_function).
@@ -664,6 +714,16 @@ static int eval_jacobian(tdi_batch* b, FrameList fl, const float* theta, int mix
     if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
                         dI_area(w), s)))
         return rc;
+    if (L.K > SMALL_JTJ_MAXK) {
+        // hundreds of unknowns (the identity stage): J^T J on MFMA over 64 x 64 tiles (BF16
+        // hi/mid/lo, FP32 accumulation, FP64 sums) and the barrier's rows in global scratch
+        if ((rc = big_scratch(b))) return rc;
+        if ((rc = launch_jtj_big(L, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
+                                 b->big_U, b->big_partials, b->big_S, w.max_area, s)))
+            return rc;
+        return launch_barrier_big(bounds_model(b), L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks,
+                                  b->big_D, w.H, w.g, s);
+    }
     if ((rc = launch_normal_eq(b, fl, mixed, s, &L))) return rc;
     return launch_barrier(bounds_model(b), L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 1, w.obj,
                           w.objb, w.H, w.g, s);
@@ -834,6 +894,228 @@ int tdi_normal_equations(tdi_batch* b, const float* theta, float* H, float* g, f
 int tdi_normal_equations_mixed(tdi_batch* b, const float* theta, float* H, float* g, float* obj, void* stream)
 {
     return normal_eq(b, theta, H, g, obj, stream, true);
+}
+
+// ---------------------------------------------------------------------------- identity stage
+// The joint fit of frames that share the albedo and identity coefficients (fit_to_multi_images'
+// shape fit, fit_3DMM_shape_rigid over T frames, model_fitter.cu:121-375, with the frames'
+// products summed as in get_obj_hessian_and_gradient_multiframe, model_fitter.cu:21-116).
+// Per-frame layout [albedo | identity | expression | pose | light]; the shared block is the
+// first Ks = ntex + nid unknowns; joint vector [shared | frame 0's rest | frame 1's rest | ...].
+static void joint_sizes(const tdi_batch* b, int& Ks, long long& KJ)
+{
+    const Layout& L = b->ctx->L;
+    Ks = L.ntex + L.nid;
+    KJ = Ks + (long long)b->n * (L.K - Ks);
+}
+
+// Every frame evaluated at its theta (device [n][K]) without the shared bounds, then the joint
+// matrix HJ and gradient gJ (device FP64) and the shared bounds' objective (*objs, device).
+static int joint_eval(tdi_batch* b, const float* dtheta, double* HJ, double* gJ, double* objs, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    Workspace& w = b->ws;
+    int Ks;
+    long long KJ;
+    joint_sizes(b, Ks, KJ);
+    b->joint = 1;
+    int rc = eval_full(b, all_frames(b->n), dtheta, 0, s);
+    b->joint = 0;
+    if (rc) return rc;
+    if ((rc = launch_assemble_joint(b->n, c->L.K, Ks, w.H, w.g, HJ, gJ, s))) return rc;
+    if ((rc = launch_shared_bounds(Ks, KJ, dtheta, c->m.lb, c->m.ub, 1, nullptr, HJ, gJ, s))) return rc;
+    return launch_shared_bounds(Ks, KJ, dtheta, c->m.lb, c->m.ub, 0, objs, nullptr, nullptr, s);
+}
+
+static int joint_objective(tdi_batch* b, const float* dtheta, double* objs, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    int Ks;
+    long long KJ;
+    joint_sizes(b, Ks, KJ);
+    b->joint = 1;
+    int rc = eval_objective(b, all_frames(b->n), dtheta, 0, s);
+    b->joint = 0;
+    if (rc) return rc;
+    return launch_shared_bounds(Ks, KJ, dtheta, c->m.lb, c->m.ub, 0, objs, nullptr, nullptr, s);
+}
+
+static int joint_check(tdi_batch* b, const char* who)
+{
+    if (b->n > b->ws.C) {
+        set_error("%s: %d frames do not fit one workspace chunk (%d)", who, b->n, b->ws.C);
+        return TDI_ERR_ARG;
+    }
+    return TDI_OK;
+}
+
+// Joint unknowns of the batch's frames (Ks + n (K - Ks)).
+int64_t tdi_joint_unknowns(tdi_batch* b)
+{
+    if (!b) return 0;
+    int Ks;
+    long long KJ;
+    joint_sizes(b, Ks, KJ);
+    return KJ;
+}
+
+int tdi_normal_equations_joint(tdi_batch* b, const float* theta, double* HJ, double* gJ, double* obj, void* stream)
+{
+    if (!b || !theta || !HJ || !gJ) { set_error("tdi_normal_equations_joint: null argument"); return TDI_ERR_ARG; }
+    int rc = joint_check(b, "tdi_normal_equations_joint");
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    Workspace& w = b->ws;
+    if ((rc = ws_load(b, 0, b->n, s))) return rc;
+    double* objs = nullptr;
+    if (hipMalloc((void**)&objs, sizeof(double)) != hipSuccess) { set_error("joint: allocation failed"); return TDI_ERR_NOMEM; }
+    rc = joint_eval(b, theta, HJ, gJ, objs, s);
+    if (!rc && obj) {
+        std::vector<double> h(b->n + 1);
+        if (hipMemcpyAsync(h.data(), w.obj, sizeof(double) * b->n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(h.data() + b->n, objs, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = TDI_ERR_HIP;
+        double sum = 0;
+        for (double v : h) sum += v;
+        if (!rc && (hipMemcpyAsync(obj, &sum, sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess))
+            rc = TDI_ERR_HIP;
+        if (rc) set_error("tdi_normal_equations_joint: copy failed");
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(objs);
+    return rc;
+}
+
+int tdi_fit_multiframe(tdi_batch* b, float* theta_host, tdi_fit_report* report)
+{
+    if (!b || !theta_host || !report) { set_error("tdi_fit_multiframe: null argument"); return TDI_ERR_ARG; }
+    int rc = joint_check(b, "tdi_fit_multiframe");
+    if (rc) return rc;
+    tdi_ctx* c = b->ctx;
+    const tdi_fit_options& o = c->opts;
+    const int n = b->n, K = c->L.K, off_tz = c->L.off_pose + 2;
+    int Ks;
+    long long KJl;
+    joint_sizes(b, Ks, KJl);
+    const int KJ = (int)KJl, Kf = K - Ks;
+    if (KJl > CHOL_BIG_MAXK) { set_error("tdi_fit_multiframe: %lld joint unknowns > %d", KJl, CHOL_BIG_MAXK); return TDI_ERR_ARG; }
+    hipStream_t s = 0;
+    Workspace& w = b->ws;
+    // device scratch of the call, released on every return
+    Alloc A;
+    struct Release {
+        Alloc& a;
+        hipStream_t s;
+        ~Release() { (void)hipStreamSynchronize(s); a.release(); }
+    } release{A, s};
+    float* dth = A.get<float>((size_t)n * K);
+    double* HJ = A.get<double>((size_t)KJ * KJ);
+    double* gJ = A.get<double>(KJ);
+    double* y = A.get<double>(KJ);
+    float* dJ = A.get<float>(KJ);
+    int* st = A.get<int>(1);
+    double* objs = A.get<double>(1);
+    if (!dth || !HJ || !gJ || !y || !dJ || !st || !objs) { set_error("tdi_fit_multiframe: allocation failed"); return TDI_ERR_NOMEM; }
+    auto fail = [](int e) { return e; };
+    if ((rc = ws_load(b, 0, n, s))) return fail(rc);
+    // joint vector from the frames' thetas (shared block from frame 0)
+    std::vector<float> x(KJ), xt(KJ), th((size_t)n * K), delta(KJ);
+    std::vector<double> g(KJ), ob(n), obb(n);
+    std::vector<int> fok(n), np(n);
+    for (int k = 0; k < Ks; ++k) x[k] = theta_host[k];
+    for (int t = 0; t < n; ++t)
+        for (int k = 0; k < Kf; ++k) x[Ks + (size_t)t * Kf + k] = theta_host[(size_t)t * K + Ks + k];
+    auto expand = [&](const std::vector<float>& v) {
+        for (int t = 0; t < n; ++t) {
+            for (int k = 0; k < Ks; ++k) th[(size_t)t * K + k] = v[k];
+            for (int k = 0; k < Kf; ++k) th[(size_t)t * K + Ks + k] = v[Ks + (size_t)t * Kf + k];
+        }
+    };
+    auto upload = [&](const std::vector<float>& v) -> int {
+        expand(v);
+        TDI_HIP(hipMemcpyAsync(dth, th.data(), sizeof(float) * th.size(), hipMemcpyHostToDevice, s));
+        TDI_HIP(hipStreamSynchronize(s));
+        return TDI_OK;
+    };
+    // the reference's step threshold: 0.075 for one frame, 0.001 for several (model_fitter.cu:149-151)
+    const float t_min = n > 1 ? 0.001f : o.t_min;
+    tdi_fit_report& rep = *report;
+    std::memset(&rep, 0, sizeof(rep));
+    int outer = 0, performed = 0, ls_evals = 0;
+    bool terminate = false;
+    for (int it = 0; it < o.max_iters; ++it) {
+        outer++;
+        if (terminate) break;
+        if ((rc = upload(x))) return fail(rc);
+        if ((rc = joint_eval(b, dth, HJ, gJ, objs, s))) return fail(rc);
+        double os = 0;
+        TDI_HIP(hipMemcpyAsync(g.data(), gJ, sizeof(double) * KJ, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipMemcpyAsync(ob.data(), w.obj, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipMemcpyAsync(np.data(), w.npix, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipMemcpyAsync(&os, objs, sizeof(double), hipMemcpyDeviceToHost, s));
+        if ((rc = launch_cholesky_big_f64(1, KJ, HJ, gJ, y, dJ, st, s))) return fail(rc);
+        int sst = 0;
+        TDI_HIP(hipMemcpyAsync(&sst, st, sizeof(int), hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipMemcpyAsync(delta.data(), dJ, sizeof(float) * KJ, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipStreamSynchronize(s));
+        performed++;
+        rep.n_pixels = 0;
+        for (int t = 0; t < n; ++t) rep.n_pixels += np[t];
+        if (sst != TDI_OK) { rep.stop |= TDI_STOP_NOT_PD; break; }
+        if (std::isnan(delta[0])) { rep.stop |= TDI_STOP_NAN; break; }
+        double obj0 = os, dg = 0;
+        for (int t = 0; t < n; ++t) obj0 += ob[t];
+        for (int k = 0; k < KJ; ++k) dg += (double)delta[k] * g[k];
+        float tc = 1.f;
+        int inner = 0;
+        while (inner < o.max_ls) {
+            if (tc < t_min) { terminate = true; rep.stop |= TDI_STOP_STEP; break; }
+            for (int k = 0; k < KJ; ++k) xt[k] = x[k] + tc * delta[k];
+            // the reference checks the depth of the frame its variables point at (frame 0)
+            if (xt[off_tz] < 0.f) break;
+            if ((rc = upload(xt))) return fail(rc);
+            if ((rc = joint_objective(b, dth, objs, s))) return fail(rc);
+            double ost = 0;
+            TDI_HIP(hipMemcpyAsync(ob.data(), w.obj, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(obb.data(), w.objb, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(fok.data(), w.face_ok, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(&ost, objs, sizeof(double), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipStreamSynchronize(s));
+            bool skip = std::isnan(ost);
+            double objt = ost;
+            for (int t = 0; t < n; ++t) {
+                if (!fok[t] || std::isnan(obb[t])) skip = true;
+                objt += ob[t];
+            }
+            if (skip) { tc *= o.armijo_beta; continue; }
+            inner++;
+            ls_evals++;
+            if (-dg < o.decrement_tol) { terminate = true; rep.stop |= TDI_STOP_DECREMENT; break; }
+            if (objt < obj0 + (double)(o.armijo_alpha * tc) * dg) {
+                x = xt;
+                break;
+            }
+            tc *= o.armijo_beta;
+        }
+    }
+    if (!terminate && !(rep.stop & (TDI_STOP_NAN | TDI_STOP_NOT_PD))) rep.stop |= TDI_STOP_MAXITER;
+    if ((rc = upload(x))) return fail(rc);
+    if ((rc = joint_objective(b, dth, objs, s))) return fail(rc);
+    double fin = 0;
+    TDI_HIP(hipMemcpyAsync(ob.data(), w.obj, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    TDI_HIP(hipMemcpyAsync(&fin, objs, sizeof(double), hipMemcpyDeviceToHost, s));
+    TDI_HIP(hipStreamSynchronize(s));
+    for (int t = 0; t < n; ++t) fin += ob[t];
+    rep.objective = (float)fin;
+    rep.iters = performed;
+    rep.ls_evals = ls_evals;
+    rep.success = outer > o.niters_success;
+    rep.status = (rep.stop & TDI_STOP_NOT_PD) ? TDI_NOT_PD : (rep.stop & TDI_STOP_NAN) ? TDI_NAN : TDI_OK;
+    expand(x);
+    std::memcpy(theta_host, th.data(), sizeof(float) * th.size());
+    return TDI_OK;
 }
 
 int tdi_cholesky_solve(int32_t n, int32_t K, const float* H, const float* g, float* delta, int32_t* status,

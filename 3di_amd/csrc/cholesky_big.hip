@@ -134,7 +134,8 @@ __global__ void __launch_bounds__(256) k_chol_update(int K, int k0, double* __re
     }
 }
 
-__global__ void __launch_bounds__(256) k_chol_trsv(int K, const double* __restrict__ A, const float* __restrict__ g,
+template <typename TG>
+__global__ void __launch_bounds__(256) k_chol_trsv(int K, const double* __restrict__ A, const TG* __restrict__ g,
                                                     double* __restrict__ y, float* __restrict__ delta,
                                                     const int* __restrict__ status)
 {
@@ -166,6 +167,20 @@ __global__ void __launch_bounds__(256) k_chol_trsv(int K, const double* __restri
 
 } // namespace
 
+// Factorisation and solve on the FP64 matrices A (overwritten by the factor); g FP32 or FP64.
+template <typename TG>
+static int chol_big_run(int n, int K, double* A, const TG* g, double* y, float* delta, int* status, hipStream_t s)
+{
+    for (int k0 = 0; k0 < K; k0 += CB) {
+        hipLaunchKernelGGL(k_chol_panel, dim3(n), dim3(256), 0, s, K, k0, A, status);
+        const int T = (K - k0 - CB + CB - 1) / CB;
+        if (T > 0) hipLaunchKernelGGL(k_chol_update, dim3(T * (T + 1) / 2, n), dim3(256), 0, s, K, k0, A, status);
+        if (hipGetLastError() != hipSuccess) return TDI_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_chol_trsv<TG>, dim3(n), dim3(256), 0, s, K, A, g, y, delta, status);
+    return hipGetLastError() != hipSuccess ? TDI_ERR_HIP : TDI_OK;
+}
+
 int launch_cholesky_big(int n, int K, const float* H, const float* g, float* delta, int* status, hipStream_t s)
 {
     if (K > CHOL_BIG_MAXK) {
@@ -187,22 +202,29 @@ int launch_cholesky_big(int n, int K, const float* H, const float* g, float* del
     if (!rc) {
         hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)std::min<size_t>((nkk + 255) / 256, 65536)), dim3(256), 0, s, nkk,
                            H, A);
-        for (int k0 = 0; k0 < K && !rc; k0 += CB) {
-            hipLaunchKernelGGL(k_chol_panel, dim3(n), dim3(256), 0, s, K, k0, A, status);
-            const int T = (K - k0 - CB + CB - 1) / CB;
-            if (T > 0) hipLaunchKernelGGL(k_chol_update, dim3(T * (T + 1) / 2, n), dim3(256), 0, s, K, k0, A, status);
-            if (hipGetLastError() != hipSuccess) rc = TDI_ERR_HIP;
-        }
-        if (!rc) {
-            hipLaunchKernelGGL(k_chol_trsv, dim3(n), dim3(256), 0, s, K, A, g, y, delta, status);
-            if (hipGetLastError() != hipSuccess) rc = TDI_ERR_HIP;
-        }
+        rc = chol_big_run(n, K, A, g, y, delta, status, s);
     }
     const hipError_t e = hipStreamSynchronize(s);
     if (!rc && e != hipSuccess) rc = TDI_ERR_HIP;
     if (rc) set_error("cholesky (large K): %s", hipGetErrorString(e));
     (void)hipFree(A);
     (void)hipFree(y);
+    return rc;
+}
+
+int launch_cholesky_big_f64(int n, int K, double* A, const double* g, double* y, float* delta, int* status,
+                            hipStream_t s)
+{
+    if (K > CHOL_BIG_MAXK) {
+        set_error("cholesky: K=%d > %d", K, CHOL_BIG_MAXK);
+        return TDI_ERR_ARG;
+    }
+    std::vector<int> ok(n, TDI_OK);
+    TDI_HIP(hipMemcpyAsync(status, ok.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    const int rc = chol_big_run(n, K, A, g, y, delta, status, s);
+    // the status upload reads the host vector: it must finish before the vector goes
+    TDI_HIP(hipStreamSynchronize(s));
+    if (rc) set_error("cholesky (large K): launch failed");
     return rc;
 }
 

@@ -347,6 +347,13 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameL
               float* dI, int mean_area, hipStream_t s);
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);
+// J^T J and J^T r for frames with more columns than one workgroup's LDS holds (the identity
+// stage): U [crop pixels][jtj_big_row_stride(K)] and partials [jtj_big_partials_floats] scratch.
+size_t jtj_big_partials_floats(int n, int S, int K);
+int jtj_big_row_stride(int K);
+int launch_jtj_big(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
+                   const float* pcoef, const float4* pixg, const float* dI, float w, double* H, float* g, float* U,
+                   float* partials, int S, int max_area, hipStream_t s);
 int launch_jtj_f32(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
                    const float* pcoef, const float4* pixg, const float* dI, float w, double* H, float* g,
                    float* partials, int S, hipStream_t s);
@@ -358,9 +365,19 @@ int launch_jtj_mfma(const Layout& L, const FrameGPU* frames, FrameList fl, const
 int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                    const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
                    double* H, float* g, hipStream_t s);
+// The barrier's derivative terms (mode 1) at any K: D is [n][2][NLMK][K] doubles of scratch.
+int launch_barrier_big(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
+                       const LmkArrays& lm, int use_landmarks, double* D, double* H, float* g, hipStream_t s);
 // cholesky_big.hip: FP64 blocked Cholesky solve for K above what one workgroup holds in LDS
 constexpr int CHOL_BIG_MAXK = 4096;
 int launch_cholesky_big(int n, int K, const float* H, const float* g, float* delta, int* status, hipStream_t s);
+// The same on FP64 matrices A [n][K][K] factored in place, g FP64 [n][K], y [n][K] scratch.
+int launch_cholesky_big_f64(int n, int K, double* A, const double* g, double* y, float* delta, int* status,
+                            hipStream_t s);
+// identity.hip: the identity stage's joint system from the frames' normal equations.
+int launch_assemble_joint(int n, int K, int Ks, const double* H, const float* g, double* HJ, double* gJ, hipStream_t s);
+int launch_shared_bounds(int Ks, long long KJ, const float* x, const float* lb, const float* ub, int mode, double* obj,
+                         double* HJ, double* gJ, hipStream_t s);
 
 // adjoint.hip: the render loss and its adjoint (scratch: device buffers of the caller)
 struct AdjScratch {

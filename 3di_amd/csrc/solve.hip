@@ -521,14 +521,173 @@ __global__ void __launch_bounds__(CMP_THREADS) k_compact_frames(int n, const int
     if (tid == 0) act[0] = running;
 }
 
+// The barrier's derivative terms (k_barrier mode 1) for frames with more unknowns than the
+// landmark-derivative rows fit in LDS (the identity stage): the rows go to global scratch D
+// [frame][2][NLMK][K] and the Gauss-Newton sum is formed over 32 x 32 tiles of H by a grid.
+// Same formulas, FP64 throughout.
+__global__ void __launch_bounds__(256) k_barrier_rows(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+                                                       const int* __restrict__ act, int n, const float* __restrict__ theta,
+                                                       LmkArrays lm, int use_lmk, double* __restrict__ D,
+                                                       double* __restrict__ H, float* __restrict__ g)
+{
+    if ((int)blockIdx.x >= fl_count(act, n)) return;
+    const int f = fl_frame(act, blockIdx.x);
+    const int K = L.K;
+    const FrameGPU fr = frames[f];
+    const float* tf = theta + (size_t)f * K;
+    double* sdx = D + (size_t)f * 2 * NLMK * K;
+    double* sdy = sdx + (size_t)NLMK * K;
+    __shared__ double sR[9], sdR[27];
+    __shared__ double sv[NLMK][3], sp[NLMK][3];
+    __shared__ double swx[NLMK], swy[NLMK], sgx[NLMK], sgy[NLMK];
+    const int tid = threadIdx.x;
+    if (tid == 0) rodrigues_f64_d(tf + L.off_pose + 3, sR, sdR);
+    __syncthreads();
+    for (int j = tid; j < K; j += blockDim.x) {
+        const float lbf = m.lb[j], ubf = m.ub[j];
+        if (!(ubf < INFINITY) || !(lbf > -INFINITY)) continue;
+        const double x = tf[j], lb = lbf, ub = ubf;
+        const double fub = x - ub, flb = lb - x;
+        g[(size_t)f * K + j] += (float)(-1.0 / fub + 1.0 / flb);
+        const double a = -1.0 / (fub + 1e-12), b = 1.0 / (flb + 1e-12);
+        H[(size_t)f * K * K + (size_t)j * K + j] += a * a + b * b;
+    }
+    if (!use_lmk) return;
+    if (tid < NLMK) {
+        const int i = tid, v = m.lmk_ids[i];
+        double p[3];
+        for (int a = 0; a < 3; ++a) {
+            const int r = 3 * v + a;
+            double sacc = m.mean[r];
+            for (int k = 0; k < L.nexp; ++k) sacc = fma((double)m.exp_basis[(size_t)r * m.exp_ld + k], (double)tf[L.off_exp + k], sacc);
+            for (int k = 0; k < L.nid; ++k) sacc = fma((double)m.id_basis[(size_t)r * m.id_ld + k], (double)tf[L.off_id + k], sacc);
+            p[a] = sacc;
+            sp[i][a] = sacc;
+        }
+        const double vx = sR[0] * p[0] + sR[3] * p[1] + sR[6] * p[2] + tf[L.off_pose + 0];
+        const double vy = sR[1] * p[0] + sR[4] * p[1] + sR[7] * p[2] + tf[L.off_pose + 1];
+        const double vz = sR[2] * p[0] + sR[5] * p[1] + sR[8] * p[2] + tf[L.off_pose + 2];
+        sv[i][0] = vx; sv[i][1] = vy; sv[i][2] = vz;
+        const double xproj = fr.phix * vx / vz + fr.cx;
+        const double yproj = fr.phiy * vy / vz + fr.cy;
+        const float* L6 = lm.base + (size_t)f * 6 * NLMK;
+        const double lx = L6[i], ly = L6[NLMK + i];
+        const double blx = L6[2 * NLMK + i], bux = L6[3 * NLMK + i];
+        const double bly = L6[4 * NLMK + i], buy = L6[5 * NLMK + i];
+        const double fs = fr.face_size;
+        const double Gxm = xproj - (lx + blx * fs);
+        const double Gym = yproj - (ly + bly * fs);
+        const double Gxp = (lx + bux * fs) - xproj;
+        const double Gyp = (ly + buy * fs) - yproj;
+        swx[i] = 1.0 / (Gxm * Gxm) + 1.0 / (Gxp * Gxp);
+        swy[i] = 1.0 / (Gym * Gym) + 1.0 / (Gyp * Gyp);
+        sgx[i] = -1.0 / Gxm + 1.0 / Gxp;
+        sgy[i] = -1.0 / Gym + 1.0 / Gyp;
+    }
+    __syncthreads();
+    for (int e = tid; e < NLMK * K; e += blockDim.x) {
+        const int i = e / K, j = e - i * K;
+        double dv[3] = {0.0, 0.0, 0.0};
+        bool geo = true;
+        if (j >= L.off_pose && j < L.off_pose + 3) {
+            dv[j - L.off_pose] = 1.0;
+        } else if (j >= L.off_pose + 3 && j < L.off_light) {
+            const double* d = sdR + 9 * (j - L.off_pose - 3);
+            for (int a = 0; a < 3; ++a) dv[a] = d[a] * sp[i][0] + d[3 + a] * sp[i][1] + d[6 + a] * sp[i][2];
+        } else if (j >= L.off_exp && j < L.off_pose) {
+            const int v = m.lmk_ids[i], c = j - L.off_exp;
+            double e3[3];
+            for (int a = 0; a < 3; ++a) e3[a] = m.exp_basis[(size_t)(3 * v + a) * m.exp_ld + c];
+            for (int a = 0; a < 3; ++a) dv[a] = sR[a] * e3[0] + sR[3 + a] * e3[1] + sR[6 + a] * e3[2];
+        } else if (j >= L.off_id && j < L.off_exp) {
+            const int v = m.lmk_ids[i], c = j - L.off_id;
+            double e3[3];
+            for (int a = 0; a < 3; ++a) e3[a] = m.id_basis[(size_t)(3 * v + a) * m.id_ld + c];
+            for (int a = 0; a < 3; ++a) dv[a] = sR[a] * e3[0] + sR[3 + a] * e3[1] + sR[6 + a] * e3[2];
+        } else {
+            geo = false;
+        }
+        double dxj = 0.0, dyj = 0.0;
+        if (geo) {
+            const double vx = sv[i][0], vy = sv[i][1], vz = sv[i][2];
+            dxj = fr.phix * (dv[0] * vz - dv[2] * vx) / (vz * vz);
+            dyj = fr.phiy * (dv[1] * vz - dv[2] * vy) / (vz * vz);
+        }
+        sdx[(size_t)i * K + j] = dxj;
+        sdy[(size_t)i * K + j] = dyj;
+    }
+    __syncthreads();
+    for (int j = tid; j < K; j += blockDim.x) {
+        double sacc = 0.0;
+        for (int i = 0; i < NLMK; ++i) sacc += sgx[i] * sdx[(size_t)i * K + j] + sgy[i] * sdy[(size_t)i * K + j];
+        g[(size_t)f * K + j] += (float)sacc;
+    }
+    __syncthreads();
+    for (int e = tid; e < NLMK * K; e += blockDim.x) {
+        const int i = e / K;
+        sdx[e] *= sqrt(swx[i]);
+        sdy[e] *= sqrt(swy[i]);
+    }
+}
+
+constexpr int BH_T = 32;
+
+// H += sum_i dx_i dx_i^T + dy_i dy_i^T over the tiles (I, J >= I) of 32 x 32; both halves written.
+__global__ void __launch_bounds__(256) k_barrier_hess(int K, const int* __restrict__ act, int n,
+                                                       const double* __restrict__ D, double* __restrict__ H)
+{
+    const int T = (K + BH_T - 1) / BH_T;
+    int t = blockIdx.x, I = 0;
+    while (t >= T - I) { t -= T - I; ++I; }
+    const int J = I + t;
+    __shared__ double sa[2][NLMK][BH_T], sb[2][NLMK][BH_T];
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        const double* Df = D + (size_t)f * 2 * NLMK * K;
+        for (int e = threadIdx.x; e < 2 * NLMK * BH_T; e += blockDim.x) {
+            const int c = e % BH_T, i = (e / BH_T) % NLMK, q = e / (BH_T * NLMK);
+            const int a = I * BH_T + c, b = J * BH_T + c;
+            sa[q][i][c] = a < K ? Df[((size_t)q * NLMK + i) * K + a] : 0.0;
+            sb[q][i][c] = b < K ? Df[((size_t)q * NLMK + i) * K + b] : 0.0;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < BH_T * BH_T; e += blockDim.x) {
+            const int r = e / BH_T, c = e - r * BH_T;
+            const int a = I * BH_T + r, b = J * BH_T + c;
+            if (a >= K || b >= K || (I == J && c < r)) continue;
+            double sacc = 0.0;
+            for (int i = 0; i < NLMK; ++i) sacc += sa[0][i][r] * sb[0][i][c] + sa[1][i][r] * sb[1][i][c];
+            H[(size_t)f * K * K + (size_t)a * K + b] += sacc;
+            if (a != b) H[(size_t)f * K * K + (size_t)b * K + a] += sacc;
+        }
+        __syncthreads();
+    }
+}
+
 } // namespace
+
+int launch_barrier_big(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
+                       const LmkArrays& lm, int use_landmarks, double* D, double* H, float* g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_barrier_rows, dim3(fl.n > 0 ? fl.n : 1), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta,
+                       lm, use_landmarks, D, H, g);
+    if (use_landmarks) {
+        const int T = (L.K + BH_T - 1) / BH_T;
+        hipLaunchKernelGGL(k_barrier_hess, dim3(T * (T + 1) / 2, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, L.K,
+                           fl.act, fl.n, D, H);
+    }
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
 
 int launch_barrier(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                    const float* theta, const LmkArrays& lm, int use_landmarks, int mode, double* obj, double* objb,
                    double* H, float* g, hipStream_t s)
 {
-    if (L.K > BMAXK) {
-        set_error("barrier: K=%d > %d", L.K, BMAXK);
+    // the objective needs no LDS rows; the derivative terms above BMAXK go through launch_barrier_big
+    if (L.K > BMAXK && mode == 1 && use_landmarks) {
+        set_error("barrier: K=%d > %d (derivative terms: launch_barrier_big)", L.K, BMAXK);
         return TDI_ERR_ARG;
     }
     const size_t lds = mode == 1 && use_landmarks ? 2 * sizeof(double) * NLMK * L.K : 0;

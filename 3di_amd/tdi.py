@@ -49,6 +49,10 @@ def lib():
         L.tdi_render.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_normal_equations.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.tdi_normal_equations_mixed.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_joint_unknowns.argtypes = [_vp]
+        L.tdi_joint_unknowns.restype = C.c_int64
+        L.tdi_normal_equations_joint.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_fit_multiframe.argtypes = [_vp, _vp, _vp]
         L.tdi_cholesky_solve.argtypes = [C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, C.c_int32, _vp]
         L.tdi_fit_plain.argtypes = [_vp, _vp, _vp]
         L.tdi_fit_batched.argtypes = [_vp, _vp, _vp, _vp]
@@ -298,6 +302,32 @@ class Batch:
         fn = lib().tdi_normal_equations_mixed if mixed else lib().tdi_normal_equations
         _check(fn(self.h, _dptr(th), _dptr(H), _dptr(g), _dptr(obj), _stream()))
         return H, g, obj
+
+    # -------------------------------------------------------------- identity stage
+    def joint_unknowns(self) -> int:
+        """Unknowns of the joint fit of the batch's frames: the shared albedo and identity block
+        plus every frame's expression, pose and light."""
+        return int(lib().tdi_joint_unknowns(self.h))
+
+    def joint_normal_equations(self, theta):
+        """(HJ [KJ, KJ] float64, gJ [KJ] float64, obj [1] float64) of the joint system at the
+        frames' theta [n, K] (the shared block is frame 0's)."""
+        torch = _torch()
+        th = self._theta(theta)
+        KJ = self.joint_unknowns()
+        H = torch.empty((KJ, KJ), device="cuda", dtype=torch.float64)
+        g = torch.empty(KJ, device="cuda", dtype=torch.float64)
+        obj = torch.empty(1, device="cuda", dtype=torch.float64)
+        _check(lib().tdi_normal_equations_joint(self.h, _dptr(th), _dptr(H), _dptr(g), _dptr(obj), _stream()))
+        return H, g, obj
+
+    def fit_multiframe(self, theta0):
+        """The identity stage's joint Gauss-Newton fit (tdi_fit_multiframe): theta0 host [n, K];
+        returns (theta [n, K] with the shared block on every frame, report)."""
+        th = np.ascontiguousarray(np.array(theta0, np.float32).reshape(self.n, self.K))
+        rep = abi.tdi_fit_report()
+        _check(lib().tdi_fit_multiframe(self.h, th.ctypes.data, C.addressof(rep)))
+        return th, rep
 
     def stage_times(self, theta, mixed_flags: int = 7, reps: int = 3) -> np.ndarray:
         """Mean ms per stage of one evaluation + solve (see tdi_stage_times)."""

@@ -212,6 +212,33 @@ int tdi_normal_equations(tdi_batch* b, const float* theta, float* H, float* g, f
 /* Same, with J split into BF16 hi/lo parts and J^T J, J^T r on MFMA (FP32 accumulate). */
 int tdi_normal_equations_mixed(tdi_batch* b, const float* theta, float* H, float* g, float* obj, void* stream);
 
+/* Above 111 unknowns per frame (the identity stage: albedo + identity + expression + pose +
+ * light) the normal equations take J^T J and J^T r on MFMA over 64 x 64 tiles (BF16 hi/mid/lo
+ * parts, FP32 accumulation, FP64 sums; both entry points above) and the barrier's derivative
+ * rows in scratch the batch allocates on first use.
+ *
+ * Identity stage: the joint fit of the batch's frames, which share the albedo and identity
+ * coefficients — the shape fit of fit_to_multi_images (model_fitter.cu:1252-1479), i.e.
+ * fit_3DMM_shape_rigid over T frames (model_fitter.cu:121-375) with the frames' products summed
+ * into one matrix as get_obj_hessian_and_gradient_multiframe does (model_fitter.cu:21-116).
+ * The shared block is the first Ks = k_tex + k_id unknowns of every frame's layout; the joint
+ * vector is [shared | frame 0's expression, pose, light | frame 1's | ...], KJ = Ks + n (K - Ks).
+ * The coefficient bounds of the shared block count once, every frame's landmark barrier and
+ * expression bounds once per frame.  All frames must fit one workspace chunk. */
+int64_t tdi_joint_unknowns(tdi_batch* b);
+
+/* Joint normal matrix HJ [KJ][KJ] FP64, gradient gJ [KJ] FP64 and objective obj [1] FP64 (may be
+ * NULL) at the frames' theta [n][K] (DEVICE; the shared block is read from frame 0). */
+int tdi_normal_equations_joint(tdi_batch* b, const float* theta, double* HJ, double* gJ, double* obj, void* stream);
+
+/* The joint Gauss-Newton fit: theta [n][K] HOST in/out (the shared block is read from frame 0 and
+ * written to every frame).  Per iteration the joint system is assembled on the device and solved
+ * by the blocked FP64 Cholesky; Armijo line search on the summed objective with the reference's
+ * constants (ALPHA 0.4, BETA 0.8, -d.g < 3e-4, step threshold 0.001 for several frames, a
+ * negative depth of frame 0 ends the search).  report: one tdi_fit_report for the whole fit
+ * (n_pixels summed over the frames). */
+int tdi_fit_multiframe(tdi_batch* b, float* theta, tdi_fit_report* report);
+
 /* Batched Cholesky solve of H delta = -g.  mode 0: FP64 factorisation (Solver::solve,
  * solver.h:70-105, cusolverDnDpotrf/Dpotrs); mode 1: FP32 factorisation of the equilibrated
  * matrix D^-1/2 H D^-1/2 + one FP64 residual-refinement step, with the FP64 factorisation

@@ -300,3 +300,128 @@ def paint_innermouth_black(frames, lmk_x, lmk_y, ref_face_size=86.0):
     ly = np.ascontiguousarray(lmk_y, np.float32)
     lib().orc_paint_innermouth_black(n, fr.ctypes.data, W, H, H * W * 3, p(lx, _fp), p(ly, _fp), ref_face_size)
     return fr
+
+
+class JointScene:
+    """The identity stage's joint system on the oracle (the shape fit of fit_to_multi_images,
+    model_fitter.cu:1252-1479: fit_3DMM_shape_rigid over T frames, model_fitter.cu:121-375, with
+    the frames' products summed as get_obj_hessian_and_gradient_multiframe does, 21-116).  The
+    frames share the albedo and identity coefficients, the first Ks unknowns of every frame's
+    layout; x = [shared | frame 0's rest | frame 1's rest | ...].  Every frame's normal equations
+    come from the oracle with the shared bounds left out (a copy of the model whose albedo and
+    identity bounds are infinite, so that the barrier skips them); the shared bounds' terms are
+    the oracle's barrier with and without them at frame 0, counted once."""
+
+    def __init__(self, model, frames, opts):
+        import dataclasses
+        self.full = Scene(model, frames, opts)
+        inf = lambda a: None if a is None else np.full(np.shape(a), np.inf, np.float32)
+        free = dataclasses.replace(model, id_lb=None if model.id_lb is None else -inf(model.id_lb),
+                                   id_ub=inf(model.id_ub), tex_lb=None if model.tex_lb is None else -inf(model.tex_lb),
+                                   tex_ub=inf(model.tex_ub))
+        self.free = Scene(free, frames, opts)
+        self.n = len(frames)
+        self.K = self.full.K
+        self.Ks = (model.k_tex if opts.fit_tex else 0) + (model.k_id if opts.fit_id else 0)
+        self.Kf = self.K - self.Ks
+        self.KJ = self.Ks + self.n * self.Kf
+        self.opts = opts
+
+    def join(self, thetas):
+        th = np.asarray(thetas, np.float64).reshape(self.n, self.K)
+        return np.concatenate([th[0, :self.Ks]] + [th[t, self.Ks:] for t in range(self.n)])
+
+    def split(self, x):
+        x = np.asarray(x, np.float64)
+        th = np.zeros((self.n, self.K))
+        for t in range(self.n):
+            th[t, :self.Ks] = x[:self.Ks]
+            th[t, self.Ks:] = x[self.Ks + t * self.Kf:self.Ks + (t + 1) * self.Kf]
+        return th
+
+    def shared_bounds(self, th0):
+        """(obj, g [Ks], H diagonal [Ks]) of the shared block's bounds at frame 0's theta."""
+        of, gf, Hf = self.full.barrier(0, th0)
+        oo, go, Ho = self.free.barrier(0, th0)
+        Ks = self.Ks
+        return of - oo, (gf - go)[:Ks], np.diag(Hf - Ho)[:Ks]
+
+    def normal_equations(self, x):
+        """(HJ [KJ, KJ], gJ [KJ], objective) at x."""
+        th = self.split(x)
+        Ks, Kf, KJ = self.Ks, self.Kf, self.KJ
+        H = np.zeros((KJ, KJ))
+        g = np.zeros(KJ)
+        obj = 0.0
+        for t in range(self.n):
+            ne = self.free.normal_equations(t, th[t])
+            idx = np.concatenate([np.arange(Ks), Ks + t * Kf + np.arange(Kf)])
+            H[np.ix_(idx, idx)] += ne["H"]
+            g[idx] += ne["g"]
+            obj += ne["obj"][2]
+        ob, gb, hb = self.shared_bounds(th[0])
+        g[:Ks] += gb
+        H[np.arange(Ks), np.arange(Ks)] += hb
+        return H, g, obj + ob
+
+    def objective(self, x):
+        th = self.split(x)
+        obj = sum(self.free.render(t, th[t])["obj"][2] for t in range(self.n))
+        return obj + self.shared_bounds(th[0])[0]
+
+    def fit(self, x0, max_iters=None):
+        """The joint Gauss-Newton fit from x0 (FP64, numpy Cholesky): Armijo line search with the
+        options' constants and the reference's step threshold for several frames (0.001).
+        Returns (x, iterations performed, stop flags)."""
+        o = self.opts
+        x = np.asarray(x0, np.float64).copy()
+        t_min = 0.001 if self.n > 1 else o.t_min
+        performed, stop, terminate = 0, 0, False
+        iters = o.max_iters if max_iters is None else max_iters
+        pose0 = self._pose_offset()
+        for _ in range(iters):
+            if terminate:
+                break
+            H, g, obj0 = self.normal_equations(x)
+            performed += 1
+            try:
+                Lc = np.linalg.cholesky(H)
+            except np.linalg.LinAlgError:
+                stop |= 16
+                break
+            d = -np.linalg.solve(Lc.T, np.linalg.solve(Lc, g))
+            dg = float(d @ g)
+            t = 1.0
+            inner = 0
+            while inner < o.max_ls:
+                if t < t_min:
+                    terminate = True
+                    stop |= 4
+                    break
+                xt = x + t * d
+                if xt[pose0 + 2] < 0:
+                    break
+                objt = self.objective(xt)
+                if not np.isfinite(objt):
+                    t *= o.armijo_beta
+                    continue
+                inner += 1
+                if -dg < o.decrement_tol:
+                    terminate = True
+                    stop |= 2
+                    break
+                if objt < obj0 + o.armijo_alpha * t * dg:
+                    x = xt
+                    break
+                t *= o.armijo_beta
+        if not terminate and not stop & 24:
+            stop |= 1
+        return x, performed, stop
+
+    def _pose_offset(self):
+        """Offset of frame 0's pose in x (the per-frame layout's pose offset)."""
+        m, o = self.full.model, self.opts
+        ntex = m.k_tex if o.fit_tex else 0
+        nid = m.k_id if o.fit_id else 0
+        nexp = m.k_exp if o.fit_exp else 0
+        return ntex + nid + nexp

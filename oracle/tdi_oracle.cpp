@@ -367,6 +367,7 @@ double barrier(const Frame& F, const double* th, const Geometry& G, double* grad
     double obj = 0;
     auto bound = [&](int off, int n, const float* lb, const float* ub) {
         for (int k = 0; k < n; ++k) {
+            if (!(ub[k] < INFINITY) || !(lb[k] > -INFINITY)) continue;   // unbounded: no term
             double x = th[off + k];
             double fub = x - ub[k], flb = lb[k] - x;
             obj += -std::log(-fub) - std::log(-flb);

@@ -142,3 +142,35 @@ def branch_scene(iters=3, fit_light=2):
     # normal frames stay below the limit 2 * ref throughout their short fits, the large one is above
     sc.opts.ref_face_size = float(0.25 * (1.3 * normal + 0.7 * big))
     return sc
+
+
+def joint_scene(grid=48, k_id=12, k_tex=12, k_exp=10, fit_light=2, crops=(128, 160, 144), pose_deg=20.0,
+                noise=0.02, iters=8, seed=11, cfg=61, coef_pert=0.05):
+    """Frames of one face for the identity stage: a shared albedo and identity draw, each frame
+    with its own pose, expression and light; observations rendered by the oracle.  The start
+    shares one perturbed albedo/identity block, and every frame's start is pulled towards its
+    truth (the shared block with it) until all landmark-box constraints hold with 0.5 px."""
+    m = model(grid, k_id, k_tex, k_exp, seed)
+    lay = synth.layout(m, True, True, True, fit_light)
+    opts = abi.default_options(max_iters=iters, fit_light=int(fit_light), fit_id=1, fit_tex=1)
+    Ks = lay["exp"]
+    rng0 = np.random.default_rng(synth.frame_seed(cfg, 1000))
+    shared = np.clip(rng0.standard_normal(Ks), -3, 3)
+    shared_init = np.clip(shared + rng0.normal(0, coef_pert, Ks), -3.2, 3.2)
+    frames, truths = [], []
+    for i, S in enumerate(crops):
+        rng = np.random.default_rng(synth.frame_seed(cfg, i))
+        tr = synth.draw_truth(m, rng, lay, pose_deg, fit_light=fit_light, coef_pert=coef_pert)
+        tr.theta[:Ks] = shared
+        tr.init[:Ks] = shared_init
+        fr = synth.make_frame(m, tr, lay, rng, synth.cfg2_focal(S), S, S)
+        raw = O.Scene(m, [fr], opts).render(0, tr.theta)["raw"].astype(np.float32)
+        fr.obs = synth.add_noise(raw, rng, noise) if noise > 0 else raw
+        frames.append(fr)
+        truths.append(tr)
+    for _ in range(30):
+        if all(synth.landmark_slack(m, tr.init, lay, fr) > 0.5 for tr, fr in zip(truths, frames)):
+            break
+        for tr in truths:
+            tr.init = tr.theta + 0.7 * (tr.init - tr.theta)
+    return Scene(model=m, frames=frames, truths=truths, opts=opts, lay=lay)
