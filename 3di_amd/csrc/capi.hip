@@ -148,7 +148,7 @@ struct tdi_batch {
     // scratch of the normal equations above 111 unknowns (allocated on first use): the Jacobian
     // rows U, the J^T J partials and the barrier's landmark-derivative rows
     float* big_U = nullptr;
-    float* big_partials = nullptr;
+    double* big_partials = nullptr;
     double* big_D = nullptr;
     int big_S = 0;
     hipStream_t stream = nullptr;     // capture-capable stream owned by the batch (default priority)
@@ -590,10 +590,10 @@ static int big_scratch(tdi_batch* b)
     const int K = b->ctx->L.K;
     int S = w.max_area / 512 + 1;    // pixel splits per frame: about 512 pixels of a full crop or more each
     if (S > 16) S = 16;
-    const size_t nu = (size_t)w.cap * jtj_big_row_stride(K), np = jtj_big_partials_floats(w.C, S, K);
+    const size_t nu = (size_t)w.cap * jtj_big_row_stride(K), np = jtj_big_partials_doubles(w.C, S, K);
     const size_t nd = (size_t)w.C * 2 * NLMK * K;
     if (hipMalloc((void**)&b->big_U, sizeof(float) * nu) != hipSuccess ||
-        hipMalloc((void**)&b->big_partials, sizeof(float) * np) != hipSuccess ||
+        hipMalloc((void**)&b->big_partials, sizeof(double) * np) != hipSuccess ||
         hipMalloc((void**)&b->big_D, sizeof(double) * nd) != hipSuccess) {
         (void)hipFree(b->big_U);
         (void)hipFree(b->big_partials);
@@ -602,7 +602,7 @@ static int big_scratch(tdi_batch* b)
         b->big_partials = nullptr;
         b->big_D = nullptr;
         set_error("normal equations at K=%d: %zu bytes of scratch not available", K,
-                  sizeof(float) * (nu + np) + sizeof(double) * nd);
+                  sizeof(float) * nu + sizeof(double) * (np + nd));
         return TDI_ERR_NOMEM;
     }
     b->big_S = S;
@@ -715,8 +715,8 @@ static int eval_jacobian(tdi_batch* b, FrameList fl, const float* theta, int mix
                         dI_area(w), s)))
         return rc;
     if (L.K > SMALL_JTJ_MAXK) {
-        // hundreds of unknowns (the identity stage): J^T J on MFMA over 64 x 64 tiles (BF16
-        // hi/mid/lo, FP32 accumulation, FP64 sums) and the barrier's rows in global scratch
+        // hundreds of unknowns (the identity stage): J^T J on FP64 MFMA over 64 x 64 tiles and
+        // the barrier's rows in global scratch
         if ((rc = big_scratch(b))) return rc;
         if ((rc = launch_jtj_big(L, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
                                  b->big_U, b->big_partials, b->big_S, w.max_area, s)))

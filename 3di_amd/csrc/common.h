@@ -348,12 +348,13 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameL
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);
 // J^T J and J^T r for frames with more columns than one workgroup's LDS holds (the identity
-// stage): U [crop pixels][jtj_big_row_stride(K)] and partials [jtj_big_partials_floats] scratch.
-size_t jtj_big_partials_floats(int n, int S, int K);
+// stage), FP64 MFMA on FP32 rows: U [crop pixels][jtj_big_row_stride(K)] floats and partials
+// [jtj_big_partials_doubles] of scratch.
+size_t jtj_big_partials_doubles(int n, int S, int K);
 int jtj_big_row_stride(int K);
 int launch_jtj_big(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
                    const float* pcoef, const float4* pixg, const float* dI, float w, double* H, float* g, float* U,
-                   float* partials, int S, int max_area, hipStream_t s);
+                   double* partials, int S, int max_area, hipStream_t s);
 int launch_jtj_f32(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
                    const float* pcoef, const float4* pixg, const float* dI, float w, double* H, float* g,
                    float* partials, int S, hipStream_t s);

@@ -693,19 +693,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T >= 7
 
 // ------------------------------------------------------------------ J^T J for hundreds of columns
 // The identity stage's frames have K = 199 + 199 + 94 unknowns: more columns than one workgroup's
-// LDS holds.  Two passes:
+// LDS holds, and a joint system whose condition number reaches 1e10 (the albedo and identity
+// components that barely move the image have eigenvalues of the barrier's size, 0.16, next to
+// data eigenvalues of 3e9).  FP32 accumulation leaves errors of 1e-7 of the largest entries in
+// J^T J, far above those eigenvalues, and the joint matrix is then not positive definite
+// (measured on the identity-stage workload: the factorisation fails at the first iteration).
+// So J^T J is formed with exact products and FP64 sums: FP32 rows, FP64 MFMA.  Two passes:
 //   k_urows     the one-row form of the normalised-gradient Jacobian, u = c |d| s (see k_jtj_lds),
 //               written for the first N positions of each frame to U [pixel][KP] (FP32, row
 //               stride KP = K + 1 rounded up to 64); column K holds r = (g^s_x dy - g^s_y dx) / |d|
 //               so that J^T r is the last column of the product, columns past K are zero;
-//   k_syrk_big  U^T U over 64 x 64 output tiles (upper triangle of tiles), a pixel split per
-//               grid column: 64 pixels per step are converted to BF16 hi/mid/lo in the MFMA
-//               operand layout in LDS, then each wavefront runs the six products of its 16-row
-//               block against the tile's four 16-column blocks; FP32 partials per split, summed in
-//               FP64 by k_jtj_reduce (TS = 64).
+//   k_syrk_f64  U^T U over 64 x 64 output tiles (upper triangle of tiles), a pixel split per grid
+//               column: 64 pixels per step staged in LDS as FP32, each wavefront runs
+//               v_mfma_f64_16x16x4f64 for its 16-row block against the tile's four 16-column
+//               blocks (FP32 values widened exactly to FP64); FP64 partials per split, summed in
+//               order by k_syrk_reduce.
 constexpr int UB_T = 64;       // output tile
 constexpr int UB_PX = 64;      // pixels per step
 constexpr int UR_PX = 16;      // pixels per k_urows workgroup row
+
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256) k_urows(int K, int KP, const FrameGPU* __restrict__ frames,
                                                 const int* __restrict__ act, int n, const int* __restrict__ npix,
@@ -756,14 +763,14 @@ __device__ __forceinline__ void tri_tile(int t, int T, int& I, int& J)
     J = I + t;
 }
 
-__global__ void __launch_bounds__(256) k_syrk_big(int KP, const FrameGPU* __restrict__ frames,
+// FP64 MFMA operand layout, v_mfma_f64_16x16x4f64: A[i][k] at lane i + 16 k, B[k][j] at lane
+// j + 16 k; lane l's accumulator element r holds C[l / 16 + 4 r][l % 16].
+__global__ void __launch_bounds__(256) k_syrk_f64(int KP, const FrameGPU* __restrict__ frames,
                                                    const int* __restrict__ act, int n, const int* __restrict__ npix,
-                                                   const float* __restrict__ U, int S, int nsplit,
-                                                   float* __restrict__ Hp)
+                                                   const float* __restrict__ U, int S, double* __restrict__ Hp)
 {
-    constexpr int KS = UB_PX / 16;                                 // MFMA k-steps per step
-    __shared__ uint2 sA[KS * 4 * 3 * 64];                          // [kstep][block][part][lane]
-    __shared__ uint2 sB[KS * 4 * 3 * 64];
+    __shared__ float sA[UB_PX][UB_T + 1];                  // [pixel][column], padded rows
+    __shared__ float sB[UB_PX][UB_T + 1];
     const int T = KP / UB_T, NT = T * (T + 1) / 2;
     int I, J;
     tri_tile(blockIdx.x, T, I, J);
@@ -771,6 +778,7 @@ __global__ void __launch_bounds__(256) k_syrk_big(int KP, const FrameGPU* __rest
     const int s = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int li = lane & 15, kq = lane >> 4;
     const int nact = fl_count(act, n);
     for (int yi = blockIdx.z; yi < nact; yi += gridDim.z) {
         const int f = fl_frame(act, yi);
@@ -778,73 +786,73 @@ __global__ void __launch_bounds__(256) k_syrk_big(int KP, const FrameGPU* __rest
         const int N = npix[f];
         const int p0 = (int)((long long)N * s / S), p1 = (int)((long long)N * (s + 1) / S);
         const float* Uf = U + (size_t)fr.pix_off * KP;
-        v4f acc[4];
+        v4d acc[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[b] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int b = 0; b < 4; ++b) acc[b] = (v4d){0.0, 0.0, 0.0, 0.0};
         for (int kb = p0; kb < p1; kb += UB_PX) {
-            // (slice, quad of pixels, column) items: 16 quads x 64 columns per slice
-            const int nitems = (diag ? 1 : 2) * (UB_PX / 4) * UB_T;
+            // 64 pixels x 64 columns per slice, a row segment of 64 floats per pixel (coalesced)
+            const int nitems = (diag ? 1 : 2) * UB_PX * UB_T;
             for (int it = tid; it < nitems; it += 256) {
-                const int sl = it / ((UB_PX / 4) * UB_T);
-                const int r = it - sl * ((UB_PX / 4) * UB_T);
-                const int q = r / UB_T, c = r - q * UB_T;
-                const int col = (sl ? J : I) * UB_T + c;
-                float u[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int p = kb + 4 * q + e;
-                    u[e] = p < p1 ? Uf[(size_t)p * KP + col] : 0.f;
-                }
-                const unsigned ph0 = pack_bf16(u[0], u[1]), ph1 = pack_bf16(u[2], u[3]);
-                const float r0 = u[0] - lo_f(ph0), r1 = u[1] - hi_f(ph0);
-                const float r2 = u[2] - lo_f(ph1), r3 = u[3] - hi_f(ph1);
-                const unsigned pm0 = pack_bf16(r0, r1), pm1 = pack_bf16(r2, r3);
-                const int ks = q >> 2, lp = ((q & 3) << 4) | (c & 15), blk = c >> 4;
-                uint2* dst = (sl ? sB : sA) + ((ks * 4 + blk) * 3) * 64 + lp;
-                dst[0] = make_uint2(ph0, ph1);
-                dst[64] = make_uint2(pm0, pm1);
-                dst[128] = make_uint2(pack_bf16(r0 - lo_f(pm0), r1 - hi_f(pm0)), pack_bf16(r2 - lo_f(pm1), r3 - hi_f(pm1)));
+                const int sl = it / (UB_PX * UB_T);
+                const int r = it - sl * (UB_PX * UB_T);
+                const int p = r / UB_T, c = r - p * UB_T;
+                const int pix = kb + p;
+                const float v = pix < p1 ? Uf[(size_t)pix * KP + (sl ? J : I) * UB_T + c] : 0.f;
+                (sl ? sB : sA)[p][c] = v;
             }
             __syncthreads();
-            const uint2* sb = diag ? sA : sB;
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                const uint2* qa = sA + ((ks * 4 + wave) * 3) * 64 + lane;
-                const v4s ah = __builtin_bit_cast(v4s, qa[0]);
-                const v4s am = __builtin_bit_cast(v4s, qa[64]);
-                const v4s al = __builtin_bit_cast(v4s, qa[128]);
+            const float (*sb)[UB_T + 1] = diag ? sA : sB;
+#pragma unroll 4
+            for (int ks = 0; ks < UB_PX / 4; ++ks) {
+                const double a = (double)sA[4 * ks + kq][16 * wave + li];
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const uint2* qb = sb + ((ks * 4 + b) * 3) * 64 + lane;
-                    const v4s bh = __builtin_bit_cast(v4s, qb[0]);
-                    const v4s bm = __builtin_bit_cast(v4s, qb[64]);
-                    v4f a = acc[b];
-                    a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bm, a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, a, 0, 0, 0);
-                    if (nsplit == 3) {
-                        const v4s bl = __builtin_bit_cast(v4s, qb[128]);
-                        a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, a, 0, 0, 0);
-                        a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, a, 0, 0, 0);
-                        a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm, a, 0, 0, 0);
-                    }
-                    acc[b] = a;
+                    const double bv = (double)sb[4 * ks + kq][16 * b + li];
+                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[b], 0, 0, 0);
                 }
             }
             __syncthreads();   // the next step rewrites the staging
         }
-        float* Hout = Hp + (((size_t)f * S + s) * NT + blockIdx.x) * (UB_T * UB_T);
-        const int grp = lane >> 4, col = lane & 15;
+        double* Hout = Hp + (((size_t)f * S + s) * NT + blockIdx.x) * (UB_T * UB_T);
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Hout[(size_t)(16 * wave + 4 * grp + r) * UB_T + 16 * b + col] = acc[b][r];
+            for (int r = 0; r < 4; ++r) Hout[(size_t)(16 * wave + kq + 4 * r) * UB_T + 16 * b + li] = acc[b][r];
+    }
+}
+
+// H [f][K][K] = w sum_s of the tiles (both triangles), g [f][K] = -w sum_s of column K.
+__global__ void __launch_bounds__(256) k_syrk_reduce(const int* __restrict__ act, int n, int K, int KP, int S,
+                                                      const double* __restrict__ Hp, float w, double* __restrict__ H,
+                                                      float* __restrict__ g)
+{
+    const int T = KP / UB_T, NT = T * (T + 1) / 2;
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K * K + K; e += gridDim.x * blockDim.x) {
+            int a, b;
+            if (e < K * K) {
+                a = e / K;
+                b = e - a * K;
+            } else {
+                a = e - K * K;
+                b = K;
+            }
+            if (a / UB_T > b / UB_T) { const int t = a; a = b; b = t; }
+            const int tile = tri_index(a / UB_T, b / UB_T, T);
+            const size_t off = (size_t)(a % UB_T) * UB_T + (b % UB_T);
+            double sum = 0.0;
+            for (int s = 0; s < S; ++s) sum += Hp[(((size_t)f * S + s) * NT + tile) * (UB_T * UB_T) + off];
+            if (e < K * K) H[(size_t)f * K * K + e] = (double)w * sum;
+            else g[(size_t)f * K + (e - K * K)] = (float)(-(double)w * sum);
+        }
     }
 }
 
 } // namespace
 
-size_t jtj_big_partials_floats(int n, int S, int K)
+size_t jtj_big_partials_doubles(int n, int S, int K)
 {
     const int KP = ((K + 1 + UB_T - 1) / UB_T) * UB_T;
     const int T = KP / UB_T;
@@ -855,7 +863,7 @@ int jtj_big_row_stride(int K) { return ((K + 1 + UB_T - 1) / UB_T) * UB_T; }
 
 int launch_jtj_big(const Layout& L, const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb,
                    const float* pcoef, const float4* pixg, const float* dI, float w, double* H, float* g, float* U,
-                   float* partials, int S, int max_area, hipStream_t s)
+                   double* partials, int S, int max_area, hipStream_t s)
 {
     const int K = L.K;
     const int KP = jtj_big_row_stride(K);
@@ -868,11 +876,11 @@ int launch_jtj_big(const Layout& L, const FrameGPU* frames, FrameList fl, const 
     if (npb < 1) npb = 1;
     hipLaunchKernelGGL(k_urows, dim3(ncb * npb, gy), dim3(256), 0, s, K, KP, frames, fl.act, n, npix, pnb, pcoef, pixg,
                        dI, U);
-    hipLaunchKernelGGL(k_syrk_big, dim3(T * (T + 1) / 2, S, gy), dim3(256), 0, s, KP, frames, fl.act, n, npix, U, S, 3,
+    hipLaunchKernelGGL(k_syrk_f64, dim3(T * (T + 1) / 2, S, gy), dim3(256), 0, s, KP, frames, fl.act, n, npix, U, S,
                        partials);
     int bx = (K * K + K + 255) / 256;
-    hipLaunchKernelGGL(k_jtj_reduce, dim3(bx, gy), dim3(256), 0, s, fl.act, n, K, KP, UB_T, S, partials,
-                       (const float*)nullptr, w, H, g, 1);
+    if (bx > 4096) bx = 4096;
+    hipLaunchKernelGGL(k_syrk_reduce, dim3(bx, gy), dim3(256), 0, s, fl.act, n, K, KP, S, partials, w, H, g);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

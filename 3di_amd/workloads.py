@@ -41,10 +41,21 @@ class Workload:
 _cfg1_cache = {}
 
 
+_base_cache = {}
+
+
+def base_model(grid: int = 231, k_exp: int = 79, k_id: int = 199, k_tex: int = 199):
+    """The model with its identity and albedo bases (generated once per process)."""
+    key = (grid, k_exp, k_id, k_tex)
+    if key not in _base_cache:
+        _base_cache[key] = synth.make_model(grid, k_id, k_tex, k_exp, synth.BASE_SEED + 1)
+    return _base_cache[key]
+
+
 def cfg2_model(grid: int = 231, k_exp: int = 79, k_id: int = 199, k_tex: int = 199):
     key = (grid, k_exp, k_id, k_tex)
     if key not in _cfg1_cache:
-        base = synth.make_model(grid, k_id, k_tex, k_exp, synth.BASE_SEED + 1)
+        base = base_model(grid, k_exp, k_id, k_tex)
         rng = np.random.default_rng(synth.BASE_SEED + 1)
         alpha = np.clip(rng.standard_normal(k_id), -3, 3)
         beta = np.clip(rng.standard_normal(k_tex), -3, 3)
@@ -237,3 +248,54 @@ def long_video(n_frames: int = 2048, width: int = 1920, height: int = 1080, face
 
     lv.source = source
     return lv
+
+
+def identity_stage(n_frames: int = 10, S: int = 512, iters: int = 30, noise: float = 0.02, coef_pert: float = 0.05,
+                   log=print) -> Workload:
+    """The identity stage at reference scale: the 231x231-grid model with its bases (K_id = K_tex =
+    199, K_exp = 79), n_frames frames of S x S sharing one albedo and identity draw, each with its
+    own pose (+-30 deg), expression and light (nine light unknowns): 398 + n_frames * 94 joint
+    unknowns.  Observations are the GPU render of the truth plus N(0, noise); the start shares one
+    perturbed albedo/identity block and is pulled towards the truth until every frame's
+    landmark-box constraints hold."""
+    import torch
+    t0 = time.time()
+    m = base_model(231, 79, 199, 199)
+    lay = synth.layout(m, True, True, True, 2)
+    opts = abi.default_options(max_iters=iters, fit_light=2, fit_id=1, fit_tex=1)
+    Ks = lay["exp"]
+    rng0 = np.random.default_rng(synth.frame_seed(1, 100000))
+    shared = np.clip(rng0.standard_normal(Ks), -3, 3)
+    shared_init = np.clip(shared + rng0.normal(0, coef_pert, Ks), -3.2, 3.2)
+    frames, truths = [], []
+    for i in range(n_frames):
+        rng = np.random.default_rng(synth.frame_seed(1, i))
+        tr = synth.draw_truth(m, rng, lay, 30.0, fit_light=2, coef_pert=coef_pert)
+        tr.theta[:Ks] = shared
+        tr.init[:Ks] = shared_init
+        fr = synth.make_frame(m, tr, lay, rng, synth.cfg2_focal(S), S, S)
+        frames.append(fr)
+        truths.append(tr)
+    ctx = tdi.Context(m, opts)
+    b = tdi.Batch(ctx, frames)
+    img, _, npix = b.render(torch.from_numpy(np.stack([t.theta for t in truths]).astype(np.float32)).cuda())
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(synth.BASE_SEED + 1)
+    host = (img + noise * torch.randn(img.shape, device="cuda", generator=gen)).cpu().numpy()
+    b.close()
+    ctx.close()
+    off = 0
+    for fr in frames:
+        fr.obs = np.ascontiguousarray(host[off:off + S * S])
+        off += S * S
+    for _ in range(30):
+        if all(synth.landmark_slack(m, tr.init, lay, fr) > 0.5 for tr, fr in zip(truths, frames)):
+            break
+        for tr in truths:
+            tr.init = tr.theta + 0.7 * (tr.init - tr.theta)
+    log(f"[workload] identity stage: {n_frames} frames {S}x{S}, {float(npix.float().mean()):.0f} face pixels/frame "
+        f"({time.time() - t0:.1f}s)")
+    return Workload(model=m, frames=frames, truths=truths, opts=opts, lay=lay,
+                    name=f"identity stage: {n_frames} frames {S}x{S} sharing albedo and identity, 231x231 grid "
+                         f"({m.n_vertices} V, {m.n_triangles} T), K_id = K_tex = 199, K_exp = 79, nine light unknowns, "
+                         f"{Ks + n_frames * (lay['K'] - Ks)} joint unknowns, {iters} GN iterations")

@@ -22,6 +22,9 @@ Also measured in the same run:
     workload's grey crops, against the oracle on a prefix (time and bit-exactness);
   * the render loss and its adjoint (tdi_render_loss_grad) on 256 frames of 256 x 256 crops:
     forward, and forward + backward with the rasterisation recomputed;
+  * the identity stage: the joint fit of 10 frames of 512 x 512 sharing albedo and identity
+    (398 + 10 x 94 = 1338 joint unknowns, 30 iterations, tdi_fit_multiframe), its joint normal
+    equations and first step against the FP64 oracle's;
   * a long video (1024 frames of 1920x1080 BGR8, a random-walk pose and AR(1) expressions)
     through the chunked per-frame stage (tdi_fit_video_long), frames rendered by the frame source
     chunk by chunk; frames/s of the stage and the fitted poses against the truth.
@@ -221,6 +224,8 @@ def main():
     ap.add_argument("--video-frames", type=int, default=1024,
                     help="frames of the long-video run (1920x1080 BGR8, tdi_fit_video_long); 0 skips it")
     ap.add_argument("--video-chunk", type=int, default=512, help="frames per chunk of the long-video run")
+    ap.add_argument("--identity-frames", type=int, default=10,
+                    help="frames of the identity-stage joint fit (512 x 512, 398 + 94 per frame unknowns); 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for scaling rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -506,6 +511,69 @@ def main():
             f"median pose error {long_video['pose_angle_error_deg']['median']} deg")
         ctxv.close()
 
+    # the identity stage: joint fit of frames sharing albedo and identity (tdi_fit_multiframe), and
+    # the joint normal equations and first step against the FP64 oracle
+    identity = None
+    if args.identity_frames > 0:
+        wi = workloads.identity_stage(n_frames=args.identity_frames, log=log)
+        ctxi = tdi.Context(wi.model, wi.opts)
+        bi = tdi.Batch(ctxi, wi.frames)
+        thi0 = wi.init()
+        KJ = bi.joint_unknowns()
+        bi.joint_normal_equations(thi0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        HJg, gJg, objg = bi.joint_normal_equations(thi0)
+        torch.cuda.synchronize()
+        ne_ms = 1e3 * (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        thi, repi = bi.fit_multiframe(thi0)
+        fit_s = time.perf_counter() - t0
+        Ks = wi.lay["exp"]
+        tru = wi.truth()
+        identity = dict(workload=wi.name, frames=args.identity_frames, joint_unknowns=KJ,
+                        joint_normal_equations_ms=round(ne_ms, 2), fit_seconds=round(fit_s, 3),
+                        iterations=int(repi.iters), ls_evals=int(repi.ls_evals), stop=int(repi.stop),
+                        status=int(repi.status),
+                        ms_per_iteration=round(1e3 * fit_s / max(1, repi.iters), 2),
+                        shared_rms_vs_truth=dict(start=round(float(np.sqrt(np.mean((thi0[0, :Ks] - tru[0, :Ks]) ** 2))), 4),
+                                                 end=round(float(np.sqrt(np.mean((thi[0, :Ks] - tru[0, :Ks]) ** 2))), 4)))
+        log(f"[bench] identity stage: {KJ} joint unknowns, {repi.iters} iterations in {fit_s:.2f} s "
+            f"(joint normal equations {ne_ms:.1f} ms)")
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle as O
+            threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+            js = O.JointScene(wi.model, wi.frames, wi.opts)
+            log(f"[bench] identity stage: the oracle's joint normal equations on {min(threads, args.identity_frames)} threads")
+            t0 = time.perf_counter()
+            Ho, go, oo = js.normal_equations(js.join(thi0.astype(np.float64)), threads=min(threads, args.identity_frames))
+            ot = time.perf_counter() - t0
+            Hg = HJg.cpu().numpy()
+            gg = gJg.cpu().numpy()
+            dg_ = -np.linalg.solve(Hg, gg)
+            do_ = -np.linalg.solve(Ho, go)
+            lay = wi.lay
+            Kf = lay["K"] - Ks
+            blocks = {"albedo": np.arange(0, lay["id"]), "identity": np.arange(lay["id"], Ks)}
+            for name, a, b_ in (("expression", lay["exp"], lay["pose"]), ("pose", lay["pose"], lay["pose"] + 6),
+                                ("light", lay["light"], lay["K"])):
+                blocks[name] = np.concatenate([Ks + t * Kf + np.arange(a - Ks, b_ - Ks) for t in range(args.identity_frames)])
+            identity["vs_fp64_oracle"] = dict(
+                H_rel=float(np.linalg.norm(Hg - Ho) / np.linalg.norm(Ho)),
+                g_rel=float(np.linalg.norm(gg - go) / np.linalg.norm(go)),
+                objective_rel=float(abs(float(objg.cpu()[0]) - oo) / abs(oo)),
+                first_step_rms_diff={k: float(np.sqrt(np.mean((dg_[v] - do_[v]) ** 2))) for k, v in blocks.items()},
+                first_step_rms={k: float(np.sqrt(np.mean(do_[v] ** 2))) for k, v in blocks.items()},
+                oracle_seconds=round(ot, 1), oracle_threads=min(threads, args.identity_frames),
+                note="the oracle's joint normal equations at the start (FP64, frames on parallel threads) and the "
+                     "first Gauss-Newton step from each system; 30 oracle iterations at this size take hours on "
+                     "the host")
+            log(f"[bench] identity stage vs oracle: |dH|/|H| {identity['vs_fp64_oracle']['H_rel']:.2e}, "
+                f"|dg|/|g| {identity['vs_fp64_oracle']['g_rel']:.2e} (oracle {ot:.1f} s)")
+        bi.close()
+        ctxi.close()
+
     # FP64 Cholesky solve of one joint identity-stage system (K = 199 + 199 + 10 x 94 = 1338)
     rngc = np.random.default_rng(1338)
     Ac = rngc.normal(size=(1338, 1338)) / np.sqrt(1338)
@@ -600,6 +668,7 @@ def main():
         "differentiation": differentiation,
         "large_k_cholesky": large_k_cholesky,
         "long_video": long_video,
+        "identity_stage": identity,
         "frame_prep": frame_prep,
     }
     print(json.dumps(out), flush=True)

@@ -346,15 +346,21 @@ class JointScene:
         Ks = self.Ks
         return of - oo, (gf - go)[:Ks], np.diag(Hf - Ho)[:Ks]
 
-    def normal_equations(self, x):
-        """(HJ [KJ, KJ], gJ [KJ], objective) at x."""
+    def normal_equations(self, x, threads=1):
+        """(HJ [KJ, KJ], gJ [KJ], objective) at x; threads > 1 evaluates frames concurrently."""
         th = self.split(x)
         Ks, Kf, KJ = self.Ks, self.Kf, self.KJ
         H = np.zeros((KJ, KJ))
         g = np.zeros(KJ)
         obj = 0.0
+        if threads > 1:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(threads) as ex:
+                nes = list(ex.map(lambda t: self.free.normal_equations(t, th[t]), range(self.n)))
+        else:
+            nes = [self.free.normal_equations(t, th[t]) for t in range(self.n)]
         for t in range(self.n):
-            ne = self.free.normal_equations(t, th[t])
+            ne = nes[t]
             idx = np.concatenate([np.arange(Ks), Ks + t * Kf + np.arange(Kf)])
             H[np.ix_(idx, idx)] += ne["H"]
             g[idx] += ne["g"]
