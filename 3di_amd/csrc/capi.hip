@@ -587,7 +587,7 @@ static int big_scratch(tdi_batch* b)
 {
     if (b->big_U) return TDI_OK;
     const Workspace& w = b->ws;
-    const int K = b->ctx->L.K;
+    const int K = b->ctx->Lx.K;      // the light sub-fits evaluate the layout with the light appended
     int S = w.max_area / 512 + 1;    // pixel splits per frame: about 512 pixels of a full crop or more each
     if (S > 16) S = 16;
     const size_t nu = (size_t)w.cap * jtj_big_row_stride(K), np = jtj_big_partials_doubles(w.C, S, K);
@@ -1736,6 +1736,15 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
             Lint_host[i0 + i] = x[(size_t)i * Kx + ol + 3];
         }
     }
+    return TDI_OK;
+}
+
+int tdi_ctx_layout(tdi_ctx* c, int32_t* layout)
+{
+    if (!c || !layout) { set_error("tdi_ctx_layout: null argument"); return TDI_ERR_ARG; }
+    const Layout& L = c->L;
+    layout[0] = L.off_tex; layout[1] = L.off_id; layout[2] = L.off_exp;
+    layout[3] = L.off_pose; layout[4] = L.off_light; layout[5] = L.K;
     return TDI_OK;
 }
 

@@ -13,6 +13,7 @@
 //   * the config-file keys of this path (config.cu:300-560).
 // Only the C ABI of include/tdi_api.h is called; the batched shape fit takes a device copy of
 // the unknowns, made with the HIP runtime.
+#include <random>
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -1010,6 +1011,347 @@ int tdi_fit_video_long(tdi_ctx* ctx, const tdi_video_params* p, const tdi_long_v
     Lint[0] = st.Lint[0];
     if (frames_done) *frames_done = st.frames_done;
     return rc;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------- identity stage
+namespace tdi_host {
+
+// check_if_bb_OK (funcs.cu:582-612) with PAD_RENDERING: the box extended by half its size to the
+// left and above must stay inside the frame.
+static bool bb_ok(const float* x, const float* y, bool pad)
+{
+    const int NL = TDI_NLANDMARKS;
+    const float x0 = *std::min_element(x, x + NL), y0 = *std::min_element(y, y + NL);
+    const float xs = *std::max_element(x, x + NL) - x0, ys = *std::max_element(y, y + NL) - y0;
+    if (!pad) return true;
+    return !(x0 - xs / 2 <= 0) && !(y0 - ys / 2 <= 0);
+}
+
+// check_if_face_in_frame (funcs.cu:614-626)
+static bool face_in_frame(const float* x, const float* y, float cols, float rows)
+{
+    const int NL = TDI_NLANDMARKS;
+    return !(*std::min_element(x, x + NL) < 0 || *std::max_element(x, x + NL) >= cols ||
+             *std::min_element(y, y + NL) < 0 || *std::max_element(y, y + NL) >= rows);
+}
+
+// Nearest centroid index (the first on a tie, as std::min_element finds it).
+static size_t nearest(const std::vector<int>& centroids, double deg)
+{
+    size_t best = 0;
+    double bd = std::fabs(centroids[0] - deg);
+    for (size_t i = 1; i < centroids.size(); ++i) {
+        const double d = std::fabs(centroids[i] - deg);
+        if (d < bd) { bd = d; best = i; }
+    }
+    return best;
+}
+
+// A batch of frames whose crops are 1 x 1 (the landmark fits read no pixels).
+static int landmark_batch(tdi_ctx* ctx, int n, const float* lx, const float* ly, const float* cams,
+                          const float* face_size, const float* box_lo, const float* box_hi, const float* one_pixel,
+                          tdi_batch** b)
+{
+    const int NL = TDI_NLANDMARKS;
+    std::vector<tdi_frame_desc> fd(n);
+    for (int i = 0; i < n; ++i) {
+        tdi_frame_desc& f = fd[i];
+        std::memset(&f, 0, sizeof(f));
+        f.phix = cams[4 * (size_t)i]; f.phiy = cams[4 * (size_t)i + 1];
+        f.cx = cams[4 * (size_t)i + 2]; f.cy = cams[4 * (size_t)i + 3];
+        f.width = 1; f.height = 1;
+        f.face_size = face_size[i];
+        f.obs = one_pixel;
+        f.lmk_x = lx + (size_t)i * NL;
+        f.lmk_y = ly + (size_t)i * NL;
+        f.box_lx = box_lo; f.box_ux = box_hi; f.box_ly = box_lo; f.box_uy = box_hi;
+        f.lambda[0] = -7.3627f; f.lambda[1] = 51.1364f; f.lambda[2] = 100.1784f;
+        f.Lintensity = 0.005f;
+    }
+    return tdi_batch_create(ctx, n, fd.data(), b);
+}
+
+} // namespace tdi_host
+
+extern "C" {
+
+void tdi_default_identity_params(tdi_identity_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->nframes = 7;                 // config.cu:45
+    p->nframes_per_angle_bin = 14;  // config.cu:79
+    p->nperms = 10;                 // config.cu:27
+    p->ntot_reconstrs = 6;          // config.cu:29
+    p->nmax_frames = 432000;        // config.cu:28
+    p->max_vid_frames = 50000;      // config.cu:59
+    p->skip_first_n_secs = 0.f;     // config.cu:55
+    p->fps = 30.f;
+    p->ref_face_size = 86.f;        // config.cu:18
+    p->pad = 1;                     // config.cu:43
+    p->crop_w = 256;                // constants.h:55-56
+    p->crop_h = 256;
+    p->paint_innermouth_black = 1;  // config.cu:92
+    p->lmk_k_exp = 79;              // config.cu:112
+    p->box_half_width = 0.15f;
+}
+
+int tdi_bin_identity_frames(const tdi_identity_params* p, int32_t n, const float* yaw, const float* pitch,
+                            int32_t* selected, int32_t cap, int32_t* n_selected)
+{
+    if (!p || n < 0 || (n > 0 && (!yaw || !pitch)) || !n_selected || cap < 0 || (cap > 0 && !selected) ||
+        p->nframes < 1 || p->nperms < 0 || p->nframes_per_angle_bin < 0 || p->ntot_reconstrs < 0)
+        return TDI_ERR_ARG;
+    // 13 x 13 (yaw, pitch) centroids at -55 .. 65 degrees (video_fitter.cu:2380-2400)
+    std::vector<int> yc, pc;
+    for (int c = -60; c <= 60; c += 10) yc.push_back(c + 5);
+    for (int c = -60; c <= 60; c += 10) pc.push_back(c + 5);
+    std::vector<std::vector<std::vector<int>>> bins(yc.size(), std::vector<std::vector<int>>(pc.size()));
+    const double r2d = 180.0 / M_PI;
+    for (int fi = 0; fi < n; ++fi) {
+        if (std::isnan(yaw[fi])) continue;
+        bins[tdi_host::nearest(yc, r2d * yaw[fi])][tdi_host::nearest(pc, r2d * pitch[fi])].push_back(fi);
+    }
+    std::default_random_engine e_seed(1907);
+    std::vector<int> chosen;
+    for (size_t yi = 0; yi < yc.size(); ++yi)
+        for (size_t pi = 0; pi < pc.size(); ++pi) {
+            std::vector<int>& v = bins[yi][pi];
+            std::shuffle(v.begin(), v.end(), e_seed);
+            for (size_t ci = 0; ci < std::min<size_t>(v.size(), (size_t)p->nframes_per_angle_bin); ++ci)
+                chosen.push_back(v[ci]);
+        }
+    // the chosen frames in frame order (the video is read again from its start, 2455-2478)
+    std::sort(chosen.begin(), chosen.end());
+    chosen.erase(std::unique(chosen.begin(), chosen.end()), chosen.end());
+    std::vector<int> order;
+    for (int k = 0; k < p->nperms; ++k) {
+        std::vector<int> iv(chosen.size());
+        for (size_t i = 0; i < iv.size(); ++i) iv[i] = (int)i;
+        std::shuffle(iv.begin(), iv.end(), e_seed);
+        order.insert(order.end(), iv.begin(), iv.end());
+    }
+    std::vector<int> out, group;
+    for (int ri : order) {
+        group.push_back(chosen[ri]);
+        if ((int)group.size() == p->nframes) {
+            out.insert(out.end(), group.begin(), group.end());
+            group.clear();
+        }
+        if ((long long)out.size() >= p->nmax_frames) break;
+        if ((long long)out.size() >= (long long)p->ntot_reconstrs * p->nframes) break;
+    }
+    *n_selected = (int32_t)out.size();
+    if ((int)out.size() > cap) return TDI_ERR_ARG;
+    for (size_t i = 0; i < out.size(); ++i) selected[i] = out[i];
+    return TDI_OK;
+}
+
+int tdi_select_identity_frames(tdi_ctx* ctx, const tdi_identity_params* p, int32_t n, const float* lmk_x,
+                               const float* lmk_y, int32_t frame_w, int32_t frame_h, const float* cam, float* angles,
+                               int32_t* selected, int32_t cap, int32_t* n_selected, float* resize_coef)
+{
+    if (!ctx || !p || n <= 0 || !lmk_x || !lmk_y || !cam || !n_selected || frame_w < 1 || frame_h < 1)
+        return TDI_ERR_ARG;
+    const int NL = TDI_NLANDMARKS;
+    // first pass (video_fitter.cu:2118-2240): the frames considered and their median face size
+    int every = 1;
+    if (n > 800) every = n / 800;
+    std::vector<float> sizes;
+    for (int idx = 1; idx - 1 < n; ++idx) {
+        if (idx < p->fps * p->skip_first_n_secs) continue;
+        if (idx % every != 0) continue;
+        if (idx >= p->max_vid_frames) break;
+        const float* x = lmk_x + (size_t)(idx - 1) * NL;
+        const float* y = lmk_y + (size_t)(idx - 1) * NL;
+        bool finite = true;
+        for (int i = 0; i < NL; ++i) finite = finite && std::isfinite(x[i]) && std::isfinite(y[i]);
+        if (!finite) continue;
+        const int xmn = (int)*std::min_element(x, x + NL), xmx = (int)*std::max_element(x, x + NL);
+        const int ymn = (int)*std::min_element(y, y + NL), ymx = (int)*std::max_element(y, y + NL);
+        if (xmn <= 0 || ymn <= 0 || xmx >= frame_w || ymx >= frame_h) continue;
+        sizes.push_back(tdi_host::compute_face_size(x, y));
+    }
+    if (sizes.empty()) { *n_selected = 0; return TDI_OK; }
+    const size_t ni = sizes.size() / 2;
+    std::nth_element(sizes.begin(), sizes.begin() + ni, sizes.end());
+    const float coef = 76.0f / sizes[ni];
+    if (resize_coef) *resize_coef = coef;
+    // second pass (2300-2372): every frame's resized landmarks, checks, landmark fits, angles
+    std::vector<float> lx((size_t)n * NL), ly((size_t)n * NL), cams((size_t)n * 4), fs(n);
+    std::vector<char> ok(n, 0);
+    for (int f = 0; f < n; ++f) {
+        bool finite = true;
+        for (int i = 0; i < NL; ++i) {
+            lx[(size_t)f * NL + i] = coef * lmk_x[(size_t)f * NL + i];
+            ly[(size_t)f * NL + i] = coef * lmk_y[(size_t)f * NL + i];
+            finite = finite && std::isfinite(lx[(size_t)f * NL + i]) && std::isfinite(ly[(size_t)f * NL + i]);
+        }
+        for (int a = 0; a < 4; ++a) cams[4 * (size_t)f + a] = coef * cam[a];
+        if (!finite) { fs[f] = 1.f; continue; }
+        const float* x = &lx[(size_t)f * NL];
+        const float* y = &ly[(size_t)f * NL];
+        fs[f] = tdi_host::compute_face_size(x, y);
+        ok[f] = x[0] != 0.0f && tdi_host::bb_ok(x, y, p->pad != 0) &&
+                tdi_host::face_in_frame(x, y, coef * (float)frame_w, coef * (float)frame_h);
+    }
+    std::vector<float> pose((size_t)n * 6), yaw(n, NAN), pitch(n, NAN), roll(n, NAN);
+    int rc = tdi_host::initialize_with_orthographic_t(ctx, n, lx.data(), ly.data(), cams.data(), pose.data(), nullptr,
+                                                      nullptr);
+    if (rc) return rc;
+    std::vector<float> lo(NL, -p->box_half_width), hi(NL, p->box_half_width);
+    float* one = nullptr;
+    if (hipMalloc((void**)&one, sizeof(float)) != hipSuccess) return TDI_ERR_NOMEM;
+    (void)hipMemset(one, 0, sizeof(float));
+    tdi_batch* b = nullptr;
+    rc = tdi_host::landmark_batch(ctx, n, lx.data(), ly.data(), cams.data(), fs.data(), lo.data(), hi.data(), one, &b);
+    if (!rc) {
+        const int K = tdi_batch_unknowns(b);
+        int32_t lay[6];
+        tdi_ctx_layout(ctx, lay);
+        const int op = lay[3];
+        // identity, albedo and expression at zero, the orthographic pose
+        std::vector<float> th((size_t)n * K, 0.f);
+        for (int f = 0; f < n; ++f)
+            for (int a = 0; a < 6; ++a) th[(size_t)f * K + op + a] = pose[(size_t)f * 6 + a];
+        const int kl = std::min(p->lmk_k_exp, op - lay[2]);
+        std::vector<tdi_lmk_report> rep(n);
+        if (kl > 0) rc = tdi_host::landmark_fit(b, n, K, kl, th.data(), rep.data());
+        for (int f = 0; !rc && f < n; ++f) {
+            if (!ok[f] || (kl > 0 && rep[f].status != TDI_OK)) continue;
+            float R[9], ypr[3];
+            tdi_host::rotation_f32(&th[(size_t)f * K + op + 3], R);
+            tdi_host::compute_euler_angles(R, ypr[0], ypr[1], ypr[2]);
+            yaw[f] = ypr[0]; pitch[f] = ypr[1]; roll[f] = ypr[2];
+        }
+        tdi_batch_destroy(b);
+    }
+    (void)hipFree(one);
+    if (rc) return rc;
+    if (angles)
+        for (int f = 0; f < n; ++f) { angles[3 * (size_t)f] = yaw[f]; angles[3 * (size_t)f + 1] = pitch[f]; angles[3 * (size_t)f + 2] = roll[f]; }
+    return tdi_bin_identity_frames(p, n, yaw.data(), pitch.data(), selected, cap, n_selected);
+}
+
+} // extern "C"
+
+extern "C" {
+
+int tdi_learn_identity(tdi_ctx* ctx, const tdi_identity_params* p, int32_t n, uint8_t* bgr, int32_t frame_w,
+                       int32_t frame_h, int64_t frame_stride, const float* lmk_x, const float* lmk_y, const float* cam,
+                       float* alpha, float* beta, int32_t* n_used)
+{
+    if (!ctx || !p || n < 0 || (n > 0 && (!bgr || !lmk_x || !lmk_y)) || !cam || !n_used || p->nframes < 1 ||
+        frame_w < 1 || frame_h < 1)
+        return TDI_ERR_ARG;
+    const int NL = TDI_NLANDMARKS, nf = p->nframes;
+    int32_t lay[6];
+    int rc = tdi_ctx_layout(ctx, lay);
+    if (rc) return rc;
+    const int off_tex = lay[0], off_id = lay[1], off_exp = lay[2], off_pose = lay[3], K = lay[5];
+    const int ntex = off_id - off_tex, nid = off_exp - off_id;
+    if ((ntex && !beta) || (nid && !alpha)) return TDI_ERR_ARG;
+    // learn_identity's sums in FP32: the first used fit assigns, the others add (1962-1982)
+    std::vector<float> sa(nid, 0.f), sb(ntex, 0.f);
+    int used = 0;
+    std::vector<float> cams_in((size_t)nf * 4);
+    for (int f = 0; f < nf; ++f)
+        for (int a = 0; a < 4; ++a) cams_in[4 * (size_t)f + a] = cam[a];
+    std::vector<float> box_lo(NL, -p->box_half_width), box_hi(NL, p->box_half_width);
+    for (int g = 0; g < p->ntot_reconstrs && (g + 1) * nf <= n; ++g) {
+        const int g0 = g * nf;
+        const float* gx = lmk_x + (size_t)g0 * NL;
+        const float* gy = lmk_y + (size_t)g0 * NL;
+        uint8_t* gb = bgr + (size_t)g0 * frame_stride;
+        // fit_multiframe (video_fitter.cu:2023-2083): each frame's camera scaled to REF_FACE_SIZE;
+        // fit_to_multi_images (model_fitter.cu:1252-1479): grey, inner mouth, resize, crop
+        std::vector<float> coef(nf), lx((size_t)nf * NL), ly((size_t)nf * NL), cam2((size_t)nf * 4), fsz(nf);
+        std::vector<int32_t> crop((size_t)nf * 4), st(nf);
+        rc = tdi_frame_geometry(nf, gx, gy, cams_in.data(), frame_w, frame_h, p->ref_face_size, p->pad, p->crop_w,
+                                p->crop_h, coef.data(), lx.data(), ly.data(), cam2.data(), crop.data(), fsz.data(),
+                                st.data());
+        if (rc) return rc;
+        bool all_in = true;
+        for (int f = 0; f < nf; ++f) all_in = all_in && st[f] == TDI_OK;
+        if (!all_in) continue;
+        if (p->paint_innermouth_black &&
+            (rc = tdi_paint_innermouth_black(nf, gb, frame_w, frame_h, frame_stride, gx, gy, p->ref_face_size, nullptr)))
+            return rc;
+        long long pixels = 0;
+        for (int f = 0; f < nf; ++f) pixels += (long long)crop[4 * (size_t)f + 2] * crop[4 * (size_t)f + 3];
+        float* dcrops = nullptr;
+        if (hipMalloc((void**)&dcrops, sizeof(float) * (size_t)pixels) != hipSuccess) return TDI_ERR_NOMEM;
+        rc = tdi_prepare_crops(nf, gb, frame_w, frame_h, frame_stride, coef.data(), crop.data(), dcrops, nullptr);
+        std::vector<tdi_frame_desc> fd(nf);
+        long long off = 0;
+        for (int f = 0; f < nf; ++f) {
+            tdi_frame_desc& d = fd[f];
+            std::memset(&d, 0, sizeof(d));
+            d.phix = cam2[4 * (size_t)f]; d.phiy = cam2[4 * (size_t)f + 1];
+            d.cx = cam2[4 * (size_t)f + 2]; d.cy = cam2[4 * (size_t)f + 3];
+            d.x0 = crop[4 * (size_t)f]; d.y0 = crop[4 * (size_t)f + 1];
+            d.width = crop[4 * (size_t)f + 2]; d.height = crop[4 * (size_t)f + 3];
+            d.face_size = fsz[f];
+            d.obs = dcrops + off;
+            off += (long long)d.width * d.height;
+            d.lmk_x = &lx[(size_t)f * NL];
+            d.lmk_y = &ly[(size_t)f * NL];
+            d.box_lx = box_lo.data(); d.box_ux = box_hi.data(); d.box_ly = box_lo.data(); d.box_uy = box_hi.data();
+            d.lambda[0] = -7.3627f; d.lambda[1] = 51.1364f; d.lambda[2] = 100.1784f;
+            d.Lintensity = 0.005f;
+        }
+        tdi_batch* b = nullptr;
+        if (!rc) rc = tdi_batch_create(ctx, nf, fd.data(), &b);
+        bool success = false;
+        std::vector<float> th((size_t)nf * K, 0.f);
+        if (!rc) {
+            // the start: each frame's orthographic pose, then its landmark fit with slack (identity
+            // and albedo at the mean; the reference fits the identity jointly there)
+            std::vector<float> pose((size_t)nf * 6);
+            rc = tdi_host::initialize_with_orthographic_t(ctx, nf, lx.data(), ly.data(), cam2.data(), pose.data(), nullptr,
+                                                          nullptr);
+            for (int f = 0; !rc && f < nf; ++f)
+                for (int a = 0; a < 6; ++a) th[(size_t)f * K + off_pose + a] = pose[(size_t)f * 6 + a];
+            const int kl = std::min(p->lmk_k_exp, off_pose - off_exp);
+            if (!rc && kl > 0) {
+                std::vector<float> tt(th);
+                std::vector<tdi_lmk_report> lrep(nf);
+                rc = tdi_host::landmark_fit(b, nf, K, kl, tt.data(), lrep.data());
+                for (int f = 0; !rc && f < nf; ++f)
+                    if (lrep[f].status == TDI_OK) std::memcpy(&th[(size_t)f * K], &tt[(size_t)f * K], sizeof(float) * K);
+            }
+            // the light sub-fits of fit_to_multi_images (model_fitter.cu:1374-1386)
+            std::vector<float> lam((size_t)nf * 3), Lint(nf, 0.005f);
+            for (int f = 0; f < nf; ++f) { lam[3 * (size_t)f] = -7.3627f; lam[3 * (size_t)f + 1] = 51.1364f; lam[3 * (size_t)f + 2] = 100.1784f; }
+            if (!rc) rc = tdi_batch_set_light(b, lam.data(), Lint.data());
+            for (int k = 0; !rc && k < 2; ++k) {
+                rc = tdi_host::light_fit(b, nf, th.data(), lam.data(), Lint.data(), true, nullptr);
+                if (!rc) rc = tdi_host::light_fit(b, nf, th.data(), lam.data(), Lint.data(), false, nullptr);
+            }
+            // the joint shape fit over the group's frames
+            tdi_fit_report rep;
+            if (!rc) rc = tdi_fit_multiframe(b, th.data(), &rep);
+            success = !rc && rep.success && rep.status == TDI_OK;
+        }
+        if (b) tdi_batch_destroy(b);
+        (void)hipFree(dcrops);
+        if (rc) return rc;
+        if (success) {
+            ++used;
+            for (int k = 0; k < nid; ++k) sa[k] = used == 1 ? th[off_id + k] : sa[k] + th[off_id + k];
+            for (int k = 0; k < ntex; ++k) sb[k] = used == 1 ? th[off_tex + k] : sb[k] + th[off_tex + k];
+        }
+    }
+    *n_used = used;
+    if (used) {
+        // the average over the groups (learn_identity's running sum times 1 / num_used_recs)
+        const float w = (float)1. / (float)used;
+        for (int k = 0; k < nid; ++k) alpha[k] = sa[k] * w;
+        for (int k = 0; k < ntex; ++k) beta[k] = sb[k] * w;
+    }
+    return TDI_OK;
 }
 
 } // extern "C"

@@ -51,6 +51,12 @@ def lib():
         L.tdi_read_video_config.argtypes = [C.c_char_p, _vp, _vp]
         L.tdi_fit_video_frames_nres.argtypes = [_vp, _vp, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp,
                                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.tdi_default_identity_params.argtypes = [_vp]
+        L.tdi_bin_identity_frames.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp]
+        L.tdi_select_identity_frames.argtypes = [_vp, _vp, C.c_int32, _vp, _vp, C.c_int32, C.c_int32, _vp, _vp, _vp,
+                                                 C.c_int32, _vp, _vp]
+        L.tdi_learn_identity.argtypes = [_vp, _vp, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp,
+                                         _vp, _vp]
         _lib = L
     return _lib
 
@@ -336,3 +342,80 @@ def prepare_frames(frames, lmk_x, lmk_y, cams, ref_face_size: float, pad: bool =
     else:
         g["obs"] = None
     return g
+
+
+# ------------------------------------------------------------------------ identity stage
+class IdentityParams(C.Structure):
+    """tdi_identity_params (include/tdi_host.h)."""
+    _fields_ = [("nframes", C.c_int32), ("nframes_per_angle_bin", C.c_int32), ("nperms", C.c_int32),
+                ("ntot_reconstrs", C.c_int32), ("nmax_frames", C.c_int32), ("max_vid_frames", C.c_int32),
+                ("skip_first_n_secs", C.c_float), ("fps", C.c_float), ("ref_face_size", C.c_float),
+                ("pad", C.c_int32), ("crop_w", C.c_int32), ("crop_h", C.c_int32),
+                ("paint_innermouth_black", C.c_int32), ("lmk_k_exp", C.c_int32), ("box_half_width", C.c_float)]
+
+
+def identity_params(**kw) -> IdentityParams:
+    """The identity stage's parameters, the reference's defaults (config.cu) overridden by kw."""
+    p = IdentityParams()
+    lib().tdi_default_identity_params(C.addressof(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def bin_identity_frames(p, yaw, pitch, cap=None):
+    """tdi_bin_identity_frames: selected frame indices (groups of p.nframes) from per-frame yaw and
+    pitch in radians (NaN: no angle)."""
+    yaw = np.ascontiguousarray(yaw, np.float32)
+    pitch = np.ascontiguousarray(pitch, np.float32)
+    n = yaw.size
+    cap = cap or max(1, p.ntot_reconstrs * p.nframes)
+    out = np.zeros(cap, np.int32)
+    ns = C.c_int32(0)
+    rc = lib().tdi_bin_identity_frames(C.addressof(p), n, yaw.ctypes.data, pitch.ctypes.data, out.ctypes.data, cap,
+                                       C.addressof(ns))
+    if rc:
+        raise tdi.TdiError(f"tdi_bin_identity_frames: {rc}")
+    return out[:ns.value].copy()
+
+
+def select_identity_frames(ctx, p, lmk_x, lmk_y, width, height, cam):
+    """tdi_select_identity_frames: (selected frame indices, angles [n, 3] radians, resize coefficient)."""
+    lx = np.ascontiguousarray(lmk_x, np.float32)
+    ly = np.ascontiguousarray(lmk_y, np.float32)
+    n = lx.shape[0]
+    cm = np.ascontiguousarray(cam, np.float32).reshape(4)
+    ang = np.zeros((n, 3), np.float32)
+    cap = max(1, p.ntot_reconstrs * p.nframes)
+    out = np.zeros(cap, np.int32)
+    ns = C.c_int32(0)
+    coef = C.c_float(0)
+    rc = lib().tdi_select_identity_frames(ctx.h, C.addressof(p), n, lx.ctypes.data, ly.ctypes.data, int(width),
+                                          int(height), cm.ctypes.data, ang.ctypes.data, out.ctypes.data, cap,
+                                          C.addressof(ns), C.addressof(coef))
+    if rc:
+        raise tdi.TdiError(f"tdi_select_identity_frames: {rc}: {tdi.lib().tdi_last_error().decode()}")
+    return out[:ns.value].copy(), ang, coef.value
+
+
+def learn_identity(ctx, p, frames, lmk_x, lmk_y, cam):
+    """tdi_learn_identity over the selected frames (uint8 CUDA tensor [n, H, W, 3], contiguous, in
+    selection order; masked in place): (alpha [k_id], beta [k_tex], groups used)."""
+    n, H, W = (int(v) for v in frames.shape[:3])
+    if not frames.is_contiguous():
+        raise ValueError("learn_identity: frames must be a contiguous uint8 CUDA tensor [n, H, W, 3]")
+    lx = np.ascontiguousarray(lmk_x, np.float32).reshape(n, 51)
+    ly = np.ascontiguousarray(lmk_y, np.float32).reshape(n, 51)
+    cm = np.ascontiguousarray(cam, np.float32).reshape(4)
+    m = ctx.model
+    alpha = np.zeros(max(1, m.k_id), np.float32)
+    beta = np.zeros(max(1, m.k_tex), np.float32)
+    used = C.c_int32(0)
+    rc = lib().tdi_learn_identity(ctx.h, C.addressof(p), n, frames.data_ptr(), W, H, W * H * 3, lx.ctypes.data,
+                                  ly.ctypes.data, cm.ctypes.data, alpha.ctypes.data, beta.ctypes.data,
+                                  C.addressof(used))
+    if rc:
+        raise tdi.TdiError(f"tdi_learn_identity: {rc}: {tdi.lib().tdi_last_error().decode()}")
+    return alpha[:m.k_id], beta[:m.k_tex], used.value

@@ -158,6 +158,9 @@ typedef struct tdi_batch tdi_batch;   /* frames resident in HBM + per-frame scra
 
 /* Number of unknowns per frame for a model/options pair. */
 int32_t tdi_num_unknowns(const tdi_model_desc* model, const tdi_fit_options* opts);
+/* The unknown-vector layout of a context: layout[0..5] = offsets of the albedo, identity,
+ * expression, pose and light blocks, then K. */
+int tdi_ctx_layout(tdi_ctx* ctx, int32_t* layout);
 
 /* Fill *opts with the reference defaults (config.cu:15-140, model_fitter.cu:137-345). */
 void tdi_default_options(tdi_fit_options* opts);

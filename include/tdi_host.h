@@ -146,6 +146,69 @@ int tdi_fit_video_long(tdi_ctx* ctx, const tdi_video_params* p, const tdi_long_v
 int tdi_vo_dump(tdi_video_output* vo, const char* path);
 int tdi_vo_load(tdi_video_output* vo, const char* path);
 
+/* ---- Identity stage: frame selection and identity learning (VideoFitter::learn_identity,
+ *      video_fitter.cu:1925-2016; fit_video_frames_landmarks_sparse, 2086-2583) ---- */
+typedef struct tdi_identity_params {
+    int32_t nframes;                /* NFRAMES (config.cu:45: 7): frames of one multi-frame fit        */
+    int32_t nframes_per_angle_bin;  /* NFRAMES_PER_ANGLE_BIN (config.cu:79: 14)                        */
+    int32_t nperms;                 /* NPERMS (config.cu:27: 10): shuffled passes over the binned frames */
+    int32_t ntot_reconstrs;         /* NTOT_RECONSTRS (config.cu:29: 6): multi-frame fits averaged       */
+    int32_t nmax_frames;            /* NMAX_FRAMES (config.cu:28: 432000)                              */
+    int32_t max_vid_frames;         /* MAX_VID_FRAMES_TO_PROCESS (config.cu:59: 50000)                 */
+    float skip_first_n_secs;        /* SKIP_FIRST_N_SECS (config.cu:55: 0)                             */
+    float fps;                      /* the video's frame rate (for skip_first_n_secs)                  */
+    float ref_face_size;            /* REF_FACE_SIZE of the multi-frame fits' frames (config.cu:18: 86) */
+    int32_t pad;                    /* PAD_RENDERING (config.cu:43: 1)                                 */
+    int32_t crop_w, crop_h;         /* DIMX, DIMY (constants.h:55-56: 256)                             */
+    int32_t paint_innermouth_black; /* PAINT_INNERMOUTH_BLACK (config.cu:92: 1)                        */
+    int32_t lmk_k_exp;              /* K_EPSILON_L (config.cu:112: 79): expression coefficients of the
+                                     * landmark fits (the reference also fits K_ALPHA_L = 60 identity
+                                     * coefficients there; these fits hold the identity at the mean)   */
+    float box_half_width;           /* landmark boxes (face-size units): 0.15                          */
+} tdi_identity_params;
+
+void tdi_default_identity_params(tdi_identity_params* p);
+
+/* The binning of fit_video_frames_landmarks_sparse (video_fitter.cu:2380-2550), host arithmetic:
+ * yaw, pitch [n] (radians; NaN: the frame has no angle) are binned to the nearest of the 13 x 13
+ * centroids -55, -45, ..., 65 degrees; a std::default_random_engine seeded with 1907 shuffles every
+ * bin (yaw-major order) and at most nframes_per_angle_bin frames of each are kept; the kept frames
+ * in frame order are then shuffled nperms times by the same engine and concatenated; groups of
+ * nframes consecutive entries of that order are taken until ntot_reconstrs groups (or nmax_frames
+ * frames).  selected [cap] receives the frame indices in group order, *n_selected their count (a
+ * multiple of nframes).  The reference accepts a group when a joint landmark-only fit of its
+ * frames succeeds (fit_to_multi_images_landmarks_only); here every frame with an angle has passed
+ * its own landmark fit, and every group is accepted. */
+int tdi_bin_identity_frames(const tdi_identity_params* p, int32_t n, const float* yaw, const float* pitch,
+                            int32_t* selected, int32_t cap, int32_t* n_selected);
+
+/* The whole selection of fit_video_frames_landmarks_sparse from a video's landmarks: the frames
+ * considered (every max(1, n / 800)-th, after skip_first_n_secs, before max_vid_frames, with all
+ * 51 landmarks inside the frame), the median face size m of those (resize coefficient 76 / m,
+ * returned in *resize_coef), then for every frame the resized landmarks' bounding-box checks
+ * (check_if_bb_OK, check_if_face_in_frame, funcs.cu:582-626) and the orthographic start and
+ * landmark-only fit with slack on the GPU (all frames in one launch each) giving yaw, pitch, roll
+ * (angles [n][3], radians, NaN where there is none; may be NULL), then tdi_bin_identity_frames.
+ * ctx: the identity stage's context (fit_id = fit_tex = 1).  lmk_x, lmk_y [n][51] image pixels of
+ * frame_w x frame_h frames, cam [4] (phix, phiy, cx, cy). */
+int tdi_select_identity_frames(tdi_ctx* ctx, const tdi_identity_params* p, int32_t n, const float* lmk_x,
+                               const float* lmk_y, int32_t frame_w, int32_t frame_h, const float* cam, float* angles,
+                               int32_t* selected, int32_t cap, int32_t* n_selected, float* resize_coef);
+
+/* learn_identity (video_fitter.cu:1925-2016) over the selected frames: for each group of nframes
+ * consecutive frames (at most ntot_reconstrs), fit_multiframe (2023-2083) — each frame resized to
+ * ref_face_size, masked and cropped as in the per-frame stage, the start from each frame's
+ * orthographic pose and landmark fit (identity and albedo at the mean), the light sub-fits of
+ * fit_to_multi_images (L, lambda, L, lambda per frame, model_fitter.cu:1374-1386), then the
+ * joint fit (tdi_fit_multiframe); alpha [k_id] and beta [k_tex] are the averages over the groups
+ * whose joint fit succeeded (more than NITERS_SUCCESS iterations), *n_used their count (0: alpha
+ * and beta untouched, TDI_OK returned).  ctx: fit_id = fit_tex = 1, fit_light = 0 (the light is held
+ * by the sub-fits, as in the reference).  bgr: DEVICE [n][frame_h][frame_w][3] the selected frames
+ * in selection order (masked in place when paint_innermouth_black); lmk_x, lmk_y [n][51]. */
+int tdi_learn_identity(tdi_ctx* ctx, const tdi_identity_params* p, int32_t n, uint8_t* bgr, int32_t frame_w,
+                       int32_t frame_h, int64_t frame_stride, const float* lmk_x, const float* lmk_y, const float* cam,
+                       float* alpha, float* beta, int32_t* n_used);
+
 /* ---- Frame preparation (host side; the pixels are tdi_prepare_crops in tdi_api.h) ---- */
 
 /* Landmark text file, one frame per line "x0 y0 x1 y1 ... x50 y50", as
