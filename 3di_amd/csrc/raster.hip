@@ -39,27 +39,36 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, long long n, uint64_t v)
 // RN(nb/den)) <= 1.  The sum is first formed from na * rcp(den) and nb * rcp(den): v_rcp_f32
 // is within 1 ulp, so that sum is within 1.5 * 2^-22 * (alpha + beta) of the correctly rounded
 // one, and only samples whose approximate sum lies within 2^-18 of 1 (or with den outside
-// [2^-60, 2^60]) take the two IEEE divisions.  The samples are scanned x-major in one loop, so
-// a wavefront runs max(nx * ny) trips rather than nested maxima.
-__device__ __forceinline__ void raster_frame(const FrameGPU& fr, float4 B, float4 Cc, float4 A, int t, int ntmp,
-                                             unsigned long long* __restrict__ keys)
+// [2^-60, 2^60]) take the two IEEE divisions.  The samples are scanned x-major: sample i of the
+// scan box is (xl + i / ny, yl + i % ny).
+struct RasterTri {
+    float Ax, Ay, v0x, v0y, v1x, v1y, d00, d01, d11, den, q;
+    int xl, yl, ny;
+    bool den_ok, dpos, dneg;
+};
+
+// The scan box and the edge quantities of a triangle; returns the number of samples to test
+// (0: none can be covered).
+__device__ __forceinline__ int raster_setup(const FrameGPU& fr, float4 B, float4 Cc, float4 A, RasterTri& r)
 {
 #pragma clang fp contract(off)
     const float xmin = floorf(fminf(fminf(B.x, Cc.x), A.x));
     const float xmax = ceilf(fmaxf(fmaxf(B.x, Cc.x), A.x));
     const float ymin = floorf(fminf(fminf(B.y, Cc.y), A.y));
     const float ymax = ceilf(fmaxf(fmaxf(B.y, Cc.y), A.y));
-    if (!(xmin == xmin) || !(xmax == xmax) || !(ymin == ymin) || !(ymax == ymax)) return;
+    if (!(xmin == xmin) || !(xmax == xmax) || !(ymin == ymin) || !(ymax == ymax)) return 0;
     // clip the scan to the crop window [x0, x0+W) x [y0, y0+H); the clipped bounds are integers
     float xlo = fmaxf(xmin, (float)fr.x0), xhi = fminf(xmax, (float)(fr.x0 + fr.W));
     float ylo = fmaxf(ymin, (float)fr.y0), yhi = fminf(ymax, (float)(fr.y0 + fr.H));
-    if (!(xlo < xhi) || !(ylo < yhi)) return;
-    const float v0x = B.x - A.x, v1x = Cc.x - A.x;
-    const float v0y = B.y - A.y, v1y = Cc.y - A.y;
-    const float d00 = v0x * v0x + v0y * v0y;
-    const float d01 = v0x * v1x + v0y * v1y;
-    const float d11 = v1x * v1x + v1y * v1y;
-    const float den = d00 * d11 - d01 * d01;
+    if (!(xlo < xhi) || !(ylo < yhi)) return 0;
+    r.Ax = A.x;
+    r.Ay = A.y;
+    r.v0x = B.x - A.x; r.v1x = Cc.x - A.x;
+    r.v0y = B.y - A.y; r.v1y = Cc.y - A.y;
+    r.d00 = r.v0x * r.v0x + r.v0y * r.v0y;
+    r.d01 = r.v0x * r.v1x + r.v0y * r.v1y;
+    r.d11 = r.v1x * r.v1x + r.v1y * r.v1y;
+    r.den = r.d00 * r.d11 - r.d01 * r.d01;
     {
         // The [floor, ceil) box holds samples outside the triangle's own extent, which only a
         // near-degenerate triangle can cover through rounding (brute force over millions of
@@ -69,50 +78,76 @@ __device__ __forceinline__ void raster_frame(const FrameGPU& fr, float4 B, float
         // by 1e-3 px; the decisions for every sample are unchanged.
         const float exn = fmaxf(fmaxf(B.x, Cc.x), A.x) - fminf(fminf(B.x, Cc.x), A.x);
         const float eyn = fmaxf(fmaxf(B.y, Cc.y), A.y) - fminf(fminf(B.y, Cc.y), A.y);
-        if (exn <= 4.f && eyn <= 4.f && d00 > 0.f && d11 > 0.f && den >= 1e-4f * (d00 * d11)) {
+        if (exn <= 4.f && eyn <= 4.f && r.d00 > 0.f && r.d11 > 0.f && r.den >= 1e-4f * (r.d00 * r.d11)) {
             xlo = fmaxf(xlo, ceilf(fminf(fminf(B.x, Cc.x), A.x) - 1e-3f));
             xhi = fminf(xhi, floorf(fmaxf(fmaxf(B.x, Cc.x), A.x) + 1e-3f) + 1.f);
             ylo = fmaxf(ylo, ceilf(fminf(fminf(B.y, Cc.y), A.y) - 1e-3f));
             yhi = fminf(yhi, floorf(fmaxf(fmaxf(B.y, Cc.y), A.y) + 1e-3f) + 1.f);
-            if (!(xlo < xhi) || !(ylo < yhi)) return;
+            if (!(xlo < xhi) || !(ylo < yhi)) return 0;
         }
     }
-    const int xl = (int)xlo, yl = (int)ylo, ny = (int)yhi - yl;
-    const int nxy = ((int)xhi - xl) * ny;
-    const unsigned long long key = ((unsigned long long)orderable_bits(B.z) << 32) | (unsigned)t;
-    unsigned long long* kf = keys + fr.pix_off;
-    const float q = __builtin_amdgcn_rcpf(den);
-    const bool den_ok = fabsf(den) >= 0x1p-60f && fabsf(den) <= 0x1p60f;
-    const bool dpos = den > 0.f, dneg = den < 0.f;
+    r.xl = (int)xlo;
+    r.yl = (int)ylo;
+    r.ny = (int)yhi - r.yl;
+    r.q = __builtin_amdgcn_rcpf(r.den);
+    r.den_ok = fabsf(r.den) >= 0x1p-60f && fabsf(r.den) <= 0x1p60f;
+    r.dpos = r.den > 0.f;
+    r.dneg = r.den < 0.f;
+    return ((int)xhi - r.xl) * r.ny;
+}
+
+__device__ __forceinline__ bool raster_covers(const RasterTri& r, int x, int y)
+{
+#pragma clang fp contract(off)
+    const float v2x = (float)x - r.Ax, v2y = (float)y - r.Ay;
+    const float d20 = r.v0x * v2x + r.v0y * v2y;
+    const float d21 = r.v1x * v2x + r.v1y * v2y;
+    const float na = r.d11 * d20 - r.d01 * d21;
+    const float nb = r.d00 * d21 - r.d01 * d20;
+    const bool pa = (na > 0.f && r.dpos) || (na < 0.f && r.dneg);
+    const bool pb = (nb > 0.f && r.dpos) || (nb < 0.f && r.dneg);
+    const float s1 = na * r.q + nb * r.q;
+    bool cov = pa && pb && s1 < 1.f;
+    if (pa && pb && !(r.den_ok && fabsf(s1 - 1.f) > 0x1p-18f)) {
+        const float a = na / r.den, b = nb / r.den;
+        cov = a > 0.f && b > 0.f && (a + b <= 1.f);
+    }
+    return cov;
+}
+
+// Every sample of the scan box in order, at most ntmp covered ones (ntmp <= 0: all).
+__device__ __forceinline__ void raster_loop(const FrameGPU& fr, const RasterTri& r, int nxy, unsigned long long key,
+                                            int ntmp, unsigned long long* __restrict__ kf)
+{
     int covered = 0;
-    int x = xl, y = yl;
-    int off = (xl - fr.x0) * fr.H + (yl - fr.y0);
+    int x = r.xl, y = r.yl;
+    int off = (r.xl - fr.x0) * fr.H + (r.yl - fr.y0);
     for (int i = 0; i < nxy; ++i) {
-        const float v2x = (float)x - A.x, v2y = (float)y - A.y;
-        const float d20 = v0x * v2x + v0y * v2y;
-        const float d21 = v1x * v2x + v1y * v2y;
-        const float na = d11 * d20 - d01 * d21;
-        const float nb = d00 * d21 - d01 * d20;
-        const bool pa = (na > 0.f && dpos) || (na < 0.f && dneg);
-        const bool pb = (nb > 0.f && dpos) || (nb < 0.f && dneg);
-        const float s1 = na * q + nb * q;
-        bool cov = pa && pb && s1 < 1.f;
-        if (pa && pb && !(den_ok && fabsf(s1 - 1.f) > 0x1p-18f)) {
-            const float a = na / den, b = nb / den;
-            cov = a > 0.f && b > 0.f && (a + b <= 1.f);
-        }
-        if (cov && (ntmp <= 0 || covered < ntmp)) {
+        if (raster_covers(r, x, y) && (ntmp <= 0 || covered < ntmp)) {
             ++covered;
             atomicMin(kf + off, key);
         }
         ++y;
         ++off;
-        if (y - yl == ny) {
-            y = yl;
+        if (y - r.yl == r.ny) {
+            y = r.yl;
             ++x;
-            off += fr.H - ny;
+            off += fr.H - r.ny;
         }
     }
+}
+
+__device__ __forceinline__ unsigned long long tri_key(float z, int t)
+{
+    return ((unsigned long long)orderable_bits(z) << 32) | (unsigned)t;
+}
+
+__device__ __forceinline__ void raster_frame(const FrameGPU& fr, float4 B, float4 Cc, float4 A, int t, int ntmp,
+                                             unsigned long long* __restrict__ keys)
+{
+    RasterTri r;
+    const int nxy = raster_setup(fr, B, Cc, A, r);
+    if (nxy > 0) raster_loop(fr, r, nxy, tri_key(B.z, t), ntmp, keys + fr.pix_off);
 }
 
 __global__ void __launch_bounds__(256) k_raster(const int* __restrict__ tris, int NT, int V,
