@@ -103,16 +103,15 @@ __global__ void __launch_bounds__(256) k_synth_gemv(ModelGPU m, Layout L, const 
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// Vertex tiles for v_mfma_f64_16x16x4f64 arranged so that a lane's accumulator holds one
-// vertex: tile row i = 4 c + q is coordinate c (x, y, z; c = 3 is a zero row) of vertex q of a
-// 4-vertex row block, so lane l, element c holds coordinate c of vertex l/16 for frame l%16.
-// A workgroup owns SY_VERTS vertices: their basis rows are staged in LDS (as FP64) once; then
-// each wavefront walks its own groups of 16 list entries with no further barrier.  A lane
-// loads the coefficients of its frame for its four k residues up front (one batch of
-// independent loads), and the epilogue (mean, FP64 rotation, translation, projection) runs
-// from registers.
-constexpr int SY_RB = 4;                   // 4-vertex row blocks per wavefront
-constexpr int SY_VERTS = 4 * SY_RB;        // vertices per workgroup
+// Vertex tiles for v_mfma_f64_16x16x4f64: a workgroup owns SY_VERTS = 16 vertices and keeps one
+// accumulator per coordinate, whose tile row i is vertex i (A = that coordinate's basis rows of the
+// 16 vertices, B = the coefficients of 16 frames), so lane l, element r of the x, y and z
+// accumulators holds vertex l/16 + 4 r for frame l%16: every MFMA row is a vertex coordinate
+// (three MFMAs per k-step for 16 vertices).  The basis rows are staged in LDS (as FP64) once; then
+// each wavefront walks its own groups of 16 list entries with no further barrier.  A lane loads
+// the coefficients of its frame for its four k residues up front (one batch of independent loads),
+// and the epilogue (mean, FP64 rotation, translation, projection) runs from registers.
+constexpr int SY_VERTS = 16;               // vertices per workgroup
 constexpr int SY_KMAX = 80;                // expression coefficients held in LDS (K_exp <= 80)
 
 // Three wavefronts per SIMD (register budget 168) hide the coefficient loads of one behind the
@@ -122,32 +121,32 @@ k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const fl
              const int* __restrict__ act, int n, const double* __restrict__ rt, double* __restrict__ P0,
              float* __restrict__ P, float4* __restrict__ vproj)
 {
-    __shared__ double sA[SY_RB][SY_KMAX][16];          // [row block][k][tile row]
+    __shared__ double sA[3][SY_KMAX][16];              // [coordinate][k][vertex]
     __shared__ float smean[SY_VERTS * 3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (fl_count(act, n) == 0) return;       // an emptied list: skip the basis staging
     const int v0 = blockIdx.x * SY_VERTS;
     const int Ke = L.nexp, ke = m.exp_ld;
     const int K4 = (Ke + 3) & ~3;
-    // basis rows of the workgroup's vertices, zero padded (coordinate 3, k >= Ke, v >= V)
+    // basis rows of the workgroup's vertices, zero padded (k >= Ke, v >= V)
     // (a fixed count of loads per thread, issued unconditionally from clamped indices and zeroed
     // by a select, so that they are in flight together)
-    constexpr int NST = SY_RB * SY_KMAX * 16 / 256;
+    constexpr int NST = 3 * SY_KMAX * 16 / 256;
     float av[NST];
 #pragma unroll
     for (int q = 0; q < NST; ++q) {
         const int e = tid + 256 * q;
-        const int k = e % K4, i = (e / K4) & 15, rb = (e / K4) >> 4;
-        const int c = i >> 2, v = v0 + 4 * rb + (i & 3);
+        const int k = e % K4, i = (e / K4) & 15, c = (e / K4) >> 4;
+        const int v = v0 + i;
         av[q] = m.exp_basis[(size_t)(3 * min(v, m.V - 1) + min(c, 2)) * ke + min(k, Ke - 1)];
     }
 #pragma unroll
     for (int q = 0; q < NST; ++q) {
         const int e = tid + 256 * q;
-        if (e < SY_RB * K4 * 16) {
-            const int k = e % K4, i = (e / K4) & 15, rb = (e / K4) >> 4;
-            const int c = i >> 2, v = v0 + 4 * rb + (i & 3);
-            sA[rb][k][i] = (c < 3 && k < Ke && v < m.V) ? (double)av[q] : 0.0;
+        if (e < 3 * K4 * 16) {
+            const int k = e % K4, i = (e / K4) & 15, c = (e / K4) >> 4;
+            const int v = v0 + i;
+            sA[c][k][i] = (k < Ke && v < m.V) ? (double)av[q] : 0.0;
         }
     }
     for (int e = tid; e < SY_VERTS * 3; e += blockDim.x) {
@@ -170,16 +169,16 @@ k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const fl
         float bv[SY_KMAX / 4];
 #pragma unroll
         for (int q = 0; q < SY_KMAX / 4; ++q) bv[q] = tb[min(kq + 4 * q, Ke - 1)];
-        v4d acc[SY_RB];
+        v4d acc[3];
 #pragma unroll
-        for (int rb = 0; rb < SY_RB; ++rb) acc[rb] = (v4d){0, 0, 0, 0};
+        for (int a = 0; a < 3; ++a) acc[a] = (v4d){0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < SY_KMAX / 4; ++q) {
             if (4 * q < K4) {
                 const double b = (double)bv[q];
 #pragma unroll
-                for (int rb = 0; rb < SY_RB; ++rb)
-                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[rb][4 * q + kq][j], b, acc[rb], 0, 0, 0);
+                for (int a = 0; a < 3; ++a)
+                    acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[a][4 * q + kq][j], b, acc[a], 0, 0, 0);
             }
         }
         if (live) {
@@ -188,12 +187,12 @@ k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const fl
 #pragma unroll
             for (int q = 0; q < 12; ++q) R[q] = rt[(size_t)f * 12 + q];
 #pragma unroll
-            for (int rb = 0; rb < SY_RB; ++rb) {
-                const int vl = 4 * rb + kq, v = v0 + vl;
+            for (int r = 0; r < 4; ++r) {
+                const int vl = kq + 4 * r, v = v0 + vl;
                 if (v < m.V) {
                     double p[3];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) p[a] = (double)smean[3 * vl + a] + acc[rb][a];
+                    for (int a = 0; a < 3; ++a) p[a] = (double)smean[3 * vl + a] + acc[a][r];
                     store_vertex(fr, R, R + 9, p, (size_t)f * m.V + v, P0, P, vproj);
                 }
             }
