@@ -1013,11 +1013,10 @@ int tdi_fit_multiframe(tdi_batch* b, float* theta_host, tdi_fit_report* report)
     float* dth = A.get<float>((size_t)n * K);
     double* HJ = A.get<double>((size_t)KJ * KJ);
     double* gJ = A.get<double>(KJ);
-    double* y = A.get<double>(KJ);
     float* dJ = A.get<float>(KJ);
     int* st = A.get<int>(1);
     double* objs = A.get<double>(1);
-    if (!dth || !HJ || !gJ || !y || !dJ || !st || !objs) { set_error("tdi_fit_multiframe: allocation failed"); return TDI_ERR_NOMEM; }
+    if (!dth || !HJ || !gJ || !dJ || !st || !objs) { set_error("tdi_fit_multiframe: allocation failed"); return TDI_ERR_NOMEM; }
     auto fail = [](int e) { return e; };
     if ((rc = ws_load(b, 0, n, s))) return fail(rc);
     // joint vector from the frames' thetas (shared block from frame 0)
@@ -1055,7 +1054,7 @@ int tdi_fit_multiframe(tdi_batch* b, float* theta_host, tdi_fit_report* report)
         TDI_HIP(hipMemcpyAsync(ob.data(), w.obj, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         TDI_HIP(hipMemcpyAsync(np.data(), w.npix, sizeof(int) * n, hipMemcpyDeviceToHost, s));
         TDI_HIP(hipMemcpyAsync(&os, objs, sizeof(double), hipMemcpyDeviceToHost, s));
-        if ((rc = launch_cholesky_big_f64(1, KJ, HJ, gJ, y, dJ, st, s))) return fail(rc);
+        if ((rc = launch_cholesky_big_f64(1, KJ, HJ, gJ, dJ, st, s))) return fail(rc);
         int sst = 0;
         TDI_HIP(hipMemcpyAsync(&sst, st, sizeof(int), hipMemcpyDeviceToHost, s));
         TDI_HIP(hipMemcpyAsync(delta.data(), dJ, sizeof(float) * KJ, hipMemcpyDeviceToHost, s));

@@ -372,9 +372,8 @@ int launch_barrier_big(const ModelGPU& m, const Layout& L, const FrameGPU* frame
 // cholesky_big.hip: FP64 blocked Cholesky solve for K above what one workgroup holds in LDS
 constexpr int CHOL_BIG_MAXK = 4096;
 int launch_cholesky_big(int n, int K, const float* H, const float* g, float* delta, int* status, hipStream_t s);
-// The same on FP64 matrices A [n][K][K] factored in place, g FP64 [n][K], y [n][K] scratch.
-int launch_cholesky_big_f64(int n, int K, double* A, const double* g, double* y, float* delta, int* status,
-                            hipStream_t s);
+// The same on FP64 matrices A [n][K][K] factored in place, g FP64 [n][K].
+int launch_cholesky_big_f64(int n, int K, double* A, const double* g, float* delta, int* status, hipStream_t s);
 // identity.hip: the identity stage's joint system from the frames' normal equations.
 int launch_assemble_joint(int n, int K, int Ks, const double* H, const float* g, double* HJ, double* gJ, hipStream_t s);
 int launch_shared_bounds(int Ks, long long KJ, const float* x, const float* lb, const float* ub, int mode, double* obj,
