@@ -121,6 +121,7 @@ struct Workspace {
     FitState* st = nullptr;
     float* partials = nullptr;
     float* lmk = nullptr;      // [C][6][51]
+    float* alb = nullptr;      // [C][V] vertex albedos of the current render (fitted albedo only)
     int S = 1;                 // J^T J splits per frame (the heuristic's, or the autotuner's)
     int S_cap = 1;             // splits the partials buffer holds
     int di_area = 0;           // the area that sizes k_dI's workgroups per frame (0: the mean crop)
@@ -133,6 +134,7 @@ struct Workspace {
 
 struct tdi_batch {
     tdi_ctx* ctx = nullptr;
+    ModelGPU m{};                     // the context's model with this batch's vertex albedos (alb)
     int n = 0;
     long long total = 0;
     int max_area = 0;
@@ -364,6 +366,8 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     const size_t pf = std::max(jtj_partials_floats(C, w.S_cap, K, false), jtj_partials_floats(C, w.S_cap, K, true));
     w.partials = A.get<float>(pf);
     w.lmk = A.get<float>((size_t)C * 6 * NLMK);
+    // per-frame vertex albedos when the albedo is fitted (read by the shading and the Jacobian)
+    w.alb = c->L.ntex ? A.get<float>((size_t)C * V) : nullptr;
     for (void* p : A.ptrs)
         if (!p) { set_error("workspace allocation failed (C=%d, cap=%lld)", C, cap); return TDI_ERR_NOMEM; }
     return TDI_OK;
@@ -534,6 +538,8 @@ extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd,
     }
     int rc = ws_alloc(b, C, cap);
     if (rc) { b->mem.release(); delete b; return rc; }
+    b->m = c->m;
+    b->m.alb = b->ws.alb;
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         const auto it = g_tuned.find(TuneKey(c->L.K, C, cap, b->max_area, c->opts.mixed_flags));
@@ -623,9 +629,9 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
     Workspace& w = b->ws;
     int rc;
     if (mixed & TDI_MIXED_SYNTH_MFMA)
-        rc = launch_synthesize_gemm(c->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
+        rc = launch_synthesize_gemm(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
     else
-        rc = launch_synthesize(c->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
+        rc = launch_synthesize(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
     if (rc) return rc;
     if (dverts && (rc = launch_add_offsets(c->m.V, w.frames, fl.n, w.rt, dverts, w.P0, w.vproj, s))) return rc;
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
@@ -636,7 +642,7 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
     if ((rc = launch_barrier(bounds_model(b), L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
         return rc;
-    if ((rc = launch_shade(c->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix,
+    if ((rc = launch_shade(b->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix,
                            full_crop ? nullptr : w.box, w.img, w.max_area, s)))
         return rc;
     if ((rc = launch_blur(w.frames, fl, c->taps, full_crop ? nullptr : w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
@@ -656,9 +662,9 @@ static int eval_objective(tdi_batch* b, FrameList fl, const float* theta, int mi
     Workspace& w = b->ws;
     int rc;
     if (mixed & TDI_MIXED_SYNTH_MFMA)
-        rc = launch_synthesize_gemm(c->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
+        rc = launch_synthesize_gemm(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
     else
-        rc = launch_synthesize(c->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
+        rc = launch_synthesize(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
     if (rc) return rc;
     if ((rc = launch_face_check(c->m.V, c->m.lmk_ids, w.vproj, c->opts.ref_face_size, fl, w.face_ok, s))) return rc;
     if ((rc = launch_clear_keys_frames(w.frames, fl, w.keys, w.box, w.max_area, s))) return rc;
@@ -667,7 +673,7 @@ static int eval_objective(tdi_batch* b, FrameList fl, const float* theta, int mi
     if ((rc = launch_barrier(bounds_model(b), L, w.frames, fl, theta, lmk_arrays(w), c->opts.use_landmarks, 0, w.obj,
                              w.objb, w.H, w.g, s)))
         return rc;
-    if ((rc = launch_shade_keys(c->m, L, w.frames, fl, theta, w.P0, w.vproj, w.keys, w.box, w.img, s))) return rc;
+    if ((rc = launch_shade_keys(b->m, L, w.frames, fl, theta, w.P0, w.vproj, w.keys, w.box, w.img, s))) return rc;
     if ((rc = launch_blur(w.frames, fl, c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
     return launch_objective(w.frames, fl, w.keys, w.box, w.blurred, b->obs_g, c->opts.data_weight, w.objp, w.obj, s);
 }
@@ -711,7 +717,7 @@ static int eval_jacobian(tdi_batch* b, FrameList fl, const float* theta, int mix
     const Layout& L = Lp ? *Lp : c->L;
     Workspace& w = b->ws;
     int rc;
-    if ((rc = launch_dI(c->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
+    if ((rc = launch_dI(b->m, L, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
                         dI_area(w), s)))
         return rc;
     if (L.K > SMALL_JTJ_MAXK) {
@@ -767,7 +773,7 @@ int tdi_synthesize(tdi_batch* b, const float* theta, float* xyz, float* xyp, voi
         const int n = std::min(b->ws.C, b->n - i0);
         int rc = ws_load(b, i0, n, s);
         if (rc) return rc;
-        if ((rc = launch_synthesize(c->m, c->L, b->ws.frames, all_frames(n), theta + (size_t)i0 * K, b->ws.rt, b->ws.P0, b->ws.P,
+        if ((rc = launch_synthesize(b->m, c->L, b->ws.frames, all_frames(n), theta + (size_t)i0 * K, b->ws.rt, b->ws.P0, b->ws.P,
                                     b->ws.vproj, s)))
             return rc;
         if ((rc = launch_misc_unpack(n, V, b->ws.P, b->ws.vproj, xyz + (size_t)i0 * V * 3, xyp + (size_t)i0 * V * 2, s)))
@@ -1292,7 +1298,7 @@ int tdi_batch_autotune(tdi_batch* b, const float* theta, int32_t reps, int32_t* 
     int best_area = a0;
     float best = 1e30f;
     for (int i = 0; i < 4 && !rc; ++i) {
-        rc = timed([&] { return launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix,
+        rc = timed([&] { return launch_dI(b->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix,
                                           w.pixg, w.dI, areas[i], s); },
                    rep[i]);
         if (!rc && rep[i] < best) { best = rep[i]; best_area = areas[i]; }
@@ -1347,9 +1353,9 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
     for (int r = 0; r < reps; ++r) {
         TDI_HIP(hipEventRecord(ev[0], s));
         if (mixed & TDI_MIXED_SYNTH_MFMA)
-            rc = launch_synthesize_gemm(c->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, nullptr, w.vproj, s);
+            rc = launch_synthesize_gemm(b->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, nullptr, w.vproj, s);
         else
-            rc = launch_synthesize(c->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, nullptr, w.vproj, s);
+            rc = launch_synthesize(b->m, c->L, w.frames, all_frames(n), theta, w.rt, w.P0, nullptr, w.vproj, s);
         if (rc) return rc;
         TDI_HIP(hipEventRecord(ev[1], s));
         if ((rc = launch_clear_keys_frames(w.frames, all_frames(n), w.keys, w.box, w.max_area, s))) return rc;
@@ -1357,7 +1363,7 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
         if ((rc = launch_resolve(c->m, w.frames, all_frames(n), w.vproj, w.keys, w.cum, w.vis, w.npix, nullptr, w.pvert, w.pbary, w.max_area, w.tcount, w.tmax, w.box, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[2], s));
-        if ((rc = launch_shade(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix, w.box, w.img,
+        if ((rc = launch_shade(b->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix, w.box, w.img,
                                w.max_area, s)))
             return rc;
         if ((rc = launch_blur(w.frames, all_frames(n), c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
@@ -1369,7 +1375,7 @@ int tdi_stage_times(tdi_batch* b, const float* theta, int32_t mixed, int32_t rep
                                  w.objb, w.H, w.g, s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[4], s));
-        if ((rc = launch_dI(c->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
+        if ((rc = launch_dI(b->m, c->L, w.frames, all_frames(n), theta, w.P0, w.pvert, w.pbary, w.npix, w.pixg, w.dI,
                             dI_area(w), s)))
             return rc;
         TDI_HIP(hipEventRecord(ev[5], s));
@@ -1566,7 +1572,7 @@ static int eval_light_full(tdi_batch* b, FrameList fl, const float* theta, hipSt
     Workspace& w = b->ws;
     int rc;
     if ((rc = launch_obj_from_barrier(fl, w.objb, w.obj, s))) return rc;
-    if ((rc = launch_shade(c->m, Lx, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.box, w.img, w.max_area, s)))
+    if ((rc = launch_shade(b->m, Lx, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.box, w.img, w.max_area, s)))
         return rc;
     if ((rc = launch_blur(w.frames, fl, c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
     if ((rc = launch_grads(w.frames, fl, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
@@ -1582,7 +1588,7 @@ static int eval_light_objective(tdi_batch* b, FrameList fl, const float* theta, 
     Workspace& w = b->ws;
     int rc;
     if ((rc = launch_obj_from_barrier(fl, w.objb, w.obj, s))) return rc;
-    if ((rc = launch_shade_keys(c->m, c->Lx, w.frames, fl, theta, w.P0, w.vproj, w.keys, w.box, w.img, s))) return rc;
+    if ((rc = launch_shade_keys(b->m, c->Lx, w.frames, fl, theta, w.P0, w.vproj, w.keys, w.box, w.img, s))) return rc;
     if ((rc = launch_blur(w.frames, fl, c->taps, w.box, w.img, w.tmp, w.blurred, w.max_area, s))) return rc;
     return launch_objective(w.frames, fl, w.keys, w.box, w.blurred, b->obs_g, c->opts.data_weight, w.objp, w.obj, s);
 }

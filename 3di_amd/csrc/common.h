@@ -51,6 +51,10 @@ struct ModelGPU {
     const int* lmk_ids;      // [51]
     const float* lb;         // [K] lower bounds of the unknowns (-inf = unbounded)
     const float* ub;         // [K]
+    // per-frame vertex albedos [frames][V] of the current render (tex_mean + tex_basis beta, in
+    // the order the shading sums them), when the albedo is fitted: the shading and the Jacobian
+    // then read them instead of summing the basis at every pixel (NULL: they sum it)
+    const float* alb = nullptr;
 };
 
 // One frame of a batch, device-resident.
@@ -293,6 +297,8 @@ struct LmkArrays {
 // P (camera-space vertices) may be null: the fit itself only needs P0 and the projections
 int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
                       double* rt, double* P0, float* P, float4* vproj, hipStream_t s);
+// The per-frame vertex albedos of the listed frames (ModelGPU::alb) from their theta.
+int launch_albedo(const ModelGPU& m, const Layout& L, FrameList fl, const float* theta, float* alb, hipStream_t s);
 int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
                            const float* theta, double* rt, double* P0, float* P, float4* vproj, hipStream_t s);
 

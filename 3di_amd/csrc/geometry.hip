@@ -222,6 +222,64 @@ int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames
     dim3 grid((m.V + 255) / 256, grid_frames(fl.n, 64));
     hipLaunchKernelGGL(k_synth_gemv, grid, dim3(256), 0, s, m, L, frames, theta, fl.act, fl.n, rt, P0, P, vproj);
     TDI_HIP(hipGetLastError());
+    if (m.alb && L.ntex) return launch_albedo(m, L, fl, theta, const_cast<float*>(m.alb), s);
+    return TDI_OK;
+}
+
+namespace {
+
+// alb[f][v] = tex_mean[v] + sum_q tex_basis[v][q] beta_q, summed in q order with fmaf as the
+// shading's per-pixel loop does, so that both give the same value.  A workgroup owns 64 vertices:
+// their basis rows are staged in LDS 128 columns at a time (coalesced row segments), then its four
+// wavefronts run the sums of four frames at once, lane = vertex (conflict-free LDS rows, the
+// frame's coefficients broadcast); a sum carried over column chunks goes through alb itself.
+constexpr int AL_V = 64, AL_Q = 128;
+
+__global__ void __launch_bounds__(256) k_albedo(ModelGPU m, Layout L, const int* __restrict__ act, int n,
+                                                 const float* __restrict__ theta, float* __restrict__ alb)
+{
+    __shared__ float sB[AL_V][AL_Q + 1];
+    __shared__ float sbeta[4][AL_Q];
+    const int v0 = blockIdx.x * AL_V;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int v = v0 + lane;
+    const int nact = fl_count(act, n);
+    for (int q0 = 0; q0 < L.ntex; q0 += AL_Q) {
+        const int nq = min(AL_Q, L.ntex - q0);
+        __syncthreads();   // the previous chunk's staging is no longer read
+        for (int e = threadIdx.x; e < AL_V * nq; e += blockDim.x) {
+            const int vl = e / nq, q = e - vl * nq;
+            sB[vl][q] = v0 + vl < m.V ? m.tex_basis[(size_t)(v0 + vl) * m.tex_ld + q0 + q] : 0.f;
+        }
+        for (int yi0 = 0; yi0 < nact; yi0 += 4) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < 4 * nq; e += blockDim.x) {
+                const int w = e / nq, q = e - w * nq;
+                const int yi = yi0 + w;
+                sbeta[w][q] = yi < nact ? theta[(size_t)fl_frame(act, yi) * L.K + L.off_tex + q0 + q] : 0.f;
+            }
+            __syncthreads();
+            const int yi = yi0 + wave;
+            if (yi < nact && v < m.V) {
+                const int f = fl_frame(act, yi);
+                float r = q0 == 0 ? m.tex_mean[v] : alb[(size_t)f * m.V + v];
+                for (int q = 0; q < nq; ++q) r = fmaf(sB[lane][q], sbeta[wave][q], r);
+                alb[(size_t)f * m.V + v] = r;
+            }
+        }
+    }
+}
+
+} // namespace
+
+int launch_albedo(const ModelGPU& m, const Layout& L, FrameList fl, const float* theta, float* alb, hipStream_t s)
+{
+    if (L.ntex > 512) {
+        set_error("albedo: k_tex %d > 512", L.ntex);
+        return TDI_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_albedo, dim3((m.V + AL_V - 1) / AL_V), dim3(256), 0, s, m, L, fl.act, fl.n, theta, alb);
+    TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
 
@@ -235,6 +293,7 @@ int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* f
     const dim3 grid((m.V + SY_VERTS - 1) / SY_VERTS);
     hipLaunchKernelGGL(k_synth_tile, grid, dim3(256), 0, s, m, L, frames, theta, fl.act, fl.n, rt, P0, P, vproj);
     TDI_HIP(hipGetLastError());
+    if (m.alb && L.ntex) return launch_albedo(m, L, fl, theta, const_cast<float*>(m.alb), s);
     return TDI_OK;
 }
 

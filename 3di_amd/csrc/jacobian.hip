@@ -140,8 +140,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             const double lz = sr.R[6] * d0 + sr.R[7] * d1 + sr.R[8] * d2;
             Id = (float)(ndx * lx + ndy * ly + ndz * lz - (ndx * p0d[0] + ndy * p0d[1] + ndz * p0d[2]));
             const float nx = (float)ndx, ny = (float)ndy, nz = (float)ndz;
-            float rb = m.tex_mean[vb], rc = m.tex_mean[vc], ra = m.tex_mean[va];
-            for (int q = 0; q < L.ntex; ++q) {
+            const float* rsrc = m.alb ? m.alb + (size_t)f * m.V : m.tex_mean;
+            float rb = rsrc[vb], rc = rsrc[vc], ra = rsrc[va];
+            for (int q = 0; !m.alb && q < L.ntex; ++q) {
                 rb = fmaf(m.tex_basis[(size_t)vb * m.tex_ld + q], stex[q], rb);
                 rc = fmaf(m.tex_basis[(size_t)vc * m.tex_ld + q], stex[q], rc);
                 ra = fmaf(m.tex_basis[(size_t)va * m.tex_ld + q], stex[q], ra);

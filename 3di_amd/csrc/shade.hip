@@ -132,8 +132,8 @@ struct ShadeIn {
     float rb, rc, ra;
 };
 
-__device__ __forceinline__ void shade_load(const ModelGPU& m, const double* __restrict__ P0f, int vb, int vc, int va,
-                                           ShadeIn& in)
+__device__ __forceinline__ void shade_load(const ModelGPU& m, const double* __restrict__ P0f,
+                                           const float* __restrict__ albf, int vb, int vc, int va, ShadeIn& in)
 {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -141,9 +141,10 @@ __device__ __forceinline__ void shade_load(const ModelGPU& m, const double* __re
         in.c[i] = P0f[3 * vc + i];
         in.a[i] = P0f[3 * va + i];
     }
-    in.rb = m.tex_mean[vb];
-    in.rc = m.tex_mean[vc];
-    in.ra = m.tex_mean[va];
+    const float* rsrc = albf ? albf : m.tex_mean;
+    in.rb = rsrc[vb];
+    in.rc = rsrc[vc];
+    in.ra = rsrc[va];
 }
 
 // I = rho (1 + L Id) at barycentric (al, be) of the triangle (vb, vc, va).  The geometry (normal,
@@ -173,7 +174,7 @@ __device__ __forceinline__ float shade_eval(const ModelGPU& m, const Layout& L, 
     const double lz = (double)sf.R[6] * sf.dl[0] + (double)sf.R[7] * sf.dl[1] + (double)sf.R[8] * sf.dl[2];
     const double Id = nx * lx + ny * ly + nz * lz - (nx * px + ny * py + nz * pz);
     float rb = in.rb, rc = in.rc, ra = in.ra;
-    for (int q = 0; q < L.ntex; ++q) {
+    for (int q = 0; !m.alb && q < L.ntex; ++q) {
         rb = fmaf(m.tex_basis[(size_t)vb * m.tex_ld + q], stex[q], rb);
         rc = fmaf(m.tex_basis[(size_t)vc * m.tex_ld + q], stex[q], rc);
         ra = fmaf(m.tex_basis[(size_t)va * m.tex_ld + q], stex[q], ra);
@@ -191,11 +192,11 @@ __device__ __forceinline__ float shade_eval(const ModelGPU& m, const Layout& L, 
 }
 
 __device__ __forceinline__ float shade_pixel(const ModelGPU& m, const Layout& L, const double* __restrict__ P0f,
-                                             const ShadeFrame& sf, const float* stex, int vb, int vc, int va,
-                                             float al, float be)
+                                             const float* __restrict__ albf, const ShadeFrame& sf, const float* stex,
+                                             int vb, int vc, int va, float al, float be)
 {
     ShadeIn in;
-    shade_load(m, P0f, vb, vc, va, in);
+    shade_load(m, P0f, albf, vb, vc, va, in);
     return shade_eval(m, L, sf, stex, vb, vc, va, in, al, be);
 }
 
@@ -220,10 +221,11 @@ __global__ void __launch_bounds__(256) k_shade(ModelGPU m, Layout L, const Frame
         __syncthreads();
         const long long base = fr.pix_off;
         const double* P0f = P0 + (size_t)f * m.V * 3;
+        const float* albf = m.alb ? m.alb + (size_t)f * m.V : nullptr;
         for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += gridDim.x * blockDim.x) {
             const int4 pv = pvert[base + pos];
             const float2 ab = pbary[base + pos];
-            img[base + pv.w] = shade_pixel(m, L, P0f, sf, stex, pv.x, pv.y, pv.z, ab.x, ab.y);
+            img[base + pv.w] = shade_pixel(m, L, P0f, albf, sf, stex, pv.x, pv.y, pv.z, ab.x, ab.y);
         }
         __syncthreads();   // the frame's shared constants are rewritten by the next iteration
     }
@@ -257,6 +259,7 @@ __global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const 
         __syncthreads();
         const long long base = fr.pix_off;
         const double* P0f = P0 + (size_t)f * m.V * 3;
+        const float* albf = m.alb ? m.alb + (size_t)f * m.V : nullptr;
         const float4* vp = vproj + (size_t)f * m.V;
         for (int x = r.X0 + cm.sub * nw + wave; x < r.X1; x += cm.fpb * nw)
             for (int y0 = r.Y0 + lane; y0 < r.Y1; y0 += 128) {
@@ -281,11 +284,11 @@ __global__ void __launch_bounds__(256) k_shade_keys(ModelGPU m, Layout L, const 
                 ShadeIn in0, in1;
                 if (c0) {
                     B0 = vp[v0[0]]; C0 = vp[v0[1]]; A0 = vp[v0[2]];
-                    shade_load(m, P0f, v0[0], v0[1], v0[2], in0);
+                    shade_load(m, P0f, albf, v0[0], v0[1], v0[2], in0);
                 }
                 if (c1) {
                     B1 = vp[v1[0]]; C1 = vp[v1[1]]; A1 = vp[v1[2]];
-                    shade_load(m, P0f, v1[0], v1[1], v1[2], in1);
+                    shade_load(m, P0f, albf, v1[0], v1[1], v1[2], in1);
                 }
                 float s0 = 0.f, s1 = 0.f;
                 if (c0) {
