@@ -53,6 +53,10 @@ int launch_misc_unpack(int n, int V, const float* P, const float4* vproj, float*
 int launch_misc_fill_f32(float* p, long long count, float v, hipStream_t s);
 int launch_misc_f64_to_f32(const double* in, float* out, int n, hipStream_t s);
 int launch_misc_rotation(int n, const float* u, float* R, float* dR, hipStream_t s);
+// the light sub-fits' per-frame results packed into 16 doubles per slot (light block of H, light
+// gradient, objective, barrier objective, pixel count) for one device-to-host copy
+int launch_misc_light_pack(int n, int K, int j0, int nd, const double* H, const float* g, const double* obj,
+                           const double* objb, const int* npix, double* out, hipStream_t s);
 
 } // namespace tdi
 
@@ -122,6 +126,7 @@ struct Workspace {
     float* partials = nullptr;
     float* lmk = nullptr;      // [C][6][51]
     float* alb = nullptr;      // [C][V] vertex albedos of the current render (fitted albedo only)
+    double* lpack = nullptr;   // [C][16] the light sub-fits' per-frame results, packed
     int S = 1;                 // J^T J splits per frame (the heuristic's, or the autotuner's)
     int S_cap = 1;             // splits the partials buffer holds
     int di_area = 0;           // the area that sizes k_dI's workgroups per frame (0: the mean crop)
@@ -368,6 +373,7 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     w.lmk = A.get<float>((size_t)C * 6 * NLMK);
     // per-frame vertex albedos when the albedo is fitted (read by the shading and the Jacobian)
     w.alb = c->L.ntex ? A.get<float>((size_t)C * V) : nullptr;
+    w.lpack = A.get<double>((size_t)C * 16);
     for (void* p : A.ptrs)
         if (!p) { set_error("workspace allocation failed (C=%d, cap=%lld)", C, cap); return TDI_ERR_NOMEM; }
     return TDI_OK;
@@ -1632,9 +1638,8 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
             for (int a = 0; a < 3; ++a) xi[ol + a] = lam_host[3 * (i0 + i) + a];
             xi[ol + 3] = Lint_host[i0 + i];
         }
-        std::vector<double> Hh, ob0(n), ob1(n);
-        std::vector<float> gh((size_t)n * Kx);
-        std::vector<int> np(n), list;
+        std::vector<double> pk((size_t)16 * n), ob0(n), ob1(n);
+        std::vector<int> list;
         bool first = true;
         for (;;) {
             // frames starting an outer iteration (10000 at most)
@@ -1650,29 +1655,26 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
             // the first evaluation renders every frame of the chunk; the light changes nothing else
             if ((rc = first ? eval_full(b, fl, W.theta, 0, s, &Lx) : eval_light_full(b, fl, W.theta, s))) return rc;
             first = false;
-            Hh.resize((size_t)n * Kx * Kx);
-            TDI_HIP(hipMemcpyAsync(Hh.data(), W.H, sizeof(double) * Hh.size(), hipMemcpyDeviceToHost, s));
-            TDI_HIP(hipMemcpyAsync(gh.data(), W.g, sizeof(float) * gh.size(), hipMemcpyDeviceToHost, s));
-            TDI_HIP(hipMemcpyAsync(ob0.data(), W.obj, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            TDI_HIP(hipMemcpyAsync(ob1.data(), W.objb, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            TDI_HIP(hipMemcpyAsync(np.data(), W.npix, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+            // the light block, gradient, objectives and pixel counts, one copy
+            if ((rc = launch_misc_light_pack(n, Kx, j0, nd, W.H, W.g, W.obj, W.objb, W.npix, W.lpack, s))) return rc;
+            TDI_HIP(hipMemcpyAsync(pk.data(), W.lpack, sizeof(double) * 16 * n, hipMemcpyDeviceToHost, s));
             TDI_HIP(hipStreamSynchronize(s));
             for (int i : list) {
                 LightFrame& f = st[i];
                 tdi_fit_report& rep = reports[i0 + i];
+                const double* P = &pk[(size_t)i * 16];
                 f.outer++;
-                rep.n_pixels = np[i];
-                f.obj0 = (ob0[i] - ob1[i]) / w;
-                const double* Hi = &Hh[(size_t)i * Kx * Kx];
+                rep.n_pixels = (int)P[14];
+                f.obj0 = (P[12] - P[13]) / w;
                 double g[3] = {0, 0, 0};
                 f.d[0] = f.d[1] = f.d[2] = 0;
-                for (int a = 0; a < nd; ++a) g[a] = gh[(size_t)i * Kx + j0 + a] / w;
+                for (int a = 0; a < nd; ++a) g[a] = P[9 + a] / w;
                 if (intensity) {
-                    f.d[0] = -g[0] / (Hi[(size_t)j0 * Kx + j0] / w);     // compute_Lintensity_search_dir
+                    f.d[0] = -g[0] / (P[0] / w);     // compute_Lintensity_search_dir
                 } else {
                     double A[9];
                     for (int a = 0; a < 3; ++a)
-                        for (int bb = 0; bb < 3; ++bb) A[a * 3 + bb] = Hi[(size_t)(j0 + a) * Kx + j0 + bb] / w;
+                        for (int bb = 0; bb < 3; ++bb) A[a * 3 + bb] = P[3 * a + bb] / w;
                     if (!solve3(A, g, f.d)) {
                         rep.stop |= TDI_STOP_NOT_PD;
                         f.phase = LightFrame::DONE;
@@ -1706,9 +1708,13 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
                 TDI_HIP(hipMemcpyAsync(W.theta_try, xt.data(), sizeof(float) * xt.size(), hipMemcpyHostToDevice, s));
                 if ((rc = upload_list(W, W.act_e, list, s))) return rc;
                 if ((rc = eval_light_objective(b, FrameList{W.act_e, n}, W.theta_try, s))) return rc;
-                TDI_HIP(hipMemcpyAsync(ob0.data(), W.obj, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-                TDI_HIP(hipMemcpyAsync(ob1.data(), W.objb, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                if ((rc = launch_misc_light_pack(n, Kx, j0, nd, W.H, W.g, W.obj, W.objb, W.npix, W.lpack, s))) return rc;
+                TDI_HIP(hipMemcpyAsync(pk.data(), W.lpack, sizeof(double) * 16 * n, hipMemcpyDeviceToHost, s));
                 TDI_HIP(hipStreamSynchronize(s));
+                for (int i : list) {
+                    ob0[i] = pk[(size_t)i * 16 + 12];
+                    ob1[i] = pk[(size_t)i * 16 + 13];
+                }
                 for (int i : list) {
                     LightFrame& f = st[i];
                     tdi_fit_report& rep = reports[i0 + i];

@@ -48,6 +48,25 @@ __global__ void k_rotation(int n, const float* __restrict__ u, float* __restrict
         for (int k = 0; k < 27; ++k) dR[27 * i + k] = d[k];
 }
 
+// Per slot f < n: the light block of the normal matrix (nd x nd from row/column j0), the light
+// gradient, objective, barrier objective and pixel count, packed into 16 doubles.
+__global__ void k_light_pack(int n, int K, int j0, int nd, const double* __restrict__ H, const float* __restrict__ g,
+                             const double* __restrict__ obj, const double* __restrict__ objb,
+                             const int* __restrict__ npix, double* __restrict__ out)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    double* o = out + (size_t)f * 16;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            o[3 * a + b] = (a < nd && b < nd) ? H[(size_t)f * K * K + (size_t)(j0 + a) * K + j0 + b] : 0.0;
+    for (int a = 0; a < 3; ++a) o[9 + a] = a < nd ? (double)g[(size_t)f * K + j0 + a] : 0.0;
+    o[12] = obj[f];
+    o[13] = objb[f];
+    o[14] = (double)npix[f];
+    o[15] = 0.0;
+}
+
 } // namespace
 
 int launch_misc_pack(int n, int V, const float* xyz, const float* xyp, float4* vproj, hipStream_t s)
@@ -78,6 +97,14 @@ int launch_misc_fill_f32(float* p, long long count, float v, hipStream_t s)
 int launch_misc_f64_to_f32(const double* in, float* out, int n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_f64_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_misc_light_pack(int n, int K, int j0, int nd, const double* H, const float* g, const double* obj,
+                           const double* objb, const int* npix, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_light_pack, dim3((n + 63) / 64), dim3(64), 0, s, n, K, j0, nd, H, g, obj, objb, npix, out);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
