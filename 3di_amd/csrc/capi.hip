@@ -127,6 +127,7 @@ struct Workspace {
     float* lmk = nullptr;      // [C][6][51]
     float* alb = nullptr;      // [C][V] vertex albedos of the current render (fitted albedo only)
     double* lpack = nullptr;   // [C][16] the light sub-fits' per-frame results, packed
+    double* idshape = nullptr; // [V][3] the shared identity shape of an identity-stage evaluation
     int S = 1;                 // J^T J splits per frame (the heuristic's, or the autotuner's)
     int S_cap = 1;             // splits the partials buffer holds
     int di_area = 0;           // the area that sizes k_dI's workgroups per frame (0: the mean crop)
@@ -374,6 +375,7 @@ static int ws_alloc(tdi_batch* b, int C, long long cap)
     // per-frame vertex albedos when the albedo is fitted (read by the shading and the Jacobian)
     w.alb = c->L.ntex ? A.get<float>((size_t)C * V) : nullptr;
     w.lpack = A.get<double>((size_t)C * 16);
+    w.idshape = c->L.nid ? A.get<double>((size_t)V * 3) : nullptr;
     for (void* p : A.ptrs)
         if (!p) { set_error("workspace allocation failed (C=%d, cap=%lld)", C, cap); return TDI_ERR_NOMEM; }
     return TDI_OK;
@@ -634,7 +636,9 @@ static int eval_render(tdi_batch* b, FrameList fl, const float* theta, int mixed
     const Layout& L = Lp ? *Lp : c->L;
     Workspace& w = b->ws;
     int rc;
-    if (mixed & TDI_MIXED_SYNTH_MFMA)
+    if (b->joint && L.nid && w.idshape)   // the identity stage's frames share alpha
+        rc = launch_synthesize_shared_id(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, w.idshape, s);
+    else if (mixed & TDI_MIXED_SYNTH_MFMA)
         rc = launch_synthesize_gemm(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
     else
         rc = launch_synthesize(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
@@ -667,7 +671,9 @@ static int eval_objective(tdi_batch* b, FrameList fl, const float* theta, int mi
     const Layout& L = Lp ? *Lp : c->L;
     Workspace& w = b->ws;
     int rc;
-    if (mixed & TDI_MIXED_SYNTH_MFMA)
+    if (b->joint && L.nid && w.idshape)   // the identity stage's frames share alpha
+        rc = launch_synthesize_shared_id(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, w.idshape, s);
+    else if (mixed & TDI_MIXED_SYNTH_MFMA)
         rc = launch_synthesize_gemm(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);
     else
         rc = launch_synthesize(b->m, L, w.frames, fl, theta, w.rt, w.P0, nullptr, w.vproj, s);

@@ -297,6 +297,11 @@ struct LmkArrays {
 // P (camera-space vertices) may be null: the fit itself only needs P0 and the projections
 int launch_synthesize(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
                       double* rt, double* P0, float* P, float4* vproj, hipStream_t s);
+// Synthesis of frames that share their identity coefficients (the identity stage's evaluations):
+// the identity shape once (FP64, idshape [V][3]), then the expression MFMA tiles on it.
+int launch_synthesize_shared_id(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
+                                const float* theta, double* rt, double* P0, float* P, float4* vproj, double* idshape,
+                                hipStream_t s);
 // The per-frame vertex albedos of the listed frames (ModelGPU::alb) from their theta.
 int launch_albedo(const ModelGPU& m, const Layout& L, FrameList fl, const float* theta, float* alb, hipStream_t s);
 int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,

@@ -118,11 +118,11 @@ constexpr int SY_KMAX = 80;                // expression coefficients held in LD
 // MFMAs of the others.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const float* __restrict__ theta,
-             const int* __restrict__ act, int n, const double* __restrict__ rt, double* __restrict__ P0,
-             float* __restrict__ P, float4* __restrict__ vproj)
+             const int* __restrict__ act, int n, const double* __restrict__ rt, const double* __restrict__ mean64,
+             double* __restrict__ P0, float* __restrict__ P, float4* __restrict__ vproj)
 {
     __shared__ double sA[3][SY_KMAX][16];              // [coordinate][k][vertex]
-    __shared__ float smean[SY_VERTS * 3];
+    __shared__ double smean[SY_VERTS * 3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (fl_count(act, n) == 0) return;       // an emptied list: skip the basis staging
     const int v0 = blockIdx.x * SY_VERTS;
@@ -149,9 +149,10 @@ k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const fl
             sA[c][k][i] = (k < Ke && v < m.V) ? (double)av[q] : 0.0;
         }
     }
+    // the base shape: the model's mean, or a given FP64 shape (the identity stage's shared identity)
     for (int e = tid; e < SY_VERTS * 3; e += blockDim.x) {
         const int v = v0 + e / 3;
-        smean[e] = v < m.V ? m.mean[3 * v0 + e] : 0.f;
+        smean[e] = v < m.V ? (mean64 ? mean64[3 * v0 + e] : (double)m.mean[3 * v0 + e]) : 0.0;
     }
     __syncthreads();
     const int nact = fl_count(act, n);
@@ -192,7 +193,7 @@ k_synth_tile(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames, const fl
                 if (v < m.V) {
                     double p[3];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) p[a] = (double)smean[3 * vl + a] + acc[a][r];
+                    for (int a = 0; a < 3; ++a) p[a] = smean[3 * vl + a] + acc[a][r];
                     store_vertex(fr, R, R + 9, p, (size_t)f * m.V + v, P0, P, vproj);
                 }
             }
@@ -291,7 +292,50 @@ int launch_synthesize_gemm(const ModelGPU& m, const Layout& L, const FrameGPU* f
     if (rc) return rc;
     if (L.nexp > SY_KMAX || L.nexp == 0) return launch_synthesize(m, L, frames, fl, theta, rt, P0, P, vproj, s);
     const dim3 grid((m.V + SY_VERTS - 1) / SY_VERTS);
-    hipLaunchKernelGGL(k_synth_tile, grid, dim3(256), 0, s, m, L, frames, theta, fl.act, fl.n, rt, P0, P, vproj);
+    hipLaunchKernelGGL(k_synth_tile, grid, dim3(256), 0, s, m, L, frames, theta, fl.act, fl.n, rt,
+                       (const double*)nullptr, P0, P, vproj);
+    TDI_HIP(hipGetLastError());
+    if (m.alb && L.ntex) return launch_albedo(m, L, fl, theta, const_cast<float*>(m.alb), s);
+    return TDI_OK;
+}
+
+namespace {
+
+// mean + identity basis x alpha in FP64 for every vertex, alpha of the first listed frame (the
+// frames of an identity-stage evaluation share it).
+__global__ void __launch_bounds__(256) k_id_shape(ModelGPU m, Layout L, const int* __restrict__ act, int n,
+                                                   const float* __restrict__ theta, double* __restrict__ S)
+{
+    __shared__ float sa[MAXK_LDS];
+    if (fl_count(act, n) == 0) return;
+    const float* tf = theta + (size_t)fl_frame(act, 0) * L.K + L.off_id;
+    for (int k = threadIdx.x; k < L.nid; k += blockDim.x) sa[k] = tf[k];
+    __syncthreads();
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m.V) return;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int r = 3 * v + a;
+        double sacc = m.mean[r];
+        const float* ib = m.id_basis + (size_t)r * m.id_ld;
+        for (int k = 0; k < L.nid; ++k) sacc = fma((double)ib[k], (double)sa[k], sacc);
+        S[r] = sacc;
+    }
+}
+
+} // namespace
+
+int launch_synthesize_shared_id(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl,
+                                const float* theta, double* rt, double* P0, float* P, float4* vproj, double* idshape,
+                                hipStream_t s)
+{
+    if (L.nexp == 0 || L.nexp > SY_KMAX || L.nid > MAXK_LDS) return launch_synthesize(m, L, frames, fl, theta, rt, P0, P, vproj, s);
+    int rc = launch_frame_rt(L, fl, theta, rt, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_id_shape, dim3((m.V + 255) / 256), dim3(256), 0, s, m, L, fl.act, fl.n, theta, idshape);
+    const dim3 grid((m.V + SY_VERTS - 1) / SY_VERTS);
+    hipLaunchKernelGGL(k_synth_tile, grid, dim3(256), 0, s, m, L, frames, theta, fl.act, fl.n, rt,
+                       (const double*)idshape, P0, P, vproj);
     TDI_HIP(hipGetLastError());
     if (m.alb && L.ntex) return launch_albedo(m, L, fl, theta, const_cast<float*>(m.alb), s);
     return TDI_OK;

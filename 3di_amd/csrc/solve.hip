@@ -68,17 +68,18 @@ __global__ void __launch_bounds__(256) k_barrier(ModelGPU m, Layout L, const Fra
         }
     }
     if (use_lmk) {
+        // the model-space landmark positions, one thread per coordinate
+        if (tid < 3 * NLMK) {
+            const int i = tid / 3, a = tid - 3 * i, r = 3 * m.lmk_ids[i] + a;
+            double s = m.mean[r];
+            for (int k = 0; k < L.nexp; ++k) s = fma((double)m.exp_basis[(size_t)r * m.exp_ld + k], (double)tf[L.off_exp + k], s);
+            for (int k = 0; k < L.nid; ++k) s = fma((double)m.id_basis[(size_t)r * m.id_ld + k], (double)tf[L.off_id + k], s);
+            sp[i][a] = s;
+        }
+        __syncthreads();
         if (tid < NLMK) {
-            const int i = tid, v = m.lmk_ids[i];
-            double p[3];
-            for (int a = 0; a < 3; ++a) {
-                const int r = 3 * v + a;
-                double s = m.mean[r];
-                for (int k = 0; k < L.nexp; ++k) s = fma((double)m.exp_basis[(size_t)r * m.exp_ld + k], (double)tf[L.off_exp + k], s);
-                for (int k = 0; k < L.nid; ++k) s = fma((double)m.id_basis[(size_t)r * m.id_ld + k], (double)tf[L.off_id + k], s);
-                p[a] = s;
-                sp[i][a] = s;
-            }
+            const int i = tid;
+            const double p[3] = {sp[i][0], sp[i][1], sp[i][2]};
             const double vx = sR[0] * p[0] + sR[3] * p[1] + sR[6] * p[2] + tf[L.off_pose + 0];
             const double vy = sR[1] * p[0] + sR[4] * p[1] + sR[7] * p[2] + tf[L.off_pose + 1];
             const double vz = sR[2] * p[0] + sR[5] * p[1] + sR[8] * p[2] + tf[L.off_pose + 2];
@@ -553,17 +554,17 @@ __global__ void __launch_bounds__(256) k_barrier_rows(ModelGPU m, Layout L, cons
         H[(size_t)f * K * K + (size_t)j * K + j] += a * a + b * b;
     }
     if (!use_lmk) return;
+    if (tid < 3 * NLMK) {
+        const int i = tid / 3, a = tid - 3 * i, r = 3 * m.lmk_ids[i] + a;
+        double sacc = m.mean[r];
+        for (int k = 0; k < L.nexp; ++k) sacc = fma((double)m.exp_basis[(size_t)r * m.exp_ld + k], (double)tf[L.off_exp + k], sacc);
+        for (int k = 0; k < L.nid; ++k) sacc = fma((double)m.id_basis[(size_t)r * m.id_ld + k], (double)tf[L.off_id + k], sacc);
+        sp[i][a] = sacc;
+    }
+    __syncthreads();
     if (tid < NLMK) {
-        const int i = tid, v = m.lmk_ids[i];
-        double p[3];
-        for (int a = 0; a < 3; ++a) {
-            const int r = 3 * v + a;
-            double sacc = m.mean[r];
-            for (int k = 0; k < L.nexp; ++k) sacc = fma((double)m.exp_basis[(size_t)r * m.exp_ld + k], (double)tf[L.off_exp + k], sacc);
-            for (int k = 0; k < L.nid; ++k) sacc = fma((double)m.id_basis[(size_t)r * m.id_ld + k], (double)tf[L.off_id + k], sacc);
-            p[a] = sacc;
-            sp[i][a] = sacc;
-        }
+        const int i = tid;
+        const double p[3] = {sp[i][0], sp[i][1], sp[i][2]};
         const double vx = sR[0] * p[0] + sR[3] * p[1] + sR[6] * p[2] + tf[L.off_pose + 0];
         const double vy = sR[1] * p[0] + sR[4] * p[1] + sR[7] * p[2] + tf[L.off_pose + 1];
         const double vz = sR[2] * p[0] + sR[5] * p[1] + sR[8] * p[2] + tf[L.off_pose + 2];
