@@ -4,13 +4,16 @@
 // matrix, copyMatsFloatToDouble / copyVectorDoubleToFloatNegated, solver.cu:11-81) for the
 // joint identity-stage system (K up to a few thousand: 199 + 199 + 94 per frame).  Blocked
 // right-looking factorisation in place on an FP64 copy in HBM, 32-column panels:
-//   k_chol_diag    one workgroup per matrix: factor the 32x32 diagonal block in LDS;
-//   k_chol_offdiag a grid of 128-row blocks per matrix: the panel below it, L21 = A21 L11^-T
-//                  (rows and L11 staged in LDS, one row per thread);
+//   k_chol_diag    one wavefront per matrix: the 32x32 diagonal block factored in registers,
+//                  lane i holding row i, the column entries passed by lane reads;
+//   k_chol_offdiag a grid of 64-row blocks per matrix: the panel below it, L21 = A21 L11^-T
+//                  (one row per thread in registers, L11 and its reciprocal diagonal in LDS);
 //   k_chol_update  a grid of 32x32 tiles of the trailing lower triangle per matrix:
 //                  A22 -= L21 L21^T, both 32-row slices of the panel staged in LDS;
-//   k_chol_trsv    one workgroup per matrix: L y = -g, then L^T delta = y, by 32-row panels
-//                  (the solution in LDS, the panel triangles on one wavefront).
+//   k_chol_trsv    one workgroup per matrix: L y = -g, then L^T delta = y, by 32-row panels,
+//                  right-looking (the solution in LDS; each panel's triangle on one wavefront
+//                  from registers, then every later entry updated by all threads with
+//                  independent loads of the panel's rows or columns).
 // A non-positive (or NaN) pivot marks the matrix TDI_NOT_PD; its later launches do nothing.
 #include "common.h"
 
@@ -26,85 +29,88 @@ __global__ void __launch_bounds__(256) k_f32_to_f64(size_t n, const float* __res
         out[i] = in[i];
 }
 
-// The 32 x 32 diagonal block of the panel at k0, factored in LDS and written back.
-__global__ void __launch_bounds__(256) k_chol_diag(int K, int k0, double* __restrict__ A, int* __restrict__ status)
+// lane j's value of v (j uniform)
+__device__ __forceinline__ double lane_value(double v, int j)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+    return __hiloint2double(hi, lo);
+}
+
+// The 32 x 32 diagonal block of the panel at k0, factored on one wavefront and written back:
+// lane i holds row i (entries j <= i; rows and columns past the matrix padded with the identity).
+__global__ void __launch_bounds__(64) k_chol_diag(int K, int k0, double* __restrict__ A, int* __restrict__ status)
 {
     const int mtx = blockIdx.x;
     if (status[mtx] != TDI_OK) return;
     double* M = A + (size_t)mtx * K * K;
-    __shared__ double D[CB][CB + 1];
-    __shared__ int bad;
-    const int nb = min(CB, K - k0), tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    for (int e = tid; e < nb * nb; e += blockDim.x) {
-        const int i = e / nb, j = e - i * nb;
-        D[i][j] = M[(size_t)(k0 + i) * K + k0 + j];
-    }
-    __syncthreads();
-    // unblocked Cholesky of the diagonal block (lower), column by column
-    for (int k = 0; k < nb; ++k) {
-        if (tid == 0) {
-            const double d = D[k][k];
-            if (!(d > 0.0)) bad = 1;
-            else D[k][k] = sqrt(d);
+    const int nb = min(CB, K - k0), lane = threadIdx.x;
+    const bool row = lane < nb;
+    const double* Mr = M + (size_t)(k0 + (row ? lane : 0)) * K + k0;
+    double r[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) r[j] = (row && j < nb && j <= lane) ? Mr[min(j, nb - 1)] : (lane == j ? 1.0 : 0.0);
+    // column k: pivot from lane k, the rows below scaled, then the trailing rows updated
+#pragma unroll
+    for (int k = 0; k < CB; ++k) {
+        const double d = lane_value(r[k], k);
+        if (!(d > 0.0)) {
+            if (lane == 0) status[mtx] = TDI_NOT_PD;
+            return;
         }
-        __syncthreads();
-        if (bad) break;
-        const double dk = D[k][k];
-        for (int i = k + 1 + tid; i < nb; i += blockDim.x) D[i][k] /= dk;
-        __syncthreads();
-        const int w = nb - k - 1;
-        for (int e = tid; e < w * w; e += blockDim.x) {
-            const int i = k + 1 + e / w, j = k + 1 + e % w;
-            if (j <= i) D[i][j] -= D[i][k] * D[j][k];
+        const double dk = sqrt(d), inv = 1.0 / dk;
+        r[k] = lane == k ? dk : (lane > k ? r[k] * inv : r[k]);
+#pragma unroll
+        for (int j = k + 1; j < CB; ++j) {
+            const double ljk = lane_value(r[k], j);
+            if (lane >= j) r[j] = fma(-r[k], ljk, r[j]);
         }
-        __syncthreads();
     }
-    if (bad) {
-        if (tid == 0) status[mtx] = TDI_NOT_PD;
-        return;
-    }
-    for (int e = tid; e < nb * nb; e += blockDim.x) {
-        const int i = e / nb, j = e - i * nb;
-        if (j <= i) M[(size_t)(k0 + i) * K + k0 + j] = D[i][j];
+    if (row) {
+        double* Mw = M + (size_t)(k0 + lane) * K + k0;
+#pragma unroll
+        for (int j = 0; j < CB; ++j)
+            if (j < nb && j <= lane) Mw[j] = r[j];
     }
 }
 
 // The panel below the diagonal block: row r of A21 solves x L11^T = a (forward substitution over
-// the panel's columns), 128 rows per workgroup (a grid over the rows), rows and L11 staged in LDS.
-__global__ void __launch_bounds__(128) k_chol_offdiag(int K, int k0, double* __restrict__ A,
-                                                       const int* __restrict__ status)
+// the panel's columns), 64 rows per workgroup (a grid over the rows), each row in its thread's
+// registers, L11 and the reciprocals of its diagonal in LDS (read as broadcasts).
+__global__ void __launch_bounds__(64) k_chol_offdiag(int K, int k0, double* __restrict__ A,
+                                                      const int* __restrict__ status)
 {
     const int mtx = blockIdx.y;
     if (status[mtx] != TDI_OK) return;
     double* M = A + (size_t)mtx * K * K;
     const int nb = min(CB, K - k0), tid = threadIdx.x;
-    const int r0 = k0 + nb + blockIdx.x * 128;
+    const int r0 = k0 + nb + blockIdx.x * 64;
     if (r0 >= K) return;
-    const int nr = min(128, K - r0);
-    __shared__ double D[CB][CB + 1];
-    __shared__ double Pn[128][CB + 1];
-    for (int e = tid; e < nb * nb; e += blockDim.x) {
-        const int i = e / nb, j = e - i * nb;
-        D[i][j] = j <= i ? M[(size_t)(k0 + i) * K + k0 + j] : 0.0;
-    }
-    for (int e = tid; e < nr * nb; e += blockDim.x) {
-        const int i = e / nb, j = e - i * nb;
-        Pn[i][j] = M[(size_t)(r0 + i) * K + k0 + j];
+    __shared__ double D[CB][CB];
+    __shared__ double sinv[CB];
+    for (int e = tid; e < CB * CB; e += blockDim.x) {
+        const int i = e / CB, j = e - i * CB;
+        D[i][j] = (i < nb && j < nb && j <= i) ? M[(size_t)(k0 + i) * K + k0 + j] : (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
-    if (tid < nr) {
-        for (int j = 0; j < nb; ++j) {
-            double sacc = Pn[tid][j];
-            for (int q = 0; q < j; ++q) sacc -= Pn[tid][q] * D[j][q];
-            Pn[tid][j] = sacc / D[j][j];
-        }
-    }
+    if (tid < CB) sinv[tid] = 1.0 / D[tid][tid];
     __syncthreads();
-    for (int e = tid; e < nr * nb; e += blockDim.x) {
-        const int i = e / nb, j = e - i * nb;
-        M[(size_t)(r0 + i) * K + k0 + j] = Pn[i][j];
+    const int gr = r0 + tid;
+    if (gr >= K) return;
+    double* Mr = M + (size_t)gr * K + k0;
+    double p[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) p[j] = j < nb ? Mr[min(j, nb - 1)] : 0.0;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        double sacc = p[j];
+#pragma unroll
+        for (int q = 0; q < j; ++q) sacc = fma(-p[q], D[j][q], sacc);
+        p[j] = sacc * sinv[j];
     }
+#pragma unroll
+    for (int j = 0; j < CB; ++j)
+        if (j < nb) Mr[j] = p[j];
 }
 
 // Tile (ti, tj), ti >= tj, of the trailing matrix below/right of the panel at k0.
@@ -148,81 +154,90 @@ __global__ void __launch_bounds__(256) k_chol_update(int K, int k0, double* __re
     }
 }
 
-// L y = -g, then L^T delta = y, one workgroup per matrix, blocked by 32-row panels with the
-// solution in LDS (K <= CHOL_BIG_MAXK): per panel, the product of the rows (forward) or columns
-// (backward) already solved with the solution so far, by all threads with coalesced reads of L,
-// then the panel's triangle by one wavefront from LDS, the running values in lanes and each
-// solved entry broadcast by a shuffle.
+// L y = -g, then L^T delta = y, one workgroup per matrix, by 32-row panels with the solution in
+// LDS (K <= CHOL_BIG_MAXK), right-looking: each panel's triangle solved on wavefront 0 (lane i
+// holding row i of the triangle, forward, or column i, backward, in registers, the solved entries
+// passed by lane reads), then every entry not yet solved updated with the panel's solution by all
+// threads (forward: one later row of the panel's columns per thread; backward: one earlier column
+// of the panel's rows per thread, coalesced across the threads).
+constexpr int TRSV_THREADS = 1024;
+
 template <typename TG>
-__global__ void __launch_bounds__(256) k_chol_trsv(int K, const double* __restrict__ A, const TG* __restrict__ g,
-                                                    float* __restrict__ delta, const int* __restrict__ status)
+__global__ void __launch_bounds__(TRSV_THREADS) k_chol_trsv(int K, const double* __restrict__ A,
+                                                             const TG* __restrict__ g, float* __restrict__ delta,
+                                                             const int* __restrict__ status)
 {
     const int mtx = blockIdx.x;
     if (status[mtx] != TDI_OK) return;
     const double* L = A + (size_t)mtx * K * K;
     __shared__ double sy[CHOL_BIG_MAXK];
-    __shared__ double sL[CB][CB + 1];
-    __shared__ double sred[8][CB];
-    __shared__ double sr[CB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < K; i += blockDim.x) sy[i] = -(double)g[(size_t)mtx * K + i];
     __syncthreads();
-    // forward: y_p = L_pp^-1 (b_p - L_p,<p y_<p)
+    // forward: y_p = L_pp^-1 b_p, then b_i -= L_ip y_p for the rows below
     for (int p0 = 0; p0 < K; p0 += CB) {
         const int nb = min(CB, K - p0);
-        {
-            const int row = tid >> 3, sub = tid & 7;
-            double acc = 0.0;
-            if (row < nb)
-                for (int c = sub; c < p0; c += 8) acc += L[(size_t)(p0 + row) * K + c] * sy[c];
-            for (int o = 4; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-            if (sub == 0 && row < nb) sr[row] = sy[p0 + row] - acc;
-        }
-        for (int e = tid; e < nb * nb; e += blockDim.x) {
-            const int i = e / nb, j = e - i * nb;
-            sL[i][j] = L[(size_t)(p0 + i) * K + p0 + j];
+        if (wave == 0) {
+            const bool in = lane < nb;
+            const double* Lr = L + (size_t)(p0 + (in ? lane : 0)) * K + p0;
+            double lr[CB];
+#pragma unroll
+            for (int j = 0; j < CB; ++j) lr[j] = (in && j < lane) ? Lr[min(j, nb - 1)] : 0.0;
+            const double rinv = in ? 1.0 / Lr[in ? lane : 0] : 1.0;
+            double r = in ? sy[p0 + (in ? lane : 0)] : 0.0;
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                if (lane == j) r *= rinv;
+                const double yj = lane_value(r, j);
+                if (lane > j) r = fma(-lr[j], yj, r);
+            }
+            if (in) sy[p0 + lane] = r;
         }
         __syncthreads();
-        if (wave == 0) {
-            double r = lane < nb ? sr[lane] : 0.0;
-            for (int j = 0; j < nb; ++j) {
-                const double yj = __shfl(r, j, 64) / sL[j][j];
-                if (lane == j) r = yj;
-                else if (lane > j && lane < nb) r -= sL[lane][j] * yj;
+        for (int i = p0 + nb + tid; i < K; i += blockDim.x) {
+            const double* Lr = L + (size_t)i * K + p0;
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const int jj = min(j, nb - 1);
+                const double l = Lr[jj];
+                acc = fma(j < nb ? l : 0.0, sy[p0 + jj], acc);
             }
-            if (lane < nb) sy[p0 + lane] = r;
+            sy[i] -= acc;
         }
         __syncthreads();
     }
-    // backward: x_p = L_pp^-T (y_p - L_>p,p^T x_>p)
+    // backward: x_p = L_pp^-T y_p, then y_i -= L_pi x_p for the entries before the panel
     const int last = ((K - 1) / CB) * CB;
     for (int p0 = last; p0 >= 0; p0 -= CB) {
         const int nb = min(CB, K - p0);
-        {
-            const int c = tid & 31, rg = tid >> 5;
-            double acc = 0.0;
-            if (c < nb)
-                for (int r = p0 + nb + rg; r < K; r += 8) acc += L[(size_t)r * K + p0 + c] * sy[r];
-            sred[rg][c] = acc;
-        }
-        for (int e = tid; e < nb * nb; e += blockDim.x) {
-            const int i = e / nb, j = e - i * nb;
-            sL[i][j] = L[(size_t)(p0 + i) * K + p0 + j];
+        if (wave == 0) {
+            const bool in = lane < nb;
+            const double* Lc = L + (size_t)p0 * K + p0 + (in ? lane : 0);
+            double lc[CB];
+#pragma unroll
+            for (int j = 0; j < CB; ++j) lc[j] = (in && j < nb && j > lane) ? Lc[(size_t)min(j, nb - 1) * K] : 0.0;
+            const double rinv = in ? 1.0 / Lc[(size_t)(in ? lane : 0) * K] : 1.0;
+            double r = in ? sy[p0 + (in ? lane : 0)] : 0.0;
+#pragma unroll
+            for (int j = CB - 1; j >= 0; --j) {
+                if (lane == j) r *= rinv;
+                const double xj = lane_value(r, j);
+                if (lane < j) r = fma(-lc[j], xj, r);
+            }
+            if (in) sy[p0 + lane] = r;
         }
         __syncthreads();
-        if (wave == 0) {
-            double r = 0.0;
-            if (lane < nb) {
-                double sacc = 0.0;
-                for (int q = 0; q < 8; ++q) sacc += sred[q][lane];
-                r = sy[p0 + lane] - sacc;
+        for (int i = tid; i < p0; i += blockDim.x) {
+            const double* Lc = L + (size_t)p0 * K + i;
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const int jj = min(j, nb - 1);
+                const double l = Lc[(size_t)jj * K];
+                acc = fma(j < nb ? l : 0.0, sy[p0 + jj], acc);
             }
-            for (int j = nb - 1; j >= 0; --j) {
-                const double xj = __shfl(r, j, 64) / sL[j][j];
-                if (lane == j) r = xj;
-                else if (lane < j) r -= sL[j][lane] * xj;
-            }
-            if (lane < nb) sy[p0 + lane] = r;
+            sy[i] -= acc;
         }
         __syncthreads();
     }
@@ -236,15 +251,15 @@ template <typename TG>
 static int chol_big_run(int n, int K, double* A, const TG* g, float* delta, int* status, hipStream_t s)
 {
     for (int k0 = 0; k0 < K; k0 += CB) {
-        hipLaunchKernelGGL(k_chol_diag, dim3(n), dim3(256), 0, s, K, k0, A, status);
+        hipLaunchKernelGGL(k_chol_diag, dim3(n), dim3(64), 0, s, K, k0, A, status);
         const int below = K - k0 - CB;
         if (below > 0)
-            hipLaunchKernelGGL(k_chol_offdiag, dim3((below + 127) / 128, n), dim3(128), 0, s, K, k0, A, status);
+            hipLaunchKernelGGL(k_chol_offdiag, dim3((below + 63) / 64, n), dim3(64), 0, s, K, k0, A, status);
         const int T = (K - k0 - CB + CB - 1) / CB;
         if (T > 0) hipLaunchKernelGGL(k_chol_update, dim3(T * (T + 1) / 2, n), dim3(256), 0, s, K, k0, A, status);
         if (hipGetLastError() != hipSuccess) return TDI_ERR_HIP;
     }
-    hipLaunchKernelGGL(k_chol_trsv<TG>, dim3(n), dim3(256), 0, s, K, A, g, delta, status);
+    hipLaunchKernelGGL(k_chol_trsv<TG>, dim3(n), dim3(TRSV_THREADS), 0, s, K, A, g, delta, status);
     return hipGetLastError() != hipSuccess ? TDI_ERR_HIP : TDI_OK;
 }
 

@@ -200,6 +200,7 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_count(const FrameGPU* _
                                                                 int4* __restrict__ box)
 {
     __shared__ int swave[RES_NW];
+    __shared__ int4 sbox[RES_NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nact = fl_count(act, n);
     for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
@@ -213,23 +214,22 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_count(const FrameGPU* _
             const int cnt = tile_keys(keys + fr.pix_off, k0, A, key);
             const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
             if (lane == 0) swave[wave] = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
-            if (box && (b0 | b1 | b2)) {
-                // bounding box of the visible pixels (x = k / H, y = k % H)
+            if (box) {
+                // bounding box of the visible pixels (x = k / H, y = k % H), per wavefront
                 int xl = fr.W, xh = -1, yl = fr.H, yh = -1;
+                if (b0 | b1 | b2) {
 #pragma unroll
-                for (int q = 0; q < RES_PER_THREAD; ++q) {
-                    if (key[q] == KEY_EMPTY) continue;
-                    const int k = k0 + q, x = k / fr.H, y = k - x * fr.H;
-                    xl = min(xl, x); xh = max(xh, x); yl = min(yl, y); yh = max(yh, y);
+                    for (int q = 0; q < RES_PER_THREAD; ++q) {
+                        if (key[q] == KEY_EMPTY) continue;
+                        const int k = k0 + q, x = k / fr.H, y = k - x * fr.H;
+                        xl = min(xl, x); xh = max(xh, x); yl = min(yl, y); yh = max(yh, y);
+                    }
+                    for (int o = 32; o > 0; o >>= 1) {
+                        xl = min(xl, __shfl_xor(xl, o, 64)); xh = max(xh, __shfl_xor(xh, o, 64));
+                        yl = min(yl, __shfl_xor(yl, o, 64)); yh = max(yh, __shfl_xor(yh, o, 64));
+                    }
                 }
-                for (int o = 32; o > 0; o >>= 1) {
-                    xl = min(xl, __shfl_xor(xl, o, 64)); xh = max(xh, __shfl_xor(xh, o, 64));
-                    yl = min(yl, __shfl_xor(yl, o, 64)); yh = max(yh, __shfl_xor(yh, o, 64));
-                }
-                if (lane == 0) {
-                    atomicMin(&box[f].x, xl); atomicMax(&box[f].y, xh);
-                    atomicMin(&box[f].z, yl); atomicMax(&box[f].w, yh);
-                }
+                if (lane == 0) sbox[wave] = make_int4(xl, xh, yl, yh);
             }
             __syncthreads();
             if (tid == 0) {
@@ -237,6 +237,22 @@ __global__ void __launch_bounds__(RES_THREADS) k_resolve_count(const FrameGPU* _
 #pragma unroll
                 for (int w = 0; w < RES_NW; ++w) t += swave[w];
                 tcount[(size_t)f * tmax + tile] = t;
+                if (box && t > 0) {
+                    // one workgroup's box; the atomics only where it extends the box read so far
+                    // (the box only grows, so a value read before another workgroup's update
+                    // can only cause a redundant atomic, never a skipped one)
+                    int4 b = sbox[0];
+#pragma unroll
+                    for (int w = 1; w < RES_NW; ++w) {
+                        const int4 c = sbox[w];
+                        b.x = min(b.x, c.x); b.y = max(b.y, c.y); b.z = min(b.z, c.z); b.w = max(b.w, c.w);
+                    }
+                    const int4 cur = box[f];
+                    if (b.x < cur.x) atomicMin(&box[f].x, b.x);
+                    if (b.y > cur.y) atomicMax(&box[f].y, b.y);
+                    if (b.z < cur.z) atomicMin(&box[f].z, b.z);
+                    if (b.w > cur.w) atomicMax(&box[f].w, b.w);
+                }
             }
             __syncthreads();
         }
