@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 
@@ -62,21 +63,89 @@ int launch_misc_light_pack(int n, int K, int j0, int nd, const double* H, const 
 
 using namespace tdi;
 
+// Device blocks that a context keeps for its later batches: creating and destroying batches (the
+// video drivers make one per chunk and reference face size) then reuses workspaces of the same
+// sizes instead of calling hipMalloc/hipFree for gigabytes each time.  Blocks are rounded to 2 MiB
+// (256 B below that); a request takes the smallest cached block that is at most 25% + 2 MiB larger.
+// The pool holds at most a quarter of the device's memory; what would exceed it is freed, and the
+// whole pool is freed when an allocation fails (then retried) and with the context.
+struct DevPool {
+    std::mutex mu;
+    std::multimap<size_t, void*> blocks;
+    size_t cached = 0, limit = 0;
+    DevPool()
+    {
+        size_t fb = 0, tb = 0;
+        limit = hipMemGetInfo(&fb, &tb) == hipSuccess ? tb / 4 : 0;
+    }
+    ~DevPool() { trim(); }
+    void* take(size_t bytes)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        const auto it = blocks.lower_bound(bytes);
+        if (it == blocks.end() || it->first > bytes + bytes / 4 + (2u << 20)) return nullptr;
+        void* p = it->second;
+        cached -= it->first;
+        blocks.erase(it);
+        return p;
+    }
+    void put(size_t bytes, void* p)
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (cached + bytes > limit) { (void)hipFree(p); return; }
+        blocks.emplace(bytes, p);
+        cached += bytes;
+    }
+    void trim()
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (auto& kv : blocks) (void)hipFree(kv.second);
+        blocks.clear();
+        cached = 0;
+    }
+    size_t cached_bytes()
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        return cached;
+    }
+};
+
+// Device allocations of one owner (a context's model arrays, a batch's buffers); a failed
+// allocation returns nullptr and is recorded as a null entry.  With a pool, released blocks go
+// back to it once the device has finished every queued use of them.
 struct Alloc {
     std::vector<void*> ptrs;
+    std::vector<size_t> sizes;
+    std::shared_ptr<DevPool> pool;
     template <typename T>
     T* get(size_t count)
     {
-        void* p = nullptr;
         if (count == 0) count = 1;
-        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return nullptr;
+        size_t bytes = count * sizeof(T);
+        bytes = bytes >= (2u << 20) ? (bytes + (2u << 20) - 1) / (2u << 20) * (2u << 20) : (bytes + 255) / 256 * 256;
+        void* p = pool ? pool->take(bytes) : nullptr;
+        if (!p && hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            if (pool) {
+                pool->trim();
+                if (hipMalloc(&p, bytes) != hipSuccess) { (void)hipGetLastError(); p = nullptr; }
+            }
+        }
         ptrs.push_back(p);
+        sizes.push_back(bytes);
         return (T*)p;
     }
     void release()
     {
-        for (void* p : ptrs) (void)hipFree(p);
+        if (pool && !ptrs.empty()) (void)hipDeviceSynchronize();
+        for (size_t i = 0; i < ptrs.size(); ++i) {
+            if (!ptrs[i]) continue;
+            if (pool) pool->put(sizes[i], ptrs[i]);
+            else (void)hipFree(ptrs[i]);
+        }
         ptrs.clear();
+        sizes.clear();
     }
 };
 
@@ -90,6 +159,7 @@ struct tdi_ctx {
     ModelGPU m_joint{}; // m without the albedo and identity bounds (the joint fit adds them once)
     float taps[15];
     Alloc mem;
+    std::shared_ptr<DevPool> pool;   // blocks for this context's batches
 };
 
 struct Workspace {
@@ -298,6 +368,7 @@ int tdi_ctx_destroy(tdi_ctx* c)
     if (!c) return TDI_OK;
     (void)hipSetDevice(c->device);
     c->mem.release();
+    c->pool.reset();   // freed here unless a batch still holds it
     delete c;
     return TDI_OK;
 }
@@ -461,15 +532,20 @@ extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd,
     b->max_area = maxA;
     hipStream_t s = 0;
     // observations: upload, blur, normalised gradients (once per batch)
+    if (!c->pool) c->pool = std::make_shared<DevPool>();
+    b->mem.pool = c->pool;
     b->obs_g = b->mem.get<float2>(off);
     b->lmk = b->mem.get<float>((size_t)n * 6 * NLMK);
     FrameGPU* dfr = b->mem.get<FrameGPU>(n);
     if (!b->obs_g || !b->lmk || !dfr) { b->mem.release(); delete b; set_error("batch allocation failed"); return TDI_ERR_NOMEM; }
     {
-        float *raw = nullptr, *tmp = nullptr, *blr = nullptr;
-        if (hipMalloc(&raw, sizeof(float) * off) != hipSuccess || hipMalloc(&tmp, sizeof(float) * off) != hipSuccess ||
-            hipMalloc(&blr, sizeof(float) * off) != hipSuccess) {
-            (void)hipFree(raw); (void)hipFree(tmp); (void)hipFree(blr);
+        Alloc stage;
+        stage.pool = c->pool;
+        float* raw = stage.get<float>(off);
+        float* tmp = stage.get<float>(off);
+        float* blr = stage.get<float>(off);
+        if (!raw || !tmp || !blr) {
+            stage.release();
             b->mem.release();
             delete b;
             set_error("observation staging allocation failed");
@@ -493,7 +569,7 @@ extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd,
         (void)hipMemcpy(dfr, b->hframes.data(), sizeof(FrameGPU) * n, hipMemcpyHostToDevice);
         if (!rc) rc = launch_obs_prepare(dfr, n, c->taps, raw, tmp, blr, b->obs_g, maxA, s);
         (void)hipDeviceSynchronize();
-        (void)hipFree(raw); (void)hipFree(tmp); (void)hipFree(blr);
+        stage.release();
         if (rc) { b->mem.release(); delete b; return rc; }
     }
     {
@@ -524,7 +600,8 @@ extern "C" int tdi_batch_create(tdi_ctx* c, int32_t n, const tdi_frame_desc* fd,
         budget_gb = atof(env);
     } else {
         size_t fb = 0, tb = 0;
-        if (hipMemGetInfo(&fb, &tb) == hipSuccess && fb > 0) budget_gb = 0.5 * (double)fb / 1e9;
+        // the context's cached blocks count as free: the workspace reuses them
+        if (hipMemGetInfo(&fb, &tb) == hipSuccess && fb > 0) budget_gb = 0.5 * (double)(fb + c->pool->cached_bytes()) / 1e9;
     }
     const int V = c->m.V, K = c->Lx.K;
     const double per_frame = (double)V * 40 + 64.0 * K * K + 8.0 * resolve_tiles(maxA);

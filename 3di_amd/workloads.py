@@ -111,8 +111,8 @@ def cfg2(n_frames: int = 4096, iters: int = 12, grid: int = 231, k_exp: int = 79
 
 @dataclasses.dataclass
 class LongVideo:
-    """A synthetic long video: per-frame truths, landmarks and cameras on the host; the frames are
-    rendered on the GPU chunk by chunk by `source` (8-bit BGR, width x height)."""
+    """A synthetic long video: per-frame truths, landmarks and cameras on the host; the frames
+    (8-bit BGR, width x height) rendered on the GPU, handed out chunk by chunk by `source`."""
     model: object
     opts: object
     lay: dict
@@ -122,7 +122,7 @@ class LongVideo:
     cams: np.ndarray           # [n, 4]
     width: int
     height: int
-    render_ms: list            # time spent rendering frames, per source call
+    render_ms: list            # time spent rendering frames, per rendered chunk
     call_times: list = dataclasses.field(default_factory=list)   # host clock at each source call
     source: object = None
 
@@ -132,13 +132,16 @@ class LongVideo:
 
 
 def long_video(n_frames: int = 2048, width: int = 1920, height: int = 1080, face_px=(280.0, 420.0),
-               iters: int = 12, crop: int = 896, noise: float = 0.02, seed: int = 0x3D1 + 4) -> LongVideo:
+               iters: int = 12, crop: int = 896, noise: float = 0.02, seed: int = 0x3D1 + 4,
+               resident: bool = True) -> LongVideo:
     """A face video: the pose follows a random walk (0.5 deg and 2 mm per frame per axis, angles
     within +-30 deg), the expression an AR(1) process (rho = 0.95), and the steps are redrawn until
     the 51 landmarks stay 20 px inside the frame with an extent within face_px; the light is fixed
     at the reference's initial values.  The fit's unknowns: 6 pose + 79 expression (the per-frame
     stage fits the light in its own sub-fits).  The frames are the render of the truth (crop x crop
-    around the face) over a grey N(0.5, noise) background, + N(0, noise), as 8-bit BGR."""
+    around the face) over a grey N(0.5, noise) background, + N(0, noise), as 8-bit BGR.
+    resident: every frame rendered into one device buffer here (n x height x width x 3 bytes), the
+    source handing out views of it; else the source renders each chunk when it is asked for it."""
     import torch
     m = cfg2_model(231, 79)
     lay = synth.layout(m, False, False, True, 0)
@@ -208,9 +211,7 @@ def long_video(n_frames: int = 2048, width: int = 1920, height: int = 1080, face
     zero_obs = np.zeros(crop * crop, np.float32)      # the renderer ignores the observation
     frng = np.random.default_rng(seed + 1)
 
-    def source(first: int, count: int):
-        t0 = time.perf_counter()
-        lv.call_times.append(t0)
+    def render(first: int, count: int):
         if "ctx" not in state:
             state["ctx"] = tdi.Context(m, opts)
         frames = []
@@ -228,9 +229,6 @@ def long_video(n_frames: int = 2048, width: int = 1920, height: int = 1080, face
         th = torch.from_numpy(theta[first:first + count].astype(np.float32)).cuda()
         img, _, _ = b.render(th)
         b.close()
-        buf = state.get("buf")
-        if buf is None or buf.shape[0] < count:
-            buf = state["buf"] = torch.empty((count, height, width, 3), dtype=torch.uint8, device="cuda")
         gen = torch.Generator(device="cuda")
         gen.manual_seed(seed + first)
         grey = (0.5 + noise * torch.randn((count, height, width), device="cuda", generator=gen))
@@ -238,13 +236,35 @@ def long_video(n_frames: int = 2048, width: int = 1920, height: int = 1080, face
         noisy = face + noise * torch.randn(face.shape, device="cuda", generator=gen)
         for j, fr in enumerate(frames):
             win = grey[j, fr.y0:fr.y0 + crop, fr.x0:fr.x0 + crop]
-            mask = face[j] != 0
-            win[mask] = noisy[j][mask]
-        out = buf[:count]
-        out.copy_((grey.clamp(0, 1) * 255.0 + 0.5).to(torch.uint8).unsqueeze(-1).expand(count, height, width, 3))
-        torch.cuda.synchronize()
-        lv.render_ms.append(1e3 * (time.perf_counter() - t0))
-        return out
+            win.copy_(torch.where(face[j] != 0, noisy[j], win))
+        return (grey.clamp(0, 1) * 255.0 + 0.5).to(torch.uint8)
+
+    if resident:
+        allf = torch.empty((n_frames, height, width, 3), dtype=torch.uint8, device="cuda")
+        step = 256
+        for first in range(0, n_frames, step):
+            t0 = time.perf_counter()
+            count = min(step, n_frames - first)
+            allf[first:first + count].copy_(render(first, count).unsqueeze(-1).expand(count, height, width, 3))
+            torch.cuda.synchronize()
+            lv.render_ms.append(1e3 * (time.perf_counter() - t0))
+        state.pop("ctx").close()
+
+        def source(first: int, count: int):
+            lv.call_times.append(time.perf_counter())
+            return allf[first:first + count]
+    else:
+        def source(first: int, count: int):
+            t0 = time.perf_counter()
+            lv.call_times.append(t0)
+            buf = state.get("buf")
+            if buf is None or buf.shape[0] < count:
+                buf = state["buf"] = torch.empty((count, height, width, 3), dtype=torch.uint8, device="cuda")
+            out = buf[:count]
+            out.copy_(render(first, count).unsqueeze(-1).expand(count, height, width, 3))
+            torch.cuda.synchronize()
+            lv.render_ms.append(1e3 * (time.perf_counter() - t0))
+            return out
 
     lv.source = source
     return lv
