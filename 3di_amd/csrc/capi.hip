@@ -1654,6 +1654,23 @@ bool solve3(double A[9], const double g[3], double d[3])
 // barrier value in the workspace hold; only the shading and what follows it are redone.
 // Objective, gradient and normal matrix (eval_full's shading, blur, normalised gradients and
 // Jacobian stages):
+// The light block of the normal equations: the light columns of the Jacobian and their J^T J,
+// H [n][4][4] and g [n][4] over the four appended light unknowns (lambda, Lintensity).  The
+// barrier adds nothing to this block: the light has no bounds and moves no landmark.
+static int eval_light_jacobian(tdi_batch* b, FrameList fl, const float* theta, hipStream_t s)
+{
+    tdi_ctx* c = b->ctx;
+    const Layout& Lx = c->Lx;
+    Workspace& w = b->ws;
+    int rc;
+    if ((rc = launch_dI_light(b->m, Lx, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.dI, w.max_area, s)))
+        return rc;
+    Layout Ll{};
+    Ll.K = Ll.nlight = Lx.K - Lx.off_light;
+    return launch_jtj_f32(Ll, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
+                          w.partials, w.S, s);
+}
+
 static int eval_light_full(tdi_batch* b, FrameList fl, const float* theta, hipStream_t s)
 {
     tdi_ctx* c = b->ctx;
@@ -1667,7 +1684,7 @@ static int eval_light_full(tdi_batch* b, FrameList fl, const float* theta, hipSt
     if ((rc = launch_grads(w.frames, fl, w.vis, w.npix, w.cum, w.blurred, b->obs_g, c->opts.data_weight, w.pixg, w.pnb,
                            w.pcoef, w.objp, w.obj, w.max_area, s)))
         return rc;
-    return eval_jacobian(b, fl, theta, 0, s, &Lx);
+    return eval_light_jacobian(b, fl, theta, s);
 }
 
 // Objective only (eval_objective's shading, blur and objective stages):
@@ -1700,7 +1717,7 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
         return TDI_ERR_ARG;
     }
     const Layout& Lx = c->Lx;
-    const int K = c->L.K, Kx = Lx.K, ol = Lx.off_light;
+    const int K = c->L.K, Kx = Lx.K, ol = Lx.off_light, Kl = Kx - ol;   // Kl: the light block's rows in H, g
     const double w = c->opts.data_weight;
     const double alpha = intensity ? 0.3 : 0.5, beta = 0.5, t_min = intensity ? 0.1 : 0.01,
                  dec_tol = intensity ? 0.0025 : 0.0003;
@@ -1736,10 +1753,14 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
             if ((rc = upload_list(W, W.act_e, list, s))) return rc;
             const FrameList fl{W.act_e, n};
             // the first evaluation renders every frame of the chunk; the light changes nothing else
-            if ((rc = first ? eval_full(b, fl, W.theta, 0, s, &Lx) : eval_light_full(b, fl, W.theta, s))) return rc;
+            if (first) {
+                if ((rc = eval_render(b, fl, W.theta, 0, s, &Lx)) || (rc = eval_light_jacobian(b, fl, W.theta, s))) return rc;
+            } else if ((rc = eval_light_full(b, fl, W.theta, s))) {
+                return rc;
+            }
             first = false;
             // the light block, gradient, objectives and pixel counts, one copy
-            if ((rc = launch_misc_light_pack(n, Kx, j0, nd, W.H, W.g, W.obj, W.objb, W.npix, W.lpack, s))) return rc;
+            if ((rc = launch_misc_light_pack(n, Kl, j0 - ol, nd, W.H, W.g, W.obj, W.objb, W.npix, W.lpack, s))) return rc;
             TDI_HIP(hipMemcpyAsync(pk.data(), W.lpack, sizeof(double) * 16 * n, hipMemcpyDeviceToHost, s));
             TDI_HIP(hipStreamSynchronize(s));
             for (int i : list) {
@@ -1791,7 +1812,7 @@ static int light_fit(tdi_batch* b, const float* theta_host, float* lam_host, flo
                 TDI_HIP(hipMemcpyAsync(W.theta_try, xt.data(), sizeof(float) * xt.size(), hipMemcpyHostToDevice, s));
                 if ((rc = upload_list(W, W.act_e, list, s))) return rc;
                 if ((rc = eval_light_objective(b, FrameList{W.act_e, n}, W.theta_try, s))) return rc;
-                if ((rc = launch_misc_light_pack(n, Kx, j0, nd, W.H, W.g, W.obj, W.objb, W.npix, W.lpack, s))) return rc;
+                if ((rc = launch_misc_light_pack(n, Kl, j0 - ol, nd, W.H, W.g, W.obj, W.objb, W.npix, W.lpack, s))) return rc;
                 TDI_HIP(hipMemcpyAsync(pk.data(), W.lpack, sizeof(double) * 16 * n, hipMemcpyDeviceToHost, s));
                 TDI_HIP(hipStreamSynchronize(s));
                 for (int i : list) {

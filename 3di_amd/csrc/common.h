@@ -356,6 +356,10 @@ int launch_ortho_init(const ModelGPU& m, int n, const float* lmk, const float* c
 int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
               const double* P0, const int4* pvert, const float2* pbary, const int* npix, const float4* pixg,
               float* dI, int mean_area, hipStream_t s);
+// the light columns alone ([pixel][4]: lambda, Lintensity) of the four appended light unknowns
+int launch_dI_light(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
+                    const double* P0, const int4* pvert, const float2* pbary, const int* npix, float* dIl, int max_area,
+                    hipStream_t s);
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);
 // J^T J and J^T r for frames with more columns than one workgroup's LDS holds (the identity

@@ -225,6 +225,74 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
 }
 
+// The light columns alone (the light sub-fits, fit_3DMM_lambdas / fit_3DMM_Lintensity, which move
+// only the light): one thread per visible pixel, rows of nl = L.K - L.off_light values (lambda,
+// then Lintensity for the four appended light unknowns), the same shading quantities and FP32
+// expressions as k_dI's light columns.
+__global__ void __launch_bounds__(256) k_dI_light(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
+                                                   const int* __restrict__ act, int n, const float* __restrict__ theta,
+                                                   const double* __restrict__ P0, const int4* __restrict__ pvert,
+                                                   const float2* __restrict__ pbary, const int* __restrict__ npix,
+                                                   float* __restrict__ dIl)
+{
+    __shared__ FrameRot sr;
+    __shared__ float stex[512];
+    const int nl = L.K - L.off_light;
+    const int nact = fl_count(act, n);
+    for (int yi = blockIdx.y; yi < nact; yi += gridDim.y) {
+        const int f = fl_frame(act, yi);
+        const int N = npix[f];
+        if ((int)blockIdx.x * 256 >= N) continue;
+        const FrameGPU fr = frames[f];
+        const float* tf = theta + (size_t)f * L.K;
+        if (threadIdx.x == 0) load_frame_rot(L, fr, tf, sr);
+        for (int q = threadIdx.x; q < L.ntex; q += blockDim.x) stex[q] = tf[L.off_tex + q];
+        __syncthreads();
+        const long long base = fr.pix_off;
+        const double* P0f = P0 + (size_t)f * m.V * 3;
+        for (int pos = blockIdx.x * 256 + threadIdx.x; pos < N; pos += gridDim.x * 256) {
+            const int4 pv = pvert[base + pos];
+            const float2 ab = pbary[base + pos];
+            const float al = ab.x, be = ab.y, ga = gamma_rn(al, be);
+            const int vb = pv.x, vc = pv.y, va = pv.z;
+            double p0d[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                p0d[a] = (double)al * P0f[3 * vb + a] + (double)be * P0f[3 * vc + a] + (double)ga * P0f[3 * va + a];
+            const double* B0 = P0f + 3 * vb;
+            const double* C0 = P0f + 3 * vc;
+            const double* A0 = P0f + 3 * va;
+            const double e1x = B0[0] - A0[0], e1y = B0[1] - A0[1], e1z = B0[2] - A0[2];
+            const double e2x = C0[0] - A0[0], e2y = C0[1] - A0[1], e2z = C0[2] - A0[2];
+            double ndx = e1y * e2z - e1z * e2y, ndy = e1z * e2x - e1x * e2z, ndz = e1x * e2y - e1y * e2x;
+            const double inv = 1.0 / sqrt(ndx * ndx + ndy * ndy + ndz * ndz);
+            ndx *= inv; ndy *= inv; ndz *= inv;
+            const double d0 = (double)sr.lam[0] - sr.tau[0], d1 = (double)sr.lam[1] - sr.tau[1],
+                         d2 = (double)sr.lam[2] - sr.tau[2];
+            const double lx = sr.R[0] * d0 + sr.R[1] * d1 + sr.R[2] * d2;
+            const double ly = sr.R[3] * d0 + sr.R[4] * d1 + sr.R[5] * d2;
+            const double lz = sr.R[6] * d0 + sr.R[7] * d1 + sr.R[8] * d2;
+            const float Id = (float)(ndx * lx + ndy * ly + ndz * lz - (ndx * p0d[0] + ndy * p0d[1] + ndz * p0d[2]));
+            const float nx = (float)ndx, ny = (float)ndy, nz = (float)ndz;
+            const float* rsrc = m.alb ? m.alb + (size_t)f * m.V : m.tex_mean;
+            float rb = rsrc[vb], rc = rsrc[vc], ra = rsrc[va];
+            for (int q = 0; !m.alb && q < L.ntex; ++q) {
+                rb = fmaf(m.tex_basis[(size_t)vb * m.tex_ld + q], stex[q], rb);
+                rc = fmaf(m.tex_basis[(size_t)vc * m.tex_ld + q], stex[q], rc);
+                ra = fmaf(m.tex_basis[(size_t)va * m.tex_ld + q], stex[q], ra);
+            }
+            const float rho = al * rb + be * rc + ga * ra;
+            float rn[3];
+            rn[0] = sr.R[0] * nx + sr.R[3] * ny + sr.R[6] * nz;
+            rn[1] = sr.R[1] * nx + sr.R[4] * ny + sr.R[7] * nz;
+            rn[2] = sr.R[2] * nx + sr.R[5] * ny + sr.R[8] * nz;
+            float* out = dIl + (size_t)(base + pos) * nl;
+            for (int c = 0; c < nl; ++c) out[c] = (c < 3) ? sr.L * rho * rn[c] : rho * Id;
+        }
+        __syncthreads();   // sr and stex are rewritten by the next frame
+    }
+}
+
 // Per-pixel coefficients for the normalised-gradient Jacobian.
 struct PixCoef {
     int4 nb;
@@ -918,6 +986,24 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameL
     if (nsb < 1) nsb = 1;
     hipLaunchKernelGGL(k_dI, dim3(8 * nsb, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0,
                        pvert, pbary, npix, pixg, dI);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_dI_light(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
+                    const double* P0, const int4* pvert, const float2* pbary, const int* npix, float* dIl, int max_area,
+                    hipStream_t s)
+{
+    if (L.nlight != 4 || L.K - L.off_light != 4 || L.ntex > 512) {
+        set_error("dI (light columns): needs the four appended light unknowns (nlight %d, K %d, off_light %d)", L.nlight,
+                  L.K, L.off_light);
+        return TDI_ERR_ARG;
+    }
+    int gx = (max_area + 255) / 256;
+    if (gx > 64) gx = 64;
+    if (gx < 1) gx = 1;
+    hipLaunchKernelGGL(k_dI_light, dim3(gx, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act, fl.n,
+                       theta, P0, pvert, pbary, npix, dIl);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
