@@ -773,7 +773,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(T >= 7
 //               stride KP = K + 1 rounded up to 64); column K holds r = (g^s_x dy - g^s_y dx) / |d|
 //               so that J^T r is the last column of the product, columns past K are zero;
 //   k_syrk_f64  U^T U over 64 x 64 output tiles (upper triangle of tiles), a pixel split per grid
-//               column: 64 pixels per step staged in LDS as FP32, each wavefront runs
+//               column: 64 pixels per step staged in LDS as FP32 (the next step's slices loaded
+//               into registers meanwhile), each wavefront runs
 //               v_mfma_f64_16x16x4f64 for its 16-row block against the tile's four 16-column
 //               blocks (FP32 values widened exactly to FP64); FP64 partials per split, summed in
 //               order by k_syrk_reduce.
@@ -834,16 +835,22 @@ __device__ __forceinline__ void tri_tile(int t, int T, int& I, int& J)
 
 // FP64 MFMA operand layout, v_mfma_f64_16x16x4f64: A[i][k] at lane i + 16 k, B[k][j] at lane
 // j + 16 k; lane l's accumulator element r holds C[l / 16 + 4 r][l % 16].
+// LDS row stride of the staged slices: the 64 lanes (kq, li) of an operand read,
+// [4 ks + kq][16 w + li], fall in 64 distinct banks (kq * 16 + li).
+constexpr int UB_SP = UB_T + 16;
+
 __global__ void __launch_bounds__(256) k_syrk_f64(int KP, const FrameGPU* __restrict__ frames,
                                                    const int* __restrict__ act, int n, const int* __restrict__ npix,
                                                    const float* __restrict__ U, int S, double* __restrict__ Hp)
 {
-    __shared__ float sA[UB_PX][UB_T + 1];                  // [pixel][column], padded rows
-    __shared__ float sB[UB_PX][UB_T + 1];
+    __shared__ __align__(16) float sA[UB_PX][UB_SP];       // [pixel][column]
+    __shared__ __align__(16) float sB[UB_PX][UB_SP];
     const int T = KP / UB_T, NT = T * (T + 1) / 2;
     int I, J;
     tri_tile(blockIdx.x, T, I, J);
     const bool diag = I == J;
+    // float4 items per thread and step: 64 pixels x 16 float4 per slice, one slice on the diagonal
+    const int nq = diag ? 4 : 8;
     const int s = blockIdx.y;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -858,19 +865,35 @@ __global__ void __launch_bounds__(256) k_syrk_f64(int KP, const FrameGPU* __rest
         v4d acc[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[b] = (v4d){0.0, 0.0, 0.0, 0.0};
-        for (int kb = p0; kb < p1; kb += UB_PX) {
-            // 64 pixels x 64 columns per slice, a row segment of 64 floats per pixel (coalesced)
-            const int nitems = (diag ? 1 : 2) * UB_PX * UB_T;
-            for (int it = tid; it < nitems; it += 256) {
-                const int sl = it / (UB_PX * UB_T);
-                const int r = it - sl * (UB_PX * UB_T);
-                const int p = r / UB_T, c = r - p * UB_T;
+        // the next step's slices are loaded into registers while the current step's MFMAs run:
+        // item it = tid + 256 q is slice it / 1024 (I, then J), pixel (it % 1024) / 16, columns
+        // 4 (it % 16) .. + 3 (16 consecutive lanes read a pixel's 64-float row segment)
+        float4 pre[8];
+        auto fetch = [&](int kb) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (q >= nq) continue;
+                const int it = tid + 256 * q;
+                const int sl = it >> 10, r = it & 1023;
+                const int p = r >> 4, c4 = r & 15;
                 const int pix = kb + p;
-                const float v = pix < p1 ? Uf[(size_t)pix * KP + (sl ? J : I) * UB_T + c] : 0.f;
-                (sl ? sB : sA)[p][c] = v;
+                pre[q] = pix < p1 ? *(const float4*)(Uf + (size_t)pix * KP + (sl ? J : I) * UB_T + 4 * c4)
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        };
+        if (p0 < p1) fetch(p0);
+        for (int kb = p0; kb < p1; kb += UB_PX) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (q >= nq) continue;
+                const int it = tid + 256 * q;
+                const int sl = it >> 10, r = it & 1023;
+                const int p = r >> 4, c4 = r & 15;
+                *(float4*)&(sl ? sB : sA)[p][4 * c4] = pre[q];
             }
             __syncthreads();
-            const float (*sb)[UB_T + 1] = diag ? sA : sB;
+            if (kb + UB_PX < p1) fetch(kb + UB_PX);
+            const float (*sb)[UB_SP] = diag ? sA : sB;
 #pragma unroll 4
             for (int ks = 0; ks < UB_PX / 4; ++ks) {
                 const double a = (double)sA[4 * ks + kq][16 * wave + li];
