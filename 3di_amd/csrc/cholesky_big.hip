@@ -11,9 +11,9 @@
 //   k_chol_update  a grid of 32x32 tiles of the trailing lower triangle per matrix:
 //                  A22 -= L21 L21^T, both 32-row slices of the panel staged in LDS;
 //   k_chol_trsv    one workgroup per matrix: L y = -g, then L^T delta = y, by 32-row panels,
-//                  right-looking (the solution in LDS; each panel's triangle on one wavefront
-//                  from registers, then every later entry updated by all threads with
-//                  independent loads of the panel's rows or columns).
+//                  right-looking (the solution and the panel's diagonal block in LDS; each
+//                  panel's triangle on one wavefront from registers, then every later entry
+//                  updated by all threads with coalesced reads of the panel's rows or columns).
 // A non-positive (or NaN) pivot marks the matrix TDI_NOT_PD; its later launches do nothing.
 #include "common.h"
 
@@ -88,9 +88,21 @@ __global__ void __launch_bounds__(64) k_chol_offdiag(int K, int k0, double* __re
     if (r0 >= K) return;
     __shared__ double D[CB][CB];
     __shared__ double sinv[CB];
-    for (int e = tid; e < CB * CB; e += blockDim.x) {
-        const int i = e / CB, j = e - i * CB;
-        D[i][j] = (i < nb && j < nb && j <= i) ? M[(size_t)(k0 + i) * K + k0 + j] : (i == j ? 1.0 : 0.0);
+    {
+        // every load of the block issued before the LDS stores (64 threads, 16 entries each)
+        double v[CB * CB / 64];
+#pragma unroll
+        for (int q = 0; q < CB * CB / 64; ++q) {
+            const int e = tid + 64 * q, i = e / CB, j = e - i * CB;
+            const bool in = i < nb && j < nb && j <= i;
+            const double x = M[(size_t)(k0 + min(i, nb - 1)) * K + k0 + min(j, nb - 1)];
+            v[q] = in ? x : (i == j ? 1.0 : 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < CB * CB / 64; ++q) {
+            const int e = tid + 64 * q;
+            D[e / CB][e % CB] = v[q];
+        }
     }
     __syncthreads();
     if (tid < CB) sinv[tid] = 1.0 / D[tid][tid];
@@ -129,11 +141,24 @@ __global__ void __launch_bounds__(256) k_chol_update(int K, int k0, double* __re
     __shared__ double Li[CB][CB + 1], Lj[CB][CB + 1];
     const int tid = threadIdx.x;
     const int nb = min(CB, K - k0);
-    for (int e = tid; e < CB * CB; e += blockDim.x) {
-        const int r = e / CB, c = e - r * CB;
-        const int ri = r0 + ti * CB + r, rj = r0 + tj * CB + r;
-        Li[r][c] = (ri < K && c < nb) ? M[(size_t)ri * K + k0 + c] : 0.0;
-        Lj[r][c] = (rj < K && c < nb) ? M[(size_t)rj * K + k0 + c] : 0.0;
+    {
+        // every load of both slices issued before the LDS stores (256 threads, 4 entries each)
+        double vi[CB * CB / 256], vj[CB * CB / 256];
+#pragma unroll
+        for (int q = 0; q < CB * CB / 256; ++q) {
+            const int e = tid + 256 * q, r = e / CB, c = e - r * CB;
+            const int ri = r0 + ti * CB + r, rj = r0 + tj * CB + r;
+            const double xi = M[(size_t)min(ri, K - 1) * K + k0 + min(c, nb - 1)];
+            const double xj = M[(size_t)min(rj, K - 1) * K + k0 + min(c, nb - 1)];
+            vi[q] = (ri < K && c < nb) ? xi : 0.0;
+            vj[q] = (rj < K && c < nb) ? xj : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < CB * CB / 256; ++q) {
+            const int e = tid + 256 * q, r = e / CB, c = e - r * CB;
+            Li[r][c] = vi[q];
+            Lj[r][c] = vj[q];
+        }
     }
     __syncthreads();
     // 256 threads x 4 entries: thread (a, b4) computes row a, columns b4..b4+3 of the tile
@@ -155,11 +180,13 @@ __global__ void __launch_bounds__(256) k_chol_update(int K, int k0, double* __re
 }
 
 // L y = -g, then L^T delta = y, one workgroup per matrix, by 32-row panels with the solution in
-// LDS (K <= CHOL_BIG_MAXK), right-looking: each panel's triangle solved on wavefront 0 (lane i
-// holding row i of the triangle, forward, or column i, backward, in registers, the solved entries
-// passed by lane reads), then every entry not yet solved updated with the panel's solution by all
-// threads (forward: one later row of the panel's columns per thread; backward: one earlier column
-// of the panel's rows per thread, coalesced across the threads).
+// LDS (K <= CHOL_BIG_MAXK), right-looking: each panel's 32 x 32 diagonal block staged in LDS by all
+// threads, its triangle solved on wavefront 0 (lane i holding row i, forward, or column i,
+// backward, in registers, the solved entries passed by lane reads), then every entry not yet
+// solved updated with the panel's solution by all threads: forward, eight lanes per later row,
+// each reading four consecutive entries of the panel's columns (a row's 32 entries in one
+// coalesced read) and the eight partial sums combined by lane shuffles; backward, one earlier
+// column per thread, the panel's rows read coalesced across the threads.
 constexpr int TRSV_THREADS = 1024;
 
 template <typename TG>
@@ -171,20 +198,28 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_chol_trsv(int K, const double*
     if (status[mtx] != TDI_OK) return;
     const double* L = A + (size_t)mtx * K * K;
     __shared__ double sy[CHOL_BIG_MAXK];
+    __shared__ double sD[CB][CB + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < K; i += blockDim.x) sy[i] = -(double)g[(size_t)mtx * K + i];
-    __syncthreads();
+    // the diagonal block of the panel at p0 (zero past the matrix), one entry per thread
+    auto stage = [&](int p0, int nb) {
+        const int i = tid >> 5, j = tid & 31;
+        const double x = L[(size_t)(p0 + min(i, nb - 1)) * K + p0 + min(j, nb - 1)];
+        sD[i][j] = (i < nb && j < nb) ? x : 0.0;
+    };
     // forward: y_p = L_pp^-1 b_p, then b_i -= L_ip y_p for the rows below
     for (int p0 = 0; p0 < K; p0 += CB) {
         const int nb = min(CB, K - p0);
+        stage(p0, nb);
+        __syncthreads();
         if (wave == 0) {
             const bool in = lane < nb;
-            const double* Lr = L + (size_t)(p0 + (in ? lane : 0)) * K + p0;
+            const int r0 = in ? lane : 0;
             double lr[CB];
 #pragma unroll
-            for (int j = 0; j < CB; ++j) lr[j] = (in && j < lane) ? Lr[min(j, nb - 1)] : 0.0;
-            const double rinv = in ? 1.0 / Lr[in ? lane : 0] : 1.0;
-            double r = in ? sy[p0 + (in ? lane : 0)] : 0.0;
+            for (int j = 0; j < CB; ++j) lr[j] = (in && j < lane) ? sD[r0][j] : 0.0;
+            const double rinv = in ? 1.0 / sD[r0][r0] : 1.0;
+            double r = in ? sy[p0 + r0] : 0.0;
 #pragma unroll
             for (int j = 0; j < CB; ++j) {
                 if (lane == j) r *= rinv;
@@ -194,16 +229,23 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_chol_trsv(int K, const double*
             if (in) sy[p0 + lane] = r;
         }
         __syncthreads();
-        for (int i = p0 + nb + tid; i < K; i += blockDim.x) {
-            const double* Lr = L + (size_t)i * K + p0;
-            double acc = 0.0;
+        {
+            const int sub = tid & 7, c0 = 4 * sub;
+            double yv[4];
 #pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                const int jj = min(j, nb - 1);
-                const double l = Lr[jj];
-                acc = fma(j < nb ? l : 0.0, sy[p0 + jj], acc);
+            for (int q = 0; q < 4; ++q) yv[q] = c0 + q < nb ? sy[p0 + c0 + q] : 0.0;
+            for (int i0 = p0 + nb; i0 < K; i0 += TRSV_THREADS / 8) {
+                const int i = i0 + (tid >> 3);
+                const int ic = min(i, K - 1);
+                const double* Lr = L + (size_t)ic * K + p0;
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc = fma(Lr[min(c0 + q, nb - 1)], yv[q], acc);
+                acc += __shfl_xor(acc, 1, 64);
+                acc += __shfl_xor(acc, 2, 64);
+                acc += __shfl_xor(acc, 4, 64);
+                if (sub == 0 && i < K) sy[i] -= acc;
             }
-            sy[i] -= acc;
         }
         __syncthreads();
     }
@@ -211,14 +253,16 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_chol_trsv(int K, const double*
     const int last = ((K - 1) / CB) * CB;
     for (int p0 = last; p0 >= 0; p0 -= CB) {
         const int nb = min(CB, K - p0);
+        stage(p0, nb);
+        __syncthreads();
         if (wave == 0) {
             const bool in = lane < nb;
-            const double* Lc = L + (size_t)p0 * K + p0 + (in ? lane : 0);
+            const int c = in ? lane : 0;
             double lc[CB];
 #pragma unroll
-            for (int j = 0; j < CB; ++j) lc[j] = (in && j < nb && j > lane) ? Lc[(size_t)min(j, nb - 1) * K] : 0.0;
-            const double rinv = in ? 1.0 / Lc[(size_t)(in ? lane : 0) * K] : 1.0;
-            double r = in ? sy[p0 + (in ? lane : 0)] : 0.0;
+            for (int j = 0; j < CB; ++j) lc[j] = (in && j > lane) ? sD[j][c] : 0.0;
+            const double rinv = in ? 1.0 / sD[c][c] : 1.0;
+            double r = in ? sy[p0 + c] : 0.0;
 #pragma unroll
             for (int j = CB - 1; j >= 0; --j) {
                 if (lane == j) r *= rinv;
