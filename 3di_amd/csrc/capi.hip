@@ -1267,6 +1267,164 @@ static int lmk_slack_run(tdi_batch* b, float* theta, int32_t k_exp, int mode, co
     return rc;
 }
 
+// The joint landmark-only fit over the batch's frames (one group) and its terms: the per-frame
+// terms in one launch, the frames' matrices placed on the joint one with the once-counted terms,
+// the FP64 large-K Cholesky for the Newton step, the line searches from the host (few trials per
+// step, each one launch over the frames).  theta host [n][K]; zin/Hout/gout/objout for the terms.
+static int lmk_joint(tdi_batch* b, float* theta, int Ka, int Ke, int slack, const double* zin, double tau,
+                     double* Hout, double* gout, double* objout, tdi_lmk_report* report, const char* who)
+{
+    if (!b || !theta || (zin && (!Hout || !gout || !objout)) || (!zin && !report)) {
+        set_error("%s: null argument", who);
+        return TDI_ERR_ARG;
+    }
+    tdi_ctx* c = b->ctx;
+    if (!c->m.lmk_ids) { set_error("%s: the model has no landmark ids", who); return TDI_ERR_ARG; }
+    const int n = b->n, K = c->L.K, S1 = slack ? 1 : 0;
+    const int Ks = Ka + S1, nf = Ks + Ke + 6;
+    const long long KJl = Ks + (long long)n * (Ke + 6);
+    if (n < 1 || KJl > CHOL_BIG_MAXK) {
+        set_error("%s: %d frames, %lld joint unknowns (at most %d)", who, n, KJl, CHOL_BIG_MAXK);
+        return TDI_ERR_ARG;
+    }
+    const int KJ = (int)KJl;
+    (void)hipSetDevice(c->device);
+    hipStream_t s = 0;
+    Alloc A;
+    A.pool = c->pool;
+    struct Release {
+        Alloc& a;
+        ~Release() { (void)hipDeviceSynchronize(); a.release(); }
+    } release{A};
+    FrameGPU* dfr = A.get<FrameGPU>(n);
+    float* dth = A.get<float>(K);
+    double* dz = A.get<double>(KJ);
+    double* objf = A.get<double>(n + 1);   // the frames' objectives, then the once-counted terms'
+    double* gminf = A.get<double>(n);
+    double* Hf = A.get<double>((size_t)n * nf * nf);
+    double* gf = A.get<double>((size_t)n * nf);
+    double* J = A.get<double>(lmk_joint_scratch_doubles(n, Ka, Ke, S1));
+    double* HJ = A.get<double>((size_t)KJ * KJ);
+    double* gJ = A.get<double>(KJ);
+    double* dd = A.get<double>(KJ);
+    int* dst = A.get<int>(1);
+    for (void* p : A.ptrs)
+        if (!p) { set_error("%s: allocation failed", who); return TDI_ERR_NOMEM; }
+    TDI_HIP(hipMemcpyAsync(dfr, b->hframes.data(), sizeof(FrameGPU) * n, hipMemcpyHostToDevice, s));
+    TDI_HIP(hipMemcpyAsync(dth, theta, sizeof(float) * K, hipMemcpyHostToDevice, s));
+    std::vector<double> hobj(n + 1), hgmin(n);
+    // objective (and with mode 1 the joint normal equations in HJ, gJ) at zv
+    auto eval = [&](const std::vector<double>& zv, double tau_b, int mode, double& total, bool want_gmin) -> int {
+        TDI_HIP(hipMemcpyAsync(dz, zv.data(), sizeof(double) * KJ, hipMemcpyHostToDevice, s));
+        int rc = launch_lmk_joint(c->m, c->L, dfr, b->lmk, dth, n, Ka, Ke, S1, dz, mode, objf, gminf, Hf, gf, J, s);
+        if (!rc && mode) rc = launch_assemble_joint(n, nf, Ks, Hf, gf, HJ, gJ, s);
+        if (!rc) rc = launch_lmk_joint_shared(c->m, c->L, Ka, S1, KJ, dz, tau_b, mode, objf + n, HJ, gJ, s);
+        if (rc) return rc;
+        TDI_HIP(hipMemcpyAsync(hobj.data(), objf, sizeof(double) * (n + 1), hipMemcpyDeviceToHost, s));
+        if (want_gmin) TDI_HIP(hipMemcpyAsync(hgmin.data(), gminf, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipStreamSynchronize(s));
+        total = hobj[n];
+        for (int f = 0; f < n; ++f) total += hobj[f];
+        return TDI_OK;
+    };
+    std::vector<double> z(KJ), zt(KJ), d(KJ);
+    if (zin) {
+        std::copy(zin, zin + KJ, z.begin());
+        double obj = 0.0;
+        int rc = eval(z, tau, 1, obj, false);
+        if (rc) return rc;
+        TDI_HIP(hipMemcpyAsync(Hout, HJ, sizeof(double) * (size_t)KJ * KJ, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipMemcpyAsync(gout, gJ, sizeof(double) * KJ, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipStreamSynchronize(s));
+        *objout = obj;
+        return TDI_OK;
+    }
+    const Layout& L = c->L;
+    for (int k = 0; k < Ka; ++k) z[k] = theta[L.off_id + k];
+    for (int t = 0; t < n; ++t) {
+        const float* th = theta + (size_t)t * K;
+        double* zf = &z[(size_t)Ks + (size_t)t * (Ke + 6)];
+        for (int k = 0; k < Ke; ++k) zf[k] = th[L.off_exp + k];
+        for (int a = 0; a < 6; ++a) zf[Ke + a] = th[L.off_pose + a];
+    }
+    tdi_lmk_report rep{};
+    int rc = TDI_OK;
+    if (S1) {
+        // set_minimal_slack: s = max over the frames of 5 - min G at s = 0
+        z[Ka] = 0.0;
+        double obj = 0.0;
+        if ((rc = eval(z, 0.0, 0, obj, true))) return rc;
+        double hi = -INFINITY;
+        for (int t = 0; t < n; ++t) {
+            const double st = 5.0 - hgmin[t];
+            hi = (st != st || hi != hi) ? NAN : std::max(hi, st);
+        }
+        z[Ka] = hi;
+        rep.slack0 = (float)hi;
+    }
+    double tau_b = 10.0;
+    bool success = false;
+    for (int round = 0; round < 5 && !success; ++round) {
+        tau_b *= 10.0;
+        rep.rounds++;
+        for (int it = 0; it < 100; ++it) {
+            if (S1 && z[Ka] < -0.01) { success = true; break; }
+            double obj = 0.0;
+            if ((rc = eval(z, tau_b, 1, obj, false))) return rc;
+            rep.iterations++;
+            if ((rc = launch_cholesky_big_f64(1, KJ, HJ, gJ, dd, dst, s))) return rc;
+            int hst = 0;
+            TDI_HIP(hipMemcpyAsync(d.data(), dd, sizeof(double) * KJ, hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipMemcpyAsync(&hst, dst, sizeof(int), hipMemcpyDeviceToHost, s));
+            TDI_HIP(hipStreamSynchronize(s));
+            if (hst != TDI_OK) break;
+            bool stop = false;
+            double t = 1.0;
+            for (int inner = 0; inner < 100; ++inner) {
+                if (t < 0.01) { stop = true; break; }
+                for (int j = 0; j < KJ; ++j) zt[j] = z[j] + t * d[j];
+                double ot = 0.0;
+                if ((rc = eval(zt, tau_b, 0, ot, false))) return rc;
+                if (ot < obj) {
+                    z = zt;
+                    rep.accepted++;
+                    break;
+                }
+                t *= 0.5;
+            }
+            if (stop) break;
+        }
+    }
+    rep.status = (!S1 || success) ? TDI_OK : TDI_LMK_INFEASIBLE;
+    rep.slack = S1 ? (float)z[Ka] : 0.f;
+    if (rep.status == TDI_OK)
+        for (int t = 0; t < n; ++t) {
+            float* th = theta + (size_t)t * K;
+            const double* zf = &z[(size_t)Ks + (size_t)t * (Ke + 6)];
+            for (int k = 0; k < Ka; ++k) th[L.off_id + k] = (float)z[k];
+            for (int k = 0; k < Ke; ++k) th[L.off_exp + k] = (float)zf[k];
+            for (int a = 0; a < 6; ++a) th[L.off_pose + a] = (float)zf[Ke + a];
+        }
+    *report = rep;
+    return TDI_OK;
+}
+
+int tdi_fit_landmarks_joint(tdi_batch* b, float* theta, int32_t k_id, int32_t k_exp, int32_t slack,
+                            tdi_lmk_report* report)
+{
+    return lmk_joint(b, theta, k_id, k_exp, slack, nullptr, 0.0, nullptr, nullptr, nullptr, report,
+                     "tdi_fit_landmarks_joint");
+}
+
+int tdi_landmark_joint_terms(tdi_batch* b, const float* theta, int32_t k_id, int32_t k_exp, int32_t slack,
+                             const double* z, double tau, double* H, double* g, double* obj)
+{
+    if (!z) { set_error("tdi_landmark_joint_terms: null argument"); return TDI_ERR_ARG; }
+    // the terms only read theta
+    return lmk_joint(b, const_cast<float*>(theta), k_id, k_exp, slack, z, tau, H, g, obj, nullptr,
+                     "tdi_landmark_joint_terms");
+}
+
 int tdi_fit_landmarks_slack(tdi_batch* b, float* theta, int32_t k_exp, tdi_lmk_report* reports, void* stream)
 {
     return lmk_slack_run(b, theta, k_exp, 0, nullptr, 0.0, nullptr, nullptr, nullptr, reports, (hipStream_t)stream,

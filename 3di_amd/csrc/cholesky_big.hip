@@ -189,9 +189,9 @@ __global__ void __launch_bounds__(256) k_chol_update(int K, int k0, double* __re
 // column per thread, the panel's rows read coalesced across the threads.
 constexpr int TRSV_THREADS = 1024;
 
-template <typename TG>
+template <typename TG, typename TD>
 __global__ void __launch_bounds__(TRSV_THREADS) k_chol_trsv(int K, const double* __restrict__ A,
-                                                             const TG* __restrict__ g, float* __restrict__ delta,
+                                                             const TG* __restrict__ g, TD* __restrict__ delta,
                                                              const int* __restrict__ status)
 {
     const int mtx = blockIdx.x;
@@ -285,14 +285,14 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_chol_trsv(int K, const double*
         }
         __syncthreads();
     }
-    for (int i = tid; i < K; i += blockDim.x) delta[(size_t)mtx * K + i] = (float)sy[i];
+    for (int i = tid; i < K; i += blockDim.x) delta[(size_t)mtx * K + i] = (TD)sy[i];
 }
 
 } // namespace
 
 // Factorisation and solve on the FP64 matrices A (overwritten by the factor); g FP32 or FP64.
-template <typename TG>
-static int chol_big_run(int n, int K, double* A, const TG* g, float* delta, int* status, hipStream_t s)
+template <typename TG, typename TD>
+static int chol_big_run(int n, int K, double* A, const TG* g, TD* delta, int* status, hipStream_t s)
 {
     for (int k0 = 0; k0 < K; k0 += CB) {
         hipLaunchKernelGGL(k_chol_diag, dim3(n), dim3(64), 0, s, K, k0, A, status);
@@ -303,7 +303,7 @@ static int chol_big_run(int n, int K, double* A, const TG* g, float* delta, int*
         if (T > 0) hipLaunchKernelGGL(k_chol_update, dim3(T * (T + 1) / 2, n), dim3(256), 0, s, K, k0, A, status);
         if (hipGetLastError() != hipSuccess) return TDI_ERR_HIP;
     }
-    hipLaunchKernelGGL(k_chol_trsv<TG>, dim3(n), dim3(TRSV_THREADS), 0, s, K, A, g, delta, status);
+    hipLaunchKernelGGL((k_chol_trsv<TG, TD>), dim3(n), dim3(TRSV_THREADS), 0, s, K, A, g, delta, status);
     return hipGetLastError() != hipSuccess ? TDI_ERR_HIP : TDI_OK;
 }
 
@@ -334,7 +334,8 @@ int launch_cholesky_big(int n, int K, const float* H, const float* g, float* del
     return rc;
 }
 
-int launch_cholesky_big_f64(int n, int K, double* A, const double* g, float* delta, int* status, hipStream_t s)
+template <typename TD>
+static int cholesky_big_f64(int n, int K, double* A, const double* g, TD* delta, int* status, hipStream_t s)
 {
     if (K > CHOL_BIG_MAXK) {
         set_error("cholesky: K=%d > %d", K, CHOL_BIG_MAXK);
@@ -347,6 +348,16 @@ int launch_cholesky_big_f64(int n, int K, double* A, const double* g, float* del
     TDI_HIP(hipStreamSynchronize(s));
     if (rc) set_error("cholesky (large K): launch failed");
     return rc;
+}
+
+int launch_cholesky_big_f64(int n, int K, double* A, const double* g, float* delta, int* status, hipStream_t s)
+{
+    return cholesky_big_f64(n, K, A, g, delta, status, s);
+}
+
+int launch_cholesky_big_f64(int n, int K, double* A, const double* g, double* delta, int* status, hipStream_t s)
+{
+    return cholesky_big_f64(n, K, A, g, delta, status, s);
 }
 
 } // namespace tdi

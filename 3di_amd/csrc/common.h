@@ -389,8 +389,20 @@ constexpr int CHOL_BIG_MAXK = 4096;
 int launch_cholesky_big(int n, int K, const float* H, const float* g, float* delta, int* status, hipStream_t s);
 // The same on FP64 matrices A [n][K][K] factored in place, g FP64 [n][K].
 int launch_cholesky_big_f64(int n, int K, double* A, const double* g, float* delta, int* status, hipStream_t s);
+int launch_cholesky_big_f64(int n, int K, double* A, const double* g, double* delta, int* status, hipStream_t s);
 // identity.hip: the identity stage's joint system from the frames' normal equations.
 int launch_assemble_joint(int n, int K, int Ks, const double* H, const float* g, double* HJ, double* gJ, hipStream_t s);
+int launch_assemble_joint(int n, int K, int Ks, const double* H, const double* g, double* HJ, double* gJ, hipStream_t s);
+// the joint landmark-only fit over a group of frames: per-frame terms (mode 0 objective and least
+// constraint value, mode 1 also gradient and Gauss-Newton matrix over the frame's
+// [alpha | s | eps | tau | u]) and the terms counted once (identity bounds, tau_b s)
+constexpr int LMK_JOINT_MAXN = 320;   // unknowns per frame: k_id + slack + k_exp + 6
+size_t lmk_joint_scratch_doubles(int n, int Ka, int Ke, int S1);
+int launch_lmk_joint(const ModelGPU& m, const Layout& L, const FrameGPU* frames, const float* lmk,
+                     const float* theta0, int n, int Ka, int Ke, int S1, const double* z, int mode, double* objf,
+                     double* gminf, double* H, double* g, double* J, hipStream_t s);
+int launch_lmk_joint_shared(const ModelGPU& m, const Layout& L, int Ka, int S1, long long KJ, const double* z,
+                            double tau_b, int mode, double* obj, double* HJ, double* gJ, hipStream_t s);
 int launch_shared_bounds(int Ks, long long KJ, const float* x, const float* lb, const float* ub, int mode, double* obj,
                          double* HJ, double* gJ, hipStream_t s);
 

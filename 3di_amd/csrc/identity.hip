@@ -17,9 +17,10 @@ namespace tdi {
 
 namespace {
 
-// HJ [KJ][KJ] FP64 and gJ [KJ] FP64 from the frames' H [n][K][K] FP64 and g [n][K] FP32.
+// HJ [KJ][KJ] FP64 and gJ [KJ] FP64 from the frames' H [n][K][K] FP64 and g [n][K] (FP32 or FP64).
+template <typename TG>
 __global__ void __launch_bounds__(256) k_assemble(int n, int K, int Ks, const double* __restrict__ H,
-                                                   const float* __restrict__ g, double* __restrict__ HJ,
+                                                   const TG* __restrict__ g, double* __restrict__ HJ,
                                                    double* __restrict__ gJ)
 {
     const int Kf = K - Ks;
@@ -85,14 +86,25 @@ __global__ void __launch_bounds__(256) k_shared_bounds(int Ks, long long KJ, con
 
 } // namespace
 
-int launch_assemble_joint(int n, int K, int Ks, const double* H, const float* g, double* HJ, double* gJ, hipStream_t s)
+template <typename TG>
+static int assemble_joint(int n, int K, int Ks, const double* H, const TG* g, double* HJ, double* gJ, hipStream_t s)
 {
     const long long KJ = Ks + (long long)n * (K - Ks);
     const long long total = KJ * KJ + KJ;
     const unsigned blocks = (unsigned)std::min<long long>((total + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(256), 0, s, n, K, Ks, H, g, HJ, gJ);
+    hipLaunchKernelGGL(k_assemble<TG>, dim3(blocks), dim3(256), 0, s, n, K, Ks, H, g, HJ, gJ);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
+}
+
+int launch_assemble_joint(int n, int K, int Ks, const double* H, const float* g, double* HJ, double* gJ, hipStream_t s)
+{
+    return assemble_joint(n, K, Ks, H, g, HJ, gJ, s);
+}
+
+int launch_assemble_joint(int n, int K, int Ks, const double* H, const double* g, double* HJ, double* gJ, hipStream_t s)
+{
+    return assemble_joint(n, K, Ks, H, g, HJ, gJ, s);
 }
 
 int launch_shared_bounds(int Ks, long long KJ, const float* x, const float* lb, const float* ub, int mode, double* obj,

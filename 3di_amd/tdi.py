@@ -74,6 +74,9 @@ def lib():
         L.tdi_batch_autotune.argtypes = [_vp, _vp, C.c_int32, _vp, _vp]
         L.tdi_batch_launch_shape.argtypes = [_vp, _vp]
         L.tdi_landmark_slack_terms.argtypes = [_vp, _vp, C.c_int32, _vp, C.c_double, _vp, _vp, _vp, _vp]
+        L.tdi_fit_landmarks_joint.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp]
+        L.tdi_landmark_joint_terms.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, C.c_double, _vp, _vp,
+                                               _vp]
         L.tdi_prepare_crops.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         L.tdi_prepare_crops_noalloc.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
         L.tdi_paint_innermouth_black.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, C.c_float,
@@ -291,6 +294,31 @@ class Batch:
         _check(lib().tdi_landmark_slack_terms(self.h, _dptr(th), int(k_exp), _dptr(z), float(tau), _dptr(H), _dptr(g),
                                               _dptr(obj), _stream()))
         return H, g, obj
+
+    def joint_landmark_unknowns(self, k_id: int, k_exp: int, slack: bool) -> int:
+        return int(k_id) + (1 if slack else 0) + self.n * (int(k_exp) + 6)
+
+    def fit_landmarks_joint(self, theta0, k_id: int, k_exp: int, slack: bool):
+        """The joint landmark-only fit over the batch's frames (tdi_fit_landmarks_joint): theta0
+        host [n, K]; returns (theta [n, K] float32, report)."""
+        th = np.ascontiguousarray(np.array(theta0, np.float32).reshape(self.n, self.K))
+        rep = abi.tdi_lmk_report()
+        _check(lib().tdi_fit_landmarks_joint(self.h, th.ctypes.data, int(k_id), int(k_exp), int(bool(slack)),
+                                             C.addressof(rep)))
+        return th, rep
+
+    def landmark_joint_terms(self, theta, k_id: int, k_exp: int, slack: bool, z, tau: float):
+        """(obj, g [N], H [N, N]) float64 of that fit at the joint unknowns z (host, FP64)."""
+        th = np.ascontiguousarray(np.array(theta, np.float32).reshape(self.n, self.K))
+        N = self.joint_landmark_unknowns(k_id, k_exp, slack)
+        z = np.ascontiguousarray(np.array(z, np.float64).reshape(N))
+        H = np.zeros((N, N))
+        g = np.zeros(N)
+        obj = C.c_double(0.0)
+        _check(lib().tdi_landmark_joint_terms(self.h, th.ctypes.data, int(k_id), int(k_exp), int(bool(slack)),
+                                              z.ctypes.data, float(tau), H.ctypes.data, g.ctypes.data,
+                                              C.addressof(obj)))
+        return obj.value, g, H
 
     def normal_equations(self, theta, mixed: bool = False):
         torch = _torch()

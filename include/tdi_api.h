@@ -304,6 +304,39 @@ int tdi_fit_landmarks_slack(tdi_batch* b, float* theta, int32_t k_exp, tdi_lmk_r
 int tdi_landmark_slack_terms(tdi_batch* b, const float* theta, int32_t k_exp, const double* z, double tau, double* H,
                              double* g, double* obj, void* stream);
 
+/* The joint landmark-only fit over a group of frames (the batch's n frames), replacing
+ * Logbarrier_Initializer::fit_model over T frames with identity unknowns as the identity stage
+ * runs it: fit_to_multi_images_landmarks_only (model_fitter.cu:1205-1247; with the slack, 60
+ * identity coefficients, the group kept when it succeeds) and the start of fit_to_multi_images
+ * (1291-1330: the same, then a fit without the slack over all identity coefficients from its
+ * result).  The frames share the first k_id identity coefficients (and the slack); each has its
+ * own first k_exp expression coefficients and pose.  Joint unknowns
+ *   z = [alpha (k_id) | s (if slack) | frame 0 (eps, tau, u) | frame 1 | ...],
+ * objective: tau_b s (with the slack) + the identity bounds + for every frame its landmark terms
+ * (as tdi_fit_landmarks_slack's, with the identity columns in the landmark positions) and its
+ * expression bounds; Gauss-Newton Hessian and gradient as compute_gradient_and_hessian
+ * (logbarrier_initializer.cu:1272-1842) sums them over the frames.  The loop is the single-frame
+ * fit's (rounds tau_b = 100 .. 1e6 of up to 100 Newton steps, halving line search from t = 1);
+ * with the slack, s starts at the largest over the frames of 5 - min G at s = 0 and the fit
+ * succeeds when s < -0.01; without it s = 0 throughout, and every round runs until its line
+ * search fails (the reference keeps that fit's result whatever happens).  Identity
+ * coefficients past k_id enter the landmark positions at frame 0's values of theta.
+ * theta: HOST [n][K] in/out: the start (frame 0's identity, every frame's expressions and pose);
+ * with the slack on success, and always without it, every frame's first k_id identity
+ * coefficients (the shared values), first k_exp expression coefficients and pose are
+ * overwritten.  k_id <= the layout's identity count, k_exp <= min(the layout's expression
+ * count, 192), k_id + slack + k_exp + 6 <= 320 and the joint count <= 4096.  report: HOST,
+ * status TDI_OK (without the slack always) or TDI_LMK_INFEASIBLE.  Synchronous. */
+int tdi_fit_landmarks_joint(tdi_batch* b, float* theta, int32_t k_id, int32_t k_exp, int32_t slack,
+                            tdi_lmk_report* report);
+
+/* The terms of that fit at joint unknowns z (HOST, FP64, the order above), for checks: theta
+ * HOST [n][K] (frame 0's identity past k_id), tau the barrier weight; outputs HOST: H [N][N]
+ * (Gauss-Newton, FP64), g [N], obj (NaN or +inf outside the feasible set);
+ * N = k_id + slack + n (k_exp + 6). */
+int tdi_landmark_joint_terms(tdi_batch* b, const float* theta, int32_t k_id, int32_t k_exp, int32_t slack,
+                             const double* z, double tau, double* H, double* g, double* obj);
+
 /* The integer buffers the fit itself computes, for bit-for-bit checks: a render at theta through
  * the same launches as a fit with these mixed_flags (TDI_MIXED_SYNTH_MFMA selects the FP64-MFMA
  * synthesis the batched fit uses), then

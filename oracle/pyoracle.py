@@ -56,6 +56,12 @@ def lib():
         L.orc_landmark_slack_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp, _dp, C.c_double,
                                                _dp, _dp]
         L.orc_landmark_slack_terms.restype = C.c_double
+        L.orc_fit_landmarks_joint.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                              _dp, C.c_void_p]
+        L.orc_fit_landmarks_joint.restype = C.c_int
+        L.orc_landmark_joint_terms.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                               _dp, _dp, C.c_double, _dp, _dp]
+        L.orc_landmark_joint_terms.restype = C.c_double
         L.orc_fit.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, C.c_void_p, C.c_int, C.c_int]
         L.orc_max_threads.restype = C.c_int
         L.orc_fit_light.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, _dp, _dp, _dp, C.c_void_p, C.c_int,
@@ -198,6 +204,35 @@ class Scene:
         H = np.zeros(nz * nz) if derivs else None
         obj = lib().orc_landmark_slack_terms(self.md.ref, self._f(i), C.byref(self.opts), int(k_exp), p(_d(theta)),
                                              p(z), float(tau), p(g) if derivs else None, p(H) if derivs else None)
+        return (obj, g, H.reshape(nz, nz)) if derivs else (obj, None, None)
+
+    def joint_landmark_unknowns(self, k_id, k_exp, slack):
+        return int(k_id) + (1 if slack else 0) + len(self.frames) * (int(k_exp) + 6)
+
+    def fit_landmarks_joint(self, theta0, k_id, k_exp, slack):
+        """The joint landmark-only fit over all frames of the scene (one group): (theta [n, K]
+        float64, report)."""
+        n = len(self.frames)
+        th = np.ascontiguousarray(np.array(theta0, np.float64).reshape(n, self.K))
+        rep = abi.tdi_lmk_report()
+        rc = lib().orc_fit_landmarks_joint(self.md.ref, n, self.fd.ptr(0), C.byref(self.opts), int(k_id), int(k_exp),
+                                           int(bool(slack)), p(th), C.byref(rep))
+        if rc:
+            raise ValueError(f"orc_fit_landmarks_joint: error {rc}")
+        return th, rep
+
+    def landmark_joint_terms(self, theta, k_id, k_exp, slack, z, tau, derivs=True):
+        """(objective, g [N], H [N, N]) of the joint landmark fit at z (joint order: identity,
+        slack, then each frame's expressions and pose); theta: frame 0's row supplies the fixed
+        identity coefficients past k_id."""
+        nz = self.joint_landmark_unknowns(k_id, k_exp, slack)
+        z = _d(z)
+        assert z.size == nz
+        g = np.zeros(nz) if derivs else None
+        H = np.zeros(nz * nz) if derivs else None
+        obj = lib().orc_landmark_joint_terms(self.md.ref, len(self.frames), self.fd.ptr(0), C.byref(self.opts),
+                                             int(k_id), int(k_exp), int(bool(slack)), p(_d(theta)), p(z), float(tau),
+                                             p(g) if derivs else None, p(H) if derivs else None)
         return (obj, g, H.reshape(nz, nz)) if derivs else (obj, None, None)
 
     def fit(self, theta0, n_threads=0, frames=None, finetune=False):

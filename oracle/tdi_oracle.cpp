@@ -1297,6 +1297,236 @@ tdi_lmk_report lmk_fit_run(const LmkFit& S, double* z)
     return rep;
 }
 
+// ---------------------------------------------------------------- joint landmark-only fit over T frames
+// Logbarrier_Initializer::fit_model over T frames with identity unknowns
+// (fit_to_multi_images_landmarks_only, model_fitter.cu:1205-1247, and the two landmark fits at
+// the start of fit_to_multi_images, 1291-1330: li_init with K_ALPHA_L = 60 identity
+// coefficients and the slack, then li on all identity coefficients without it).  The frames share
+// the first Ka identity coefficients (and the slack s); each has its own Ke expression
+// coefficients and pose.  Joint order z = [alpha (Ka) | s (if slack) | frame 0 (eps Ke, tau 3,
+// u 3) | frame 1 | ...].  compute_gradient_and_hessian (1272-1842) sums the frames' landmark
+// terms (the identity columns of fill_optimization_dI_dalpha_landmark next to the expression and
+// pose columns), adds the identity bounds once and the slack terms once (tau_b s, and per
+// constraint the s derivatives as for one frame); without the slack s = 0 and is no unknown.
+// Identity columns past Ka keep theta's values (frame 0's) in the landmark positions.
+struct LmkJoint {
+    const tdi_model_desc* m;
+    const tdi_frame_desc* f;
+    int T = 0, Ka = 0, Ke = 0, S1 = 0, n = 0;
+    std::vector<double> base;   // [T][NL][3]
+};
+
+void lmk_joint_setup(LmkJoint& S, const tdi_model_desc* m, const tdi_frame_desc* frames, int T, const Layout& L,
+                     const double* theta, int Ka, int Ke, int slack)
+{
+    S.m = m; S.f = frames; S.T = T; S.Ka = Ka; S.Ke = Ke; S.S1 = slack ? 1 : 0;
+    S.n = Ka + S.S1 + T * (Ke + 6);
+    S.base.assign((size_t)T * NL * 3, 0.0);
+    for (int t = 0; t < T; ++t)
+        for (int i = 0; i < NL; ++i) {
+            const int v = m->landmark_ids[i];
+            for (int a = 0; a < 3; ++a) {
+                const int r = 3 * v + a;
+                double x = m->mean_shape[r];
+                for (int k = Ka; k < L.nid; ++k) x += (double)m->id_basis[(size_t)r * m->k_id + k] * theta[L.off_id + k];
+                S.base[((size_t)t * NL + i) * 3 + a] = x;
+            }
+        }
+}
+
+// Objective at z; with g (n) and H (n x n) the gradient and Gauss-Newton Hessian.  gmin: the
+// least G over the frames at z (may be null).
+double lmk_joint_terms(const LmkJoint& S, const double* z, double tau_b, double* g, double* H, double* gmin)
+{
+    const tdi_model_desc* m = S.m;
+    const int Ka = S.Ka, Ke = S.Ke, n = S.n, is = Ka;   // is: the slack's index (when S1)
+    const double s = S.S1 ? z[is] : 0.0;
+    if (g) { std::fill(g, g + n, 0.0); std::fill(H, H + (size_t)n * n, 0.0); }
+    double obj = S.S1 ? tau_b * s : 0.0, lo = INFINITY;
+    // the identity bounds, once
+    for (int k = 0; k < Ka; ++k) {
+        if (!(m->id_ub[k] < INFINITY) || !(m->id_lb[k] > -INFINITY)) continue;
+        const double x = z[k], fub = x - m->id_ub[k], flb = m->id_lb[k] - x;
+        obj += -std::log(-fub) - std::log(-flb);
+        if (g) {
+            g[k] += -1.0 / fub + 1.0 / flb;
+            const double a = -1.0 / (fub + 1e-12), b = 1.0 / (flb + 1e-12);
+            H[(size_t)k * n + k] += a * a + b * b;
+        }
+    }
+    const int nl = Ka + Ke + 6;                 // a frame's columns: alpha, eps, tau, u
+    std::vector<int> col(nl);
+    std::vector<double> dx(nl), dy(nl);
+    for (int t = 0; t < S.T; ++t) {
+        const tdi_frame_desc* f = &S.f[t];
+        const int zf = Ka + S.S1 + t * (Ke + 6);
+        for (int j = 0; j < nl; ++j) col[j] = j < Ka ? j : zf + (j - Ka);
+        const double* eps = z + zf;
+        const double* tau = z + zf + Ke;
+        const double* u = z + zf + Ke + 3;
+        double R[9], dR[27];
+        rodrigues(u, R, g ? dR : nullptr);
+        for (int k = 0; k < Ke; ++k) {
+            if (!(m->exp_ub[k] < INFINITY) || !(m->exp_lb[k] > -INFINITY)) continue;
+            const double x = eps[k], fub = x - m->exp_ub[k], flb = m->exp_lb[k] - x;
+            obj += -std::log(-fub) - std::log(-flb);
+            if (g) {
+                g[zf + k] += -1.0 / fub + 1.0 / flb;
+                const double a = -1.0 / (fub + 1e-12), b = 1.0 / (flb + 1e-12);
+                H[(size_t)(zf + k) * n + zf + k] += a * a + b * b;
+            }
+        }
+        const double fs = f->face_size;
+        for (int i = 0; i < NL; ++i) {
+            const int v = m->landmark_ids[i];
+            double p[3];
+            for (int a = 0; a < 3; ++a) {
+                const int r = 3 * v + a;
+                double x = S.base[((size_t)t * NL + i) * 3 + a];
+                for (int k = 0; k < Ka; ++k) x += (double)m->id_basis[(size_t)r * m->k_id + k] * z[k];
+                for (int k = 0; k < Ke; ++k) x += (double)m->exp_basis[(size_t)r * m->k_exp + k] * eps[k];
+                p[a] = x;
+            }
+            double c[3];
+            for (int a = 0; a < 3; ++a) c[a] = R[a] * p[0] + R[3 + a] * p[1] + R[6 + a] * p[2] + tau[a];
+            const double xp = f->phix * c[0] / c[2] + f->cx, yp = f->phiy * c[1] / c[2] + f->cy;
+            const double Gxm = s + xp - (f->lmk_x[i] + f->box_lx[i] * fs);
+            const double Gym = s + yp - (f->lmk_y[i] + f->box_ly[i] * fs);
+            const double Gxp = s + (f->lmk_x[i] + f->box_ux[i] * fs) - xp;
+            const double Gyp = s + (f->lmk_y[i] + f->box_uy[i] * fs) - yp;
+            obj += -std::log(Gxm) - std::log(Gym) - std::log(Gxp) - std::log(Gyp);
+            for (double G : {Gxm, Gym, Gxp, Gyp}) lo = (G != G || lo != lo) ? NAN : std::min(lo, G);
+            if (!g) continue;
+            for (int j = 0; j < nl; ++j) {
+                double dv[3];
+                if (j < Ka || j < Ka + Ke) {
+                    const bool idc = j < Ka;
+                    double e[3];
+                    for (int a = 0; a < 3; ++a)
+                        e[a] = idc ? m->id_basis[(size_t)(3 * v + a) * m->k_id + j]
+                                   : m->exp_basis[(size_t)(3 * v + a) * m->k_exp + (j - Ka)];
+                    for (int a = 0; a < 3; ++a) dv[a] = R[a] * e[0] + R[3 + a] * e[1] + R[6 + a] * e[2];
+                } else if (j < Ka + Ke + 3) {
+                    dv[0] = dv[1] = dv[2] = 0.0;
+                    dv[j - Ka - Ke] = 1.0;
+                } else {
+                    const double* d = dR + 9 * (j - Ka - Ke - 3);
+                    for (int a = 0; a < 3; ++a) dv[a] = d[a] * p[0] + d[3 + a] * p[1] + d[6 + a] * p[2];
+                }
+                dx[j] = f->phix * (dv[0] * c[2] - dv[2] * c[0]) / (c[2] * c[2]);
+                dy[j] = f->phiy * (dv[1] * c[2] - dv[2] * c[1]) / (c[2] * c[2]);
+            }
+            const double cx = -1.0 / Gxm + 1.0 / Gxp, cy = -1.0 / Gym + 1.0 / Gyp;
+            const double wx = 1.0 / (Gxm * Gxm) + 1.0 / (Gxp * Gxp), wy = 1.0 / (Gym * Gym) + 1.0 / (Gyp * Gyp);
+            const double hx = 1.0 / (Gxm * Gxm) - 1.0 / (Gxp * Gxp), hy = 1.0 / (Gym * Gym) - 1.0 / (Gyp * Gyp);
+            for (int a = 0; a < nl; ++a) {
+                const int A = col[a];
+                g[A] += cx * dx[a] + cy * dy[a];
+                for (int b = 0; b < nl; ++b) H[(size_t)A * n + col[b]] += wx * dx[a] * dx[b] + wy * dy[a] * dy[b];
+                if (S.S1) {
+                    const double hs = hx * dx[a] + hy * dy[a];
+                    H[(size_t)A * n + is] += hs;
+                    H[(size_t)is * n + A] += hs;
+                }
+            }
+            if (S.S1) {
+                const double q[4] = {1.0 / (1e-6 + Gxm), 1.0 / (1e-6 + Gym), 1.0 / (1e-6 + Gxp), 1.0 / (1e-6 + Gyp)};
+                for (int k = 0; k < 4; ++k) {
+                    g[is] -= q[k];
+                    H[(size_t)is * n + is] += q[k] * q[k];
+                }
+            }
+        }
+    }
+    if (g && S.S1) g[is] += tau_b;
+    if (gmin) *gmin = lo;
+    return obj;
+}
+
+// The whole fit (fit_model): with the slack, s starts at max over the frames of 5 - min G at
+// s = 0 (set_minimal_slack, 1041-1090) and the fit succeeds once s < -0.01; without it, all five
+// rounds run until a line search fails.  z in: the start; out: the final unknowns.
+tdi_lmk_report lmk_joint_run(const LmkJoint& S, double* z)
+{
+    const int n = S.n;
+    tdi_lmk_report rep{};
+    if (S.S1) {
+        z[S.Ka] = 0.0;
+        double hi = -INFINITY;
+        for (int t = 0; t < S.T; ++t) {
+            LmkJoint one = S;
+            one.T = 1;
+            one.f = S.f + t;
+            one.n = S.Ka + 1 + (S.Ke + 6);
+            one.base.assign(S.base.begin() + (size_t)t * NL * 3, S.base.begin() + (size_t)(t + 1) * NL * 3);
+            std::vector<double> zt(one.n);
+            for (int k = 0; k < S.Ka; ++k) zt[k] = z[k];
+            zt[S.Ka] = 0.0;
+            for (int k = 0; k < S.Ke + 6; ++k) zt[S.Ka + 1 + k] = z[S.Ka + 1 + (size_t)t * (S.Ke + 6) + k];
+            double gmin = 0.0;
+            lmk_joint_terms(one, zt.data(), 0.0, nullptr, nullptr, &gmin);
+            const double st = 5.0 - gmin;
+            hi = (st != st || hi != hi) ? NAN : std::max(hi, st);
+        }
+        z[S.Ka] = hi;
+        rep.slack0 = (float)hi;
+    }
+    std::vector<double> g(n), H((size_t)n * n), d(n), zt(n);
+    double tau_b = 10.0;
+    bool success = false;
+    for (int round = 0; round < 5 && !success; ++round) {
+        tau_b *= 10.0;
+        rep.rounds++;
+        for (int it = 0; it < 100; ++it) {
+            if (S.S1 && z[S.Ka] < -0.01) { success = true; break; }
+            const double obj = lmk_joint_terms(S, z, tau_b, g.data(), H.data(), nullptr);
+            rep.iterations++;
+            if (!cholesky_solve(n, H, g.data(), d.data())) break;
+            bool stop = false;
+            double t = 1.0;
+            for (int inner = 0; inner < 100; ++inner) {
+                if (t < 0.01) { stop = true; break; }
+                for (int j = 0; j < n; ++j) zt[j] = z[j] + t * d[j];
+                const double ot = lmk_joint_terms(S, zt.data(), tau_b, nullptr, nullptr, nullptr);
+                if (ot < obj) {
+                    std::copy(zt.begin(), zt.end(), z);
+                    rep.accepted++;
+                    break;
+                }
+                t *= 0.5;
+            }
+            if (stop) break;
+        }
+    }
+    rep.status = (!S.S1 || success) ? TDI_OK : TDI_LMK_INFEASIBLE;
+    rep.slack = S.S1 ? (float)z[S.Ka] : 0.f;
+    return rep;
+}
+
+// theta [T][K] -> z (alpha from frame 0) and back (alpha to every frame).
+void lmk_joint_pack(const LmkJoint& S, const Layout& L, const double* theta, double* z)
+{
+    for (int k = 0; k < S.Ka; ++k) z[k] = theta[L.off_id + k];
+    if (S.S1) z[S.Ka] = 0.0;
+    for (int t = 0; t < S.T; ++t) {
+        const double* th = theta + (size_t)t * L.K;
+        double* zf = z + S.Ka + S.S1 + (size_t)t * (S.Ke + 6);
+        for (int k = 0; k < S.Ke; ++k) zf[k] = th[L.off_exp + k];
+        for (int a = 0; a < 6; ++a) zf[S.Ke + a] = th[L.off_pose + a];
+    }
+}
+
+void lmk_joint_unpack(const LmkJoint& S, const Layout& L, const double* z, double* theta)
+{
+    for (int t = 0; t < S.T; ++t) {
+        double* th = theta + (size_t)t * L.K;
+        const double* zf = z + S.Ka + S.S1 + (size_t)t * (S.Ke + 6);
+        for (int k = 0; k < S.Ka; ++k) th[L.off_id + k] = z[k];
+        for (int k = 0; k < S.Ke; ++k) th[L.off_exp + k] = zf[k];
+        for (int a = 0; a < 6; ++a) th[L.off_pose + a] = zf[S.Ke + a];
+    }
+}
+
 // ---------------------------------------------------------------- frame preparation
 // InputData::get_resized_frame (input_data.cu:44-53) and the crop + transpose of
 // VideoFitter::fit_to_video_frame (video_fitter.cu:537-543), on 8-bit BGR frames.  The steps are
@@ -1557,6 +1787,36 @@ double orc_landmark_slack_terms(const tdi_model_desc* m, const tdi_frame_desc* f
     LmkFit S;
     lmk_fit_setup(S, m, f, L, theta, Kl);
     return lmk_fit_terms(S, z, tau_b, g, H, nullptr);
+}
+
+// The joint landmark-only fit over the n frames of one group: theta [n][K] double in/out (frame
+// 0's identity is the start of the shared block; every frame's pose and expressions its own
+// start; with the slack, on success, and always without it, the first Ka identity coefficients
+// of every frame, the first Ke expression coefficients and the pose are overwritten).
+int orc_fit_landmarks_joint(const tdi_model_desc* m, int n, const tdi_frame_desc* frames, const tdi_fit_options* o,
+                            int Ka, int Ke, int slack, double* theta, tdi_lmk_report* report)
+{
+    const Layout L = make_layout(m, o);
+    if (n < 1 || Ka < 0 || Ka > L.nid || Ke < 0 || Ke > L.nexp) return TDI_ERR_ARG;
+    LmkJoint S;
+    lmk_joint_setup(S, m, frames, n, L, theta, Ka, Ke, slack);
+    std::vector<double> z(S.n);
+    lmk_joint_pack(S, L, theta, z.data());
+    *report = lmk_joint_run(S, z.data());
+    if (report->status == TDI_OK) lmk_joint_unpack(S, L, z.data(), theta);
+    return TDI_OK;
+}
+
+// The terms of that fit at z [joint order] (theta: frame 0's identity past Ka is fixed);
+// g [N], H [N^2] may be null together; returns the objective.  N = Ka + slack + n (Ke + 6).
+double orc_landmark_joint_terms(const tdi_model_desc* m, int n, const tdi_frame_desc* frames,
+                                const tdi_fit_options* o, int Ka, int Ke, int slack, const double* theta,
+                                const double* z, double tau_b, double* g, double* H)
+{
+    const Layout L = make_layout(m, o);
+    LmkJoint S;
+    lmk_joint_setup(S, m, frames, n, L, theta, Ka, Ke, slack);
+    return lmk_joint_terms(S, z, tau_b, g, H, nullptr);
 }
 
 int orc_blur_taps(double sigma, double* taps) { return blur_taps(sigma, taps); }

@@ -1,0 +1,101 @@
+"""The joint landmark-only fit over a group of frames on the GPU (tdi_fit_landmarks_joint,
+tdi_landmark_joint_terms) against the FP64 oracle (oracle lmk_joint_*, pinned in
+test_lmk_joint_oracle.py): the joint objective, gradient and Gauss-Newton matrix at the same
+unknowns (all identity columns fitted or the first few, with and without the slack) to 1e-9
+relative (FP64 on both sides, only summation orders differ), and whole fits (with the slack from
+zero identity and expressions and perturbed poses, then without it from that result) with the
+same report and parameters within 1e-5."""
+import importlib
+import sys
+
+import numpy as np
+import pytest
+
+from scenes import joint_scene
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+tdi = importlib.import_module("3di_amd.tdi")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _start(sc, Ka, Ke, slack, rng):
+    lay = sc.lay
+    tr = sc.truth()
+    n = len(sc.frames)
+    o = Ka + (1 if slack else 0)
+    z = np.zeros(o + n * (Ke + 6))
+    z[:Ka] = tr[0, lay["id"]:lay["id"] + Ka] + rng.normal(0, 0.05, Ka)
+    if slack:
+        z[Ka] = 40.0
+    for t in range(n):
+        b = z[o + t * (Ke + 6):o + (t + 1) * (Ke + 6)]
+        b[:Ke] = tr[t, lay["exp"]:lay["exp"] + Ke] * 0.5
+        b[Ke:] = tr[t, lay["pose"]:lay["pose"] + 6] + np.concatenate([rng.normal(0, 0.3, 3), rng.normal(0, 0.003, 3)])
+    return z
+
+
+@pytest.mark.parametrize("Ka,slack", [(12, True), (6, True), (12, False)])
+def test_joint_terms_match_the_oracle(Ka, slack):
+    _say(f"case: joint landmark terms, k_id {Ka}, slack {slack}")
+    sc = joint_scene()
+    Ke = 10
+    th = sc.truth().astype(np.float32)
+    ors = sc.oracle()
+    ctx = tdi.Context(sc.model, sc.opts)
+    b = tdi.Batch(ctx, sc.frames)
+    rng = np.random.default_rng(Ka + 3 * slack)
+    z = _start(sc, Ka, Ke, slack, rng)
+    for tau in (100.0, 1e5):
+        og, gg, Hg = b.landmark_joint_terms(th, Ka, Ke, slack, z, tau)
+        oo, go, Ho = ors.landmark_joint_terms(th.astype(np.float64), Ka, Ke, slack, z, tau)
+        assert np.isfinite(oo)
+        eo = abs(og - oo) / abs(oo)
+        eg = np.abs(gg - go).max() / np.abs(go).max()
+        eH = np.abs(Hg - Ho).max() / np.abs(Ho).max()
+        _say(f"  tau {tau:g}: |dobj| {eo:.1e}, |dg| {eg:.1e}, |dH| {eH:.1e} (relative)")
+        assert eo < 1e-9 and eg < 1e-9 and eH < 1e-9
+    b.close()
+    ctx.close()
+
+
+def test_joint_fits_match_the_oracle():
+    _say("case: joint landmark fits, with the slack then without it")
+    sc = joint_scene()
+    lay = sc.lay
+    Ka, Ke = 12, 10
+    n = len(sc.frames)
+    rng = np.random.default_rng(21)
+    th0 = sc.truth().copy()
+    th0[:, lay["id"]:lay["exp"]] = 0.0
+    th0[:, lay["exp"]:lay["pose"]] = 0.0
+    th0[:, lay["pose"]:lay["pose"] + 3] += rng.normal(0, 6, (n, 3))
+    th0[:, lay["pose"] + 3:lay["pose"] + 6] += rng.normal(0, 0.05, (n, 3))
+    th0 = th0.astype(np.float32)
+    ors = sc.oracle()
+    ctx = tdi.Context(sc.model, sc.opts)
+    b = tdi.Batch(ctx, sc.frames)
+    th1g, r1g = b.fit_landmarks_joint(th0, Ka, Ke, True)
+    th1o, r1o = ors.fit_landmarks_joint(th0.astype(np.float64), Ka, Ke, True)
+    _say(f"  slack fit: GPU status {r1g.status} iterations {r1g.iterations} accepted {r1g.accepted} slack {r1g.slack:.3f}; "
+         f"oracle {r1o.status} {r1o.iterations} {r1o.accepted} {r1o.slack:.3f}")
+    assert r1g.status == 0 and r1o.status == 0
+    assert (r1g.iterations, r1g.accepted, r1g.rounds) == (r1o.iterations, r1o.accepted, r1o.rounds)
+    assert abs(r1g.slack0 - r1o.slack0) <= 1e-5 * abs(r1o.slack0)
+    d1 = np.abs(th1g - th1o).max() / np.abs(th1o).max()
+    _say(f"  slack fit parameters: max |dtheta| / max |theta| {d1:.1e}")
+    assert d1 < 1e-5
+    th2g, r2g = b.fit_landmarks_joint(th1g, Ka, Ke, False)
+    th2o, r2o = ors.fit_landmarks_joint(th1g.astype(np.float64), Ka, Ke, False)
+    _say(f"  fit without slack: GPU iterations {r2g.iterations} accepted {r2g.accepted}; oracle {r2o.iterations} {r2o.accepted}")
+    assert r2g.status == 0 and (r2g.iterations, r2g.accepted) == (r2o.iterations, r2o.accepted)
+    d2 = np.abs(th2g - th2o).max() / np.abs(th2o).max()
+    _say(f"  fit without slack parameters: {d2:.1e}")
+    assert d2 < 1e-5
+    b.close()
+    ctx.close()
