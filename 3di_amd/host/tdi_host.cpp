@@ -1073,37 +1073,13 @@ static int landmark_batch(tdi_ctx* ctx, int n, const float* lx, const float* ly,
     return tdi_batch_create(ctx, n, fd.data(), b);
 }
 
-} // namespace tdi_host
-
-extern "C" {
-
-void tdi_default_identity_params(tdi_identity_params* p)
+// The binning of fit_video_frames_landmarks_sparse (video_fitter.cu:2380-2550): bins, shuffles,
+// permutations and groups; accept(group, keep) decides whether a full group is kept (the
+// reference's joint landmark-only fit of the group, 2505-2545).  out: the kept groups' frames.
+template <typename Accept>
+static int bin_groups(const tdi_identity_params* p, int n, const float* yaw, const float* pitch, Accept accept,
+                      std::vector<int>& out)
 {
-    if (!p) return;
-    std::memset(p, 0, sizeof(*p));
-    p->nframes = 7;                 // config.cu:45
-    p->nframes_per_angle_bin = 14;  // config.cu:79
-    p->nperms = 10;                 // config.cu:27
-    p->ntot_reconstrs = 6;          // config.cu:29
-    p->nmax_frames = 432000;        // config.cu:28
-    p->max_vid_frames = 50000;      // config.cu:59
-    p->skip_first_n_secs = 0.f;     // config.cu:55
-    p->fps = 30.f;
-    p->ref_face_size = 86.f;        // config.cu:18
-    p->pad = 1;                     // config.cu:43
-    p->crop_w = 256;                // constants.h:55-56
-    p->crop_h = 256;
-    p->paint_innermouth_black = 1;  // config.cu:92
-    p->lmk_k_exp = 79;              // config.cu:112
-    p->box_half_width = 0.15f;
-}
-
-int tdi_bin_identity_frames(const tdi_identity_params* p, int32_t n, const float* yaw, const float* pitch,
-                            int32_t* selected, int32_t cap, int32_t* n_selected)
-{
-    if (!p || n < 0 || (n > 0 && (!yaw || !pitch)) || !n_selected || cap < 0 || (cap > 0 && !selected) ||
-        p->nframes < 1 || p->nperms < 0 || p->nframes_per_angle_bin < 0 || p->ntot_reconstrs < 0)
-        return TDI_ERR_ARG;
     // 13 x 13 (yaw, pitch) centroids at -55 .. 65 degrees (video_fitter.cu:2380-2400)
     std::vector<int> yc, pc;
     for (int c = -60; c <= 60; c += 10) yc.push_back(c + 5);
@@ -1133,16 +1109,60 @@ int tdi_bin_identity_frames(const tdi_identity_params* p, int32_t n, const float
         std::shuffle(iv.begin(), iv.end(), e_seed);
         order.insert(order.end(), iv.begin(), iv.end());
     }
-    std::vector<int> out, group;
+    std::vector<int> group;
     for (int ri : order) {
         group.push_back(chosen[ri]);
         if ((int)group.size() == p->nframes) {
-            out.insert(out.end(), group.begin(), group.end());
+            bool keep = false;
+            const int rc = accept(group, keep);
+            if (rc) return rc;
+            if (keep) out.insert(out.end(), group.begin(), group.end());
             group.clear();
         }
         if ((long long)out.size() >= p->nmax_frames) break;
         if ((long long)out.size() >= (long long)p->ntot_reconstrs * p->nframes) break;
     }
+    return TDI_OK;
+}
+
+} // namespace tdi_host
+
+extern "C" {
+
+void tdi_default_identity_params(tdi_identity_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->nframes = 7;                 // config.cu:45
+    p->nframes_per_angle_bin = 14;  // config.cu:79
+    p->nperms = 10;                 // config.cu:27
+    p->ntot_reconstrs = 6;          // config.cu:29
+    p->nmax_frames = 432000;        // config.cu:28
+    p->max_vid_frames = 50000;      // config.cu:59
+    p->skip_first_n_secs = 0.f;     // config.cu:55
+    p->fps = 30.f;
+    p->ref_face_size = 86.f;        // config.cu:18
+    p->pad = 1;                     // config.cu:43
+    p->crop_w = 256;                // constants.h:55-56
+    p->crop_h = 256;
+    p->paint_innermouth_black = 1;  // config.cu:92
+    p->lmk_k_exp = 79;              // config.cu:112
+    p->box_half_width = 0.15f;
+    p->lmk_k_id = 60;               // K_ALPHA_L, config.cu:160
+}
+
+int tdi_bin_identity_frames(const tdi_identity_params* p, int32_t n, const float* yaw, const float* pitch,
+                            int32_t* selected, int32_t cap, int32_t* n_selected)
+{
+    if (!p || n < 0 || (n > 0 && (!yaw || !pitch)) || !n_selected || cap < 0 || (cap > 0 && !selected) ||
+        p->nframes < 1 || p->nperms < 0 || p->nframes_per_angle_bin < 0 || p->ntot_reconstrs < 0)
+        return TDI_ERR_ARG;
+    std::vector<int> out;
+    const int rc = tdi_host::bin_groups(p, n, yaw, pitch, [](const std::vector<int>&, bool& keep) {
+        keep = true;
+        return TDI_OK;
+    }, out);
+    if (rc) return rc;
     *n_selected = (int32_t)out.size();
     if ((int)out.size() > cap) return TDI_ERR_ARG;
     for (size_t i = 0; i < out.size(); ++i) selected[i] = out[i];
@@ -1205,34 +1225,61 @@ int tdi_select_identity_frames(tdi_ctx* ctx, const tdi_identity_params* p, int32
     float* one = nullptr;
     if (hipMalloc((void**)&one, sizeof(float)) != hipSuccess) return TDI_ERR_NOMEM;
     (void)hipMemset(one, 0, sizeof(float));
-    tdi_batch* b = nullptr;
-    rc = tdi_host::landmark_batch(ctx, n, lx.data(), ly.data(), cams.data(), fs.data(), lo.data(), hi.data(), one, &b);
-    if (!rc) {
-        const int K = tdi_batch_unknowns(b);
-        int32_t lay[6];
-        tdi_ctx_layout(ctx, lay);
-        const int op = lay[3];
-        // identity, albedo and expression at zero, the orthographic pose
-        std::vector<float> th((size_t)n * K, 0.f);
-        for (int f = 0; f < n; ++f)
-            for (int a = 0; a < 6; ++a) th[(size_t)f * K + op + a] = pose[(size_t)f * 6 + a];
-        const int kl = std::min(p->lmk_k_exp, op - lay[2]);
-        std::vector<tdi_lmk_report> rep(n);
-        if (kl > 0) rc = tdi_host::landmark_fit(b, n, K, kl, th.data(), rep.data());
-        for (int f = 0; !rc && f < n; ++f) {
-            if (!ok[f] || (kl > 0 && rep[f].status != TDI_OK)) continue;
-            float R[9], ypr[3];
-            tdi_host::rotation_f32(&th[(size_t)f * K + op + 3], R);
-            tdi_host::compute_euler_angles(R, ypr[0], ypr[1], ypr[2]);
-            yaw[f] = ypr[0]; pitch[f] = ypr[1]; roll[f] = ypr[2];
+    int32_t lay[6];
+    tdi_ctx_layout(ctx, lay);
+    const int op = lay[3], K = lay[5];
+    const int kl = std::max(0, std::min(p->lmk_k_exp, op - lay[2]));
+    const int ka = std::max(0, std::min(p->lmk_k_id, lay[2] - lay[1]));
+    // a group of frames (or one) with identity, albedo and expressions at zero and the orthographic
+    // poses, fitted jointly with the slack (li_init of video_fitter.cu:2253-2275, reset per frame,
+    // and fit_to_multi_images_landmarks_only, model_fitter.cu:1205-1247)
+    auto fit_group = [&](const std::vector<int>& idx, std::vector<float>& th, tdi_lmk_report& rep) -> int {
+        const int m = (int)idx.size();
+        std::vector<float> gx((size_t)m * NL), gy((size_t)m * NL), gc((size_t)m * 4), gf(m);
+        for (int t = 0; t < m; ++t) {
+            const int f = idx[t];
+            std::memcpy(&gx[(size_t)t * NL], &lx[(size_t)f * NL], sizeof(float) * NL);
+            std::memcpy(&gy[(size_t)t * NL], &ly[(size_t)f * NL], sizeof(float) * NL);
+            std::memcpy(&gc[(size_t)t * 4], &cams[(size_t)f * 4], sizeof(float) * 4);
+            gf[t] = fs[f];
         }
-        tdi_batch_destroy(b);
+        th.assign((size_t)m * K, 0.f);
+        for (int t = 0; t < m; ++t)
+            for (int a = 0; a < 6; ++a) th[(size_t)t * K + op + a] = pose[(size_t)idx[t] * 6 + a];
+        tdi_batch* b = nullptr;
+        int rc = tdi_host::landmark_batch(ctx, m, gx.data(), gy.data(), gc.data(), gf.data(), lo.data(), hi.data(), one,
+                                          &b);
+        if (!rc) rc = tdi_fit_landmarks_joint(b, th.data(), ka, kl, 1, &rep);
+        if (b) tdi_batch_destroy(b);
+        return rc;
+    };
+    std::vector<float> th;
+    for (int f = 0; !rc && f < n; ++f) {
+        if (!ok[f]) continue;
+        tdi_lmk_report rep{};
+        if ((rc = fit_group(std::vector<int>{f}, th, rep))) break;
+        if (rep.status != TDI_OK) continue;
+        float R[9], ypr[3];
+        tdi_host::rotation_f32(&th[op + 3], R);
+        tdi_host::compute_euler_angles(R, ypr[0], ypr[1], ypr[2]);
+        yaw[f] = ypr[0]; pitch[f] = ypr[1]; roll[f] = ypr[2];
     }
-    (void)hipFree(one);
-    if (rc) return rc;
     if (angles)
         for (int f = 0; f < n; ++f) { angles[3 * (size_t)f] = yaw[f]; angles[3 * (size_t)f + 1] = pitch[f]; angles[3 * (size_t)f + 2] = roll[f]; }
-    return tdi_bin_identity_frames(p, n, yaw.data(), pitch.data(), selected, cap, n_selected);
+    std::vector<int> out;
+    if (!rc)
+        rc = tdi_host::bin_groups(p, n, yaw.data(), pitch.data(), [&](const std::vector<int>& group, bool& keep) {
+            tdi_lmk_report rep{};
+            const int e = fit_group(group, th, rep);
+            keep = !e && rep.status == TDI_OK;
+            return e;
+        }, out);
+    (void)hipFree(one);
+    if (rc) return rc;
+    *n_selected = (int32_t)out.size();
+    if ((int)out.size() > cap) return TDI_ERR_ARG;
+    for (size_t i = 0; i < out.size(); ++i) selected[i] = out[i];
+    return TDI_OK;
 }
 
 } // extern "C"
@@ -1304,24 +1351,26 @@ int tdi_learn_identity(tdi_ctx* ctx, const tdi_identity_params* p, int32_t n, ui
         }
         tdi_batch* b = nullptr;
         if (!rc) rc = tdi_batch_create(ctx, nf, fd.data(), &b);
-        bool success = false;
+        bool success = false, start_ok = false;
         std::vector<float> th((size_t)nf * K, 0.f);
         if (!rc) {
-            // the start: each frame's orthographic pose, then its landmark fit with slack (identity
-            // and albedo at the mean; the reference fits the identity jointly there)
+            // the start of fit_to_multi_images (model_fitter.cu:1291-1330): the orthographic poses
+            // with identity, albedo and expressions at zero; the joint landmark-only fit with the
+            // slack over K_ALPHA_L identity coefficients (the group is not used if it fails); from
+            // its result the joint landmark-only fit without the slack over all identity coefficients
             std::vector<float> pose((size_t)nf * 6);
             rc = tdi_host::initialize_with_orthographic_t(ctx, nf, lx.data(), ly.data(), cam2.data(), pose.data(), nullptr,
                                                           nullptr);
             for (int f = 0; !rc && f < nf; ++f)
                 for (int a = 0; a < 6; ++a) th[(size_t)f * K + off_pose + a] = pose[(size_t)f * 6 + a];
-            const int kl = std::min(p->lmk_k_exp, off_pose - off_exp);
-            if (!rc && kl > 0) {
-                std::vector<float> tt(th);
-                std::vector<tdi_lmk_report> lrep(nf);
-                rc = tdi_host::landmark_fit(b, nf, K, kl, tt.data(), lrep.data());
-                for (int f = 0; !rc && f < nf; ++f)
-                    if (lrep[f].status == TDI_OK) std::memcpy(&th[(size_t)f * K], &tt[(size_t)f * K], sizeof(float) * K);
-            }
+            const int kl = std::max(0, std::min(p->lmk_k_exp, off_pose - off_exp));
+            const int ka = std::max(0, std::min(p->lmk_k_id, nid));
+            tdi_lmk_report lrep{};
+            if (!rc) rc = tdi_fit_landmarks_joint(b, th.data(), ka, kl, 1, &lrep);
+            start_ok = !rc && lrep.status == TDI_OK;
+            if (start_ok) rc = tdi_fit_landmarks_joint(b, th.data(), nid, kl, 0, &lrep);
+        }
+        if (!rc && start_ok) {
             // the light sub-fits of fit_to_multi_images (model_fitter.cu:1374-1386)
             std::vector<float> lam((size_t)nf * 3), Lint(nf, 0.005f);
             for (int f = 0; f < nf; ++f) { lam[3 * (size_t)f] = -7.3627f; lam[3 * (size_t)f + 1] = 51.1364f; lam[3 * (size_t)f + 2] = 100.1784f; }

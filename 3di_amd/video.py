@@ -351,7 +351,8 @@ class IdentityParams(C.Structure):
                 ("ntot_reconstrs", C.c_int32), ("nmax_frames", C.c_int32), ("max_vid_frames", C.c_int32),
                 ("skip_first_n_secs", C.c_float), ("fps", C.c_float), ("ref_face_size", C.c_float),
                 ("pad", C.c_int32), ("crop_w", C.c_int32), ("crop_h", C.c_int32),
-                ("paint_innermouth_black", C.c_int32), ("lmk_k_exp", C.c_int32), ("box_half_width", C.c_float)]
+                ("paint_innermouth_black", C.c_int32), ("lmk_k_exp", C.c_int32), ("box_half_width", C.c_float),
+                ("lmk_k_id", C.c_int32)]
 
 
 def identity_params(**kw) -> IdentityParams:

@@ -162,9 +162,10 @@ typedef struct tdi_identity_params {
     int32_t crop_w, crop_h;         /* DIMX, DIMY (constants.h:55-56: 256)                             */
     int32_t paint_innermouth_black; /* PAINT_INNERMOUTH_BLACK (config.cu:92: 1)                        */
     int32_t lmk_k_exp;              /* K_EPSILON_L (config.cu:112: 79): expression coefficients of the
-                                     * landmark fits (the reference also fits K_ALPHA_L = 60 identity
-                                     * coefficients there; these fits hold the identity at the mean)   */
+                                     * landmark fits                                                   */
     float box_half_width;           /* landmark boxes (face-size units): 0.15                          */
+    int32_t lmk_k_id;               /* K_ALPHA_L (config.cu:160: 60): identity coefficients of the
+                                     * landmark fits with the slack                                    */
 } tdi_identity_params;
 
 void tdi_default_identity_params(tdi_identity_params* p);
@@ -176,9 +177,8 @@ void tdi_default_identity_params(tdi_identity_params* p);
  * in frame order are then shuffled nperms times by the same engine and concatenated; groups of
  * nframes consecutive entries of that order are taken until ntot_reconstrs groups (or nmax_frames
  * frames).  selected [cap] receives the frame indices in group order, *n_selected their count (a
- * multiple of nframes).  The reference accepts a group when a joint landmark-only fit of its
- * frames succeeds (fit_to_multi_images_landmarks_only); here every frame with an angle has passed
- * its own landmark fit, and every group is accepted. */
+ * multiple of nframes).  Every group is taken: tdi_select_identity_frames adds the reference's
+ * acceptance test, a joint landmark-only fit of the group's frames. */
 int tdi_bin_identity_frames(const tdi_identity_params* p, int32_t n, const float* yaw, const float* pitch,
                             int32_t* selected, int32_t cap, int32_t* n_selected);
 
@@ -186,9 +186,14 @@ int tdi_bin_identity_frames(const tdi_identity_params* p, int32_t n, const float
  * considered (every max(1, n / 800)-th, after skip_first_n_secs, before max_vid_frames, with all
  * 51 landmarks inside the frame), the median face size m of those (resize coefficient 76 / m,
  * returned in *resize_coef), then for every frame the resized landmarks' bounding-box checks
- * (check_if_bb_OK, check_if_face_in_frame, funcs.cu:582-626) and the orthographic start and
- * landmark-only fit with slack on the GPU (all frames in one launch each) giving yaw, pitch, roll
- * (angles [n][3], radians, NaN where there is none; may be NULL), then tdi_bin_identity_frames.
+ * (check_if_bb_OK, check_if_face_in_frame, funcs.cu:582-626), the orthographic start (all frames
+ * in one launch) and each checked frame's landmark-only fit with the slack over lmk_k_id identity
+ * and lmk_k_exp expression coefficients (tdi_fit_landmarks_joint on the frame alone, video_fitter.cu:
+ * 2253-2275, 2330-2345) giving yaw, pitch, roll (angles [n][3], radians, NaN where there is none;
+ * may be NULL), then the binning of tdi_bin_identity_frames, where a group is kept only when the
+ * joint landmark-only fit of its frames with the slack succeeds (fit_to_multi_images_landmarks_only,
+ * model_fitter.cu:1205-1247, from zero identity and expressions and the orthographic poses;
+ * video_fitter.cu:2505-2545).
  * ctx: the identity stage's context (fit_id = fit_tex = 1).  lmk_x, lmk_y [n][51] image pixels of
  * frame_w x frame_h frames, cam [4] (phix, phiy, cx, cy). */
 int tdi_select_identity_frames(tdi_ctx* ctx, const tdi_identity_params* p, int32_t n, const float* lmk_x,
@@ -197,11 +202,14 @@ int tdi_select_identity_frames(tdi_ctx* ctx, const tdi_identity_params* p, int32
 
 /* learn_identity (video_fitter.cu:1925-2016) over the selected frames: for each group of nframes
  * consecutive frames (at most ntot_reconstrs), fit_multiframe (2023-2083) — each frame resized to
- * ref_face_size, masked and cropped as in the per-frame stage, the start from each frame's
- * orthographic pose and landmark fit (identity and albedo at the mean), the light sub-fits of
- * fit_to_multi_images (L, lambda, L, lambda per frame, model_fitter.cu:1374-1386), then the
- * joint fit (tdi_fit_multiframe); alpha [k_id] and beta [k_tex] are the averages over the groups
- * whose joint fit succeeded (more than NITERS_SUCCESS iterations), *n_used their count (0: alpha
+ * ref_face_size, masked and cropped as in the per-frame stage; the start of fit_to_multi_images
+ * (model_fitter.cu:1291-1330): the orthographic poses, the joint landmark-only fit with the slack
+ * over lmk_k_id identity coefficients (a group whose fit fails is not used), then from its result
+ * the joint landmark-only fit without the slack over all identity coefficients
+ * (tdi_fit_landmarks_joint); the light sub-fits of fit_to_multi_images (L, lambda, L, lambda per
+ * frame, 1374-1386), then the joint fit (tdi_fit_multiframe); alpha [k_id] and beta [k_tex] are
+ * the averages over the groups whose joint fit succeeded (more than NITERS_SUCCESS iterations),
+ * *n_used their count (0: alpha
  * and beta untouched, TDI_OK returned).  ctx: fit_id = fit_tex = 1, fit_light = 0 (the light is held
  * by the sub-fits, as in the reference).  bgr: DEVICE [n][frame_h][frame_w][3] the selected frames
  * in selection order (masked in place when paint_innermouth_black); lmk_x, lmk_y [n][51]. */

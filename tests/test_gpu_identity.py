@@ -158,7 +158,16 @@ def test_select_and_learn_identity():
     sel, ang, coef = video.select_identity_frames(ctx, p, lx, ly, W, H, cam)
     _say(f"  resize coefficient {coef:.4f}, yaw (deg) {np.round(np.degrees(ang[:, 0]), 1).tolist()}, selected {sel.tolist()}")
     assert sel.size == 6
-    assert np.array_equal(sel, video.bin_identity_frames(p, ang[:, 0], ang[:, 1]))
+    # the kept groups are groups of the binning, in its order (a group is kept when the joint
+    # landmark-only fit of its frames succeeds; that fit is checked against the oracle in
+    # test_gpu_lmk_joint.py)
+    every = video.bin_identity_frames(video.identity_params(nframes=3, nframes_per_angle_bin=3, nperms=2,
+                                                            ntot_reconstrs=1000, crop_w=128, crop_h=128),
+                                      ang[:, 0], ang[:, 1])
+    groups = [tuple(every[i:i + 3]) for i in range(0, every.size, 3)]
+    kept = [tuple(sel[i:i + 3]) for i in range(0, sel.size, 3)]
+    pos = [groups.index(gk) for gk in kept]
+    assert pos == sorted(pos)
     assert np.isfinite(ang[sel]).all()
     dframes = torch.from_numpy(np.ascontiguousarray(frames[sel])).cuda()
     alpha, beta, used = video.learn_identity(ctx, p, dframes, lx[sel], ly[sel], cam)
@@ -188,12 +197,14 @@ def test_select_and_learn_identity():
         pose, _, _ = video.initialize_with_orthographic(ctx, geo["lmk_x"], geo["lmk_y"], geo["cams"])
         th = np.zeros((3, K), np.float32)
         th[:, lay["pose"]:lay["pose"] + 6] = pose
-        tt = torch.from_numpy(th.copy()).cuda()
-        _, lrep = b.fit_landmarks_slack(tt, min(p.lmk_k_exp, lay["pose"] - lay["exp"]))
-        tt = tt.cpu().numpy()
-        for i in range(3):
-            if lrep[i].status == 0:
-                th[i] = tt[i]
+        kl = min(p.lmk_k_exp, lay["pose"] - lay["exp"])
+        nid = lay["exp"] - lay["id"]
+        th, lrep = b.fit_landmarks_joint(th, min(p.lmk_k_id, nid), kl, True)
+        if lrep.status != 0:
+            b.close()
+            _say(f"  group {g}: joint landmark fit with the slack failed (slack {lrep.slack:.2f})")
+            continue
+        th, _ = b.fit_landmarks_joint(th, nid, kl, False)
         lam = np.tile(synth.LAMBDA0, (3, 1)).astype(np.float32)
         Li = np.full(3, 0.005, np.float32)
         b.set_light(lam, Li)
@@ -204,7 +215,7 @@ def test_select_and_learn_identity():
             b.set_light(lam, Li)
         thj, rep = b.fit_multiframe(th)
         b.close()
-        _say(f"  group {g}: landmark fits {[r.status for r in lrep]}, light {Li.tolist()}, joint fit {rep.iters} "
+        _say(f"  group {g}: landmark fit slack {lrep.slack:.2f}, light {Li.tolist()}, joint fit {rep.iters} "
              f"iterations, stop {rep.stop}, status {rep.status}, success {rep.success}")
         if rep.success and rep.status == 0:
             used2 += 1
