@@ -1316,9 +1316,11 @@ static int lmk_joint(tdi_batch* b, float* theta, int Ka, int Ke, int slack, cons
     // objective (and with mode 1 the joint normal equations in HJ, gJ) at zv
     auto eval = [&](const std::vector<double>& zv, double tau_b, int mode, double& total, bool want_gmin) -> int {
         TDI_HIP(hipMemcpyAsync(dz, zv.data(), sizeof(double) * KJ, hipMemcpyHostToDevice, s));
-        int rc = launch_lmk_joint(c->m, c->L, dfr, b->lmk, dth, n, Ka, Ke, S1, dz, mode, objf, gminf, Hf, gf, J, s);
+        int rc = launch_lmk_joint(c->m, c->L, dfr, b->lmk, dth, n, Ka, Ke, S1, dz, mode, 0, nullptr, objf, gminf, Hf,
+                                  gf, J, s);
         if (!rc && mode) rc = launch_assemble_joint(n, nf, Ks, Hf, gf, HJ, gJ, s);
-        if (!rc) rc = launch_lmk_joint_shared(c->m, c->L, Ka, S1, KJ, dz, tau_b, mode, objf + n, HJ, gJ, s);
+        if (!rc)
+            rc = launch_lmk_joint_shared(c->m, c->L, Ka, S1, KJ, dz, tau_b, nullptr, mode, 1, nullptr, objf + n, HJ, gJ, s);
         if (rc) return rc;
         TDI_HIP(hipMemcpyAsync(hobj.data(), objf, sizeof(double) * (n + 1), hipMemcpyDeviceToHost, s));
         if (want_gmin) TDI_HIP(hipMemcpyAsync(hgmin.data(), gminf, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -1406,6 +1408,165 @@ static int lmk_joint(tdi_batch* b, float* theta, int Ka, int Ke, int slack, cons
             for (int a = 0; a < 6; ++a) th[L.off_pose + a] = (float)zf[Ke + a];
         }
     *report = rep;
+    return TDI_OK;
+}
+
+// Every frame of the batch a group of its own, with the slack (the per-frame landmark fits of the
+// identity stage's selection): each frame keeps its own state of the loop above (barrier round,
+// Newton step, line-search trial), and each evaluation round is one launch over the frames that
+// need it; the Newton steps are one batched large-K Cholesky.
+int tdi_fit_landmarks_each(tdi_batch* b, float* theta, int32_t k_id, int32_t k_exp, tdi_lmk_report* reports)
+{
+    const char* who = "tdi_fit_landmarks_each";
+    if (!b || !theta || !reports) { set_error("%s: null argument", who); return TDI_ERR_ARG; }
+    tdi_ctx* c = b->ctx;
+    if (!c->m.lmk_ids) { set_error("%s: the model has no landmark ids", who); return TDI_ERR_ARG; }
+    const int n = b->n, K = c->L.K, Ka = k_id, Ke = k_exp, Ks = Ka + 1, nf = Ks + Ke + 6;
+    if (n < 1) return TDI_OK;
+    (void)hipSetDevice(c->device);
+    hipStream_t s = 0;
+    Alloc A;
+    A.pool = c->pool;
+    struct Release {
+        Alloc& a;
+        ~Release() { (void)hipDeviceSynchronize(); a.release(); }
+    } release{A};
+    FrameGPU* dfr = A.get<FrameGPU>(n);
+    float* dth = A.get<float>((size_t)n * K);
+    double* dz = A.get<double>((size_t)n * nf);
+    double* objf = A.get<double>(2 * (size_t)n);   // the frames' landmark terms, then their once-counted terms
+    double* gminf = A.get<double>(n);
+    double* Hf = A.get<double>((size_t)n * nf * nf);
+    double* gf = A.get<double>((size_t)n * nf);
+    double* J = A.get<double>(lmk_joint_scratch_doubles(n, Ka, Ke, 1));
+    double* dd = A.get<double>((size_t)n * nf);
+    int* dst = A.get<int>(n);
+    int* dmask = A.get<int>(n);
+    double* dtau = A.get<double>(n);
+    for (void* p : A.ptrs)
+        if (!p) { set_error("%s: allocation failed", who); return TDI_ERR_NOMEM; }
+    TDI_HIP(hipMemcpyAsync(dfr, b->hframes.data(), sizeof(FrameGPU) * n, hipMemcpyHostToDevice, s));
+    TDI_HIP(hipMemcpyAsync(dth, theta, sizeof(float) * (size_t)n * K, hipMemcpyHostToDevice, s));
+    const Layout& L = c->L;
+    std::vector<double> z((size_t)n * nf), zt, d((size_t)n * nf), hobj(2 * (size_t)n), hgmin(n), htau(n, 0.0);
+    std::vector<int> mask(n, 1), hst(n);
+    for (int f = 0; f < n; ++f) {
+        const float* th = theta + (size_t)f * K;
+        double* zf = &z[(size_t)f * nf];
+        for (int k = 0; k < Ka; ++k) zf[k] = th[L.off_id + k];
+        zf[Ka] = 0.0;
+        for (int k = 0; k < Ke; ++k) zf[Ks + k] = th[L.off_exp + k];
+        for (int a = 0; a < 6; ++a) zf[Ks + Ke + a] = th[L.off_pose + a];
+    }
+    // objective (mode 1: also each frame's system in Hf, gf) of the masked frames at zv
+    auto eval = [&](const std::vector<double>& zv, int mode, bool want_gmin) -> int {
+        TDI_HIP(hipMemcpyAsync(dz, zv.data(), sizeof(double) * zv.size(), hipMemcpyHostToDevice, s));
+        TDI_HIP(hipMemcpyAsync(dmask, mask.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        TDI_HIP(hipMemcpyAsync(dtau, htau.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        int rc = launch_lmk_joint(c->m, L, dfr, b->lmk, dth, n, Ka, Ke, 1, dz, mode, 1, dmask, objf, gminf, Hf, gf, J, s);
+        if (!rc) rc = launch_lmk_joint_shared(c->m, L, Ka, 1, nf, dz, 0.0, dtau, mode, n, dmask, objf + n, Hf, gf, s);
+        if (rc) return rc;
+        TDI_HIP(hipMemcpyAsync(hobj.data(), objf, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
+        if (want_gmin) TDI_HIP(hipMemcpyAsync(hgmin.data(), gminf, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipStreamSynchronize(s));
+        return TDI_OK;
+    };
+    auto total = [&](int f) { return hobj[(size_t)n + f] + hobj[f]; };
+    enum Phase { TOP, SEARCH, DONE };
+    struct St {
+        Phase phase = TOP;
+        int round = 0, it = 0;
+        double t = 1.0, obj = 0.0;
+        bool success = false;
+    };
+    std::vector<St> st(n);
+    std::vector<tdi_lmk_report> rep(n);
+    // set_minimal_slack: s = 5 - min G at s = 0
+    int rc = eval(z, 0, true);
+    if (rc) return rc;
+    for (int f = 0; f < n; ++f) {
+        z[(size_t)f * nf + Ka] = 5.0 - hgmin[f];
+        rep[f].slack0 = (float)z[(size_t)f * nf + Ka];
+        htau[f] = 100.0;
+        rep[f].rounds = 1;
+    }
+    // the round of frame f ends: the next one starts, or the fit ends after five
+    auto next_round = [&](int f) {
+        St& q = st[f];
+        if (++q.round >= 5) { q.phase = DONE; return; }
+        htau[f] *= 10.0;
+        rep[f].rounds++;
+        q.it = 0;
+        q.phase = TOP;
+    };
+    for (;;) {
+        // the top of an iteration: success, the round's step limit, else a Newton step
+        bool any = false;
+        for (int f = 0; f < n; ++f) {
+            St& q = st[f];
+            while (q.phase == TOP) {
+                if (z[(size_t)f * nf + Ka] < -0.01) { q.success = true; q.phase = DONE; break; }
+                if (q.it >= 100) { next_round(f); continue; }
+                break;
+            }
+            mask[f] = q.phase == TOP;
+            any = any || mask[f];
+        }
+        if (!any) break;
+        if ((rc = eval(z, 1, false))) return rc;
+        if ((rc = launch_cholesky_big_f64(n, nf, Hf, gf, dd, dst, s))) return rc;
+        TDI_HIP(hipMemcpyAsync(d.data(), dd, sizeof(double) * d.size(), hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipMemcpyAsync(hst.data(), dst, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        TDI_HIP(hipStreamSynchronize(s));
+        for (int f = 0; f < n; ++f) {
+            if (!mask[f]) continue;
+            St& q = st[f];
+            rep[f].iterations++;
+            if (hst[f] != TDI_OK) { next_round(f); continue; }
+            q.obj = total(f);
+            q.t = 1.0;
+            q.phase = SEARCH;
+        }
+        // the line searches, one objective evaluation per round for every frame still searching
+        for (;;) {
+            zt = z;
+            bool trial = false;
+            for (int f = 0; f < n; ++f) {
+                St& q = st[f];
+                mask[f] = 0;
+                if (q.phase != SEARCH) continue;
+                if (q.t < 0.01) { next_round(f); continue; }
+                for (int j = 0; j < nf; ++j) zt[(size_t)f * nf + j] = z[(size_t)f * nf + j] + q.t * d[(size_t)f * nf + j];
+                mask[f] = 1;
+                trial = true;
+            }
+            if (!trial) break;
+            if ((rc = eval(zt, 0, false))) return rc;
+            for (int f = 0; f < n; ++f) {
+                if (!mask[f]) continue;
+                St& q = st[f];
+                if (total(f) < q.obj) {
+                    std::copy(zt.begin() + (size_t)f * nf, zt.begin() + (size_t)(f + 1) * nf, z.begin() + (size_t)f * nf);
+                    rep[f].accepted++;
+                    q.it++;
+                    q.phase = TOP;
+                } else {
+                    q.t *= 0.5;
+                }
+            }
+        }
+    }
+    for (int f = 0; f < n; ++f) {
+        const double* zf = &z[(size_t)f * nf];
+        rep[f].status = st[f].success ? TDI_OK : TDI_LMK_INFEASIBLE;
+        rep[f].slack = (float)zf[Ka];
+        reports[f] = rep[f];
+        if (!st[f].success) continue;
+        float* th = theta + (size_t)f * K;
+        for (int k = 0; k < Ka; ++k) th[L.off_id + k] = (float)zf[k];
+        for (int k = 0; k < Ke; ++k) th[L.off_exp + k] = (float)zf[Ks + k];
+        for (int a = 0; a < 6; ++a) th[L.off_pose + a] = (float)zf[Ks + Ke + a];
+    }
     return TDI_OK;
 }
 

@@ -398,11 +398,14 @@ int launch_assemble_joint(int n, int K, int Ks, const double* H, const double* g
 // [alpha | s | eps | tau | u]) and the terms counted once (identity bounds, tau_b s)
 constexpr int LMK_JOINT_MAXN = 320;   // unknowns per frame: k_id + slack + k_exp + 6
 size_t lmk_joint_scratch_doubles(int n, int Ka, int Ke, int S1);
+// each = 1: every frame a group of its own (z [n][nf], theta0 [n][K]); mask [n] (may be null)
+// selects the frames to evaluate
 int launch_lmk_joint(const ModelGPU& m, const Layout& L, const FrameGPU* frames, const float* lmk,
-                     const float* theta0, int n, int Ka, int Ke, int S1, const double* z, int mode, double* objf,
-                     double* gminf, double* H, double* g, double* J, hipStream_t s);
+                     const float* theta0, int n, int Ka, int Ke, int S1, const double* z, int mode, int each,
+                     const int* mask, double* objf, double* gminf, double* H, double* g, double* J, hipStream_t s);
 int launch_lmk_joint_shared(const ModelGPU& m, const Layout& L, int Ka, int S1, long long KJ, const double* z,
-                            double tau_b, int mode, double* obj, double* HJ, double* gJ, hipStream_t s);
+                            double tau_b, const double* taus, int mode, int groups, const int* mask, double* obj,
+                            double* HJ, double* gJ, hipStream_t s);
 int launch_shared_bounds(int Ks, long long KJ, const float* x, const float* lb, const float* ub, int mode, double* obj,
                          double* HJ, double* gJ, hipStream_t s);
 

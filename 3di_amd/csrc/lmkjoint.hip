@@ -35,20 +35,25 @@ struct LjShared {
     double red[LJ_THREADS / 64];
 };
 
+// each: every frame is a group of its own, its unknowns z [f][nf] in its own order, and mask [n]
+// selects the frames to evaluate (null: all); theta0 is then each frame's own row [n][K].
 __global__ void __launch_bounds__(LJ_THREADS) k_lmk_joint(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                                           const float* __restrict__ lmk, const float* __restrict__ theta0,
                                                           int n, int Ka, int Ke, int S1, const double* __restrict__ z,
-                                                          int mode, double* __restrict__ objf,
+                                                          int mode, int each, const int* __restrict__ mask,
+                                                          double* __restrict__ objf,
                                                           double* __restrict__ gminf, double* __restrict__ H,
                                                           double* __restrict__ g, double* __restrict__ J)
 {
     const int f = blockIdx.x;
-    if (f >= n) return;
+    if (f >= n || (mask && !mask[f])) return;
     __shared__ LjShared S;
     const int tid = threadIdx.x;
     const int Ks = Ka + S1, oe = Ks, op = Ks + Ke, nf = op + 6, nfp = (nf + 3) & ~3;
     const size_t zf = (size_t)Ks + (size_t)f * (Ke + 6);
-    for (int j = tid; j < nf; j += LJ_THREADS) S.q[j] = j < Ks ? z[j] : z[zf + (j - Ks)];
+    for (int j = tid; j < nf; j += LJ_THREADS)
+        S.q[j] = each ? z[(size_t)f * nf + j] : (j < Ks ? z[j] : z[zf + (j - Ks)]);
+    if (each) theta0 += (size_t)f * L.K;
     const FrameGPU fr = frames[f];
     const float* lm = lmk + (size_t)f * 6 * NLMK;
     // landmark positions without the fitted columns: the mean plus the identity columns past Ka
@@ -240,13 +245,25 @@ __global__ void __launch_bounds__(LJ_THREADS) k_lmk_joint(ModelGPU m, Layout L, 
 // The terms counted once for the group, at the joint vector z: the identity bounds (the
 // identity coefficients' bounds of the model) and tau_b s.  obj[0] = their objective; mode 1
 // also adds their gradient and Hessian diagonal to gJ and HJ (the gradient of tau_b s is tau_b).
+// With a grid of n workgroups (every frame a group of its own) workgroup f works on z, obj, HJ
+// and gJ offset by f (strides KJ, 1, KJ^2, KJ), skipping frames whose mask is 0.
 __global__ void __launch_bounds__(256) k_lmk_joint_shared(ModelGPU m, Layout L, int Ka, int S1, long long KJ,
-                                                          const double* __restrict__ z, double tau_b, int mode,
+                                                          const double* __restrict__ z, double tau_b,
+                                                          const double* __restrict__ taus, int mode,
+                                                          const int* __restrict__ mask,
                                                           double* __restrict__ obj, double* __restrict__ HJ,
                                                           double* __restrict__ gJ)
 {
     __shared__ double sred[4];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, f = blockIdx.x;
+    if (mask && !mask[f]) return;
+    if (taus) tau_b = taus[f];   // each group's own barrier weight
+    z += (size_t)f * KJ;
+    obj += f;
+    if (mode) {
+        HJ += (size_t)f * KJ * KJ;
+        gJ += (size_t)f * KJ;
+    }
     double acc = 0.0;
     for (int k = tid; k < Ka; k += blockDim.x) {
         const float lbf = m.lb[L.off_id + k], ubf = m.ub[L.off_id + k];
@@ -278,8 +295,8 @@ size_t lmk_joint_scratch_doubles(int n, int Ka, int Ke, int S1)
 }
 
 int launch_lmk_joint(const ModelGPU& m, const Layout& L, const FrameGPU* frames, const float* lmk,
-                     const float* theta0, int n, int Ka, int Ke, int S1, const double* z, int mode, double* objf,
-                     double* gminf, double* H, double* g, double* J, hipStream_t s)
+                     const float* theta0, int n, int Ka, int Ke, int S1, const double* z, int mode, int each,
+                     const int* mask, double* objf, double* gminf, double* H, double* g, double* J, hipStream_t s)
 {
     if (Ka < 0 || Ka > L.nid || Ke < 0 || Ke > L.nexp || Ke > LJ_THREADS - 64 || Ka + S1 + Ke + 6 > LMK_JOINT_MAXN) {
         set_error("joint landmark fit: k_id %d (of %d), k_exp %d (of %d, at most %d), %d unknowns per frame (at most %d)",
@@ -288,15 +305,17 @@ int launch_lmk_joint(const ModelGPU& m, const Layout& L, const FrameGPU* frames,
     }
     if (n <= 0) return TDI_OK;
     hipLaunchKernelGGL(k_lmk_joint, dim3(n), dim3(LJ_THREADS), 0, s, m, L, frames, lmk, theta0, n, Ka, Ke, S1, z, mode,
-                       objf, gminf, H, g, J);
+                       each, mask, objf, gminf, H, g, J);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
 
 int launch_lmk_joint_shared(const ModelGPU& m, const Layout& L, int Ka, int S1, long long KJ, const double* z,
-                            double tau_b, int mode, double* obj, double* HJ, double* gJ, hipStream_t s)
+                            double tau_b, const double* taus, int mode, int groups, const int* mask, double* obj,
+                            double* HJ, double* gJ, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lmk_joint_shared, dim3(1), dim3(256), 0, s, m, L, Ka, S1, KJ, z, tau_b, mode, obj, HJ, gJ);
+    hipLaunchKernelGGL(k_lmk_joint_shared, dim3(groups), dim3(256), 0, s, m, L, Ka, S1, KJ, z, tau_b, taus, mode, mask,
+                       obj, HJ, gJ);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

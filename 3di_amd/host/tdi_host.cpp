@@ -1254,15 +1254,39 @@ int tdi_select_identity_frames(tdi_ctx* ctx, const tdi_identity_params* p, int32
         return rc;
     };
     std::vector<float> th;
-    for (int f = 0; !rc && f < n; ++f) {
-        if (!ok[f]) continue;
-        tdi_lmk_report rep{};
-        if ((rc = fit_group(std::vector<int>{f}, th, rep))) break;
-        if (rep.status != TDI_OK) continue;
-        float R[9], ypr[3];
-        tdi_host::rotation_f32(&th[op + 3], R);
-        tdi_host::compute_euler_angles(R, ypr[0], ypr[1], ypr[2]);
-        yaw[f] = ypr[0]; pitch[f] = ypr[1]; roll[f] = ypr[2];
+    {
+        // every checked frame's own fit, all frames together
+        std::vector<int> idx;
+        for (int f = 0; f < n; ++f)
+            if (ok[f]) idx.push_back(f);
+        const int m = (int)idx.size();
+        std::vector<float> gx((size_t)m * NL), gy((size_t)m * NL), gc((size_t)m * 4), gf(m);
+        for (int t = 0; t < m; ++t) {
+            const int f = idx[t];
+            std::memcpy(&gx[(size_t)t * NL], &lx[(size_t)f * NL], sizeof(float) * NL);
+            std::memcpy(&gy[(size_t)t * NL], &ly[(size_t)f * NL], sizeof(float) * NL);
+            std::memcpy(&gc[(size_t)t * 4], &cams[(size_t)f * 4], sizeof(float) * 4);
+            gf[t] = fs[f];
+        }
+        th.assign((size_t)m * K, 0.f);
+        for (int t = 0; t < m; ++t)
+            for (int a = 0; a < 6; ++a) th[(size_t)t * K + op + a] = pose[(size_t)idx[t] * 6 + a];
+        std::vector<tdi_lmk_report> rep(m);
+        tdi_batch* b = nullptr;
+        if (m > 0) {
+            rc = tdi_host::landmark_batch(ctx, m, gx.data(), gy.data(), gc.data(), gf.data(), lo.data(), hi.data(), one,
+                                          &b);
+            if (!rc) rc = tdi_fit_landmarks_each(b, th.data(), ka, kl, rep.data());
+            if (b) tdi_batch_destroy(b);
+        }
+        for (int t = 0; !rc && t < m; ++t) {
+            if (rep[t].status != TDI_OK) continue;
+            float R[9], ypr[3];
+            tdi_host::rotation_f32(&th[(size_t)t * K + op + 3], R);
+            tdi_host::compute_euler_angles(R, ypr[0], ypr[1], ypr[2]);
+            const int f = idx[t];
+            yaw[f] = ypr[0]; pitch[f] = ypr[1]; roll[f] = ypr[2];
+        }
     }
     if (angles)
         for (int f = 0; f < n; ++f) { angles[3 * (size_t)f] = yaw[f]; angles[3 * (size_t)f + 1] = pitch[f]; angles[3 * (size_t)f + 2] = roll[f]; }

@@ -75,6 +75,7 @@ def lib():
         L.tdi_batch_launch_shape.argtypes = [_vp, _vp]
         L.tdi_landmark_slack_terms.argtypes = [_vp, _vp, C.c_int32, _vp, C.c_double, _vp, _vp, _vp, _vp]
         L.tdi_fit_landmarks_joint.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp]
+        L.tdi_fit_landmarks_each.argtypes = [_vp, _vp, C.c_int32, C.c_int32, _vp]
         L.tdi_landmark_joint_terms.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, C.c_double, _vp, _vp,
                                                _vp]
         L.tdi_prepare_crops.argtypes = [C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp]
@@ -306,6 +307,14 @@ class Batch:
         _check(lib().tdi_fit_landmarks_joint(self.h, th.ctypes.data, int(k_id), int(k_exp), int(bool(slack)),
                                              C.addressof(rep)))
         return th, rep
+
+    def fit_landmarks_each(self, theta0, k_id: int, k_exp: int):
+        """Every frame's landmark-only fit with the slack on its own, batched
+        (tdi_fit_landmarks_each): theta0 host [n, K]; returns (theta [n, K] float32, reports)."""
+        th = np.ascontiguousarray(np.array(theta0, np.float32).reshape(self.n, self.K))
+        reps = (abi.tdi_lmk_report * self.n)()
+        _check(lib().tdi_fit_landmarks_each(self.h, th.ctypes.data, int(k_id), int(k_exp), C.addressof(reps)))
+        return th, list(reps)
 
     def landmark_joint_terms(self, theta, k_id: int, k_exp: int, slack: bool, z, tau: float):
         """(obj, g [N], H [N, N]) float64 of that fit at the joint unknowns z (host, FP64)."""

@@ -330,6 +330,16 @@ int tdi_landmark_slack_terms(tdi_batch* b, const float* theta, int32_t k_exp, co
 int tdi_fit_landmarks_joint(tdi_batch* b, float* theta, int32_t k_id, int32_t k_exp, int32_t slack,
                             tdi_lmk_report* report);
 
+/* That fit with the slack for every frame of the batch on its own (each frame a group of one: the
+ * per-frame landmark fits of the identity stage's selection, video_fitter.cu:2253-2275,
+ * 2330-2345, where li_init carries K_ALPHA_L identity coefficients and is reset for every frame).
+ * The same steps as tdi_fit_landmarks_joint on a one-frame batch, for all frames together: each
+ * frame keeps its own barrier round, Newton step and line-search trial, each evaluation round is
+ * one launch over the frames that need it, the Newton steps one batched large-K Cholesky.
+ * theta: HOST [n][K] in/out (each frame's own identity, expressions and pose are its start;
+ * overwritten for the frames that succeed); reports: HOST [n].  Synchronous. */
+int tdi_fit_landmarks_each(tdi_batch* b, float* theta, int32_t k_id, int32_t k_exp, tdi_lmk_report* reports);
+
 /* The terms of that fit at joint unknowns z (HOST, FP64, the order above), for checks: theta
  * HOST [n][K] (frame 0's identity past k_id), tau the barrier weight; outputs HOST: H [N][N]
  * (Gauss-Newton, FP64), g [N], obj (NaN or +inf outside the feasible set);

@@ -99,3 +99,43 @@ def test_joint_fits_match_the_oracle():
     assert d2 < 1e-5
     b.close()
     ctx.close()
+
+
+def test_each_frame_on_its_own():
+    _say("case: every frame's landmark fit with identity on its own, batched")
+    sc = joint_scene(crops=(128, 160, 144, 150))
+    lay = sc.lay
+    Ka, Ke = 12, 10
+    n = len(sc.frames)
+    rng = np.random.default_rng(5)
+    th0 = sc.truth().copy()
+    th0[:, lay["id"]:lay["exp"]] = 0.0
+    th0[:, lay["exp"]:lay["pose"]] = 0.0
+    th0[:, lay["pose"]:lay["pose"] + 3] += rng.normal(0, 6, (n, 3))
+    th0[:, lay["pose"] + 3:lay["pose"] + 6] += rng.normal(0, 0.05, (n, 3))
+    th0[2, lay["pose"] + 2] += 4000.0   # a face far beyond its landmarks' scale: slower to become feasible
+    th0 = th0.astype(np.float32)
+    ctx = tdi.Context(sc.model, sc.opts)
+    b = tdi.Batch(ctx, sc.frames)
+    the, reps = b.fit_landmarks_each(th0, Ka, Ke)
+    b.close()
+    ors = sc.oracle()
+    for f in range(n):
+        bf = tdi.Batch(ctx, [sc.frames[f]])
+        thj, rj = bf.fit_landmarks_joint(th0[f:f + 1], Ka, Ke, True)
+        bf.close()
+        r = reps[f]
+        _say(f"  frame {f}: batched status {r.status} iterations {r.iterations} accepted {r.accepted} rounds {r.rounds}; "
+             f"alone {rj.status} {rj.iterations} {rj.accepted} {rj.rounds}")
+        assert (r.status, r.iterations, r.accepted, r.rounds) == (rj.status, rj.iterations, rj.accepted, rj.rounds)
+        assert r.slack0 == rj.slack0 and r.slack == rj.slack
+        assert np.array_equal(the[f], thj[0])
+        # and the oracle on the frame alone
+        O = importlib.import_module("pyoracle")
+        so = O.Scene(sc.model, [sc.frames[f]], sc.opts)
+        tho, ro = so.fit_landmarks_joint(th0[f:f + 1].astype(np.float64), Ka, Ke, True)
+        assert (ro.status, ro.iterations, ro.accepted) == (r.status, r.iterations, r.accepted)
+        if r.status == 0:
+            assert np.abs(the[f] - tho[0]).max() <= 1e-5 * np.abs(tho[0]).max()
+    assert sum(r.status == 0 for r in reps) >= 3
+    ctx.close()
