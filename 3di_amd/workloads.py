@@ -52,13 +52,19 @@ def base_model(grid: int = 231, k_exp: int = 79, k_id: int = 199, k_tex: int = 1
     return _base_cache[key]
 
 
+def folded_identity(k_id: int = 199, k_tex: int = 199):
+    """The identity (alpha) and albedo (beta) coefficients folded into the per-frame stage's model."""
+    rng = np.random.default_rng(synth.BASE_SEED + 1)
+    alpha = np.clip(rng.standard_normal(k_id), -3, 3)
+    beta = np.clip(rng.standard_normal(k_tex), -3, 3)
+    return alpha, beta
+
+
 def cfg2_model(grid: int = 231, k_exp: int = 79, k_id: int = 199, k_tex: int = 199):
     key = (grid, k_exp, k_id, k_tex)
     if key not in _cfg1_cache:
         base = base_model(grid, k_exp, k_id, k_tex)
-        rng = np.random.default_rng(synth.BASE_SEED + 1)
-        alpha = np.clip(rng.standard_normal(k_id), -3, 3)
-        beta = np.clip(rng.standard_normal(k_tex), -3, 3)
+        alpha, beta = folded_identity(k_id, k_tex)
         _cfg1_cache[key] = synth.fold_identity(base, alpha, beta)
     return _cfg1_cache[key]
 

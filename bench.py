@@ -252,6 +252,8 @@ def main():
     ap.add_argument("--video-frames", type=int, default=1024,
                     help="frames of the long-video run (1920x1080 BGR8, tdi_fit_video_long); 0 skips it")
     ap.add_argument("--video-chunk", type=int, default=512, help="frames per chunk of the long-video run")
+    ap.add_argument("--no-identity-learning", action="store_true",
+                    help="skip the identity learning (frame selection + learn_identity) over the long video's frames")
     ap.add_argument("--identity-frames", type=int, default=10,
                     help="frames of the identity-stage joint fit (512 x 512, 398 + 94 per frame unknowns); 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
@@ -268,6 +270,7 @@ def main():
     import torch
     tdi = importlib.import_module("3di_amd.tdi")
     workloads = importlib.import_module("3di_amd.workloads")
+    abi = importlib.import_module("3di_amd.abi")
 
     w = workloads.cfg2(n_frames=args.frames, log=log)
     K = w.lay["K"]
@@ -490,6 +493,7 @@ def main():
     # rendered on the GPU chunk by chunk inside the frame source, NRES_COEFS = 4 reference face
     # sizes, 256 x 256 crops, landmark fit with slack, light and shape sub-fits per size
     long_video = None
+    identity_learning = None
     if args.video_frames > 0:
         import tempfile
         from scipy.spatial.transform import Rotation
@@ -541,6 +545,47 @@ def main():
         log(f"[bench] long video: {long_video['frames_per_s']} frames/s, "
             f"median pose error {long_video['pose_angle_error_deg']['median']} deg")
         ctxv.close()
+        if not args.no_identity_learning:
+            # the identity stage over the same video (fit_video_frames_landmarks_sparse and
+            # learn_identity): frame selection from the landmarks, then the multi-frame fits of the
+            # selected groups, with the model whose identity and albedo are unknowns; the video's
+            # faces carry the identity folded into the per-frame model
+            m_id = workloads.base_model(231, 79, 199, 199)
+            ctx_id = tdi.Context(m_id, abi.default_options(max_iters=30, fit_light=0, fit_id=1, fit_tex=1))
+            p_id = video.identity_params()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sel, ang, rcoef = video.select_identity_frames(ctx_id, p_id, lvw.lmk_x, lvw.lmk_y, lvw.width, lvw.height,
+                                                           lvw.cams[0])
+            torch.cuda.synchronize()
+            t_sel = time.perf_counter() - t0
+            a_l, b_l, used = None, None, 0
+            t_learn = 0.0
+            if sel.size:
+                fsel = lvw.source(0, lvw.n)[torch.from_numpy(sel.astype(np.int64)).cuda()].contiguous()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                a_l, b_l, used = video.learn_identity(ctx_id, p_id, fsel, lvw.lmk_x[sel], lvw.lmk_y[sel], lvw.cams[0])
+                torch.cuda.synchronize()
+                t_learn = time.perf_counter() - t0
+            a_t, _ = workloads.folded_identity(199, 199)
+            B = m_id.id_basis.astype(np.float64)
+            rms = lambda d: float(np.sqrt(np.mean(np.sum((B @ d).reshape(-1, 3) ** 2, axis=1))))
+            identity_learning = dict(
+                frames=lvw.n, frames_with_angles=int(np.isfinite(ang[:, 0]).sum()), selected_frames=int(sel.size),
+                groups=int(sel.size // max(1, p_id.nframes)), groups_used=int(used),
+                nframes=int(p_id.nframes), selection_seconds=round(t_sel, 3), learn_seconds=round(t_learn, 3),
+                identity_shape_rms_mm_vs_truth=dict(
+                    mean_shape=round(rms(-a_t), 3),
+                    learned=None if not used else round(rms(a_l.astype(np.float64) - a_t), 3)),
+                note="selection: every frame's landmark fit with 60 identity coefficients (batched), the 13 x 13 "
+                     "angle bins, groups kept when their joint landmark fit succeeds; learn_identity: per group the "
+                     "two joint landmark fits, the light sub-fits and the joint shape fit (30 iterations at most), "
+                     "the identity averaged over the groups used")
+            log(f"[bench] identity learning: {sel.size} frames selected of {lvw.n} in {t_sel:.2f} s, "
+                f"{used} groups used in {t_learn:.2f} s, identity shape RMS vs truth "
+                f"{identity_learning['identity_shape_rms_mm_vs_truth']}")
+            ctx_id.close()
 
     # the identity stage: joint fit of frames sharing albedo and identity (tdi_fit_multiframe), and
     # the joint normal equations and first step against the FP64 oracle
@@ -715,6 +760,7 @@ def main():
         "differentiation": differentiation,
         "large_k_cholesky": large_k_cholesky,
         "long_video": long_video,
+        "identity_learning": identity_learning,
         "identity_stage": identity,
         "frame_prep": frame_prep,
     }
