@@ -188,35 +188,71 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 }
             }
         }
-        // the other columns: albedo and identity (before the expression block), pose and light
-        const int n_lo = L.off_exp, n_other = L.off_exp + (K - L.off_pose);
-        for (int o = l16; o < n_other; o += 16) {
-            const int j = o < n_lo ? o : L.off_pose + (o - n_lo);
+        // identity columns (the identity stage), 80 per group as the expression columns
+        if (L.nid) {
+            const int ki = m.id_ld;
+            const float* Ib = m.id_basis + (size_t)(3 * vb) * ki;
+            const float* Ic = m.id_basis + (size_t)(3 * vc) * ki;
+            const float* Ia = m.id_basis + (size_t)(3 * va) * ki;
+            for (int c0 = 0; c0 < L.nid; c0 += 80) {
+                float eb[5][3], ec[5][3], ea[5][3];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    const int c = min(c0 + 16 * i + l16, L.nid - 1);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        eb[i][a] = Ib[(size_t)a * ki + c];
+                        ec[i][a] = Ic[(size_t)a * ki + c];
+                        ea[i][a] = Ia[(size_t)a * ki + c];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    const int c = c0 + 16 * i + l16;
+                    const float ex = al * eb[i][0] + be * ec[i][0] + ga * ea[i][0];
+                    const float ey = al * eb[i][1] + be * ec[i][1] + ga * ea[i][1];
+                    const float ez = al * eb[i][2] + be * ec[i][2] + ga * ea[i][2];
+                    if (c < L.nid) out[L.off_id + c] = q0 * ex + q1 * ey + q2 * ez;
+                }
+            }
+        }
+        // albedo columns (the identity stage), 80 per group
+        if (L.ntex) {
+            const int kt = m.tex_ld;
+            const float* Tb = m.tex_basis + (size_t)vb * kt;
+            const float* Tc = m.tex_basis + (size_t)vc * kt;
+            const float* Ta = m.tex_basis + (size_t)va * kt;
+            for (int c0 = 0; c0 < L.ntex; c0 += 80) {
+                float tb[5], tc[5], ta[5];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    const int c = min(c0 + 16 * i + l16, L.ntex - 1);
+                    tb[i] = Tb[c];
+                    tc[i] = Tc[c];
+                    ta[i] = Ta[c];
+                }
+#pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    const int c = c0 + 16 * i + l16;
+                    const float tt = al * tb[i] + be * tc[i] + ga * ta[i];
+                    if (c < L.ntex) out[L.off_tex + c] = tt + sr.L * Id * tt + S * tt;
+                }
+            }
+        }
+        // the other columns: pose and light
+        for (int j = L.off_pose + l16; j < K; j += 16) {
             float val;
-            if (j >= L.off_pose && j < L.off_pose + 3) {
+            if (j < L.off_pose + 3) {
                 val = (j == L.off_pose) ? c0 : (j == L.off_pose + 1) ? c1 : c2;
-            } else if (j >= L.off_pose + 3 && j < L.off_light) {
+            } else if (j < L.off_light) {
                 const float* d = sr.dR + 9 * (j - L.off_pose - 3);
                 const float dvx = d[0] * p0[0] + d[3] * p0[1] + d[6] * p0[2];
                 const float dvy = d[1] * p0[0] + d[4] * p0[1] + d[7] * p0[2];
                 const float dvz = d[2] * p0[0] + d[5] * p0[1] + d[8] * p0[2];
                 val = c0 * dvx + c1 * dvy + c2 * dvz;
-            } else if (j >= L.off_light) {
+            } else {
                 const int c = j - L.off_light;
                 val = (c < 3) ? (L.nlight == 9 ? rho : sr.L * rho) * rn[c] : (c == 3) ? rho * Id : rho * Y[min(c - 4, 4)];
-            } else if (j >= L.off_id) {
-                const int c = j - L.off_id;
-                const float* E = m.id_basis;
-                const int ki = m.id_ld;
-                const float ex = al * E[(size_t)(3 * vb) * ki + c] + be * E[(size_t)(3 * vc) * ki + c] + ga * E[(size_t)(3 * va) * ki + c];
-                const float ey = al * E[(size_t)(3 * vb + 1) * ki + c] + be * E[(size_t)(3 * vc + 1) * ki + c] + ga * E[(size_t)(3 * va + 1) * ki + c];
-                const float ez = al * E[(size_t)(3 * vb + 2) * ki + c] + be * E[(size_t)(3 * vc + 2) * ki + c] + ga * E[(size_t)(3 * va + 2) * ki + c];
-                val = q0 * ex + q1 * ey + q2 * ez;
-            } else {
-                const int c = j - L.off_tex;
-                const float tt = al * m.tex_basis[(size_t)vb * m.tex_ld + c] + be * m.tex_basis[(size_t)vc * m.tex_ld + c]
-                               + ga * m.tex_basis[(size_t)va * m.tex_ld + c];
-                val = tt + sr.L * Id * tt + S * tt;
             }
             out[j] = val;
         }
