@@ -61,6 +61,9 @@ __device__ inline void load_frame_rot(const Layout& L, const FrameGPU& fr, const
 constexpr int DI_PX_PER_WG = 256;   // pixels per workgroup and frame in k_dI
 
 // Four wavefronts per SIMD (register budget 128): more pixels' gathers in flight per CU.
+// IDTEX: the layout has identity or albedo columns (the identity stage); the per-frame fit's
+// instantiation carries none of their code, which would cost it registers.
+template <bool IDTEX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_dI(ModelGPU m, Layout L, const FrameGPU* __restrict__ frames,
                                              const int* __restrict__ act, int n, const float* __restrict__ theta,
                                              const double* __restrict__ P0, const int4* __restrict__ pvert,
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             }
         }
         // identity columns (the identity stage), 80 per group as the expression columns
-        if (L.nid) {
+        if (IDTEX && L.nid) {
             const int ki = m.id_ld;
             const float* Ib = m.id_basis + (size_t)(3 * vb) * ki;
             const float* Ic = m.id_basis + (size_t)(3 * vc) * ki;
@@ -217,7 +220,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             }
         }
         // albedo columns (the identity stage), 80 per group
-        if (L.ntex) {
+        if (IDTEX && L.ntex) {
             const int kt = m.tex_ld;
             const float* Tb = m.tex_basis + (size_t)vb * kt;
             const float* Tc = m.tex_basis + (size_t)vc * kt;
@@ -1043,8 +1046,12 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameL
     int nsb = (mean_area / 4 + 8 * DI_PX_PER_WG - 1) / (8 * DI_PX_PER_WG);
     if (nsb > 64) nsb = 64;
     if (nsb < 1) nsb = 1;
-    hipLaunchKernelGGL(k_dI, dim3(8 * nsb, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0,
-                       pvert, pbary, npix, pixg, dI);
+    if (L.nid || L.ntex)
+        hipLaunchKernelGGL(k_dI<true>, dim3(8 * nsb, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act,
+                           fl.n, theta, P0, pvert, pbary, npix, pixg, dI);
+    else
+        hipLaunchKernelGGL(k_dI<false>, dim3(8 * nsb, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act,
+                           fl.n, theta, P0, pvert, pbary, npix, pixg, dI);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }

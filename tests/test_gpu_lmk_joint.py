@@ -24,19 +24,22 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _start(sc, Ka, Ke, slack, rng):
+def _start(sc, Ka, Ke, slack, rng, spread=1.0):
+    """Joint unknowns near the truth: with the slack (s = 40) anywhere nearby; without it the
+    landmarks must lie inside their boxes, so spread scales the perturbation down."""
     lay = sc.lay
     tr = sc.truth()
     n = len(sc.frames)
     o = Ka + (1 if slack else 0)
     z = np.zeros(o + n * (Ke + 6))
-    z[:Ka] = tr[0, lay["id"]:lay["id"] + Ka] + rng.normal(0, 0.05, Ka)
+    z[:Ka] = tr[0, lay["id"]:lay["id"] + Ka] + spread * rng.normal(0, 0.05, Ka)
     if slack:
         z[Ka] = 40.0
     for t in range(n):
         b = z[o + t * (Ke + 6):o + (t + 1) * (Ke + 6)]
-        b[:Ke] = tr[t, lay["exp"]:lay["exp"] + Ke] * 0.5
-        b[Ke:] = tr[t, lay["pose"]:lay["pose"] + 6] + np.concatenate([rng.normal(0, 0.3, 3), rng.normal(0, 0.003, 3)])
+        b[:Ke] = tr[t, lay["exp"]:lay["exp"] + Ke] * (1.0 - 0.5 * spread)
+        b[Ke:] = tr[t, lay["pose"]:lay["pose"] + 6] + spread * np.concatenate([rng.normal(0, 0.3, 3),
+                                                                                rng.normal(0, 0.003, 3)])
     return z
 
 
@@ -138,4 +141,27 @@ def test_each_frame_on_its_own():
         if r.status == 0:
             assert np.abs(the[f] - tho[0]).max() <= 1e-5 * np.abs(tho[0]).max()
     assert sum(r.status == 0 for r in reps) >= 3
+    ctx.close()
+
+
+@pytest.mark.parametrize("Ka,slack", [(60, True), (199, False)])
+def test_joint_terms_at_the_identity_stage_sizes(Ka, slack):
+    _say(f"case: joint landmark terms with 199 identity columns in the model, k_id {Ka}, slack {slack}")
+    sc = joint_scene(k_id=199, k_tex=12, k_exp=79, crops=(128, 160, 144, 136, 150, 170, 140))
+    Ke = 79
+    th = sc.truth().astype(np.float32)
+    ors = sc.oracle()
+    ctx = tdi.Context(sc.model, sc.opts)
+    b = tdi.Batch(ctx, sc.frames)
+    rng = np.random.default_rng(Ka)
+    z = _start(sc, Ka, Ke, slack, rng, spread=1.0 if slack else 0.1)
+    og, gg, Hg = b.landmark_joint_terms(th, Ka, Ke, slack, z, 1e4)
+    oo, go, Ho = ors.landmark_joint_terms(th.astype(np.float64), Ka, Ke, slack, z, 1e4)
+    assert np.isfinite(oo)
+    eo = abs(og - oo) / abs(oo)
+    eg = np.abs(gg - go).max() / np.abs(go).max()
+    eH = np.abs(Hg - Ho).max() / np.abs(Ho).max()
+    _say(f"  {z.size} joint unknowns: |dobj| {eo:.1e}, |dg| {eg:.1e}, |dH| {eH:.1e} (relative)")
+    assert eo < 1e-9 and eg < 1e-9 and eH < 1e-9
+    b.close()
     ctx.close()
