@@ -59,6 +59,7 @@ __device__ inline void load_frame_rot(const Layout& L, const FrameGPU& fr, const
 }
 
 constexpr int DI_PX_PER_WG = 256;   // pixels per workgroup and frame in k_dI
+constexpr int DI_PASSES = 4;        // strip passes of k_dI (its grid's z)
 
 // Four wavefronts per SIMD (register budget 128): more pixels' gathers in flight per CU.
 // IDTEX: the layout has identity or albedo columns (the identity stage); the per-frame fit's
@@ -82,8 +83,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     // is a multiple of 8): each XCD's L2 then holds the basis rows of one strip of the mesh.
     // A strip is shared by as many workgroups as give each about DI_PX_PER_WG pixels; the
     // others leave before the frame set-up.
-    const int strip = blockIdx.x & 7, sb = blockIdx.x >> 3;
-    const int nsb = max(1, min((int)(gridDim.x >> 3), (N + 8 * DI_PX_PER_WG - 1) / (8 * DI_PX_PER_WG)));
+    // With gridDim.z passes the frame is cut into 8 gridDim.z strips: pass z takes strips
+    // 8 z .. 8 z + 7 of every frame, and the dispatcher issues pass 0's workgroups (all frames)
+    // before pass 1's, so an XCD's L2 holds the rows of a narrower strip at a time.
+    const int nst = 8 * (int)gridDim.z;
+    const int strip = (blockIdx.x & 7) + 8 * (int)blockIdx.z, sb = blockIdx.x >> 3;
+    const int nsb = max(1, min((int)(gridDim.x >> 3), (N + nst * DI_PX_PER_WG - 1) / (nst * DI_PX_PER_WG)));
     if (sb >= nsb) continue;
     const FrameGPU fr = frames[f];
     const float* tf = theta + (size_t)f * L.K;
@@ -96,7 +101,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const int sub = lane >> 4, l16 = lane & 15;      // four pixels per wavefront, 16 lanes each
     const double* P0f = P0 + (size_t)f * m.V * 3;
     const int K = L.K;
-    const int q0 = (int)((long long)N * strip / 8), q1 = (int)((long long)N * (strip + 1) / 8);
+    const int q0 = (int)((long long)N * strip / nst), q1 = (int)((long long)N * (strip + 1) / nst);
     for (int pos0 = q0 + (sb * 4 + wave) * 4; pos0 < q1; pos0 += nsb * 16) {
         const int pos = pos0 + sub;
         if (pos >= q1) continue;
@@ -1040,18 +1045,19 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameL
         set_error("dI: k_tex %d > 512", L.ntex);
         return TDI_ERR_ARG;
     }
-    // 8 strips per frame, each shared by up to nsb workgroups; one frame per grid row
+    // 8 DI_PASSES strips per frame, each shared by up to nsb workgroups; one frame per grid row
     // (sized for a quarter-visible mean crop: larger frames loop, and few workgroups of a grid
     // row find no pixels)
-    int nsb = (mean_area / 4 + 8 * DI_PX_PER_WG - 1) / (8 * DI_PX_PER_WG);
+    int nsb = (mean_area / 4 + 8 * DI_PASSES * DI_PX_PER_WG - 1) / (8 * DI_PASSES * DI_PX_PER_WG);
     if (nsb > 64) nsb = 64;
     if (nsb < 1) nsb = 1;
+    const dim3 grid(8 * nsb, grid_frames(fl.n, 1 << 16), DI_PASSES);
     if (L.nid || L.ntex)
-        hipLaunchKernelGGL(k_dI<true>, dim3(8 * nsb, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act,
-                           fl.n, theta, P0, pvert, pbary, npix, pixg, dI);
+        hipLaunchKernelGGL(k_dI<true>, grid, dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0, pvert, pbary, npix,
+                           pixg, dI);
     else
-        hipLaunchKernelGGL(k_dI<false>, dim3(8 * nsb, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act,
-                           fl.n, theta, P0, pvert, pbary, npix, pixg, dI);
+        hipLaunchKernelGGL(k_dI<false>, grid, dim3(256), 0, s, m, L, frames, fl.act, fl.n, theta, P0, pvert, pbary, npix,
+                           pixg, dI);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
