@@ -1984,10 +1984,7 @@ static int eval_light_jacobian(tdi_batch* b, FrameList fl, const float* theta, h
     int rc;
     if ((rc = launch_dI_light(b->m, Lx, w.frames, fl, theta, w.P0, w.pvert, w.pbary, w.npix, w.dI, w.max_area, s)))
         return rc;
-    Layout Ll{};
-    Ll.K = Ll.nlight = Lx.K - Lx.off_light;
-    return launch_jtj_f32(Ll, w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g,
-                          w.partials, w.S, s);
+    return launch_jtj_light(w.frames, fl, w.npix, w.pnb, w.pcoef, w.pixg, w.dI, c->opts.data_weight, w.H, w.g, s);
 }
 
 static int eval_light_full(tdi_batch* b, FrameList fl, const float* theta, hipStream_t s)

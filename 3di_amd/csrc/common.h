@@ -360,6 +360,9 @@ int launch_dI(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameL
 int launch_dI_light(const ModelGPU& m, const Layout& L, const FrameGPU* frames, FrameList fl, const float* theta,
                     const double* P0, const int4* pvert, const float2* pbary, const int* npix, float* dIl, int max_area,
                     hipStream_t s);
+// the light sub-fits' 4 x 4 normal equations from those rows: H [n][4][4] FP64, g [n][4]
+int launch_jtj_light(const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb, const float* pcoef,
+                     const float4* pixg, const float* dIl, float w, double* H, float* g, hipStream_t s);
 int jtj_splits(int n, int max_area);
 size_t jtj_partials_floats(int n, int S, int K, bool mfma);
 // J^T J and J^T r for frames with more columns than one workgroup's LDS holds (the identity

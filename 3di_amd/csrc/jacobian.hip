@@ -505,6 +505,63 @@ __global__ void __launch_bounds__(256) k_jtj_reduce(const int* __restrict__ act,
     }
 }
 
+// The light sub-fits' 4 x 4 normal equations from the light columns' rows ([pixel][4], k_dI_light):
+// one workgroup per listed frame, each thread summing its pixels' two rows in FP64, then a
+// workgroup reduction; H = w sum J^T J (FP64), g = -w sum J^T (g^s) as k_jtj_reduce writes them.
+__global__ void __launch_bounds__(256) k_jtj_light(const FrameGPU* __restrict__ frames, const int* __restrict__ act,
+                                                    int n, const int* __restrict__ npix, const int4* __restrict__ pnb,
+                                                    const float* __restrict__ pcoef, const float4* __restrict__ pixg,
+                                                    const float* __restrict__ dIl, float w, double* __restrict__ H,
+                                                    float* __restrict__ g)
+{
+    constexpr int NL4 = 4, NS = 14;   // 10 entries of the upper triangle, 4 of the gradient
+    __shared__ double sred[4][NS];
+    const int nact = fl_count(act, n);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int yi = blockIdx.x; yi < nact; yi += gridDim.x) {
+        const int f = fl_frame(act, yi);
+        const FrameGPU fr = frames[f];
+        const int N = npix[f];
+        const float* dIf = dIl + (size_t)fr.pix_off * NL4;
+        double acc[NS];
+#pragma unroll
+        for (int e = 0; e < NS; ++e) acc[e] = 0.0;
+        for (int pos = tid; pos < N; pos += blockDim.x) {
+            const PixCoef pc = pix_coef(fr, pnb, pcoef, pixg, pos, N);
+            float jx[NL4], jy[NL4];
+#pragma unroll
+            for (int j = 0; j < NL4; ++j) jrow(pc, dIf, NL4, j, jx[j], jy[j]);
+            int e = 0;
+#pragma unroll
+            for (int a = 0; a < NL4; ++a)
+#pragma unroll
+                for (int b = a; b < NL4; ++b) acc[e++] += (double)jx[a] * jx[b] + (double)jy[a] * jy[b];
+#pragma unroll
+            for (int a = 0; a < NL4; ++a) acc[10 + a] += (double)jx[a] * pc.gsx + (double)jy[a] * pc.gsy;
+        }
+#pragma unroll
+        for (int e = 0; e < NS; ++e) {
+            double v = acc[e];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+            if (lane == 0) sred[wave][e] = v;
+        }
+        __syncthreads();
+        if (tid < NS) {
+            const double v = (sred[0][tid] + sred[1][tid]) + (sred[2][tid] + sred[3][tid]);
+            if (tid < 10) {
+                int a = 0, r = tid;
+                while (r >= NL4 - a) { r -= NL4 - a; ++a; }
+                const int b = a + r;
+                H[(size_t)f * 16 + a * NL4 + b] = (double)w * v;
+                H[(size_t)f * 16 + b * NL4 + a] = (double)w * v;
+            } else {
+                g[(size_t)f * NL4 + (tid - 10)] = (float)(-(double)w * v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ BF16 split MFMA J^T J
 // J is split into BF16 parts: x = hi + mid + lo (three terms, each rounded to nearest even),
 // and J^T J is accumulated in FP32 from the six products that matter down to 2^-24:
@@ -1076,6 +1133,15 @@ int launch_dI_light(const ModelGPU& m, const Layout& L, const FrameGPU* frames, 
     if (gx < 1) gx = 1;
     hipLaunchKernelGGL(k_dI_light, dim3(gx, grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, m, L, frames, fl.act, fl.n,
                        theta, P0, pvert, pbary, npix, dIl);
+    TDI_HIP(hipGetLastError());
+    return TDI_OK;
+}
+
+int launch_jtj_light(const FrameGPU* frames, FrameList fl, const int* npix, const int4* pnb, const float* pcoef,
+                     const float4* pixg, const float* dIl, float w, double* H, float* g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_jtj_light, dim3(grid_frames(fl.n, 1 << 16)), dim3(256), 0, s, frames, fl.act, fl.n, npix, pnb,
+                       pcoef, pixg, dIl, w, H, g);
     TDI_HIP(hipGetLastError());
     return TDI_OK;
 }
