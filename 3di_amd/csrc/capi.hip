@@ -363,6 +363,14 @@ int tdi_ctx_create(const tdi_model_desc* md, const tdi_fit_options* opts, int de
     return TDI_OK;
 }
 
+int tdi_ctx_release_cached(tdi_ctx* c)
+{
+    if (!c) { set_error("tdi_ctx_release_cached: null context"); return TDI_ERR_ARG; }
+    (void)hipSetDevice(c->device);
+    if (c->pool) c->pool->trim();
+    return TDI_OK;
+}
+
 int tdi_ctx_destroy(tdi_ctx* c)
 {
     if (!c) return TDI_OK;

@@ -40,6 +40,7 @@ def lib():
         L.tdi_num_unknowns.restype = C.c_int32
         L.tdi_ctx_create.argtypes = [_vp, _vp, C.c_int, C.POINTER(_vp)]
         L.tdi_ctx_destroy.argtypes = [_vp]
+        L.tdi_ctx_release_cached.argtypes = [_vp]
         L.tdi_batch_create.argtypes = [_vp, C.c_int32, _vp, C.POINTER(_vp)]
         L.tdi_batch_destroy.argtypes = [_vp]
         L.tdi_rotation.argtypes = [C.c_int32, _vp, _vp, _vp, _vp]
@@ -127,6 +128,10 @@ class Context:
         h = _vp()
         _check(lib().tdi_ctx_create(C.addressof(self.md.desc), C.addressof(self.opts), device, C.byref(h)))
         self.h = h
+
+    def release_cached(self):
+        """Free the device blocks the context keeps for later batches (tdi_ctx_release_cached)."""
+        _check(lib().tdi_ctx_release_cached(self.h))
 
     def close(self):
         if self.h:

@@ -173,6 +173,12 @@ int tdi_blur_taps(float sigma, float taps[15]);
  *      Optimizer.h:249-405, solver.h:20-68, video_fitter.cu:16-81). ---- */
 int tdi_ctx_create(const tdi_model_desc* model, const tdi_fit_options* opts, int device, tdi_ctx** out);
 int tdi_ctx_destroy(tdi_ctx* ctx);
+/* A context keeps the device blocks of its destroyed batches for its later batches (workspaces of
+ * the same sizes are taken back instead of allocated again; at most a quarter of the device's
+ * memory is kept, and all of it is freed with the context or when an allocation fails).  This
+ * frees the kept blocks now, for a long-lived context between workloads; batches in use keep
+ * theirs.  No counterpart in the reference (its objects allocate once per run). */
+int tdi_ctx_release_cached(tdi_ctx* ctx);
 const char* tdi_last_error(void);
 
 /* ---- Batch: frames uploaded once; the observed crop is blurred once and its normalised

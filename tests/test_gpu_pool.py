@@ -46,4 +46,11 @@ def test_pooled_workspace_gives_the_same_fit():
         _say(f"  pass {rep}: max |dtheta| against the first batch {d:.1e}")
         assert torch.equal(f3, f0)
         assert [(r.status, r.iters, r.stop, r.ls_evals) for r in r3] == [(r.status, r.iters, r.stop, r.ls_evals) for r in r0]
+    # the kept blocks freed: the next batch allocates afresh, with the same fit
+    ctx.release_cached()
+    b4 = tdi.Batch(ctx, sc.frames)
+    f4, _ = b4.fit_batched(th0.clone())
+    torch.cuda.synchronize()
+    b4.close()
+    assert torch.equal(f4, f0)
     ctx.close()
