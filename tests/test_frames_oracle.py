@@ -14,9 +14,13 @@ semantics.  How each part is pinned:
     half-pixel centres and clamped edges agrees to 1e-6 on random frames (sources of 1-97 pixels
     a side, scales 0.2-6, crops reaching past the frame);
   - crop layout: column-major, origin (x0, y0), zero beyond the resized frame.
+  - the exact 2x downscale (coef 0.5), for which OpenCV switches INTER_LINEAR to its area-averaging
+    path: the linear weights are then exactly 1/2, so the linear passes give
+    RN(RN(a + b) + RN(c + d)) / 4, the value of the vectorised area average ((a + b) + (c + d)) * 0.25
+    over each 2x2 block; checked bit for bit on even and odd source sizes.
 Left open ("parity unpinned" against OpenCV itself): OpenCV builds whose SIMD paths fuse the
-multiply-adds of the resize round differently in the last bit, and the exact 2x downscale
-(coef 0.5), for which OpenCV switches to its area-averaging path.
+multiply-adds of the resize round differently in the last bit, and the summation order of the
+area path's scalar tail (columns past the last full vector), which this build does not restate.
 """
 import importlib
 import os
@@ -80,6 +84,21 @@ def test_equal_size_is_a_copy_and_layout_is_column_major():
     a, b = _crops(np.concatenate([fr, fr]), [1.0, 1.004], [[5, 7, 30, 20], [5, 7, 30, 20]])
     assert O.resized_size(64, 1.004) == 64 and O.resized_size(40, 1.004) == 40
     assert np.array_equal(a, grey[7:27, 5:35]) and np.array_equal(b, a)
+
+
+@pytest.mark.parametrize("sh,sw", [(48, 64), (49, 65), (720, 1280)])
+def test_exact_half_scale_is_the_two_by_two_area_average(sh, sw):
+    rng = np.random.default_rng(sh + sw)
+    fr = rng.integers(0, 256, (1, sh, sw, 3), dtype=np.uint8)
+    v = _grey(fr[0]).astype(f32) * f32(1.0 / 255.0)
+    dw, dh = O.resized_size(sw, 0.5), O.resized_size(sh, 0.5)
+    assert 2 * dw <= sw and 2 * dh <= sh
+    got = _crops(fr, [0.5], [[0, 0, dw, dh]])[0]
+    a, b = v[0:2 * dh:2, 0:2 * dw:2], v[0:2 * dh:2, 1:2 * dw:2]
+    c, d = v[1:2 * dh:2, 0:2 * dw:2], v[1:2 * dh:2, 1:2 * dw:2]
+    want = ((a + b) + (c + d)) * f32(0.25)
+    assert want.dtype == f32
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
 def test_linear_image_is_reproduced_and_border_is_zero():

@@ -2,7 +2,7 @@
 resized 8-bit BGR frames, bit for bit.  Both sides perform the same FP32 operations in the same
 order (the device code is built without multiply-add contraction for this kernel), so every
 crop pixel must be identical.  Cases: random coefficients from 0.3 to 3 (down- and upscaling),
-the copy case (destination size equal to the source size), coefficient 1, crops that reach past
+the copy case (destination size equal to the source size), coefficient 1, the exact 2x downscale, crops that reach past
 the resized frame into the zero border, one-pixel crops, one-pixel-wide and one-pixel-high
 sources, a crop entirely past the resized frame, random sources of 1-97 pixels a side at scales
 0.2-6, and the host-layer chain (landmark geometry, then crops) on synthetic frames.
@@ -31,6 +31,7 @@ def _case(rng, n, sh, sw):
     frames = rng.integers(0, 256, (n, sh, sw, 3), dtype=np.uint8)
     coef = rng.uniform(0.3, 3.0, n).astype(np.float32)
     coef[0], coef[1] = 1.0, np.float32(1.0 + 0.2 / sw)      # coefficient 1, and an equal-size copy
+    coef[3] = 0.5                                           # the exact 2x downscale (2x2 area average)
     crop = np.zeros((n, 4), np.int32)
     for i in range(n):
         dw, dh = O.resized_size(sw, coef[i]), O.resized_size(sh, coef[i])
